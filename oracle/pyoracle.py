@@ -1,0 +1,219 @@
+"""ctypes binding of the FP64 CPU oracle -- TEST INFRASTRUCTURE ONLY.
+
+Imported by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg,
+never by the product package.  Units: nm, kJ/mol, kJ/mol/nm.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, 'build', 'liboracle.so')
+REF_PATH = os.path.join(HERE, '_ref', 'libref_poly.so')
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+
+
+class ElecParams(ctypes.Structure):
+    _fields_ = [('method', ctypes.c_int), ('cutoff', ctypes.c_double), ('alpha', ctypes.c_double),
+                ('grid', ctypes.c_int * 3), ('thole', ctypes.c_double * 5), ('max_iter', ctypes.c_int),
+                ('target_eps', ctypes.c_double), ('charge_redistribution', ctypes.c_int),
+                ('box', ctypes.c_double * 3)]
+
+
+class ElecStats(ctypes.Structure):
+    _fields_ = [('iterations', ctypes.c_int), ('epsilon', ctypes.c_double), ('converged', ctypes.c_int)]
+
+
+def _d(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _i(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def _pd(a):
+    return a.ctypes.data_as(_dp) if a is not None else None
+
+
+def _pi(a):
+    return a.ctypes.data_as(_ip) if a is not None else None
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError('oracle library not built: run `make -C oracle`')
+        L = ctypes.CDLL(LIB_PATH)
+        L.oracle_pip2b.restype = ctypes.c_double
+        L.oracle_pip3b.restype = ctypes.c_double
+        L.oracle_onebody.restype = ctypes.c_double
+        L.oracle_dimer_list.restype = ctypes.c_int64
+        L.oracle_dimer_list.argtypes = [ctypes.c_int, _dp, _dp, ctypes.c_int, ctypes.c_double, _ip, ctypes.c_int64]
+        L.oracle_trimer_list.restype = ctypes.c_int64
+        L.oracle_trimer_list.argtypes = [ctypes.c_int, _dp, _dp, ctypes.c_int, ctypes.c_double, ctypes.c_double, _ip,
+                                         ctypes.c_int64]
+        L.oracle_twobody.restype = ctypes.c_double
+        L.oracle_threebody.restype = ctypes.c_double
+        L.oracle_threebody.argtypes = [ctypes.c_int64, _ip, _dp, _ip, ctypes.c_int, _dp, _dp]
+        L.oracle_thole_scale.restype = ctypes.c_double
+        L.oracle_thole_scale.argtypes = [ctypes.POINTER(ElecParams), ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                         ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int,
+                                         ctypes.c_int, ctypes.c_int]
+        L.oracle_gammq.restype = ctypes.c_double
+        L.oracle_gammq.argtypes = [ctypes.c_double, ctypes.c_double]
+        L.oracle_dispersion.restype = ctypes.c_double
+        L.oracle_dispersion.argtypes = [ctypes.c_int, _dp, _ip, _ip, ctypes.c_int, _dp, _dp, ctypes.c_int,
+                                        ctypes.c_double, _dp, _dp]
+        _lib = L
+    return _lib
+
+
+def ref_lib():
+    """The reference's own polynomial/gammq sources compiled into oracle/_ref (may be absent)."""
+    if not os.path.exists(REF_PATH):
+        return None
+    L = ctypes.CDLL(REF_PATH)
+    for f in ('ref_poly_2b', 'ref_poly_3b', 'ref_gammq', 'ref_gammln'):
+        getattr(L, f).restype = ctypes.c_double
+    L.ref_gammq.argtypes = [ctypes.c_double, ctypes.c_double]
+    L.ref_gammln.argtypes = [ctypes.c_double]
+    return L
+
+
+def pip2b(x):
+    x = _d(x)
+    g = np.zeros(31)
+    e = lib().oracle_pip2b(_pd(x), _pd(g))
+    return e, g
+
+
+def pip3b(x):
+    x = _d(x)
+    g = np.zeros(36)
+    e = lib().oracle_pip3b(_pd(x), _pd(g))
+    return e, g
+
+
+def _box(box):
+    return _d(box if box is not None else np.zeros(3))
+
+
+def onebody(pos, idx, periodic=False, box=None):
+    pos, idx = _d(pos), _i(idx)
+    f = np.zeros_like(pos)
+    e = lib().oracle_onebody(len(idx), _pd(pos), _pi(idx), int(periodic), _pd(_box(box)), _pd(f))
+    return e, f
+
+
+def dimer_list(opos, rc, periodic=False, box=None):
+    opos = _d(opos)
+    n = len(opos)
+    cap = max(1, n * (n - 1) // 2)
+    out = np.zeros((cap, 2), np.int32)
+    c = lib().oracle_dimer_list(n, _pd(opos), _pd(_box(box)), int(periodic), rc, _pi(out), cap)
+    return out[:c].copy()
+
+
+def trimer_list(opos, rc, periodic=False, box=None, rmin=0.0):
+    opos = _d(opos)
+    n = len(opos)
+    c = lib().oracle_trimer_list(n, _pd(opos), _pd(_box(box)), int(periodic), rc, rmin, None, 0)
+    out = np.zeros((max(c, 1), 3), np.int32)
+    lib().oracle_trimer_list(n, _pd(opos), _pd(_box(box)), int(periodic), rc, rmin, _pi(out), c)
+    return out[:c].copy()
+
+
+def twobody(pos, idx, pairs, periodic=False, box=None):
+    pos, idx, pairs = _d(pos), _i(idx), _i(pairs)
+    f = np.zeros_like(pos)
+    e = lib().oracle_twobody(len(pairs), _pi(pairs), _pd(pos), _pi(idx), int(periodic), _pd(_box(box)), _pd(f))
+    return e, f
+
+
+def threebody(pos, idx, trip, periodic=False, box=None):
+    pos, idx, trip = _d(pos), _i(idx), _i(trip)
+    f = np.zeros_like(pos)
+    e = lib().oracle_threebody(len(trip), _pi(trip), _pd(pos), _pi(idx), int(periodic), _pd(_box(box)), _pd(f))
+    return e, f
+
+
+def elec_params(method=0, cutoff=0.9, alpha=0.0, grid=(0, 0, 0), thole=(0.4, 0.4, 0.055, 0.626, 0.055),
+                max_iter=200, target_eps=1e-7, charge_redistribution=True, box=(0, 0, 0)):
+    p = ElecParams()
+    p.method, p.cutoff, p.alpha = method, cutoff, alpha
+    p.grid[:] = list(grid)
+    p.thole[:] = list(thole)
+    p.max_iter, p.target_eps, p.charge_redistribution = max_iter, target_eps, int(charge_redistribution)
+    p.box[:] = list(box)
+    return p
+
+
+def electrostatics(params, pos, charge, mol, atype, damp, pol, want_field=False):
+    pos = _d(pos)
+    n = len(pos)
+    f = np.zeros_like(pos)
+    dip = np.zeros_like(pos)
+    fld = np.zeros_like(pos) if want_field else None
+    e = ctypes.c_double(0)
+    st = ElecStats()
+    err = ctypes.create_string_buffer(256)
+    rc = lib().oracle_electrostatics(ctypes.byref(params), n, _pd(pos), _pd(_d(charge)), _pi(_i(mol)), _pi(_i(atype)),
+                                     _pd(_d(damp)), _pd(_d(pol)), _pd(f), ctypes.byref(e), ctypes.byref(st), _pd(dip),
+                                     _pd(fld), err, 256)
+    if rc != 0:
+        raise RuntimeError(err.value.decode())
+    out = dict(energy=e.value, forces=f, dipoles=dip, iterations=st.iterations, epsilon=st.epsilon)
+    if want_field:
+        out['fixed_field'] = fld
+    return out
+
+
+def thole_scale(params, damp_i, damp_j, r, order, itype, mol=(0, 1), types=(0, 0), just_scale=False):
+    return lib().oracle_thole_scale(ctypes.byref(params), damp_i, damp_j, mol[0], mol[1], types[0], types[1], r,
+                                    int(just_scale), order, itype)
+
+
+def water_charges(pos4):
+    pos4 = _d(pos4)
+    q = np.zeros(4)
+    dq = np.zeros((4, 3, 3))
+    lib().oracle_water_charges(_pd(pos4), _pd(q), _pd(dq))
+    return q, dq
+
+
+def solve_given_field(params, pos, mol, atype, damp, pol, field):
+    pos = _d(pos)
+    dip = np.zeros_like(pos)
+    st = ElecStats()
+    rc = lib().oracle_solve_given_field(ctypes.byref(params), len(pos), _pd(pos), _pi(_i(mol)), _pi(_i(atype)),
+                                        _pd(_d(damp)), _pd(_d(pol)), _pd(_d(field)), _pd(dip), ctypes.byref(st))
+    return dip, st.iterations, st.epsilon, rc == 0
+
+
+def gammq(a, x):
+    return lib().oracle_gammq(a, x)
+
+
+def dispersion(pos, types, mol, c6, d6, method=0, cutoff=1e3, box=None):
+    pos = _d(pos)
+    c6, d6 = _d(c6), _d(d6)
+    ntypes = int(round(len(c6) ** 0.5))
+    f = np.zeros_like(pos)
+    e = lib().oracle_dispersion(len(pos), _pd(pos), _pi(_i(types)), _pi(_i(mol)), ntypes, _pd(c6), _pd(d6), method,
+                                cutoff, _pd(_box(box)), _pd(f))
+    return e, f
+
+
+def fft3d(grid, sign):
+    g = np.ascontiguousarray(grid, dtype=np.complex128).copy()
+    nx, ny, nz = g.shape
+    lib().oracle_fft3d(g.ctypes.data_as(_dp), nx, ny, nz, sign)
+    return g
