@@ -1,0 +1,168 @@
+/*
+ * mbpol_hip.h -- C ABI of the MI355X (gfx950) MB-pol platform.
+ *
+ * This is the drop-in boundary for one MB-pol energy+force evaluation.  It
+ * replaces, for the HIP platform, what the reference's OpenMM plugin exposes
+ * through its Force / KernelImpl layers:
+ *
+ *   system description  <- MBPolOneBodyForce   (openmmapi/include/openmm/MBPolOneBodyForce.h:51-147)
+ *                          MBPolTwoBodyForce   (openmmapi/include/openmm/MBPolTwoBodyForce.h:57-178)
+ *                          MBPolThreeBodyForce (openmmapi/include/openmm/MBPolThreeBodyForce.h:57-178)
+ *                          MBPolElectrostaticsForce (openmmapi/include/openmm/MBPolElectrostaticsForce.h:56-323)
+ *                          the dispersion CustomNonbondedForce built by python/mbpol.xml:37-108
+ *                          ThreeParticleAverageSite M sites (python/mbpol.xml:14)
+ *   context create      <- Calc*ForceKernel::initialize (openmmapi/include/openmm/mbpolKernels.h:49-207,
+ *                          platforms/reference/src/MBPolReferenceKernels.cpp:86-99,141-207,341-360,430-449)
+ *   evaluate            <- Calc*ForceKernel::execute, summed over the five Forces
+ *                          (MBPolReferenceKernels.cpp:101-114,246-259,362-399,451-483)
+ *   update parameters   <- copyParametersToContext (MBPolReferenceKernels.cpp:116-127,305-325,401-414,485-498)
+ *   electrostatic potential <- getElectrostaticPotential (MBPolReferenceKernels.cpp:260-283)
+ *
+ * Units follow the reference API: positions and lengths in nm, energies in
+ * kJ/mol, forces in kJ/mol/nm, charges in e, polarisabilities in nm^3.
+ * Plain pointers and sizes only; errors are returned as status codes with a
+ * per-thread message (mbpol_last_error) -- no exceptions cross this ABI.
+ * A context binds one GPU and one HIP stream; calls on one context must not
+ * overlap (the reference kernels are not re-entrant either).
+ */
+#ifndef MBPOL_HIP_H
+#define MBPOL_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define MBPOL_OK 0
+#define MBPOL_ERR_INVALID_ARGUMENT 1    /* bad sizes / indices / options (OpenMMException in the reference) */
+#define MBPOL_ERR_HIP 2                 /* a HIP runtime or library call failed */
+#define MBPOL_ERR_NOT_CONVERGED 3       /* induced dipoles did not converge
+                                           (MBPolReferenceElectrostaticsForce.cpp:888-894) */
+#define MBPOL_ERR_BOX_TOO_SMALL 4       /* box < 2 * cutoff (MBPolReferenceKernels.cpp:224-227,387-390,471-474) */
+#define MBPOL_ERR_CUTOFF_TOO_LARGE 5    /* cutoff > box/2 at initialisation (MBPol*ForceImpl.cpp:60-66) */
+#define MBPOL_ERR_COUNT_CHANGED 6       /* parameter-count change on update (MBPolReferenceKernels.cpp:117-118) */
+#define MBPOL_ERR_NO_DEVICE 7           /* no usable GPU */
+
+/* nonbonded methods, numbered as in the reference enums */
+#define MBPOL_ONEBODY_NONPERIODIC 0     /* MBPolOneBodyForce::NonPeriodic */
+#define MBPOL_ONEBODY_PERIODIC 1        /* MBPolOneBodyForce::Periodic */
+#define MBPOL_NOCUTOFF 0                /* MBPolTwoBodyForce / MBPolThreeBodyForce::NoCutoff */
+#define MBPOL_CUTOFF_PERIODIC 1         /* ::CutoffPeriodic */
+#define MBPOL_CUTOFF_NONPERIODIC 2      /* ::CutoffNonPeriodic */
+#define MBPOL_ELEC_NOCUTOFF 0           /* MBPolElectrostaticsForce::NoCutoff */
+#define MBPOL_ELEC_PME 1                /* MBPolElectrostaticsForce::PME */
+
+/* component bits for evaluate(); also the order of energies_out[] */
+#define MBPOL_ONEBODY 1u
+#define MBPOL_TWOBODY 2u
+#define MBPOL_THREEBODY 4u
+#define MBPOL_DISPERSION 8u
+#define MBPOL_ELECTROSTATICS 16u
+#define MBPOL_ALL 31u
+
+/* arithmetic of the PIP and pair kernels */
+#define MBPOL_PRECISION_MIXED 0         /* FP32 polynomial/pair math, FP64 accumulation, FP64 Ewald/residual */
+#define MBPOL_PRECISION_DOUBLE 1        /* FP64 everywhere */
+
+/* initial guess of the induced-dipole iteration */
+#define MBPOL_GUESS_ALPHA_E 0           /* mu0 = alpha * E_fixed, as the reference (:628-639) */
+#define MBPOL_GUESS_ZERO 1
+#define MBPOL_GUESS_PREVIOUS 2          /* converged dipoles of the previous evaluate() */
+#define MBPOL_GUESS_USER 3              /* dipoles passed with mbpol_context_set_dipole_guess() */
+
+typedef struct mbpol_system mbpol_system;
+typedef struct mbpol_context mbpol_context;
+
+typedef struct {
+    int32_t iterations;        /* induced-dipole iterations of the last evaluate() */
+    double epsilon;            /* final convergence measure (reference definition, :504-511, :588) */
+    int32_t converged;
+    int64_t num_dimers;        /* entries of the 2-body list */
+    int64_t num_trimers;       /* entries of the 3-body work list */
+    int64_t num_elec_pairs;    /* real-space site pairs within the electrostatics cutoff (PME) */
+    int64_t num_disp_pairs;    /* dispersion pairs within the cutoff */
+} mbpol_eval_stats;
+
+/* thread-local message of the last failed call on this thread */
+const char* mbpol_last_error(void);
+/* number of usable GPUs */
+int mbpol_device_count(int32_t* count);
+
+/* ---- system description (host side; mirrors the Force classes) ---- */
+int mbpol_system_create(int32_t num_particles, mbpol_system** out);
+void mbpol_system_destroy(mbpol_system* sys);
+/* rectangular periodic box (System::setDefaultPeriodicBoxVectors); zeros = none */
+int mbpol_system_set_box(mbpol_system* sys, const double box_nm[3]);
+/* ThreeParticleAverageSite: r_site = w1 r_p1 + w2 r_p2 + w3 r_p3; its force is folded back with the same weights */
+int mbpol_system_add_virtual_site(mbpol_system* sys, int32_t site, int32_t p1, int32_t p2, int32_t p3, double w1,
+                                  double w2, double w3);
+/* MBPolOneBodyForce::addOneBody / setNonbondedMethod; idx = n x {O, H1, H2} */
+int mbpol_system_set_onebody(mbpol_system* sys, int32_t n, const int32_t* idx, int32_t method);
+/* MBPolTwoBodyForce::addParticle / setNonbondedMethod / setCutoff */
+int mbpol_system_set_twobody(mbpol_system* sys, int32_t n, const int32_t* idx, int32_t method, double cutoff_nm);
+/* MBPolThreeBodyForce::addParticle / setNonbondedMethod / setCutoff */
+int mbpol_system_set_threebody(mbpol_system* sys, int32_t n, const int32_t* idx, int32_t method, double cutoff_nm);
+/* MBPolElectrostaticsForce::addElectrostatics (one entry per particle, O,H1,H2,M order per water when
+ * charge redistribution is on) and its settings; ewald_alpha = 0 or grid[0] = 0 selects the automatic
+ * choice from ewald_tol, as NonbondedForceImpl::calcPMEParameters (MBPolReferenceKernels.cpp:186-201) */
+int mbpol_system_set_electrostatics(mbpol_system* sys, int32_t n, const double* charge, const int32_t* molecule,
+                                    const int32_t* atom_type, const double* damping, const double* polarity,
+                                    int32_t method, double cutoff_nm, double ewald_alpha, const int32_t grid[3],
+                                    double ewald_tol, int32_t max_iter, double target_eps,
+                                    int32_t charge_redistribution, const double thole[5]);
+/* damped-C6 dispersion (python/mbpol.xml:37-108): per-particle type and molecule; pairs within one molecule and
+ * particles whose type has an all-zero C6 row are excluded; c6/d6 are num_types x num_types
+ * (kJ nm^6/mol, nm^-1); method as the two-body enum */
+int mbpol_system_set_dispersion(mbpol_system* sys, int32_t n, const int32_t* type, const int32_t* molecule,
+                                int32_t num_types, const double* c6, const double* d6, int32_t method,
+                                double cutoff_nm);
+
+/* ---- context (device side) ---- */
+int mbpol_context_create(const mbpol_system* sys, int32_t precision, int32_t device, mbpol_context** out);
+void mbpol_context_destroy(mbpol_context* ctx);
+/* positions: host, N x 3 nm (virtual-site rows are recomputed);
+ * forces_out: host N x 3 (overwritten, virtual-site forces folded back and zeroed) or NULL;
+ * energies_out: host [6] = {1B, 2B, 3B, dispersion, electrostatics, total} kJ/mol, or NULL */
+int mbpol_context_evaluate(mbpol_context* ctx, const double* positions, double* forces_out, double* energies_out,
+                           uint32_t components);
+/* same with device pointers (positions and forces stay in HBM; energies_out is host) */
+int mbpol_context_evaluate_device(mbpol_context* ctx, const double* d_positions, double* d_forces,
+                                  double* energies_out, uint32_t components);
+/* device-resident inputs: copy positions into the context once, then evaluate without host transfers;
+ * forces stay in HBM until downloaded */
+int mbpol_context_upload_positions(mbpol_context* ctx, const double* positions);
+int mbpol_context_evaluate_resident(mbpol_context* ctx, double* energies_out, uint32_t components);
+int mbpol_context_download_forces(mbpol_context* ctx, double* forces_out);
+/* per-kernel timing with HIP events on the context's stream (off by default);
+ * name one of "twobody", "threebody", "elec_iteration", "elec_setup", "elec_energy", "dispersion", "lists",
+ * "onebody", "evaluate"; returns accumulated milliseconds and the number of timed launches since the last reset */
+int mbpol_context_set_timing(mbpol_context* ctx, int32_t enabled);
+int mbpol_context_get_timing(mbpol_context* ctx, const char* name, double* total_ms, int64_t* count);
+int mbpol_context_set_box(mbpol_context* ctx, const double box_nm[3]);
+int mbpol_context_get_stats(const mbpol_context* ctx, mbpol_eval_stats* stats);
+/* converged induced dipoles of the last evaluate (electrostatics particles x 3, e nm) */
+int mbpol_context_get_dipoles(const mbpol_context* ctx, double* dipoles);
+/* charges used by the last evaluate (after redistribution) */
+int mbpol_context_get_charges(const mbpol_context* ctx, double* charges);
+int mbpol_context_set_dipole_guess(mbpol_context* ctx, int32_t mode, const double* dipoles);
+/* copyParametersToContext for the electrostatics (same count required) */
+int mbpol_context_update_electrostatics(mbpol_context* ctx, int32_t n, const double* charge, const int32_t* molecule,
+                                        const int32_t* atom_type, const double* damping, const double* polarity);
+/* neighbour lists of the last evaluate, for parity checks: pairs (i<j) and chain triplets (i>j>k), molecule
+ * indices of the two/three-body Force.  The trimer list is the full reference chain list when
+ * full_chain_list was requested at the evaluate (see mbpol_context_set_list_mode), else the work list. */
+int mbpol_context_get_dimer_list(const mbpol_context* ctx, int32_t* pairs, int64_t capacity, int64_t* count);
+int mbpol_context_get_trimer_list(const mbpol_context* ctx, int32_t* triplets, int64_t capacity, int64_t* count);
+/* 1 = keep every chain triplet (reference list); 0 (default) = drop triplets whose switch product is exactly 0 */
+int mbpol_context_set_list_mode(mbpol_context* ctx, int32_t full_chain_list);
+/* MBPolElectrostaticsForce::getElectrostaticPotential at npoints points (nm), kJ/mol/e, from the charges and
+ * induced dipoles of the last evaluate (MBPolReferenceElectrostaticsForce.cpp:1023-1086) */
+int mbpol_context_get_electrostatic_potential(mbpol_context* ctx, int32_t npoints, const double* points,
+                                              double* potential);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MBPOL_HIP_H */

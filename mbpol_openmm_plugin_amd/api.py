@@ -1,0 +1,550 @@
+"""Python host layer of the MI355X MB-pol platform.
+
+Mirrors the reference plugin's Force API (openmmapi/include/openmm/MBPol*Force.h)
+and the slice of OpenMM's System / Context / State that drives it, on top of
+the C ABI in include/mbpol_hip.h (libmbpol_hip.so, loaded with ctypes).
+Method names, argument meaning, units (nm, kJ/mol, e, nm^3) and error
+behaviour follow the reference; errors raised by the library surface as
+``MBPolError`` (the reference throws OpenMMException).
+
+There is no CPU fallback: creating a Context needs the HIP library and a GPU.
+"""
+import ctypes
+import math
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, 'libmbpol_hip.so')
+
+MBPOL_ONEBODY, MBPOL_TWOBODY, MBPOL_THREEBODY, MBPOL_DISPERSION, MBPOL_ELECTROSTATICS = 1, 2, 4, 8, 16
+MBPOL_ALL = 31
+COMPONENTS = ('onebody', 'twobody', 'threebody', 'dispersion', 'electrostatics')
+
+
+class MBPolError(RuntimeError):
+    """A failed library call (the reference raises OpenMMException)."""
+
+    def __init__(self, code, message):
+        super().__init__(message)
+        self.code = code
+
+
+class _EvalStats(ctypes.Structure):
+    _fields_ = [('iterations', ctypes.c_int32), ('epsilon', ctypes.c_double), ('converged', ctypes.c_int32),
+                ('num_dimers', ctypes.c_int64), ('num_trimers', ctypes.c_int64),
+                ('num_elec_pairs', ctypes.c_int64), ('num_disp_pairs', ctypes.c_int64)]
+
+
+_lib = None
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int32)
+
+
+def library():
+    """Load libmbpol_hip.so (fails loudly when it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise MBPolError(-1, 'libmbpol_hip.so is not built (run __graft_entry__.build())')
+        L = ctypes.CDLL(LIB_PATH)
+        L.mbpol_last_error.restype = ctypes.c_char_p
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise MBPolError(rc, library().mbpol_last_error().decode())
+
+
+def _d(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _i(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def _pd(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _pi(a):
+    return a.ctypes.data_as(_ip)
+
+
+def device_count():
+    c = ctypes.c_int32(0)
+    library().mbpol_device_count(ctypes.byref(c))
+    return c.value
+
+
+# ---------------------------------------------------------------------------
+# Forces
+# ---------------------------------------------------------------------------
+class Force:
+    def __init__(self):
+        self._group = 0
+
+    def setForceGroup(self, g):
+        self._group = int(g)
+
+    def getForceGroup(self):
+        return self._group
+
+
+class MBPolOneBodyForce(Force):
+    """openmmapi/include/openmm/MBPolOneBodyForce.h:51-147."""
+    NonPeriodic = 0
+    Periodic = 1
+
+    def __init__(self):
+        super().__init__()
+        self._mol = []
+        self._method = self.NonPeriodic
+
+    def addOneBody(self, particleIndices):
+        self._mol.append([int(v) for v in particleIndices])
+        return len(self._mol) - 1
+
+    def getNumOneBodys(self):
+        return len(self._mol)
+
+    def getOneBodyParameters(self, index):
+        return list(self._mol[index])
+
+    def setOneBodyParameters(self, index, particleIndices):
+        self._mol[index] = [int(v) for v in particleIndices]
+
+    def getNonbondedMethod(self):
+        return self._method
+
+    def setNonbondedMethod(self, method):
+        self._method = int(method)
+
+    def updateParametersInContext(self, context):
+        context._update_force(self)
+
+
+class _NBodyForce(Force):
+    NoCutoff = 0
+    CutoffPeriodic = 1
+    CutoffNonPeriodic = 2
+
+    def __init__(self):
+        super().__init__()
+        self._mol = []
+        self._method = self.NoCutoff
+        self._cutoff = 1.0e10  # openmmapi/src/MBPolTwoBodyForce.cpp:43
+
+    def addParticle(self, particleIndices):
+        self._mol.append([int(v) for v in particleIndices])
+        return len(self._mol) - 1
+
+    def getNumMolecules(self):
+        return len(self._mol)
+
+    getNumParticles = getNumMolecules
+
+    def getParticleParameters(self, index):
+        return list(self._mol[index])
+
+    def setParticleParameters(self, index, particleIndices):
+        self._mol[index] = [int(v) for v in particleIndices]
+
+    def getNonbondedMethod(self):
+        return self._method
+
+    def setNonbondedMethod(self, method):
+        self._method = int(method)
+
+    def getCutoff(self):
+        return self._cutoff
+
+    def setCutoff(self, cutoff):
+        self._cutoff = float(cutoff)
+
+    def updateParametersInContext(self, context):
+        context._update_force(self)
+
+
+class MBPolTwoBodyForce(_NBodyForce):
+    """openmmapi/include/openmm/MBPolTwoBodyForce.h:57-178."""
+
+
+class MBPolThreeBodyForce(_NBodyForce):
+    """openmmapi/include/openmm/MBPolThreeBodyForce.h:57-178."""
+
+
+class MBPolElectrostaticsForce(Force):
+    """openmmapi/include/openmm/MBPolElectrostaticsForce.h:56-323; defaults from
+    openmmapi/src/MBPolElectrostaticsForce.cpp:43-49."""
+    NoCutoff = 0
+    PME = 1
+
+    def __init__(self):
+        super().__init__()
+        self._p = []
+        self._method = self.NoCutoff
+        self._cutoff = 0.9
+        self._aewald = 0.0
+        self._grid = [0, 0, 0]
+        self._tol = 1e-4
+        self._max_iter = 200
+        self._eps = 1.0e-7
+        self._redistribute = True
+        self._thole = [0.4, 0.4, 0.055, 0.626, 0.055]
+
+    def addElectrostatics(self, charge, moleculeIndex, atomType, dampingFactor, polarity):
+        self._p.append([float(charge), int(moleculeIndex), int(atomType), float(dampingFactor), float(polarity)])
+        return len(self._p) - 1
+
+    def getNumElectrostatics(self):
+        return len(self._p)
+
+    def getElectrostaticsParameters(self, index):
+        return tuple(self._p[index])
+
+    def setElectrostaticsParameters(self, index, charge, moleculeIndex, atomType, dampingFactor, polarity):
+        self._p[index] = [float(charge), int(moleculeIndex), int(atomType), float(dampingFactor), float(polarity)]
+
+    def getNonbondedMethod(self):
+        return self._method
+
+    def setNonbondedMethod(self, method):
+        self._method = int(method)
+
+    def getCutoffDistance(self):
+        return self._cutoff
+
+    def setCutoffDistance(self, distance):
+        self._cutoff = float(distance)
+
+    def setIncludeChargeRedistribution(self, flag):
+        self._redistribute = bool(flag)
+
+    def getIncludeChargeRedistribution(self):
+        return self._redistribute
+
+    def setTholeParameters(self, thole):
+        self._thole = [float(v) for v in thole]
+
+    def getTholeParameters(self):
+        return list(self._thole)
+
+    def getAEwald(self):
+        return self._aewald
+
+    def setAEwald(self, aewald):
+        self._aewald = float(aewald)
+
+    def getPmeBSplineOrder(self):
+        return 5
+
+    def getPmeGridDimensions(self):
+        return list(self._grid)
+
+    def setPmeGridDimensions(self, dims):
+        self._grid = [int(v) for v in dims]
+
+    def getEwaldErrorTolerance(self):
+        return self._tol
+
+    def setEwaldErrorTolerance(self, tol):
+        self._tol = float(tol)
+
+    def getMutualInducedMaxIterations(self):
+        return self._max_iter
+
+    def setMutualInducedMaxIterations(self, n):
+        self._max_iter = int(n)
+
+    def getMutualInducedTargetEpsilon(self):
+        return self._eps
+
+    def setMutualInducedTargetEpsilon(self, eps):
+        self._eps = float(eps)
+
+    def updateParametersInContext(self, context):
+        context._update_force(self)
+
+    def _arrays(self):
+        p = np.array(self._p, dtype=np.float64).reshape(-1, 5)
+        return p[:, 0], p[:, 1].astype(np.int32), p[:, 2].astype(np.int32), p[:, 3], p[:, 4]
+
+
+class MBPolDispersionForce(Force):
+    """The damped-C6 dispersion that python/mbpol.xml:37-108 builds as an OpenMM
+    CustomNonbondedForce, as a Force of its own: ``-C6 tt6 / r^6`` with
+    Tang-Toennies damping, per-type C6/d6 tables, pairs inside one molecule and
+    types with an all-zero C6 row excluded, truncated at the cutoff."""
+    NoCutoff = 0
+    CutoffPeriodic = 1
+    CutoffNonPeriodic = 2
+
+    def __init__(self):
+        super().__init__()
+        self._p = []
+        self._method = self.NoCutoff
+        self._cutoff = 1.0
+        self._ntypes = 0
+        self._c6 = None
+        self._d6 = None
+
+    def setTypeTables(self, numTypes, c6, d6):
+        """c6[t1 + numTypes*t2] in kJ nm^6/mol, d6 in nm^-1 (Discrete2DFunction layout)."""
+        self._ntypes = int(numTypes)
+        self._c6 = [float(v) for v in c6]
+        self._d6 = [float(v) for v in d6]
+
+    def addParticle(self, typeIndex, moleculeIndex):
+        self._p.append([int(typeIndex), int(moleculeIndex)])
+        return len(self._p) - 1
+
+    def getNumParticles(self):
+        return len(self._p)
+
+    def getNonbondedMethod(self):
+        return self._method
+
+    def setNonbondedMethod(self, method):
+        self._method = int(method)
+
+    def getCutoff(self):
+        return self._cutoff
+
+    def setCutoff(self, cutoff):
+        self._cutoff = float(cutoff)
+
+
+# ---------------------------------------------------------------------------
+# System / Context / State
+# ---------------------------------------------------------------------------
+class ThreeParticleAverageSite:
+    def __init__(self, p1, p2, p3, w1, w2, w3):
+        self.particles = (int(p1), int(p2), int(p3))
+        self.weights = (float(w1), float(w2), float(w3))
+
+
+class System:
+    def __init__(self):
+        self._masses = []
+        self._vsites = {}
+        self._forces = []
+        self._box = None
+
+    def addParticle(self, mass):
+        self._masses.append(float(mass))
+        return len(self._masses) - 1
+
+    def getNumParticles(self):
+        return len(self._masses)
+
+    def getParticleMass(self, i):
+        return self._masses[i]
+
+    def setVirtualSite(self, index, site):
+        self._vsites[int(index)] = site
+
+    def isVirtualSite(self, index):
+        return int(index) in self._vsites
+
+    def setDefaultPeriodicBoxVectors(self, a, b, c):
+        a, b, c = (np.asarray(v, dtype=np.float64) for v in (a, b, c))
+        if np.count_nonzero(a[1:]) or b[0] or b[2] or c[0] or c[1]:
+            raise MBPolError(1, 'only rectangular boxes are supported (as in the reference)')
+        self._box = [a[0], b[1], c[2]]
+
+    def getDefaultPeriodicBoxVectors(self):
+        b = self._box or [0.0, 0.0, 0.0]
+        return [b[0], 0, 0], [0, b[1], 0], [0, 0, b[2]]
+
+    def addForce(self, force):
+        self._forces.append(force)
+        return len(self._forces) - 1
+
+    def getNumForces(self):
+        return len(self._forces)
+
+    def getForce(self, i):
+        return self._forces[i]
+
+    def removeForce(self, i):
+        del self._forces[i]
+
+    def computeVirtualSites(self, positions):
+        pos = np.array(positions, dtype=np.float64).reshape(-1, 3)
+        for s, v in self._vsites.items():
+            pos[s] = sum(w * pos[p] for p, w in zip(v.particles, v.weights))
+        return pos
+
+
+class State:
+    def __init__(self, energies, forces, stats):
+        self._e = energies
+        self._f = forces
+        self.stats = stats
+
+    def getPotentialEnergy(self):
+        """kJ/mol."""
+        return float(self._e[5])
+
+    def getForces(self, asNumpy=True):
+        """kJ/mol/nm, virtual-site forces folded onto their parents."""
+        return self._f
+
+    def getComponentEnergies(self):
+        return {k: float(self._e[i]) for i, k in enumerate(COMPONENTS)}
+
+
+def _build_native_system(system):
+    L = library()
+    n = system.getNumParticles()
+    h = ctypes.c_void_p()
+    _check(L.mbpol_system_create(ctypes.c_int32(n), ctypes.byref(h)))
+    try:
+        if system._box is not None:
+            _check(L.mbpol_system_set_box(h, _pd(_d(system._box))))
+        for s, v in sorted(system._vsites.items()):
+            _check(L.mbpol_system_add_virtual_site(h, s, *v.particles, *(ctypes.c_double(w) for w in v.weights)))
+        for f in system._forces:
+            if isinstance(f, MBPolOneBodyForce):
+                idx = _i(np.array(f._mol, dtype=np.int32).reshape(-1, 3))
+                _check(L.mbpol_system_set_onebody(h, len(f._mol), _pi(idx), f._method))
+            elif isinstance(f, (MBPolTwoBodyForce, MBPolThreeBodyForce)):
+                idx = _i(np.array(f._mol, dtype=np.int32).reshape(-1, 3))
+                fn = L.mbpol_system_set_twobody if isinstance(f, MBPolTwoBodyForce) else L.mbpol_system_set_threebody
+                _check(fn(h, len(f._mol), _pi(idx), f._method, ctypes.c_double(f._cutoff)))
+            elif isinstance(f, MBPolElectrostaticsForce):
+                q, mol, typ, damp, pol = f._arrays()
+                grid = _i(f._grid)
+                thole = _d(f._thole)
+                _check(L.mbpol_system_set_electrostatics(
+                    h, len(f._p), _pd(_d(q)), _pi(_i(mol)), _pi(_i(typ)), _pd(_d(damp)), _pd(_d(pol)), f._method,
+                    ctypes.c_double(f._cutoff), ctypes.c_double(f._aewald), _pi(grid), ctypes.c_double(f._tol),
+                    f._max_iter, ctypes.c_double(f._eps), int(f._redistribute), _pd(thole)))
+            elif isinstance(f, MBPolDispersionForce):
+                p = np.array(f._p, dtype=np.int32).reshape(-1, 2)
+                _check(L.mbpol_system_set_dispersion(h, len(f._p), _pi(_i(p[:, 0])), _pi(_i(p[:, 1])), f._ntypes,
+                                                     _pd(_d(f._c6)), _pd(_d(f._d6)), f._method,
+                                                     ctypes.c_double(f._cutoff)))
+            else:
+                raise MBPolError(1, 'unsupported force type %s' % type(f).__name__)
+    except Exception:
+        L.mbpol_system_destroy(h)
+        raise
+    return h
+
+
+class Context:
+    """A HIP context for one System on one GPU (precision 'mixed' or 'double')."""
+
+    def __init__(self, system, platform='HIP', precision='mixed', device=0):
+        if platform != 'HIP':
+            raise MBPolError(1, 'this package provides the HIP platform only')
+        self.system = system
+        self._precision = {'mixed': 0, 'double': 1}[precision]
+        self._positions = None
+        L = library()
+        sysh = _build_native_system(system)
+        self._h = ctypes.c_void_p()
+        try:
+            _check(L.mbpol_context_create(sysh, self._precision, int(device), ctypes.byref(self._h)))
+        finally:
+            L.mbpol_system_destroy(sysh)
+        self._mask = MBPOL_ALL
+
+    def __del__(self):
+        h = getattr(self, '_h', None)
+        if h is not None and h.value:
+            library().mbpol_context_destroy(h)
+            self._h = None
+
+    def setPositions(self, positions):
+        self._positions = self.system.computeVirtualSites(positions)
+
+    def getPositions(self):
+        return self._positions
+
+    def computeVirtualSites(self):
+        self._positions = self.system.computeVirtualSites(self._positions)
+
+    def setPeriodicBoxVectors(self, a, b, c):
+        box = _d([a[0], b[1], c[2]])
+        _check(library().mbpol_context_set_box(self._h, _pd(box)))
+        self.system._box = list(box)
+
+    def setComponents(self, mask):
+        self._mask = int(mask)
+
+    def setDipoleGuess(self, mode, dipoles=None):
+        modes = {'alpha_e': 0, 'zero': 1, 'previous': 2, 'user': 3}
+        arr = _d(dipoles) if dipoles is not None else None
+        _check(library().mbpol_context_set_dipole_guess(self._h, modes[mode], _pd(arr) if arr is not None else None))
+
+    def setListMode(self, full_chain_list):
+        _check(library().mbpol_context_set_list_mode(self._h, int(bool(full_chain_list))))
+
+    def getState(self, getEnergy=True, getForces=True, components=None):
+        if self._positions is None:
+            raise MBPolError(1, 'positions have not been set')
+        pos = _d(self._positions)
+        forces = np.zeros_like(pos)
+        energies = np.zeros(6)
+        mask = self._mask if components is None else int(components)
+        _check(library().mbpol_context_evaluate(self._h, _pd(pos), _pd(forces), _pd(energies), ctypes.c_uint32(mask)))
+        return State(energies, forces, self.getStats())
+
+    def getStats(self):
+        st = _EvalStats()
+        library().mbpol_context_get_stats(self._h, ctypes.byref(st))
+        return {k: getattr(st, k) for k, _ in _EvalStats._fields_}
+
+    def getInducedDipoles(self):
+        out = np.zeros((self.system.getNumParticles(), 3))
+        _check(library().mbpol_context_get_dipoles(self._h, _pd(out)))
+        return out
+
+    def getCharges(self):
+        out = np.zeros(self.system.getNumParticles())
+        _check(library().mbpol_context_get_charges(self._h, _pd(out)))
+        return out
+
+    def getDimerList(self):
+        c = ctypes.c_int64(0)
+        L = library()
+        _check(L.mbpol_context_get_dimer_list(self._h, None, ctypes.c_int64(0), ctypes.byref(c)))
+        out = np.zeros((max(c.value, 1), 2), dtype=np.int32)
+        _check(L.mbpol_context_get_dimer_list(self._h, _pi(out), ctypes.c_int64(c.value), ctypes.byref(c)))
+        return out[:c.value]
+
+    def getTrimerList(self):
+        c = ctypes.c_int64(0)
+        L = library()
+        _check(L.mbpol_context_get_trimer_list(self._h, None, ctypes.c_int64(0), ctypes.byref(c)))
+        out = np.zeros((max(c.value, 1), 3), dtype=np.int32)
+        _check(L.mbpol_context_get_trimer_list(self._h, _pi(out), ctypes.c_int64(c.value), ctypes.byref(c)))
+        return out[:c.value]
+
+    def _update_force(self, force):
+        if isinstance(force, MBPolElectrostaticsForce):
+            q, mol, typ, damp, pol = force._arrays()
+            _check(library().mbpol_context_update_electrostatics(self._h, len(q), _pd(_d(q)), _pi(_i(mol)),
+                                                                 _pi(_i(typ)), _pd(_d(damp)), _pd(_d(pol))))
+        else:
+            # molecule lists are rebuilt with a fresh context (counts must not change, as in the reference)
+            raise MBPolError(6, 'updateParametersInContext is supported for MBPolElectrostaticsForce only')
+
+
+def _context_get_potential(self, points):
+    """MBPolElectrostaticsForce::getElectrostaticPotential: potential (kJ/mol/e) at
+    points (nm) from the charges and induced dipoles of the last getState()."""
+    pts = _d(np.asarray(points, dtype=np.float64).reshape(-1, 3))
+    out = np.zeros(len(pts))
+    _check(library().mbpol_context_get_electrostatic_potential(self._h, len(pts), _pd(pts), _pd(out)))
+    return out
+
+
+Context.getElectrostaticPotential = _context_get_potential

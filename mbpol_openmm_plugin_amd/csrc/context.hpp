@@ -1,0 +1,119 @@
+// Host-side state of the MB-pol HIP platform: the system description (what
+// the reference keeps in its Force objects) and the per-GPU context (what its
+// Reference kernels keep between calls, MBPolReferenceKernels.h:45-244).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "kernels.h"
+
+struct mbpol_system {
+    int n = 0;
+    double box[3] = {0, 0, 0};
+    // ThreeParticleAverageSite
+    std::vector<int> vs_idx;     // [site, p1, p2, p3]
+    std::vector<double> vs_w;    // [w1, w2, w3]
+    // MBPolOneBodyForce
+    std::vector<int> ob_idx;
+    int ob_method = 0;
+    // MBPolTwoBodyForce
+    std::vector<int> tb_idx;
+    int tb_method = 0;
+    double tb_cutoff = 1e10;
+    // MBPolThreeBodyForce
+    std::vector<int> th_idx;
+    int th_method = 0;
+    double th_cutoff = 1e10;
+    // MBPolElectrostaticsForce
+    int ne = 0;
+    std::vector<double> el_q, el_damp, el_pol;
+    std::vector<int> el_mol, el_type;
+    int el_method = 0;
+    double el_cutoff = 0.9, el_alpha = 0.0, el_tol = 1e-4, el_eps = 1e-7;
+    int el_grid[3] = {0, 0, 0};
+    int el_max_iter = 200;
+    int el_redistribute = 1;
+    double el_thole[5] = {0.4, 0.4, 0.055, 0.626, 0.055};
+    // dispersion
+    int nd = 0;
+    std::vector<int> ds_type, ds_mol;
+    int ds_ntypes = 0;
+    std::vector<double> ds_c6, ds_d6;
+    int ds_method = 0;
+    double ds_cutoff = 1e10;
+};
+
+namespace mbpol {
+
+struct Elec;
+struct Disp;
+
+// Grow-only device allocation.
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    ~DevBuf();
+    hipError_t ensure(size_t nbytes);
+    template <typename T>
+    T* as() const { return static_cast<T*>(p); }
+};
+
+// One O-O neighbour-list build (cell grid + pass buffers).
+struct CellGrid {
+    ListGeom g{};
+    int ncell = 0;
+    DevBuf cell_of, cell_count, cell_start, cell_fill, members;
+};
+
+// HIP-event timing of kernel groups on the context's stream.
+enum TimerCat { T_EVALUATE, T_ONEBODY, T_LISTS, T_TWOBODY, T_THREEBODY, T_DISPERSION, T_ELEC_SETUP, T_ELEC_ITER,
+                T_ELEC_ENERGY, T_NCAT };
+struct Timer {
+    bool enabled = false;
+    std::vector<hipEvent_t> pool;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> pending[T_NCAT];
+    hipEvent_t open[T_NCAT] = {};
+    double ms[T_NCAT] = {};
+    long long count[T_NCAT] = {};
+};
+
+struct EvalStats {
+    int iterations = 0;
+    double epsilon = 0;
+    int converged = 1;
+    long long dimers = 0, trimers = 0, elec_pairs = 0, disp_pairs = 0;
+};
+
+}  // namespace mbpol
+
+struct mbpol_context {
+    mbpol_system sys;
+    int precision = 0;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int full_chain_list = 0;
+    mbpol::EvalStats stats;
+    mbpol::Timer timer;
+
+    // particle-level buffers
+    mbpol::DevBuf pos, pos_resident, force_fx, force, energy_fx, vs_idx, vs_w;
+    // pinned host staging
+    void* h_counts = nullptr;
+    // scan scratch
+    mbpol::DevBuf scan_tmp;
+    // 1B / 2B / 3B
+    mbpol::DevBuf ob_idx, tb_idx, th_idx;
+    mbpol::CellGrid grid2, grid3;
+    mbpol::DevBuf dimer_count, dimer_off, dimers;
+    long long dimer_cap = 0;
+    mbpol::DevBuf lo_count, lo_start, lower;
+    int lower_cap = 0;
+    mbpol::DevBuf trip_count, trip_off, trips;
+    long long trip_cap = 0;
+
+    // electrostatics and dispersion live in their own structures
+    mbpol::Elec* elec = nullptr;
+    mbpol::Disp* disp = nullptr;
+};

@@ -1,0 +1,83 @@
+// Device-side helpers shared by the MB-pol HIP kernels (gfx950).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "kernels.h"
+
+#define MBPOL_TABLE static __constant__
+#include "mbpol_params.h"
+
+namespace mbpol {
+
+// Forces and energies are accumulated as 64-bit fixed point (2^32 units per
+// kJ/mol/nm or kJ/mol): integer atomics commute, so every evaluation of the
+// same input gives bit-identical sums whatever order the waves run in.
+constexpr double kFixScale = 4294967296.0;
+constexpr double kFixInv = 1.0 / 4294967296.0;
+
+__device__ __forceinline__ unsigned long long to_fix(double v) {
+    return (unsigned long long)__double2ll_rn(v * kFixScale);
+}
+
+__device__ __forceinline__ void add_force(unsigned long long* f, int atom, double fx, double fy, double fz) {
+    atomicAdd(&f[3 * atom + 0], to_fix(fx));
+    atomicAdd(&f[3 * atom + 1], to_fix(fy));
+    atomicAdd(&f[3 * atom + 2], to_fix(fz));
+}
+
+// Wave-level sum (64 lanes) followed by one atomic per wave.
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+__device__ __forceinline__ void add_energy(unsigned long long* e, double v) {
+    v = wave_sum(v);
+    if ((threadIdx.x & 63) == 0) atomicAdd(e, to_fix(v));
+}
+
+__device__ __forceinline__ d3 load_d3(const double* p, int i) { return d3{p[3 * i], p[3 * i + 1], p[3 * i + 2]}; }
+
+// Minimum image of p relative to ref, component-wise: p += L * floor((ref - p)/L + 0.5)
+// (platforms/reference/src/MBPolReferenceTwoBodyForce.cpp:66-76).
+__device__ __forceinline__ double image_coord(double ref, double p, double L) {
+    const double d = ref - p;
+    return p + L * floor(d / L + 0.5);
+}
+
+__device__ __forceinline__ d3 image_to(const d3& ref, const d3& p, const d3& L) {
+    return d3{image_coord(ref.x, p.x, L.x), image_coord(ref.y, p.y, L.y), image_coord(ref.z, p.z, L.z)};
+}
+
+template <typename T>
+struct v3 {
+    T x, y, z;
+    __device__ __forceinline__ v3() {}
+    __device__ __forceinline__ v3(T a, T b, T c) : x(a), y(b), z(c) {}
+    __device__ __forceinline__ v3 operator+(const v3& o) const { return v3(x + o.x, y + o.y, z + o.z); }
+    __device__ __forceinline__ v3 operator-(const v3& o) const { return v3(x - o.x, y - o.y, z - o.z); }
+    __device__ __forceinline__ v3 operator*(T s) const { return v3(x * s, y * s, z * s); }
+    __device__ __forceinline__ v3& operator+=(const v3& o) { x += o.x; y += o.y; z += o.z; return *this; }
+    __device__ __forceinline__ v3& operator-=(const v3& o) { x -= o.x; y -= o.y; z -= o.z; return *this; }
+    __device__ __forceinline__ T dot(const v3& o) const { return x * o.x + y * o.y + z * o.z; }
+    __device__ __forceinline__ v3 cross(const v3& o) const {
+        return v3(y * o.z - z * o.y, z * o.x - x * o.z, x * o.y - y * o.x);
+    }
+};
+
+template <typename T>
+__device__ __forceinline__ T exp_t(T x);
+template <>
+__device__ __forceinline__ float exp_t<float>(float x) { return expf(x); }
+template <>
+__device__ __forceinline__ double exp_t<double>(double x) { return exp(x); }
+
+template <typename T>
+__device__ __forceinline__ T rsqrt_t(T x);
+template <>
+__device__ __forceinline__ float rsqrt_t<float>(float x) { return 1.0f / sqrtf(x); }
+template <>
+__device__ __forceinline__ double rsqrt_t<double>(double x) { return 1.0 / sqrt(x); }
+
+}  // namespace mbpol

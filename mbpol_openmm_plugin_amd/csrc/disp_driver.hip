@@ -1,0 +1,110 @@
+// Host driver of the damped-C6 dispersion (python/mbpol.xml:37-108).
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "../../include/mbpol_hip.h"
+#include "disp_kernels.h"
+#include "elec.hpp"
+
+namespace mbpol {
+
+#define HIP_TRY(expr)                                                                                   \
+    do {                                                                                                \
+        hipError_t _e = (expr);                                                                         \
+        if (_e != hipSuccess) return set_error(MBPOL_ERR_HIP, "%s: %s", #expr, hipGetErrorString(_e)); \
+    } while (0)
+
+struct Disp {
+    DispGeom g{};
+    DevBuf act, type, mol, c6, d6, cell_of, cell_count, cell_start, cell_fill, members, eblk, npairs;
+    int ncell = 1;
+};
+
+template <typename T>
+static int upv(DevBuf& b, const std::vector<T>& v) {
+    HIP_TRY(b.ensure(std::max<size_t>(1, v.size()) * sizeof(T)));
+    if (!v.empty()) HIP_TRY(hipMemcpy(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    return MBPOL_OK;
+}
+
+int disp_set_box(mbpol_context* ctx) {
+    Disp* D = ctx->disp;
+    const mbpol_system& s = ctx->sys;
+    D->g.L = d3{s.box[0], s.box[1], s.box[2]};
+    D->g.periodic = s.ds_method == MBPOL_CUTOFF_PERIODIC;
+    D->g.use_cells = 0;
+    D->g.nc = make_int3(1, 1, 1);
+    if (D->g.periodic) {
+        int nc[3];
+        for (int d = 0; d < 3; ++d) nc[d] = (int)std::floor(s.box[d] / s.ds_cutoff);
+        if (nc[0] >= 3 && nc[1] >= 3 && nc[2] >= 3) {
+            D->g.use_cells = 1;
+            D->g.nc = make_int3(nc[0], nc[1], nc[2]);
+        }
+    }
+    D->ncell = D->g.nc.x * D->g.nc.y * D->g.nc.z;
+    HIP_TRY(D->cell_count.ensure(sizeof(int) * (D->ncell + 1)));
+    HIP_TRY(D->cell_start.ensure(sizeof(int) * (D->ncell + 1)));
+    HIP_TRY(D->cell_fill.ensure(sizeof(int) * (D->ncell + 1)));
+    return MBPOL_OK;
+}
+
+int disp_create(mbpol_context* ctx) {
+    const mbpol_system& s = ctx->sys;
+    Disp* D = new Disp();
+    ctx->disp = D;
+    // particles whose type has a non-zero C6 row take part (M sites do not)
+    std::vector<char> active(s.ds_ntypes, 0);
+    for (int t = 0; t < s.ds_ntypes; ++t)
+        for (int u = 0; u < s.ds_ntypes; ++u)
+            if (s.ds_c6[t + s.ds_ntypes * u] != 0.0) active[t] = 1;
+    std::vector<int> act;
+    for (int i = 0; i < s.nd; ++i)
+        if (active[s.ds_type[i]]) act.push_back(i);
+    D->g.na = (int)act.size();
+    D->g.ntypes = s.ds_ntypes;
+    D->g.method = s.ds_method;
+    D->g.cutoff = s.ds_cutoff;
+    if (int e = upv(D->act, act)) return e;
+    if (int e = upv(D->type, s.ds_type)) return e;
+    if (int e = upv(D->mol, s.ds_mol)) return e;
+    if (int e = upv(D->c6, s.ds_c6)) return e;
+    if (int e = upv(D->d6, s.ds_d6)) return e;
+    HIP_TRY(D->cell_of.ensure(sizeof(int) * std::max(1, D->g.na)));
+    HIP_TRY(D->members.ensure(sizeof(int) * std::max(1, D->g.na)));
+    HIP_TRY(D->eblk.ensure(sizeof(double) * std::max(1, D->g.na)));
+    HIP_TRY(D->npairs.ensure(sizeof(unsigned long long)));
+    return disp_set_box(ctx);
+}
+
+void disp_destroy(mbpol_context* ctx) {
+    delete ctx->disp;
+    ctx->disp = nullptr;
+}
+
+void launch_energy_finish(hipStream_t s, int nblocks, const double* eblk, u64* energy);
+
+int disp_evaluate(mbpol_context* ctx, u64* energy) {
+    Disp* D = ctx->disp;
+    hipStream_t st = ctx->stream;
+    const double* pos = ctx->pos.as<double>();
+    if (D->g.na == 0) return MBPOL_OK;
+    if (D->g.use_cells) {
+        HIP_TRY(hipMemsetAsync(D->cell_count.p, 0, sizeof(int) * (D->ncell + 1), st));
+        HIP_TRY(hipMemsetAsync(D->cell_fill.p, 0, sizeof(int) * (D->ncell + 1), st));
+        launch_disp_cells(st, D->g.na, D->act.as<int>(), pos, D->g.L, D->g.nc, D->cell_of.as<int>(), D->cell_count.as<int>());
+        if (int e = scan_i32(ctx, D->cell_count.as<int>(), D->cell_start.as<int>(), D->ncell + 1)) return e;
+        launch_disp_cell_fill(st, D->g.na, D->cell_of.as<int>(), D->cell_start.as<int>(), D->cell_fill.as<int>(),
+                              D->members.as<int>());
+    }
+    HIP_TRY(hipMemsetAsync(D->npairs.p, 0, sizeof(unsigned long long), st));
+    launch_disp(st, ctx->precision == MBPOL_PRECISION_DOUBLE, D->g, D->act.as<int>(), pos, D->type.as<int>(),
+                D->mol.as<int>(), D->c6.as<double>(), D->d6.as<double>(), D->cell_of.as<int>(), D->cell_start.as<int>(),
+                D->members.as<int>(), ctx->force_fx.as<u64>(), D->eblk.as<double>(),
+                D->npairs.as<unsigned long long>());
+    launch_energy_finish(st, D->g.na, D->eblk.as<double>(), energy);
+    return MBPOL_OK;
+}
+
+}  // namespace mbpol

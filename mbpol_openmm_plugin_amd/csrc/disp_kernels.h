@@ -1,0 +1,25 @@
+// Types and launchers of the dispersion kernels (kernels_disp.hip).
+#pragma once
+#include "kernels.h"
+
+namespace mbpol {
+
+struct DispGeom {
+    int na;         // active particles (types with a non-zero C6 row)
+    int ntypes;
+    int method;     // 0 no cutoff, 1 cutoff periodic, 2 cutoff non-periodic
+    int periodic;
+    double cutoff;  // nm
+    d3 L;
+    int use_cells;
+    int3 nc;
+};
+
+void launch_disp_cells(hipStream_t s, int na, const int* act, const double* pos, d3 L, int3 nc, int* cell_of,
+                       int* cell_count);
+void launch_disp_cell_fill(hipStream_t s, int na, const int* cell_of, const int* cell_start, int* cell_fill, int* members);
+void launch_disp(hipStream_t s, int precision_double, DispGeom g, const int* act, const double* pos, const int* type,
+                 const int* mol, const double* c6, const double* d6, const int* cell_of, const int* cell_start,
+                 const int* members, u64* force, double* eblk, unsigned long long* npairs);
+
+}  // namespace mbpol

@@ -1,0 +1,28 @@
+// Declarations shared between the evaluation driver (mbpol_api.hip) and the
+// electrostatics / dispersion drivers.
+#pragma once
+#include "context.hpp"
+
+namespace mbpol {
+
+constexpr double kFixInvHost = 1.0 / 4294967296.0;
+
+int set_error(int code, const char* fmt, ...);
+int scan_i32(mbpol_context* ctx, const int* in, int* out, int n);
+int scan_i64(mbpol_context* ctx, const long long* in, long long* out, int n);
+void timer_start(mbpol_context* ctx, int cat);
+void timer_stop(mbpol_context* ctx, int cat);
+
+// electrostatics (elec_driver.hip)
+int elec_create(mbpol_context* ctx);
+void elec_destroy(mbpol_context* ctx);
+int elec_set_box(mbpol_context* ctx);
+int elec_evaluate(mbpol_context* ctx, u64* energy);
+
+// dispersion (disp_driver.hip)
+int disp_create(mbpol_context* ctx);
+void disp_destroy(mbpol_context* ctx);
+int disp_set_box(mbpol_context* ctx);
+int disp_evaluate(mbpol_context* ctx, u64* energy);
+
+}  // namespace mbpol

@@ -1,0 +1,439 @@
+// Host driver of the electrostatics: PME parameters, B-spline moduli, hipFFT
+// plans, the induced-dipole iteration and the energy/force pass.
+// Follows MBPolReferenceKernels.cpp:141-259 (set-up) and
+// MBPolReferenceElectrostaticsForce.cpp:557-647, :857-921, :2718-2777.
+#include <hipfft/hipfft.h>
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "../../include/mbpol_hip.h"
+#include "elec.hpp"
+#include "elec_kernels.h"
+#include "mbpol_params.h"
+
+namespace mbpol {
+
+#define HIP_TRY(expr)                                                                                   \
+    do {                                                                                                \
+        hipError_t _e = (expr);                                                                         \
+        if (_e != hipSuccess) return set_error(MBPOL_ERR_HIP, "%s: %s", #expr, hipGetErrorString(_e)); \
+    } while (0)
+#define FFT_TRY(expr)                                                                        \
+    do {                                                                                     \
+        hipfftResult _r = (expr);                                                            \
+        if (_r != HIPFFT_SUCCESS) return set_error(MBPOL_ERR_HIP, "%s: hipFFT error %d", #expr, (int)_r); \
+    } while (0)
+
+struct Elec {
+    ElecParams P{};
+    int n = 0, nw = 0;
+    int guess_mode = MBPOL_GUESS_ALPHA_E;
+    bool have_previous = false;
+    // parameters
+    DevBuf q0, q, damp, pol, mol, type, dq;
+    // neighbour rows
+    DevBuf cell_of, cell_count, cell_start, cell_fill, members;
+    int3 nc{1, 1, 1};
+    int use_cells = 0;
+    DevBuf nb_count, nb_start, nb, pf;
+    long long nb_cap = 0;
+    // fields and dipoles
+    DevBuf efix, efix_pol, field, mu, mu_prev, mu_user, partial, eps;
+    DevBuf phi_fix, phi_ind, fsite, V, eblk;
+    // PME
+    DevBuf grid, spec, influence, bx, by, bz;
+    hipfftHandle plan_f = 0, plan_b = 0;
+    bool plans = false;
+    double* h_eps = nullptr;
+};
+
+// B-spline moduli with the reference's small-modulus fix-up and order-5 zeta
+// correction (MBPolReferenceElectrostaticsForce.cpp:1248-1340), host FP64.
+static std::vector<double> bspline_moduli(int n) {
+    const int order = mbpol_params::el_pme_order;
+    double arr[8];
+    const double x = 0.0;
+    arr[0] = 1.0 - x;
+    arr[1] = x;
+    for (int k = 2; k < order; ++k) {
+        const double den = 1.0 / k;
+        arr[k] = x * arr[k - 1] * den;
+        for (int i = 1; i < k; ++i) arr[k - i] = ((x + i) * arr[k - i - 1] + ((k - i + 1) - x) * arr[k - i]) * den;
+        arr[0] = (1.0 - x) * arr[0] * den;
+    }
+    std::vector<double> bs(std::max(n, order + 2) + 1, 0.0);
+    for (int i = 2; i <= order + 1; ++i) bs[i] = arr[i - 2];
+    std::vector<double> mod(n, 0.0);
+    const double factor = 2.0 * M_PI / n;
+    for (int i = 0; i < n; ++i) {
+        double s1 = 0, s2 = 0;
+        for (int j = 1; j <= n; ++j) {
+            const double ang = factor * i * (j - 1);
+            s1 += bs[j] * std::cos(ang);
+            s2 += bs[j] * std::sin(ang);
+        }
+        mod[i] = s1 * s1 + s2 * s2;
+    }
+    const double tiny = 1.0e-7;
+    if (mod[0] < tiny) mod[0] = 0.5 * mod[1];
+    for (int i = 1; i < n - 1; ++i)
+        if (mod[i] < tiny) mod[i] = 0.5 * (mod[i - 1] + mod[i + 1]);
+    if (mod[n - 1] < tiny) mod[n - 1] = 0.5 * mod[n - 2];
+    for (int i = 1; i <= n; ++i) {
+        int k = i - 1;
+        if (i > n / 2) k = k - n;
+        double zeta = 1.0;
+        if (k != 0) {
+            double s1 = 1.0, s2 = 1.0;
+            const double f = M_PI * k / n;
+            for (int j = 1; j <= 50; ++j) {
+                const double a = f / (f + M_PI * j);
+                s1 += std::pow(a, order);
+                s2 += std::pow(a, 2 * order);
+            }
+            for (int j = 1; j <= 50; ++j) {
+                const double a = f / (f - M_PI * j);
+                s1 += std::pow(a, order);
+                s2 += std::pow(a, 2 * order);
+            }
+            zeta = s2 / s1;
+        }
+        mod[i - 1] *= zeta * zeta;
+    }
+    return mod;
+}
+
+static SiteParams site_params(Elec* E) {
+    return SiteParams{E->q.as<double>(), E->damp.as<double>(), E->pol.as<double>(), E->mol.as<int>(), E->type.as<int>()};
+}
+
+template <typename T>
+static int up(DevBuf& b, const std::vector<T>& v) {
+    HIP_TRY(b.ensure(std::max<size_t>(1, v.size()) * sizeof(T)));
+    if (!v.empty()) HIP_TRY(hipMemcpy(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    return MBPOL_OK;
+}
+
+int elec_set_box(mbpol_context* ctx) {
+    Elec* E = ctx->elec;
+    const mbpol_system& s = ctx->sys;
+    ElecParams& P = E->P;
+    P.box = d3{s.box[0], s.box[1], s.box[2]};
+    P.inv_box = d3{s.box[0] > 0 ? 1.0 / s.box[0] : 0.0, s.box[1] > 0 ? 1.0 / s.box[1] : 0.0,
+                   s.box[2] > 0 ? 1.0 / s.box[2] : 0.0};
+    if (!P.pme) return MBPOL_OK;
+    int nc[3];
+    for (int d = 0; d < 3; ++d) nc[d] = (int)std::floor(s.box[d] / P.cutoff);
+    E->use_cells = (nc[0] >= 3 && nc[1] >= 3 && nc[2] >= 3) ? 1 : 0;
+    E->nc = E->use_cells ? make_int3(nc[0], nc[1], nc[2]) : make_int3(1, 1, 1);
+    const int ncell = E->nc.x * E->nc.y * E->nc.z;
+    HIP_TRY(E->cell_count.ensure(sizeof(int) * (ncell + 1)));
+    HIP_TRY(E->cell_start.ensure(sizeof(int) * (ncell + 1)));
+    HIP_TRY(E->cell_fill.ensure(sizeof(int) * (ncell + 1)));
+    launch_influence(ctx->stream, P, E->bx.as<double>(), E->by.as<double>(), E->bz.as<double>(), E->influence.as<double>());
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return MBPOL_OK;
+}
+
+int elec_create(mbpol_context* ctx) {
+    const mbpol_system& s = ctx->sys;
+    Elec* E = new Elec();
+    ctx->elec = E;
+    E->n = s.ne;
+    E->nw = s.el_redistribute ? s.ne / 4 : 0;
+    ElecParams& P = E->P;
+    P.n = s.ne;
+    P.pme = s.el_method == MBPOL_ELEC_PME;
+    P.redistribute = s.el_redistribute;
+    for (int k = 0; k < 5; ++k) P.thole[k] = s.el_thole[k];
+    P.cutoff = P.pme ? s.el_cutoff : 1e30;
+    P.alpha = s.el_alpha;
+    int grid[3] = {s.el_grid[0], s.el_grid[1], s.el_grid[2]};
+    if (P.pme && (grid[0] == 0 || P.alpha == 0.0)) {
+        // NonbondedForceImpl::calcPMEParameters as called at MBPolReferenceKernels.cpp:186-199
+        const double tol = s.el_tol;
+        P.alpha = (1.0 / s.el_cutoff) * std::sqrt(-std::log(2.0 * tol));
+        for (int d = 0; d < 3; ++d)
+            grid[d] = std::max((int)std::ceil(2 * P.alpha * s.box[d] / (3 * std::pow(tol, 0.2))), 6);
+    }
+    P.grid = make_int3(grid[0], grid[1], grid[2]);
+    const int n = E->n;
+    if (int e = up(E->q0, s.el_q)) return e;
+    if (int e = up(E->damp, s.el_damp)) return e;
+    if (int e = up(E->pol, s.el_pol)) return e;
+    if (int e = up(E->mol, s.el_mol)) return e;
+    if (int e = up(E->type, s.el_type)) return e;
+    HIP_TRY(E->q.ensure(sizeof(double) * n));
+    HIP_TRY(E->dq.ensure(sizeof(double) * 27 * std::max(E->nw, 1)));
+    for (DevBuf* b : {&E->efix, &E->efix_pol, &E->field, &E->mu, &E->mu_prev, &E->mu_user, &E->fsite})
+        HIP_TRY(b->ensure(sizeof(double) * 3 * n));
+    HIP_TRY(E->V.ensure(sizeof(double) * n));
+    HIP_TRY(E->phi_fix.ensure(sizeof(double) * 10 * n));
+    HIP_TRY(E->phi_ind.ensure(sizeof(double) * 10 * n));
+    HIP_TRY(E->partial.ensure(sizeof(double) * (sor_blocks(n) + 1)));
+    HIP_TRY(E->eps.ensure(sizeof(double) * 4));
+    HIP_TRY(E->eblk.ensure(sizeof(double) * (std::max(energy_blocks(n), sor_blocks(n)) + 1)));
+    HIP_TRY(E->cell_of.ensure(sizeof(int) * n));
+    HIP_TRY(E->members.ensure(sizeof(int) * n));
+    HIP_TRY(E->nb_count.ensure(sizeof(int) * (n + 1)));
+    HIP_TRY(E->nb_start.ensure(sizeof(int) * (n + 1)));
+    HIP_TRY(E->cell_count.ensure(sizeof(int) * 2));
+    HIP_TRY(E->cell_start.ensure(sizeof(int) * 2));
+    HIP_TRY(E->cell_fill.ensure(sizeof(int) * 2));
+    HIP_TRY(hipHostMalloc((void**)&E->h_eps, 64, hipHostMallocDefault));
+    if (P.pme) {
+        const size_t nreal = (size_t)grid[0] * grid[1] * grid[2];
+        const size_t nhalf = (size_t)grid[0] * grid[1] * (grid[2] / 2 + 1);
+        HIP_TRY(E->grid.ensure(sizeof(double) * nreal));
+        HIP_TRY(E->spec.ensure(sizeof(double2) * nhalf));
+        HIP_TRY(E->influence.ensure(sizeof(double) * nhalf));
+        if (int e = up(E->bx, bspline_moduli(grid[0]))) return e;
+        if (int e = up(E->by, bspline_moduli(grid[1]))) return e;
+        if (int e = up(E->bz, bspline_moduli(grid[2]))) return e;
+        FFT_TRY(hipfftPlan3d(&E->plan_f, grid[0], grid[1], grid[2], HIPFFT_D2Z));
+        FFT_TRY(hipfftPlan3d(&E->plan_b, grid[0], grid[1], grid[2], HIPFFT_Z2D));
+        E->plans = true;
+        FFT_TRY(hipfftSetStream(E->plan_f, ctx->stream));
+        FFT_TRY(hipfftSetStream(E->plan_b, ctx->stream));
+    }
+    return elec_set_box(ctx);
+}
+
+void elec_destroy(mbpol_context* ctx) {
+    Elec* E = ctx->elec;
+    if (!E) return;
+    if (E->plans) {
+        hipfftDestroy(E->plan_f);
+        hipfftDestroy(E->plan_b);
+    }
+    if (E->h_eps) (void)hipHostFree(E->h_eps);
+    delete E;
+    ctx->elec = nullptr;
+}
+
+// neighbour rows of every site (PME: within the cutoff; direct sum: every other site)
+static int build_pairs(mbpol_context* ctx) {
+    Elec* E = ctx->elec;
+    hipStream_t st = ctx->stream;
+    const int n = E->n;
+    const double* pos = ctx->pos.as<double>();
+    const int ncell = E->nc.x * E->nc.y * E->nc.z;
+    if (E->use_cells) {
+        HIP_TRY(hipMemsetAsync(E->cell_count.p, 0, sizeof(int) * (ncell + 1), st));
+        HIP_TRY(hipMemsetAsync(E->cell_fill.p, 0, sizeof(int) * (ncell + 1), st));
+        launch_site_cells(st, n, pos, E->P, E->nc, E->cell_of.as<int>(), E->cell_count.as<int>());
+        if (int e = scan_i32(ctx, E->cell_count.as<int>(), E->cell_start.as<int>(), ncell + 1)) return e;
+        launch_site_cell_fill(st, n, E->cell_of.as<int>(), E->cell_start.as<int>(), E->cell_fill.as<int>(),
+                              E->members.as<int>());
+    }
+    HIP_TRY(hipMemsetAsync(E->nb_count.p, 0, sizeof(int) * (n + 1), st));
+    launch_site_pairs(st, n, pos, E->P, E->nc, E->use_cells, E->cell_of.as<int>(), E->cell_start.as<int>(),
+                      E->members.as<int>(), E->nb_count.as<int>(), nullptr, nullptr);
+    if (int e = scan_i32(ctx, E->nb_count.as<int>(), E->nb_start.as<int>(), n + 1)) return e;
+    int* h = reinterpret_cast<int*>(E->h_eps) + 8;
+    HIP_TRY(hipMemcpyAsync(h, E->nb_start.as<int>() + n, sizeof(int), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    const long long npair = *h;
+    if (npair > E->nb_cap) {
+        const long long cap = npair + npair / 4 + 64;
+        HIP_TRY(E->nb.ensure(sizeof(int) * cap));
+        HIP_TRY(E->pf.ensure(sizeof(double2) * cap));
+        E->nb_cap = cap;
+    }
+    launch_site_pairs(st, n, pos, E->P, E->nc, E->use_cells, E->cell_of.as<int>(), E->cell_start.as<int>(),
+                      E->members.as<int>(), nullptr, E->nb_start.as<int>(), E->nb.as<int>());
+    ctx->stats.elec_pairs = npair / 2;
+    return MBPOL_OK;
+}
+
+// reciprocal-space potential of the charges (mode 0) or the dipoles (mode 1)
+static int pme_potential(mbpol_context* ctx, int mode, double* phi) {
+    Elec* E = ctx->elec;
+    hipStream_t st = ctx->stream;
+    const ElecParams& P = E->P;
+    const size_t nreal = (size_t)P.grid.x * P.grid.y * P.grid.z;
+    const size_t nhalf = (size_t)P.grid.x * P.grid.y * (P.grid.z / 2 + 1);
+    HIP_TRY(hipMemsetAsync(E->grid.p, 0, sizeof(double) * nreal, st));
+    launch_spread(st, E->n, mode, ctx->pos.as<double>(), E->q.as<double>(), E->mu.as<double>(), P, E->grid.as<double>());
+    FFT_TRY(hipfftExecD2Z(E->plan_f, E->grid.as<double>(), reinterpret_cast<hipfftDoubleComplex*>(E->spec.p)));
+    launch_convolve(st, nhalf, E->influence.as<double>(), E->spec.as<double2>());
+    FFT_TRY(hipfftExecZ2D(E->plan_b, reinterpret_cast<hipfftDoubleComplex*>(E->spec.p), E->grid.as<double>()));
+    launch_gather(st, E->n, ctx->pos.as<double>(), P, E->grid.as<double>(), phi);
+    return MBPOL_OK;
+}
+
+int elec_evaluate(mbpol_context* ctx, u64* energy) {
+    Elec* E = ctx->elec;
+    const mbpol_system& s = ctx->sys;
+    hipStream_t st = ctx->stream;
+    const int n = E->n;
+    const ElecParams& P = E->P;
+    const double* pos = ctx->pos.as<double>();
+    const SiteParams S = site_params(E);
+    // charges (setup, :857-897)
+    timer_start(ctx, T_ELEC_SETUP);
+    launch_site_charges_copy(st, n, E->q0.as<double>(), E->q.as<double>());
+    // sites of water w are 4w..4w+3 (the reference's ii += 4 loop, :881)
+    if (P.redistribute) launch_water_charges(st, E->nw, nullptr, pos, E->q.as<double>(), E->dq.as<double>());
+    if (int e = build_pairs(ctx)) return e;
+    launch_pair_prefactors(st, n, pos, S, P, E->nb_start.as<int>(), E->nb.as<int>(), E->pf.as<double2>());
+    // fixed field
+    if (P.pme) {
+        if (int e = pme_potential(ctx, 0, E->phi_fix.as<double>())) return e;
+        launch_record_field(st, n, P, E->phi_fix.as<double>(), 0, 0.0, nullptr, E->efix.as<double>());
+    } else {
+        HIP_TRY(hipMemsetAsync(E->efix.p, 0, sizeof(double) * 3 * n, st));
+    }
+    launch_fixed_field(st, n, pos, S, P, E->nb_start.as<int>(), E->nb.as<int>(), E->efix.as<double>());
+    // alpha * E_fixed, and the initial guess
+    launch_scale_copy(st, n, E->efix.as<double>(), E->pol.as<double>(), E->efix_pol.as<double>(), nullptr);
+    switch (E->guess_mode) {
+        case MBPOL_GUESS_ZERO: HIP_TRY(hipMemsetAsync(E->mu.p, 0, sizeof(double) * 3 * n, st)); break;
+        case MBPOL_GUESS_PREVIOUS:
+            if (E->have_previous) {
+                HIP_TRY(hipMemcpyAsync(E->mu.p, E->mu_prev.p, sizeof(double) * 3 * n, hipMemcpyDeviceToDevice, st));
+                break;
+            }
+            [[fallthrough]];
+        case MBPOL_GUESS_USER:
+            if (E->guess_mode == MBPOL_GUESS_USER) {
+                HIP_TRY(hipMemcpyAsync(E->mu.p, E->mu_user.p, sizeof(double) * 3 * n, hipMemcpyDeviceToDevice, st));
+                break;
+            }
+            [[fallthrough]];
+        default: HIP_TRY(hipMemcpyAsync(E->mu.p, E->efix_pol.p, sizeof(double) * 3 * n, hipMemcpyDeviceToDevice, st));
+    }
+    timer_stop(ctx, T_ELEC_SETUP);
+    // SOR iteration (convergeInduceDipoles, :557-616)
+    const double self_term = (4.0 / 3.0) * (P.alpha * P.alpha * P.alpha) / mbpol_params::el_sqrt_pi;
+    const int nb = sor_blocks(n);
+    double cur = 1.0e+50;
+    int it = 0;
+    bool conv = false;
+    for (;;) {
+        timer_start(ctx, T_ELEC_ITER);
+        launch_induced_field(st, n, pos, P, E->nb_start.as<int>(), E->nb.as<int>(), E->pf.as<double2>(),
+                             E->mu.as<double>(), E->field.as<double>());
+        if (P.pme) {
+            if (int e = pme_potential(ctx, 1, E->phi_ind.as<double>())) return e;
+            launch_record_field(st, n, P, E->phi_ind.as<double>(), 1, self_term, E->mu.as<double>(), E->field.as<double>());
+        }
+        launch_sor_update(st, n, E->pol.as<double>(), E->efix_pol.as<double>(), E->field.as<double>(),
+                          mbpol_params::el_polar_sor, E->mu.as<double>(), E->partial.as<double>());
+        launch_eps_finish(st, nb, n, E->partial.as<double>(), E->eps.as<double>());
+        timer_stop(ctx, T_ELEC_ITER);
+        HIP_TRY(hipMemcpyAsync(E->h_eps, E->eps.p, sizeof(double), hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+        const double eps = E->h_eps[0];
+        bool done = false;
+        if (eps < s.el_eps) {
+            conv = true;
+            done = true;
+        } else if (cur < eps || it >= s.el_max_iter) {
+            done = true;
+        }
+        cur = eps;
+        ++it;
+        if (done) break;
+    }
+    ctx->stats.iterations = it;
+    ctx->stats.epsilon = cur;
+    ctx->stats.converged = conv ? 1 : 0;
+    if (!conv)
+        return set_error(MBPOL_ERR_NOT_CONVERGED, "Induced dipoles did not converge:  iterations=%d eps=%g", it, cur);
+    HIP_TRY(hipMemcpyAsync(E->mu_prev.p, E->mu.p, sizeof(double) * 3 * n, hipMemcpyDeviceToDevice, st));
+    E->have_previous = true;
+    // energy and forces (calculateElectrostatic, :838-855 direct sum, :2718-2777 PME)
+    timer_start(ctx, T_ELEC_ENERGY);
+    HIP_TRY(hipMemsetAsync(E->fsite.p, 0, sizeof(double) * 3 * n, st));
+    HIP_TRY(hipMemsetAsync(E->V.p, 0, sizeof(double) * n, st));
+    launch_elec_energy(st, n, pos, S, P, E->nb_start.as<int>(), E->nb.as<int>(), E->mu.as<double>(), E->fsite.as<double>(),
+                       E->V.as<double>(), E->eblk.as<double>());
+    launch_energy_finish(st, energy_blocks(n), E->eblk.as<double>(), energy);
+    if (P.pme) {
+        launch_recip_site(st, n, S, P, E->phi_fix.as<double>(), E->phi_ind.as<double>(), E->mu.as<double>(),
+                          E->fsite.as<double>(), E->V.as<double>(), E->eblk.as<double>());
+        launch_energy_finish(st, sor_blocks(n), E->eblk.as<double>(), energy);
+    }
+    if (P.redistribute) launch_charge_deriv_forces(st, E->nw, E->dq.as<double>(), E->V.as<double>(), E->fsite.as<double>());
+    launch_add_forces_fix(st, n, E->fsite.as<double>(), ctx->force_fx.as<u64>());
+    timer_stop(ctx, T_ELEC_ENERGY);
+    return MBPOL_OK;
+}
+
+}  // namespace mbpol
+
+using mbpol::Elec;
+
+extern "C" {
+
+int mbpol_context_get_dipoles(const mbpol_context* ctx, double* dipoles) {
+    using namespace mbpol;
+    if (!ctx->elec) return set_error(MBPOL_ERR_INVALID_ARGUMENT, "no electrostatics in this system");
+    HIP_TRY(hipMemcpy(dipoles, ctx->elec->mu.p, sizeof(double) * 3 * ctx->elec->n, hipMemcpyDeviceToHost));
+    return MBPOL_OK;
+}
+
+int mbpol_context_get_charges(const mbpol_context* ctx, double* charges) {
+    using namespace mbpol;
+    if (!ctx->elec) return set_error(MBPOL_ERR_INVALID_ARGUMENT, "no electrostatics in this system");
+    HIP_TRY(hipMemcpy(charges, ctx->elec->q.p, sizeof(double) * ctx->elec->n, hipMemcpyDeviceToHost));
+    return MBPOL_OK;
+}
+
+int mbpol_context_set_dipole_guess(mbpol_context* ctx, int32_t mode, const double* dipoles) {
+    using namespace mbpol;
+    if (!ctx->elec) return set_error(MBPOL_ERR_INVALID_ARGUMENT, "no electrostatics in this system");
+    if (mode < MBPOL_GUESS_ALPHA_E || mode > MBPOL_GUESS_USER) return set_error(MBPOL_ERR_INVALID_ARGUMENT, "bad guess mode");
+    if (mode == MBPOL_GUESS_USER) {
+        if (!dipoles) return set_error(MBPOL_ERR_INVALID_ARGUMENT, "user guess needs dipoles");
+        HIP_TRY(hipSetDevice(ctx->device));
+        HIP_TRY(hipMemcpy(ctx->elec->mu_user.p, dipoles, sizeof(double) * 3 * ctx->elec->n, hipMemcpyHostToDevice));
+    }
+    ctx->elec->guess_mode = mode;
+    return MBPOL_OK;
+}
+
+int mbpol_context_get_electrostatic_potential(mbpol_context* ctx, int32_t npoints, const double* points,
+                                              double* potential) {
+    using namespace mbpol;
+    if (!ctx->elec) return set_error(MBPOL_ERR_INVALID_ARGUMENT, "no electrostatics in this system");
+    if (npoints <= 0) return MBPOL_OK;
+    HIP_TRY(hipSetDevice(ctx->device));
+    DevBuf pts, out;
+    HIP_TRY(pts.ensure(sizeof(double) * 3 * npoints));
+    HIP_TRY(out.ensure(sizeof(double) * npoints));
+    HIP_TRY(hipMemcpy(pts.p, points, sizeof(double) * 3 * npoints, hipMemcpyHostToDevice));
+    Elec* E = ctx->elec;
+    launch_potential_at_points(ctx->stream, npoints, pts.as<double>(), E->n, ctx->pos.as<double>(), E->q.as<double>(),
+                               E->mu.as<double>(), E->P, out.as<double>());
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    HIP_TRY(hipMemcpy(potential, out.p, sizeof(double) * npoints, hipMemcpyDeviceToHost));
+    return MBPOL_OK;
+}
+
+int mbpol_context_update_electrostatics(mbpol_context* ctx, int32_t n, const double* charge, const int32_t* molecule,
+                                        const int32_t* atom_type, const double* damping, const double* polarity) {
+    using namespace mbpol;
+    if (!ctx->elec) return set_error(MBPOL_ERR_INVALID_ARGUMENT, "no electrostatics in this system");
+    if (n != ctx->elec->n)
+        return set_error(MBPOL_ERR_COUNT_CHANGED, "updateParametersInContext: The number of multipoles has changed");
+    mbpol_system& s = ctx->sys;
+    s.el_q.assign(charge, charge + n);
+    s.el_mol.assign(molecule, molecule + n);
+    s.el_type.assign(atom_type, atom_type + n);
+    s.el_damp.assign(damping, damping + n);
+    s.el_pol.assign(polarity, polarity + n);
+    HIP_TRY(hipSetDevice(ctx->device));
+    Elec* E = ctx->elec;
+    if (int e = up(E->q0, s.el_q)) return e;
+    if (int e = up(E->damp, s.el_damp)) return e;
+    if (int e = up(E->pol, s.el_pol)) return e;
+    if (int e = up(E->mol, s.el_mol)) return e;
+    if (int e = up(E->type, s.el_type)) return e;
+    return MBPOL_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,56 @@
+// Host-visible declarations shared by the HIP kernel files and the context
+// (host) code: small POD types and the launcher functions each kernel file
+// exports.  No kernel is called across translation units directly.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace mbpol {
+
+struct d3 {
+    double x, y, z;
+};
+
+typedef unsigned long long u64;
+
+// geometry of one neighbour-list build
+struct ListGeom {
+    d3 L;          // box (nm); ignored when !periodic
+    int periodic;
+    int use_cells; // 27-cell search over an nc grid (periodic, nc >= 3), else all pairs
+    int3 nc;
+    double rc;     // nm
+};
+
+// ---- kernels_water.hip ----
+void launch_place_vsites(hipStream_t s, int n, const int* vs_idx, const double* vs_w, double* pos);
+void launch_onebody(hipStream_t s, int nw, const int* idx, const double* pos, int periodic, d3 box, u64* force,
+                    u64* energy);
+void launch_water_charges(hipStream_t s, int nw, const int* sites, const double* pos, double* q, double* dq);
+void launch_forces_to_double(hipStream_t s, int n3, const u64* fx, double* f);
+void launch_fold_vsites(hipStream_t s, int n, const int* vs_idx, const double* vs_w, double* f);
+
+// ---- kernels_lists.hip ----
+void launch_cell_count(hipStream_t s, int n, const int* idx, const double* pos, d3 L, int3 nc, int* cell_of,
+                       int* cell_count);
+void launch_cell_fill(hipStream_t s, int n, const int* cell_of, const int* cell_start, int* cell_fill,
+                      int* cell_members);
+void launch_dimer_pass(hipStream_t s, int n, const int* idx, const double* pos, ListGeom g, const int* cell_of,
+                       const int* cell_start, const int* cell_members, long long* count, const long long* offset,
+                       int2* pairs);
+void launch_lower_pass(hipStream_t s, int n, const int* idx, const double* pos, ListGeom g, const int* cell_of,
+                       const int* cell_start, const int* cell_members, int* count, const int* offset, int* lower);
+void launch_trimer_pass(hipStream_t s, int n, const int* idx, const double* pos, ListGeom g, const int* lo_start,
+                        const int* lower, int full_list, long long* count, const long long* offset, int* trip);
+
+// ---- kernels_pip.hip (one object per body order and precision) ----
+void launch_twobody_f32(hipStream_t s, long long npairs, const int2* pairs, const int* idx, const double* pos,
+                        int periodic, d3 box, u64* force, u64* energy);
+void launch_twobody_f64(hipStream_t s, long long npairs, const int2* pairs, const int* idx, const double* pos,
+                        int periodic, d3 box, u64* force, u64* energy);
+void launch_threebody_f32(hipStream_t s, long long ntrip, const int* trip, const int* idx, const double* pos,
+                          int periodic, d3 box, u64* force, u64* energy);
+void launch_threebody_f64(hipStream_t s, long long ntrip, const int* trip, const int* idx, const double* pos,
+                          int periodic, d3 box, u64* force, u64* energy);
+
+}  // namespace mbpol

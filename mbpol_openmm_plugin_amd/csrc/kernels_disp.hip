@@ -1,0 +1,130 @@
+// Damped-C6 dispersion pair kernel (python/mbpol.xml:37-108):
+//   E = -C6 tt6 / r^6,  tt6 = 1 - exp(-d6 r) sum_{k<=6} (d6 r)^k / k!
+// One wave per active particle; lanes stride over its candidates (27 cells
+// or every particle), the force on the wave's particle is reduced in
+// registers and written once, each pair is visited from both ends and its
+// energy halved.  Arithmetic type T: float (mixed) or double.
+#include "device_common.h"
+#include "disp_kernels.h"
+
+namespace mbpol {
+
+__global__ void k_disp_cells(int na, const int* __restrict__ act, const double* __restrict__ pos, d3 L, int3 nc,
+                             int* __restrict__ cell_of, int* __restrict__ cell_count) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= na) return;
+    const int i = act[a];
+    const double p[3] = {pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]};
+    const double Ld[3] = {L.x, L.y, L.z};
+    const int ncd[3] = {nc.x, nc.y, nc.z};
+    int c[3];
+    for (int d = 0; d < 3; ++d) {
+        const double q = p[d] - Ld[d] * floor(p[d] / Ld[d]);
+        const int v = (int)floor(q * ncd[d] / Ld[d]);
+        c[d] = v < 0 ? 0 : (v >= ncd[d] ? ncd[d] - 1 : v);
+    }
+    const int cell = (c[0] * nc.y + c[1]) * nc.z + c[2];
+    cell_of[a] = cell;
+    atomicAdd(&cell_count[cell], 1);
+}
+
+__global__ void k_disp_cell_fill(int na, const int* __restrict__ cell_of, const int* __restrict__ cell_start,
+                                 int* __restrict__ cell_fill, int* __restrict__ members) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= na) return;
+    const int c = cell_of[a];
+    members[cell_start[c] + atomicAdd(&cell_fill[c], 1)] = a;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(64) k_disp(DispGeom g, const int* __restrict__ act, const double* __restrict__ pos,
+                                             const int* __restrict__ type, const int* __restrict__ mol,
+                                             const double* __restrict__ c6, const double* __restrict__ d6,
+                                             const int* __restrict__ cell_of, const int* __restrict__ cell_start,
+                                             const int* __restrict__ members, u64* __restrict__ force,
+                                             double* __restrict__ eblk, unsigned long long* __restrict__ npairs) {
+    const int a = blockIdx.x, lane = threadIdx.x;
+    const int i = act[a];
+    const double xi = pos[3 * i], yi = pos[3 * i + 1], zi = pos[3 * i + 2];
+    const int ti = type[i], mi = mol[i];
+    const double rc2 = g.cutoff * g.cutoff;
+    double fx = 0, fy = 0, fz = 0, en = 0;
+    unsigned long long cnt = 0;
+    auto visit = [&](int b) {
+        const int j = act[b];
+        if (j == i || mol[j] == mi) return;
+        double dx = xi - pos[3 * j], dy = yi - pos[3 * j + 1], dz = zi - pos[3 * j + 2];
+        if (g.periodic) {
+            dx -= floor(dx / g.L.x + 0.5) * g.L.x;
+            dy -= floor(dy / g.L.y + 0.5) * g.L.y;
+            dz -= floor(dz / g.L.z + 0.5) * g.L.z;
+        }
+        const double r2d = dx * dx + dy * dy + dz * dz;
+        if (g.method != 0 && r2d >= rc2) return;
+        const int tj = type[j];
+        const T C = (T)c6[ti + g.ntypes * tj], D = (T)d6[ti + g.ntypes * tj];
+        const T r2 = (T)r2d, r = sqrt(r2), x = D * r;
+        const T s = (T)1 + ((T)1 + ((T)1 + ((T)1 + ((T)1 + ((T)1 + x / (T)6) * x / (T)5) * x / (T)4) * x / (T)3) * x / (T)2) * x;
+        const T ex = exp_t<T>(-x);
+        const T tt6 = (T)1 - s * ex;
+        const T r6 = r2 * r2 * r2;
+        const T x3 = x * x * x;
+        const T dEdr = (T)6 * C * tt6 / (r6 * r) - C * D * (x3 * x3) * ex / ((T)720 * r6);
+        const double fr = -(double)(dEdr / r);
+        fx += fr * dx;
+        fy += fr * dy;
+        fz += fr * dz;
+        en += -(double)(C * tt6 / r6);
+        ++cnt;
+    };
+    if (!g.use_cells) {
+        for (int b = lane; b < g.na; b += 64) visit(b);
+    } else {
+        const int cell = cell_of[a];
+        const int cz = cell % g.nc.z, cy = (cell / g.nc.z) % g.nc.y, cx = cell / (g.nc.z * g.nc.y);
+        for (int ax = -1; ax <= 1; ++ax)
+            for (int ay = -1; ay <= 1; ++ay)
+                for (int az = -1; az <= 1; ++az) {
+                    int X = cx + ax, Y = cy + ay, Z = cz + az;
+                    X = X < 0 ? X + g.nc.x : (X >= g.nc.x ? X - g.nc.x : X);
+                    Y = Y < 0 ? Y + g.nc.y : (Y >= g.nc.y ? Y - g.nc.y : Y);
+                    Z = Z < 0 ? Z + g.nc.z : (Z >= g.nc.z ? Z - g.nc.z : Z);
+                    const int cc = (X * g.nc.y + Y) * g.nc.z + Z;
+                    for (int t = cell_start[cc] + lane; t < cell_start[cc + 1]; t += 64) visit(members[t]);
+                }
+    }
+    fx = wave_sum(fx);
+    fy = wave_sum(fy);
+    fz = wave_sum(fz);
+    en = wave_sum(en);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) cnt += __shfl_xor(cnt, off, 64);
+    if (lane == 0) {
+        add_force(force, i, fx, fy, fz);
+        eblk[a] = 0.5 * en;
+        atomicAdd(npairs, cnt);
+    }
+}
+
+static inline int nblk(long long n, int b) { return (int)((n + b - 1) / b); }
+
+void launch_disp_cells(hipStream_t s, int na, const int* act, const double* pos, d3 L, int3 nc, int* cell_of,
+                       int* cell_count) {
+    hipLaunchKernelGGL(k_disp_cells, nblk(na, 256), 256, 0, s, na, act, pos, L, nc, cell_of, cell_count);
+}
+void launch_disp_cell_fill(hipStream_t s, int na, const int* cell_of, const int* cell_start, int* cell_fill, int* members) {
+    hipLaunchKernelGGL(k_disp_cell_fill, nblk(na, 256), 256, 0, s, na, cell_of, cell_start, cell_fill, members);
+}
+void launch_disp(hipStream_t s, int precision_double, DispGeom g, const int* act, const double* pos, const int* type,
+                 const int* mol, const double* c6, const double* d6, const int* cell_of, const int* cell_start,
+                 const int* members, u64* force, double* eblk, unsigned long long* npairs) {
+    if (g.na <= 0) return;
+    if (precision_double)
+        hipLaunchKernelGGL(k_disp<double>, g.na, 64, 0, s, g, act, pos, type, mol, c6, d6, cell_of, cell_start, members,
+                           force, eblk, npairs);
+    else
+        hipLaunchKernelGGL(k_disp<float>, g.na, 64, 0, s, g, act, pos, type, mol, c6, d6, cell_of, cell_start, members,
+                           force, eblk, npairs);
+}
+
+}  // namespace mbpol

@@ -1,0 +1,844 @@
+// Thole-damped polarisable electrostatics: real-space pair kernels, the PME
+// charge/dipole spreading and potential gathering, the SOR dipole update and
+// the energy/force kernels.
+//
+// Restates platforms/reference/src/MBPolReferenceElectrostaticsForce.cpp.  The
+// reference carries two dipole sets ("d" and "p") that start equal and are
+// updated identically (:618-647, :516-531); here one set is kept and every
+// d/p-symmetric expression is written for d = p.
+//
+// Pair kernels are "site-centric": lanes of one group of G lanes share a site
+// i and stride over its neighbours, so every contribution to site i stays in
+// registers and is reduced across the group; no atomics, deterministic.
+#include "device_common.h"
+#include "elec_kernels.h"
+
+namespace mbpol {
+using namespace mbpol_params;
+
+#pragma clang fp contract(fast)
+
+// ---------------------------------------------------------------------------
+// Thole damping (getAndScaleInverseRs, :260-328)
+// ---------------------------------------------------------------------------
+// regularised upper incomplete gamma Q(3/4, x): series below a+1, Lentz
+// continued fraction above (Press et al., Numerical Recipes 3rd ed. 6.2)
+__device__ double gammq34(double x) {
+    const double a = 0.75;
+    const double eps = 2.220446049250313e-16, fpmin = 2.2250738585072014e-308 / 2.220446049250313e-16;
+    const double lg = -0.20328095143129538;  // ln Gamma(3/4)
+    if (x == 0.0) return 1.0;
+    if (x > 745.0) return 0.0;  // exp(-x) underflows: the result is 0 to double precision
+    if (x < a + 1.0) {
+        double ap = a, sum = 1.0 / a, del = sum;
+        for (int it = 0; it < 200; ++it) {
+            ap += 1.0;
+            del *= x / ap;
+            sum += del;
+            if (fabs(del) < fabs(sum) * eps) break;
+        }
+        return 1.0 - sum * exp(-x + a * log(x) - lg);
+    }
+    double b = x + 1.0 - a, c = 1.0 / fpmin, d = 1.0 / b, h = d;
+    for (int i = 1; i < 400; ++i) {
+        const double an = -i * (i - a);
+        b += 2.0;
+        d = an * d + b;
+        if (fabs(d) < fpmin) d = fpmin;
+        c = b + an / c;
+        if (fabs(c) < fpmin) c = fpmin;
+        d = 1.0 / d;
+        const double del = d * c;
+        h *= del;
+        if (fabs(del - 1.0) <= eps) break;
+    }
+    return exp(-x + a * log(x) - lg) * h;
+}
+
+constexpr double kGamma34 = 1.2254167024651776451;  // Gamma(3/4)
+
+// Thole scale factors of one pair (justScale = true forms).  u = (r/A)^4 with
+// A = (damp_i damp_j)^(1/6); A == 0 means undamped (all scales 1).
+struct Thole {
+    double A, ratio;
+    __device__ __forceinline__ Thole(double di, double dj, double r) {
+        A = pow(di * dj, 1 / 6.);
+        ratio = (A != 0.0) ? pow(r / A, 4) : 0.0;
+    }
+    __device__ __forceinline__ bool on() const { return A != 0.0 && A > -50.0; }
+    __device__ __forceinline__ double s1(double g, double r) const {
+        if (!on()) return 1.0;
+        const double x = g * ratio;
+        return 1.0 - exp(-x) + pow(g, 0.25) * (r / A) * kGamma34 * gammq34(x);
+    }
+    __device__ __forceinline__ double s3(double g) const { return on() ? 1.0 - exp(-g * ratio) : 1.0; }
+    __device__ __forceinline__ double s5(double g) const {
+        if (!on()) return 1.0;
+        const double e = exp(-g * ratio);
+        return (1.0 - e) - (4. / 3.) * g * e * ratio;
+    }
+    __device__ __forceinline__ double s7(double g, double r) const {
+        if (!on()) return 1.0;
+        const double e = exp(-g * ratio);
+        return ((1.0 - e) - (4. / 3.) * g * e * ratio) - (4. / 15.) * g * (4. * g * ratio - 1.) * e / pow(A, 4) * pow(r, 4);
+    }
+};
+
+// Thole parameter for the dipole-dipole interaction (:289-307)
+__device__ __forceinline__ double tdd_gamma(const ElecParams& P, int same, int ti, int tj) {
+    if (same) return (ti == 0 || tj == 0) ? P.thole[3] : P.thole[4];
+    return P.thole[2];
+}
+
+// Ewald real-space screening functions bn0..bn3 (:2541-2557); alpha > 0
+struct Ewald {
+    double bn0, bn1, bn2, bn3;
+    __device__ __forceinline__ Ewald(double alpha, double r, double r2) {
+        const double ra = alpha * r;
+        bn0 = erfc(ra) / r;
+        const double alsq2 = 2.0 * alpha * alpha;
+        double alsq2n = 1.0 / (el_sqrt_pi * alpha);
+        const double ex = exp(-(ra * ra));
+        alsq2n *= alsq2;
+        bn1 = (bn0 + alsq2n * ex) / r2;
+        alsq2n *= alsq2;
+        bn2 = (3.0 * bn1 + alsq2n * ex) / r2;
+        alsq2n *= alsq2;
+        bn3 = (5.0 * bn2 + alsq2n * ex) / r2;
+    }
+};
+
+__device__ __forceinline__ void min_image(double& dx, double& dy, double& dz, const ElecParams& P) {
+    if (!P.pme) return;
+    dx -= floor(dx * P.inv_box.x + 0.5) * P.box.x;
+    dy -= floor(dy * P.inv_box.y + 0.5) * P.box.y;
+    dz -= floor(dz * P.inv_box.z + 0.5) * P.box.z;
+}
+
+template <int G>
+__device__ __forceinline__ double group_sum(double v) {
+#pragma unroll
+    for (int off = G / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, G);
+    return v;
+}
+
+// iterate over the neighbours of site i: its CSR row (PME) or every other site (no cutoff)
+template <int G, typename F>
+__device__ __forceinline__ void for_neighbours(int i, int lane, const ElecParams& P, const int* nb_start,
+                                               const int* nb, F&& f) {
+    if (P.pme) {
+        for (int t = nb_start[i] + lane; t < nb_start[i + 1]; t += G) f(nb[t], t);
+    } else {
+        for (int j = lane; j < P.n; j += G)
+            if (j != i) f(j, -1);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// site set-up and neighbour lists
+// ---------------------------------------------------------------------------
+__global__ void k_site_charges_copy(int n, const double* __restrict__ q0, double* __restrict__ q) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) q[i] = q0[i];
+}
+
+__global__ void k_site_cells(int n, const double* __restrict__ pos, ElecParams P, int3 nc, int* __restrict__ cell_of,
+                             int* __restrict__ cell_count) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double x = pos[3 * i], y = pos[3 * i + 1], z = pos[3 * i + 2];
+    const double q[3] = {x - P.box.x * floor(x * P.inv_box.x), y - P.box.y * floor(y * P.inv_box.y),
+                         z - P.box.z * floor(z * P.inv_box.z)};
+    const double inv[3] = {P.inv_box.x, P.inv_box.y, P.inv_box.z};
+    const int ncd[3] = {nc.x, nc.y, nc.z};
+    int c[3];
+    for (int d = 0; d < 3; ++d) {
+        int v = (int)floor(q[d] * inv[d] * ncd[d]);
+        c[d] = v < 0 ? 0 : (v >= ncd[d] ? ncd[d] - 1 : v);
+    }
+    const int cell = (c[0] * nc.y + c[1]) * nc.z + c[2];
+    cell_of[i] = cell;
+    atomicAdd(&cell_count[cell], 1);
+}
+
+__global__ void k_site_cell_fill(int n, const int* __restrict__ cell_of, const int* __restrict__ cell_start,
+                                 int* __restrict__ cell_fill, int* __restrict__ members) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int c = cell_of[i];
+    members[cell_start[c] + atomicAdd(&cell_fill[c], 1)] = i;
+}
+
+// directed neighbour rows (j != i, minimum image r^2 <= rc^2): count pass (out == null) and fill pass
+__global__ void k_site_pairs(int n, const double* __restrict__ pos, ElecParams P, int3 nc, int use_cells,
+                             const int* __restrict__ cell_of, const int* __restrict__ cell_start,
+                             const int* __restrict__ members, int* __restrict__ count, const int* __restrict__ start,
+                             int* __restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double xi = pos[3 * i], yi = pos[3 * i + 1], zi = pos[3 * i + 2];
+    const double rc2 = P.cutoff * P.cutoff;
+    int c = 0;
+    const int o = out ? start[i] : 0;
+    auto visit = [&](int j) {
+        if (j == i) return;
+        double dx = pos[3 * j] - xi, dy = pos[3 * j + 1] - yi, dz = pos[3 * j + 2] - zi;
+        min_image(dx, dy, dz, P);
+        if (dx * dx + dy * dy + dz * dz > rc2) return;
+        if (out) out[o + c] = j;
+        ++c;
+    };
+    if (!use_cells) {
+        for (int j = 0; j < n; ++j) visit(j);
+    } else {
+        const int cell = cell_of[i];
+        const int cz = cell % nc.z, cy = (cell / nc.z) % nc.y, cx = cell / (nc.z * nc.y);
+        for (int ax = -1; ax <= 1; ++ax)
+            for (int ay = -1; ay <= 1; ++ay)
+                for (int az = -1; az <= 1; ++az) {
+                    int X = cx + ax, Y = cy + ay, Z = cz + az;
+                    X = X < 0 ? X + nc.x : (X >= nc.x ? X - nc.x : X);
+                    Y = Y < 0 ? Y + nc.y : (Y >= nc.y ? Y - nc.y : Y);
+                    Z = Z < 0 ? Z + nc.z : (Z >= nc.z ? Z - nc.z : Z);
+                    const int cc = (X * nc.y + Y) * nc.z + Z;
+                    for (int t = cell_start[cc]; t < cell_start[cc + 1]; ++t) visit(members[t]);
+                }
+    }
+    if (count) count[i] = c;
+}
+
+// per directed pair: induced-field prefactors (PME :2392-2470 with the precomputed
+// justScale factors of :2280-2298; direct sum :534-555) -- field_i += pf1 mu_j + pf2 (mu_j.d) d
+__global__ void k_pair_prefactors(int n, const double* __restrict__ pos, const SiteParams S, ElecParams P,
+                                  const int* __restrict__ nb_start, const int* __restrict__ nb, double2* __restrict__ pf) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double xi = pos[3 * i], yi = pos[3 * i + 1], zi = pos[3 * i + 2];
+    for (int t = nb_start[i]; t < nb_start[i + 1]; ++t) {
+        const int j = nb[t];
+        double dx = pos[3 * j] - xi, dy = pos[3 * j + 1] - yi, dz = pos[3 * j + 2] - zi;
+        min_image(dx, dy, dz, P);
+        const double r2 = dx * dx + dy * dy + dz * dz, r = sqrt(r2);
+        const Thole th(S.damp[i], S.damp[j], r);
+        const double g = tdd_gamma(P, S.mol[i] == S.mol[j], S.type[i], S.type[j]);
+        const double sc3 = th.s3(g), sc5 = th.s5(g);
+        double p1, p2;
+        if (P.pme) {
+            const Ewald ew(P.alpha, r, r2);
+            const double r3 = r * r2, r5 = r3 * r2;
+            p1 = (1.0 - sc3) / r3 - ew.bn1;
+            p2 = ew.bn2 - 3.0 * (1.0 - sc5) / r5;
+        } else {
+            const double ri = 1.0 / r, r2i = ri * ri;
+            p1 = -(ri * r2i) * sc3;
+            p2 = (ri * r2i * 3.0 * r2i) * sc5;
+        }
+        pf[t] = make_double2(p1, p2);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// fixed field (direct part), :361-420 direct sum, :1342-1407 PME
+// ---------------------------------------------------------------------------
+template <int G>
+__global__ void k_fixed_field(int n, const double* __restrict__ pos, const SiteParams S, ElecParams P,
+                              const int* __restrict__ nb_start, const int* __restrict__ nb, double* __restrict__ field) {
+    const int i = (blockIdx.x * blockDim.x + threadIdx.x) / G, lane = threadIdx.x % G;
+    if (i >= n) return;
+    const double xi = pos[3 * i], yi = pos[3 * i + 1], zi = pos[3 * i + 2];
+    double ex = 0, ey = 0, ez = 0;
+    for_neighbours<G>(i, lane, P, nb_start, nb, [&](int j, int) {
+        const int same = S.mol[i] == S.mol[j];
+        double dx = pos[3 * j] - xi, dy = pos[3 * j + 1] - yi, dz = pos[3 * j + 2] - zi;
+        if (!P.pme) {
+            if (same) return;
+            const double r2 = dx * dx + dy * dy + dz * dz, r = sqrt(r2);
+            const double rr3 = Thole(S.damp[i], S.damp[j], r).s3(P.thole[0]) / (r2 * r);
+            const double f = -rr3 * S.q[j];
+            ex += dx * f; ey += dy * f; ez += dz * f;
+        } else {
+            min_image(dx, dy, dz, P);
+            const double r2 = dx * dx + dy * dy + dz * dz, r = sqrt(r2);
+            const Ewald ew(P.alpha, r, r2);
+            double s3 = Thole(S.damp[i], S.damp[j], r).s3(P.thole[0]);
+            if (same) s3 = 2;  // reproduced as in the reference (:1382-1385)
+            const double rr3 = (s3 - 1.) / (r2 * r);
+            const double f = -(ew.bn1 - rr3) * S.q[j];
+            ex += dx * f; ey += dy * f; ez += dz * f;
+        }
+    });
+    ex = group_sum<G>(ex);
+    ey = group_sum<G>(ey);
+    ez = group_sum<G>(ez);
+    if (lane == 0) {
+        field[3 * i] += ex;
+        field[3 * i + 1] += ey;
+        field[3 * i + 2] += ez;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// induced field (direct part) with precomputed prefactors
+// ---------------------------------------------------------------------------
+template <int G>
+__global__ void k_induced_field(int n, const double* __restrict__ pos, ElecParams P, const int* __restrict__ nb_start,
+                                const int* __restrict__ nb, const double2* __restrict__ pf, const double* __restrict__ mu,
+                                double* __restrict__ field) {
+    const int i = (blockIdx.x * blockDim.x + threadIdx.x) / G, lane = threadIdx.x % G;
+    if (i >= n) return;
+    const double xi = pos[3 * i], yi = pos[3 * i + 1], zi = pos[3 * i + 2];
+    double ex = 0, ey = 0, ez = 0;
+    for (int t = nb_start[i] + lane; t < nb_start[i + 1]; t += G) {
+        const int j = nb[t];
+        double dx = pos[3 * j] - xi, dy = pos[3 * j + 1] - yi, dz = pos[3 * j + 2] - zi;
+        min_image(dx, dy, dz, P);
+        const double2 p = pf[t];
+        const double mx = mu[3 * j], my = mu[3 * j + 1], mz = mu[3 * j + 2];
+        const double dur = (mx * dx + my * dy + mz * dz) * p.y;
+        ex += dx * dur + mx * p.x;
+        ey += dy * dur + my * p.x;
+        ez += dz * dur + mz * p.x;
+    }
+    ex = group_sum<G>(ex);
+    ey = group_sum<G>(ey);
+    ez = group_sum<G>(ez);
+    if (lane == 0) {
+        field[3 * i] = ex;
+        field[3 * i + 1] = ey;
+        field[3 * i + 2] = ez;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// PME (:1446-2111)
+// ---------------------------------------------------------------------------
+// order-5 B-spline values and derivatives (computeBSplinePoint, :1446-1519)
+__device__ __forceinline__ void bspline5(double w, double th[5][4]) {
+    double A[6][6];
+    A[2][2] = w;
+    A[2][1] = 1.0 - w;
+    A[3][3] = 0.5 * w * A[2][2];
+    A[3][2] = 0.5 * ((1.0 + w) * A[2][1] + (2.0 - w) * A[2][2]);
+    A[3][1] = 0.5 * (1.0 - w) * A[2][1];
+#pragma unroll
+    for (int i = 4; i <= 5; ++i) {
+        const int k = i - 1;
+        const double den = 1.0 / k;
+        A[i][i] = den * w * A[k][k];
+#pragma unroll
+        for (int j = 1; j <= i - 2; ++j) A[i][i - j] = den * ((w + j) * A[k][i - j - 1] + (i - j - w) * A[k][i - j]);
+        A[i][1] = den * (1.0 - w) * A[k][1];
+    }
+    // derivative coefficient rows (differences of the lower-order rows)
+    A[4][5] = A[4][4];
+#pragma unroll
+    for (int i = 4; i >= 2; --i) A[4][i] = A[4][i - 1] - A[4][i];
+    A[4][1] = -A[4][1];
+    A[3][4] = A[3][3];
+#pragma unroll
+    for (int i = 3; i >= 2; --i) A[3][i] = A[3][i - 1] - A[3][i];
+    A[3][1] = -A[3][1];
+    A[3][5] = A[3][4];
+#pragma unroll
+    for (int i = 4; i >= 2; --i) A[3][i] = A[3][i - 1] - A[3][i];
+    A[3][1] = -A[3][1];
+    A[2][3] = A[2][2];
+    A[2][2] = A[2][1] - A[2][2];
+    A[2][1] = -A[2][1];
+    A[2][4] = A[2][3];
+#pragma unroll
+    for (int i = 3; i >= 2; --i) A[2][i] = A[2][i - 1] - A[2][i];
+    A[2][1] = -A[2][1];
+    A[2][5] = A[2][4];
+#pragma unroll
+    for (int i = 4; i >= 2; --i) A[2][i] = A[2][i - 1] - A[2][i];
+    A[2][1] = -A[2][1];
+#pragma unroll
+    for (int i = 1; i <= 5; ++i) {
+        th[i - 1][0] = A[5][i];
+        th[i - 1][1] = A[4][i];
+        th[i - 1][2] = A[3][i];
+        th[i - 1][3] = A[2][i];
+    }
+}
+
+// grid origin and splines of one site along each axis (computeMBPolBsplines, :1524-1554)
+__device__ __forceinline__ void site_splines(const double* pos, int i, const ElecParams& P, int g0[3], double th[3][5][4]) {
+    double p[3] = {pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]};
+    const double L[3] = {P.box.x, P.box.y, P.box.z}, inv[3] = {P.inv_box.x, P.inv_box.y, P.inv_box.z};
+    const int K[3] = {P.grid.x, P.grid.y, P.grid.z};
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        p[d] -= floor(p[d] * inv[d] + 0.5) * L[d];
+        const double w0 = p[d] * inv[d];
+        const double fr = K[d] * (w0 - (int)(w0 + 0.5) + 0.5);
+        const int ifr = (int)fr;
+        int g = ifr - 5 + 1;
+        g += g < 0 ? K[d] : 0;
+        g0[d] = g;
+        bspline5(fr - ifr, th[d]);
+    }
+}
+
+__device__ __forceinline__ int wrapg(int v, int K) { return v >= K ? v - K : v; }
+
+// charges (mode 0) or fractional dipoles (mode 1) onto the real grid: one lane per (site, ix)
+__global__ void k_spread(int n, int mode, const double* __restrict__ pos, const double* __restrict__ q,
+                         const double* __restrict__ mu, ElecParams P, double* __restrict__ grid) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = t / 5, ix = t % 5;
+    if (i >= n) return;
+    int g0[3];
+    double th[3][5][4];
+    site_splines(pos, i, P, g0, th);
+    const int gx = wrapg(g0[0] + ix, P.grid.x);
+    const double sc[3] = {P.grid.x * P.inv_box.x, P.grid.y * P.inv_box.y, P.grid.z * P.inv_box.z};
+    double dq = 0, dx = 0, dy = 0, dz = 0;
+    if (mode == 0) {
+        dq = q[i];
+        if (dq == 0.0) return;
+    } else {
+        dx = sc[0] * mu[3 * i];
+        dy = sc[1] * mu[3 * i + 1];
+        dz = sc[2] * mu[3 * i + 2];
+        if (dx == 0.0 && dy == 0.0 && dz == 0.0) return;
+    }
+    const double t0 = th[0][ix][0], t1 = th[0][ix][1];
+    for (int iy = 0; iy < 5; ++iy) {
+        const int gy = wrapg(g0[1] + iy, P.grid.y);
+        const double u0 = th[1][iy][0], u1 = th[1][iy][1];
+        double* row = grid + ((size_t)gx * P.grid.y + gy) * P.grid.z;
+#pragma unroll
+        for (int iz = 0; iz < 5; ++iz) {
+            const int gz = wrapg(g0[2] + iz, P.grid.z);
+            const double v0 = th[2][iz][0], v1 = th[2][iz][1];
+            const double val = mode == 0 ? dq * u0 * v0 * t0 : (dy * u1 * v0 + dz * u0 * v1) * t0 + dx * u0 * v0 * t1;
+            unsafeAtomicAdd(&row[gz], val);
+        }
+    }
+}
+
+// multiply the half-complex spectrum by the reciprocal-space influence function (:1676-1713)
+__global__ void k_convolve(size_t nhalf, const double* __restrict__ influence, double2* __restrict__ spec) {
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nhalf) return;
+    const double e = influence[t];
+    double2 v = spec[t];
+    v.x *= e;
+    v.y *= e;
+    spec[t] = v;
+}
+
+// influence function on the half grid (kz <= K3/2)
+__global__ void k_influence(ElecParams P, const double* __restrict__ bx, const double* __restrict__ by,
+                            const double* __restrict__ bz, double* __restrict__ out) {
+    const int Kz = P.grid.z / 2 + 1;
+    const size_t total = (size_t)P.grid.x * P.grid.y * Kz;
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= total) return;
+    const int kz = (int)(t % Kz), ky = (int)((t / Kz) % P.grid.y), kx = (int)(t / ((size_t)Kz * P.grid.y));
+    if (kx == 0 && ky == 0 && kz == 0) {
+        out[t] = 0.0;
+        return;
+    }
+    const double expf_ = (M_PI * M_PI) / (P.alpha * P.alpha);
+    const double sf = 1.0 / (M_PI * P.box.x * P.box.y * P.box.z);
+    const int mx = (kx < (P.grid.x + 1) / 2) ? kx : (kx - P.grid.x);
+    const int my = (ky < (P.grid.y + 1) / 2) ? ky : (ky - P.grid.y);
+    const int mz = (kz < (P.grid.z + 1) / 2) ? kz : (kz - P.grid.z);
+    const double hx = mx * P.inv_box.x, hy = my * P.inv_box.y, hz = mz * P.inv_box.z;
+    const double m2 = hx * hx + hy * hy + hz * hz;
+    const double den = m2 * bx[kx] * by[ky] * bz[kz];
+    out[t] = sf * exp(-expf_ * m2) / den;
+}
+
+// potential and derivatives to second order at each site (10 components,
+// order 000 100 010 001 200 020 002 110 101 011), :1715-1821
+__global__ void k_gather(int n, const double* __restrict__ pos, ElecParams P, const double* __restrict__ grid,
+                         double* __restrict__ phi) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int g0[3];
+    double th[3][5][4];
+    site_splines(pos, i, P, g0, th);
+    double acc[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int iz = 0; iz < 5; ++iz) {
+        const int k = wrapg(g0[2] + iz, P.grid.z);
+        const double v0 = th[2][iz][0], v1 = th[2][iz][1], v2 = th[2][iz][2];
+        double tu00 = 0, tu10 = 0, tu01 = 0, tu20 = 0, tu11 = 0, tu02 = 0;
+        for (int iy = 0; iy < 5; ++iy) {
+            const int j = wrapg(g0[1] + iy, P.grid.y);
+            const double u0 = th[1][iy][0], u1 = th[1][iy][1], u2 = th[1][iy][2];
+            double t0 = 0, t1 = 0, t2 = 0;
+#pragma unroll
+            for (int ix = 0; ix < 5; ++ix) {
+                const int ii = wrapg(g0[0] + ix, P.grid.x);
+                const double gv = grid[((size_t)ii * P.grid.y + j) * P.grid.z + k];
+                t0 += gv * th[0][ix][0];
+                t1 += gv * th[0][ix][1];
+                t2 += gv * th[0][ix][2];
+            }
+            tu00 += t0 * u0;
+            tu10 += t1 * u0;
+            tu01 += t0 * u1;
+            tu20 += t2 * u0;
+            tu11 += t1 * u1;
+            tu02 += t0 * u2;
+        }
+        acc[0] += tu00 * v0;
+        acc[1] += tu10 * v0;
+        acc[2] += tu01 * v0;
+        acc[3] += tu00 * v1;
+        acc[4] += tu20 * v0;
+        acc[5] += tu02 * v0;
+        acc[6] += tu00 * v2;
+        acc[7] += tu11 * v0;
+        acc[8] += tu10 * v1;
+        acc[9] += tu01 * v1;
+    }
+#pragma unroll
+    for (int c = 0; c < 10; ++c) phi[10 * i + c] = acc[c];
+}
+
+// field = -scale * grad(phi) (+= for the induced field, = for the fixed field); :2267-2278, :2300-2324
+__global__ void k_record_field(int n, ElecParams P, const double* __restrict__ phi, int accumulate, double self_term,
+                               const double* __restrict__ mu, double* __restrict__ field) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double sc[3] = {P.grid.x * P.inv_box.x, P.grid.y * P.inv_box.y, P.grid.z * P.inv_box.z};
+    for (int d = 0; d < 3; ++d) {
+        double f = -sc[d] * phi[10 * i + 1 + d];
+        if (mu) f += mu[3 * i + d] * self_term;
+        field[3 * i + d] = accumulate ? field[3 * i + d] + f : f;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// SOR update (:516-531) with per-block partial sums of |delta|^2
+// ---------------------------------------------------------------------------
+__global__ void k_sor_update(int n, const double* __restrict__ pol, const double* __restrict__ efix_pol,
+                             const double* __restrict__ field, double omega, double* __restrict__ mu,
+                             double* __restrict__ partial) {
+    __shared__ double red[256];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double e = 0.0;
+    if (i < n) {
+        for (int d = 0; d < 3; ++d) {
+            const double old = mu[3 * i + d];
+            const double nv = efix_pol[3 * i + d] + field[3 * i + d] * pol[i];
+            const double delta = nv - old;
+            mu[3 * i + d] = old + delta * omega;
+            e += delta * delta;
+        }
+    }
+    red[threadIdx.x] = e;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+// epsilon = omega * debye * sqrt(sum / n) from the block partials, in a fixed order (:504-511, :588)
+__global__ void k_eps_finish(int nblocks, int n, const double* __restrict__ partial, double* __restrict__ eps_out) {
+    __shared__ double red[256];
+    double s = 0.0;
+    for (int b = threadIdx.x; b < nblocks; b += blockDim.x) s += partial[b];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int k = blockDim.x / 2; k > 0; k >>= 1) {
+        if (threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *eps_out = el_polar_sor * el_debye * sqrt(red[0] / (double)n);
+}
+
+__global__ void k_scale_copy(int n3, const double* __restrict__ a, const double* __restrict__ pol, double* __restrict__ out,
+                             double* __restrict__ out2) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n3) return;
+    const double v = a[t] * pol[t / 3];
+    out[t] = v;
+    if (out2) out2[t] = v;
+}
+
+// ---------------------------------------------------------------------------
+// energy, forces and site potentials (direct space)
+// PME: calculatePmeDirectElectrostaticPairIxn (:2510-2716)
+// no cutoff: calculateElectrostaticPairIxn (:649-836), with the charge-derivative
+// terms (:791-827) gathered as a potential V (same sums, see the driver)
+// Each directed pair (i, j) adds its share to site i only; energy is halved.
+// ---------------------------------------------------------------------------
+template <int G>
+__global__ void k_elec_energy(int n, const double* __restrict__ pos, const SiteParams S, ElecParams P,
+                              const int* __restrict__ nb_start, const int* __restrict__ nb, const double* __restrict__ mu,
+                              double* __restrict__ force, double* __restrict__ V, double* __restrict__ eblk) {
+    const int i = (blockIdx.x * blockDim.x + threadIdx.x) / G, lane = threadIdx.x % G;
+    double en = 0, fx = 0, fy = 0, fz = 0, vi = 0;
+    if (i < n) {
+        const double xi = pos[3 * i], yi = pos[3 * i + 1], zi = pos[3 * i + 2];
+        const double ci = S.q[i];
+        const double mix = mu[3 * i], miy = mu[3 * i + 1], miz = mu[3 * i + 2];
+        for_neighbours<G>(i, lane, P, nb_start, nb, [&](int j, int) {
+            double dx = pos[3 * j] - xi, dy = pos[3 * j + 1] - yi, dz = pos[3 * j + 2] - zi;
+            min_image(dx, dy, dz, P);
+            const double r2 = dx * dx + dy * dy + dz * dz, r = sqrt(r2);
+            const double ck = S.q[j];
+            const double mjx = mu[3 * j], mjy = mu[3 * j + 1], mjz = mu[3 * j + 2];
+            const int same = S.mol[i] == S.mol[j];
+            const Thole th(S.damp[i], S.damp[j], r);
+            const double gdd = tdd_gamma(P, same, S.type[i], S.type[j]);
+            const double rr1 = 1.0 / r, rr3 = rr1 / r2, rr5 = 3.0 * rr3 / r2, rr7 = 5.0 * rr5 / r2;
+            const double sci3 = mix * dx + miy * dy + miz * dz;  // mu_i . d
+            const double sci4 = mjx * dx + mjy * dy + mjz * dz;  // mu_j . d
+            const double mm = mix * mjx + miy * mjy + miz * mjz;
+            const double gl0 = ci * ck;
+            const double gli1 = ck * sci3 - ci * sci4;
+            double gx, gy, gz;  // (ftm2 + ftm2i): force on i is -f * this
+            if (P.pme) {
+                const Ewald ew(P.alpha, r, r2);
+                double s1CC = th.s1(P.thole[0], r), s3CD = th.s3(P.thole[1]);
+                if (same) s1CC = s3CD = 0.;
+                const double e = ew.bn0 * gl0 - rr1 * gl0 * (1 - s1CC);
+                const double ei = 0.5 * (ew.bn1 * gli1) - 0.5 * (rr3 * gli1 * (1 - s3CD));
+                en += e + ei;
+                vi += ck * (ew.bn0 - rr1 * (1 - s1CC)) - sci4 * (ew.bn1 - rr3 * (1 - s3CD));
+                const double s5DD = th.s5(gdd), s7DD = th.s7(gdd, r);
+                const double s3CC = same ? 0. : th.s3(P.thole[0]);
+                const double s5CD = same ? 0. : th.s5(P.thole[1]);
+                const double gf1 = ew.bn1 * gl0 - (1 - s3CC) * rr3 * gl0;
+                const double gfi1 = ew.bn2 * (gli1 + mm) - ew.bn3 * sci3 * sci4;
+                const double gfri1 = rr5 * (gli1 * (1 - s5CD) + mm * (1 - s5DD)) - rr7 * sci3 * sci4 * (1 - s7DD);
+                const double a = gf1 + gfi1 - gfri1;
+                const double bi = -ck * ew.bn1 + sci4 * ew.bn2 - sci4 * rr5 * (1 - s5DD) + ck * rr3 * (1 - s3CD);
+                const double bj = ci * ew.bn1 + sci3 * ew.bn2 - sci3 * rr5 * (1 - s5DD) - ci * rr3 * (1 - s3CD);
+                gx = dx * a + mix * bi + mjx * bj;
+                gy = dy * a + miy * bi + mjy * bj;
+                gz = dz * a + miz * bi + mjz * bj;
+            } else {
+                const double s1CC = th.s1(P.thole[0], r), s3CD = th.s3(P.thole[1]);
+                const double s3CC = th.s3(P.thole[0]), s5CD = th.s5(P.thole[1]);
+                const double s5DD = th.s5(gdd), s7DD = th.s7(gdd, r);
+                const double g0 = same ? 0. : gl0, gi = same ? 0. : gli1;
+                en += rr1 * g0 * s1CC + 0.5 * (rr3 * gi * s3CD);
+                const double gf0 = rr3 * g0 * s3CC;
+                const double gfi0 = rr5 * gi * s5CD + rr5 * mm * s5DD - rr7 * sci3 * sci4 * s7DD;
+                const double a = gf0 + gfi0;
+                double bi = sci4 * rr5 * s5DD, bj = sci3 * rr5 * s5DD;
+                if (!same) {
+                    bi += -ck * rr3 * s3CD;
+                    bj += ci * rr3 * s3CD;
+                    if (P.redistribute) {  // potential at i from the charge and dipole of j (TCC damping)
+                        vi += ck * rr1 * s1CC + th.s3(P.thole[0]) * (-sci4) * rr3;
+                    }
+                }
+                gx = dx * a + mix * bi + mjx * bj;
+                gy = dy * a + miy * bi + mjy * bj;
+                gz = dz * a + miz * bi + mjz * bj;
+            }
+            fx -= gx;
+            fy -= gy;
+            fz -= gz;
+        });
+        fx = group_sum<G>(fx);
+        fy = group_sum<G>(fy);
+        fz = group_sum<G>(fz);
+        vi = group_sum<G>(vi);
+        if (lane == 0) {
+            const double f = el_electric;
+            force[3 * i] += fx * f;
+            force[3 * i + 1] += fy * f;
+            force[3 * i + 2] += fz * f;
+            V[i] += vi;
+        }
+    }
+    // energy: each pair seen twice
+    __shared__ double red[256];
+    red[threadIdx.x] = 0.5 * en * el_electric;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) eblk[blockIdx.x] = red[0];
+}
+
+// reciprocal-space and self terms per site (:2113-2265, :2472-2508)
+__global__ void k_recip_site(int n, const SiteParams S, ElecParams P, const double* __restrict__ phi_fix,
+                             const double* __restrict__ phi_ind, const double* __restrict__ mu, double* __restrict__ force,
+                             double* __restrict__ V, double* __restrict__ eblk) {
+    __shared__ double red[256];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double en = 0.0;
+    if (i < n) {
+        const double f = el_electric;
+        const double sc[3] = {P.grid.x * P.inv_box.x, P.grid.y * P.inv_box.y, P.grid.z * P.inv_box.z};
+        const double* pf = phi_fix + 10 * i;
+        const double* pd = phi_ind + 10 * i;
+        const double q = S.q[i];
+        const double m[3] = {mu[3 * i], mu[3 * i + 1], mu[3 * i + 2]};
+        // second-derivative component index [a][k]
+        const int d2[3][3] = {{4, 7, 8}, {7, 5, 9}, {8, 9, 6}};
+        double fv[3];
+        for (int a = 0; a < 3; ++a) {
+            double s = 0;
+            for (int k = 0; k < 3; ++k) s += 2.0 * m[k] * sc[k] * (pf[d2[a][k]] + pd[d2[a][k]]);
+            fv[a] = sc[a] * (s + 2.0 * q * pd[1 + a]);
+        }
+        for (int a = 0; a < 3; ++a) force[3 * i + a] -= fv[a] * 0.5 * f + q * pf[1 + a] * sc[a] * f;
+        // energies: induced 0.5 f sum mu.grad(phi_fix) (fractional), fixed 0.5 f q phi, self
+        en = 0.5 * f * (sc[0] * m[0] * pf[1] + sc[1] * m[1] * pf[2] + sc[2] * m[2] * pf[3]);
+        en += 0.5 * f * q * pf[0];
+        en += q * q * -(f * P.alpha / el_sqrt_pi);
+        V[i] += pf[0] + pd[0] + q * -(P.alpha / el_sqrt_pi) * 2;
+    }
+    red[threadIdx.x] = en;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) eblk[blockIdx.x] = red[0];
+}
+
+// charge-derivative chain rule (:2767-2773): F_a -= f sum_s dq_s/dr_a V_s, a in O,H1,H2
+__global__ void k_charge_deriv_forces(int nw, const double* __restrict__ dq, const double* __restrict__ V,
+                                      double* __restrict__ force) {
+    const int w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= nw) return;
+    const int base = 4 * w;
+    const double Vs[3] = {V[base + 1], V[base + 2], V[base + 3]};
+    for (int a = 0; a < 3; ++a)
+        for (int d = 0; d < 3; ++d) {
+            double s = 0;
+            for (int q = 0; q < 3; ++q) s += dq[(size_t)w * 27 + (a * 3 + q) * 3 + d] * Vs[q];
+            force[3 * (base + a) + d] += s * -el_electric;
+        }
+}
+
+// sum block partials into the fixed-point energy slot (fixed order)
+__global__ void k_energy_finish(int nblocks, const double* __restrict__ eblk, u64* __restrict__ energy) {
+    __shared__ double red[256];
+    double s = 0.0;
+    for (int b = threadIdx.x; b < nblocks; b += blockDim.x) s += eblk[b];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int k = blockDim.x / 2; k > 0; k >>= 1) {
+        if (threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicAdd(energy, to_fix(red[0]));
+}
+
+// double forces of the electrostatics particles -> fixed-point particle forces
+__global__ void k_add_forces_fix(int n, const double* __restrict__ f, u64* __restrict__ fx) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < 3 * n) atomicAdd(&fx[t], to_fix(f[t]));
+}
+
+// potential of the charges and induced dipoles at arbitrary points
+// (calculateElectrostaticPotentialForParticleGridPoint, :1023-1086): plain
+// Coulomb sum over every site, nearest image under PME
+__global__ void k_potential_at_points(int npts, const double* __restrict__ pts, int n, const double* __restrict__ pos,
+                                      const double* __restrict__ q, const double* __restrict__ mu, ElecParams P,
+                                      double* __restrict__ out) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= npts) return;
+    double v = 0.0;
+    for (int i = 0; i < n; ++i) {
+        double dx = pos[3 * i] - pts[3 * k], dy = pos[3 * i + 1] - pts[3 * k + 1], dz = pos[3 * i + 2] - pts[3 * k + 2];
+        min_image(dx, dy, dz, P);
+        const double r2 = dx * dx + dy * dy + dz * dz, r = sqrt(r2);
+        const double rr1 = 1.0 / r, rr3 = rr1 / r2;
+        v += q[i] * rr1 - (mu[3 * i] * dx + mu[3 * i + 1] * dy + mu[3 * i + 2] * dz) * rr3;
+    }
+    out[k] = v * el_electric;
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+static inline int nblk(long long n, int b) { return (int)((n + b - 1) / b); }
+constexpr int kG = 16;  // lanes per site in the pair kernels
+
+void launch_site_charges_copy(hipStream_t s, int n, const double* q0, double* q) {
+    hipLaunchKernelGGL(k_site_charges_copy, nblk(n, 256), 256, 0, s, n, q0, q);
+}
+void launch_site_cells(hipStream_t s, int n, const double* pos, ElecParams P, int3 nc, int* cell_of, int* cell_count) {
+    hipLaunchKernelGGL(k_site_cells, nblk(n, 256), 256, 0, s, n, pos, P, nc, cell_of, cell_count);
+}
+void launch_site_cell_fill(hipStream_t s, int n, const int* cell_of, const int* cell_start, int* cell_fill, int* members) {
+    hipLaunchKernelGGL(k_site_cell_fill, nblk(n, 256), 256, 0, s, n, cell_of, cell_start, cell_fill, members);
+}
+void launch_site_pairs(hipStream_t s, int n, const double* pos, ElecParams P, int3 nc, int use_cells, const int* cell_of,
+                       const int* cell_start, const int* members, int* count, const int* start, int* out) {
+    hipLaunchKernelGGL(k_site_pairs, nblk(n, 64), 64, 0, s, n, pos, P, nc, use_cells, cell_of, cell_start, members, count,
+                       start, out);
+}
+void launch_pair_prefactors(hipStream_t s, int n, const double* pos, SiteParams S, ElecParams P, const int* nb_start,
+                            const int* nb, double2* pf) {
+    hipLaunchKernelGGL(k_pair_prefactors, nblk(n, 64), 64, 0, s, n, pos, S, P, nb_start, nb, pf);
+}
+void launch_fixed_field(hipStream_t s, int n, const double* pos, SiteParams S, ElecParams P, const int* nb_start,
+                        const int* nb, double* field) {
+    hipLaunchKernelGGL(k_fixed_field<kG>, nblk((long long)n * kG, 256), 256, 0, s, n, pos, S, P, nb_start, nb, field);
+}
+void launch_induced_field(hipStream_t s, int n, const double* pos, ElecParams P, const int* nb_start, const int* nb,
+                          const double2* pf, const double* mu, double* field) {
+    hipLaunchKernelGGL(k_induced_field<kG>, nblk((long long)n * kG, 256), 256, 0, s, n, pos, P, nb_start, nb, pf, mu, field);
+}
+void launch_spread(hipStream_t s, int n, int mode, const double* pos, const double* q, const double* mu, ElecParams P,
+                   double* grid) {
+    hipLaunchKernelGGL(k_spread, nblk((long long)n * 5, 256), 256, 0, s, n, mode, pos, q, mu, P, grid);
+}
+void launch_convolve(hipStream_t s, size_t nhalf, const double* influence, double2* spec) {
+    hipLaunchKernelGGL(k_convolve, nblk(nhalf, 256), 256, 0, s, nhalf, influence, spec);
+}
+void launch_influence(hipStream_t s, ElecParams P, const double* bx, const double* by, const double* bz, double* out) {
+    const size_t total = (size_t)P.grid.x * P.grid.y * (P.grid.z / 2 + 1);
+    hipLaunchKernelGGL(k_influence, nblk(total, 256), 256, 0, s, P, bx, by, bz, out);
+}
+void launch_gather(hipStream_t s, int n, const double* pos, ElecParams P, const double* grid, double* phi) {
+    hipLaunchKernelGGL(k_gather, nblk(n, 64), 64, 0, s, n, pos, P, grid, phi);
+}
+void launch_record_field(hipStream_t s, int n, ElecParams P, const double* phi, int accumulate, double self_term,
+                         const double* mu, double* field) {
+    hipLaunchKernelGGL(k_record_field, nblk(n, 256), 256, 0, s, n, P, phi, accumulate, self_term, mu, field);
+}
+int sor_blocks(int n) { return nblk(n, 256); }
+void launch_sor_update(hipStream_t s, int n, const double* pol, const double* efix_pol, const double* field, double omega,
+                       double* mu, double* partial) {
+    hipLaunchKernelGGL(k_sor_update, nblk(n, 256), 256, 0, s, n, pol, efix_pol, field, omega, mu, partial);
+}
+void launch_eps_finish(hipStream_t s, int nblocks, int n, const double* partial, double* eps_out) {
+    hipLaunchKernelGGL(k_eps_finish, 1, 256, 0, s, nblocks, n, partial, eps_out);
+}
+void launch_scale_copy(hipStream_t s, int n, const double* a, const double* pol, double* out, double* out2) {
+    hipLaunchKernelGGL(k_scale_copy, nblk(3 * n, 256), 256, 0, s, 3 * n, a, pol, out, out2);
+}
+int energy_blocks(int n) { return nblk((long long)n * kG, 256); }
+void launch_elec_energy(hipStream_t s, int n, const double* pos, SiteParams S, ElecParams P, const int* nb_start,
+                        const int* nb, const double* mu, double* force, double* V, double* eblk) {
+    hipLaunchKernelGGL(k_elec_energy<kG>, nblk((long long)n * kG, 256), 256, 0, s, n, pos, S, P, nb_start, nb, mu, force,
+                       V, eblk);
+}
+void launch_recip_site(hipStream_t s, int n, SiteParams S, ElecParams P, const double* phi_fix, const double* phi_ind,
+                       const double* mu, double* force, double* V, double* eblk) {
+    hipLaunchKernelGGL(k_recip_site, nblk(n, 256), 256, 0, s, n, S, P, phi_fix, phi_ind, mu, force, V, eblk);
+}
+void launch_charge_deriv_forces(hipStream_t s, int nw, const double* dq, const double* V, double* force) {
+    hipLaunchKernelGGL(k_charge_deriv_forces, nblk(nw, 64), 64, 0, s, nw, dq, V, force);
+}
+void launch_energy_finish(hipStream_t s, int nblocks, const double* eblk, u64* energy) {
+    hipLaunchKernelGGL(k_energy_finish, 1, 256, 0, s, nblocks, eblk, energy);
+}
+void launch_potential_at_points(hipStream_t s, int npts, const double* pts, int n, const double* pos, const double* q,
+                                const double* mu, ElecParams P, double* out) {
+    hipLaunchKernelGGL(k_potential_at_points, nblk(npts, 64), 64, 0, s, npts, pts, n, pos, q, mu, P, out);
+}
+void launch_add_forces_fix(hipStream_t s, int n, const double* f, u64* fx) {
+    hipLaunchKernelGGL(k_add_forces_fix, nblk(3 * n, 256), 256, 0, s, n, f, fx);
+}
+
+}  // namespace mbpol

@@ -1,0 +1,379 @@
+// 2-body and 3-body permutationally invariant polynomial kernels.
+//
+// One dimer (trimer) per lane.  Positions are read and minimum-imaged in FP64,
+// then made relative to the first oxygen so that the arithmetic type T
+// (float on the mixed-precision path, double on the FP64 path) only ever sees
+// small coordinates.  The polynomial itself (generated, pip_eval_gen.h) keeps
+// variables, gradients and coefficients in registers.  Per-atom forces are
+// accumulated in T inside the lane and scattered once as 64-bit fixed point.
+//
+// This file is compiled once per body order and arithmetic type:
+//   -DPIP_BODY=2|3  -DPIP_REAL=float|double
+#include "device_common.h"
+#include "pip_eval_gen.h"
+
+#ifndef PIP_BODY
+#error "PIP_BODY must be 2 or 3"
+#endif
+#ifndef PIP_REAL
+#error "PIP_REAL must be float or double"
+#endif
+
+namespace mbpol {
+namespace {  // kernels have internal linkage: this file is compiled once per body order and type
+using namespace mbpol_params;
+typedef PIP_REAL real;
+typedef v3<real> vr;
+
+__device__ __forceinline__ vr rel(const d3& p, const d3& o) { return vr((real)(p.x - o.x), (real)(p.y - o.y), (real)(p.z - o.z)); }
+
+#if PIP_BODY == 2
+// ---------------------------------------------------------------------------
+// 2-body: platforms/reference/src/MBPolReferenceTwoBodyForce.cpp:110-296 and
+// mbpol_2body_constants.cpp:10-111.  Points (Angstrom, relative to Oa):
+// 0 Oa 1 Ha1 2 Ha2 3 Ob 4 Hb1 5 Hb2 6 Xa1 7 Xa2 8 Xb1 9 Xb2.
+// ---------------------------------------------------------------------------
+struct Var2 {
+    int a, b, coul;
+    double k, r0;
+};
+// the 31 variables in reference order (:170-207)
+#define V2(a, b, c, k, r) {a, b, c, k, r}
+__device__ __forceinline__ Var2 var2(int n) {
+    const double di = tb_d0_intra, de = tb_d0_inter;
+    switch (n) {
+        case 0: return V2(1, 2, 0, tb_k_HH_intra, di);
+        case 1: return V2(4, 5, 0, tb_k_HH_intra, di);
+        case 2: return V2(0, 1, 0, tb_k_OH_intra, di);
+        case 3: return V2(0, 2, 0, tb_k_OH_intra, di);
+        case 4: return V2(3, 4, 0, tb_k_OH_intra, di);
+        case 5: return V2(3, 5, 0, tb_k_OH_intra, di);
+        case 6: return V2(1, 4, 1, tb_k_HH_coul, de);
+        case 7: return V2(1, 5, 1, tb_k_HH_coul, de);
+        case 8: return V2(2, 4, 1, tb_k_HH_coul, de);
+        case 9: return V2(2, 5, 1, tb_k_HH_coul, de);
+        case 10: return V2(0, 4, 1, tb_k_OH_coul, de);
+        case 11: return V2(0, 5, 1, tb_k_OH_coul, de);
+        case 12: return V2(3, 1, 1, tb_k_OH_coul, de);
+        case 13: return V2(3, 2, 1, tb_k_OH_coul, de);
+        case 14: return V2(0, 3, 1, tb_k_OO_coul, de);
+        case 15: return V2(6, 4, 0, tb_k_XH_main, de);
+        case 16: return V2(6, 5, 0, tb_k_XH_main, de);
+        case 17: return V2(7, 4, 0, tb_k_XH_main, de);
+        case 18: return V2(7, 5, 0, tb_k_XH_main, de);
+        case 19: return V2(8, 1, 0, tb_k_XH_main, de);
+        case 20: return V2(8, 2, 0, tb_k_XH_main, de);
+        case 21: return V2(9, 1, 0, tb_k_XH_main, de);
+        case 22: return V2(9, 2, 0, tb_k_XH_main, de);
+        case 23: return V2(0, 8, 0, tb_k_XO_main, de);
+        case 24: return V2(0, 9, 0, tb_k_XO_main, de);
+        case 25: return V2(3, 6, 0, tb_k_XO_main, de);
+        case 26: return V2(3, 7, 0, tb_k_XO_main, de);
+        case 27: return V2(6, 8, 0, tb_k_XX_main, de);
+        case 28: return V2(6, 9, 0, tb_k_XX_main, de);
+        case 29: return V2(7, 8, 0, tb_k_XX_main, de);
+        default: return V2(7, 9, 0, tb_k_XX_main, de);
+    }
+}
+#undef V2
+
+// extra points of one monomer (mbpol_2body_constants.cpp:62-76)
+__device__ __forceinline__ void extra_points(const vr& O, const vr& H1, const vr& H2, vr& x1, vr& x2) {
+    const vr oh1 = H1 - O, oh2 = H2 - O;
+    const vr in_plane = O + (oh1 + oh2) * (real)(0.5 * tb_in_plane_gamma);
+    const vr out_of_plane = oh1.cross(oh2) * (real)tb_out_of_plane_gamma;
+    x1 = in_plane + out_of_plane;
+    x2 = in_plane - out_of_plane;
+}
+
+// back-propagation of the extra-point gradients (mbpol_2body_constants.cpp:80-96)
+__device__ __forceinline__ void extra_grads(const vr& O, const vr& H1, const vr& H2, const vr& g1, const vr& g2, vr& fO,
+                                            vr& fH1, vr& fH2) {
+    const vr oh1 = H1 - O, oh2 = H2 - O;
+    const vr gm = g1 - g2;
+    const vr t1 = oh2.cross(gm), t2 = oh1.cross(gm);
+    const vr gsum = g1 + g2;
+    const vr in_plane = gsum * (real)(0.5 * tb_in_plane_gamma);
+    const vr gh1 = in_plane + t1 * (real)tb_out_of_plane_gamma;
+    const vr gh2 = in_plane - t2 * (real)tb_out_of_plane_gamma;
+    fO += gsum - (gh1 + gh2);
+    fH1 += gh1;
+    fH2 += gh2;
+}
+
+__global__ void __launch_bounds__(64) k_twobody(long long np, const int2* __restrict__ pairs, const int* __restrict__ idx,
+                                                const double* __restrict__ pos, int periodic, d3 boxA,
+                                                u64* __restrict__ force, u64* __restrict__ energy) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    double e_out = 0.0;
+    if (t < np) {
+        const int2 pr = pairs[t];
+        int at[6];
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            at[q] = idx[3 * pr.x + q];
+            at[3 + q] = idx[3 * pr.y + q];
+        }
+        d3 P[6];
+#pragma unroll
+        for (int q = 0; q < 6; ++q) {
+            const d3 p = load_d3(pos, at[q]);
+            P[q] = d3{p.x * 10., p.y * 10., p.z * 10.};
+        }
+        if (periodic) {  // imageMolecules (MBPolReferenceTwoBodyForce.cpp:78-100)
+            P[1] = image_to(P[0], P[1], boxA);
+            P[2] = image_to(P[0], P[2], boxA);
+            P[3] = image_to(P[0], P[3], boxA);
+            P[4] = image_to(P[3], P[4], boxA);
+            P[5] = image_to(P[3], P[5], boxA);
+        }
+        const double dx = P[0].x - P[3].x, dy = P[0].y - P[3].y, dz = P[0].z - P[3].z;
+        const double rOO = sqrt(dx * dx + dy * dy + dz * dz);
+        if (rOO <= tb_r2f && rOO >= 2.) {
+            vr R[10];
+#pragma unroll
+            for (int q = 0; q < 6; ++q) R[q] = rel(P[q], P[0]);
+            extra_points(R[0], R[1], R[2], R[6], R[7]);
+            extra_points(R[3], R[4], R[5], R[8], R[9]);
+            real x[31], g[31];
+#pragma unroll
+            for (int n = 0; n < 31; ++n) {
+                const Var2 v = var2(n);
+                const vr d = R[v.a] - R[v.b];
+                const real r = sqrt(d.dot(d));
+                const real e = exp_t<real>((real)v.k * ((real)v.r0 - r));
+                x[n] = v.coul ? e / r : e;
+            }
+            const real Ep = pip2b_eval<real>(x, g);
+            // chain rule through the variables (recomputed rather than kept live)
+            vr F[10];
+#pragma unroll
+            for (int q = 0; q < 10; ++q) F[q] = vr(0, 0, 0);
+#pragma unroll
+            for (int n = 0; n < 31; ++n) {
+                const Var2 v = var2(n);
+                const vr d = R[v.a] - R[v.b];
+                const real r = sqrt(d.dot(d));
+                const real e = exp_t<real>((real)v.k * ((real)v.r0 - r));
+                real gg;
+                if (v.coul) {
+                    const real ri = (real)1 / r, val = e * ri;
+                    gg = -((real)v.k + ri) * val * ri;
+                } else {
+                    gg = -(real)v.k * e / r;
+                }
+                const vr dd = d * (gg * g[n]);
+                F[v.a] += dd;
+                F[v.b] -= dd;
+            }
+            extra_grads(R[0], R[1], R[2], F[6], F[7], F[0], F[1], F[2]);
+            extra_grads(R[3], R[4], R[5], F[8], F[9], F[3], F[4], F[5]);
+            // switch (mbpol_2body_constants.cpp:98-111), FP64
+            double sw, gsw;
+            if (rOO > tb_r2i) {
+                const double t1 = M_PI / (tb_r2f - tb_r2i);
+                const double xs = (rOO - tb_r2i) * t1;
+                gsw = -sin(xs) * t1 / 2.0;
+                sw = (1.0 + cos(xs)) / 2.0;
+            } else {
+                sw = 1.0;
+                gsw = 0.0;
+            }
+            const double s = sw * cal2joule * -10.;
+            const double gs = gsw * (double)Ep / rOO * cal2joule * -10.;
+#pragma unroll
+            for (int q = 0; q < 6; ++q) {
+                double fx = (double)F[q].x * s, fy = (double)F[q].y * s, fz = (double)F[q].z * s;
+                if (q == 0) { fx += gs * dx; fy += gs * dy; fz += gs * dz; }
+                if (q == 3) { fx -= gs * dx; fy -= gs * dy; fz -= gs * dz; }
+                add_force(force, at[q], fx, fy, fz);
+            }
+            e_out = sw * (double)Ep * cal2joule;
+        }
+    }
+    add_energy(energy, e_out);
+}
+
+#else
+// ---------------------------------------------------------------------------
+// 3-body: platforms/reference/src/MBPolReferenceThreeBodyForce.cpp:67-293.
+// Points (nm, relative to Oa): 0 Oa 1 Ha1 2 Ha2 3 Ob 4 Hb1 5 Hb2 6 Oc 7 Hc1 8 Hc2.
+// ---------------------------------------------------------------------------
+struct Var3 {
+    int a, b;
+    double k, r0;
+};
+#define V3(a, b, k, r) {a, b, k, r}
+__device__ __forceinline__ Var3 var3(int n) {
+    switch (n) {  // reference order (:170-206)
+        case 0: return V3(1, 2, th_kHH_intra, th_dHH_intra);
+        case 1: return V3(4, 5, th_kHH_intra, th_dHH_intra);
+        case 2: return V3(7, 8, th_kHH_intra, th_dHH_intra);
+        case 3: return V3(0, 1, th_kOH_intra, th_dOH_intra);
+        case 4: return V3(0, 2, th_kOH_intra, th_dOH_intra);
+        case 5: return V3(3, 4, th_kOH_intra, th_dOH_intra);
+        case 6: return V3(3, 5, th_kOH_intra, th_dOH_intra);
+        case 7: return V3(6, 7, th_kOH_intra, th_dOH_intra);
+        case 8: return V3(6, 8, th_kOH_intra, th_dOH_intra);
+        case 9: return V3(1, 4, th_kHH, th_dHH);
+        case 10: return V3(1, 5, th_kHH, th_dHH);
+        case 11: return V3(1, 7, th_kHH, th_dHH);
+        case 12: return V3(1, 8, th_kHH, th_dHH);
+        case 13: return V3(2, 4, th_kHH, th_dHH);
+        case 14: return V3(2, 5, th_kHH, th_dHH);
+        case 15: return V3(2, 7, th_kHH, th_dHH);
+        case 16: return V3(2, 8, th_kHH, th_dHH);
+        case 17: return V3(4, 7, th_kHH, th_dHH);
+        case 18: return V3(4, 8, th_kHH, th_dHH);
+        case 19: return V3(5, 7, th_kHH, th_dHH);
+        case 20: return V3(5, 8, th_kHH, th_dHH);
+        case 21: return V3(0, 4, th_kOH, th_dOH);
+        case 22: return V3(0, 5, th_kOH, th_dOH);
+        case 23: return V3(0, 7, th_kOH, th_dOH);
+        case 24: return V3(0, 8, th_kOH, th_dOH);
+        case 25: return V3(3, 1, th_kOH, th_dOH);
+        case 26: return V3(3, 2, th_kOH, th_dOH);
+        case 27: return V3(3, 7, th_kOH, th_dOH);
+        case 28: return V3(3, 8, th_kOH, th_dOH);
+        case 29: return V3(6, 1, th_kOH, th_dOH);
+        case 30: return V3(6, 2, th_kOH, th_dOH);
+        case 31: return V3(6, 4, th_kOH, th_dOH);
+        case 32: return V3(6, 5, th_kOH, th_dOH);
+        case 33: return V3(0, 3, th_kOO, th_dOO);
+        case 34: return V3(0, 6, th_kOO, th_dOO);
+        default: return V3(3, 6, th_kOO, th_dOO);
+    }
+}
+#undef V3
+
+__device__ __forceinline__ double switch3(double r, double& g) {  // (:103-120)
+    if (r > th_r3f) { g = 0.0; return 0.0; }
+    if (r > th_r3i) {
+        const double t1 = M_PI / (th_r3f - th_r3i);
+        const double x = (r - th_r3i) * t1;
+        g = -sin(x) * t1 / 2.0;
+        return (1.0 + cos(x)) / 2.0;
+    }
+    g = 0.0;
+    return 1.0;
+}
+
+__global__ void __launch_bounds__(64) k_threebody(long long nt, const int* __restrict__ trip, const int* __restrict__ idx,
+                                                  const double* __restrict__ pos, int periodic, d3 box,
+                                                  u64* __restrict__ force, u64* __restrict__ energy) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    double e_out = 0.0;
+    if (t < nt) {
+        int at[9];
+#pragma unroll
+        for (int m = 0; m < 3; ++m) {
+            const int mol = trip[3 * t + m];
+#pragma unroll
+            for (int q = 0; q < 3; ++q) at[3 * m + q] = idx[3 * mol + q];
+        }
+        d3 P[9];
+#pragma unroll
+        for (int q = 0; q < 9; ++q) P[q] = load_d3(pos, at[q]);
+        if (periodic) {  // imageMolecules in nm (MBPolReferenceTwoBodyForce.cpp:78-109)
+            P[1] = image_to(P[0], P[1], box);
+            P[2] = image_to(P[0], P[2], box);
+            P[3] = image_to(P[0], P[3], box);
+            P[4] = image_to(P[3], P[4], box);
+            P[5] = image_to(P[3], P[5], box);
+            P[6] = image_to(P[0], P[6], box);
+            P[7] = image_to(P[6], P[7], box);
+            P[8] = image_to(P[6], P[8], box);
+        }
+        const d3 rab{(P[0].x - P[3].x) * 10., (P[0].y - P[3].y) * 10., (P[0].z - P[3].z) * 10.};
+        const d3 rac{(P[0].x - P[6].x) * 10., (P[0].y - P[6].y) * 10., (P[0].z - P[6].z) * 10.};
+        const d3 rbc{(P[3].x - P[6].x) * 10., (P[3].y - P[6].y) * 10., (P[3].z - P[6].z) * 10.};
+        const double drab = sqrt(rab.x * rab.x + rab.y * rab.y + rab.z * rab.z);
+        const double drac = sqrt(rac.x * rac.x + rac.y * rac.y + rac.z * rac.z);
+        const double drbc = sqrt(rbc.x * rbc.x + rbc.y * rbc.y + rbc.z * rbc.z);
+        if (!(drab < 2 || drac < 2 || drbc < 2)) {
+            vr R[9];  // Angstrom, relative to Oa
+#pragma unroll
+            for (int q = 0; q < 9; ++q) {
+                const d3 p{P[q].x * 10., P[q].y * 10., P[q].z * 10.};
+                const d3 o{P[0].x * 10., P[0].y * 10., P[0].z * 10.};
+                R[q] = rel(p, o);
+            }
+            real x[36], g[36];
+#pragma unroll
+            for (int n = 0; n < 36; ++n) {
+                const Var3 v = var3(n);
+                const vr d = R[v.a] - R[v.b];
+                const real r = sqrt(d.dot(d));
+                x[n] = exp_t<real>(-(real)v.k * (r - (real)v.r0));
+            }
+            const real Ep = pip3b_eval<real>(x, g);
+            double gab, gac, gbc;
+            const double sab = switch3(drab, gab), sac = switch3(drac, gac), sbc = switch3(drbc, gbc);
+            const double s = sab * sac + sab * sbc + sac * sbc;
+            // g_var (:80-101): forces of the variables, times s; units kJ/mol/nm
+            vr F[9];
+#pragma unroll
+            for (int q = 0; q < 9; ++q) F[q] = vr(0, 0, 0);
+#pragma unroll
+            for (int n = 0; n < 36; ++n) {
+                const Var3 v = var3(n);
+                const vr d = R[v.a] - R[v.b];
+                const real r = sqrt(d.dot(d));
+                const real gg = -(real)v.k * g[n] * exp_t<real>(-(real)v.k * (r - (real)v.r0)) / r;
+                const vr dd = d * gg;
+                F[v.a] += dd;
+                F[v.b] -= dd;
+            }
+            const double fs = s * cal2joule * 10. * -1;
+            const double E = (double)Ep;
+            gab *= (sac + sbc) * E / drab;
+            gac *= (sab + sbc) * E / drac;
+            gbc *= (sab + sac) * E / drbc;
+            const double u = cal2joule * -nm_to_A;
+#pragma unroll
+            for (int q = 0; q < 9; ++q) {
+                double fx = (double)F[q].x * fs, fy = (double)F[q].y * fs, fz = (double)F[q].z * fs;
+                if (q == 0) {
+                    fx += (gab * rab.x + gac * rac.x) * u;
+                    fy += (gab * rab.y + gac * rac.y) * u;
+                    fz += (gab * rab.z + gac * rac.z) * u;
+                } else if (q == 3) {
+                    fx += (gbc * rbc.x - gab * rab.x) * u;
+                    fy += (gbc * rbc.y - gab * rab.y) * u;
+                    fz += (gbc * rbc.z - gab * rab.z) * u;
+                } else if (q == 6) {
+                    fx -= (gac * rac.x + gbc * rbc.x) * u;
+                    fy -= (gac * rac.y + gbc * rbc.y) * u;
+                    fz -= (gac * rac.z + gbc * rbc.z) * u;
+                }
+                add_force(force, at[q], fx, fy, fz);
+            }
+            e_out = E * s * cal2joule;
+        }
+    }
+    add_energy(energy, e_out);
+}
+#endif
+
+}  // namespace
+}  // namespace mbpol
+
+#define PIP_CAT2(a, b) a##b
+#define PIP_CAT(a, b) PIP_CAT2(a, b)
+namespace mbpol {
+static inline int nblk(long long n, int b) { return (int)((n + b - 1) / b); }
+#if PIP_BODY == 2
+void PIP_CAT(launch_twobody_, PIP_SUFFIX)(hipStream_t s, long long npairs, const int2* pairs, const int* idx,
+                                          const double* pos, int periodic, d3 box, u64* force, u64* energy) {
+    const d3 boxA{box.x * 10., box.y * 10., box.z * 10.};
+    if (npairs > 0)
+        hipLaunchKernelGGL(k_twobody, nblk(npairs, 64), 64, 0, s, npairs, pairs, idx, pos, periodic, boxA, force, energy);
+}
+#else
+void PIP_CAT(launch_threebody_, PIP_SUFFIX)(hipStream_t s, long long ntrip, const int* trip, const int* idx,
+                                            const double* pos, int periodic, d3 box, u64* force, u64* energy) {
+    if (ntrip > 0)
+        hipLaunchKernelGGL(k_threebody, nblk(ntrip, 64), 64, 0, s, ntrip, trip, idx, pos, periodic, box, force, energy);
+}
+#endif
+}  // namespace mbpol

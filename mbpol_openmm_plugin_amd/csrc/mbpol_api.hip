@@ -1,0 +1,673 @@
+// C ABI of the MB-pol HIP platform (include/mbpol_hip.h): system description,
+// context set-up and the evaluation driver that sequences the kernels of one
+// energy+force evaluation on the context's stream.
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "../../include/mbpol_hip.h"
+#include "context.hpp"
+#include "elec.hpp"
+
+using namespace mbpol;
+
+static thread_local std::string g_last_error;
+
+int mbpol::set_error(int code, const char* fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                                   \
+    do {                                                                                                \
+        hipError_t _e = (expr);                                                                         \
+        if (_e != hipSuccess) return set_error(MBPOL_ERR_HIP, "%s: %s", #expr, hipGetErrorString(_e)); \
+    } while (0)
+
+DevBuf::~DevBuf() {
+    if (p) (void)hipFree(p);
+}
+
+hipError_t DevBuf::ensure(size_t nbytes) {
+    if (nbytes <= bytes && p) return hipSuccess;
+    if (p) {
+        hipError_t e = hipFree(p);
+        if (e != hipSuccess) return e;
+        p = nullptr;
+    }
+    size_t want = std::max<size_t>(nbytes, 256);
+    hipError_t e = hipMalloc(&p, want);
+    if (e == hipSuccess) bytes = want;
+    else bytes = 0;
+    return e;
+}
+
+// ---------------------------------------------------------------------------
+// scans (hipCUB)
+// ---------------------------------------------------------------------------
+template <typename In, typename Out>
+static int exclusive_scan(mbpol_context* ctx, const In* in, Out* out, int n) {
+    size_t bytes = 0;
+    HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, in, out, n, ctx->stream));
+    HIP_TRY(ctx->scan_tmp.ensure(bytes));
+    HIP_TRY(hipcub::DeviceScan::ExclusiveSum(ctx->scan_tmp.p, bytes, in, out, n, ctx->stream));
+    return MBPOL_OK;
+}
+int mbpol::scan_i32(mbpol_context* ctx, const int* in, int* out, int n) { return exclusive_scan(ctx, in, out, n); }
+int mbpol::scan_i64(mbpol_context* ctx, const long long* in, long long* out, int n) {
+    return exclusive_scan(ctx, in, out, n);
+}
+
+// ---------------------------------------------------------------------------
+// HIP-event timing
+// ---------------------------------------------------------------------------
+static hipEvent_t take_event(mbpol_context* ctx) {
+    Timer& t = ctx->timer;
+    if (!t.pool.empty()) {
+        hipEvent_t e = t.pool.back();
+        t.pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+void mbpol::timer_start(mbpol_context* ctx, int cat) {
+    if (!ctx->timer.enabled) return;
+    hipEvent_t e = take_event(ctx);
+    if (!e) return;
+    (void)hipEventRecord(e, ctx->stream);
+    ctx->timer.open[cat] = e;
+}
+
+void mbpol::timer_stop(mbpol_context* ctx, int cat) {
+    Timer& t = ctx->timer;
+    if (!t.enabled || !t.open[cat]) return;
+    hipEvent_t e = take_event(ctx);
+    if (!e) return;
+    (void)hipEventRecord(e, ctx->stream);
+    t.pending[cat].push_back({t.open[cat], e});
+    t.open[cat] = nullptr;
+}
+
+static void timer_collect(mbpol_context* ctx) {
+    Timer& t = ctx->timer;
+    (void)hipStreamSynchronize(ctx->stream);
+    for (int c = 0; c < T_NCAT; ++c) {
+        for (auto& pr : t.pending[c]) {
+            float ms = 0;
+            if (hipEventElapsedTime(&ms, pr.first, pr.second) == hipSuccess) {
+                t.ms[c] += ms;
+                t.count[c] += 1;
+            }
+            t.pool.push_back(pr.first);
+            t.pool.push_back(pr.second);
+        }
+        t.pending[c].clear();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// system description
+// ---------------------------------------------------------------------------
+extern "C" {
+
+const char* mbpol_last_error(void) { return g_last_error.c_str(); }
+
+int mbpol_device_count(int32_t* count) {
+    int c = 0;
+    hipError_t e = hipGetDeviceCount(&c);
+    if (e != hipSuccess) c = 0;
+    *count = c;
+    return MBPOL_OK;
+}
+
+int mbpol_system_create(int32_t num_particles, mbpol_system** out) {
+    if (!out || num_particles <= 0) return set_error(MBPOL_ERR_INVALID_ARGUMENT, "num_particles must be positive");
+    mbpol_system* s = new mbpol_system();
+    s->n = num_particles;
+    *out = s;
+    return MBPOL_OK;
+}
+
+void mbpol_system_destroy(mbpol_system* sys) { delete sys; }
+
+int mbpol_system_set_box(mbpol_system* sys, const double box_nm[3]) {
+    for (int d = 0; d < 3; ++d) {
+        if (box_nm[d] < 0) return set_error(MBPOL_ERR_INVALID_ARGUMENT, "negative box length");
+        sys->box[d] = box_nm[d];
+    }
+    return MBPOL_OK;
+}
+
+static bool valid_particle(const mbpol_system* s, int i) { return i >= 0 && i < s->n; }
+
+int mbpol_system_add_virtual_site(mbpol_system* sys, int32_t site, int32_t p1, int32_t p2, int32_t p3, double w1,
+                                  double w2, double w3) {
+    if (!valid_particle(sys, site) || !valid_particle(sys, p1) || !valid_particle(sys, p2) || !valid_particle(sys, p3))
+        return set_error(MBPOL_ERR_INVALID_ARGUMENT, "virtual site index out of range");
+    sys->vs_idx.insert(sys->vs_idx.end(), {site, p1, p2, p3});
+    sys->vs_w.insert(sys->vs_w.end(), {w1, w2, w3});
+    return MBPOL_OK;
+}
+
+static int check_molecules(const mbpol_system* sys, int32_t n, const int32_t* idx, const char* what) {
+    if (n < 0 || (n > 0 && !idx)) return set_error(MBPOL_ERR_INVALID_ARGUMENT, "%s: bad molecule list", what);
+    for (int i = 0; i < 3 * n; ++i)
+        if (!valid_particle(sys, idx[i]))
+            return set_error(MBPOL_ERR_INVALID_ARGUMENT, "%s: particle index %d out of range", what, idx[i]);
+    return MBPOL_OK;
+}
+
+int mbpol_system_set_onebody(mbpol_system* sys, int32_t n, const int32_t* idx, int32_t method) {
+    if (int rc = check_molecules(sys, n, idx, "MBPolOneBodyForce")) return rc;
+    if (method != MBPOL_ONEBODY_NONPERIODIC && method != MBPOL_ONEBODY_PERIODIC)
+        return set_error(MBPOL_ERR_INVALID_ARGUMENT, "MBPolOneBodyForce: unknown nonbonded method %d", method);
+    sys->ob_idx.assign(idx, idx + 3 * n);
+    sys->ob_method = method;
+    return MBPOL_OK;
+}
+
+static int set_nbody(mbpol_system* sys, int32_t n, const int32_t* idx, int32_t method, double cutoff,
+                     std::vector<int>& dst, int& m, double& rc, const char* what) {
+    if (int e = check_molecules(sys, n, idx, what)) return e;
+    if (method < 0 || method > 2) return set_error(MBPOL_ERR_INVALID_ARGUMENT, "%s: unknown nonbonded method %d", what, method);
+    if (!(cutoff > 0)) return set_error(MBPOL_ERR_INVALID_ARGUMENT, "%s: cutoff must be positive", what);
+    dst.assign(idx, idx + 3 * n);
+    m = method;
+    rc = cutoff;
+    return MBPOL_OK;
+}
+
+int mbpol_system_set_twobody(mbpol_system* sys, int32_t n, const int32_t* idx, int32_t method, double cutoff_nm) {
+    return set_nbody(sys, n, idx, method, cutoff_nm, sys->tb_idx, sys->tb_method, sys->tb_cutoff, "MBPolTwoBodyForce");
+}
+
+int mbpol_system_set_threebody(mbpol_system* sys, int32_t n, const int32_t* idx, int32_t method, double cutoff_nm) {
+    return set_nbody(sys, n, idx, method, cutoff_nm, sys->th_idx, sys->th_method, sys->th_cutoff,
+                     "MBPolThreeBodyForce");
+}
+
+int mbpol_system_set_electrostatics(mbpol_system* sys, int32_t n, const double* charge, const int32_t* molecule,
+                                    const int32_t* atom_type, const double* damping, const double* polarity,
+                                    int32_t method, double cutoff_nm, double ewald_alpha, const int32_t grid[3],
+                                    double ewald_tol, int32_t max_iter, double target_eps,
+                                    int32_t charge_redistribution, const double thole[5]) {
+    // MBPolElectrostaticsForceImpl.cpp:53-54: one entry per particle of the System
+    if (n != sys->n)
+        return set_error(MBPOL_ERR_INVALID_ARGUMENT,
+                         "MBPolElectrostaticsForce must have exactly as many particles as the System it belongs to.");
+    if (method != MBPOL_ELEC_NOCUTOFF && method != MBPOL_ELEC_PME)
+        return set_error(MBPOL_ERR_INVALID_ARGUMENT, "MBPolElectrostaticsForce: unknown nonbonded method %d", method);
+    if (charge_redistribution && n % 4)
+        return set_error(MBPOL_ERR_INVALID_ARGUMENT,
+                         "charge redistribution needs O,H1,H2,M sites per water (4 per molecule)");
+    for (int i = 0; i < n; ++i)
+        if (atom_type[i] < 0 || atom_type[i] > 2)
+            return set_error(MBPOL_ERR_INVALID_ARGUMENT, "atom type must be 0 (O), 1 (H) or 2 (M)");
+    sys->ne = n;
+    sys->el_q.assign(charge, charge + n);
+    sys->el_mol.assign(molecule, molecule + n);
+    sys->el_type.assign(atom_type, atom_type + n);
+    sys->el_damp.assign(damping, damping + n);
+    sys->el_pol.assign(polarity, polarity + n);
+    sys->el_method = method;
+    sys->el_cutoff = cutoff_nm;
+    sys->el_alpha = ewald_alpha;
+    for (int d = 0; d < 3; ++d) sys->el_grid[d] = grid ? grid[d] : 0;
+    sys->el_tol = ewald_tol;
+    sys->el_max_iter = max_iter;
+    sys->el_eps = target_eps;
+    sys->el_redistribute = charge_redistribution;
+    if (thole)
+        for (int k = 0; k < 5; ++k) sys->el_thole[k] = thole[k];
+    return MBPOL_OK;
+}
+
+int mbpol_system_set_dispersion(mbpol_system* sys, int32_t n, const int32_t* type, const int32_t* molecule,
+                                int32_t num_types, const double* c6, const double* d6, int32_t method,
+                                double cutoff_nm) {
+    if (n != sys->n) return set_error(MBPOL_ERR_INVALID_ARGUMENT, "dispersion: one type per particle required");
+    if (num_types <= 0 || method < 0 || method > 2)
+        return set_error(MBPOL_ERR_INVALID_ARGUMENT, "dispersion: bad type count or method");
+    for (int i = 0; i < n; ++i)
+        if (type[i] < 0 || type[i] >= num_types) return set_error(MBPOL_ERR_INVALID_ARGUMENT, "dispersion: bad type");
+    sys->nd = n;
+    sys->ds_type.assign(type, type + n);
+    sys->ds_mol.assign(molecule, molecule + n);
+    sys->ds_ntypes = num_types;
+    sys->ds_c6.assign(c6, c6 + num_types * num_types);
+    sys->ds_d6.assign(d6, d6 + num_types * num_types);
+    sys->ds_method = method;
+    sys->ds_cutoff = cutoff_nm;
+    return MBPOL_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// context
+// ---------------------------------------------------------------------------
+static bool periodic_box(const mbpol_system& s) { return s.box[0] > 0 && s.box[1] > 0 && s.box[2] > 0; }
+
+template <typename T>
+static int upload(mbpol_context* ctx, DevBuf& b, const std::vector<T>& v) {
+    HIP_TRY(b.ensure(std::max<size_t>(1, v.size()) * sizeof(T)));
+    if (!v.empty()) HIP_TRY(hipMemcpy(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    return MBPOL_OK;
+}
+
+// box/cutoff checks of the reference (MBPol*ForceImpl.cpp:60-66, MBPolReferenceKernels.cpp:224-227,387-390,471-474)
+static int check_box(const mbpol_system& s) {
+    const bool pbc = periodic_box(s);
+    auto chk = [&](bool uses_pbc, double rc, const char* what) -> int {
+        if (!uses_pbc) return MBPOL_OK;
+        if (!pbc) return set_error(MBPOL_ERR_INVALID_ARGUMENT, "%s: periodic method without a periodic box", what);
+        for (int d = 0; d < 3; ++d)
+            if (s.box[d] < 1.999999 * rc)
+                return set_error(MBPOL_ERR_BOX_TOO_SMALL,
+                                 "%s: The periodic box size has decreased to less than twice the cutoff.", what);
+        return MBPOL_OK;
+    };
+    if (int e = chk(!s.tb_idx.empty() && s.tb_method == MBPOL_CUTOFF_PERIODIC, s.tb_cutoff, "MBPolTwoBodyForce")) return e;
+    if (int e = chk(!s.th_idx.empty() && s.th_method == MBPOL_CUTOFF_PERIODIC, s.th_cutoff, "MBPolThreeBodyForce")) return e;
+    if (int e = chk(s.ne > 0 && s.el_method == MBPOL_ELEC_PME, s.el_cutoff, "MBPolElectrostaticsForce")) return e;
+    if (int e = chk(s.nd > 0 && s.ds_method == MBPOL_CUTOFF_PERIODIC, s.ds_cutoff, "dispersion")) return e;
+    if (!s.ob_idx.empty() && s.ob_method == MBPOL_ONEBODY_PERIODIC && !pbc)
+        return set_error(MBPOL_ERR_INVALID_ARGUMENT, "MBPolOneBodyForce: periodic method without a periodic box");
+    return MBPOL_OK;
+}
+
+// Cell grid for an O-O list: 27-cell search when the periodic box holds >= 3 cells of edge >= rc per
+// dimension, otherwise every pair is visited (small clusters, huge cutoffs).
+static int setup_grid(mbpol_context* ctx, CellGrid& cg, int n, bool periodic, double rc) {
+    const mbpol_system& s = ctx->sys;
+    cg.g.periodic = periodic ? 1 : 0;
+    cg.g.L = d3{s.box[0], s.box[1], s.box[2]};
+    cg.g.rc = rc;
+    cg.g.use_cells = 0;
+    cg.g.nc = make_int3(1, 1, 1);
+    if (periodic) {
+        int nc[3];
+        for (int d = 0; d < 3; ++d) nc[d] = (int)std::floor(s.box[d] / rc);
+        if (nc[0] >= 3 && nc[1] >= 3 && nc[2] >= 3) {
+            cg.g.use_cells = 1;
+            cg.g.nc = make_int3(nc[0], nc[1], nc[2]);
+        }
+    }
+    cg.ncell = cg.g.nc.x * cg.g.nc.y * cg.g.nc.z;
+    HIP_TRY(cg.cell_of.ensure(sizeof(int) * std::max(n, 1)));
+    HIP_TRY(cg.cell_count.ensure(sizeof(int) * (cg.ncell + 1)));
+    HIP_TRY(cg.cell_start.ensure(sizeof(int) * (cg.ncell + 1)));
+    HIP_TRY(cg.cell_fill.ensure(sizeof(int) * (cg.ncell + 1)));
+    HIP_TRY(cg.members.ensure(sizeof(int) * std::max(n, 1)));
+    return MBPOL_OK;
+}
+
+static int bin_cells(mbpol_context* ctx, CellGrid& cg, int n, const int* idx) {
+    if (!cg.g.use_cells) return MBPOL_OK;
+    HIP_TRY(hipMemsetAsync(cg.cell_count.p, 0, sizeof(int) * (cg.ncell + 1), ctx->stream));
+    HIP_TRY(hipMemsetAsync(cg.cell_fill.p, 0, sizeof(int) * (cg.ncell + 1), ctx->stream));
+    launch_cell_count(ctx->stream, n, idx, ctx->pos.as<double>(), cg.g.L, cg.g.nc, cg.cell_of.as<int>(),
+                      cg.cell_count.as<int>());
+    if (int e = scan_i32(ctx, cg.cell_count.as<int>(), cg.cell_start.as<int>(), cg.ncell + 1)) return e;
+    launch_cell_fill(ctx->stream, n, cg.cell_of.as<int>(), cg.cell_start.as<int>(), cg.cell_fill.as<int>(),
+                     cg.members.as<int>());
+    return MBPOL_OK;
+}
+
+extern "C" {
+
+int mbpol_context_create(const mbpol_system* sys, int32_t precision, int32_t device, mbpol_context** out) {
+    if (!sys || !out) return set_error(MBPOL_ERR_INVALID_ARGUMENT, "null argument");
+    if (precision != MBPOL_PRECISION_MIXED && precision != MBPOL_PRECISION_DOUBLE)
+        return set_error(MBPOL_ERR_INVALID_ARGUMENT, "unknown precision mode %d", precision);
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
+        return set_error(MBPOL_ERR_NO_DEVICE, "no HIP device available");
+    if (device < 0 || device >= ndev) return set_error(MBPOL_ERR_NO_DEVICE, "device %d not present", device);
+    // cutoff <= half box at initialisation (MBPolTwoBodyForceImpl.cpp:60-66 and siblings)
+    if (periodic_box(*sys)) {
+        auto half = [&](bool on, double rc, const char* what) -> int {
+            if (!on) return MBPOL_OK;
+            for (int d = 0; d < 3; ++d)
+                if (rc > 0.5 * sys->box[d])
+                    return set_error(MBPOL_ERR_CUTOFF_TOO_LARGE,
+                                     "%s: The cutoff distance cannot be greater than half the periodic box size.", what);
+            return MBPOL_OK;
+        };
+        if (int e = half(!sys->tb_idx.empty() && sys->tb_method == MBPOL_CUTOFF_PERIODIC, sys->tb_cutoff, "MBPolTwoBodyForce")) return e;
+        if (int e = half(!sys->th_idx.empty() && sys->th_method == MBPOL_CUTOFF_PERIODIC, sys->th_cutoff, "MBPolThreeBodyForce")) return e;
+        if (int e = half(sys->ne > 0 && sys->el_method == MBPOL_ELEC_PME, sys->el_cutoff, "MBPolElectrostaticsForce")) return e;
+    }
+    if (int e = check_box(*sys)) return e;
+    HIP_TRY(hipSetDevice(device));
+    mbpol_context* ctx = new mbpol_context();
+    ctx->sys = *sys;
+    ctx->precision = precision;
+    ctx->device = device;
+    auto fail = [&](int code) {
+        mbpol_context_destroy(ctx);
+        return code;
+    };
+    if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess)
+        return fail(set_error(MBPOL_ERR_HIP, "hipStreamCreate failed"));
+    if (hipHostMalloc(&ctx->h_counts, 4096, hipHostMallocDefault) != hipSuccess)
+        return fail(set_error(MBPOL_ERR_HIP, "hipHostMalloc failed"));
+    const int n = sys->n;
+    if (ctx->pos.ensure(sizeof(double) * 3 * n) || ctx->force_fx.ensure(sizeof(u64) * 3 * n) ||
+        ctx->force.ensure(sizeof(double) * 3 * n) || ctx->energy_fx.ensure(sizeof(u64) * 8))
+        return fail(set_error(MBPOL_ERR_HIP, "device allocation failed"));
+    if (int e = upload(ctx, ctx->vs_idx, sys->vs_idx)) return fail(e);
+    if (int e = upload(ctx, ctx->vs_w, sys->vs_w)) return fail(e);
+    if (int e = upload(ctx, ctx->ob_idx, sys->ob_idx)) return fail(e);
+    if (int e = upload(ctx, ctx->tb_idx, sys->tb_idx)) return fail(e);
+    if (int e = upload(ctx, ctx->th_idx, sys->th_idx)) return fail(e);
+    const int n2 = (int)sys->tb_idx.size() / 3, n3 = (int)sys->th_idx.size() / 3;
+    if (n2 > 0) {
+        if (int e = setup_grid(ctx, ctx->grid2, n2, sys->tb_method == MBPOL_CUTOFF_PERIODIC, sys->tb_cutoff)) return fail(e);
+        if (ctx->dimer_count.ensure(sizeof(long long) * (n2 + 1)) || ctx->dimer_off.ensure(sizeof(long long) * (n2 + 1)))
+            return fail(set_error(MBPOL_ERR_HIP, "device allocation failed"));
+    }
+    if (n3 > 0) {
+        if (int e = setup_grid(ctx, ctx->grid3, n3, sys->th_method == MBPOL_CUTOFF_PERIODIC, sys->th_cutoff)) return fail(e);
+        if (ctx->lo_count.ensure(sizeof(int) * (n3 + 1)) || ctx->lo_start.ensure(sizeof(int) * (n3 + 1)) ||
+            ctx->trip_count.ensure(sizeof(long long) * (n3 + 1)) || ctx->trip_off.ensure(sizeof(long long) * (n3 + 1)))
+            return fail(set_error(MBPOL_ERR_HIP, "device allocation failed"));
+    }
+    if (sys->ne > 0) {
+        if (int e = elec_create(ctx)) return fail(e);
+    }
+    if (sys->nd > 0) {
+        if (int e = disp_create(ctx)) return fail(e);
+    }
+    *out = ctx;
+    return MBPOL_OK;
+}
+
+void mbpol_context_destroy(mbpol_context* ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    timer_collect(ctx);
+    for (hipEvent_t e : ctx->timer.pool) (void)hipEventDestroy(e);
+    elec_destroy(ctx);
+    disp_destroy(ctx);
+    if (ctx->h_counts) (void)hipHostFree(ctx->h_counts);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+int mbpol_context_set_box(mbpol_context* ctx, const double box_nm[3]) {
+    mbpol_system s = ctx->sys;
+    for (int d = 0; d < 3; ++d) s.box[d] = box_nm[d];
+    if (int e = check_box(s)) return e;
+    for (int d = 0; d < 3; ++d) ctx->sys.box[d] = box_nm[d];
+    HIP_TRY(hipSetDevice(ctx->device));
+    const int n2 = (int)ctx->sys.tb_idx.size() / 3, n3 = (int)ctx->sys.th_idx.size() / 3;
+    if (n2 > 0)
+        if (int e = setup_grid(ctx, ctx->grid2, n2, ctx->sys.tb_method == MBPOL_CUTOFF_PERIODIC, ctx->sys.tb_cutoff)) return e;
+    if (n3 > 0)
+        if (int e = setup_grid(ctx, ctx->grid3, n3, ctx->sys.th_method == MBPOL_CUTOFF_PERIODIC, ctx->sys.th_cutoff)) return e;
+    if (ctx->elec)
+        if (int e = elec_set_box(ctx)) return e;
+    if (ctx->disp)
+        if (int e = disp_set_box(ctx)) return e;
+    return MBPOL_OK;
+}
+
+int mbpol_context_set_list_mode(mbpol_context* ctx, int32_t full_chain_list) {
+    ctx->full_chain_list = full_chain_list ? 1 : 0;
+    return MBPOL_OK;
+}
+
+}  // extern "C"
+
+// Build the dimer list and the trimer work list; returns their sizes after one sync.
+static int build_lists(mbpol_context* ctx, bool want2, bool want3) {
+    const mbpol_system& s = ctx->sys;
+    const int n2 = (int)s.tb_idx.size() / 3, n3 = (int)s.th_idx.size() / 3;
+    const double* pos = ctx->pos.as<double>();
+    long long* hc = static_cast<long long*>(ctx->h_counts);
+    if (want2) {
+        CellGrid& g = ctx->grid2;
+        if (int e = bin_cells(ctx, g, n2, ctx->tb_idx.as<int>())) return e;
+        HIP_TRY(hipMemsetAsync(ctx->dimer_count.p, 0, sizeof(long long) * (n2 + 1), ctx->stream));
+        launch_dimer_pass(ctx->stream, n2, ctx->tb_idx.as<int>(), pos, g.g, g.cell_of.as<int>(), g.cell_start.as<int>(),
+                          g.members.as<int>(), ctx->dimer_count.as<long long>(), nullptr, nullptr);
+        if (int e = scan_i64(ctx, ctx->dimer_count.as<long long>(), ctx->dimer_off.as<long long>(), n2 + 1)) return e;
+        HIP_TRY(hipMemcpyAsync(&hc[0], ctx->dimer_off.as<long long>() + n2, sizeof(long long), hipMemcpyDeviceToHost,
+                               ctx->stream));
+    }
+    if (want3) {
+        CellGrid& g = ctx->grid3;
+        if (int e = bin_cells(ctx, g, n3, ctx->th_idx.as<int>())) return e;
+        HIP_TRY(hipMemsetAsync(ctx->lo_count.p, 0, sizeof(int) * (n3 + 1), ctx->stream));
+        launch_lower_pass(ctx->stream, n3, ctx->th_idx.as<int>(), pos, g.g, g.cell_of.as<int>(), g.cell_start.as<int>(),
+                          g.members.as<int>(), ctx->lo_count.as<int>(), nullptr, nullptr);
+        if (int e = scan_i32(ctx, ctx->lo_count.as<int>(), ctx->lo_start.as<int>(), n3 + 1)) return e;
+        HIP_TRY(hipMemcpyAsync(&hc[1], ctx->lo_start.as<int>() + n3, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    }
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if (want2) {
+        const long long nd = hc[0];
+        if (nd > ctx->dimer_cap) {
+            HIP_TRY(ctx->dimers.ensure(sizeof(int2) * (nd + nd / 4 + 64)));
+            ctx->dimer_cap = nd + nd / 4 + 64;
+        }
+        CellGrid& g = ctx->grid2;
+        launch_dimer_pass(ctx->stream, n2, ctx->tb_idx.as<int>(), pos, g.g, g.cell_of.as<int>(), g.cell_start.as<int>(),
+                          g.members.as<int>(), nullptr, ctx->dimer_off.as<long long>(), ctx->dimers.as<int2>());
+        ctx->stats.dimers = nd;
+    }
+    if (want3) {
+        const int nl = (int)(reinterpret_cast<int*>(&hc[1])[0]);
+        if (nl > ctx->lower_cap) {
+            HIP_TRY(ctx->lower.ensure(sizeof(int) * (nl + nl / 4 + 64)));
+            ctx->lower_cap = nl + nl / 4 + 64;
+        }
+        CellGrid& g = ctx->grid3;
+        launch_lower_pass(ctx->stream, n3, ctx->th_idx.as<int>(), pos, g.g, g.cell_of.as<int>(), g.cell_start.as<int>(),
+                          g.members.as<int>(), nullptr, ctx->lo_start.as<int>(), ctx->lower.as<int>());
+        HIP_TRY(hipMemsetAsync(ctx->trip_count.p, 0, sizeof(long long) * (n3 + 1), ctx->stream));
+        launch_trimer_pass(ctx->stream, n3, ctx->th_idx.as<int>(), pos, g.g, ctx->lo_start.as<int>(), ctx->lower.as<int>(),
+                           ctx->full_chain_list, ctx->trip_count.as<long long>(), nullptr, nullptr);
+        if (int e = scan_i64(ctx, ctx->trip_count.as<long long>(), ctx->trip_off.as<long long>(), n3 + 1)) return e;
+        HIP_TRY(hipMemcpyAsync(&hc[2], ctx->trip_off.as<long long>() + n3, sizeof(long long), hipMemcpyDeviceToHost,
+                               ctx->stream));
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        const long long nt = hc[2];
+        if (nt > ctx->trip_cap) {
+            HIP_TRY(ctx->trips.ensure(sizeof(int) * 3 * (nt + nt / 4 + 64)));
+            ctx->trip_cap = nt + nt / 4 + 64;
+        }
+        launch_trimer_pass(ctx->stream, n3, ctx->th_idx.as<int>(), pos, g.g, ctx->lo_start.as<int>(), ctx->lower.as<int>(),
+                           ctx->full_chain_list, nullptr, ctx->trip_off.as<long long>(), ctx->trips.as<int>());
+        ctx->stats.trimers = nt;
+    }
+    return MBPOL_OK;
+}
+
+static int evaluate(mbpol_context* ctx, double* forces_dev_out, double* energies_out, uint32_t comp) {
+    const mbpol_system& s = ctx->sys;
+    hipStream_t st = ctx->stream;
+    u64* F = ctx->force_fx.as<u64>();
+    u64* E = ctx->energy_fx.as<u64>();
+    double* pos = ctx->pos.as<double>();
+    const d3 box{s.box[0], s.box[1], s.box[2]};
+    HIP_TRY(hipMemsetAsync(F, 0, sizeof(u64) * 3 * s.n, st));
+    HIP_TRY(hipMemsetAsync(E, 0, sizeof(u64) * 8, st));
+    launch_place_vsites(st, (int)s.vs_idx.size() / 4, ctx->vs_idx.as<int>(), ctx->vs_w.as<double>(), pos);
+    const bool do1 = (comp & MBPOL_ONEBODY) && !s.ob_idx.empty();
+    const bool do2 = (comp & MBPOL_TWOBODY) && !s.tb_idx.empty();
+    const bool do3 = (comp & MBPOL_THREEBODY) && !s.th_idx.empty();
+    if (do1) {
+        timer_start(ctx, T_ONEBODY);
+        launch_onebody(st, (int)s.ob_idx.size() / 3, ctx->ob_idx.as<int>(), pos, s.ob_method == MBPOL_ONEBODY_PERIODIC,
+                       box, F, E + 0);
+        timer_stop(ctx, T_ONEBODY);
+    }
+    if (do2 || do3) {
+        timer_start(ctx, T_LISTS);
+        if (int e = build_lists(ctx, do2, do3)) return e;
+        timer_stop(ctx, T_LISTS);
+    }
+    const bool dp = ctx->precision == MBPOL_PRECISION_DOUBLE;
+    if (do2) {
+        const int per = s.tb_method == MBPOL_CUTOFF_PERIODIC;
+        timer_start(ctx, T_TWOBODY);
+        if (dp)
+            launch_twobody_f64(st, ctx->stats.dimers, ctx->dimers.as<int2>(), ctx->tb_idx.as<int>(), pos, per, box, F, E + 1);
+        else
+            launch_twobody_f32(st, ctx->stats.dimers, ctx->dimers.as<int2>(), ctx->tb_idx.as<int>(), pos, per, box, F, E + 1);
+        timer_stop(ctx, T_TWOBODY);
+    }
+    if (do3) {
+        const int per = s.th_method == MBPOL_CUTOFF_PERIODIC;
+        timer_start(ctx, T_THREEBODY);
+        if (dp)
+            launch_threebody_f64(st, ctx->stats.trimers, ctx->trips.as<int>(), ctx->th_idx.as<int>(), pos, per, box, F, E + 2);
+        else
+            launch_threebody_f32(st, ctx->stats.trimers, ctx->trips.as<int>(), ctx->th_idx.as<int>(), pos, per, box, F, E + 2);
+        timer_stop(ctx, T_THREEBODY);
+    }
+    if ((comp & MBPOL_DISPERSION) && ctx->disp) {
+        timer_start(ctx, T_DISPERSION);
+        if (int e = disp_evaluate(ctx, E + 3)) return e;
+        timer_stop(ctx, T_DISPERSION);
+    }
+    if ((comp & MBPOL_ELECTROSTATICS) && ctx->elec)
+        if (int e = elec_evaluate(ctx, E + 4)) return e;
+    double* fout = forces_dev_out ? forces_dev_out : ctx->force.as<double>();
+    launch_forces_to_double(st, 3 * s.n, F, fout);
+    launch_fold_vsites(st, (int)s.vs_idx.size() / 4, ctx->vs_idx.as<int>(), ctx->vs_w.as<double>(), fout);
+    u64* he = static_cast<u64*>(ctx->h_counts) + 8;
+    HIP_TRY(hipMemcpyAsync(he, E, sizeof(u64) * 8, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    if (energies_out) {
+        double tot = 0;
+        for (int k = 0; k < 5; ++k) {
+            energies_out[k] = (double)(long long)he[k] * kFixInvHost;
+            tot += energies_out[k];
+        }
+        energies_out[5] = tot;
+    }
+    return MBPOL_OK;
+}
+
+extern "C" {
+
+int mbpol_context_evaluate(mbpol_context* ctx, const double* positions, double* forces_out, double* energies_out,
+                           uint32_t components) {
+    if (!ctx || !positions) return set_error(MBPOL_ERR_INVALID_ARGUMENT, "null argument");
+    HIP_TRY(hipSetDevice(ctx->device));
+    HIP_TRY(hipMemcpyAsync(ctx->pos.p, positions, sizeof(double) * 3 * ctx->sys.n, hipMemcpyHostToDevice, ctx->stream));
+    timer_start(ctx, T_EVALUATE);
+    if (int e = evaluate(ctx, nullptr, energies_out, components)) return e;
+    timer_stop(ctx, T_EVALUATE);
+    if (forces_out) {
+        HIP_TRY(hipMemcpyAsync(forces_out, ctx->force.p, sizeof(double) * 3 * ctx->sys.n, hipMemcpyDeviceToHost,
+                               ctx->stream));
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+    }
+    return MBPOL_OK;
+}
+
+int mbpol_context_evaluate_device(mbpol_context* ctx, const double* d_positions, double* d_forces, double* energies_out,
+                                  uint32_t components) {
+    if (!ctx || !d_positions) return set_error(MBPOL_ERR_INVALID_ARGUMENT, "null argument");
+    HIP_TRY(hipSetDevice(ctx->device));
+    if (d_positions != ctx->pos.p)
+        HIP_TRY(hipMemcpyAsync(ctx->pos.p, d_positions, sizeof(double) * 3 * ctx->sys.n, hipMemcpyDeviceToDevice,
+                               ctx->stream));
+    timer_start(ctx, T_EVALUATE);
+    if (int e = evaluate(ctx, d_forces, energies_out, components)) return e;
+    timer_stop(ctx, T_EVALUATE);
+    return MBPOL_OK;
+}
+
+int mbpol_context_upload_positions(mbpol_context* ctx, const double* positions) {
+    if (!ctx || !positions) return set_error(MBPOL_ERR_INVALID_ARGUMENT, "null argument");
+    HIP_TRY(hipSetDevice(ctx->device));
+    HIP_TRY(hipMemcpy(ctx->pos.p, positions, sizeof(double) * 3 * ctx->sys.n, hipMemcpyHostToDevice));
+    // keep an untouched copy: evaluate() rewrites the virtual-site rows in place
+    HIP_TRY(ctx->pos_resident.ensure(sizeof(double) * 3 * ctx->sys.n));
+    HIP_TRY(hipMemcpy(ctx->pos_resident.p, ctx->pos.p, sizeof(double) * 3 * ctx->sys.n, hipMemcpyDeviceToDevice));
+    return MBPOL_OK;
+}
+
+int mbpol_context_evaluate_resident(mbpol_context* ctx, double* energies_out, uint32_t components) {
+    if (!ctx || !ctx->pos_resident.p) return set_error(MBPOL_ERR_INVALID_ARGUMENT, "no resident positions");
+    return mbpol_context_evaluate_device(ctx, ctx->pos_resident.as<double>(), nullptr, energies_out, components);
+}
+
+int mbpol_context_download_forces(mbpol_context* ctx, double* forces_out) {
+    HIP_TRY(hipSetDevice(ctx->device));
+    HIP_TRY(hipMemcpy(forces_out, ctx->force.p, sizeof(double) * 3 * ctx->sys.n, hipMemcpyDeviceToHost));
+    return MBPOL_OK;
+}
+
+int mbpol_context_set_timing(mbpol_context* ctx, int32_t enabled) {
+    timer_collect(ctx);
+    ctx->timer.enabled = enabled != 0;
+    for (int c = 0; c < T_NCAT; ++c) {
+        ctx->timer.ms[c] = 0;
+        ctx->timer.count[c] = 0;
+    }
+    return MBPOL_OK;
+}
+
+int mbpol_context_get_timing(mbpol_context* ctx, const char* name, double* total_ms, int64_t* count) {
+    static const char* names[T_NCAT] = {"evaluate", "onebody", "lists", "twobody", "threebody",
+                                        "dispersion", "elec_setup", "elec_iteration", "elec_energy"};
+    timer_collect(ctx);
+    for (int c = 0; c < T_NCAT; ++c)
+        if (std::strcmp(name, names[c]) == 0) {
+            *total_ms = ctx->timer.ms[c];
+            *count = ctx->timer.count[c];
+            return MBPOL_OK;
+        }
+    return set_error(MBPOL_ERR_INVALID_ARGUMENT, "unknown timer '%s'", name);
+}
+
+int mbpol_context_get_stats(const mbpol_context* ctx, mbpol_eval_stats* out) {
+    out->iterations = ctx->stats.iterations;
+    out->epsilon = ctx->stats.epsilon;
+    out->converged = ctx->stats.converged;
+    out->num_dimers = ctx->stats.dimers;
+    out->num_trimers = ctx->stats.trimers;
+    out->num_elec_pairs = ctx->stats.elec_pairs;
+    out->num_disp_pairs = ctx->stats.disp_pairs;
+    return MBPOL_OK;
+}
+
+int mbpol_context_get_dimer_list(const mbpol_context* ctx, int32_t* pairs, int64_t capacity, int64_t* count) {
+    *count = ctx->stats.dimers;
+    if (pairs && capacity > 0 && ctx->stats.dimers > 0) {
+        const long long m = std::min<long long>(capacity, ctx->stats.dimers);
+        HIP_TRY(hipMemcpy(pairs, ctx->dimers.p, sizeof(int2) * m, hipMemcpyDeviceToHost));
+    }
+    return MBPOL_OK;
+}
+
+int mbpol_context_get_trimer_list(const mbpol_context* ctx, int32_t* trip, int64_t capacity, int64_t* count) {
+    *count = ctx->stats.trimers;
+    if (trip && capacity > 0 && ctx->stats.trimers > 0) {
+        const long long m = std::min<long long>(capacity, ctx->stats.trimers);
+        HIP_TRY(hipMemcpy(trip, ctx->trips.p, sizeof(int) * 3 * m, hipMemcpyDeviceToHost));
+    }
+    return MBPOL_OK;
+}
+
+}  // extern "C"

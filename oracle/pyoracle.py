@@ -217,3 +217,86 @@ def fft3d(grid, sign):
     nx, ny, nz = g.shape
     lib().oracle_fft3d(g.ctypes.data_as(_dp), nx, ny, nz, sign)
     return g
+
+
+# ---------------------------------------------------------------------------
+# whole-System evaluation: the sum over the Forces of a System as the reference
+# Context would compute it (virtual sites placed, their forces folded back).
+# ---------------------------------------------------------------------------
+COMPONENTS = ('onebody', 'twobody', 'threebody', 'dispersion', 'electrostatics')
+
+
+def pme_parameters(cutoff, tol, box):
+    """NonbondedForceImpl::calcPMEParameters (OpenMM 6.3-7.x; called at
+    MBPolReferenceKernels.cpp:186-199): alpha = sqrt(-ln(2 tol))/rc and
+    n = max(ceil(2 alpha L / (3 tol^(1/5))), 6)."""
+    import math
+    alpha = (1.0 / cutoff) * math.sqrt(-math.log(2.0 * tol))
+    grid = [max(int(math.ceil(2 * alpha * L / (3 * tol ** 0.2))), 6) for L in box]
+    return alpha, grid
+
+
+def evaluate_system(system, positions, components=31, return_lists=False):
+    """Energies (dict, kJ/mol) and forces (N x 3, kJ/mol/nm) of a System built with
+    mbpol_openmm_plugin_amd.api (read through its Force getters only)."""
+    pos = system.computeVirtualSites(positions)
+    n = len(pos)
+    F = np.zeros((n, 3))
+    E = dict.fromkeys(COMPONENTS, 0.0)
+    box = np.array(system._box) if system._box is not None else None
+    lists = {}
+    for f in system._forces:
+        name = type(f).__name__
+        if name == 'MBPolOneBodyForce' and components & 1:
+            idx = np.array([f.getOneBodyParameters(i) for i in range(f.getNumOneBodys())])
+            e, ff = onebody(pos, idx, periodic=f.getNonbondedMethod() == 1, box=box)
+            E['onebody'] += e
+            F += ff
+        elif name in ('MBPolTwoBodyForce', 'MBPolThreeBodyForce'):
+            two = name == 'MBPolTwoBodyForce'
+            if not components & (2 if two else 4):
+                continue
+            idx = np.array([f.getParticleParameters(i) for i in range(f.getNumMolecules())])
+            periodic = f.getNonbondedMethod() == 1
+            opos = pos[idx[:, 0]]
+            rc = f.getCutoff()
+            if two:
+                pairs = dimer_list(opos, rc, periodic, box)
+                e, ff = twobody(pos, idx, pairs, periodic, box)
+                E['twobody'] += e
+                lists['dimers'] = pairs
+            else:
+                trip = trimer_list(opos, rc, periodic, box)
+                e, ff = threebody(pos, idx, trip, periodic, box)
+                E['threebody'] += e
+                lists['trimers'] = trip
+            F += ff
+        elif name == 'MBPolElectrostaticsForce' and components & 16:
+            q, mol, typ, damp, pol = f._arrays()
+            method = f.getNonbondedMethod()
+            alpha, grid = f.getAEwald(), f.getPmeGridDimensions()
+            if method == 1 and (grid[0] == 0 or alpha == 0.0):
+                alpha, grid = pme_parameters(f.getCutoffDistance(), f.getEwaldErrorTolerance(), box)
+            P = elec_params(method=method, cutoff=f.getCutoffDistance(), alpha=alpha, grid=grid,
+                            thole=f.getTholeParameters(), max_iter=f.getMutualInducedMaxIterations(),
+                            target_eps=f.getMutualInducedTargetEpsilon(),
+                            charge_redistribution=f.getIncludeChargeRedistribution(),
+                            box=box if box is not None else (0, 0, 0))
+            r = electrostatics(P, pos, q, mol, typ, damp, pol)
+            E['electrostatics'] += r['energy']
+            F += r['forces']
+            lists['elec'] = r
+        elif name == 'MBPolDispersionForce' and components & 8:
+            p = np.array(f._p).reshape(-1, 2)
+            e, ff = dispersion(pos, p[:, 0], p[:, 1], f._c6, f._d6, method=f.getNonbondedMethod(),
+                               cutoff=f.getCutoff(), box=box)
+            E['dispersion'] += e
+            F += ff
+    for s, v in system._vsites.items():
+        for p, w in zip(v.particles, v.weights):
+            F[p] += w * F[s]
+        F[s] = 0.0
+    E['total'] = sum(E[k] for k in COMPONENTS)
+    if return_lists:
+        return E, F, lists
+    return E, F

@@ -1,0 +1,154 @@
+"""HIP path (libmbpol_hip.so through its C ABI) against the FP64 oracle.
+
+Tolerances: the FP64 GPU path must agree with the oracle to 1e-9 relative
+energy and 1e-8 relative RMS force (only summation order differs); the
+mixed-precision path (FP32 polynomial / pair arithmetic, FP64 accumulation)
+to <= 1e-6 of the total energy per component and <= 1e-4 relative RMS force.
+Neighbour lists must be identical as sets."""
+import os
+
+import numpy as np
+import pytest
+
+import kat_data as K
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(__file__), 'golden')
+
+
+@pytest.fixture(scope='module')
+def mb():
+    import mbpol_openmm_plugin_amd as m
+    if m.device_count() < 1:
+        pytest.fail('no GPU visible to libmbpol_hip.so')
+    return m
+
+
+def rel_rms(a, b):
+    return float(np.sqrt(np.mean((a - b) ** 2)) / max(np.sqrt(np.mean(b ** 2)), 1e-30))
+
+
+TOL = {'double': (1e-9, 1e-8), 'mixed': (1e-6, 1e-4)}
+
+
+def compare(mb, oracle, sys, pos, precision, components=31, label=''):
+    ctx = mb.Context(sys, precision=precision)
+    ctx.setPositions(pos)
+    st = ctx.getState(components=components)
+    E, F = oracle.evaluate_system(sys, pos, components=components)
+    got = st.getComponentEnergies()
+    etol, ftol = TOL[precision]
+    scale = max(abs(E['total']), 1.0)
+    report = {k: (got[k], E[k]) for k in got}
+    print(label, precision, {k: '%.9g vs %.9g' % v for k, v in report.items()}, 'stats', st.stats, flush=True)
+    for k in got:
+        assert abs(got[k] - E[k]) <= etol * scale, (label, k, got[k], E[k])
+    assert abs(st.getPotentialEnergy() - E['total']) <= etol * scale
+    err = rel_rms(st.getForces(), F)
+    print(label, precision, 'force rel rms err %.3e' % err, flush=True)
+    assert err <= ftol, (label, err)
+    return ctx, st
+
+
+@pytest.mark.parametrize('precision', ['double', 'mixed'])
+def test_kat_waters_onebody_twobody_threebody(mb, oracle, precision):
+    waters = K.WATERS_NM
+    sys = mb.forcefield.create_system(3, nonbonded_method='CutoffNonPeriodic', twobody_cutoff=10.0,
+                                      threebody_cutoff=10.0, components=1 | 2 | 4)
+    pos = mb.forcefield.water_positions_with_m(waters)
+    ctx, st = compare(mb, oracle, sys, pos, precision, components=7, label='kat3')
+    e = st.getComponentEnergies()
+    assert abs(e['onebody'] / K.KCAL - 0.55975882 - 0) < 5  # three monomers: finite and close to the KAT scale
+
+
+@pytest.mark.parametrize('precision', ['double', 'mixed'])
+def test_config0_cluster_all_components(mb, oracle, precision):
+    waters = mb.systems.config0()
+    sys = mb.forcefield.create_system(14, nonbonded_method='CutoffNonPeriodic', twobody_cutoff=1e10,
+                                      threebody_cutoff=1e10, nonbonded_cutoff=1e3, dipole_epsilon=1e-8)
+    pos = mb.forcefield.water_positions_with_m(waters)
+    compare(mb, oracle, sys, pos, precision, label='config0')
+
+
+@pytest.mark.parametrize('precision', ['double', 'mixed'])
+def test_config1_pbc_all_components(mb, oracle, precision):
+    waters, L = mb.systems.config1()
+    sys = mb.forcefield.create_system(256, nonbonded_method='PME', box=L, ewald_error_tolerance=1e-5,
+                                      dipole_epsilon=1e-8)
+    pos = mb.forcefield.water_positions_with_m(waters)
+    compare(mb, oracle, sys, pos, precision, label='config1')
+
+
+def test_lists_bit_exact_config1(mb, oracle):
+    waters, L = mb.systems.config1()
+    sys = mb.forcefield.create_system(256, nonbonded_method='PME', box=L, components=2 | 4)
+    pos = mb.forcefield.water_positions_with_m(waters)
+    ctx = mb.Context(sys, precision='double')
+    ctx.setListMode(True)
+    ctx.setPositions(pos)
+    ctx.getState(components=2 | 4)
+    opos = pos[0::4]
+    want2 = oracle.dimer_list(opos, 0.65, True, [L] * 3)
+    want3 = oracle.trimer_list(opos, 0.45, True, [L] * 3)
+    got2, got3 = ctx.getDimerList(), ctx.getTrimerList()
+    print('dimers', len(got2), len(want2), 'trimers', len(got3), len(want3), flush=True)
+    assert sorted(map(tuple, got2.tolist())) == sorted(map(tuple, want2.tolist()))
+    assert sorted(map(tuple, got3.tolist())) == sorted(map(tuple, want3.tolist()))
+
+
+def test_lists_bit_exact_cluster(mb, oracle):
+    waters = mb.systems.cluster(40, seed=7)
+    sys = mb.forcefield.create_system(40, nonbonded_method='CutoffNonPeriodic', twobody_cutoff=0.65,
+                                      threebody_cutoff=0.45, components=2 | 4)
+    pos = mb.forcefield.water_positions_with_m(waters)
+    ctx = mb.Context(sys, precision='double')
+    ctx.setListMode(True)
+    ctx.setPositions(pos)
+    ctx.getState(components=2 | 4)
+    opos = pos[0::4]
+    assert sorted(map(tuple, ctx.getDimerList().tolist())) == sorted(
+        map(tuple, oracle.dimer_list(opos, 0.65).tolist()))
+    assert sorted(map(tuple, ctx.getTrimerList().tolist())) == sorted(
+        map(tuple, oracle.trimer_list(opos, 0.45).tolist()))
+
+
+@pytest.mark.parametrize('name,method,cutoff,box,comp', [('water3', 'CutoffNonPeriodic', 0.9, None, 31),
+                                                         ('water50', 'PME', 0.9, 1.8, 31),
+                                                         ('water_and_ion', 'CutoffNonPeriodic', 1.0, None, 8)])
+def test_reference_fixture_systems(mb, oracle, name, method, cutoff, box, comp):
+    pos, names, res, _ = mb.forcefield.read_pdb(os.path.join(GOLDEN, 'pdb', name + '.pdb'))
+    nw = sum(1 for r in res if r == 'HOH') // 4
+    sys = mb.forcefield.create_system(nw, nonbonded_method=method, nonbonded_cutoff=cutoff, box=box,
+                                      ions=len(res) - 4 * nw, components=comp)
+    compare(mb, oracle, sys, pos, 'double', components=comp, label=name)
+
+
+def test_config2_properties(mb):
+    """4096 waters at full size: deterministic, translation invariant, forces sum to ~0,
+    mixed and FP64 paths agree."""
+    waters, L = mb.systems.config2()
+    sys = mb.forcefield.create_system(4096, nonbonded_method='PME', box=L, ewald_error_tolerance=1e-5,
+                                      dipole_epsilon=1e-8, ewald_alpha=3.654826, pme_grid=(121, 121, 121))
+    pos = mb.forcefield.water_positions_with_m(waters)
+    out = {}
+    for prec in ('double', 'mixed'):
+        ctx = mb.Context(sys, precision=prec)
+        ctx.setPositions(pos)
+        a = ctx.getState()
+        b = ctx.getState()
+        assert a.getPotentialEnergy() == b.getPotentialEnergy()
+        assert np.array_equal(a.getForces(), b.getForces())
+        ctx.setPositions(pos + np.array([0.37, -0.21, 1.3]))
+        c = ctx.getState()
+        assert abs(c.getPotentialEnergy() - a.getPotentialEnergy()) < 1e-6 * abs(a.getPotentialEnergy())
+        f = a.getForces()
+        assert np.all(np.isfinite(f))
+        assert np.abs(f.sum(axis=0)).max() < 1e-3 * np.abs(f).max() * np.sqrt(len(f))
+        out[prec] = a
+        print(prec, a.getComponentEnergies(), a.stats, flush=True)
+    ed, em = out['double'].getComponentEnergies(), out['mixed'].getComponentEnergies()
+    tot = abs(out['double'].getPotentialEnergy())
+    for k in ed:
+        assert abs(ed[k] - em[k]) <= 1e-6 * tot, k
+    assert rel_rms(out['mixed'].getForces(), out['double'].getForces()) <= 1e-4
