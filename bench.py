@@ -30,7 +30,7 @@ def parse():
     ap.add_argument('--gpus', type=int, default=1)
     ap.add_argument('--steps', type=int, default=20)
     ap.add_argument('--warmup', type=int, default=3)
-    ap.add_argument('--precision', default='mixed', choices=['mixed', 'double'])
+    ap.add_argument('--precision', default='mixed', choices=['mixed', 'double', 'single-pip'])
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--cpu-sample-seconds', type=float, default=20.0)
     return ap.parse_args()

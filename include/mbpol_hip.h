@@ -63,8 +63,11 @@ extern "C" {
 #define MBPOL_ALL 31u
 
 /* arithmetic of the PIP and pair kernels */
-#define MBPOL_PRECISION_MIXED 0         /* FP32 polynomial/pair math, FP64 accumulation, FP64 Ewald/residual */
+#define MBPOL_PRECISION_MIXED 0         /* FP64 polynomials and electrostatics, FP32 dispersion pair math,
+                                           FP64 accumulation */
 #define MBPOL_PRECISION_DOUBLE 1        /* FP64 everywhere */
+#define MBPOL_PRECISION_SINGLE_PIP 2    /* MIXED with FP32 2-/3-body polynomials: faster, but the 2-/3-body
+                                           energies carry relative errors of order 1e-3 */
 
 /* initial guess of the induced-dipole iteration */
 #define MBPOL_GUESS_ALPHA_E 0           /* mu0 = alpha * E_fixed, as the reference (:628-639) */

@@ -438,14 +438,24 @@ def _build_native_system(system):
     return h
 
 
+PRECISIONS = {'mixed': 0, 'double': 1, 'single-pip': 2}
+
+
 class Context:
-    """A HIP context for one System on one GPU (precision 'mixed' or 'double')."""
+    """A HIP context for one System on one GPU.
+
+    precision: 'mixed' (FP64 polynomials and electrostatics, FP32 dispersion
+    pair arithmetic, FP64 accumulation), 'double' (FP64 throughout) or
+    'single-pip' (mixed with FP32 2-/3-body polynomials; their energies then
+    carry relative errors of order 1e-3)."""
 
     def __init__(self, system, platform='HIP', precision='mixed', device=0):
         if platform != 'HIP':
             raise MBPolError(1, 'this package provides the HIP platform only')
         self.system = system
-        self._precision = {'mixed': 0, 'double': 1}[precision]
+        if precision not in PRECISIONS:
+            raise MBPolError(1, 'unknown precision %r (one of %s)' % (precision, ', '.join(PRECISIONS)))
+        self._precision = PRECISIONS[precision]
         self._positions = None
         L = library()
         sysh = _build_native_system(system)

@@ -26,7 +26,7 @@ using namespace mbpol_params;
 __device__ double gammq34(double x) {
     const double a = 0.75;
     const double eps = 2.220446049250313e-16, fpmin = 2.2250738585072014e-308 / 2.220446049250313e-16;
-    const double lg = -0.20328095143129538;  // ln Gamma(3/4)
+    const double lg = 0.20328095143129538;  // ln Gamma(3/4) (Gamma(3/4) = 1.2254...)
     if (x == 0.0) return 1.0;
     if (x > 745.0) return 0.0;  // exp(-x) underflows: the result is 0 to double precision
     if (x < a + 1.0) {

@@ -101,95 +101,115 @@ __device__ __forceinline__ void extra_grads(const vr& O, const vr& H1, const vr&
     fH2 += gh2;
 }
 
+// Geometry of dimer t: atom indices, imaged positions (Angstrom) and, when the
+// O-O distance is inside [2, r2f] Angstrom, the ten points relative to Oa.
+// Returns whether the dimer contributes.
+__device__ __forceinline__ bool dimer_geometry(long long t, const int2* __restrict__ pairs,
+                                               const int* __restrict__ idx, const double* __restrict__ pos,
+                                               int periodic, const d3& boxA, int (&at)[6], vr (&R)[10],
+                                               d3& dOO, double& rOO) {
+    const int2 pr = pairs[t];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        at[q] = idx[3 * pr.x + q];
+        at[3 + q] = idx[3 * pr.y + q];
+    }
+    d3 P[6];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+        const d3 p = load_d3(pos, at[q]);
+        P[q] = d3{p.x * 10., p.y * 10., p.z * 10.};
+    }
+    if (periodic) {  // imageMolecules (MBPolReferenceTwoBodyForce.cpp:78-100)
+        P[1] = image_to(P[0], P[1], boxA);
+        P[2] = image_to(P[0], P[2], boxA);
+        P[3] = image_to(P[0], P[3], boxA);
+        P[4] = image_to(P[3], P[4], boxA);
+        P[5] = image_to(P[3], P[5], boxA);
+    }
+    dOO = d3{P[0].x - P[3].x, P[0].y - P[3].y, P[0].z - P[3].z};
+    rOO = sqrt(dOO.x * dOO.x + dOO.y * dOO.y + dOO.z * dOO.z);
+    if (!(rOO <= tb_r2f && rOO >= 2.)) return false;
+#pragma unroll
+    for (int q = 0; q < 6; ++q) R[q] = rel(P[q], P[0]);
+    extra_points(R[0], R[1], R[2], R[6], R[7]);
+    extra_points(R[3], R[4], R[5], R[8], R[9]);
+    return true;
+}
+
 __global__ void __launch_bounds__(64) k_twobody(long long np, const int2* __restrict__ pairs, const int* __restrict__ idx,
                                                 const double* __restrict__ pos, int periodic, d3 boxA,
                                                 u64* __restrict__ force, u64* __restrict__ energy) {
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     double e_out = 0.0;
-    if (t < np) {
-        const int2 pr = pairs[t];
-        int at[6];
+    int at[6];
+    vr R[10];
+    d3 dOO;
+    double rOO;
+    if (t < np && dimer_geometry(t, pairs, idx, pos, periodic, boxA, at, R, dOO, rOO)) {
+        real x[31], g[31];
 #pragma unroll
-        for (int q = 0; q < 3; ++q) {
-            at[q] = idx[3 * pr.x + q];
-            at[3 + q] = idx[3 * pr.y + q];
+        for (int n = 0; n < 31; ++n) {
+            const Var2 v = var2(n);
+            const vr d = R[v.a] - R[v.b];
+            const real r = sqrt(d.dot(d));
+            const real e = exp_t<real>((real)v.k * ((real)v.r0 - r));
+            x[n] = v.coul ? e / r : e;
         }
-        d3 P[6];
+        real Ep = pip2b_eval<real>(x, g);
+        // Only t, Ep and g stay live across the polynomial: the geometry is
+        // reloaded from memory (the opaque copy of t keeps the compiler from
+        // holding the first loads in registers through the polynomial).
+        long long tt = t;
+        asm volatile("" : "+v"(tt));
+        asm volatile("" : "+v"(Ep));
+#pragma unroll
+        for (int n = 0; n < 31; ++n) asm volatile("" : "+v"(g[n]));
+        dimer_geometry(tt, pairs, idx, pos, periodic, boxA, at, R, dOO, rOO);
+        // chain rule through the variables
+        vr F[10];
+#pragma unroll
+        for (int q = 0; q < 10; ++q) F[q] = vr(0, 0, 0);
+#pragma unroll
+        for (int n = 0; n < 31; ++n) {
+            const Var2 v = var2(n);
+            const vr d = R[v.a] - R[v.b];
+            const real r = sqrt(d.dot(d));
+            const real e = exp_t<real>((real)v.k * ((real)v.r0 - r));
+            real gg;
+            if (v.coul) {
+                const real ri = (real)1 / r, val = e * ri;
+                gg = -((real)v.k + ri) * val * ri;
+            } else {
+                gg = -(real)v.k * e / r;
+            }
+            const vr dd = d * (gg * g[n]);
+            F[v.a] += dd;
+            F[v.b] -= dd;
+        }
+        extra_grads(R[0], R[1], R[2], F[6], F[7], F[0], F[1], F[2]);
+        extra_grads(R[3], R[4], R[5], F[8], F[9], F[3], F[4], F[5]);
+        // switch (mbpol_2body_constants.cpp:98-111), FP64
+        double sw, gsw;
+        if (rOO > tb_r2i) {
+            const double t1 = M_PI / (tb_r2f - tb_r2i);
+            const double xs = (rOO - tb_r2i) * t1;
+            gsw = -sin(xs) * t1 / 2.0;
+            sw = (1.0 + cos(xs)) / 2.0;
+        } else {
+            sw = 1.0;
+            gsw = 0.0;
+        }
+        const double s = sw * cal2joule * -10.;
+        const double gs = gsw * (double)Ep / rOO * cal2joule * -10.;
 #pragma unroll
         for (int q = 0; q < 6; ++q) {
-            const d3 p = load_d3(pos, at[q]);
-            P[q] = d3{p.x * 10., p.y * 10., p.z * 10.};
+            double fx = (double)F[q].x * s, fy = (double)F[q].y * s, fz = (double)F[q].z * s;
+            if (q == 0) { fx += gs * dOO.x; fy += gs * dOO.y; fz += gs * dOO.z; }
+            if (q == 3) { fx -= gs * dOO.x; fy -= gs * dOO.y; fz -= gs * dOO.z; }
+            add_force(force, at[q], fx, fy, fz);
         }
-        if (periodic) {  // imageMolecules (MBPolReferenceTwoBodyForce.cpp:78-100)
-            P[1] = image_to(P[0], P[1], boxA);
-            P[2] = image_to(P[0], P[2], boxA);
-            P[3] = image_to(P[0], P[3], boxA);
-            P[4] = image_to(P[3], P[4], boxA);
-            P[5] = image_to(P[3], P[5], boxA);
-        }
-        const double dx = P[0].x - P[3].x, dy = P[0].y - P[3].y, dz = P[0].z - P[3].z;
-        const double rOO = sqrt(dx * dx + dy * dy + dz * dz);
-        if (rOO <= tb_r2f && rOO >= 2.) {
-            vr R[10];
-#pragma unroll
-            for (int q = 0; q < 6; ++q) R[q] = rel(P[q], P[0]);
-            extra_points(R[0], R[1], R[2], R[6], R[7]);
-            extra_points(R[3], R[4], R[5], R[8], R[9]);
-            real x[31], g[31];
-#pragma unroll
-            for (int n = 0; n < 31; ++n) {
-                const Var2 v = var2(n);
-                const vr d = R[v.a] - R[v.b];
-                const real r = sqrt(d.dot(d));
-                const real e = exp_t<real>((real)v.k * ((real)v.r0 - r));
-                x[n] = v.coul ? e / r : e;
-            }
-            const real Ep = pip2b_eval<real>(x, g);
-            // chain rule through the variables (recomputed rather than kept live)
-            vr F[10];
-#pragma unroll
-            for (int q = 0; q < 10; ++q) F[q] = vr(0, 0, 0);
-#pragma unroll
-            for (int n = 0; n < 31; ++n) {
-                const Var2 v = var2(n);
-                const vr d = R[v.a] - R[v.b];
-                const real r = sqrt(d.dot(d));
-                const real e = exp_t<real>((real)v.k * ((real)v.r0 - r));
-                real gg;
-                if (v.coul) {
-                    const real ri = (real)1 / r, val = e * ri;
-                    gg = -((real)v.k + ri) * val * ri;
-                } else {
-                    gg = -(real)v.k * e / r;
-                }
-                const vr dd = d * (gg * g[n]);
-                F[v.a] += dd;
-                F[v.b] -= dd;
-            }
-            extra_grads(R[0], R[1], R[2], F[6], F[7], F[0], F[1], F[2]);
-            extra_grads(R[3], R[4], R[5], F[8], F[9], F[3], F[4], F[5]);
-            // switch (mbpol_2body_constants.cpp:98-111), FP64
-            double sw, gsw;
-            if (rOO > tb_r2i) {
-                const double t1 = M_PI / (tb_r2f - tb_r2i);
-                const double xs = (rOO - tb_r2i) * t1;
-                gsw = -sin(xs) * t1 / 2.0;
-                sw = (1.0 + cos(xs)) / 2.0;
-            } else {
-                sw = 1.0;
-                gsw = 0.0;
-            }
-            const double s = sw * cal2joule * -10.;
-            const double gs = gsw * (double)Ep / rOO * cal2joule * -10.;
-#pragma unroll
-            for (int q = 0; q < 6; ++q) {
-                double fx = (double)F[q].x * s, fy = (double)F[q].y * s, fz = (double)F[q].z * s;
-                if (q == 0) { fx += gs * dx; fy += gs * dy; fz += gs * dz; }
-                if (q == 3) { fx -= gs * dx; fy -= gs * dy; fz -= gs * dz; }
-                add_force(force, at[q], fx, fy, fz);
-            }
-            e_out = sw * (double)Ep * cal2joule;
-        }
+        e_out = sw * (double)Ep * cal2joule;
     }
     add_energy(energy, e_out);
 }
@@ -258,98 +278,116 @@ __device__ __forceinline__ double switch3(double r, double& g) {  // (:103-120)
     return 1.0;
 }
 
+// Geometry of trimer t: atom indices, imaged positions, the three O-O vectors
+// (Angstrom) and, unless an O-O distance is below 2 Angstrom (:160-163), the
+// nine points in Angstrom relative to Oa.  Returns whether the trimer contributes.
+__device__ __forceinline__ bool trimer_geometry(long long t, const int* __restrict__ trip, const int* __restrict__ idx,
+                                                const double* __restrict__ pos, int periodic, const d3& box,
+                                                int (&at)[9], vr (&R)[9], d3 (&roo)[3], double (&doo)[3]) {
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+        const int mol = trip[3 * t + m];
+#pragma unroll
+        for (int q = 0; q < 3; ++q) at[3 * m + q] = idx[3 * mol + q];
+    }
+    d3 P[9];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) P[q] = load_d3(pos, at[q]);
+    if (periodic) {  // imageMolecules in nm (MBPolReferenceTwoBodyForce.cpp:78-109)
+        P[1] = image_to(P[0], P[1], box);
+        P[2] = image_to(P[0], P[2], box);
+        P[3] = image_to(P[0], P[3], box);
+        P[4] = image_to(P[3], P[4], box);
+        P[5] = image_to(P[3], P[5], box);
+        P[6] = image_to(P[0], P[6], box);
+        P[7] = image_to(P[6], P[7], box);
+        P[8] = image_to(P[6], P[8], box);
+    }
+    roo[0] = d3{(P[0].x - P[3].x) * 10., (P[0].y - P[3].y) * 10., (P[0].z - P[3].z) * 10.};  // ab
+    roo[1] = d3{(P[0].x - P[6].x) * 10., (P[0].y - P[6].y) * 10., (P[0].z - P[6].z) * 10.};  // ac
+    roo[2] = d3{(P[3].x - P[6].x) * 10., (P[3].y - P[6].y) * 10., (P[3].z - P[6].z) * 10.};  // bc
+#pragma unroll
+    for (int q = 0; q < 3; ++q) doo[q] = sqrt(roo[q].x * roo[q].x + roo[q].y * roo[q].y + roo[q].z * roo[q].z);
+    if (doo[0] < 2 || doo[1] < 2 || doo[2] < 2) return false;
+    const d3 o{P[0].x * 10., P[0].y * 10., P[0].z * 10.};
+#pragma unroll
+    for (int q = 0; q < 9; ++q) {
+        const d3 p{P[q].x * 10., P[q].y * 10., P[q].z * 10.};
+        R[q] = rel(p, o);
+    }
+    return true;
+}
+
 __global__ void __launch_bounds__(64) k_threebody(long long nt, const int* __restrict__ trip, const int* __restrict__ idx,
                                                   const double* __restrict__ pos, int periodic, d3 box,
                                                   u64* __restrict__ force, u64* __restrict__ energy) {
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     double e_out = 0.0;
-    if (t < nt) {
-        int at[9];
+    int at[9];
+    vr R[9];
+    d3 roo[3];
+    double doo[3];
+    if (t < nt && trimer_geometry(t, trip, idx, pos, periodic, box, at, R, roo, doo)) {
+        real x[36], g[36];
 #pragma unroll
-        for (int m = 0; m < 3; ++m) {
-            const int mol = trip[3 * t + m];
-#pragma unroll
-            for (int q = 0; q < 3; ++q) at[3 * m + q] = idx[3 * mol + q];
+        for (int n = 0; n < 36; ++n) {
+            const Var3 v = var3(n);
+            const vr d = R[v.a] - R[v.b];
+            const real r = sqrt(d.dot(d));
+            x[n] = exp_t<real>(-(real)v.k * (r - (real)v.r0));
         }
-        d3 P[9];
+        real Ep = pip3b_eval<real>(x, g);
+        // Only t, Ep and g stay live across the polynomial; the geometry is
+        // reloaded (through an opaque copy of t, so that the first loads are
+        // not kept in registers).
+        long long tt = t;
+        asm volatile("" : "+v"(tt));
+        asm volatile("" : "+v"(Ep));
 #pragma unroll
-        for (int q = 0; q < 9; ++q) P[q] = load_d3(pos, at[q]);
-        if (periodic) {  // imageMolecules in nm (MBPolReferenceTwoBodyForce.cpp:78-109)
-            P[1] = image_to(P[0], P[1], box);
-            P[2] = image_to(P[0], P[2], box);
-            P[3] = image_to(P[0], P[3], box);
-            P[4] = image_to(P[3], P[4], box);
-            P[5] = image_to(P[3], P[5], box);
-            P[6] = image_to(P[0], P[6], box);
-            P[7] = image_to(P[6], P[7], box);
-            P[8] = image_to(P[6], P[8], box);
+        for (int n = 0; n < 36; ++n) asm volatile("" : "+v"(g[n]));
+        trimer_geometry(tt, trip, idx, pos, periodic, box, at, R, roo, doo);
+        double gab, gac, gbc;
+        const double sab = switch3(doo[0], gab), sac = switch3(doo[1], gac), sbc = switch3(doo[2], gbc);
+        const double s = sab * sac + sab * sbc + sac * sbc;
+        // g_var (:80-101): forces of the variables, times s; units kJ/mol/nm
+        vr F[9];
+#pragma unroll
+        for (int q = 0; q < 9; ++q) F[q] = vr(0, 0, 0);
+#pragma unroll
+        for (int n = 0; n < 36; ++n) {
+            const Var3 v = var3(n);
+            const vr d = R[v.a] - R[v.b];
+            const real r = sqrt(d.dot(d));
+            const real gg = -(real)v.k * g[n] * exp_t<real>(-(real)v.k * (r - (real)v.r0)) / r;
+            const vr dd = d * gg;
+            F[v.a] += dd;
+            F[v.b] -= dd;
         }
-        const d3 rab{(P[0].x - P[3].x) * 10., (P[0].y - P[3].y) * 10., (P[0].z - P[3].z) * 10.};
-        const d3 rac{(P[0].x - P[6].x) * 10., (P[0].y - P[6].y) * 10., (P[0].z - P[6].z) * 10.};
-        const d3 rbc{(P[3].x - P[6].x) * 10., (P[3].y - P[6].y) * 10., (P[3].z - P[6].z) * 10.};
-        const double drab = sqrt(rab.x * rab.x + rab.y * rab.y + rab.z * rab.z);
-        const double drac = sqrt(rac.x * rac.x + rac.y * rac.y + rac.z * rac.z);
-        const double drbc = sqrt(rbc.x * rbc.x + rbc.y * rbc.y + rbc.z * rbc.z);
-        if (!(drab < 2 || drac < 2 || drbc < 2)) {
-            vr R[9];  // Angstrom, relative to Oa
+        const double fs = s * cal2joule * 10. * -1;
+        const double E = (double)Ep;
+        gab *= (sac + sbc) * E / doo[0];
+        gac *= (sab + sbc) * E / doo[1];
+        gbc *= (sab + sac) * E / doo[2];
+        const double u = cal2joule * -nm_to_A;
 #pragma unroll
-            for (int q = 0; q < 9; ++q) {
-                const d3 p{P[q].x * 10., P[q].y * 10., P[q].z * 10.};
-                const d3 o{P[0].x * 10., P[0].y * 10., P[0].z * 10.};
-                R[q] = rel(p, o);
+        for (int q = 0; q < 9; ++q) {
+            double fx = (double)F[q].x * fs, fy = (double)F[q].y * fs, fz = (double)F[q].z * fs;
+            if (q == 0) {
+                fx += (gab * roo[0].x + gac * roo[1].x) * u;
+                fy += (gab * roo[0].y + gac * roo[1].y) * u;
+                fz += (gab * roo[0].z + gac * roo[1].z) * u;
+            } else if (q == 3) {
+                fx += (gbc * roo[2].x - gab * roo[0].x) * u;
+                fy += (gbc * roo[2].y - gab * roo[0].y) * u;
+                fz += (gbc * roo[2].z - gab * roo[0].z) * u;
+            } else if (q == 6) {
+                fx -= (gac * roo[1].x + gbc * roo[2].x) * u;
+                fy -= (gac * roo[1].y + gbc * roo[2].y) * u;
+                fz -= (gac * roo[1].z + gbc * roo[2].z) * u;
             }
-            real x[36], g[36];
-#pragma unroll
-            for (int n = 0; n < 36; ++n) {
-                const Var3 v = var3(n);
-                const vr d = R[v.a] - R[v.b];
-                const real r = sqrt(d.dot(d));
-                x[n] = exp_t<real>(-(real)v.k * (r - (real)v.r0));
-            }
-            const real Ep = pip3b_eval<real>(x, g);
-            double gab, gac, gbc;
-            const double sab = switch3(drab, gab), sac = switch3(drac, gac), sbc = switch3(drbc, gbc);
-            const double s = sab * sac + sab * sbc + sac * sbc;
-            // g_var (:80-101): forces of the variables, times s; units kJ/mol/nm
-            vr F[9];
-#pragma unroll
-            for (int q = 0; q < 9; ++q) F[q] = vr(0, 0, 0);
-#pragma unroll
-            for (int n = 0; n < 36; ++n) {
-                const Var3 v = var3(n);
-                const vr d = R[v.a] - R[v.b];
-                const real r = sqrt(d.dot(d));
-                const real gg = -(real)v.k * g[n] * exp_t<real>(-(real)v.k * (r - (real)v.r0)) / r;
-                const vr dd = d * gg;
-                F[v.a] += dd;
-                F[v.b] -= dd;
-            }
-            const double fs = s * cal2joule * 10. * -1;
-            const double E = (double)Ep;
-            gab *= (sac + sbc) * E / drab;
-            gac *= (sab + sbc) * E / drac;
-            gbc *= (sab + sac) * E / drbc;
-            const double u = cal2joule * -nm_to_A;
-#pragma unroll
-            for (int q = 0; q < 9; ++q) {
-                double fx = (double)F[q].x * fs, fy = (double)F[q].y * fs, fz = (double)F[q].z * fs;
-                if (q == 0) {
-                    fx += (gab * rab.x + gac * rac.x) * u;
-                    fy += (gab * rab.y + gac * rac.y) * u;
-                    fz += (gab * rab.z + gac * rac.z) * u;
-                } else if (q == 3) {
-                    fx += (gbc * rbc.x - gab * rab.x) * u;
-                    fy += (gbc * rbc.y - gab * rab.y) * u;
-                    fz += (gbc * rbc.z - gab * rab.z) * u;
-                } else if (q == 6) {
-                    fx -= (gac * rac.x + gbc * rbc.x) * u;
-                    fy -= (gac * rac.y + gbc * rbc.y) * u;
-                    fz -= (gac * rac.z + gbc * rbc.z) * u;
-                }
-                add_force(force, at[q], fx, fy, fz);
-            }
-            e_out = E * s * cal2joule;
+            add_force(force, at[q], fx, fy, fz);
         }
+        e_out = E * s * cal2joule;
     }
     add_energy(energy, e_out);
 }

@@ -331,11 +331,14 @@ extern "C" {
 
 int mbpol_context_create(const mbpol_system* sys, int32_t precision, int32_t device, mbpol_context** out) {
     if (!sys || !out) return set_error(MBPOL_ERR_INVALID_ARGUMENT, "null argument");
-    if (precision != MBPOL_PRECISION_MIXED && precision != MBPOL_PRECISION_DOUBLE)
+    if (precision != MBPOL_PRECISION_MIXED && precision != MBPOL_PRECISION_DOUBLE &&
+        precision != MBPOL_PRECISION_SINGLE_PIP)
         return set_error(MBPOL_ERR_INVALID_ARGUMENT, "unknown precision mode %d", precision);
     int ndev = 0;
-    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
-        return set_error(MBPOL_ERR_NO_DEVICE, "no HIP device available");
+    const hipError_t derr = hipGetDeviceCount(&ndev);
+    if (derr != hipSuccess || ndev <= 0)
+        return set_error(MBPOL_ERR_NO_DEVICE, "no HIP device available (%s)",
+                         derr != hipSuccess ? hipGetErrorString(derr) : "0 devices");
     if (device < 0 || device >= ndev) return set_error(MBPOL_ERR_NO_DEVICE, "device %d not present", device);
     // cutoff <= half box at initialisation (MBPolTwoBodyForceImpl.cpp:60-66 and siblings)
     if (periodic_box(*sys)) {
@@ -523,7 +526,10 @@ static int evaluate(mbpol_context* ctx, double* forces_dev_out, double* energies
         if (int e = build_lists(ctx, do2, do3)) return e;
         timer_stop(ctx, T_LISTS);
     }
-    const bool dp = ctx->precision == MBPOL_PRECISION_DOUBLE;
+    // The polynomials run in FP64 unless FP32 was asked for explicitly: their
+    // terms cancel to ~1e-3 of their magnitudes, so FP32 evaluation loses three
+    // digits of the 2- and 3-body energies.
+    const bool dp = ctx->precision != MBPOL_PRECISION_SINGLE_PIP;
     if (do2) {
         const int per = s.tb_method == MBPOL_CUTOFF_PERIODIC;
         timer_start(ctx, T_TWOBODY);
