@@ -28,25 +28,30 @@ namespace mbpol {
 
 struct Elec {
     ElecParams P{};
-    int n = 0, nw = 0;
+    int n = 0, nw = 0, npol = 0;
     int guess_mode = MBPOL_GUESS_ALPHA_E;
     bool have_previous = false;
-    // parameters
-    DevBuf q0, q, damp, pol, mol, type, dq;
-    // neighbour rows
+    // parameters; polidx lists the sites of non-zero polarisability
+    DevBuf q0, q, damp, pol, mol, type, dq, polidx;
+    std::vector<unsigned char> h_pol;  // host copy of (polarisability != 0)
+    // cells for the neighbour rows
     DevBuf cell_of, cell_count, cell_start, cell_fill, members;
     int3 nc{1, 1, 1};
-    int use_cells = 0;
-    DevBuf nb_count, nb_start, nb, pf;
-    long long nb_cap = 0;
-    // fields and dipoles
-    DevBuf efix, efix_pol, field, mu, mu_prev, mu_user, partial, eps;
+    int reach = 1, use_cells = 0;
+    // neighbour rows (ELL) and induced-field prefactors
+    DevBuf nbP, cntP, nbQ, cntQ, pf, maxcnt;
+    int capP = 0, capQ = 0;
+    // fields and dipoles: the iteration alternates between mu_a and mu_b
+    DevBuf efix, efix_pol, mu_a, mu_b, mu_prev, mu_user, partial, eps;
+    double* mu_cur = nullptr;  // dipoles of the last evaluate
     DevBuf phi_fix, phi_ind, fsite, V, eblk;
     // PME
     DevBuf grid, spec, influence, bx, by, bz;
+    BrickGeom B{};
+    DevBuf bin_of, bin_count, bin_start, bin_fill, bin_members;
     hipfftHandle plan_f = 0, plan_b = 0;
     bool plans = false;
-    double* h_eps = nullptr;
+    double* h_eps = nullptr;  // pinned: [0] epsilon, ints from [8] on
 };
 
 // B-spline moduli with the reference's small-modulus fix-up and order-5 zeta
@@ -116,6 +121,39 @@ static int up(DevBuf& b, const std::vector<T>& v) {
     return MBPOL_OK;
 }
 
+// Brick size along one grid axis for the spreading kernel: every brick at least
+// 4 points wide and none above 12 (the last one takes the remainder).
+static void brick_axis(int K, int& s, int& nb) {
+    if (K <= 12) {
+        s = K;
+        nb = 1;
+        return;
+    }
+    for (int c : {9, 10, 8, 11, 12, 7}) {
+        const int r = K % c;
+        if (r == 0 || r >= 4) {
+            s = c;
+            nb = (K + c - 1) / c;
+            return;
+        }
+    }
+    s = 6;  // remainder merged into the last brick: 6..11 points
+    nb = K / 6;
+}
+
+// polarisable-site index list and its host mirror
+static int set_polarisable(Elec* E, const std::vector<double>& pol) {
+    std::vector<int> idx;
+    E->h_pol.assign(pol.size(), 0);
+    for (size_t i = 0; i < pol.size(); ++i)
+        if (pol[i] != 0.0) {
+            idx.push_back((int)i);
+            E->h_pol[i] = 1;
+        }
+    E->npol = (int)idx.size();
+    return up(E->polidx, idx);
+}
+
 int elec_set_box(mbpol_context* ctx) {
     Elec* E = ctx->elec;
     const mbpol_system& s = ctx->sys;
@@ -124,14 +162,34 @@ int elec_set_box(mbpol_context* ctx) {
     P.inv_box = d3{s.box[0] > 0 ? 1.0 / s.box[0] : 0.0, s.box[1] > 0 ? 1.0 / s.box[1] : 0.0,
                    s.box[2] > 0 ? 1.0 / s.box[2] : 0.0};
     if (!P.pme) return MBPOL_OK;
-    int nc[3];
-    for (int d = 0; d < 3; ++d) nc[d] = (int)std::floor(s.box[d] / P.cutoff);
-    E->use_cells = (nc[0] >= 3 && nc[1] >= 3 && nc[2] >= 3) ? 1 : 0;
-    E->nc = E->use_cells ? make_int3(nc[0], nc[1], nc[2]) : make_int3(1, 1, 1);
+    // cells of at least rc / reach, searched +-reach cells: half-cutoff cells
+    // when the box holds five of them per axis, else cutoff cells, else none
+    E->use_cells = 0;
+    E->nc = make_int3(1, 1, 1);
+    for (int reach = 2; reach >= 1 && !E->use_cells; --reach) {
+        int nc[3];
+        bool ok = true;
+        for (int d = 0; d < 3; ++d) {
+            nc[d] = (int)std::floor(s.box[d] / (P.cutoff / reach));
+            ok = ok && nc[d] >= 2 * reach + 1;
+        }
+        if (ok) {
+            E->use_cells = 1;
+            E->reach = reach;
+            E->nc = make_int3(nc[0], nc[1], nc[2]);
+        }
+    }
     const int ncell = E->nc.x * E->nc.y * E->nc.z;
     HIP_TRY(E->cell_count.ensure(sizeof(int) * (ncell + 1)));
     HIP_TRY(E->cell_start.ensure(sizeof(int) * (ncell + 1)));
     HIP_TRY(E->cell_fill.ensure(sizeof(int) * (ncell + 1)));
+    // row capacities from the mean site density (grown on overflow)
+    const double vol = s.box[0] * s.box[1] * s.box[2];
+    const double mean = E->n / vol * (4.0 / 3.0) * M_PI * P.cutoff * P.cutoff * P.cutoff;
+    auto cap = [](double v) { return std::max(32, ((int)std::ceil(v) + 31) / 32 * 32); };
+    const double fp = E->n > 0 ? (double)E->npol / E->n : 0.0;
+    E->capP = std::max(E->capP, cap(1.25 * mean * fp + 48));
+    E->capQ = std::max(E->capQ, cap(1.25 * mean * (1.0 - fp) + 48));
     launch_influence(ctx->stream, P, E->bx.as<double>(), E->by.as<double>(), E->bz.as<double>(), E->influence.as<double>());
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     return MBPOL_OK;
@@ -158,6 +216,9 @@ int elec_create(mbpol_context* ctx) {
         for (int d = 0; d < 3; ++d)
             grid[d] = std::max((int)std::ceil(2 * P.alpha * s.box[d] / (3 * std::pow(tol, 0.2))), 6);
     }
+    if (P.pme && (grid[0] < 5 || grid[1] < 5 || grid[2] < 5))
+        return set_error(MBPOL_ERR_INVALID_ARGUMENT, "PME grid %d x %d x %d is smaller than the spline order 5",
+                         grid[0], grid[1], grid[2]);
     P.grid = make_int3(grid[0], grid[1], grid[2]);
     const int n = E->n;
     if (int e = up(E->q0, s.el_q)) return e;
@@ -165,24 +226,35 @@ int elec_create(mbpol_context* ctx) {
     if (int e = up(E->pol, s.el_pol)) return e;
     if (int e = up(E->mol, s.el_mol)) return e;
     if (int e = up(E->type, s.el_type)) return e;
+    if (int e = set_polarisable(E, s.el_pol)) return e;
     HIP_TRY(E->q.ensure(sizeof(double) * n));
     HIP_TRY(E->dq.ensure(sizeof(double) * 27 * std::max(E->nw, 1)));
-    for (DevBuf* b : {&E->efix, &E->efix_pol, &E->field, &E->mu, &E->mu_prev, &E->mu_user, &E->fsite})
-        HIP_TRY(b->ensure(sizeof(double) * 3 * n));
-    HIP_TRY(E->V.ensure(sizeof(double) * n));
-    HIP_TRY(E->phi_fix.ensure(sizeof(double) * 10 * n));
-    HIP_TRY(E->phi_ind.ensure(sizeof(double) * 10 * n));
-    HIP_TRY(E->partial.ensure(sizeof(double) * (sor_blocks(n) + 1)));
+    for (DevBuf* b : {&E->efix, &E->efix_pol, &E->mu_a, &E->mu_b, &E->mu_prev, &E->mu_user, &E->fsite})
+        HIP_TRY(b->ensure(sizeof(double) * 3 * std::max(n, 1)));
+    // sites the iteration never visits keep zero dipoles in both buffers
+    HIP_TRY(hipMemset(E->mu_a.p, 0, sizeof(double) * 3 * std::max(n, 1)));
+    HIP_TRY(hipMemset(E->mu_b.p, 0, sizeof(double) * 3 * std::max(n, 1)));
+    HIP_TRY(hipMemset(E->mu_prev.p, 0, sizeof(double) * 3 * std::max(n, 1)));
+    E->mu_cur = E->mu_a.as<double>();
+    HIP_TRY(E->V.ensure(sizeof(double) * std::max(n, 1)));
+    HIP_TRY(E->phi_fix.ensure(sizeof(double) * 10 * std::max(n, 1)));
+    HIP_TRY(E->phi_ind.ensure(sizeof(double) * 10 * std::max(n, 1)));
+    HIP_TRY(E->partial.ensure(sizeof(double) * (iterate_blocks(E->npol) + 1)));
     HIP_TRY(E->eps.ensure(sizeof(double) * 4));
+    HIP_TRY(E->maxcnt.ensure(sizeof(int) * 4));
     HIP_TRY(E->eblk.ensure(sizeof(double) * (std::max(energy_blocks(n), sor_blocks(n)) + 1)));
-    HIP_TRY(E->cell_of.ensure(sizeof(int) * n));
-    HIP_TRY(E->members.ensure(sizeof(int) * n));
-    HIP_TRY(E->nb_count.ensure(sizeof(int) * (n + 1)));
-    HIP_TRY(E->nb_start.ensure(sizeof(int) * (n + 1)));
+    HIP_TRY(E->cell_of.ensure(sizeof(int) * std::max(n, 1)));
+    HIP_TRY(E->members.ensure(sizeof(int) * std::max(n, 1)));
+    HIP_TRY(E->cntP.ensure(sizeof(int) * std::max(n, 1)));
+    HIP_TRY(E->cntQ.ensure(sizeof(int) * std::max(n, 1)));
     HIP_TRY(E->cell_count.ensure(sizeof(int) * 2));
     HIP_TRY(E->cell_start.ensure(sizeof(int) * 2));
     HIP_TRY(E->cell_fill.ensure(sizeof(int) * 2));
     HIP_TRY(hipHostMalloc((void**)&E->h_eps, 64, hipHostMallocDefault));
+    if (!P.pme) {  // direct sum: rows hold every other site
+        E->capP = std::max(E->npol, 1);
+        E->capQ = std::max(n - E->npol, 1);
+    }
     if (P.pme) {
         const size_t nreal = (size_t)grid[0] * grid[1] * grid[2];
         const size_t nhalf = (size_t)grid[0] * grid[1] * (grid[2] / 2 + 1);
@@ -192,6 +264,13 @@ int elec_create(mbpol_context* ctx) {
         if (int e = up(E->bx, bspline_moduli(grid[0]))) return e;
         if (int e = up(E->by, bspline_moduli(grid[1]))) return e;
         if (int e = up(E->bz, bspline_moduli(grid[2]))) return e;
+        for (int d = 0; d < 3; ++d) brick_axis(grid[d], E->B.s[d], E->B.nb[d]);
+        const int nbins = E->B.nb[0] * E->B.nb[1] * E->B.nb[2];
+        HIP_TRY(E->bin_of.ensure(sizeof(int) * std::max(n, 1)));
+        HIP_TRY(E->bin_members.ensure(sizeof(int) * std::max(n, 1)));
+        HIP_TRY(E->bin_count.ensure(sizeof(int) * (nbins + 1)));
+        HIP_TRY(E->bin_start.ensure(sizeof(int) * (nbins + 1)));
+        HIP_TRY(E->bin_fill.ensure(sizeof(int) * (nbins + 1)));
         FFT_TRY(hipfftPlan3d(&E->plan_f, grid[0], grid[1], grid[2], HIPFFT_D2Z));
         FFT_TRY(hipfftPlan3d(&E->plan_b, grid[0], grid[1], grid[2], HIPFFT_Z2D));
         E->plans = true;
@@ -213,8 +292,14 @@ void elec_destroy(mbpol_context* ctx) {
     ctx->elec = nullptr;
 }
 
-// neighbour rows of every site (PME: within the cutoff; direct sum: every other site)
-static int build_pairs(mbpol_context* ctx) {
+static ElecRows rows(const Elec* E) {
+    return ElecRows{E->capP, E->capQ, E->nbP.as<int>(), E->cntP.as<int>(), E->nbQ.as<int>(), E->cntQ.as<int>(),
+                    E->pf.as<double2>()};
+}
+
+// neighbour rows of every site (PME: within the cutoff; direct sum: every
+// other site), the induced-field prefactors, and the PME bins
+static int build_rows(mbpol_context* ctx) {
     Elec* E = ctx->elec;
     hipStream_t st = ctx->stream;
     const int n = E->n;
@@ -225,42 +310,54 @@ static int build_pairs(mbpol_context* ctx) {
         HIP_TRY(hipMemsetAsync(E->cell_fill.p, 0, sizeof(int) * (ncell + 1), st));
         launch_site_cells(st, n, pos, E->P, E->nc, E->cell_of.as<int>(), E->cell_count.as<int>());
         if (int e = scan_i32(ctx, E->cell_count.as<int>(), E->cell_start.as<int>(), ncell + 1)) return e;
-        launch_site_cell_fill(st, n, E->cell_of.as<int>(), E->cell_start.as<int>(), E->cell_fill.as<int>(),
-                              E->members.as<int>());
+        launch_bucket_fill(st, n, E->cell_of.as<int>(), E->cell_start.as<int>(), E->cell_fill.as<int>(),
+                           E->members.as<int>());
+        launch_sort_segments(st, ncell, E->cell_start.as<int>(), E->members.as<int>());
     }
-    HIP_TRY(hipMemsetAsync(E->nb_count.p, 0, sizeof(int) * (n + 1), st));
-    launch_site_pairs(st, n, pos, E->P, E->nc, E->use_cells, E->cell_of.as<int>(), E->cell_start.as<int>(),
-                      E->members.as<int>(), E->nb_count.as<int>(), nullptr, nullptr);
-    if (int e = scan_i32(ctx, E->nb_count.as<int>(), E->nb_start.as<int>(), n + 1)) return e;
     int* h = reinterpret_cast<int*>(E->h_eps) + 8;
-    HIP_TRY(hipMemcpyAsync(h, E->nb_start.as<int>() + n, sizeof(int), hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipStreamSynchronize(st));
-    const long long npair = *h;
-    if (npair > E->nb_cap) {
-        const long long cap = npair + npair / 4 + 64;
-        HIP_TRY(E->nb.ensure(sizeof(int) * cap));
-        HIP_TRY(E->pf.ensure(sizeof(double2) * cap));
-        E->nb_cap = cap;
+    for (int attempt = 0; attempt < 2; ++attempt) {
+        const size_t rp = (size_t)n * E->capP, rq = (size_t)n * E->capQ;
+        HIP_TRY(E->nbP.ensure(sizeof(int) * std::max<size_t>(rp, 1)));
+        HIP_TRY(E->pf.ensure(sizeof(double2) * std::max<size_t>(rp, 1)));
+        HIP_TRY(E->nbQ.ensure(sizeof(int) * std::max<size_t>(rq, 1)));
+        HIP_TRY(hipMemsetAsync(E->maxcnt.p, 0, sizeof(int) * 4, st));
+        launch_site_lists(st, n, pos, E->pol.as<double>(), E->P, E->nc, E->reach, E->use_cells, E->cell_of.as<int>(),
+                          E->cell_start.as<int>(), E->members.as<int>(), E->capP, E->capQ, E->nbP.as<int>(),
+                          E->cntP.as<int>(), E->nbQ.as<int>(), E->cntQ.as<int>(), E->maxcnt.as<int>());
+        HIP_TRY(hipMemcpyAsync(h, E->maxcnt.p, sizeof(int) * 3, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+        if (h[0] <= E->capP && h[1] <= E->capQ) break;
+        if (attempt == 1) return set_error(MBPOL_ERR_HIP, "electrostatics neighbour rows overflowed twice");
+        E->capP = std::max(E->capP, (h[0] + h[0] / 4 + 63) / 32 * 32);
+        E->capQ = std::max(E->capQ, (h[1] + h[1] / 4 + 63) / 32 * 32);
     }
-    launch_site_pairs(st, n, pos, E->P, E->nc, E->use_cells, E->cell_of.as<int>(), E->cell_start.as<int>(),
-                      E->members.as<int>(), nullptr, E->nb_start.as<int>(), E->nb.as<int>());
-    ctx->stats.elec_pairs = npair / 2;
+    ctx->stats.elec_pairs = h[2] / 2;
+    launch_prefactors(st, E->npol, E->polidx.as<int>(), pos, site_params(E), E->P, E->capP, E->nbP.as<int>(),
+                      E->cntP.as<int>(), E->pf.as<double2>());
+    if (E->P.pme) {
+        const int nbins = E->B.nb[0] * E->B.nb[1] * E->B.nb[2];
+        HIP_TRY(hipMemsetAsync(E->bin_count.p, 0, sizeof(int) * (nbins + 1), st));
+        HIP_TRY(hipMemsetAsync(E->bin_fill.p, 0, sizeof(int) * (nbins + 1), st));
+        launch_site_bins(st, n, pos, E->P, E->B, E->bin_of.as<int>(), E->bin_count.as<int>());
+        if (int e = scan_i32(ctx, E->bin_count.as<int>(), E->bin_start.as<int>(), nbins + 1)) return e;
+        launch_bucket_fill(st, n, E->bin_of.as<int>(), E->bin_start.as<int>(), E->bin_fill.as<int>(),
+                           E->bin_members.as<int>());
+        launch_sort_segments(st, nbins, E->bin_start.as<int>(), E->bin_members.as<int>());
+    }
     return MBPOL_OK;
 }
 
-// reciprocal-space potential of the charges (mode 0) or the dipoles (mode 1)
-static int pme_potential(mbpol_context* ctx, int mode, double* phi) {
+// reciprocal-space potential on the grid of the charges (mode 0) or of the dipoles mu (mode 1)
+static int pme_grid(mbpol_context* ctx, int mode, const double* mu) {
     Elec* E = ctx->elec;
     hipStream_t st = ctx->stream;
     const ElecParams& P = E->P;
-    const size_t nreal = (size_t)P.grid.x * P.grid.y * P.grid.z;
     const size_t nhalf = (size_t)P.grid.x * P.grid.y * (P.grid.z / 2 + 1);
-    HIP_TRY(hipMemsetAsync(E->grid.p, 0, sizeof(double) * nreal, st));
-    launch_spread(st, E->n, mode, ctx->pos.as<double>(), E->q.as<double>(), E->mu.as<double>(), P, E->grid.as<double>());
+    launch_spread_bricks(st, mode, ctx->pos.as<double>(), E->q.as<double>(), mu, P, E->B, E->bin_start.as<int>(),
+                         E->bin_members.as<int>(), E->grid.as<double>());
     FFT_TRY(hipfftExecD2Z(E->plan_f, E->grid.as<double>(), reinterpret_cast<hipfftDoubleComplex*>(E->spec.p)));
     launch_convolve(st, nhalf, E->influence.as<double>(), E->spec.as<double2>());
     FFT_TRY(hipfftExecZ2D(E->plan_b, reinterpret_cast<hipfftDoubleComplex*>(E->spec.p), E->grid.as<double>()));
-    launch_gather(st, E->n, ctx->pos.as<double>(), P, E->grid.as<double>(), phi);
     return MBPOL_OK;
 }
 
@@ -277,55 +374,55 @@ int elec_evaluate(mbpol_context* ctx, u64* energy) {
     launch_site_charges_copy(st, n, E->q0.as<double>(), E->q.as<double>());
     // sites of water w are 4w..4w+3 (the reference's ii += 4 loop, :881)
     if (P.redistribute) launch_water_charges(st, E->nw, nullptr, pos, E->q.as<double>(), E->dq.as<double>());
-    if (int e = build_pairs(ctx)) return e;
-    launch_pair_prefactors(st, n, pos, S, P, E->nb_start.as<int>(), E->nb.as<int>(), E->pf.as<double2>());
-    // fixed field
+    if (int e = build_rows(ctx)) return e;
+    const ElecRows R = rows(E);
+    // fixed field: reciprocal part at every site, real-space part at the polarisable ones
     if (P.pme) {
-        if (int e = pme_potential(ctx, 0, E->phi_fix.as<double>())) return e;
+        if (int e = pme_grid(ctx, 0, nullptr)) return e;
+        launch_gather(st, n, pos, P, E->grid.as<double>(), E->phi_fix.as<double>());
         launch_record_field(st, n, P, E->phi_fix.as<double>(), 0, 0.0, nullptr, E->efix.as<double>());
     } else {
         HIP_TRY(hipMemsetAsync(E->efix.p, 0, sizeof(double) * 3 * n, st));
     }
-    launch_fixed_field(st, n, pos, S, P, E->nb_start.as<int>(), E->nb.as<int>(), E->efix.as<double>());
-    // alpha * E_fixed, and the initial guess
+    launch_fixed_field(st, E->npol, E->polidx.as<int>(), pos, S, P, R, E->efix.as<double>());
+    // alpha * E_fixed, and the initial guess (into mu_a)
     launch_scale_copy(st, n, E->efix.as<double>(), E->pol.as<double>(), E->efix_pol.as<double>(), nullptr);
+    double* mu_in = E->mu_a.as<double>();
+    double* mu_out = E->mu_b.as<double>();
+    const size_t mbytes = sizeof(double) * 3 * n;
     switch (E->guess_mode) {
-        case MBPOL_GUESS_ZERO: HIP_TRY(hipMemsetAsync(E->mu.p, 0, sizeof(double) * 3 * n, st)); break;
+        case MBPOL_GUESS_ZERO: HIP_TRY(hipMemsetAsync(mu_in, 0, mbytes, st)); break;
         case MBPOL_GUESS_PREVIOUS:
             if (E->have_previous) {
-                HIP_TRY(hipMemcpyAsync(E->mu.p, E->mu_prev.p, sizeof(double) * 3 * n, hipMemcpyDeviceToDevice, st));
+                HIP_TRY(hipMemcpyAsync(mu_in, E->mu_prev.p, mbytes, hipMemcpyDeviceToDevice, st));
                 break;
             }
             [[fallthrough]];
         case MBPOL_GUESS_USER:
             if (E->guess_mode == MBPOL_GUESS_USER) {
-                HIP_TRY(hipMemcpyAsync(E->mu.p, E->mu_user.p, sizeof(double) * 3 * n, hipMemcpyDeviceToDevice, st));
+                HIP_TRY(hipMemcpyAsync(mu_in, E->mu_user.p, mbytes, hipMemcpyDeviceToDevice, st));
                 break;
             }
             [[fallthrough]];
-        default: HIP_TRY(hipMemcpyAsync(E->mu.p, E->efix_pol.p, sizeof(double) * 3 * n, hipMemcpyDeviceToDevice, st));
+        default: HIP_TRY(hipMemcpyAsync(mu_in, E->efix_pol.p, mbytes, hipMemcpyDeviceToDevice, st));
     }
     timer_stop(ctx, T_ELEC_SETUP);
     // SOR iteration (convergeInduceDipoles, :557-616)
-    const double self_term = (4.0 / 3.0) * (P.alpha * P.alpha * P.alpha) / mbpol_params::el_sqrt_pi;
-    const int nb = sor_blocks(n);
+    const double self_term = P.pme ? (4.0 / 3.0) * (P.alpha * P.alpha * P.alpha) / mbpol_params::el_sqrt_pi : 0.0;
     double cur = 1.0e+50;
     int it = 0;
     bool conv = false;
     for (;;) {
         timer_start(ctx, T_ELEC_ITER);
-        launch_induced_field(st, n, pos, P, E->nb_start.as<int>(), E->nb.as<int>(), E->pf.as<double2>(),
-                             E->mu.as<double>(), E->field.as<double>());
-        if (P.pme) {
-            if (int e = pme_potential(ctx, 1, E->phi_ind.as<double>())) return e;
-            launch_record_field(st, n, P, E->phi_ind.as<double>(), 1, self_term, E->mu.as<double>(), E->field.as<double>());
-        }
-        launch_sor_update(st, n, E->pol.as<double>(), E->efix_pol.as<double>(), E->field.as<double>(),
-                          mbpol_params::el_polar_sor, E->mu.as<double>(), E->partial.as<double>());
-        launch_eps_finish(st, nb, n, E->partial.as<double>(), E->eps.as<double>());
+        if (P.pme)
+            if (int e = pme_grid(ctx, 1, mu_in)) return e;
+        launch_iterate(st, E->npol, E->polidx.as<int>(), pos, P, R, mu_in, mu_out, E->efix_pol.as<double>(),
+                       E->pol.as<double>(), E->grid.as<double>(), self_term, mbpol_params::el_polar_sor, n,
+                       E->partial.as<double>(), E->eps.as<double>());
         timer_stop(ctx, T_ELEC_ITER);
         HIP_TRY(hipMemcpyAsync(E->h_eps, E->eps.p, sizeof(double), hipMemcpyDeviceToHost, st));
         HIP_TRY(hipStreamSynchronize(st));
+        std::swap(mu_in, mu_out);  // mu_in: dipoles after the step, mu_out: before it
         const double eps = E->h_eps[0];
         bool done = false;
         if (eps < s.el_eps) {
@@ -341,20 +438,24 @@ int elec_evaluate(mbpol_context* ctx, u64* energy) {
     ctx->stats.iterations = it;
     ctx->stats.epsilon = cur;
     ctx->stats.converged = conv ? 1 : 0;
+    E->mu_cur = mu_in;
     if (!conv)
         return set_error(MBPOL_ERR_NOT_CONVERGED, "Induced dipoles did not converge:  iterations=%d eps=%g", it, cur);
-    HIP_TRY(hipMemcpyAsync(E->mu_prev.p, E->mu.p, sizeof(double) * 3 * n, hipMemcpyDeviceToDevice, st));
+    HIP_TRY(hipMemcpyAsync(E->mu_prev.p, mu_in, mbytes, hipMemcpyDeviceToDevice, st));
     E->have_previous = true;
     // energy and forces (calculateElectrostatic, :838-855 direct sum, :2718-2777 PME)
     timer_start(ctx, T_ELEC_ENERGY);
     HIP_TRY(hipMemsetAsync(E->fsite.p, 0, sizeof(double) * 3 * n, st));
     HIP_TRY(hipMemsetAsync(E->V.p, 0, sizeof(double) * n, st));
-    launch_elec_energy(st, n, pos, S, P, E->nb_start.as<int>(), E->nb.as<int>(), E->mu.as<double>(), E->fsite.as<double>(),
-                       E->V.as<double>(), E->eblk.as<double>());
+    launch_elec_energy(st, n, pos, S, P, R, mu_in, E->fsite.as<double>(), E->V.as<double>(), E->eblk.as<double>());
     launch_energy_finish(st, energy_blocks(n), E->eblk.as<double>(), energy);
     if (P.pme) {
-        launch_recip_site(st, n, S, P, E->phi_fix.as<double>(), E->phi_ind.as<double>(), E->mu.as<double>(),
-                          E->fsite.as<double>(), E->V.as<double>(), E->eblk.as<double>());
+        // the reciprocal induced potential of the reference is the one of the last
+        // iteration's field evaluation, i.e. of the dipoles before the final step
+        if (int e = pme_grid(ctx, 1, mu_out)) return e;
+        launch_gather(st, n, pos, P, E->grid.as<double>(), E->phi_ind.as<double>());
+        launch_recip_site(st, n, S, P, E->phi_fix.as<double>(), E->phi_ind.as<double>(), mu_in, E->fsite.as<double>(),
+                          E->V.as<double>(), E->eblk.as<double>());
         launch_energy_finish(st, sor_blocks(n), E->eblk.as<double>(), energy);
     }
     if (P.redistribute) launch_charge_deriv_forces(st, E->nw, E->dq.as<double>(), E->V.as<double>(), E->fsite.as<double>());
@@ -372,7 +473,8 @@ extern "C" {
 int mbpol_context_get_dipoles(const mbpol_context* ctx, double* dipoles) {
     using namespace mbpol;
     if (!ctx->elec) return set_error(MBPOL_ERR_INVALID_ARGUMENT, "no electrostatics in this system");
-    HIP_TRY(hipMemcpy(dipoles, ctx->elec->mu.p, sizeof(double) * 3 * ctx->elec->n, hipMemcpyDeviceToHost));
+    HIP_TRY(hipSetDevice(ctx->device));
+    HIP_TRY(hipMemcpy(dipoles, ctx->elec->mu_cur, sizeof(double) * 3 * ctx->elec->n, hipMemcpyDeviceToHost));
     return MBPOL_OK;
 }
 
@@ -389,8 +491,13 @@ int mbpol_context_set_dipole_guess(mbpol_context* ctx, int32_t mode, const doubl
     if (mode < MBPOL_GUESS_ALPHA_E || mode > MBPOL_GUESS_USER) return set_error(MBPOL_ERR_INVALID_ARGUMENT, "bad guess mode");
     if (mode == MBPOL_GUESS_USER) {
         if (!dipoles) return set_error(MBPOL_ERR_INVALID_ARGUMENT, "user guess needs dipoles");
+        // sites of zero polarisability keep zero dipoles (their guess entries are ignored)
+        Elec* E = ctx->elec;
+        std::vector<double> g(dipoles, dipoles + 3 * (size_t)E->n);
+        for (int i = 0; i < E->n; ++i)
+            if (!E->h_pol[i]) g[3 * i] = g[3 * i + 1] = g[3 * i + 2] = 0.0;
         HIP_TRY(hipSetDevice(ctx->device));
-        HIP_TRY(hipMemcpy(ctx->elec->mu_user.p, dipoles, sizeof(double) * 3 * ctx->elec->n, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(E->mu_user.p, g.data(), sizeof(double) * 3 * (size_t)E->n, hipMemcpyHostToDevice));
     }
     ctx->elec->guess_mode = mode;
     return MBPOL_OK;
@@ -408,7 +515,7 @@ int mbpol_context_get_electrostatic_potential(mbpol_context* ctx, int32_t npoint
     HIP_TRY(hipMemcpy(pts.p, points, sizeof(double) * 3 * npoints, hipMemcpyHostToDevice));
     Elec* E = ctx->elec;
     launch_potential_at_points(ctx->stream, npoints, pts.as<double>(), E->n, ctx->pos.as<double>(), E->q.as<double>(),
-                               E->mu.as<double>(), E->P, out.as<double>());
+                               E->mu_cur, E->P, out.as<double>());
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     HIP_TRY(hipMemcpy(potential, out.p, sizeof(double) * npoints, hipMemcpyDeviceToHost));
     return MBPOL_OK;
@@ -433,6 +540,18 @@ int mbpol_context_update_electrostatics(mbpol_context* ctx, int32_t n, const dou
     if (int e = up(E->pol, s.el_pol)) return e;
     if (int e = up(E->mol, s.el_mol)) return e;
     if (int e = up(E->type, s.el_type)) return e;
+    if (int e = set_polarisable(E, s.el_pol)) return e;
+    HIP_TRY(E->partial.ensure(sizeof(double) * (iterate_blocks(E->npol) + 1)));
+    // the polarisable set may have changed: clear the dipoles the iteration no longer visits
+    HIP_TRY(hipMemset(E->mu_a.p, 0, sizeof(double) * 3 * std::max(n, 1)));
+    HIP_TRY(hipMemset(E->mu_b.p, 0, sizeof(double) * 3 * std::max(n, 1)));
+    HIP_TRY(hipMemset(E->mu_prev.p, 0, sizeof(double) * 3 * std::max(n, 1)));
+    E->mu_cur = E->mu_a.as<double>();
+    E->have_previous = false;
+    if (!E->P.pme) {
+        E->capP = std::max(E->npol, 1);
+        E->capQ = std::max(n - E->npol, 1);
+    }
     return MBPOL_OK;
 }
 

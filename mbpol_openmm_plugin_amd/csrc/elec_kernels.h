@@ -23,32 +23,51 @@ struct SiteParams {
     const int* type;
 };
 
+// fixed-capacity neighbour rows: row i of P (Q) holds the polarisable
+// (non-polarisable) neighbours of site i; pf holds the induced-field
+// prefactors of the P entries of polarisable sites
+struct ElecRows {
+    int capP, capQ;
+    const int *nbP, *cntP, *nbQ, *cntQ;
+    const double2* pf;
+};
+
+// PME grid bricks: s[d] points per brick along axis d (every brick, the last
+// one included, at least 4 points wide), nb[d] bricks
+struct BrickGeom {
+    int s[3];
+    int nb[3];
+};
+
 void launch_site_charges_copy(hipStream_t s, int n, const double* q0, double* q);
 void launch_site_cells(hipStream_t s, int n, const double* pos, ElecParams P, int3 nc, int* cell_of, int* cell_count);
-void launch_site_cell_fill(hipStream_t s, int n, const int* cell_of, const int* cell_start, int* cell_fill, int* members);
-void launch_site_pairs(hipStream_t s, int n, const double* pos, ElecParams P, int3 nc, int use_cells, const int* cell_of,
-                       const int* cell_start, const int* members, int* count, const int* start, int* out);
-void launch_pair_prefactors(hipStream_t s, int n, const double* pos, SiteParams S, ElecParams P, const int* nb_start,
-                            const int* nb, double2* pf);
-void launch_fixed_field(hipStream_t s, int n, const double* pos, SiteParams S, ElecParams P, const int* nb_start,
-                        const int* nb, double* field);
-void launch_induced_field(hipStream_t s, int n, const double* pos, ElecParams P, const int* nb_start, const int* nb,
-                          const double2* pf, const double* mu, double* field);
-void launch_spread(hipStream_t s, int n, int mode, const double* pos, const double* q, const double* mu, ElecParams P,
-                   double* grid);
+void launch_bucket_fill(hipStream_t s, int n, const int* bucket_of, const int* bucket_start, int* bucket_fill,
+                        int* members);
+void launch_sort_segments(hipStream_t s, int nseg, const int* start, int* data);
+void launch_site_lists(hipStream_t s, int n, const double* pos, const double* pol, ElecParams P, int3 nc, int reach,
+                       int use_cells, const int* cell_of, const int* cell_start, const int* members, int capP, int capQ,
+                       int* nbP, int* cntP, int* nbQ, int* cntQ, int* maxcnt);
+void launch_prefactors(hipStream_t s, int npol, const int* polidx, const double* pos, SiteParams S, ElecParams P,
+                       int capP, const int* nbP, const int* cntP, double2* pf);
+void launch_fixed_field(hipStream_t s, int npol, const int* polidx, const double* pos, SiteParams S, ElecParams P,
+                        const ElecRows& R, double* field);
+void launch_site_bins(hipStream_t s, int n, const double* pos, ElecParams P, BrickGeom B, int* bin_of, int* bin_count);
+void launch_spread_bricks(hipStream_t s, int mode, const double* pos, const double* q, const double* mu, ElecParams P,
+                          BrickGeom B, const int* bin_start, const int* members, double* grid);
 void launch_convolve(hipStream_t s, size_t nhalf, const double* influence, double2* spec);
 void launch_influence(hipStream_t s, ElecParams P, const double* bx, const double* by, const double* bz, double* out);
 void launch_gather(hipStream_t s, int n, const double* pos, ElecParams P, const double* grid, double* phi);
 void launch_record_field(hipStream_t s, int n, ElecParams P, const double* phi, int accumulate, double self_term,
                          const double* mu, double* field);
-int sor_blocks(int n);
-void launch_sor_update(hipStream_t s, int n, const double* pol, const double* efix_pol, const double* field, double omega,
-                       double* mu, double* partial);
-void launch_eps_finish(hipStream_t s, int nblocks, int n, const double* partial, double* eps_out);
+int iterate_blocks(int npol);
+void launch_iterate(hipStream_t s, int npol, const int* polidx, const double* pos, ElecParams P, const ElecRows& R,
+                    const double* mu_in, double* mu_out, const double* efix_pol, const double* pol, const double* grid,
+                    double self_term, double omega, int n_all, double* partial, double* eps);
 void launch_scale_copy(hipStream_t s, int n, const double* a, const double* pol, double* out, double* out2);
 int energy_blocks(int n);
-void launch_elec_energy(hipStream_t s, int n, const double* pos, SiteParams S, ElecParams P, const int* nb_start,
-                        const int* nb, const double* mu, double* force, double* V, double* eblk);
+int sor_blocks(int n);
+void launch_elec_energy(hipStream_t s, int n, const double* pos, SiteParams S, ElecParams P, const ElecRows& R,
+                        const double* mu, double* force, double* V, double* eblk);
 void launch_recip_site(hipStream_t s, int n, SiteParams S, ElecParams P, const double* phi_fix, const double* phi_ind,
                        const double* mu, double* force, double* V, double* eblk);
 void launch_charge_deriv_forces(hipStream_t s, int nw, const double* dq, const double* V, double* force);

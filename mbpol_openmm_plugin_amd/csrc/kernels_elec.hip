@@ -8,8 +8,13 @@
 // d/p-symmetric expression is written for d = p.
 //
 // Pair kernels are "site-centric": lanes of one group of G lanes share a site
-// i and stride over its neighbours, so every contribution to site i stays in
-// registers and is reduced across the group; no atomics, deterministic.
+// i and stride over its neighbour rows, so every contribution to site i stays
+// in registers and is reduced across the group.  Neighbour rows, bins and
+// spreading are ordered by site index, so results do not depend on scheduling.
+// Sites of zero polarisability keep zero induced dipoles throughout; the
+// iteration therefore runs over the polarisable sites only.
+#include <algorithm>
+
 #include "device_common.h"
 #include "elec_kernels.h"
 
@@ -115,6 +120,8 @@ __device__ __forceinline__ void min_image(double& dx, double& dy, double& dz, co
     dz -= floor(dz * P.inv_box.z + 0.5) * P.box.z;
 }
 
+
+
 template <int G>
 __device__ __forceinline__ double group_sum(double v) {
 #pragma unroll
@@ -122,20 +129,16 @@ __device__ __forceinline__ double group_sum(double v) {
     return v;
 }
 
-// iterate over the neighbours of site i: its CSR row (PME) or every other site (no cutoff)
-template <int G, typename F>
-__device__ __forceinline__ void for_neighbours(int i, int lane, const ElecParams& P, const int* nb_start,
-                                               const int* nb, F&& f) {
-    if (P.pme) {
-        for (int t = nb_start[i] + lane; t < nb_start[i + 1]; t += G) f(nb[t], t);
-    } else {
-        for (int j = lane; j < P.n; j += G)
-            if (j != i) f(j, -1);
-    }
+// bits of the G-lane group that contains this lane, and of the lanes below it
+template <int G>
+__device__ __forceinline__ unsigned long long group_bits() {
+    const int lane = threadIdx.x & 63;
+    return G == 64 ? ~0ull : (((1ull << G) - 1ull) << (lane & ~(G - 1)));
 }
+__device__ __forceinline__ unsigned long long lanes_below() { return (1ull << (threadIdx.x & 63)) - 1ull; }
 
 // ---------------------------------------------------------------------------
-// site set-up and neighbour lists
+// site set-up and neighbour rows
 // ---------------------------------------------------------------------------
 __global__ void k_site_charges_copy(int n, const double* __restrict__ q0, double* __restrict__ q) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -161,67 +164,128 @@ __global__ void k_site_cells(int n, const double* __restrict__ pos, ElecParams P
     atomicAdd(&cell_count[cell], 1);
 }
 
-__global__ void k_site_cell_fill(int n, const int* __restrict__ cell_of, const int* __restrict__ cell_start,
-                                 int* __restrict__ cell_fill, int* __restrict__ members) {
+// scatter item i into its bucket (slot order is arbitrary; k_sort_segments restores index order)
+__global__ void k_bucket_fill(int n, const int* __restrict__ bucket_of, const int* __restrict__ bucket_start,
+                              int* __restrict__ bucket_fill, int* __restrict__ members) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const int c = cell_of[i];
-    members[cell_start[c] + atomicAdd(&cell_fill[c], 1)] = i;
+    const int c = bucket_of[i];
+    members[bucket_start[c] + atomicAdd(&bucket_fill[c], 1)] = i;
 }
 
-// directed neighbour rows (j != i, minimum image r^2 <= rc^2): count pass (out == null) and fill pass
-__global__ void k_site_pairs(int n, const double* __restrict__ pos, ElecParams P, int3 nc, int use_cells,
-                             const int* __restrict__ cell_of, const int* __restrict__ cell_start,
-                             const int* __restrict__ members, int* __restrict__ count, const int* __restrict__ start,
-                             int* __restrict__ out) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+// ascending insertion sort of each (short) segment, one lane per segment: the
+// order of every neighbour row and spreading sum then depends on the input only
+__global__ void k_sort_segments(int nseg, const int* __restrict__ start, int* __restrict__ data) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nseg) return;
+    const int b = start[s], e = start[s + 1];
+    for (int k = b + 1; k < e; ++k) {
+        const int v = data[k];
+        int m = k - 1;
+        while (m >= b && data[m] > v) {
+            data[m + 1] = data[m];
+            --m;
+        }
+        data[m + 1] = v;
+    }
+}
+
+// Neighbour rows of every site, in fixed-capacity (ELL) layout: row i of the
+// P table lists the polarisable sites j != i with r_ij <= rc (minimum image
+// under PME), row i of the Q table the non-polarisable ones.  One group of G
+// lanes per site; candidates are taken in cell order and compacted with a
+// wavefront ballot.  Counts are kept beyond the capacity so that the host can
+// detect an overflow (maxcnt[0..1]) and rebuild with larger rows; maxcnt[2]
+// accumulates the number of directed pairs.
+template <int G>
+__global__ void __launch_bounds__(256) k_site_lists(int n, const double* __restrict__ pos, const double* __restrict__ pol,
+                                                    ElecParams P, int3 nc, int reach, int use_cells,
+                                                    const int* __restrict__ cell_of, const int* __restrict__ cell_start,
+                                                    const int* __restrict__ members, int capP, int capQ,
+                                                    int* __restrict__ nbP, int* __restrict__ cntP,
+                                                    int* __restrict__ nbQ, int* __restrict__ cntQ,
+                                                    int* __restrict__ maxcnt) {
+    const int i = (blockIdx.x * blockDim.x + threadIdx.x) / G, lane = threadIdx.x % G;
+    if (i >= n) return;  // uniform over the group
     const double xi = pos[3 * i], yi = pos[3 * i + 1], zi = pos[3 * i + 2];
     const double rc2 = P.cutoff * P.cutoff;
-    int c = 0;
-    const int o = out ? start[i] : 0;
-    auto visit = [&](int j) {
-        if (j == i) return;
-        double dx = pos[3 * j] - xi, dy = pos[3 * j + 1] - yi, dz = pos[3 * j + 2] - zi;
-        min_image(dx, dy, dz, P);
-        if (dx * dx + dy * dy + dz * dz > rc2) return;
-        if (out) out[o + c] = j;
-        ++c;
+    const unsigned long long gb = group_bits<G>(), below = lanes_below();
+    int cP = 0, cQ = 0;
+    int* rowP = nbP + (size_t)i * capP;
+    int* rowQ = nbQ + (size_t)i * capQ;
+    auto consider = [&](int j, bool active) {
+        bool ok = false, pj = false;
+        if (active && j != i) {
+            double dx = pos[3 * j] - xi, dy = pos[3 * j + 1] - yi, dz = pos[3 * j + 2] - zi;
+            min_image(dx, dy, dz, P);
+            ok = dx * dx + dy * dy + dz * dz <= rc2;
+            pj = pol[j] != 0.0;
+        }
+        const bool inP = ok && pj, inQ = ok && !pj;
+        const unsigned long long bP = __ballot(inP) & gb, bQ = __ballot(inQ) & gb;
+        if (inP) {
+            const int k = cP + __popcll(bP & below);
+            if (k < capP) rowP[k] = j;
+        }
+        if (inQ) {
+            const int k = cQ + __popcll(bQ & below);
+            if (k < capQ) rowQ[k] = j;
+        }
+        cP += __popcll(bP);
+        cQ += __popcll(bQ);
     };
     if (!use_cells) {
-        for (int j = 0; j < n; ++j) visit(j);
+        for (int b = 0; b < n; b += G) consider(b + lane, b + lane < n);
     } else {
         const int cell = cell_of[i];
         const int cz = cell % nc.z, cy = (cell / nc.z) % nc.y, cx = cell / (nc.z * nc.y);
-        for (int ax = -1; ax <= 1; ++ax)
-            for (int ay = -1; ay <= 1; ++ay)
-                for (int az = -1; az <= 1; ++az) {
+        for (int ax = -reach; ax <= reach; ++ax)
+            for (int ay = -reach; ay <= reach; ++ay)
+                for (int az = -reach; az <= reach; ++az) {
                     int X = cx + ax, Y = cy + ay, Z = cz + az;
                     X = X < 0 ? X + nc.x : (X >= nc.x ? X - nc.x : X);
                     Y = Y < 0 ? Y + nc.y : (Y >= nc.y ? Y - nc.y : Y);
                     Z = Z < 0 ? Z + nc.z : (Z >= nc.z ? Z - nc.z : Z);
                     const int cc = (X * nc.y + Y) * nc.z + Z;
-                    for (int t = cell_start[cc]; t < cell_start[cc + 1]; ++t) visit(members[t]);
+                    const int b0 = cell_start[cc], b1 = cell_start[cc + 1];
+                    for (int t = b0; t < b1; t += G) {
+                        const int tt = t + lane;
+                        consider(tt < b1 ? members[tt] : -1, tt < b1);
+                    }
                 }
     }
-    if (count) count[i] = c;
+    if (lane == 0) {
+        cntP[i] = cP;
+        cntQ[i] = cQ;
+        atomicMax(&maxcnt[0], cP);
+        atomicMax(&maxcnt[1], cQ);
+        atomicAdd(&maxcnt[2], cP + cQ);
+    }
 }
 
-// per directed pair: induced-field prefactors (PME :2392-2470 with the precomputed
-// justScale factors of :2280-2298; direct sum :534-555) -- field_i += pf1 mu_j + pf2 (mu_j.d) d
-__global__ void k_pair_prefactors(int n, const double* __restrict__ pos, const SiteParams S, ElecParams P,
-                                  const int* __restrict__ nb_start, const int* __restrict__ nb, double2* __restrict__ pf) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+// induced-field prefactors of every (polarisable i, polarisable j) entry:
+// field_i += p1 mu_j + p2 (mu_j.d) d (PME :2374-2390 with the precomputed
+// justScale factors of :2280-2298; direct sum :438-486 with :534-555)
+template <int G>
+__global__ void __launch_bounds__(256) k_prefactors(int npol, const int* __restrict__ polidx, const double* __restrict__ pos,
+                                                    const SiteParams S, ElecParams P, int capP,
+                                                    const int* __restrict__ nbP, const int* __restrict__ cntP,
+                                                    double2* __restrict__ pf) {
+    const int g = (blockIdx.x * blockDim.x + threadIdx.x) / G, lane = threadIdx.x % G;
+    if (g >= npol) return;
+    const int i = polidx[g];
     const double xi = pos[3 * i], yi = pos[3 * i + 1], zi = pos[3 * i + 2];
-    for (int t = nb_start[i]; t < nb_start[i + 1]; ++t) {
-        const int j = nb[t];
+    const int c = min(cntP[i], capP);
+    const int* row = nbP + (size_t)i * capP;
+    double2* prow = pf + (size_t)i * capP;
+    for (int k = lane; k < c; k += G) {
+        const int j = row[k];
         double dx = pos[3 * j] - xi, dy = pos[3 * j + 1] - yi, dz = pos[3 * j + 2] - zi;
         min_image(dx, dy, dz, P);
         const double r2 = dx * dx + dy * dy + dz * dz, r = sqrt(r2);
         const Thole th(S.damp[i], S.damp[j], r);
-        const double g = tdd_gamma(P, S.mol[i] == S.mol[j], S.type[i], S.type[j]);
-        const double sc3 = th.s3(g), sc5 = th.s5(g);
+        const double gam = tdd_gamma(P, S.mol[i] == S.mol[j], S.type[i], S.type[j]);
+        const double sc3 = th.s3(gam), sc5 = th.s5(gam);
         double p1, p2;
         if (P.pme) {
             const Ewald ew(P.alpha, r, r2);
@@ -233,21 +297,38 @@ __global__ void k_pair_prefactors(int n, const double* __restrict__ pos, const S
             p1 = -(ri * r2i) * sc3;
             p2 = (ri * r2i * 3.0 * r2i) * sc5;
         }
-        pf[t] = make_double2(p1, p2);
+        prow[k] = make_double2(p1, p2);
     }
 }
 
+// visit every neighbour of site i (P row, then Q row) with G lanes
+template <int G, typename F>
+__device__ __forceinline__ void for_rows(int i, int lane, int capP, const int* __restrict__ nbP,
+                                         const int* __restrict__ cntP, int capQ, const int* __restrict__ nbQ,
+                                         const int* __restrict__ cntQ, F&& f) {
+    const int cp = min(cntP[i], capP), cq = min(cntQ[i], capQ);
+    const int* rp = nbP + (size_t)i * capP;
+    const int* rq = nbQ + (size_t)i * capQ;
+    for (int k = lane; k < cp; k += G) f(rp[k]);
+    for (int k = lane; k < cq; k += G) f(rq[k]);
+}
+
 // ---------------------------------------------------------------------------
-// fixed field (direct part), :361-420 direct sum, :1342-1407 PME
+// fixed field (direct part) at the polarisable sites, :361-420 direct sum,
+// :1342-1407 PME
 // ---------------------------------------------------------------------------
 template <int G>
-__global__ void k_fixed_field(int n, const double* __restrict__ pos, const SiteParams S, ElecParams P,
-                              const int* __restrict__ nb_start, const int* __restrict__ nb, double* __restrict__ field) {
-    const int i = (blockIdx.x * blockDim.x + threadIdx.x) / G, lane = threadIdx.x % G;
-    if (i >= n) return;
+__global__ void __launch_bounds__(256) k_fixed_field(int npol, const int* __restrict__ polidx,
+                                                     const double* __restrict__ pos, const SiteParams S, ElecParams P,
+                                                     int capP, const int* __restrict__ nbP, const int* __restrict__ cntP,
+                                                     int capQ, const int* __restrict__ nbQ,
+                                                     const int* __restrict__ cntQ, double* __restrict__ field) {
+    const int g = (blockIdx.x * blockDim.x + threadIdx.x) / G, lane = threadIdx.x % G;
+    if (g >= npol) return;
+    const int i = polidx[g];
     const double xi = pos[3 * i], yi = pos[3 * i + 1], zi = pos[3 * i + 2];
     double ex = 0, ey = 0, ez = 0;
-    for_neighbours<G>(i, lane, P, nb_start, nb, [&](int j, int) {
+    for_rows<G>(i, lane, capP, nbP, cntP, capQ, nbQ, cntQ, [&](int j) {
         const int same = S.mol[i] == S.mol[j];
         double dx = pos[3 * j] - xi, dy = pos[3 * j + 1] - yi, dz = pos[3 * j + 2] - zi;
         if (!P.pme) {
@@ -255,7 +336,9 @@ __global__ void k_fixed_field(int n, const double* __restrict__ pos, const SiteP
             const double r2 = dx * dx + dy * dy + dz * dz, r = sqrt(r2);
             const double rr3 = Thole(S.damp[i], S.damp[j], r).s3(P.thole[0]) / (r2 * r);
             const double f = -rr3 * S.q[j];
-            ex += dx * f; ey += dy * f; ez += dz * f;
+            ex += dx * f;
+            ey += dy * f;
+            ez += dz * f;
         } else {
             min_image(dx, dy, dz, P);
             const double r2 = dx * dx + dy * dy + dz * dz, r = sqrt(r2);
@@ -264,7 +347,9 @@ __global__ void k_fixed_field(int n, const double* __restrict__ pos, const SiteP
             if (same) s3 = 2;  // reproduced as in the reference (:1382-1385)
             const double rr3 = (s3 - 1.) / (r2 * r);
             const double f = -(ew.bn1 - rr3) * S.q[j];
-            ex += dx * f; ey += dy * f; ez += dz * f;
+            ex += dx * f;
+            ey += dy * f;
+            ez += dz * f;
         }
     });
     ex = group_sum<G>(ex);
@@ -274,38 +359,6 @@ __global__ void k_fixed_field(int n, const double* __restrict__ pos, const SiteP
         field[3 * i] += ex;
         field[3 * i + 1] += ey;
         field[3 * i + 2] += ez;
-    }
-}
-
-// ---------------------------------------------------------------------------
-// induced field (direct part) with precomputed prefactors
-// ---------------------------------------------------------------------------
-template <int G>
-__global__ void k_induced_field(int n, const double* __restrict__ pos, ElecParams P, const int* __restrict__ nb_start,
-                                const int* __restrict__ nb, const double2* __restrict__ pf, const double* __restrict__ mu,
-                                double* __restrict__ field) {
-    const int i = (blockIdx.x * blockDim.x + threadIdx.x) / G, lane = threadIdx.x % G;
-    if (i >= n) return;
-    const double xi = pos[3 * i], yi = pos[3 * i + 1], zi = pos[3 * i + 2];
-    double ex = 0, ey = 0, ez = 0;
-    for (int t = nb_start[i] + lane; t < nb_start[i + 1]; t += G) {
-        const int j = nb[t];
-        double dx = pos[3 * j] - xi, dy = pos[3 * j + 1] - yi, dz = pos[3 * j + 2] - zi;
-        min_image(dx, dy, dz, P);
-        const double2 p = pf[t];
-        const double mx = mu[3 * j], my = mu[3 * j + 1], mz = mu[3 * j + 2];
-        const double dur = (mx * dx + my * dy + mz * dz) * p.y;
-        ex += dx * dur + mx * p.x;
-        ey += dy * dur + my * p.x;
-        ez += dz * dur + mz * p.x;
-    }
-    ex = group_sum<G>(ex);
-    ey = group_sum<G>(ey);
-    ez = group_sum<G>(ez);
-    if (lane == 0) {
-        field[3 * i] = ex;
-        field[3 * i + 1] = ey;
-        field[3 * i + 2] = ez;
     }
 }
 
@@ -362,8 +415,10 @@ __device__ __forceinline__ void bspline5(double w, double th[5][4]) {
     }
 }
 
-// grid origin and splines of one site along each axis (computeMBPolBsplines, :1524-1554)
-__device__ __forceinline__ void site_splines(const double* pos, int i, const ElecParams& P, int g0[3], double th[3][5][4]) {
+
+// grid origin g0 and fractional offsets of one site along each axis
+// (computeMBPolBsplines, :1524-1554); the single place both are computed
+__device__ __forceinline__ void site_origin(const double* pos, int i, const ElecParams& P, int g0[3], double w[3]) {
     double p[3] = {pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]};
     const double L[3] = {P.box.x, P.box.y, P.box.z}, inv[3] = {P.inv_box.x, P.inv_box.y, P.inv_box.z};
     const int K[3] = {P.grid.x, P.grid.y, P.grid.z};
@@ -376,44 +431,190 @@ __device__ __forceinline__ void site_splines(const double* pos, int i, const Ele
         int g = ifr - 5 + 1;
         g += g < 0 ? K[d] : 0;
         g0[d] = g;
-        bspline5(fr - ifr, th[d]);
+        w[d] = fr - ifr;
     }
+}
+
+__device__ __forceinline__ void site_splines(const double* pos, int i, const ElecParams& P, int g0[3], double th[3][5][4]) {
+    double w[3];
+    site_origin(pos, i, P, g0, w);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) bspline5(w[d], th[d]);
 }
 
 __device__ __forceinline__ int wrapg(int v, int K) { return v >= K ? v - K : v; }
 
-// charges (mode 0) or fractional dipoles (mode 1) onto the real grid: one lane per (site, ix)
-__global__ void k_spread(int n, int mode, const double* __restrict__ pos, const double* __restrict__ q,
-                         const double* __restrict__ mu, ElecParams P, double* __restrict__ grid) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    const int i = t / 5, ix = t % 5;
+// th[k][c] for a lane-dependent k in [0, 5), as a masked sum over the five
+// entries: the spline table stays in registers (a dynamic index, or a select
+// chain the compiler folds back into one, would move it to scratch memory)
+__device__ __forceinline__ double pick5(const double (&th)[5][4], int k, int c) {
+    double v = 0.0;
+#pragma unroll
+    for (int m = 0; m < 5; ++m) v += th[m][c] * (double)(k == m);
+    return v;
+}
+
+// ---------------------------------------------------------------------------
+// PME spreading by grid bricks.  The grid is cut into bricks of B.s points per
+// axis, the last brick of an axis taking the remainder (4 to 2 B.s - 1 points,
+// never more than 12); a site belongs to the bin of the brick holding its
+// origin g0.  Its 5x5x5 support reaches the brick of its own bin and the next
+// one along each axis only (every brick is at least 4 points wide), so brick b
+// is complete after visiting bins b-1 and b per axis.  One workgroup owns one brick and
+// writes every point of it exactly once: no atomics, no clearing pass, and a
+// summation order fixed by the (sorted) bin contents.
+// ---------------------------------------------------------------------------
+__global__ void k_site_bins(int n, const double* __restrict__ pos, ElecParams P, BrickGeom B, int* __restrict__ bin_of,
+                            int* __restrict__ bin_count) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     int g0[3];
-    double th[3][5][4];
-    site_splines(pos, i, P, g0, th);
-    const int gx = wrapg(g0[0] + ix, P.grid.x);
-    const double sc[3] = {P.grid.x * P.inv_box.x, P.grid.y * P.inv_box.y, P.grid.z * P.inv_box.z};
-    double dq = 0, dx = 0, dy = 0, dz = 0;
-    if (mode == 0) {
-        dq = q[i];
-        if (dq == 0.0) return;
-    } else {
-        dx = sc[0] * mu[3 * i];
-        dy = sc[1] * mu[3 * i + 1];
-        dz = sc[2] * mu[3 * i + 2];
-        if (dx == 0.0 && dy == 0.0 && dz == 0.0) return;
-    }
-    const double t0 = th[0][ix][0], t1 = th[0][ix][1];
-    for (int iy = 0; iy < 5; ++iy) {
-        const int gy = wrapg(g0[1] + iy, P.grid.y);
-        const double u0 = th[1][iy][0], u1 = th[1][iy][1];
-        double* row = grid + ((size_t)gx * P.grid.y + gy) * P.grid.z;
+    double w[3];
+    site_origin(pos, i, P, g0, w);
+    const int b = (min(g0[0] / B.s[0], B.nb[0] - 1) * B.nb[1] + min(g0[1] / B.s[1], B.nb[1] - 1)) * B.nb[2] +
+                  min(g0[2] / B.s[2], B.nb[2] - 1);
+    bin_of[i] = b;
+    atomicAdd(&bin_count[b], 1);
+}
+
+constexpr int kBrickThreads = 256;
+constexpr int kBrickCand = 64;  // candidate sites staged in LDS per pass: one per lane of a wavefront
+constexpr int kBrickZ = 12;     // points per thread along z (a brick is at most 12 points wide)
+
+// charges (mode 0) or fractional dipoles (mode 1) onto the real grid (:1556-1620, :2326-2337).
+// Each thread owns a run of z points of one (x, y) column of the brick; the
+// candidates' grid origins are broadcast from the lanes that loaded them.
+__global__ void __launch_bounds__(kBrickThreads) k_spread_bricks(int mode, const double* __restrict__ pos,
+                                                                 const double* __restrict__ q,
+                                                                 const double* __restrict__ mu, ElecParams P,
+                                                                 BrickGeom B, const int* __restrict__ bin_start,
+                                                                 const int* __restrict__ members,
+                                                                 double* __restrict__ grid) {
+    // per candidate: contribution at support offset (ix, iy, iz) is
+    //   u0[iy] v0[iz] A[ix] + T0[ix] (U1[iy] v0[iz] + u0[iy] V1[iz])
+    __shared__ double cA[kBrickCand][5], cT0[kBrickCand][5], cu0[kBrickCand][5], cU1[kBrickCand][5],
+        cv0[kBrickCand][5], cV1[kBrickCand][5];
+    __shared__ int cg[kBrickCand][3];
+    __shared__ int binlist[8], binoff[9], nbins;
+    const int K[3] = {P.grid.x, P.grid.y, P.grid.z};
+    const int bz = blockIdx.x % B.nb[2], by = (blockIdx.x / B.nb[2]) % B.nb[1], bx = blockIdx.x / (B.nb[2] * B.nb[1]);
+    const int bb[3] = {bx, by, bz};
+    int lo[3], sz[3];
 #pragma unroll
-        for (int iz = 0; iz < 5; ++iz) {
-            const int gz = wrapg(g0[2] + iz, P.grid.z);
-            const double v0 = th[2][iz][0], v1 = th[2][iz][1];
-            const double val = mode == 0 ? dq * u0 * v0 * t0 : (dy * u1 * v0 + dz * u0 * v1) * t0 + dx * u0 * v0 * t1;
-            unsafeAtomicAdd(&row[gz], val);
+    for (int d = 0; d < 3; ++d) {
+        lo[d] = bb[d] * B.s[d];
+        sz[d] = bb[d] == B.nb[d] - 1 ? K[d] - lo[d] : B.s[d];
+    }
+    if (threadIdx.x == 0) {
+        int bl[3][2], nbl[3];
+        for (int d = 0; d < 3; ++d) {
+            if (B.nb[d] == 1) {
+                bl[d][0] = 0;
+                nbl[d] = 1;
+            } else {
+                bl[d][0] = (bb[d] - 1 + B.nb[d]) % B.nb[d];
+                bl[d][1] = bb[d];
+                nbl[d] = 2;
+            }
+        }
+        int m = 0;
+        binoff[0] = 0;
+        for (int a = 0; a < nbl[0]; ++a)
+            for (int b = 0; b < nbl[1]; ++b)
+                for (int c = 0; c < nbl[2]; ++c) {
+                    const int id = (bl[0][a] * B.nb[1] + bl[1][b]) * B.nb[2] + bl[2][c];
+                    binlist[m] = id;
+                    binoff[m + 1] = binoff[m] + (bin_start[id + 1] - bin_start[id]);
+                    ++m;
+                }
+        nbins = m;
+    }
+    __syncthreads();
+    const int total = binoff[nbins];
+    // this thread's column and z run
+    const int ncol = sz[0] * sz[1];
+    const int zparts = max(1, min(kBrickThreads / ncol, sz[2]));
+    const int zlen = (sz[2] + zparts - 1) / zparts;
+    const int t = threadIdx.x;
+    const bool active = t < ncol * zparts;
+    const int col = t % ncol, zp = t / ncol;
+    const int gx = lo[0] + col / sz[1], gy = lo[1] + col % sz[1];
+    const int z0 = lo[2] + zp * zlen, z1 = min(lo[2] + sz[2], z0 + zlen);
+    const int lane = threadIdx.x & 63;
+    const double sc[3] = {K[0] * P.inv_box.x, K[1] * P.inv_box.y, K[2] * P.inv_box.z};
+    double acc[kBrickZ];
+#pragma unroll
+    for (int k = 0; k < kBrickZ; ++k) acc[k] = 0.0;
+    for (int base = 0; base < total; base += kBrickCand) {
+        const int nc = min(kBrickCand, total - base);
+        if ((int)threadIdx.x < nc) {
+            const int c = base + threadIdx.x;
+            int m = 0;
+            while (c >= binoff[m + 1]) ++m;
+            const int site = members[bin_start[binlist[m]] + (c - binoff[m])];
+            int g0[3];
+            double th[3][5][4];
+            site_splines(pos, site, P, g0, th);
+            double wa, wy, wz;
+            if (mode == 0) {
+                wa = q[site];
+                wy = wz = 0.0;
+            } else {
+                wa = sc[0] * mu[3 * site];
+                wy = sc[1] * mu[3 * site + 1];
+                wz = sc[2] * mu[3 * site + 2];
+            }
+            const int u = threadIdx.x;
+#pragma unroll
+            for (int k = 0; k < 5; ++k) {
+                cA[u][k] = wa * (mode == 0 ? th[0][k][0] : th[0][k][1]);
+                cT0[u][k] = th[0][k][0];
+                cu0[u][k] = th[1][k][0];
+                cU1[u][k] = wy * th[1][k][1];
+                cv0[u][k] = th[2][k][0];
+                cV1[u][k] = wz * th[2][k][1];
+            }
+            cg[u][0] = g0[0];
+            cg[u][1] = g0[1];
+            cg[u][2] = g0[2];
+        }
+        __syncthreads();
+        int ox = 0, oy = 0, oz = 0;
+        if (lane < nc) {
+            ox = cg[lane][0];
+            oy = cg[lane][1];
+            oz = cg[lane][2];
+        }
+        for (int c = 0; c < nc; ++c) {  // uniform loop: the broadcasts run with every lane active
+            const int cx = __builtin_amdgcn_readlane(ox, c), cy = __builtin_amdgcn_readlane(oy, c),
+                      cz = __builtin_amdgcn_readlane(oz, c);
+            int ix = gx - cx;
+            ix += ix < 0 ? K[0] : 0;
+            int iy = gy - cy;
+            iy += iy < 0 ? K[1] : 0;
+            if (active && ix < 5 && iy < 5) {
+                const double a = cA[c][ix], t0 = cT0[c][ix], u0 = cu0[c][iy], U1 = cU1[c][iy];
+#pragma unroll
+                for (int k = 0; k < kBrickZ; ++k) {
+                    const int gz = z0 + k;
+                    if (gz < z1) {
+                        int iz = gz - cz;
+                        iz += iz < 0 ? K[2] : 0;
+                        if (iz < 5) {
+                            const double v0 = cv0[c][iz];
+                            acc[k] += u0 * v0 * a + t0 * (U1 * v0 + u0 * cV1[c][iz]);
+                        }
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (active) {
+#pragma unroll
+        for (int k = 0; k < kBrickZ; ++k) {
+            const int gz = z0 + k;
+            if (gz < z1) grid[((size_t)gx * K[1] + gy) * K[2] + gz] = acc[k];
         }
     }
 }
@@ -454,7 +655,7 @@ __global__ void k_influence(ElecParams P, const double* __restrict__ bx, const d
 
 // potential and derivatives to second order at each site (10 components,
 // order 000 100 010 001 200 020 002 110 101 011), :1715-1821
-__global__ void k_gather(int n, const double* __restrict__ pos, ElecParams P, const double* __restrict__ grid,
+__global__ void __launch_bounds__(64) k_gather(int n, const double* __restrict__ pos, ElecParams P, const double* __restrict__ grid,
                          double* __restrict__ phi) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -462,10 +663,12 @@ __global__ void k_gather(int n, const double* __restrict__ pos, ElecParams P, co
     double th[3][5][4];
     site_splines(pos, i, P, g0, th);
     double acc[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
     for (int iz = 0; iz < 5; ++iz) {
         const int k = wrapg(g0[2] + iz, P.grid.z);
         const double v0 = th[2][iz][0], v1 = th[2][iz][1], v2 = th[2][iz][2];
         double tu00 = 0, tu10 = 0, tu01 = 0, tu20 = 0, tu11 = 0, tu02 = 0;
+#pragma unroll
         for (int iy = 0; iy < 5; ++iy) {
             const int j = wrapg(g0[1] + iy, P.grid.y);
             const double u0 = th[1][iy][0], u1 = th[1][iy][1], u2 = th[1][iy][2];
@@ -513,34 +716,90 @@ __global__ void k_record_field(int n, ElecParams P, const double* __restrict__ p
     }
 }
 
+
 // ---------------------------------------------------------------------------
-// SOR update (:516-531) with per-block partial sums of |delta|^2
+// One induced-dipole iteration for every polarisable site (convergeInduceDipoles
+// :557-616, calculateInducedDipoleFields :438-486 / :2300-2390, updateInducedDipole
+// :516-531): real-space field from the P row and the prefactors, PME gradient of
+// the dipole potential (4 of the 10 gather components, one lane per (ix, iy)
+// column), self term, SOR step mu_out = mu_in + omega (alpha (E_fix + E_ind) - mu_in)
+// and the per-block convergence sums.  mu_in and mu_out are distinct buffers,
+// so every field is taken from the dipoles before the step, as in the reference.
 // ---------------------------------------------------------------------------
-__global__ void k_sor_update(int n, const double* __restrict__ pol, const double* __restrict__ efix_pol,
-                             const double* __restrict__ field, double omega, double* __restrict__ mu,
-                             double* __restrict__ partial) {
-    __shared__ double red[256];
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    double e = 0.0;
-    if (i < n) {
-        for (int d = 0; d < 3; ++d) {
-            const double old = mu[3 * i + d];
-            const double nv = efix_pol[3 * i + d] + field[3 * i + d] * pol[i];
-            const double delta = nv - old;
-            mu[3 * i + d] = old + delta * omega;
-            e += delta * delta;
+template <int G>
+__global__ void __launch_bounds__(256) k_iterate(int npol, const int* __restrict__ polidx, const double* __restrict__ pos,
+                                                 ElecParams P, int capP, const int* __restrict__ nbP,
+                                                 const int* __restrict__ cntP, const double2* __restrict__ pf,
+                                                 const double* __restrict__ mu_in, double* __restrict__ mu_out,
+                                                 const double* __restrict__ efix_pol, const double* __restrict__ pol,
+                                                 const double* __restrict__ grid, double self_term, double omega,
+                                                 double* __restrict__ partial) {
+    __shared__ double red[256 / G];
+    const int g = (blockIdx.x * blockDim.x + threadIdx.x) / G, lane = threadIdx.x % G;
+    double e2 = 0.0;
+    if (g < npol) {
+        const int i = polidx[g];
+        const double xi = pos[3 * i], yi = pos[3 * i + 1], zi = pos[3 * i + 2];
+        double ex = 0, ey = 0, ez = 0;
+        const int c = min(cntP[i], capP);
+        const int* row = nbP + (size_t)i * capP;
+        const double2* prow = pf + (size_t)i * capP;
+        for (int k = lane; k < c; k += G) {
+            const int j = row[k];
+            double dx = pos[3 * j] - xi, dy = pos[3 * j + 1] - yi, dz = pos[3 * j + 2] - zi;
+            min_image(dx, dy, dz, P);
+            const double2 p = prow[k];
+            const double mx = mu_in[3 * j], my = mu_in[3 * j + 1], mz = mu_in[3 * j + 2];
+            const double dur = (mx * dx + my * dy + mz * dz) * p.y;
+            ex += dx * dur + mx * p.x;
+            ey += dy * dur + my * p.x;
+            ez += dz * dur + mz * p.x;
+        }
+        if (P.pme && lane < 25) {
+            int g0[3];
+            double th[3][5][4];
+            site_splines(pos, i, P, g0, th);
+            const int a = lane / 5, b = lane % 5;
+            const int ii = wrapg(g0[0] + a, P.grid.x), jj = wrapg(g0[1] + b, P.grid.y);
+            const double* line = grid + ((size_t)ii * P.grid.y + jj) * P.grid.z;
+            double t0 = 0, t1 = 0;
+#pragma unroll
+            for (int iz = 0; iz < 5; ++iz) {
+                const double gv = line[wrapg(g0[2] + iz, P.grid.z)];
+                t0 += gv * th[2][iz][0];
+                t1 += gv * th[2][iz][1];
+            }
+            const double x0 = pick5(th[0], a, 0), x1 = pick5(th[0], a, 1), y0 = pick5(th[1], b, 0),
+                         y1 = pick5(th[1], b, 1);
+            ex -= P.grid.x * P.inv_box.x * (x1 * y0 * t0);
+            ey -= P.grid.y * P.inv_box.y * (x0 * y1 * t0);
+            ez -= P.grid.z * P.inv_box.z * (x0 * y0 * t1);
+        }
+        ex = group_sum<G>(ex);
+        ey = group_sum<G>(ey);
+        ez = group_sum<G>(ez);
+        if (lane == 0) {
+            const double m[3] = {mu_in[3 * i], mu_in[3 * i + 1], mu_in[3 * i + 2]};
+            const double f[3] = {ex + m[0] * self_term, ey + m[1] * self_term, ez + m[2] * self_term};
+            const double a = pol[i];
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                const double delta = efix_pol[3 * i + d] + f[d] * a - m[d];
+                mu_out[3 * i + d] = m[d] + delta * omega;
+                e2 += delta * delta;
+            }
         }
     }
-    red[threadIdx.x] = e;
+    if (lane == 0) red[threadIdx.x / G] = e2;
     __syncthreads();
-    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-        if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
-        __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = 0.0;
+        for (int k = 0; k < 256 / G; ++k) s += red[k];
+        partial[blockIdx.x] = s;
     }
-    if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
 }
 
-// epsilon = omega * debye * sqrt(sum / n) from the block partials, in a fixed order (:504-511, :588)
+// epsilon = omega * debye * sqrt(sum / n) from the block partials, in block order (:504-511, :588)
 __global__ void k_eps_finish(int nblocks, int n, const double* __restrict__ partial, double* __restrict__ eps_out) {
     __shared__ double red[256];
     double s = 0.0;
@@ -571,16 +830,19 @@ __global__ void k_scale_copy(int n3, const double* __restrict__ a, const double*
 // Each directed pair (i, j) adds its share to site i only; energy is halved.
 // ---------------------------------------------------------------------------
 template <int G>
-__global__ void k_elec_energy(int n, const double* __restrict__ pos, const SiteParams S, ElecParams P,
-                              const int* __restrict__ nb_start, const int* __restrict__ nb, const double* __restrict__ mu,
-                              double* __restrict__ force, double* __restrict__ V, double* __restrict__ eblk) {
+__global__ void __launch_bounds__(256) k_elec_energy(int n, const double* __restrict__ pos, const SiteParams S,
+                                                     ElecParams P, int capP, const int* __restrict__ nbP,
+                                                     const int* __restrict__ cntP, int capQ,
+                                                     const int* __restrict__ nbQ, const int* __restrict__ cntQ,
+                                                     const double* __restrict__ mu, double* __restrict__ force,
+                                                     double* __restrict__ V, double* __restrict__ eblk) {
     const int i = (blockIdx.x * blockDim.x + threadIdx.x) / G, lane = threadIdx.x % G;
     double en = 0, fx = 0, fy = 0, fz = 0, vi = 0;
     if (i < n) {
         const double xi = pos[3 * i], yi = pos[3 * i + 1], zi = pos[3 * i + 2];
         const double ci = S.q[i];
         const double mix = mu[3 * i], miy = mu[3 * i + 1], miz = mu[3 * i + 2];
-        for_neighbours<G>(i, lane, P, nb_start, nb, [&](int j, int) {
+        for_rows<G>(i, lane, capP, nbP, cntP, capQ, nbQ, cntQ, [&](int j) {
             double dx = pos[3 * j] - xi, dy = pos[3 * j + 1] - yi, dz = pos[3 * j + 2] - zi;
             min_image(dx, dy, dz, P);
             const double r2 = dx * dx + dy * dy + dz * dz, r = sqrt(r2);
@@ -663,6 +925,7 @@ __global__ void k_elec_energy(int n, const double* __restrict__ pos, const SiteP
     }
     if (threadIdx.x == 0) eblk[blockIdx.x] = red[0];
 }
+
 
 // reciprocal-space and self terms per site (:2113-2265, :2472-2508)
 __global__ void k_recip_site(int n, const SiteParams S, ElecParams P, const double* __restrict__ phi_fix,
@@ -756,11 +1019,13 @@ __global__ void k_potential_at_points(int npts, const double* __restrict__ pts, 
     out[k] = v * el_electric;
 }
 
+
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
 static inline int nblk(long long n, int b) { return (int)((n + b - 1) / b); }
-constexpr int kG = 16;  // lanes per site in the pair kernels
+constexpr int kG = 16;   // lanes per site: list build, fixed field, energy
+constexpr int kGI = 32;  // lanes per site in the iteration (>= 25 for the PME gather)
 
 void launch_site_charges_copy(hipStream_t s, int n, const double* q0, double* q) {
     hipLaunchKernelGGL(k_site_charges_copy, nblk(n, 256), 256, 0, s, n, q0, q);
@@ -768,29 +1033,38 @@ void launch_site_charges_copy(hipStream_t s, int n, const double* q0, double* q)
 void launch_site_cells(hipStream_t s, int n, const double* pos, ElecParams P, int3 nc, int* cell_of, int* cell_count) {
     hipLaunchKernelGGL(k_site_cells, nblk(n, 256), 256, 0, s, n, pos, P, nc, cell_of, cell_count);
 }
-void launch_site_cell_fill(hipStream_t s, int n, const int* cell_of, const int* cell_start, int* cell_fill, int* members) {
-    hipLaunchKernelGGL(k_site_cell_fill, nblk(n, 256), 256, 0, s, n, cell_of, cell_start, cell_fill, members);
+void launch_bucket_fill(hipStream_t s, int n, const int* bucket_of, const int* bucket_start, int* bucket_fill,
+                        int* members) {
+    hipLaunchKernelGGL(k_bucket_fill, nblk(n, 256), 256, 0, s, n, bucket_of, bucket_start, bucket_fill, members);
 }
-void launch_site_pairs(hipStream_t s, int n, const double* pos, ElecParams P, int3 nc, int use_cells, const int* cell_of,
-                       const int* cell_start, const int* members, int* count, const int* start, int* out) {
-    hipLaunchKernelGGL(k_site_pairs, nblk(n, 64), 64, 0, s, n, pos, P, nc, use_cells, cell_of, cell_start, members, count,
-                       start, out);
+void launch_sort_segments(hipStream_t s, int nseg, const int* start, int* data) {
+    hipLaunchKernelGGL(k_sort_segments, nblk(nseg, 64), 64, 0, s, nseg, start, data);
 }
-void launch_pair_prefactors(hipStream_t s, int n, const double* pos, SiteParams S, ElecParams P, const int* nb_start,
-                            const int* nb, double2* pf) {
-    hipLaunchKernelGGL(k_pair_prefactors, nblk(n, 64), 64, 0, s, n, pos, S, P, nb_start, nb, pf);
+void launch_site_lists(hipStream_t s, int n, const double* pos, const double* pol, ElecParams P, int3 nc, int reach,
+                       int use_cells, const int* cell_of, const int* cell_start, const int* members, int capP, int capQ,
+                       int* nbP, int* cntP, int* nbQ, int* cntQ, int* maxcnt) {
+    hipLaunchKernelGGL(k_site_lists<kG>, nblk((long long)n * kG, 256), 256, 0, s, n, pos, pol, P, nc, reach, use_cells,
+                       cell_of, cell_start, members, capP, capQ, nbP, cntP, nbQ, cntQ, maxcnt);
 }
-void launch_fixed_field(hipStream_t s, int n, const double* pos, SiteParams S, ElecParams P, const int* nb_start,
-                        const int* nb, double* field) {
-    hipLaunchKernelGGL(k_fixed_field<kG>, nblk((long long)n * kG, 256), 256, 0, s, n, pos, S, P, nb_start, nb, field);
+void launch_prefactors(hipStream_t s, int npol, const int* polidx, const double* pos, SiteParams S, ElecParams P,
+                       int capP, const int* nbP, const int* cntP, double2* pf) {
+    if (npol > 0)
+        hipLaunchKernelGGL(k_prefactors<kG>, nblk((long long)npol * kG, 256), 256, 0, s, npol, polidx, pos, S, P, capP,
+                           nbP, cntP, pf);
 }
-void launch_induced_field(hipStream_t s, int n, const double* pos, ElecParams P, const int* nb_start, const int* nb,
-                          const double2* pf, const double* mu, double* field) {
-    hipLaunchKernelGGL(k_induced_field<kG>, nblk((long long)n * kG, 256), 256, 0, s, n, pos, P, nb_start, nb, pf, mu, field);
+void launch_fixed_field(hipStream_t s, int npol, const int* polidx, const double* pos, SiteParams S, ElecParams P,
+                        const ElecRows& R, double* field) {
+    if (npol > 0)
+        hipLaunchKernelGGL(k_fixed_field<kG>, nblk((long long)npol * kG, 256), 256, 0, s, npol, polidx, pos, S, P,
+                           R.capP, R.nbP, R.cntP, R.capQ, R.nbQ, R.cntQ, field);
 }
-void launch_spread(hipStream_t s, int n, int mode, const double* pos, const double* q, const double* mu, ElecParams P,
-                   double* grid) {
-    hipLaunchKernelGGL(k_spread, nblk((long long)n * 5, 256), 256, 0, s, n, mode, pos, q, mu, P, grid);
+void launch_site_bins(hipStream_t s, int n, const double* pos, ElecParams P, BrickGeom B, int* bin_of, int* bin_count) {
+    hipLaunchKernelGGL(k_site_bins, nblk(n, 256), 256, 0, s, n, pos, P, B, bin_of, bin_count);
+}
+void launch_spread_bricks(hipStream_t s, int mode, const double* pos, const double* q, const double* mu, ElecParams P,
+                          BrickGeom B, const int* bin_start, const int* members, double* grid) {
+    hipLaunchKernelGGL(k_spread_bricks, B.nb[0] * B.nb[1] * B.nb[2], kBrickThreads, 0, s, mode, pos, q, mu, P, B,
+                       bin_start, members, grid);
 }
 void launch_convolve(hipStream_t s, size_t nhalf, const double* influence, double2* spec) {
     hipLaunchKernelGGL(k_convolve, nblk(nhalf, 256), 256, 0, s, nhalf, influence, spec);
@@ -806,23 +1080,25 @@ void launch_record_field(hipStream_t s, int n, ElecParams P, const double* phi, 
                          const double* mu, double* field) {
     hipLaunchKernelGGL(k_record_field, nblk(n, 256), 256, 0, s, n, P, phi, accumulate, self_term, mu, field);
 }
-int sor_blocks(int n) { return nblk(n, 256); }
-void launch_sor_update(hipStream_t s, int n, const double* pol, const double* efix_pol, const double* field, double omega,
-                       double* mu, double* partial) {
-    hipLaunchKernelGGL(k_sor_update, nblk(n, 256), 256, 0, s, n, pol, efix_pol, field, omega, mu, partial);
-}
-void launch_eps_finish(hipStream_t s, int nblocks, int n, const double* partial, double* eps_out) {
-    hipLaunchKernelGGL(k_eps_finish, 1, 256, 0, s, nblocks, n, partial, eps_out);
+int iterate_blocks(int npol) { return std::max(1, nblk((long long)npol * kGI, 256)); }
+void launch_iterate(hipStream_t s, int npol, const int* polidx, const double* pos, ElecParams P, const ElecRows& R,
+                    const double* mu_in, double* mu_out, const double* efix_pol, const double* pol, const double* grid,
+                    double self_term, double omega, int n_all, double* partial, double* eps) {
+    const int nb = iterate_blocks(npol);
+    hipLaunchKernelGGL(k_iterate<kGI>, nb, 256, 0, s, npol, polidx, pos, P, R.capP, R.nbP, R.cntP, R.pf, mu_in, mu_out,
+                       efix_pol, pol, grid, self_term, omega, partial);
+    hipLaunchKernelGGL(k_eps_finish, 1, 256, 0, s, nb, n_all, partial, eps);
 }
 void launch_scale_copy(hipStream_t s, int n, const double* a, const double* pol, double* out, double* out2) {
     hipLaunchKernelGGL(k_scale_copy, nblk(3 * n, 256), 256, 0, s, 3 * n, a, pol, out, out2);
 }
 int energy_blocks(int n) { return nblk((long long)n * kG, 256); }
-void launch_elec_energy(hipStream_t s, int n, const double* pos, SiteParams S, ElecParams P, const int* nb_start,
-                        const int* nb, const double* mu, double* force, double* V, double* eblk) {
-    hipLaunchKernelGGL(k_elec_energy<kG>, nblk((long long)n * kG, 256), 256, 0, s, n, pos, S, P, nb_start, nb, mu, force,
-                       V, eblk);
+void launch_elec_energy(hipStream_t s, int n, const double* pos, SiteParams S, ElecParams P, const ElecRows& R,
+                        const double* mu, double* force, double* V, double* eblk) {
+    hipLaunchKernelGGL(k_elec_energy<kG>, energy_blocks(n), 256, 0, s, n, pos, S, P, R.capP, R.nbP, R.cntP, R.capQ,
+                       R.nbQ, R.cntQ, mu, force, V, eblk);
 }
+int sor_blocks(int n) { return nblk(n, 256); }
 void launch_recip_site(hipStream_t s, int n, SiteParams S, ElecParams P, const double* phi_fix, const double* phi_ind,
                        const double* mu, double* force, double* V, double* eblk) {
     hipLaunchKernelGGL(k_recip_site, nblk(n, 256), 256, 0, s, n, S, P, phi_fix, phi_ind, mu, force, V, eblk);
