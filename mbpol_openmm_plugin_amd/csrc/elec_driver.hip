@@ -48,7 +48,11 @@ struct Elec {
     // PME
     DevBuf grid, spec, influence, bx, by, bz;
     BrickGeom B{};
-    DevBuf bin_of, bin_count, bin_start, bin_fill, bin_members;
+    struct Bins {
+        DevBuf of, count, start, fill, members;
+    } binq;  // sites with a charge, by grid brick
+    DevBuf g0tab, sptab;  // per-site spline table
+    DevBuf rowcount, rowstart, rowfill, esite, ecoef;  // dipole-spreading table in grid-point order
     hipfftHandle plan_f = 0, plan_b = 0;
     bool plans = false;
     double* h_eps = nullptr;  // pinned: [0] epsilon, ints from [8] on
@@ -266,11 +270,14 @@ int elec_create(mbpol_context* ctx) {
         if (int e = up(E->bz, bspline_moduli(grid[2]))) return e;
         for (int d = 0; d < 3; ++d) brick_axis(grid[d], E->B.s[d], E->B.nb[d]);
         const int nbins = E->B.nb[0] * E->B.nb[1] * E->B.nb[2];
-        HIP_TRY(E->bin_of.ensure(sizeof(int) * std::max(n, 1)));
-        HIP_TRY(E->bin_members.ensure(sizeof(int) * std::max(n, 1)));
-        HIP_TRY(E->bin_count.ensure(sizeof(int) * (nbins + 1)));
-        HIP_TRY(E->bin_start.ensure(sizeof(int) * (nbins + 1)));
-        HIP_TRY(E->bin_fill.ensure(sizeof(int) * (nbins + 1)));
+        HIP_TRY(E->binq.of.ensure(sizeof(int) * std::max(n, 1)));
+        HIP_TRY(E->binq.members.ensure(sizeof(int) * std::max(n, 1)));
+        HIP_TRY(E->binq.count.ensure(sizeof(int) * (nbins + 1)));
+        HIP_TRY(E->binq.start.ensure(sizeof(int) * (nbins + 1)));
+        HIP_TRY(E->binq.fill.ensure(sizeof(int) * (nbins + 1)));
+        HIP_TRY(E->g0tab.ensure(sizeof(int) * 3 * std::max(n, 1)));
+        HIP_TRY(E->sptab.ensure(sizeof(double) * 30 * std::max(n, 1)));
+        for (DevBuf* b : {&E->rowcount, &E->rowstart, &E->rowfill}) HIP_TRY(b->ensure(sizeof(int) * (nreal + 1)));
         FFT_TRY(hipfftPlan3d(&E->plan_f, grid[0], grid[1], grid[2], HIPFFT_D2Z));
         FFT_TRY(hipfftPlan3d(&E->plan_b, grid[0], grid[1], grid[2], HIPFFT_Z2D));
         E->plans = true;
@@ -336,13 +343,27 @@ static int build_rows(mbpol_context* ctx) {
                       E->cntP.as<int>(), E->pf.as<double2>());
     if (E->P.pme) {
         const int nbins = E->B.nb[0] * E->B.nb[1] * E->B.nb[2];
-        HIP_TRY(hipMemsetAsync(E->bin_count.p, 0, sizeof(int) * (nbins + 1), st));
-        HIP_TRY(hipMemsetAsync(E->bin_fill.p, 0, sizeof(int) * (nbins + 1), st));
-        launch_site_bins(st, n, pos, E->P, E->B, E->bin_of.as<int>(), E->bin_count.as<int>());
-        if (int e = scan_i32(ctx, E->bin_count.as<int>(), E->bin_start.as<int>(), nbins + 1)) return e;
-        launch_bucket_fill(st, n, E->bin_of.as<int>(), E->bin_start.as<int>(), E->bin_fill.as<int>(),
-                           E->bin_members.as<int>());
-        launch_sort_segments(st, nbins, E->bin_start.as<int>(), E->bin_members.as<int>());
+        launch_site_spline_table(st, n, pos, E->P, E->g0tab.as<int>(), E->sptab.as<double>());
+        // dipole-spreading table
+        const size_t ngrid = (size_t)E->P.grid.x * E->P.grid.y * E->P.grid.z;
+        const size_t nent = (size_t)E->npol * 125;
+        HIP_TRY(E->esite.ensure(sizeof(int) * std::max<size_t>(nent, 1)));
+        HIP_TRY(E->ecoef.ensure(sizeof(double) * 3 * std::max<size_t>(nent, 1)));
+        HIP_TRY(hipMemsetAsync(E->rowcount.p, 0, sizeof(int) * (ngrid + 1), st));
+        HIP_TRY(hipMemsetAsync(E->rowfill.p, 0, sizeof(int) * (ngrid + 1), st));
+        launch_csr_count(st, E->npol, E->polidx.as<int>(), E->g0tab.as<int>(), E->P, E->rowcount.as<int>());
+        if (int e = scan_i32(ctx, E->rowcount.as<int>(), E->rowstart.as<int>(), (int)ngrid + 1)) return e;
+        launch_csr_fill(st, E->npol, E->polidx.as<int>(), E->g0tab.as<int>(), E->sptab.as<double>(), E->P,
+                        E->rowstart.as<int>(), E->rowfill.as<int>(), E->esite.as<int>(), E->ecoef.as<double>());
+        launch_csr_sort(st, ngrid, E->rowstart.as<int>(), E->esite.as<int>(), E->ecoef.as<double>());
+        // charged sites by brick, for the charge spreading
+        Elec::Bins& b = E->binq;
+        HIP_TRY(hipMemsetAsync(b.count.p, 0, sizeof(int) * (nbins + 1), st));
+        HIP_TRY(hipMemsetAsync(b.fill.p, 0, sizeof(int) * (nbins + 1), st));
+        launch_site_bins(st, n, pos, E->q.as<double>(), E->P, E->B, b.of.as<int>(), b.count.as<int>());
+        if (int e = scan_i32(ctx, b.count.as<int>(), b.start.as<int>(), nbins + 1)) return e;
+        launch_bucket_fill(st, n, b.of.as<int>(), b.start.as<int>(), b.fill.as<int>(), b.members.as<int>());
+        launch_sort_segments(st, nbins, b.start.as<int>(), b.members.as<int>());
     }
     return MBPOL_OK;
 }
@@ -353,8 +374,12 @@ static int pme_grid(mbpol_context* ctx, int mode, const double* mu) {
     hipStream_t st = ctx->stream;
     const ElecParams& P = E->P;
     const size_t nhalf = (size_t)P.grid.x * P.grid.y * (P.grid.z / 2 + 1);
-    launch_spread_bricks(st, mode, ctx->pos.as<double>(), E->q.as<double>(), mu, P, E->B, E->bin_start.as<int>(),
-                         E->bin_members.as<int>(), E->grid.as<double>());
+    if (mode == 0)
+        launch_spread_bricks(st, mode, E->q.as<double>(), mu, P, E->B, E->binq.start.as<int>(),
+                             E->binq.members.as<int>(), E->g0tab.as<int>(), E->sptab.as<double>(), E->grid.as<double>());
+    else
+        launch_spread_csr(st, (size_t)P.grid.x * P.grid.y * P.grid.z, E->rowstart.as<int>(), E->esite.as<int>(),
+                          E->ecoef.as<double>(), mu, E->grid.as<double>());
     FFT_TRY(hipfftExecD2Z(E->plan_f, E->grid.as<double>(), reinterpret_cast<hipfftDoubleComplex*>(E->spec.p)));
     launch_convolve(st, nhalf, E->influence.as<double>(), E->spec.as<double2>());
     FFT_TRY(hipfftExecZ2D(E->plan_b, reinterpret_cast<hipfftDoubleComplex*>(E->spec.p), E->grid.as<double>()));

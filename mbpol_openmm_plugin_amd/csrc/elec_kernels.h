@@ -51,9 +51,18 @@ void launch_prefactors(hipStream_t s, int npol, const int* polidx, const double*
                        int capP, const int* nbP, const int* cntP, double2* pf);
 void launch_fixed_field(hipStream_t s, int npol, const int* polidx, const double* pos, SiteParams S, ElecParams P,
                         const ElecRows& R, double* field);
-void launch_site_bins(hipStream_t s, int n, const double* pos, ElecParams P, BrickGeom B, int* bin_of, int* bin_count);
-void launch_spread_bricks(hipStream_t s, int mode, const double* pos, const double* q, const double* mu, ElecParams P,
-                          BrickGeom B, const int* bin_start, const int* members, double* grid);
+void launch_site_bins(hipStream_t s, int n, const double* pos, const double* weight, ElecParams P, BrickGeom B,
+                      int* bin_of, int* bin_count);
+void launch_site_spline_table(hipStream_t s, int n, const double* pos, ElecParams P, int* g0tab, double* sptab);
+void launch_spread_bricks(hipStream_t s, int mode, const double* q, const double* mu, ElecParams P, BrickGeom B,
+                          const int* bin_start, const int* members, const int* g0tab, const double* sptab,
+                          double* grid);
+void launch_csr_count(hipStream_t s, int npol, const int* polidx, const int* g0tab, ElecParams P, int* rowcount);
+void launch_csr_fill(hipStream_t s, int npol, const int* polidx, const int* g0tab, const double* sptab, ElecParams P,
+                     const int* rowstart, int* fill, int* esite, double* ecoef);
+void launch_csr_sort(hipStream_t s, size_t ngrid, const int* rowstart, int* esite, double* ecoef);
+void launch_spread_csr(hipStream_t s, size_t ngrid, const int* rowstart, const int* esite, const double* ecoef,
+                       const double* mu, double* grid);
 void launch_convolve(hipStream_t s, size_t nhalf, const double* influence, double2* spec);
 void launch_influence(hipStream_t s, ElecParams P, const double* bx, const double* by, const double* bz, double* out);
 void launch_gather(hipStream_t s, int n, const double* pos, ElecParams P, const double* grid, double* phi);

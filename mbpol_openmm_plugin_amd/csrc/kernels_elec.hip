@@ -170,6 +170,7 @@ __global__ void k_bucket_fill(int n, const int* __restrict__ bucket_of, const in
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int c = bucket_of[i];
+    if (c < 0) return;  // item without a bucket
     members[bucket_start[c] + atomicAdd(&bucket_fill[c], 1)] = i;
 }
 
@@ -464,10 +465,15 @@ __device__ __forceinline__ double pick5(const double (&th)[5][4], int k, int c) 
 // writes every point of it exactly once: no atomics, no clearing pass, and a
 // summation order fixed by the (sorted) bin contents.
 // ---------------------------------------------------------------------------
-__global__ void k_site_bins(int n, const double* __restrict__ pos, ElecParams P, BrickGeom B, int* __restrict__ bin_of,
-                            int* __restrict__ bin_count) {
+// sites whose weight (charge or polarisability) is zero add nothing to the grid and get no bin
+__global__ void k_site_bins(int n, const double* __restrict__ pos, const double* __restrict__ weight, ElecParams P,
+                            BrickGeom B, int* __restrict__ bin_of, int* __restrict__ bin_count) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    if (weight[i] == 0.0) {
+        bin_of[i] = -1;
+        return;
+    }
     int g0[3];
     double w[3];
     site_origin(pos, i, P, g0, w);
@@ -477,24 +483,49 @@ __global__ void k_site_bins(int n, const double* __restrict__ pos, ElecParams P,
     atomicAdd(&bin_count[b], 1);
 }
 
-constexpr int kBrickThreads = 256;
-constexpr int kBrickCand = 64;  // candidate sites staged in LDS per pass: one per lane of a wavefront
-constexpr int kBrickZ = 12;     // points per thread along z (a brick is at most 12 points wide)
+// Per-site spline table for the spreading kernel, computed once per
+// evaluation (positions do not change during the dipole iteration):
+// g0[3 i + d] = grid origin, sp[30 i + 10 d + k] = value (k < 5) and first
+// derivative (k >= 5) of the order-5 B-spline along axis d.
+__global__ void k_site_spline_table(int n, const double* __restrict__ pos, ElecParams P, int* __restrict__ g0tab,
+                                    double* __restrict__ sptab) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int g0[3];
+    double th[3][5][4];
+    site_splines(pos, i, P, g0, th);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        g0tab[3 * i + d] = g0[d];
+#pragma unroll
+        for (int k = 0; k < 5; ++k) {
+            sptab[30 * i + 10 * d + k] = th[d][k][0];
+            sptab[30 * i + 10 * d + 5 + k] = th[d][k][1];
+        }
+    }
+}
+
+constexpr int kBrickThreads = 128;
+constexpr int kBrickCand = 64;  // candidate sites staged in LDS per pass
+constexpr int kBrickZ = 12;     // a brick is at most 12 points wide along every axis
 
 // charges (mode 0) or fractional dipoles (mode 1) onto the real grid (:1556-1620, :2326-2337).
-// Each thread owns a run of z points of one (x, y) column of the brick; the
-// candidates' grid origins are broadcast from the lanes that loaded them.
-__global__ void __launch_bounds__(kBrickThreads) k_spread_bricks(int mode, const double* __restrict__ pos,
-                                                                 const double* __restrict__ q,
+// Each thread owns whole (x, y) columns of the brick (at most two: a brick
+// face has at most 144 columns); the candidates' grid origins are broadcast
+// from the lanes that staged them, so a column tests a candidate with scalar
+// operands and reads LDS only for the candidates that cover it.
+__global__ void __launch_bounds__(kBrickThreads) k_spread_bricks(int mode, const double* __restrict__ q,
                                                                  const double* __restrict__ mu, ElecParams P,
                                                                  BrickGeom B, const int* __restrict__ bin_start,
                                                                  const int* __restrict__ members,
+                                                                 const int* __restrict__ g0tab,
+                                                                 const double* __restrict__ sptab,
                                                                  double* __restrict__ grid) {
     // per candidate: contribution at support offset (ix, iy, iz) is
     //   u0[iy] v0[iz] A[ix] + T0[ix] (U1[iy] v0[iz] + u0[iy] V1[iz])
-    __shared__ double cA[kBrickCand][5], cT0[kBrickCand][5], cu0[kBrickCand][5], cU1[kBrickCand][5],
-        cv0[kBrickCand][5], cV1[kBrickCand][5];
+    __shared__ double cs[kBrickCand][30];  // A, T0, u0, U1, v0, V1 (5 each)
     __shared__ int cg[kBrickCand][3];
+    __shared__ int csite[kBrickCand];
     __shared__ int binlist[8], binoff[9], nbins;
     const int K[3] = {P.grid.x, P.grid.y, P.grid.z};
     const int bz = blockIdx.x % B.nb[2], by = (blockIdx.x / B.nb[2]) % B.nb[1], bx = blockIdx.x / (B.nb[2] * B.nb[1]);
@@ -531,20 +562,40 @@ __global__ void __launch_bounds__(kBrickThreads) k_spread_bricks(int mode, const
     }
     __syncthreads();
     const int total = binoff[nbins];
-    // this thread's column and z run
     const int ncol = sz[0] * sz[1];
-    const int zparts = max(1, min(kBrickThreads / ncol, sz[2]));
-    const int zlen = (sz[2] + zparts - 1) / zparts;
     const int t = threadIdx.x;
-    const bool active = t < ncol * zparts;
-    const int col = t % ncol, zp = t / ncol;
-    const int gx = lo[0] + col / sz[1], gy = lo[1] + col % sz[1];
-    const int z0 = lo[2] + zp * zlen, z1 = min(lo[2] + sz[2], z0 + zlen);
+    const bool has0 = t < ncol, has1 = t + kBrickThreads < ncol;
+    const int gx0 = lo[0] + t / sz[1], gy0 = lo[1] + t % sz[1];
+    const int t1 = t + kBrickThreads;
+    const int gx1 = lo[0] + t1 / sz[1], gy1 = lo[1] + t1 % sz[1];
     const int lane = threadIdx.x & 63;
     const double sc[3] = {K[0] * P.inv_box.x, K[1] * P.inv_box.y, K[2] * P.inv_box.z};
-    double acc[kBrickZ];
+    double acc0[kBrickZ], acc1[kBrickZ];
 #pragma unroll
-    for (int k = 0; k < kBrickZ; ++k) acc[k] = 0.0;
+    for (int k = 0; k < kBrickZ; ++k) acc0[k] = acc1[k] = 0.0;
+
+    // add candidate c to the column (gx, gy) if its support covers it
+    auto column = [&](int c, int cx, int cy, int cz, int gx, int gy, double (&acc)[kBrickZ]) {
+        int ix = gx - cx;
+        ix += ix < 0 ? K[0] : 0;
+        int iy = gy - cy;
+        iy += iy < 0 ? K[1] : 0;
+        if (ix < 5 && iy < 5) {
+            const double a = cs[c][ix], t0 = cs[c][5 + ix], u0 = cs[c][10 + iy], U1 = cs[c][15 + iy];
+            int iz = lo[2] - cz;
+            iz += iz < 0 ? K[2] : 0;
+#pragma unroll
+            for (int k = 0; k < kBrickZ; ++k) {
+                // iz walks the column; it leaves [0, 5) for good once it passes 4
+                const int z = iz + k >= K[2] ? iz + k - K[2] : iz + k;
+                if (k < sz[2] && z < 5) {
+                    const double v0 = cs[c][20 + z];
+                    acc[k] += u0 * v0 * a + t0 * (U1 * v0 + u0 * cs[c][25 + z]);
+                }
+            }
+        }
+    };
+
     for (int base = 0; base < total; base += kBrickCand) {
         const int nc = min(kBrickCand, total - base);
         if ((int)threadIdx.x < nc) {
@@ -552,72 +603,134 @@ __global__ void __launch_bounds__(kBrickThreads) k_spread_bricks(int mode, const
             int m = 0;
             while (c >= binoff[m + 1]) ++m;
             const int site = members[bin_start[binlist[m]] + (c - binoff[m])];
-            int g0[3];
-            double th[3][5][4];
-            site_splines(pos, site, P, g0, th);
-            double wa, wy, wz;
-            if (mode == 0) {
-                wa = q[site];
-                wy = wz = 0.0;
-            } else {
-                wa = sc[0] * mu[3 * site];
-                wy = sc[1] * mu[3 * site + 1];
-                wz = sc[2] * mu[3 * site + 2];
-            }
-            const int u = threadIdx.x;
-#pragma unroll
-            for (int k = 0; k < 5; ++k) {
-                cA[u][k] = wa * (mode == 0 ? th[0][k][0] : th[0][k][1]);
-                cT0[u][k] = th[0][k][0];
-                cu0[u][k] = th[1][k][0];
-                cU1[u][k] = wy * th[1][k][1];
-                cv0[u][k] = th[2][k][0];
-                cV1[u][k] = wz * th[2][k][1];
-            }
-            cg[u][0] = g0[0];
-            cg[u][1] = g0[1];
-            cg[u][2] = g0[2];
+            csite[threadIdx.x] = site;
+            cg[threadIdx.x][0] = g0tab[3 * site];
+            cg[threadIdx.x][1] = g0tab[3 * site + 1];
+            cg[threadIdx.x][2] = g0tab[3 * site + 2];
         }
         __syncthreads();
+        // stage the weighted spline rows: element e of candidate c
+        for (int e = threadIdx.x; e < nc * 30; e += kBrickThreads) {
+            const int c = e / 30, k = e % 30, site = csite[c];
+            const double* sp = sptab + 30 * site;
+            const int f = k / 5, r = k % 5;
+            double v;
+            switch (f) {
+                case 0: v = mode == 0 ? q[site] * sp[r] : sc[0] * mu[3 * site] * sp[5 + r]; break;  // A
+                case 1: v = sp[r]; break;                                                            // T0
+                case 2: v = sp[10 + r]; break;                                                       // u0
+                case 3: v = mode == 0 ? 0.0 : sc[1] * mu[3 * site + 1] * sp[15 + r]; break;         // U1
+                case 4: v = sp[20 + r]; break;                                                       // v0
+                default: v = mode == 0 ? 0.0 : sc[2] * mu[3 * site + 2] * sp[25 + r]; break;        // V1
+            }
+            cs[c][k] = v;
+        }
         int ox = 0, oy = 0, oz = 0;
         if (lane < nc) {
             ox = cg[lane][0];
             oy = cg[lane][1];
             oz = cg[lane][2];
         }
+        __syncthreads();
         for (int c = 0; c < nc; ++c) {  // uniform loop: the broadcasts run with every lane active
             const int cx = __builtin_amdgcn_readlane(ox, c), cy = __builtin_amdgcn_readlane(oy, c),
                       cz = __builtin_amdgcn_readlane(oz, c);
-            int ix = gx - cx;
-            ix += ix < 0 ? K[0] : 0;
-            int iy = gy - cy;
-            iy += iy < 0 ? K[1] : 0;
-            if (active && ix < 5 && iy < 5) {
-                const double a = cA[c][ix], t0 = cT0[c][ix], u0 = cu0[c][iy], U1 = cU1[c][iy];
-#pragma unroll
-                for (int k = 0; k < kBrickZ; ++k) {
-                    const int gz = z0 + k;
-                    if (gz < z1) {
-                        int iz = gz - cz;
-                        iz += iz < 0 ? K[2] : 0;
-                        if (iz < 5) {
-                            const double v0 = cv0[c][iz];
-                            acc[k] += u0 * v0 * a + t0 * (U1 * v0 + u0 * cV1[c][iz]);
-                        }
-                    }
-                }
-            }
+            if (has0) column(c, cx, cy, cz, gx0, gy0, acc0);
+            if (has1) column(c, cx, cy, cz, gx1, gy1, acc1);
         }
         __syncthreads();
     }
-    if (active) {
 #pragma unroll
-        for (int k = 0; k < kBrickZ; ++k) {
-            const int gz = z0 + k;
-            if (gz < z1) grid[((size_t)gx * K[1] + gy) * K[2] + gz] = acc[k];
+    for (int k = 0; k < kBrickZ; ++k) {
+        if (k < sz[2]) {
+            if (has0) grid[((size_t)gx0 * K[1] + gy0) * K[2] + lo[2] + k] = acc0[k];
+            if (has1) grid[((size_t)gx1 * K[1] + gy1) * K[2] + lo[2] + k] = acc1[k];
         }
     }
 }
+
+// ---------------------------------------------------------------------------
+// Dipole spreading as a sparse product.  With the positions fixed for a whole
+// evaluation, the fractional-dipole grid of every iteration is linear in the
+// dipoles: grid[g] = sum over the (site, support point) entries of g of
+// mu_site . c, c = (sc_x t1 u0 v0, sc_y t0 u1 v0, sc_z t0 u0 v1).  The entries
+// are tabulated once per evaluation in grid-point order (rows sorted by site,
+// so every sum has a fixed order); each iteration then streams the table.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ size_t support_point(const int* g0, int k, const ElecParams& P, int& a, int& b, int& c) {
+    a = k / 25;
+    b = (k / 5) % 5;
+    c = k % 5;
+    const int gx = wrapg(g0[0] + a, P.grid.x), gy = wrapg(g0[1] + b, P.grid.y), gz = wrapg(g0[2] + c, P.grid.z);
+    return ((size_t)gx * P.grid.y + gy) * P.grid.z + gz;
+}
+
+__global__ void k_csr_count(int npol, const int* __restrict__ polidx, const int* __restrict__ g0tab, ElecParams P,
+                            int* __restrict__ rowcount) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long long)npol * 125) return;
+    const int s = polidx[t / 125];
+    const int g0[3] = {g0tab[3 * s], g0tab[3 * s + 1], g0tab[3 * s + 2]};
+    int a, b, c;
+    atomicAdd(&rowcount[support_point(g0, (int)(t % 125), P, a, b, c)], 1);
+}
+
+__global__ void k_csr_fill(int npol, const int* __restrict__ polidx, const int* __restrict__ g0tab,
+                           const double* __restrict__ sptab, ElecParams P, const int* __restrict__ rowstart,
+                           int* __restrict__ fill, int* __restrict__ esite, double* __restrict__ ecoef) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long long)npol * 125) return;
+    const int s = polidx[t / 125];
+    const int g0[3] = {g0tab[3 * s], g0tab[3 * s + 1], g0tab[3 * s + 2]};
+    int a, b, c;
+    const size_t g = support_point(g0, (int)(t % 125), P, a, b, c);
+    const int e = rowstart[g] + atomicAdd(&fill[g], 1);
+    const double* sp = sptab + 30 * (size_t)s;
+    const double t0 = sp[a], t1 = sp[5 + a], u0 = sp[10 + b], u1 = sp[15 + b], v0 = sp[20 + c], v1 = sp[25 + c];
+    esite[e] = s;
+    ecoef[3 * (size_t)e] = P.grid.x * P.inv_box.x * (t1 * u0 * v0);
+    ecoef[3 * (size_t)e + 1] = P.grid.y * P.inv_box.y * (t0 * u1 * v0);
+    ecoef[3 * (size_t)e + 2] = P.grid.z * P.inv_box.z * (t0 * u0 * v1);
+}
+
+// rows ascending by site (a row holds a handful of entries)
+__global__ void k_csr_sort(size_t ngrid, const int* __restrict__ rowstart, int* __restrict__ esite,
+                           double* __restrict__ ecoef) {
+    const size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= ngrid) return;
+    const int b = rowstart[g], e = rowstart[g + 1];
+    for (int k = b + 1; k < e; ++k) {
+        const int v = esite[k];
+        const double c0 = ecoef[3 * (size_t)k], c1 = ecoef[3 * (size_t)k + 1], c2 = ecoef[3 * (size_t)k + 2];
+        int m = k - 1;
+        while (m >= b && esite[m] > v) {
+            esite[m + 1] = esite[m];
+            ecoef[3 * (size_t)(m + 1)] = ecoef[3 * (size_t)m];
+            ecoef[3 * (size_t)(m + 1) + 1] = ecoef[3 * (size_t)m + 1];
+            ecoef[3 * (size_t)(m + 1) + 2] = ecoef[3 * (size_t)m + 2];
+            --m;
+        }
+        esite[m + 1] = v;
+        ecoef[3 * (size_t)(m + 1)] = c0;
+        ecoef[3 * (size_t)(m + 1) + 1] = c1;
+        ecoef[3 * (size_t)(m + 1) + 2] = c2;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_spread_csr(size_t ngrid, const int* __restrict__ rowstart,
+                                                    const int* __restrict__ esite, const double* __restrict__ ecoef,
+                                                    const double* __restrict__ mu, double* __restrict__ grid) {
+    const size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= ngrid) return;
+    double v = 0.0;
+    for (int e = rowstart[g]; e < rowstart[g + 1]; ++e) {
+        const int s = esite[e];
+        v += mu[3 * s] * ecoef[3 * (size_t)e] + mu[3 * s + 1] * ecoef[3 * (size_t)e + 1] +
+             mu[3 * s + 2] * ecoef[3 * (size_t)e + 2];
+    }
+    grid[g] = v;
+}
+
 
 // multiply the half-complex spectrum by the reciprocal-space influence function (:1676-1713)
 __global__ void k_convolve(size_t nhalf, const double* __restrict__ influence, double2* __restrict__ spec) {
@@ -1058,13 +1171,35 @@ void launch_fixed_field(hipStream_t s, int npol, const int* polidx, const double
         hipLaunchKernelGGL(k_fixed_field<kG>, nblk((long long)npol * kG, 256), 256, 0, s, npol, polidx, pos, S, P,
                            R.capP, R.nbP, R.cntP, R.capQ, R.nbQ, R.cntQ, field);
 }
-void launch_site_bins(hipStream_t s, int n, const double* pos, ElecParams P, BrickGeom B, int* bin_of, int* bin_count) {
-    hipLaunchKernelGGL(k_site_bins, nblk(n, 256), 256, 0, s, n, pos, P, B, bin_of, bin_count);
+void launch_site_bins(hipStream_t s, int n, const double* pos, const double* weight, ElecParams P, BrickGeom B,
+                      int* bin_of, int* bin_count) {
+    hipLaunchKernelGGL(k_site_bins, nblk(n, 256), 256, 0, s, n, pos, weight, P, B, bin_of, bin_count);
 }
-void launch_spread_bricks(hipStream_t s, int mode, const double* pos, const double* q, const double* mu, ElecParams P,
-                          BrickGeom B, const int* bin_start, const int* members, double* grid) {
-    hipLaunchKernelGGL(k_spread_bricks, B.nb[0] * B.nb[1] * B.nb[2], kBrickThreads, 0, s, mode, pos, q, mu, P, B,
-                       bin_start, members, grid);
+void launch_site_spline_table(hipStream_t s, int n, const double* pos, ElecParams P, int* g0tab, double* sptab) {
+    hipLaunchKernelGGL(k_site_spline_table, nblk(n, 128), 128, 0, s, n, pos, P, g0tab, sptab);
+}
+void launch_spread_bricks(hipStream_t s, int mode, const double* q, const double* mu, ElecParams P, BrickGeom B,
+                          const int* bin_start, const int* members, const int* g0tab, const double* sptab,
+                          double* grid) {
+    hipLaunchKernelGGL(k_spread_bricks, B.nb[0] * B.nb[1] * B.nb[2], kBrickThreads, 0, s, mode, q, mu, P, B, bin_start,
+                       members, g0tab, sptab, grid);
+}
+void launch_csr_count(hipStream_t s, int npol, const int* polidx, const int* g0tab, ElecParams P, int* rowcount) {
+    if (npol > 0)
+        hipLaunchKernelGGL(k_csr_count, nblk((long long)npol * 125, 256), 256, 0, s, npol, polidx, g0tab, P, rowcount);
+}
+void launch_csr_fill(hipStream_t s, int npol, const int* polidx, const int* g0tab, const double* sptab, ElecParams P,
+                     const int* rowstart, int* fill, int* esite, double* ecoef) {
+    if (npol > 0)
+        hipLaunchKernelGGL(k_csr_fill, nblk((long long)npol * 125, 256), 256, 0, s, npol, polidx, g0tab, sptab, P,
+                           rowstart, fill, esite, ecoef);
+}
+void launch_csr_sort(hipStream_t s, size_t ngrid, const int* rowstart, int* esite, double* ecoef) {
+    hipLaunchKernelGGL(k_csr_sort, nblk(ngrid, 256), 256, 0, s, ngrid, rowstart, esite, ecoef);
+}
+void launch_spread_csr(hipStream_t s, size_t ngrid, const int* rowstart, const int* esite, const double* ecoef,
+                       const double* mu, double* grid) {
+    hipLaunchKernelGGL(k_spread_csr, nblk(ngrid, 256), 256, 0, s, ngrid, rowstart, esite, ecoef, mu, grid);
 }
 void launch_convolve(hipStream_t s, size_t nhalf, const double* influence, double2* spec) {
     hipLaunchKernelGGL(k_convolve, nblk(nhalf, 256), 256, 0, s, nhalf, influence, spec);
