@@ -6,11 +6,14 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
+#include <string>
 #include <vector>
 
 #include "../../include/mbpol_hip.h"
 #include "elec.hpp"
 #include "elec_kernels.h"
+#include "fft_kernels.h"
 #include "mbpol_params.h"
 
 namespace mbpol {
@@ -53,6 +56,11 @@ struct Elec {
     } binq;  // sites with a charge, by grid brick
     DevBuf g0tab, sptab;  // per-site spline table
     DevBuf rowcount, rowstart, rowfill, esite, ecoef;  // dipole-spreading table in grid-point order
+    // grid transforms: the LDS transforms of kernels_fft.hip when every axis
+    // factors into radices 2..13, else hipFFT
+    PmeFft fft{};
+    DevBuf tw[3];
+    bool lds_fft = false;
     hipfftHandle plan_f = 0, plan_b = 0;
     bool plans = false;
     double* h_eps = nullptr;  // pinned: [0] epsilon, ints from [8] on
@@ -123,6 +131,31 @@ static int up(DevBuf& b, const std::vector<T>& v) {
     HIP_TRY(b.ensure(std::max<size_t>(1, v.size()) * sizeof(T)));
     if (!v.empty()) HIP_TRY(hipMemcpy(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
     return MBPOL_OK;
+}
+
+// factor n into the radices of the LDS transforms; false if some factor is not one of them
+static bool fft_factor(int n, FftAxis& a) {
+    a.n = n;
+    a.nfac = 0;
+    int r = n;
+    for (int p : {4, 2, 3, 5, 7, 11, 13})
+        while (r % p == 0 && a.nfac < 12) {
+            a.fac[a.nfac++] = p;
+            r /= p;
+        }
+    if (r != 1 || n > kFftMaxLength) return false;
+    int ns = 1, maxnb = 1;
+    for (int s = 0; s < a.nfac; ++s) {
+        a.ns[s] = ns;
+        a.nb[s] = n / a.fac[s];
+        a.step[s] = n / (ns * a.fac[s]);
+        a.inv_ns[s] = 1.0f / (float)ns;
+        maxnb = std::max(maxnb, a.nb[s]);
+        ns *= a.fac[s];
+    }
+    a.lgP = 0;
+    while ((1 << a.lgP) < maxnb && a.lgP < 7) ++a.lgP;
+    return true;
 }
 
 // Brick size along one grid axis for the spreading kernel: every brick at least
@@ -278,11 +311,29 @@ int elec_create(mbpol_context* ctx) {
         HIP_TRY(E->g0tab.ensure(sizeof(int) * 3 * std::max(n, 1)));
         HIP_TRY(E->sptab.ensure(sizeof(double) * 30 * std::max(n, 1)));
         for (DevBuf* b : {&E->rowcount, &E->rowstart, &E->rowfill}) HIP_TRY(b->ensure(sizeof(int) * (nreal + 1)));
-        FFT_TRY(hipfftPlan3d(&E->plan_f, grid[0], grid[1], grid[2], HIPFFT_D2Z));
-        FFT_TRY(hipfftPlan3d(&E->plan_b, grid[0], grid[1], grid[2], HIPFFT_Z2D));
-        E->plans = true;
-        FFT_TRY(hipfftSetStream(E->plan_f, ctx->stream));
-        FFT_TRY(hipfftSetStream(E->plan_b, ctx->stream));
+        bool lds = true;
+        for (int d = 0; d < 3; ++d) lds = lds && fft_factor(grid[d], E->fft.ax[d]);
+        const char* force = std::getenv("MBPOL_HIP_PME_FFT");  // "hipfft" selects the library transforms
+        if (force && std::string(force) == "hipfft") lds = false;
+        E->lds_fft = lds;
+        if (lds) {
+            for (int d = 0; d < 3; ++d) {
+                const int m = grid[d];
+                std::vector<double2> tw(m);
+                for (int t = 0; t < m; ++t) {
+                    const long double a = 2.0L * 3.141592653589793238462643383279502884L * t / m;
+                    tw[t] = make_double2((double)cosl(a), (double)-sinl(a));
+                }
+                if (int e = up(E->tw[d], tw)) return e;
+                E->fft.ax[d].tw = E->tw[d].as<double2>();
+            }
+        } else {
+            FFT_TRY(hipfftPlan3d(&E->plan_f, grid[0], grid[1], grid[2], HIPFFT_D2Z));
+            FFT_TRY(hipfftPlan3d(&E->plan_b, grid[0], grid[1], grid[2], HIPFFT_Z2D));
+            E->plans = true;
+            FFT_TRY(hipfftSetStream(E->plan_f, ctx->stream));
+            FFT_TRY(hipfftSetStream(E->plan_b, ctx->stream));
+        }
     }
     return elec_set_box(ctx);
 }
@@ -380,9 +431,13 @@ static int pme_grid(mbpol_context* ctx, int mode, const double* mu) {
     else
         launch_spread_csr(st, (size_t)P.grid.x * P.grid.y * P.grid.z, E->rowstart.as<int>(), E->esite.as<int>(),
                           E->ecoef.as<double>(), mu, E->grid.as<double>());
-    FFT_TRY(hipfftExecD2Z(E->plan_f, E->grid.as<double>(), reinterpret_cast<hipfftDoubleComplex*>(E->spec.p)));
-    launch_convolve(st, nhalf, E->influence.as<double>(), E->spec.as<double2>());
-    FFT_TRY(hipfftExecZ2D(E->plan_b, reinterpret_cast<hipfftDoubleComplex*>(E->spec.p), E->grid.as<double>()));
+    if (E->lds_fft) {
+        launch_pme_convolution(st, E->fft, E->grid.as<double>(), E->spec.as<double2>(), E->influence.as<double>());
+    } else {
+        FFT_TRY(hipfftExecD2Z(E->plan_f, E->grid.as<double>(), reinterpret_cast<hipfftDoubleComplex*>(E->spec.p)));
+        launch_convolve(st, nhalf, E->influence.as<double>(), E->spec.as<double2>());
+        FFT_TRY(hipfftExecZ2D(E->plan_b, reinterpret_cast<hipfftDoubleComplex*>(E->spec.p), E->grid.as<double>()));
+    }
     return MBPOL_OK;
 }
 

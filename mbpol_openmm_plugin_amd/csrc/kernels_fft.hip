@@ -1,0 +1,396 @@
+// Reciprocal-space convolution of the PME grid: forward real-to-complex 3D DFT,
+// multiplication by the influence function, inverse complex-to-real 3D DFT
+// (the role hipfftExecD2Z / convolve / hipfftExecZ2D play otherwise; the
+// reference does the same with its own complex FFT, MBPolReferenceElectrostaticsForce.cpp:1622-1713).
+//
+// Five passes over the grid, each a batch of 1D transforms done in LDS by one
+// workgroup on a few lines at a time (mixed-radix Stockham, radices 2-13):
+//   z: real lines -> half spectrum (kz < K2/2 + 1)
+//   y: forward
+//   x: forward, times the influence function, inverse (one pass: the x lines
+//      are complete in k-space after the forward transform)
+//   y: inverse
+//   z: half spectrum -> real lines (Hermitian extension)
+// The half-spectrum layout [kx][ky][kz] is the one of a D2Z transform, and
+// both directions are unnormalised, as there.
+#include <algorithm>
+
+#include "device_common.h"
+#include "fft_kernels.h"
+
+namespace mbpol {
+namespace {
+
+// cos / sin of 2 pi t / R, t < R
+template <int R>
+struct Roots;
+template <>
+struct Roots<2> {
+    static constexpr double c[2] = {1.0, -1.0};
+    static constexpr double s[2] = {0.0, 0.0};
+};
+template <>
+struct Roots<3> {
+    static constexpr double c[3] = {1.0, -0.5, -0.5};
+    static constexpr double s[3] = {0.0, 0.8660254037844386, -0.8660254037844386};
+};
+template <>
+struct Roots<4> {
+    static constexpr double c[4] = {1.0, 0.0, -1.0, 0.0};
+    static constexpr double s[4] = {0.0, 1.0, 0.0, -1.0};
+};
+template <>
+struct Roots<5> {
+    static constexpr double c[5] = {1.0, 0.30901699437494745, -0.8090169943749475, -0.8090169943749475, 0.30901699437494745};
+    static constexpr double s[5] = {0.0, 0.9510565162951535, 0.5877852522924731, -0.5877852522924731, -0.9510565162951535};
+};
+template <>
+struct Roots<7> {
+    static constexpr double c[7] = {1.0, 0.6234898018587335, -0.2225209339563144, -0.9009688679024191, -0.9009688679024191, -0.2225209339563144, 0.6234898018587335};
+    static constexpr double s[7] = {0.0, 0.7818314824680298, 0.9749279121818236, 0.4338837391175581, -0.4338837391175581, -0.9749279121818236, -0.7818314824680298};
+};
+template <>
+struct Roots<11> {
+    static constexpr double c[11] = {1.0, 0.8412535328311812, 0.41541501300188644, -0.14231483827328514, -0.6548607339452851, -0.9594929736144974, -0.9594929736144974, -0.6548607339452851, -0.14231483827328514, 0.41541501300188644, 0.8412535328311812};
+    static constexpr double s[11] = {0.0, 0.5406408174555976, 0.9096319953545183, 0.9898214418809327, 0.7557495743542583, 0.28173255684142967, -0.28173255684142967, -0.7557495743542583, -0.9898214418809327, -0.9096319953545183, -0.5406408174555976};
+};
+template <>
+struct Roots<13> {
+    static constexpr double c[13] = {1.0, 0.8854560256532099, 0.5680647467311558, 0.12053668025532305, -0.3546048870425356, -0.7485107481711011, -0.970941817426052, -0.970941817426052, -0.7485107481711011, -0.3546048870425356, 0.12053668025532305, 0.5680647467311558, 0.8854560256532099};
+    static constexpr double s[13] = {0.0, 0.46472317204376856, 0.8229838658936564, 0.992708874098054, 0.9350162426854148, 0.6631226582407952, 0.23931566428755777, -0.23931566428755777, -0.6631226582407952, -0.9350162426854148, -0.992708874098054, -0.8229838658936564, -0.46472317204376856};
+};
+
+// One radix-R Stockham step (stage with Ns = product of the earlier radices)
+// for butterfly j of a line of length N: inputs at j + r N/R, twiddle
+// exp(-+2 pi i (j mod Ns) r / (Ns R)) = tw[(j mod Ns) r step], step = N/(Ns R),
+// R-point DFT, outputs at (j / Ns) Ns R + j mod Ns + q Ns.  tw[t] = exp(-2 pi i t / N).
+template <int R>
+__device__ __forceinline__ void butterfly(const double2* __restrict__ in, double2* __restrict__ out, int j, int jq,
+                                          int Ns, int stride, int step, const double2* __restrict__ tw, bool inv) {
+    // components kept in plain double arrays (struct copies of double2 would go through scratch memory)
+    double vx[R], vy[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        vx[r] = in[j + r * stride].x;
+        vy[r] = in[j + r * stride].y;
+    }
+    const int k = j - jq * Ns;
+    if (Ns > 1) {
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+            const double wx = tw[k * r * step].x, wy0 = tw[k * r * step].y;
+            const double wy = inv ? -wy0 : wy0;
+            const double x = vx[r] * wx - vy[r] * wy, y = vx[r] * wy + vy[r] * wx;
+            vx[r] = x;
+            vy[r] = y;
+        }
+    }
+    const int base = jq * Ns * R + k;
+    if constexpr (R % 2 == 0) {  // radices 2 and 4: plain R-point DFT
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+            double ax = vx[0], ay = vy[0];
+#pragma unroll
+            for (int r = 1; r < R; ++r) {
+                const int t = (r * q) % R;
+                const double c = Roots<R>::c[t], sn = inv ? Roots<R>::s[t] : -Roots<R>::s[t];
+                ax += vx[r] * c - vy[r] * sn;
+                ay += vx[r] * sn + vy[r] * c;
+            }
+            out[base + q * Ns].x = ax;
+            out[base + q * Ns].y = ay;
+        }
+    } else {
+        // odd R: with s_r = v_r + v_(R-r), d_r = v_r - v_(R-r) (r = 1..(R-1)/2),
+        // X_q = v_0 + sum_r s_r cos(2 pi r q / R) -+ i sum_r d_r sin(2 pi r q / R)
+        // and X_(R-q) the same with the sign of the sine term flipped
+        constexpr int H = (R - 1) / 2;
+        double sx[H], sy[H], dx[H], dy[H];
+        double x0 = vx[0], y0 = vy[0];
+#pragma unroll
+        for (int r = 1; r <= H; ++r) {
+            sx[r - 1] = vx[r] + vx[R - r];
+            sy[r - 1] = vy[r] + vy[R - r];
+            dx[r - 1] = vx[r] - vx[R - r];
+            dy[r - 1] = vy[r] - vy[R - r];
+            x0 += sx[r - 1];
+            y0 += sy[r - 1];
+        }
+        out[base].x = x0;
+        out[base].y = y0;
+#pragma unroll
+        for (int q = 1; q <= H; ++q) {
+            double ax = vx[0], ay = vy[0], bx = 0.0, by = 0.0;
+#pragma unroll
+            for (int r = 1; r <= H; ++r) {
+                const int t = (r * q) % R;
+                const double c = Roots<R>::c[t], sn = Roots<R>::s[t];
+                ax += sx[r - 1] * c;
+                ay += sy[r - 1] * c;
+                bx += dx[r - 1] * sn;
+                by += dy[r - 1] * sn;
+            }
+            // forward: X_q = A - i B, X_(R-q) = A + i B; inverse the other way round (i B = (-by, bx))
+            const double sgn = inv ? -1.0 : 1.0;
+            out[base + q * Ns].x = ax + sgn * by;
+            out[base + q * Ns].y = ay - sgn * bx;
+            out[base + (R - q) * Ns].x = ax - sgn * by;
+            out[base + (R - q) * Ns].y = ay + sgn * bx;
+        }
+    }
+}
+
+// j / d for 0 <= j < 2^24 and small d, from a float reciprocal and one correction each way
+__device__ __forceinline__ int div_small(int j, int d, float inv_d) {
+    int q = (int)((float)j * inv_d);
+    q -= q * d > j ? 1 : 0;
+    q += (q + 1) * d <= j ? 1 : 0;
+    return q;
+}
+
+// Transform the line of this thread's group (P = 2^lgP threads per line,
+// line = threadIdx.x >> lgP) held in a; b is scratch.  Every thread of the
+// block calls it (active = the line exists); returns the buffer holding the
+// result, with the block synchronised.
+__device__ double2* fft_line(double2* a, double2* b, bool active, const FftAxis& ax, bool inv) {
+    const int P = 1 << ax.lgP, lane = threadIdx.x & (P - 1);
+    for (int s = 0; s < ax.nfac; ++s) {
+        if (active) {
+            const int R = ax.fac[s], Ns = ax.ns[s], nb = ax.nb[s], step = ax.step[s];
+            const float inv_ns = ax.inv_ns[s];
+            for (int j = lane; j < nb; j += P) {
+                const int jq = Ns == 1 ? j : div_small(j, Ns, inv_ns);
+                switch (R) {
+                    case 2: butterfly<2>(a, b, j, jq, Ns, nb, step, ax.tw, inv); break;
+                    case 3: butterfly<3>(a, b, j, jq, Ns, nb, step, ax.tw, inv); break;
+                    case 4: butterfly<4>(a, b, j, jq, Ns, nb, step, ax.tw, inv); break;
+                    case 5: butterfly<5>(a, b, j, jq, Ns, nb, step, ax.tw, inv); break;
+                    case 7: butterfly<7>(a, b, j, jq, Ns, nb, step, ax.tw, inv); break;
+                    case 11: butterfly<11>(a, b, j, jq, Ns, nb, step, ax.tw, inv); break;
+                    default: butterfly<13>(a, b, j, jq, Ns, nb, step, ax.tw, inv); break;
+                }
+            }
+        }
+        __syncthreads();
+        double2* t = a;
+        a = b;
+        b = t;
+    }
+    return a;
+}
+
+// z pass, forward: lines of the real grid -> half spectrum
+template <int C>
+__global__ void __launch_bounds__(kFftThreads) k_fft_z_r2c(const double* __restrict__ real, double2* __restrict__ H,
+                                                           int nlines, FftAxis az, int Kh) {
+    extern __shared__ double2 lds[];
+    const int N = az.n, M = kFftThreads >> az.lgP, P = 1 << az.lgP;
+    const int L0 = blockIdx.x * M, m = min(M, nlines - L0);
+    const int l = threadIdx.x >> az.lgP, lane = threadIdx.x & (P - 1);
+    {  // all loads in flight before the LDS stores
+        double v[C];
+#pragma unroll
+        for (int u = 0; u < C; ++u) {
+            const int e = threadIdx.x + u * kFftThreads;
+            if (e < m * N) v[u] = real[(size_t)L0 * N + e];
+        }
+#pragma unroll
+        for (int u = 0; u < C; ++u) {
+            const int e = threadIdx.x + u * kFftThreads;
+            if (e < m * N) lds[e] = make_double2(v[u], 0.0);
+        }
+    }
+    __syncthreads();
+    const double2* r = fft_line(lds + l * N, lds + (M + l) * N, l < m, az, false);
+    if (l < m) {
+#pragma unroll
+        for (int u = 0; u < C; ++u) {
+            const int k = lane + u * P;
+            if (k < Kh) {
+                H[(size_t)(L0 + l) * Kh + k].x = r[k].x;
+                H[(size_t)(L0 + l) * Kh + k].y = r[k].y;
+            }
+        }
+    }
+}
+
+// y pass (either direction): block = (x, tile of M consecutive kz)
+template <int C>
+__global__ void __launch_bounds__(kFftThreads) k_fft_y(double2* __restrict__ H, int K1, int Kh, FftAxis ay, int inv) {
+    extern __shared__ double2 lds[];
+    const int N = ay.n, lgM = 7 - ay.lgP, M = 1 << lgM;
+    const int ntile = (Kh + M - 1) >> lgM;
+    const int x = blockIdx.x / ntile, kz0 = (blockIdx.x - x * ntile) * M, m = min(M, Kh - kz0);
+    const int cl = threadIdx.x & (M - 1), cy = threadIdx.x >> lgM;  // copy mapping: consecutive kz per row
+    const int ystep = kFftThreads >> lgM;
+    if (cl < m) {
+        double vx[C], vy[C];
+#pragma unroll
+        for (int u = 0; u < C; ++u) {
+            const int y = cy + u * ystep;
+            if (y < N) {
+                const double2* src = H + ((size_t)x * K1 + y) * Kh + kz0 + cl;
+                vx[u] = src->x;
+                vy[u] = src->y;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < C; ++u) {
+            const int y = cy + u * ystep;
+            if (y < N) {
+                lds[cl * N + y].x = vx[u];
+                lds[cl * N + y].y = vy[u];
+            }
+        }
+    }
+    __syncthreads();
+    const int l = threadIdx.x >> ay.lgP;
+    const double2* r = fft_line(lds + l * N, lds + (M + l) * N, l < m, ay, inv != 0);
+    const int off = (int)(r - (lds + l * N));  // 0 or M N: which buffer holds the result
+    if (cl < m) {
+#pragma unroll
+        for (int u = 0; u < C; ++u) {
+            const int y = cy + u * ystep;
+            if (y < N) {
+                double2* dst = H + ((size_t)x * K1 + y) * Kh + kz0 + cl;
+                dst->x = lds[off + cl * N + y].x;
+                dst->y = lds[off + cl * N + y].y;
+            }
+        }
+    }
+}
+
+// x pass: forward, influence function, inverse; block = (y, tile of M consecutive kz)
+template <int C>
+__global__ void __launch_bounds__(kFftThreads) k_fft_x_conv(double2* __restrict__ H, const double* __restrict__ influence,
+                                                            int K1, int Kh, FftAxis ax) {
+    extern __shared__ double2 lds[];
+    const int N = ax.n, lgM = 7 - ax.lgP, M = 1 << lgM, P = 1 << ax.lgP;
+    const int ntile = (Kh + M - 1) >> lgM;
+    const int y = blockIdx.x / ntile, kz0 = (blockIdx.x - y * ntile) * M, m = min(M, Kh - kz0);
+    const int cl = threadIdx.x & (M - 1), cx = threadIdx.x >> lgM;
+    const int xstep = kFftThreads >> lgM;
+    const int l = threadIdx.x >> ax.lgP, lane = threadIdx.x & (P - 1);
+    double f[C];  // influence factors of this thread's k-space points, fetched early
+#pragma unroll
+    for (int u = 0; u < C; ++u) {
+        const int kx = lane + u * P;
+        f[u] = (l < m && kx < N) ? influence[((size_t)kx * K1 + y) * Kh + kz0 + l] : 0.0;
+    }
+    if (cl < m) {
+        double vx[C], vy[C];
+#pragma unroll
+        for (int u = 0; u < C; ++u) {
+            const int x = cx + u * xstep;
+            if (x < N) {
+                const double2* src = H + ((size_t)x * K1 + y) * Kh + kz0 + cl;
+                vx[u] = src->x;
+                vy[u] = src->y;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < C; ++u) {
+            const int x = cx + u * xstep;
+            if (x < N) {
+                lds[cl * N + x].x = vx[u];
+                lds[cl * N + x].y = vy[u];
+            }
+        }
+    }
+    __syncthreads();
+    double2* a = lds + l * N;
+    double2* b = lds + (M + l) * N;
+    double2* r = fft_line(a, b, l < m, ax, false);
+    if (l < m) {
+#pragma unroll
+        for (int u = 0; u < C; ++u) {
+            const int kx = lane + u * P;
+            if (kx < N) {
+                r[kx].x = r[kx].x * f[u];
+                r[kx].y = r[kx].y * f[u];
+            }
+        }
+    }
+    __syncthreads();
+    r = fft_line(r, r == a ? b : a, l < m, ax, true);
+    const int off = (int)(r - (lds + l * N));
+    if (cl < m) {
+#pragma unroll
+        for (int u = 0; u < C; ++u) {
+            const int x = cx + u * xstep;
+            if (x < N) {
+                double2* dst = H + ((size_t)x * K1 + y) * Kh + kz0 + cl;
+                dst->x = lds[off + cl * N + x].x;
+                dst->y = lds[off + cl * N + x].y;
+            }
+        }
+    }
+}
+
+// z pass, inverse: half spectrum -> real lines (kz >= Kh from conj(H[N - kz]))
+template <int C>
+__global__ void __launch_bounds__(kFftThreads) k_fft_z_c2r(const double2* __restrict__ H, double* __restrict__ real,
+                                                           int nlines, FftAxis az, int Kh) {
+    extern __shared__ double2 lds[];
+    const int N = az.n, M = kFftThreads >> az.lgP, P = 1 << az.lgP;
+    const int L0 = blockIdx.x * M, m = min(M, nlines - L0);
+    const int l = threadIdx.x >> az.lgP, lane = threadIdx.x & (P - 1);
+    double2* a = lds + l * N;
+    if (l < m) {
+        const size_t row = (size_t)(L0 + l) * Kh;
+        double vx[C], vy[C];
+#pragma unroll
+        for (int u = 0; u < C; ++u) {
+            const int k = lane + u * P;
+            if (k < N) {
+                const double2* src = H + row + (k < Kh ? k : N - k);
+                vx[u] = src->x;
+                vy[u] = src->y;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < C; ++u) {
+            const int k = lane + u * P;
+            if (k < N) {
+                a[k].x = vx[u];
+                a[k].y = k < Kh ? vy[u] : -vy[u];
+            }
+        }
+    }
+    __syncthreads();
+    const double2* r = fft_line(a, lds + (M + l) * N, l < m, az, true);
+    const int off = (int)(r - a);
+#pragma unroll
+    for (int u = 0; u < C; ++u) {
+        const int e = threadIdx.x + u * kFftThreads;
+        if (e < m * N) real[(size_t)L0 * N + e] = lds[off + e].x;
+    }
+}
+
+}  // namespace
+
+static inline int nblk(long long n, int b) { return (int)((n + b - 1) / b); }
+
+void launch_pme_convolution(hipStream_t s, const PmeFft& F, double* real, double2* H, const double* influence) {
+    const int K0 = F.ax[0].n, K1 = F.ax[1].n, K2 = F.ax[2].n, Kh = K2 / 2 + 1;
+    const int nl = K0 * K1;
+    const int Mz = kFftThreads >> F.ax[2].lgP, My = kFftThreads >> F.ax[1].lgP, Mx = kFftThreads >> F.ax[0].lgP;
+    const size_t lz = 2 * sizeof(double2) * Mz * K2, ly = 2 * sizeof(double2) * My * K1,
+                 lx = 2 * sizeof(double2) * Mx * K0;
+    // C: elements a thread moves per copy phase, >= ceil(n / P) (at most 13 for the supported lengths)
+    auto copies = [](const FftAxis& a) { const int c = (a.n + (1 << a.lgP) - 1) >> a.lgP; return c <= 4 ? 4 : c <= 8 ? 8 : 16; };
+#define MBPOL_FFT_LAUNCH(kern, axis, ...)                                                              \
+    switch (copies(axis)) {                                                                            \
+        case 4: hipLaunchKernelGGL(kern<4>, __VA_ARGS__); break;                                       \
+        case 8: hipLaunchKernelGGL(kern<8>, __VA_ARGS__); break;                                       \
+        default: hipLaunchKernelGGL(kern<16>, __VA_ARGS__); break;                                     \
+    }
+    MBPOL_FFT_LAUNCH(k_fft_z_r2c, F.ax[2], nblk(nl, Mz), kFftThreads, lz, s, real, H, nl, F.ax[2], Kh)
+    MBPOL_FFT_LAUNCH(k_fft_y, F.ax[1], K0 * nblk(Kh, My), kFftThreads, ly, s, H, K1, Kh, F.ax[1], 0)
+    MBPOL_FFT_LAUNCH(k_fft_x_conv, F.ax[0], K1 * nblk(Kh, Mx), kFftThreads, lx, s, H, influence, K1, Kh, F.ax[0])
+    MBPOL_FFT_LAUNCH(k_fft_y, F.ax[1], K0 * nblk(Kh, My), kFftThreads, ly, s, H, K1, Kh, F.ax[1], 1)
+    MBPOL_FFT_LAUNCH(k_fft_z_c2r, F.ax[2], nblk(nl, Mz), kFftThreads, lz, s, H, real, nl, F.ax[2], Kh)
+#undef MBPOL_FFT_LAUNCH
+}
+
+}  // namespace mbpol

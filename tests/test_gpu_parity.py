@@ -201,3 +201,31 @@ def test_config2_ewald_split_invariance(mb):
         e[rc] = st.getComponentEnergies()['electrostatics']
         print('rc', rc, 'E_elec', e[rc], st.stats, flush=True)
     assert abs(e[0.9] - e[1.2]) <= 1e-3 * abs(e[0.9])
+
+
+def test_pme_transform_paths_agree(mb, oracle):
+    """The LDS grid transforms (every axis a product of radices 2..13) and the
+    hipFFT transforms give the same PME electrostatics; an axis with a prime
+    factor 17 selects hipFFT by itself, and that path matches the oracle too."""
+    waters, L = mb.systems.config1()
+    pos = mb.forcefield.water_positions_with_m(waters)
+    sys = mb.forcefield.create_system(256, nonbonded_method='PME', box=L, ewald_error_tolerance=1e-5,
+                                      dipole_epsilon=1e-8, components=16)
+    res = {}
+    for mode in ('lds', 'hipfft'):
+        if mode == 'hipfft':
+            os.environ['MBPOL_HIP_PME_FFT'] = 'hipfft'
+        try:
+            ctx = mb.Context(sys, precision='double')
+        finally:
+            os.environ.pop('MBPOL_HIP_PME_FFT', None)
+        ctx.setPositions(pos)
+        st = ctx.getState(components=16)
+        res[mode] = (st.getComponentEnergies()['electrostatics'], st.getForces())
+        print(mode, res[mode][0], st.stats, flush=True)
+    assert abs(res['lds'][0] - res['hipfft'][0]) <= 1e-10 * abs(res['hipfft'][0])
+    assert rel_rms(res['lds'][1], res['hipfft'][1]) <= 1e-10
+    # 51 = 3 x 17: not an LDS length
+    sys17 = mb.forcefield.create_system(256, nonbonded_method='PME', box=L, ewald_alpha=3.4, pme_grid=(48, 51, 50),
+                                        dipole_epsilon=1e-8, components=16)
+    compare(mb, oracle, sys17, pos, 'double', components=16, label='grid48x51x50')
