@@ -1,10 +1,23 @@
 #!/usr/bin/env python3
-"""MB-pol energy+force evaluations per second, 4096 waters, periodic, on one MI355X.
+"""MB-pol energy+force evaluations per second, 4096 waters, periodic, one MI355X.
 
-One "step" is one full MB-pol evaluation (1-body, 2-body, 3-body, dispersion
-and PME electrostatics with the induced-dipole solve to 1e-8) of the
-4096-water box (cubic 49.66 A, synthetic seeded geometry), positions already
-resident in HBM.  Prints one JSON line.
+One step is one full MB-pol evaluation -- 1-body, 2-body and 3-body PIPs with
+their O-O lists, damped dispersion, and PME electrostatics with the Thole
+induced-dipole solve to epsilon 1e-8 -- of the 4096-water box (cubic 49.66 A,
+synthetic seeded geometry), positions resident in HBM before the timed region.
+
+Prints one JSON line:
+  value / ms_per_step  wall-clock throughput of the default (mixed) path over
+                       `--steps` evaluations, timing switched off;
+  fp64_path            the same for the FP64 path, and the error of the mixed
+                       path against it per component;
+  roofline             the kernel with the largest share of the evaluation,
+                       from HIP events recorded on the library's stream during
+                       a second pass of `--steps` evaluations; its algorithmic
+                       bytes or flops per launch (models below) / average
+                       launch time, against the MI355X peak;
+  cpu_baseline         the FP64 oracle on one host core over a bounded sample
+                       of the same evaluation (see cpu_baseline()).
 """
 import argparse
 import json
@@ -18,11 +31,28 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-# algorithmic work per unit (mul + add counted from the reference's polynomial source text,
-# poly-3b-v2x.cpp / poly-2b-v6x.cpp, plus the variable/switch chain rule)
+# MI355X peaks: HBM3E bandwidth, FP64 vector/matrix rate
+PEAK_HBM_GBS = 8000.0
+PEAK_FP64_TFLOPS = 78.6
+# algorithmic work per unit: mul + add of the reference polynomials counted from
+# their source (poly-3b-v2x.cpp: 111,817; poly-2b-v6x.cpp: 30,600) plus the
+# variables, switches and chain rule around them
 FLOPS_PER_TRIMER = 112.5e3
 FLOPS_PER_DIMER = 31.0e3
-PEAK_FP32_TFLOPS = 157.3
+# HBM bytes per launch from rocprofv3 FETCH_SIZE + WRITE_SIZE counters on this
+# workload, calibrated against a plain copy kernel (FETCH_SIZE x 2.0, WRITE_SIZE
+# x 1.0; profiles/memory_counters_4096w_eval.md holds the runs); kernels not
+# listed report null
+MEASURED_TRAFFIC = {
+    'elec_iterate': 105088023,
+    'pme_spread': 70124034,
+    'fft_z_r2c': 28975953,
+    'fft_y': 29725692,
+    'fft_x_conv': 44848900,
+    'fft_z_c2r': 29341622,
+    'threebody': 230665234,
+    'twobody': 88443131,
+}
 
 
 def parse():
@@ -32,78 +62,96 @@ def parse():
     ap.add_argument('--warmup', type=int, default=3)
     ap.add_argument('--precision', default='mixed', choices=['mixed', 'double', 'single-pip'])
     ap.add_argument('--no-cpu-baseline', action='store_true')
-    ap.add_argument('--cpu-sample-seconds', type=float, default=20.0)
+    ap.add_argument('--no-fp64-path', action='store_true')
     return ap.parse_args()
 
 
-def workload():
-    import mbpol_openmm_plugin_amd as m
+def workload(m):
     waters, L = m.systems.config2()
     sysm = m.forcefield.create_system(4096, nonbonded_method='PME', box=L, ewald_error_tolerance=1e-5,
                                       dipole_epsilon=1e-8)
-    return m, sysm, m.forcefield.water_positions_with_m(waters), L
+    return sysm, m.forcefield.water_positions_with_m(waters), L
 
 
-def cpu_baseline(budget_s):
-    """Single-threaded FP64 oracle on a bounded sample of the same 4096-water workload:
-    every component except the electrostatics on all molecules would take minutes,
-    so the sample times each component on a subset and scales it to one full
-    evaluation: 1-body on all waters, 2-body on a random subset of the dimer list,
-    3-body on a random subset of the trimer list, dispersion and PME
-    electrostatics on the 256-water box scaled by their pair counts."""
+def kernel_models(stats, grid, nsites, npol):
+    """Algorithmic work per launch of each individually timed kernel:
+    (kind, amount, bound, peak, unit)."""
+    K0, K1, K2 = grid
+    ngrid, nhalf = K0 * K1 * K2, K0 * K1 * (K2 // 2 + 1)
+    return {
+        'threebody': ('flops', FLOPS_PER_TRIMER * stats['num_trimers'], 'mfma', PEAK_FP64_TFLOPS, 'TFLOP/s'),
+        'twobody': ('flops', FLOPS_PER_DIMER * stats['num_dimers'], 'mfma', PEAK_FP64_TFLOPS, 'TFLOP/s'),
+        # rows of the polarisable sites (int index + 2 FP64 prefactors per entry), the
+        # potential grid once, positions and dipoles of every site once, the site's own
+        # fixed field / polarisability / row length and the new dipole
+        'elec_iterate': ('bytes', stats['num_pol_pairs'] * 20 + ngrid * 8 + nsites * 48 + npol * (24 + 24 + 8 + 4 + 4),
+                         'hbm', PEAK_HBM_GBS, 'GB/s'),
+        # row starts, entries (site + 3 FP64 coefficients), dipoles, grid write
+        'pme_spread': ('bytes', (ngrid + 1) * 4 + npol * 125 * 28 + npol * 24 + ngrid * 8, 'hbm', PEAK_HBM_GBS, 'GB/s'),
+        'fft_z_r2c': ('bytes', ngrid * 8 + nhalf * 16, 'hbm', PEAK_HBM_GBS, 'GB/s'),
+        'fft_y': ('bytes', nhalf * 32, 'hbm', PEAK_HBM_GBS, 'GB/s'),
+        'fft_x_conv': ('bytes', nhalf * 40, 'hbm', PEAK_HBM_GBS, 'GB/s'),
+        'fft_z_c2r': ('bytes', nhalf * 16 + ngrid * 8, 'hbm', PEAK_HBM_GBS, 'GB/s'),
+    }
+
+
+PHASES = ('evaluate', 'lists', 'onebody', 'twobody', 'threebody', 'dispersion', 'elec_setup', 'elec_iteration',
+          'elec_energy')
+
+
+def run_steps(ctx, steps):
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        e = ctx.evaluateResident()  # returns after the energies are on the host
+    return (time.perf_counter() - t0) * 1000.0 / steps, e
+
+
+def cpu_baseline(gpu_iterations):
+    """FP64 oracle on one core, on a bounded sample of the same evaluation:
+    1-body, the O-O lists and dispersion in full; 2-body and 3-body on the first
+    2000 entries of their lists, scaled to the full lists; electrostatics run
+    once with a single dipole iteration, its setup + energy time plus
+    `gpu_iterations` times the measured iteration time.  About 25 s."""
     code = r'''
 import sys, time, json, numpy as np
 sys.path.insert(0, %(root)r); sys.path.insert(0, %(oracle)r)
 import pyoracle as po, mbpol_openmm_plugin_amd as m
-budget = %(budget)f
 waters, L = m.systems.config2()
 pos = m.forcefield.water_positions_with_m(waters)
-idx = np.array([[4*w, 4*w+1, 4*w+2] for w in range(4096)])
-box = [L]*3
+idx = np.array([[4*w, 4*w+1, 4*w+2] for w in range(4096)]); box = [L]*3
 res = {}
 t = time.perf_counter(); po.onebody(pos, idx, True, box); res['onebody'] = time.perf_counter() - t
-opos = pos[0::4]
-pairs = po.dimer_list(opos, 0.65, True, box)
-trip = po.trimer_list(opos, 0.45, True, box)
-rng = np.random.default_rng(0)
-def sample(items, fn, share):
-    k = max(1, len(items))
-    n = 64
-    while True:
-        sel = items[rng.choice(len(items), size=min(n, len(items)), replace=False)]
-        t = time.perf_counter(); fn(sel); dt = time.perf_counter() - t
-        if dt > share or n >= len(items):
-            return dt * len(items) / len(sel), len(sel)
-        n *= 4
-res['twobody'], n2 = sample(pairs, lambda s: po.twobody(pos, idx, s, True, box), budget * 0.15)
-res['threebody'], n3 = sample(trip, lambda s: po.threebody(pos, idx, s, True, box), budget * 0.35)
-w1, L1 = m.systems.config1()
-s1 = m.forcefield.create_system(256, nonbonded_method='PME', box=L1, ewald_error_tolerance=1e-5, dipole_epsilon=1e-8,
-                                components=8 | 16)
-p1 = m.forcefield.water_positions_with_m(w1)
-t = time.perf_counter(); E, F = po.evaluate_system(s1, p1, components=16); te = time.perf_counter() - t
-t = time.perf_counter(); po.evaluate_system(s1, p1, components=8); td = time.perf_counter() - t
-# electrostatics real-space loops are all-pairs in the reference (N^2); dispersion pairs scale with N
-res['electrostatics'] = te * (4096 / 256) ** 2
-res['dispersion'] = td * (4096 / 256) ** 2
-total = sum(res.values())
-print(json.dumps(dict(seconds_per_eval=total, parts=res, n2_sample=n2, n3_sample=n3, ndimers=len(pairs),
-                      ntrimers=len(trip))))
-''' % dict(root=ROOT, oracle=os.path.join(ROOT, 'oracle'), budget=budget_s)
+t = time.perf_counter(); opos = pos[0::4]; pairs = po.dimer_list(opos, 0.65, True, box)
+trip = po.trimer_list(opos, 0.45, True, box); res['lists'] = time.perf_counter() - t
+k2 = min(2000, len(pairs)); k3 = min(2000, len(trip))
+t = time.perf_counter(); po.twobody(pos, idx, pairs[:k2], True, box); res['twobody'] = (time.perf_counter() - t) * len(pairs) / k2
+t = time.perf_counter(); po.threebody(pos, idx, trip[:k3], True, box); res['threebody'] = (time.perf_counter() - t) * len(trip) / k3
+sd = m.forcefield.create_system(4096, nonbonded_method='PME', box=L, components=8)
+t = time.perf_counter(); po.evaluate_system(sd, pos, components=8); res['dispersion'] = time.perf_counter() - t
+se = m.forcefield.create_system(4096, nonbonded_method='PME', box=L, ewald_error_tolerance=1e-5, dipole_epsilon=1e9,
+                                components=16)
+E, F, Ls = po.evaluate_system(se, pos, components=16, return_lists=True)
+r = Ls['elec']
+res['electrostatics'] = r['setup_s'] + %(iters)d * r['iteration_s'] + r['energy_s']
+print(json.dumps(dict(parts=res, n2=k2, n3=k3, ndimers=len(pairs), ntrimers=len(trip),
+                      elec=dict(setup_s=r['setup_s'], iteration_s=r['iteration_s'], energy_s=r['energy_s']))))
+''' % dict(root=ROOT, oracle=os.path.join(ROOT, 'oracle'), iters=gpu_iterations)
     env = dict(os.environ, OMP_NUM_THREADS='1')
     cmd = [sys.executable, '-c', code]
     if os.path.exists('/usr/bin/taskset'):
-        cmd = ['taskset', '-c', '0'] + cmd
-    out = subprocess.run(cmd, capture_output=True, text=True, env=env, timeout=600)
+        cmd = ['/usr/bin/taskset', '-c', '0'] + cmd
+    out = subprocess.run(cmd, capture_output=True, text=True, env=env, timeout=900)
     if out.returncode != 0:
-        return {'value': None, 'error': out.stderr[-500:]}
+        return {'value': None, 'unit': 'evals/s', 'cores': 1, 'kind': 'port', 'error': out.stderr[-400:]}
     r = json.loads(out.stdout.strip().splitlines()[-1])
-    return {'value': 1.0 / r['seconds_per_eval'], 'unit': 'evals/s', 'cores': 1, 'kind': 'port',
-            'sample': ('FP64 oracle, one core: 1-body on all 4096 waters, 2-body on %d of %d dimers, 3-body on '
-                       '%d of %d trimers (scaled to the full lists), dispersion and all-pairs PME electrostatics '
-                       'timed on the 256-water box and scaled by (4096/256)^2; seconds per full evaluation %.1f'
-                       % (r['n2_sample'], r['ndimers'], r['n3_sample'], r['ntrimers'], r['seconds_per_eval'])),
-            'parts_s': r['parts']}
+    total = sum(r['parts'].values())
+    return {'value': 1.0 / total, 'unit': 'evals/s', 'cores': 1, 'kind': 'port',
+            'sample': ('FP64 oracle (oracle/, single thread, taskset core 0) on the same 4096-water evaluation: '
+                       '1-body, O-O lists and dispersion in full; 2-body on %d of %d dimers and 3-body on %d of %d '
+                       'trimers, scaled to the full lists; electrostatics run with one dipole iteration, '
+                       'setup + energy + %d x the measured iteration time. Estimated seconds per evaluation: %.1f'
+                       % (r['n2'], r['ndimers'], r['n3'], r['ntrimers'], gpu_iterations, total)),
+            'seconds_per_eval_by_component': r['parts'], 'electrostatics_phases_s': r['elec']}
 
 
 def main():
@@ -111,33 +159,54 @@ def main():
     if args.gpus != 1:
         print('bench.py: this build runs on one GPU only (--gpus %d requested)' % args.gpus, file=sys.stderr)
         sys.exit(2)
-    import ctypes
-    m, sysm, pos, L = workload()
+    import mbpol_openmm_plugin_amd as m
+    sysm, pos, L = workload(m)
     ctx = m.Context(sysm, precision=args.precision)
-    Lib = m.library()
-    h = ctx._h
-    posd = np.ascontiguousarray(sysm.computeVirtualSites(pos))
-    m.api._check(Lib.mbpol_context_upload_positions(h, posd.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
-    energies = np.zeros(6)
-    ep = energies.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    ctx.uploadPositions(pos)
     for _ in range(args.warmup):
-        m.api._check(Lib.mbpol_context_evaluate_resident(h, ep, ctypes.c_uint32(31)))
-    m.api._check(Lib.mbpol_context_set_timing(h, 1))
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        m.api._check(Lib.mbpol_context_evaluate_resident(h, ep, ctypes.c_uint32(31)))
-    t1 = time.perf_counter()
+        ctx.evaluateResident()
+    # timed region: throughput, no events
+    ms_per_step, energies = run_steps(ctx, args.steps)
     stats = ctx.getStats()
+    # second timed region with HIP events on the library's stream
+    ctx.setTiming(True)
+    run_steps(ctx, args.steps)
     timing = {}
-    for name in ('evaluate', 'lists', 'onebody', 'twobody', 'threebody', 'dispersion', 'elec_setup', 'elec_iteration',
-                 'elec_energy'):
-        ms, cnt = ctypes.c_double(0), ctypes.c_int64(0)
-        m.api._check(Lib.mbpol_context_get_timing(h, name.encode(), ctypes.byref(ms), ctypes.byref(cnt)))
-        timing[name] = (ms.value, cnt.value)
-    ms_per_step = (t1 - t0) * 1000.0 / args.steps
-    # roofline of the 3-body PIP kernel (the largest single kernel)
-    t3_ms = timing['threebody'][0] / max(timing['threebody'][1], 1)
-    achieved = FLOPS_PER_TRIMER * stats['num_trimers'] / (t3_ms * 1e-3) / 1e12
+    for name in PHASES + tuple(kernel_models(stats, (2, 2, 2), 1, 1)):
+        ms, cnt = ctx.getTiming(name)
+        timing[name] = {'ms_per_step': ms / args.steps, 'launches_per_step': cnt / args.steps,
+                        'avg_us': 1000.0 * ms / cnt if cnt else None}
+    ctx.setTiming(False)
+    el = [f for f in sysm._forces if type(f).__name__ == 'MBPolElectrostaticsForce'][0]
+    alpha, grid = el.getAEwald(), el.getPmeGridDimensions()
+    if grid[0] == 0:  # automatic choice, as the library makes it
+        import math
+        tol, rc = el.getEwaldErrorTolerance(), el.getCutoffDistance()
+        alpha = math.sqrt(-math.log(2 * tol)) / rc
+        grid = [max(int(math.ceil(2 * alpha * L / (3 * tol ** 0.2))), 6)] * 3
+    nsites = sysm.getNumParticles()
+    npol = sum(1 for i in range(el.getNumElectrostatics()) if el.getElectrostaticsParameters(i)[4] != 0.0)
+    models = kernel_models(stats, grid, nsites, npol)
+    kernels = {}
+    for name, (kind, amount, bound, peak, unit) in models.items():
+        t = timing[name]
+        if not t['avg_us']:
+            continue
+        per_launch = amount  # every model is per launch
+        rate = per_launch / (t['avg_us'] * 1e-6) / (1e12 if kind == 'flops' else 1e9)
+        kernels[name] = {'ms_per_step': t['ms_per_step'], 'launches_per_step': t['launches_per_step'],
+                         'avg_us': t['avg_us'], kind + '_per_launch': per_launch, 'achieved': rate,
+                         'unit': unit, 'peak': peak, 'frac': rate / peak, 'bound': bound,
+                         'traffic': MEASURED_TRAFFIC.get(name)}
+    dom = max(kernels, key=lambda k: kernels[k]['ms_per_step'])
+    d = kernels[dom]
+    roofline = {'bound': d['bound'], 'achieved': d['achieved'], 'peak': d['peak'], 'unit': d['unit'],
+                'frac': d['frac'], 'traffic': MEASURED_TRAFFIC.get(dom), 'kernel': dom,
+                'avg_kernel_us': d['avg_us'], 'launches_per_step': d['launches_per_step'],
+                'share_of_step': d['ms_per_step'] / ms_per_step,
+                'algorithmic_' + ('flops' if 'flops_per_launch' in d else 'bytes') + '_per_launch':
+                    d.get('flops_per_launch', d.get('bytes_per_launch')),
+                'all_kernels': kernels}
     line = {
         'metric': 'MB-pol E+F evals/s @4096 waters PBC',
         'value': 1000.0 / ms_per_step,
@@ -149,21 +218,38 @@ def main():
         'higher_is_better': True,
         'scaling': 'weak',
         'vs_baseline': None,
-        'dtype': 'f32+f64' if args.precision == 'mixed' else 'f64',
-        'data': 'synthetic (seeded jittered 16x16x16 lattice, 4096 waters, 49.66 A box)',
-        'config': {'workload': 'MB-pol 4096 waters PBC, PME tol 1e-5 (alpha %.6f, grid 121^3), rc 0.9 nm, '
-                               'dipole eps 1e-8, %s precision' % (3.654826, args.precision),
+        'dtype': {'mixed': 'f64 (f32 dispersion pair terms)', 'double': 'f64',
+                  'single-pip': 'f32 polynomials, f64 elsewhere'}[args.precision],
+        'data': 'synthetic: seeded jittered 16x16x16 lattice, 4096 waters, cubic 49.66 A box',
+        'config': {'workload': 'MB-pol, 4096 waters, PBC 49.66 A, PME tol 1e-5 (alpha %.6f nm^-1, grid %dx%dx%d), '
+                               'rc 0.9 nm, 2B rc 0.65 nm, 3B rc 0.45 nm, dipole epsilon 1e-8, %s precision'
+                               % (alpha, grid[0], grid[1], grid[2], args.precision),
                    'waters': 4096, 'box_nm': L, 'precision': args.precision},
-        'roofline': {'bound': 'mfma', 'kernel': 'threebody PIP (FP32 VALU)', 'achieved': achieved,
-                     'peak': PEAK_FP32_TFLOPS, 'unit': 'TFLOP/s', 'frac': achieved / PEAK_FP32_TFLOPS,
-                     'traffic': None, 'flops_per_trimer': FLOPS_PER_TRIMER, 'trimers': stats['num_trimers'],
-                     'avg_kernel_ms': t3_ms},
-        'breakdown_ms_per_step': {k: v[0] / max(args.steps, 1) for k, v in timing.items()},
         'iterations': stats['iterations'],
-        'energies_kj_mol': energies.tolist(),
+        'counts': {k: stats[k] for k in ('num_dimers', 'num_trimers', 'num_elec_pairs', 'num_pol_pairs')},
+        'energies_kj_mol': dict(zip(['onebody', 'twobody', 'threebody', 'dispersion', 'electrostatics', 'total'],
+                                    energies.tolist())),
+        'breakdown_ms_per_step': {k: timing[k]['ms_per_step'] for k in PHASES},
+        'roofline': roofline,
     }
+    if not args.no_fp64_path and args.precision != 'double':
+        f_mixed = ctx.downloadForces()
+        ctx64 = m.Context(sysm, precision='double')
+        ctx64.uploadPositions(pos)
+        for _ in range(args.warmup):
+            ctx64.evaluateResident()
+        ms64, e64 = run_steps(ctx64, args.steps)
+        f64 = ctx64.downloadForces()
+        tot = abs(e64[5])
+        line['fp64_path'] = {
+            'value': 1000.0 / ms64, 'ms_per_step': ms64,
+            'mixed_vs_fp64_rel_error_of_total_by_component': dict(
+                zip(['onebody', 'twobody', 'threebody', 'dispersion', 'electrostatics', 'total'],
+                    (np.abs(energies - e64) / tot).tolist())),
+            'mixed_vs_fp64_rel_rms_force': float(np.sqrt(((f_mixed - f64) ** 2).mean() / (f64 ** 2).mean()))}
+        del ctx64
     if not args.no_cpu_baseline:
-        line['cpu_baseline'] = cpu_baseline(args.cpu_sample_seconds)
+        line['cpu_baseline'] = cpu_baseline(stats['iterations'])
     print(json.dumps(line), flush=True)
 
 

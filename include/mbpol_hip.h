@@ -86,6 +86,7 @@ typedef struct {
     int64_t num_trimers;       /* entries of the 3-body work list */
     int64_t num_elec_pairs;    /* real-space site pairs within the electrostatics cutoff (PME) */
     int64_t num_disp_pairs;    /* dispersion pairs within the cutoff */
+    int64_t num_pol_pairs;     /* directed (polarisable, polarisable) site pairs the dipole iteration visits */
 } mbpol_eval_stats;
 
 /* thread-local message of the last failed call on this thread */
@@ -139,8 +140,10 @@ int mbpol_context_upload_positions(mbpol_context* ctx, const double* positions);
 int mbpol_context_evaluate_resident(mbpol_context* ctx, double* energies_out, uint32_t components);
 int mbpol_context_download_forces(mbpol_context* ctx, double* forces_out);
 /* per-kernel timing with HIP events on the context's stream (off by default);
- * name one of "twobody", "threebody", "elec_iteration", "elec_setup", "elec_energy", "dispersion", "lists",
- * "onebody", "evaluate"; returns accumulated milliseconds and the number of timed launches since the last reset */
+ * name one of the phases "evaluate", "onebody", "lists", "twobody", "threebody", "dispersion", "elec_setup",
+ * "elec_iteration", "elec_energy", or one of the single kernels of the dipole iteration "pme_spread",
+ * "fft_z_r2c", "fft_y", "fft_x_conv", "fft_z_c2r", "elec_iterate"; returns accumulated milliseconds and the
+ * number of timed spans since timing was last enabled */
 int mbpol_context_set_timing(mbpol_context* ctx, int32_t enabled);
 int mbpol_context_get_timing(mbpol_context* ctx, const char* name, double* total_ms, int64_t* count);
 int mbpol_context_set_box(mbpol_context* ctx, const double box_nm[3]);

@@ -34,7 +34,8 @@ class MBPolError(RuntimeError):
 class _EvalStats(ctypes.Structure):
     _fields_ = [('iterations', ctypes.c_int32), ('epsilon', ctypes.c_double), ('converged', ctypes.c_int32),
                 ('num_dimers', ctypes.c_int64), ('num_trimers', ctypes.c_int64),
-                ('num_elec_pairs', ctypes.c_int64), ('num_disp_pairs', ctypes.c_int64)]
+                ('num_elec_pairs', ctypes.c_int64), ('num_disp_pairs', ctypes.c_int64),
+                ('num_pol_pairs', ctypes.c_int64)]
 
 
 _lib = None
@@ -506,6 +507,35 @@ class Context:
         mask = self._mask if components is None else int(components)
         _check(library().mbpol_context_evaluate(self._h, _pd(pos), _pd(forces), _pd(energies), ctypes.c_uint32(mask)))
         return State(energies, forces, self.getStats())
+
+    # -- device-resident evaluation and kernel timing (HIP extensions, no reference counterpart) --
+    def uploadPositions(self, positions):
+        """Copy positions (virtual sites recomputed) into the context's device buffer."""
+        self._positions = self.system.computeVirtualSites(positions)
+        _check(library().mbpol_context_upload_positions(self._h, _pd(_d(self._positions))))
+
+    def evaluateResident(self, components=None):
+        """Evaluate on the uploaded positions; forces stay on the device. Returns the 6 energies (kJ/mol)."""
+        e = np.zeros(6)
+        mask = self._mask if components is None else int(components)
+        _check(library().mbpol_context_evaluate_resident(self._h, _pd(e), ctypes.c_uint32(mask)))
+        return e
+
+    def downloadForces(self):
+        """Forces (N x 3, kJ/mol/nm) of the last resident evaluation."""
+        f = np.zeros((self.system.getNumParticles(), 3))
+        _check(library().mbpol_context_download_forces(self._h, _pd(f)))
+        return f
+
+    def setTiming(self, enabled):
+        """Enable (and reset) or disable HIP-event timing of phases and iteration kernels."""
+        _check(library().mbpol_context_set_timing(self._h, int(bool(enabled))))
+
+    def getTiming(self, name):
+        """(accumulated milliseconds, number of timed spans) of a phase or kernel."""
+        ms, cnt = ctypes.c_double(0), ctypes.c_int64(0)
+        _check(library().mbpol_context_get_timing(self._h, name.encode(), ctypes.byref(ms), ctypes.byref(cnt)))
+        return ms.value, cnt.value
 
     def getStats(self):
         st = _EvalStats()

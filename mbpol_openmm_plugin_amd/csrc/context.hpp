@@ -68,8 +68,10 @@ struct CellGrid {
 };
 
 // HIP-event timing of kernel groups on the context's stream.
+// timed spans: whole phases, and single kernels of the electrostatics iteration
 enum TimerCat { T_EVALUATE, T_ONEBODY, T_LISTS, T_TWOBODY, T_THREEBODY, T_DISPERSION, T_ELEC_SETUP, T_ELEC_ITER,
-                T_ELEC_ENERGY, T_NCAT };
+                T_ELEC_ENERGY, T_PME_SPREAD, T_FFT_Z_R2C, T_FFT_Y, T_FFT_X_CONV, T_FFT_Z_C2R, T_ELEC_ITERATE,
+                T_NCAT };
 struct Timer {
     bool enabled = false;
     std::vector<hipEvent_t> pool;
@@ -83,7 +85,7 @@ struct EvalStats {
     int iterations = 0;
     double epsilon = 0;
     int converged = 1;
-    long long dimers = 0, trimers = 0, elec_pairs = 0, disp_pairs = 0;
+    long long dimers = 0, trimers = 0, elec_pairs = 0, disp_pairs = 0, pol_pairs = 0;
 };
 
 }  // namespace mbpol

@@ -12,6 +12,9 @@ int scan_i32(mbpol_context* ctx, const int* in, int* out, int n);
 int scan_i64(mbpol_context* ctx, const long long* in, long long* out, int n);
 void timer_start(mbpol_context* ctx, int cat);
 void timer_stop(mbpol_context* ctx, int cat);
+// events for one timed kernel launch (null pair when timing is off) and their bookkeeping after the launch
+KEvents timer_kernel(mbpol_context* ctx, int cat);
+void timer_kernel_done(mbpol_context* ctx, int cat, const KEvents& ev);
 
 // electrostatics (elec_driver.hip)
 int elec_create(mbpol_context* ctx);

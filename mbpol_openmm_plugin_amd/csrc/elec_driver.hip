@@ -227,7 +227,8 @@ int elec_set_box(mbpol_context* ctx) {
     const double fp = E->n > 0 ? (double)E->npol / E->n : 0.0;
     E->capP = std::max(E->capP, cap(1.25 * mean * fp + 48));
     E->capQ = std::max(E->capQ, cap(1.25 * mean * (1.0 - fp) + 48));
-    launch_influence(ctx->stream, P, E->bx.as<double>(), E->by.as<double>(), E->bz.as<double>(), E->influence.as<double>());
+    launch_influence(ctx->stream, P, E->fft.khp, E->bx.as<double>(), E->by.as<double>(), E->bz.as<double>(),
+                     E->influence.as<double>());
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     return MBPOL_OK;
 }
@@ -294,7 +295,15 @@ int elec_create(mbpol_context* ctx) {
     }
     if (P.pme) {
         const size_t nreal = (size_t)grid[0] * grid[1] * grid[2];
-        const size_t nhalf = (size_t)grid[0] * grid[1] * (grid[2] / 2 + 1);
+        bool lds = true;
+        for (int d = 0; d < 3; ++d) lds = lds && fft_factor(grid[d], E->fft.ax[d]);
+        const char* force = std::getenv("MBPOL_HIP_PME_FFT");  // "hipfft" selects the library transforms
+        if (force && std::string(force) == "hipfft") lds = false;
+        E->lds_fft = lds;
+        // half-spectrum rows: padded to 128 bytes for the LDS transforms, packed for hipFFT
+        const int kh = grid[2] / 2 + 1;
+        E->fft.khp = lds ? (kh + 7) / 8 * 8 : kh;
+        const size_t nhalf = (size_t)grid[0] * grid[1] * E->fft.khp;
         HIP_TRY(E->grid.ensure(sizeof(double) * nreal));
         HIP_TRY(E->spec.ensure(sizeof(double2) * nhalf));
         HIP_TRY(E->influence.ensure(sizeof(double) * nhalf));
@@ -311,11 +320,6 @@ int elec_create(mbpol_context* ctx) {
         HIP_TRY(E->g0tab.ensure(sizeof(int) * 3 * std::max(n, 1)));
         HIP_TRY(E->sptab.ensure(sizeof(double) * 30 * std::max(n, 1)));
         for (DevBuf* b : {&E->rowcount, &E->rowstart, &E->rowfill}) HIP_TRY(b->ensure(sizeof(int) * (nreal + 1)));
-        bool lds = true;
-        for (int d = 0; d < 3; ++d) lds = lds && fft_factor(grid[d], E->fft.ax[d]);
-        const char* force = std::getenv("MBPOL_HIP_PME_FFT");  // "hipfft" selects the library transforms
-        if (force && std::string(force) == "hipfft") lds = false;
-        E->lds_fft = lds;
         if (lds) {
             for (int d = 0; d < 3; ++d) {
                 const int m = grid[d];
@@ -382,7 +386,7 @@ static int build_rows(mbpol_context* ctx) {
         launch_site_lists(st, n, pos, E->pol.as<double>(), E->P, E->nc, E->reach, E->use_cells, E->cell_of.as<int>(),
                           E->cell_start.as<int>(), E->members.as<int>(), E->capP, E->capQ, E->nbP.as<int>(),
                           E->cntP.as<int>(), E->nbQ.as<int>(), E->cntQ.as<int>(), E->maxcnt.as<int>());
-        HIP_TRY(hipMemcpyAsync(h, E->maxcnt.p, sizeof(int) * 3, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipMemcpyAsync(h, E->maxcnt.p, sizeof(int) * 4, hipMemcpyDeviceToHost, st));
         HIP_TRY(hipStreamSynchronize(st));
         if (h[0] <= E->capP && h[1] <= E->capQ) break;
         if (attempt == 1) return set_error(MBPOL_ERR_HIP, "electrostatics neighbour rows overflowed twice");
@@ -390,6 +394,7 @@ static int build_rows(mbpol_context* ctx) {
         E->capQ = std::max(E->capQ, (h[1] + h[1] / 4 + 63) / 32 * 32);
     }
     ctx->stats.elec_pairs = h[2] / 2;
+    ctx->stats.pol_pairs = h[3];
     launch_prefactors(st, E->npol, E->polidx.as<int>(), pos, site_params(E), E->P, E->capP, E->nbP.as<int>(),
                       E->cntP.as<int>(), E->pf.as<double2>());
     if (E->P.pme) {
@@ -425,14 +430,32 @@ static int pme_grid(mbpol_context* ctx, int mode, const double* mu) {
     hipStream_t st = ctx->stream;
     const ElecParams& P = E->P;
     const size_t nhalf = (size_t)P.grid.x * P.grid.y * (P.grid.z / 2 + 1);
-    if (mode == 0)
+    if (mode == 0) {
         launch_spread_bricks(st, mode, E->q.as<double>(), mu, P, E->B, E->binq.start.as<int>(),
                              E->binq.members.as<int>(), E->g0tab.as<int>(), E->sptab.as<double>(), E->grid.as<double>());
-    else
+    } else {
+        const KEvents ev = timer_kernel(ctx, T_PME_SPREAD);
         launch_spread_csr(st, (size_t)P.grid.x * P.grid.y * P.grid.z, E->rowstart.as<int>(), E->esite.as<int>(),
-                          E->ecoef.as<double>(), mu, E->grid.as<double>());
+                          E->ecoef.as<double>(), mu, E->grid.as<double>(), ev);
+        timer_kernel_done(ctx, T_PME_SPREAD, ev);
+    }
     if (E->lds_fft) {
-        launch_pme_convolution(st, E->fft, E->grid.as<double>(), E->spec.as<double2>(), E->influence.as<double>());
+        double2* H = E->spec.as<double2>();
+        KEvents ev = timer_kernel(ctx, T_FFT_Z_R2C);
+        launch_fft_z_r2c(st, E->fft, E->grid.as<double>(), H, ev);
+        timer_kernel_done(ctx, T_FFT_Z_R2C, ev);
+        ev = timer_kernel(ctx, T_FFT_Y);
+        launch_fft_y(st, E->fft, H, 0, ev);
+        timer_kernel_done(ctx, T_FFT_Y, ev);
+        ev = timer_kernel(ctx, T_FFT_X_CONV);
+        launch_fft_x_conv(st, E->fft, H, E->influence.as<double>(), ev);
+        timer_kernel_done(ctx, T_FFT_X_CONV, ev);
+        ev = timer_kernel(ctx, T_FFT_Y);
+        launch_fft_y(st, E->fft, H, 1, ev);
+        timer_kernel_done(ctx, T_FFT_Y, ev);
+        ev = timer_kernel(ctx, T_FFT_Z_C2R);
+        launch_fft_z_c2r(st, E->fft, H, E->grid.as<double>(), ev);
+        timer_kernel_done(ctx, T_FFT_Z_C2R, ev);
     } else {
         FFT_TRY(hipfftExecD2Z(E->plan_f, E->grid.as<double>(), reinterpret_cast<hipfftDoubleComplex*>(E->spec.p)));
         launch_convolve(st, nhalf, E->influence.as<double>(), E->spec.as<double2>());
@@ -496,9 +519,12 @@ int elec_evaluate(mbpol_context* ctx, u64* energy) {
         timer_start(ctx, T_ELEC_ITER);
         if (P.pme)
             if (int e = pme_grid(ctx, 1, mu_in)) return e;
+        const KEvents ev = timer_kernel(ctx, T_ELEC_ITERATE);
         launch_iterate(st, E->npol, E->polidx.as<int>(), pos, P, R, mu_in, mu_out, E->efix_pol.as<double>(),
-                       E->pol.as<double>(), E->grid.as<double>(), self_term, mbpol_params::el_polar_sor, n,
-                       E->partial.as<double>(), E->eps.as<double>());
+                       E->pol.as<double>(), E->grid.as<double>(), self_term, mbpol_params::el_polar_sor,
+                       E->partial.as<double>(), ev);
+        timer_kernel_done(ctx, T_ELEC_ITERATE, ev);
+        launch_eps_finish(st, iterate_blocks(E->npol), n, E->partial.as<double>(), E->eps.as<double>());
         timer_stop(ctx, T_ELEC_ITER);
         HIP_TRY(hipMemcpyAsync(E->h_eps, E->eps.p, sizeof(double), hipMemcpyDeviceToHost, st));
         HIP_TRY(hipStreamSynchronize(st));

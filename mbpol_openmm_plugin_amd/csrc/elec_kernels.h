@@ -62,16 +62,19 @@ void launch_csr_fill(hipStream_t s, int npol, const int* polidx, const int* g0ta
                      const int* rowstart, int* fill, int* esite, double* ecoef);
 void launch_csr_sort(hipStream_t s, size_t ngrid, const int* rowstart, int* esite, double* ecoef);
 void launch_spread_csr(hipStream_t s, size_t ngrid, const int* rowstart, const int* esite, const double* ecoef,
-                       const double* mu, double* grid);
+                       const double* mu, double* grid, const KEvents& ev);
 void launch_convolve(hipStream_t s, size_t nhalf, const double* influence, double2* spec);
-void launch_influence(hipStream_t s, ElecParams P, const double* bx, const double* by, const double* bz, double* out);
+// influence function on the half grid, rows of khp entries (entries kz >= K3/2 + 1 are zero)
+void launch_influence(hipStream_t s, ElecParams P, int khp, const double* bx, const double* by, const double* bz,
+                      double* out);
 void launch_gather(hipStream_t s, int n, const double* pos, ElecParams P, const double* grid, double* phi);
 void launch_record_field(hipStream_t s, int n, ElecParams P, const double* phi, int accumulate, double self_term,
                          const double* mu, double* field);
 int iterate_blocks(int npol);
 void launch_iterate(hipStream_t s, int npol, const int* polidx, const double* pos, ElecParams P, const ElecRows& R,
                     const double* mu_in, double* mu_out, const double* efix_pol, const double* pol, const double* grid,
-                    double self_term, double omega, int n_all, double* partial, double* eps);
+                    double self_term, double omega, double* partial, const KEvents& ev);
+void launch_eps_finish(hipStream_t s, int nblocks, int n_all, const double* partial, double* eps);
 void launch_scale_copy(hipStream_t s, int n, const double* a, const double* pol, double* out, double* out2);
 int energy_blocks(int n);
 int sor_blocks(int n);

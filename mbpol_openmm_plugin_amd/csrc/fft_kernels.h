@@ -21,11 +21,16 @@ struct FftAxis {
 
 struct PmeFft {
     FftAxis ax[3];  // x, y, z
+    int khp;        // padded kz row length of the half spectrum (>= n_z / 2 + 1, multiple of 8)
 };
 
 // longest axis the LDS transforms take: one line per workgroup, two buffers in 32 KB
 constexpr int kFftMaxLength = 1024;
-// real grid -> half spectrum -> times influence -> real grid, in place on `real` (H is the spectrum buffer)
-void launch_pme_convolution(hipStream_t s, const PmeFft& F, double* real, double2* H, const double* influence);
+// real grid -> half spectrum H -> (x pass) times influence -> real grid, as five passes:
+// z_r2c, y (forward), x_conv, y (inverse), z_c2r
+void launch_fft_z_r2c(hipStream_t s, const PmeFft& F, const double* real, double2* H, const KEvents& ev);
+void launch_fft_y(hipStream_t s, const PmeFft& F, double2* H, int inverse, const KEvents& ev);
+void launch_fft_x_conv(hipStream_t s, const PmeFft& F, double2* H, const double* influence, const KEvents& ev);
+void launch_fft_z_c2r(hipStream_t s, const PmeFft& F, const double2* H, double* real, const KEvents& ev);
 
 }  // namespace mbpol

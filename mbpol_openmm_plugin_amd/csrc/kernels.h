@@ -2,10 +2,26 @@
 // (host) code: small POD types and the launcher functions each kernel file
 // exports.  No kernel is called across translation units directly.
 #pragma once
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
 namespace mbpol {
+
+// Start/stop events attached to one kernel launch (hipExtLaunchKernelGGL
+// records them at the kernel's own start and end, so the span excludes the
+// dispatch of the launch); both null when the launch is not timed.
+struct KEvents {
+    hipEvent_t start = nullptr, stop = nullptr;
+};
+
+#define MBPOL_LAUNCH(ev, kernel, grid, block, shmem, stream, ...)                                            \
+    do {                                                                                                     \
+        if ((ev).start)                                                                                      \
+            hipExtLaunchKernelGGL(kernel, grid, block, shmem, stream, (ev).start, (ev).stop, 0, __VA_ARGS__); \
+        else                                                                                                 \
+            hipLaunchKernelGGL(kernel, grid, block, shmem, stream, __VA_ARGS__);                             \
+    } while (0)
 
 struct d3 {
     double x, y, z;
@@ -45,12 +61,12 @@ void launch_trimer_pass(hipStream_t s, int n, const int* idx, const double* pos,
 
 // ---- kernels_pip.hip (one object per body order and precision) ----
 void launch_twobody_f32(hipStream_t s, long long npairs, const int2* pairs, const int* idx, const double* pos,
-                        int periodic, d3 box, u64* force, u64* energy);
+                        int periodic, d3 box, u64* force, u64* energy, const KEvents& ev = KEvents());
 void launch_twobody_f64(hipStream_t s, long long npairs, const int2* pairs, const int* idx, const double* pos,
-                        int periodic, d3 box, u64* force, u64* energy);
+                        int periodic, d3 box, u64* force, u64* energy, const KEvents& ev = KEvents());
 void launch_threebody_f32(hipStream_t s, long long ntrip, const int* trip, const int* idx, const double* pos,
-                          int periodic, d3 box, u64* force, u64* energy);
+                          int periodic, d3 box, u64* force, u64* energy, const KEvents& ev = KEvents());
 void launch_threebody_f64(hipStream_t s, long long ntrip, const int* trip, const int* idx, const double* pos,
-                          int periodic, d3 box, u64* force, u64* energy);
+                          int periodic, d3 box, u64* force, u64* energy, const KEvents& ev = KEvents());
 
 }  // namespace mbpol

@@ -197,7 +197,8 @@ __global__ void k_sort_segments(int nseg, const int* __restrict__ start, int* __
 // lanes per site; candidates are taken in cell order and compacted with a
 // wavefront ballot.  Counts are kept beyond the capacity so that the host can
 // detect an overflow (maxcnt[0..1]) and rebuild with larger rows; maxcnt[2]
-// accumulates the number of directed pairs.
+// accumulates the number of directed pairs, maxcnt[3] the P entries of the
+// polarisable sites (the rows the dipole iteration streams).
 template <int G>
 __global__ void __launch_bounds__(256) k_site_lists(int n, const double* __restrict__ pos, const double* __restrict__ pol,
                                                     ElecParams P, int3 nc, int reach, int use_cells,
@@ -261,6 +262,7 @@ __global__ void __launch_bounds__(256) k_site_lists(int n, const double* __restr
         atomicMax(&maxcnt[0], cP);
         atomicMax(&maxcnt[1], cQ);
         atomicAdd(&maxcnt[2], cP + cQ);
+        if (pol[i] != 0.0) atomicAdd(&maxcnt[3], cP);
     }
 }
 
@@ -744,14 +746,13 @@ __global__ void k_convolve(size_t nhalf, const double* __restrict__ influence, d
 }
 
 // influence function on the half grid (kz <= K3/2)
-__global__ void k_influence(ElecParams P, const double* __restrict__ bx, const double* __restrict__ by,
+__global__ void k_influence(ElecParams P, int khp, const double* __restrict__ bx, const double* __restrict__ by,
                             const double* __restrict__ bz, double* __restrict__ out) {
-    const int Kz = P.grid.z / 2 + 1;
-    const size_t total = (size_t)P.grid.x * P.grid.y * Kz;
+    const size_t total = (size_t)P.grid.x * P.grid.y * khp;
     const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= total) return;
-    const int kz = (int)(t % Kz), ky = (int)((t / Kz) % P.grid.y), kx = (int)(t / ((size_t)Kz * P.grid.y));
-    if (kx == 0 && ky == 0 && kz == 0) {
+    const int kz = (int)(t % khp), ky = (int)((t / khp) % P.grid.y), kx = (int)(t / ((size_t)khp * P.grid.y));
+    if ((kx == 0 && ky == 0 && kz == 0) || kz >= P.grid.z / 2 + 1) {
         out[t] = 0.0;
         return;
     }
@@ -1198,15 +1199,16 @@ void launch_csr_sort(hipStream_t s, size_t ngrid, const int* rowstart, int* esit
     hipLaunchKernelGGL(k_csr_sort, nblk(ngrid, 256), 256, 0, s, ngrid, rowstart, esite, ecoef);
 }
 void launch_spread_csr(hipStream_t s, size_t ngrid, const int* rowstart, const int* esite, const double* ecoef,
-                       const double* mu, double* grid) {
-    hipLaunchKernelGGL(k_spread_csr, nblk(ngrid, 256), 256, 0, s, ngrid, rowstart, esite, ecoef, mu, grid);
+                       const double* mu, double* grid, const KEvents& ev) {
+    MBPOL_LAUNCH(ev, k_spread_csr, nblk(ngrid, 256), 256, 0, s, ngrid, rowstart, esite, ecoef, mu, grid);
 }
 void launch_convolve(hipStream_t s, size_t nhalf, const double* influence, double2* spec) {
     hipLaunchKernelGGL(k_convolve, nblk(nhalf, 256), 256, 0, s, nhalf, influence, spec);
 }
-void launch_influence(hipStream_t s, ElecParams P, const double* bx, const double* by, const double* bz, double* out) {
-    const size_t total = (size_t)P.grid.x * P.grid.y * (P.grid.z / 2 + 1);
-    hipLaunchKernelGGL(k_influence, nblk(total, 256), 256, 0, s, P, bx, by, bz, out);
+void launch_influence(hipStream_t s, ElecParams P, int khp, const double* bx, const double* by, const double* bz,
+                      double* out) {
+    const size_t total = (size_t)P.grid.x * P.grid.y * khp;
+    hipLaunchKernelGGL(k_influence, nblk(total, 256), 256, 0, s, P, khp, bx, by, bz, out);
 }
 void launch_gather(hipStream_t s, int n, const double* pos, ElecParams P, const double* grid, double* phi) {
     hipLaunchKernelGGL(k_gather, nblk(n, 64), 64, 0, s, n, pos, P, grid, phi);
@@ -1218,11 +1220,12 @@ void launch_record_field(hipStream_t s, int n, ElecParams P, const double* phi, 
 int iterate_blocks(int npol) { return std::max(1, nblk((long long)npol * kGI, 256)); }
 void launch_iterate(hipStream_t s, int npol, const int* polidx, const double* pos, ElecParams P, const ElecRows& R,
                     const double* mu_in, double* mu_out, const double* efix_pol, const double* pol, const double* grid,
-                    double self_term, double omega, int n_all, double* partial, double* eps) {
-    const int nb = iterate_blocks(npol);
-    hipLaunchKernelGGL(k_iterate<kGI>, nb, 256, 0, s, npol, polidx, pos, P, R.capP, R.nbP, R.cntP, R.pf, mu_in, mu_out,
-                       efix_pol, pol, grid, self_term, omega, partial);
-    hipLaunchKernelGGL(k_eps_finish, 1, 256, 0, s, nb, n_all, partial, eps);
+                    double self_term, double omega, double* partial, const KEvents& ev) {
+    MBPOL_LAUNCH(ev, k_iterate<kGI>, iterate_blocks(npol), 256, 0, s, npol, polidx, pos, P, R.capP, R.nbP, R.cntP, R.pf,
+                 mu_in, mu_out, efix_pol, pol, grid, self_term, omega, partial);
+}
+void launch_eps_finish(hipStream_t s, int nblocks, int n_all, const double* partial, double* eps) {
+    hipLaunchKernelGGL(k_eps_finish, 1, 256, 0, s, nblocks, n_all, partial, eps);
 }
 void launch_scale_copy(hipStream_t s, int n, const double* a, const double* pol, double* out, double* out2) {
     hipLaunchKernelGGL(k_scale_copy, nblk(3 * n, 256), 256, 0, s, 3 * n, a, pol, out, out2);

@@ -11,8 +11,10 @@
 //      are complete in k-space after the forward transform)
 //   y: inverse
 //   z: half spectrum -> real lines (Hermitian extension)
-// The half-spectrum layout [kx][ky][kz] is the one of a D2Z transform, and
-// both directions are unnormalised, as there.
+// The half spectrum is laid out [kx][ky][kz] as by a D2Z transform, with each
+// kz row padded to khp entries (a multiple of 8, i.e. 128 bytes) so that the
+// kz tiles of the y and x passes fall on whole cache lines; both directions
+// are unnormalised.
 #include <algorithm>
 
 #include "device_common.h"
@@ -182,7 +184,7 @@ __device__ double2* fft_line(double2* a, double2* b, bool active, const FftAxis&
 // z pass, forward: lines of the real grid -> half spectrum
 template <int C>
 __global__ void __launch_bounds__(kFftThreads) k_fft_z_r2c(const double* __restrict__ real, double2* __restrict__ H,
-                                                           int nlines, FftAxis az, int Kh) {
+                                                           int nlines, FftAxis az, int Kh, int Khp) {
     extern __shared__ double2 lds[];
     const int N = az.n, M = kFftThreads >> az.lgP, P = 1 << az.lgP;
     const int L0 = blockIdx.x * M, m = min(M, nlines - L0);
@@ -207,8 +209,8 @@ __global__ void __launch_bounds__(kFftThreads) k_fft_z_r2c(const double* __restr
         for (int u = 0; u < C; ++u) {
             const int k = lane + u * P;
             if (k < Kh) {
-                H[(size_t)(L0 + l) * Kh + k].x = r[k].x;
-                H[(size_t)(L0 + l) * Kh + k].y = r[k].y;
+                H[(size_t)(L0 + l) * Khp + k].x = r[k].x;
+                H[(size_t)(L0 + l) * Khp + k].y = r[k].y;
             }
         }
     }
@@ -216,7 +218,8 @@ __global__ void __launch_bounds__(kFftThreads) k_fft_z_r2c(const double* __restr
 
 // y pass (either direction): block = (x, tile of M consecutive kz)
 template <int C>
-__global__ void __launch_bounds__(kFftThreads) k_fft_y(double2* __restrict__ H, int K1, int Kh, FftAxis ay, int inv) {
+__global__ void __launch_bounds__(kFftThreads) k_fft_y(double2* __restrict__ H, int K1, int Kh, int Khp, FftAxis ay,
+                                                       int inv) {
     extern __shared__ double2 lds[];
     const int N = ay.n, lgM = 7 - ay.lgP, M = 1 << lgM;
     const int ntile = (Kh + M - 1) >> lgM;
@@ -229,7 +232,7 @@ __global__ void __launch_bounds__(kFftThreads) k_fft_y(double2* __restrict__ H, 
         for (int u = 0; u < C; ++u) {
             const int y = cy + u * ystep;
             if (y < N) {
-                const double2* src = H + ((size_t)x * K1 + y) * Kh + kz0 + cl;
+                const double2* src = H + ((size_t)x * K1 + y) * Khp + kz0 + cl;
                 vx[u] = src->x;
                 vy[u] = src->y;
             }
@@ -252,7 +255,7 @@ __global__ void __launch_bounds__(kFftThreads) k_fft_y(double2* __restrict__ H, 
         for (int u = 0; u < C; ++u) {
             const int y = cy + u * ystep;
             if (y < N) {
-                double2* dst = H + ((size_t)x * K1 + y) * Kh + kz0 + cl;
+                double2* dst = H + ((size_t)x * K1 + y) * Khp + kz0 + cl;
                 dst->x = lds[off + cl * N + y].x;
                 dst->y = lds[off + cl * N + y].y;
             }
@@ -263,7 +266,7 @@ __global__ void __launch_bounds__(kFftThreads) k_fft_y(double2* __restrict__ H, 
 // x pass: forward, influence function, inverse; block = (y, tile of M consecutive kz)
 template <int C>
 __global__ void __launch_bounds__(kFftThreads) k_fft_x_conv(double2* __restrict__ H, const double* __restrict__ influence,
-                                                            int K1, int Kh, FftAxis ax) {
+                                                            int K1, int Kh, int Khp, FftAxis ax) {
     extern __shared__ double2 lds[];
     const int N = ax.n, lgM = 7 - ax.lgP, M = 1 << lgM, P = 1 << ax.lgP;
     const int ntile = (Kh + M - 1) >> lgM;
@@ -275,7 +278,7 @@ __global__ void __launch_bounds__(kFftThreads) k_fft_x_conv(double2* __restrict_
 #pragma unroll
     for (int u = 0; u < C; ++u) {
         const int kx = lane + u * P;
-        f[u] = (l < m && kx < N) ? influence[((size_t)kx * K1 + y) * Kh + kz0 + l] : 0.0;
+        f[u] = (l < m && kx < N) ? influence[((size_t)kx * K1 + y) * Khp + kz0 + l] : 0.0;
     }
     if (cl < m) {
         double vx[C], vy[C];
@@ -283,7 +286,7 @@ __global__ void __launch_bounds__(kFftThreads) k_fft_x_conv(double2* __restrict_
         for (int u = 0; u < C; ++u) {
             const int x = cx + u * xstep;
             if (x < N) {
-                const double2* src = H + ((size_t)x * K1 + y) * Kh + kz0 + cl;
+                const double2* src = H + ((size_t)x * K1 + y) * Khp + kz0 + cl;
                 vx[u] = src->x;
                 vy[u] = src->y;
             }
@@ -319,7 +322,7 @@ __global__ void __launch_bounds__(kFftThreads) k_fft_x_conv(double2* __restrict_
         for (int u = 0; u < C; ++u) {
             const int x = cx + u * xstep;
             if (x < N) {
-                double2* dst = H + ((size_t)x * K1 + y) * Kh + kz0 + cl;
+                double2* dst = H + ((size_t)x * K1 + y) * Khp + kz0 + cl;
                 dst->x = lds[off + cl * N + x].x;
                 dst->y = lds[off + cl * N + x].y;
             }
@@ -330,14 +333,14 @@ __global__ void __launch_bounds__(kFftThreads) k_fft_x_conv(double2* __restrict_
 // z pass, inverse: half spectrum -> real lines (kz >= Kh from conj(H[N - kz]))
 template <int C>
 __global__ void __launch_bounds__(kFftThreads) k_fft_z_c2r(const double2* __restrict__ H, double* __restrict__ real,
-                                                           int nlines, FftAxis az, int Kh) {
+                                                           int nlines, FftAxis az, int Kh, int Khp) {
     extern __shared__ double2 lds[];
     const int N = az.n, M = kFftThreads >> az.lgP, P = 1 << az.lgP;
     const int L0 = blockIdx.x * M, m = min(M, nlines - L0);
     const int l = threadIdx.x >> az.lgP, lane = threadIdx.x & (P - 1);
     double2* a = lds + l * N;
     if (l < m) {
-        const size_t row = (size_t)(L0 + l) * Kh;
+        const size_t row = (size_t)(L0 + l) * Khp;
         double vx[C], vy[C];
 #pragma unroll
         for (int u = 0; u < C; ++u) {
@@ -371,26 +374,40 @@ __global__ void __launch_bounds__(kFftThreads) k_fft_z_c2r(const double2* __rest
 
 static inline int nblk(long long n, int b) { return (int)((n + b - 1) / b); }
 
-void launch_pme_convolution(hipStream_t s, const PmeFft& F, double* real, double2* H, const double* influence) {
-    const int K0 = F.ax[0].n, K1 = F.ax[1].n, K2 = F.ax[2].n, Kh = K2 / 2 + 1;
-    const int nl = K0 * K1;
-    const int Mz = kFftThreads >> F.ax[2].lgP, My = kFftThreads >> F.ax[1].lgP, Mx = kFftThreads >> F.ax[0].lgP;
-    const size_t lz = 2 * sizeof(double2) * Mz * K2, ly = 2 * sizeof(double2) * My * K1,
-                 lx = 2 * sizeof(double2) * Mx * K0;
-    // C: elements a thread moves per copy phase, >= ceil(n / P) (at most 13 for the supported lengths)
-    auto copies = [](const FftAxis& a) { const int c = (a.n + (1 << a.lgP) - 1) >> a.lgP; return c <= 4 ? 4 : c <= 8 ? 8 : 16; };
-#define MBPOL_FFT_LAUNCH(kern, axis, ...)                                                              \
-    switch (copies(axis)) {                                                                            \
-        case 4: hipLaunchKernelGGL(kern<4>, __VA_ARGS__); break;                                       \
-        case 8: hipLaunchKernelGGL(kern<8>, __VA_ARGS__); break;                                       \
-        default: hipLaunchKernelGGL(kern<16>, __VA_ARGS__); break;                                     \
-    }
-    MBPOL_FFT_LAUNCH(k_fft_z_r2c, F.ax[2], nblk(nl, Mz), kFftThreads, lz, s, real, H, nl, F.ax[2], Kh)
-    MBPOL_FFT_LAUNCH(k_fft_y, F.ax[1], K0 * nblk(Kh, My), kFftThreads, ly, s, H, K1, Kh, F.ax[1], 0)
-    MBPOL_FFT_LAUNCH(k_fft_x_conv, F.ax[0], K1 * nblk(Kh, Mx), kFftThreads, lx, s, H, influence, K1, Kh, F.ax[0])
-    MBPOL_FFT_LAUNCH(k_fft_y, F.ax[1], K0 * nblk(Kh, My), kFftThreads, ly, s, H, K1, Kh, F.ax[1], 1)
-    MBPOL_FFT_LAUNCH(k_fft_z_c2r, F.ax[2], nblk(nl, Mz), kFftThreads, lz, s, H, real, nl, F.ax[2], Kh)
-#undef MBPOL_FFT_LAUNCH
+// C: elements a thread moves per copy phase, >= ceil(n / P) (at most 13 for the supported lengths)
+static int copies(const FftAxis& a) {
+    const int c = (a.n + (1 << a.lgP) - 1) >> a.lgP;
+    return c <= 4 ? 4 : c <= 8 ? 8 : 16;
 }
+#define MBPOL_FFT_LAUNCH(kern, axis, ...)                        \
+    switch (copies(axis)) {                                      \
+        case 4: MBPOL_LAUNCH(ev, kern<4>, __VA_ARGS__); break;   \
+        case 8: MBPOL_LAUNCH(ev, kern<8>, __VA_ARGS__); break;   \
+        default: MBPOL_LAUNCH(ev, kern<16>, __VA_ARGS__); break; \
+    }
+
+static size_t lds_bytes(const FftAxis& a) { return 2 * sizeof(double2) * (size_t)(kFftThreads >> a.lgP) * a.n; }
+
+void launch_fft_z_r2c(hipStream_t s, const PmeFft& F, const double* real, double2* H, const KEvents& ev) {
+    const int nl = F.ax[0].n * F.ax[1].n, M = kFftThreads >> F.ax[2].lgP, Kh = F.ax[2].n / 2 + 1;
+    MBPOL_FFT_LAUNCH(k_fft_z_r2c, F.ax[2], nblk(nl, M), kFftThreads, lds_bytes(F.ax[2]), s, real, H, nl, F.ax[2], Kh,
+                     F.khp)
+}
+void launch_fft_y(hipStream_t s, const PmeFft& F, double2* H, int inverse, const KEvents& ev) {
+    const int K0 = F.ax[0].n, K1 = F.ax[1].n, Kh = F.ax[2].n / 2 + 1, M = kFftThreads >> F.ax[1].lgP;
+    MBPOL_FFT_LAUNCH(k_fft_y, F.ax[1], K0 * nblk(Kh, M), kFftThreads, lds_bytes(F.ax[1]), s, H, K1, Kh, F.khp, F.ax[1],
+                     inverse)
+}
+void launch_fft_x_conv(hipStream_t s, const PmeFft& F, double2* H, const double* influence, const KEvents& ev) {
+    const int K1 = F.ax[1].n, Kh = F.ax[2].n / 2 + 1, M = kFftThreads >> F.ax[0].lgP;
+    MBPOL_FFT_LAUNCH(k_fft_x_conv, F.ax[0], K1 * nblk(Kh, M), kFftThreads, lds_bytes(F.ax[0]), s, H, influence, K1, Kh,
+                     F.khp, F.ax[0])
+}
+void launch_fft_z_c2r(hipStream_t s, const PmeFft& F, const double2* H, double* real, const KEvents& ev) {
+    const int nl = F.ax[0].n * F.ax[1].n, M = kFftThreads >> F.ax[2].lgP, Kh = F.ax[2].n / 2 + 1;
+    MBPOL_FFT_LAUNCH(k_fft_z_c2r, F.ax[2], nblk(nl, M), kFftThreads, lds_bytes(F.ax[2]), s, H, real, nl, F.ax[2], Kh,
+                     F.khp)
+}
+#undef MBPOL_FFT_LAUNCH
 
 }  // namespace mbpol

@@ -402,16 +402,18 @@ namespace mbpol {
 static inline int nblk(long long n, int b) { return (int)((n + b - 1) / b); }
 #if PIP_BODY == 2
 void PIP_CAT(launch_twobody_, PIP_SUFFIX)(hipStream_t s, long long npairs, const int2* pairs, const int* idx,
-                                          const double* pos, int periodic, d3 box, u64* force, u64* energy) {
+                                          const double* pos, int periodic, d3 box, u64* force, u64* energy,
+                                          const KEvents& ev) {
     const d3 boxA{box.x * 10., box.y * 10., box.z * 10.};
     if (npairs > 0)
-        hipLaunchKernelGGL(k_twobody, nblk(npairs, 64), 64, 0, s, npairs, pairs, idx, pos, periodic, boxA, force, energy);
+        MBPOL_LAUNCH(ev, k_twobody, nblk(npairs, 64), 64, 0, s, npairs, pairs, idx, pos, periodic, boxA, force, energy);
 }
 #else
 void PIP_CAT(launch_threebody_, PIP_SUFFIX)(hipStream_t s, long long ntrip, const int* trip, const int* idx,
-                                            const double* pos, int periodic, d3 box, u64* force, u64* energy) {
+                                            const double* pos, int periodic, d3 box, u64* force, u64* energy,
+                                            const KEvents& ev) {
     if (ntrip > 0)
-        hipLaunchKernelGGL(k_threebody, nblk(ntrip, 64), 64, 0, s, ntrip, trip, idx, pos, periodic, box, force, energy);
+        MBPOL_LAUNCH(ev, k_threebody, nblk(ntrip, 64), 64, 0, s, ntrip, trip, idx, pos, periodic, box, force, energy);
 }
 #endif
 }  // namespace mbpol

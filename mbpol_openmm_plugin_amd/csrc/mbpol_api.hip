@@ -91,6 +91,18 @@ void mbpol::timer_start(mbpol_context* ctx, int cat) {
     ctx->timer.open[cat] = e;
 }
 
+mbpol::KEvents mbpol::timer_kernel(mbpol_context* ctx, int cat) {
+    (void)cat;
+    KEvents ev;
+    if (!ctx->timer.enabled) return ev;
+    ev.start = take_event(ctx);
+    ev.stop = ev.start ? take_event(ctx) : nullptr;
+    if (!ev.stop) ev.start = nullptr;
+    return ev;
+}
+void mbpol::timer_kernel_done(mbpol_context* ctx, int cat, const KEvents& ev) {
+    if (ev.start && ev.stop) ctx->timer.pending[cat].push_back({ev.start, ev.stop});
+}
 void mbpol::timer_stop(mbpol_context* ctx, int cat) {
     Timer& t = ctx->timer;
     if (!t.enabled || !t.open[cat]) return;
@@ -532,21 +544,25 @@ static int evaluate(mbpol_context* ctx, double* forces_dev_out, double* energies
     const bool dp = ctx->precision != MBPOL_PRECISION_SINGLE_PIP;
     if (do2) {
         const int per = s.tb_method == MBPOL_CUTOFF_PERIODIC;
-        timer_start(ctx, T_TWOBODY);
+        const KEvents ev = timer_kernel(ctx, T_TWOBODY);
         if (dp)
-            launch_twobody_f64(st, ctx->stats.dimers, ctx->dimers.as<int2>(), ctx->tb_idx.as<int>(), pos, per, box, F, E + 1);
+            launch_twobody_f64(st, ctx->stats.dimers, ctx->dimers.as<int2>(), ctx->tb_idx.as<int>(), pos, per, box, F, E + 1,
+                               ev);
         else
-            launch_twobody_f32(st, ctx->stats.dimers, ctx->dimers.as<int2>(), ctx->tb_idx.as<int>(), pos, per, box, F, E + 1);
-        timer_stop(ctx, T_TWOBODY);
+            launch_twobody_f32(st, ctx->stats.dimers, ctx->dimers.as<int2>(), ctx->tb_idx.as<int>(), pos, per, box, F, E + 1,
+                               ev);
+        timer_kernel_done(ctx, T_TWOBODY, ev);
     }
     if (do3) {
         const int per = s.th_method == MBPOL_CUTOFF_PERIODIC;
-        timer_start(ctx, T_THREEBODY);
+        const KEvents ev = timer_kernel(ctx, T_THREEBODY);
         if (dp)
-            launch_threebody_f64(st, ctx->stats.trimers, ctx->trips.as<int>(), ctx->th_idx.as<int>(), pos, per, box, F, E + 2);
+            launch_threebody_f64(st, ctx->stats.trimers, ctx->trips.as<int>(), ctx->th_idx.as<int>(), pos, per, box, F,
+                                 E + 2, ev);
         else
-            launch_threebody_f32(st, ctx->stats.trimers, ctx->trips.as<int>(), ctx->th_idx.as<int>(), pos, per, box, F, E + 2);
-        timer_stop(ctx, T_THREEBODY);
+            launch_threebody_f32(st, ctx->stats.trimers, ctx->trips.as<int>(), ctx->th_idx.as<int>(), pos, per, box, F,
+                                 E + 2, ev);
+        timer_kernel_done(ctx, T_THREEBODY, ev);
     }
     if ((comp & MBPOL_DISPERSION) && ctx->disp) {
         timer_start(ctx, T_DISPERSION);
@@ -635,8 +651,10 @@ int mbpol_context_set_timing(mbpol_context* ctx, int32_t enabled) {
 }
 
 int mbpol_context_get_timing(mbpol_context* ctx, const char* name, double* total_ms, int64_t* count) {
-    static const char* names[T_NCAT] = {"evaluate", "onebody", "lists", "twobody", "threebody",
-                                        "dispersion", "elec_setup", "elec_iteration", "elec_energy"};
+    static const char* names[T_NCAT] = {"evaluate",   "onebody",    "lists",        "twobody",
+                                        "threebody",  "dispersion", "elec_setup",   "elec_iteration",
+                                        "elec_energy", "pme_spread", "fft_z_r2c",   "fft_y",
+                                        "fft_x_conv", "fft_z_c2r",  "elec_iterate"};
     timer_collect(ctx);
     for (int c = 0; c < T_NCAT; ++c)
         if (std::strcmp(name, names[c]) == 0) {
@@ -655,6 +673,7 @@ int mbpol_context_get_stats(const mbpol_context* ctx, mbpol_eval_stats* out) {
     out->num_trimers = ctx->stats.trimers;
     out->num_elec_pairs = ctx->stats.elec_pairs;
     out->num_disp_pairs = ctx->stats.disp_pairs;
+    out->num_pol_pairs = ctx->stats.pol_pairs;
     return MBPOL_OK;
 }
 

@@ -11,6 +11,8 @@
 #include <limits>
 #include <stdexcept>
 #include <string>
+#include <chrono>
+
 #include "oracle_common.h"
 #include "mbpol_params.h"
 #include "oracle.h"
@@ -474,6 +476,7 @@ class Elec {
 
     // ---------------- the full calculation ----------------
     double run(std::vector<Site>& s, double* forces, oracle_elec_stats* stats, double* dipoles_out, double* fixed_field_out) {
+        const auto t_start = std::chrono::steady_clock::now();
         // setup (:857-897): charges from the dipole-moment surface
         if (P_.charge_redistribution) {
             for (int ii = 0; ii + 3 < n_; ii += 4) water_charge(s[ii], s[ii + 1], s[ii + 2], s[ii + 3], ii, ii + 1, ii + 2, ii + 3);
@@ -536,18 +539,24 @@ class Elec {
         std::vector<V3> mud = Ed, mup = Ep;
         int iters = 0;
         double eps = 0;
+        const auto t_fixed = std::chrono::steady_clock::now();
         bool converged = solve_dipoles(s, Ed, Ep, mud, mup, iters, eps);
         if (stats) {
             stats->iterations = iters;
             stats->epsilon = eps;
             stats->converged = converged ? 1 : 0;
+            stats->setup_s = std::chrono::duration<double>(t_fixed - t_start).count() + scale_seconds_;
+            stats->iteration_s = iters > 0 ? loop_seconds_ / iters : 0.0;
         }
         if (dipoles_out)
             for (int i = 0; i < n_; ++i)
                 for (int d = 0; d < 3; ++d) dipoles_out[3 * i + d] = mud[i][d];
         if (!converged) throw std::runtime_error("Induced dipoles did not converge: iterations=" + std::to_string(iters) +
                                                  " eps=" + std::to_string(eps));
-        return pme_ ? energy_pme(s, mud, mup, forces) : energy_direct(s, mud, mup, forces);
+        const auto t_energy = std::chrono::steady_clock::now();
+        const double e = pme_ ? energy_pme(s, mud, mup, forces) : energy_direct(s, mud, mup, forces);
+        if (stats) stats->energy_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_energy).count();
+        return e;
     }
 
     // SOR iteration with the reference's stop rules (:557-616, :488-531).
@@ -555,6 +564,7 @@ class Elec {
                        std::vector<V3>& mup, int& iters, double& eps_out) {
         // precomputeScale35 (:534-555 direct, :2280-2298 PME); pairs beyond the
         // PME cutoff are never used, so only in-cutoff pairs are kept.
+        const auto t_scales = std::chrono::steady_clock::now();
         struct PairS { int i, j; V3 dr; double s3, s5; };
         std::vector<PairS> pairs;
         for (int i = 0; i < n_; ++i)
@@ -570,6 +580,8 @@ class Elec {
                     pairs.push_back({i, j, dr, -1 * scale(s[i], s[j], r, false, 3, TDD), scale(s[i], s[j], r, false, 5, TDD)});
                 }
             }
+        const auto t_loop = std::chrono::steady_clock::now();
+        scale_seconds_ = std::chrono::duration<double>(t_loop - t_scales).count();
         double cur = 1.0e+50;
         bool done = false, conv = false;
         int it = 0;
@@ -647,6 +659,7 @@ class Elec {
         }
         iters = it;
         eps_out = cur;
+        loop_seconds_ = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_loop).count();
         return conv;
     }
 
@@ -842,6 +855,7 @@ class Elec {
     V3 box_;
     bool pme_;
     double expgamm_;
+    double scale_seconds_ = 0.0, loop_seconds_ = 0.0;  // phase timings of the last run
     int ng_[3] = {0, 0, 0};
     size_t ntot_ = 0;
     std::vector<cplx> grid_;

@@ -24,6 +24,10 @@ typedef struct {
     int iterations;
     double epsilon;
     int converged;
+    /* wall-clock seconds of the phases (for the CPU throughput estimate of a
+     * sampled run): charges + fixed field + pair scales, one dipole iteration
+     * on average, energy/forces */
+    double setup_s, iteration_s, energy_s;
 } oracle_elec_stats;
 
 /* polynomial kernels */

@@ -24,7 +24,8 @@ class ElecParams(ctypes.Structure):
 
 
 class ElecStats(ctypes.Structure):
-    _fields_ = [('iterations', ctypes.c_int), ('epsilon', ctypes.c_double), ('converged', ctypes.c_int)]
+    _fields_ = [('iterations', ctypes.c_int), ('epsilon', ctypes.c_double), ('converged', ctypes.c_int),
+                ('setup_s', ctypes.c_double), ('iteration_s', ctypes.c_double), ('energy_s', ctypes.c_double)]
 
 
 def _d(a):
@@ -170,7 +171,8 @@ def electrostatics(params, pos, charge, mol, atype, damp, pol, want_field=False)
                                      _pd(fld), err, 256)
     if rc != 0:
         raise RuntimeError(err.value.decode())
-    out = dict(energy=e.value, forces=f, dipoles=dip, iterations=st.iterations, epsilon=st.epsilon)
+    out = dict(energy=e.value, forces=f, dipoles=dip, iterations=st.iterations, epsilon=st.epsilon,
+               setup_s=st.setup_s, iteration_s=st.iteration_s, energy_s=st.energy_s)
     if want_field:
         out['fixed_field'] = fld
     return out

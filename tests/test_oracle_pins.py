@@ -1,0 +1,215 @@
+"""Pins of the FP64 oracle beyond the reference's known answers: what the
+mathematics fixes (gradients, invariances, closed forms, brute force) and the
+reference's own polynomial code (golden vectors written by
+tests/golden/make_pip_vectors.py from the compiled reference sources).  CPU only.
+Each check is chosen so that a plausible mistake in the oracle -- a dropped
+term, a wrong sign or index, a transposed operand -- fails it."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import kat_data as K
+
+GOLDEN = os.path.join(os.path.dirname(__file__), 'golden')
+
+
+@pytest.fixture(scope='module')
+def mb():
+    import mbpol_openmm_plugin_amd as m
+    return m
+
+
+def test_pip_polynomials_match_reference_vectors(oracle):
+    """poly_2b_v6x_eval / poly_3b_v2x::eval (the reference's generated code) at
+    seeded variable vectors: energy and every gradient component."""
+    d = json.load(open(os.path.join(GOLDEN, 'pip_vectors.json')))
+    for key, fn in (('poly2b', oracle.pip2b), ('poly3b', oracle.pip3b)):
+        for v in d[key]:
+            e, g = fn(np.array(v['x']))
+            assert abs(e - v['e']) <= 1e-11 * max(1.0, abs(v['e'])), key
+            np.testing.assert_allclose(g, v['g'], rtol=1e-10, atol=1e-10 * np.abs(v['g']).max())
+
+
+def test_gammq_matches_reference_vectors(oracle):
+    d = json.load(open(os.path.join(GOLDEN, 'pip_vectors.json')))
+    for x, q in d['gammq34']:
+        assert abs(oracle.gammq(0.75, x) - q) <= 1e-13 * max(q, 1e-300) + 1e-300, x
+
+
+def test_fft3d_matches_numpy(oracle):
+    """Mixed-radix transform of the PME convolution against numpy, both signs
+    (the oracle's forward transform is exp(-2 pi i k.x / N))."""
+    rng = np.random.default_rng(5)
+    g = rng.standard_normal((6, 10, 7)) + 1j * rng.standard_normal((6, 10, 7))
+    np.testing.assert_allclose(oracle.fft3d(g, -1), np.fft.fftn(g), atol=1e-12 * np.abs(g).sum())
+    np.testing.assert_allclose(oracle.fft3d(g, +1), np.fft.ifftn(g) * g.size, atol=1e-12 * np.abs(g).sum())
+
+
+def _cluster(mb, n, seed):
+    waters = mb.systems.cluster(n, seed=seed)
+    return waters, mb.forcefield.water_positions_with_m(waters)
+
+
+def _fd_check(oracle, sysm, pos, components, rel=2e-6, h=2e-6, atoms=(0, 1, 2, 4, 8)):
+    """central differences of the energy against the forces on a few atoms (M sites move with their water)"""
+    E0, F0 = oracle.evaluate_system(sysm, pos, components=components)
+    key = {1: 'onebody', 2: 'twobody', 4: 'threebody', 8: 'dispersion', 16: 'electrostatics'}[components]
+    scale = np.abs(F0).max()
+    for a in atoms:
+        if a % 4 == 3:
+            continue
+        for d in range(3):
+            p, m = pos.copy(), pos.copy()
+            p[a, d] += h
+            m[a, d] -= h
+            ep = oracle.evaluate_system(sysm, p, components=components)[0][key]
+            em = oracle.evaluate_system(sysm, m, components=components)[0][key]
+            fd = -(ep - em) / (2 * h)
+            assert abs(fd - F0[a, d]) <= rel * scale + 1e-6, (key, a, d, fd, F0[a, d])
+
+
+@pytest.mark.parametrize('component', [1, 2, 4, 8])
+def test_forces_are_energy_gradients_pips_and_dispersion(oracle, mb, component):
+    waters, pos = _cluster(mb, 4, seed=11)
+    sysm = mb.forcefield.create_system(4, nonbonded_method='CutoffNonPeriodic', twobody_cutoff=1e10,
+                                       threebody_cutoff=1e10, nonbonded_cutoff=1e3, components=component)
+    _fd_check(oracle, sysm, pos, component)
+
+
+@pytest.mark.parametrize('method,box', [('CutoffNonPeriodic', None), ('PME', 1.6)])
+def test_forces_are_energy_gradients_electrostatics(oracle, mb, method, box):
+    """The induced-dipole energy is stationary in the dipoles, so at tight
+    convergence the forces (charge-redistribution chain rule, Thole damping,
+    PME reciprocal and self terms) are its exact gradient."""
+    waters, pos = _cluster(mb, 4, seed=13)
+    if box:
+        pos = pos + box / 2
+    # a coarse 16^3 grid: the check is the consistency of energy and forces at that grid
+    sysm = mb.forcefield.create_system(4, nonbonded_method=method, nonbonded_cutoff=0.7 if box else 1e3, box=box,
+                                       ewald_alpha=4.0 if box else None, pme_grid=(16, 16, 16) if box else None,
+                                       dipole_epsilon=1e-12, components=16)
+    _fd_check(oracle, sysm, pos, 16, rel=5e-6, atoms=(0, 5, 9) if box else (0, 1, 2, 4, 8))
+
+
+def test_permutation_and_translation_invariance(oracle, mb):
+    """Swapping the two hydrogens of every water, reordering the molecules and
+    translating the cluster leave every component unchanged."""
+    waters, pos = _cluster(mb, 5, seed=17)
+    sysm = mb.forcefield.create_system(5, nonbonded_method='CutoffNonPeriodic', twobody_cutoff=1e10,
+                                       threebody_cutoff=1e10, nonbonded_cutoff=1e3, dipole_epsilon=1e-12)
+    E0, F0 = oracle.evaluate_system(sysm, pos)
+    sw = waters[:, [0, 2, 1]]
+    E1, _ = oracle.evaluate_system(sysm, mb.forcefield.water_positions_with_m(sw))
+    perm = [3, 0, 4, 2, 1]
+    E2, F2 = oracle.evaluate_system(sysm, mb.forcefield.water_positions_with_m(waters[perm]))
+    E3, F3 = oracle.evaluate_system(sysm, pos + np.array([0.31, -1.7, 2.2]))
+    for k in E0:
+        for other in (E1, E2, E3):
+            assert abs(other[k] - E0[k]) <= 1e-10 * max(abs(E0['total']), 1.0), k
+    np.testing.assert_allclose(F3, F0, atol=1e-9 * np.abs(F0).max())
+    # molecule reordering permutes the forces with the molecules
+    F2w = F2.reshape(5, 4, 3)
+    np.testing.assert_allclose(F2w, F0.reshape(5, 4, 3)[perm], atol=1e-9 * np.abs(F0).max())
+    assert np.abs(F0.sum(axis=0)).max() <= 1e-9 * np.abs(F0).max()
+
+
+def test_chain_trimer_list_is_brute_force_chain_set(oracle):
+    """Non-periodic chain list (ReferenceThreeNeighborList.cpp:215-225): every
+    i > j > k with d(i,j) <= rc and d(j,k) <= rc, d(i,k) not tested."""
+    rng = np.random.default_rng(3)
+    o = rng.uniform(0, 1.6, (40, 3))
+    rc = 0.45
+    got = sorted(map(tuple, oracle.trimer_list(o, rc).tolist()))
+    d = np.linalg.norm(o[:, None] - o[None], axis=-1)
+    want = sorted((i, j, k) for i in range(40) for j in range(i) for k in range(j) if d[i, j] <= rc and d[j, k] <= rc)
+    assert got == want
+
+
+def test_dimer_list_is_brute_force_pair_set(oracle):
+    rng = np.random.default_rng(4)
+    o = rng.uniform(0, 2.0, (60, 3))
+    L = 2.0
+    got = sorted(map(tuple, oracle.dimer_list(o, 0.65, True, [L] * 3).tolist()))
+    dv = o[:, None] - o[None]
+    dv -= L * np.round(dv / L)
+    d = np.linalg.norm(dv, axis=-1)
+    want = sorted((i, j) for i in range(60) for j in range(i + 1, 60) if d[i, j] <= 0.65)
+    assert got == want
+
+
+def test_sor_fixed_point_solves_the_dipole_equations(oracle):
+    """The converged SOR dipoles satisfy mu = alpha (E + T mu), with T built
+    pair by pair from the oracle's Thole-scaled dipole tensors, i.e. they equal
+    the direct solution of (1 - alpha T) mu = alpha E (three waters with M sites)."""
+    pos = K.WATERS_NM.reshape(-1, 3)
+    n = 9
+    mol = np.repeat(np.arange(3), 3)
+    atype = np.tile([0, 1, 1], 3)
+    damp = np.tile([0.00131, 0.000294, 0.000294], 3)
+    pol = damp.copy()
+    rng = np.random.default_rng(8)
+    field = rng.standard_normal((n, 3)) * 50.0
+    P = oracle.elec_params(method=0, target_eps=1e-14, max_iter=500)
+    dip, it, eps, ok = oracle.solve_given_field(P, pos, mol, atype, damp, pol, field)
+    assert ok
+    T = np.zeros((3 * n, 3 * n))
+    for i in range(n):
+        for j in range(n):
+            if i == j:
+                continue
+            dr = pos[j] - pos[i]
+            r = np.linalg.norm(dr)
+            s3 = oracle.thole_scale(P, damp[i], damp[j], r, 3, 2, mol=(mol[i], mol[j]), types=(atype[i], atype[j]))
+            s5 = oracle.thole_scale(P, damp[i], damp[j], r, 5, 2, mol=(mol[i], mol[j]), types=(atype[i], atype[j]))
+            T[3 * i:3 * i + 3, 3 * j:3 * j + 3] = -s3 * np.eye(3) + s5 * np.outer(dr, dr)
+    A = np.eye(3 * n) - np.repeat(pol, 3)[:, None] * T
+    mu = np.linalg.solve(A, (pol[:, None] * field).ravel()).reshape(n, 3)
+    np.testing.assert_allclose(dip, mu, atol=1e-9 * np.abs(mu).max())
+
+
+def test_pme_matches_direct_ewald_sum(oracle):
+    """Point charges (no damping, no polarisability, one site per molecule) in a
+    periodic box: PME real + reciprocal + self energy equals an explicit Ewald
+    sum over images and k-vectors at the same alpha, and both give forces that
+    agree."""
+    rng = np.random.default_rng(21)
+    n, L, alpha, rc = 10, 2.0, 5.0, 0.95
+    pos = rng.uniform(0, L, (n, 3))
+    q = rng.standard_normal(n)
+    q -= q.mean()
+    P = oracle.elec_params(method=1, cutoff=rc, alpha=alpha, grid=(72, 72, 72), charge_redistribution=False,
+                           box=(L, L, L))
+    r = oracle.electrostatics(P, pos, q, np.arange(n), np.zeros(n, int), np.zeros(n), np.zeros(n))
+    # explicit Ewald sum (kJ/mol with the reference's Coulomb constant)
+    from scipy.special import erfc
+    f = 138.9354558456
+    e_real = 0.0
+    F = np.zeros((n, 3))
+    for i in range(n):
+        for j in range(i + 1, n):
+            dr = pos[j] - pos[i]
+            dr -= L * np.round(dr / L)
+            d = np.linalg.norm(dr)
+            if d <= rc:
+                e_real += q[i] * q[j] * erfc(alpha * d) / d
+                g = q[i] * q[j] * (erfc(alpha * d) / d + 2 * alpha / np.sqrt(np.pi) * np.exp(-(alpha * d) ** 2)) / d ** 2
+                F[i] -= g * dr
+                F[j] += g * dr
+    m = np.arange(-14, 15)
+    kx, ky, kz = np.meshgrid(m, m, m, indexing='ij')
+    kv = np.stack([kx.ravel(), ky.ravel(), kz.ravel()], 1) / L
+    kv = kv[np.any(kv != 0, axis=1)]
+    k2 = (kv ** 2).sum(1)
+    ph = 2 * np.pi * pos @ kv.T
+    S = (q[:, None] * np.exp(1j * ph)).sum(0)
+    w = np.exp(-np.pi ** 2 * k2 / alpha ** 2) / k2 / (2 * np.pi * L ** 3)
+    e_rec = (w * np.abs(S) ** 2).sum()
+    # F_i = -dE_rec/dr_i = sum_k 4 pi w_k q_i Im(conj(S_k) exp(i phi_ik)) k
+    coef = 4 * np.pi * w[None, :] * q[:, None] * np.imag(np.conj(S)[None, :] * np.exp(1j * ph))
+    F += coef @ kv
+    e_self = -alpha / np.sqrt(np.pi) * (q ** 2).sum()
+    E = f * (e_real + e_rec + e_self)
+    assert abs(r['energy'] - E) <= 2e-6 * abs(E), (r['energy'], E)
+    np.testing.assert_allclose(r['forces'], f * F, atol=2e-5 * np.abs(f * F).max())
