@@ -232,6 +232,20 @@ def main():
         'breakdown_ms_per_step': {k: timing[k]['ms_per_step'] for k in PHASES},
         'roofline': roofline,
     }
+    # the same evaluation through the host-buffer entry point mbpol_context_evaluate
+    # (positions copied in, virtual sites placed on the device, forces copied out)
+    import ctypes
+    lib = m.library()
+    dp = ctypes.POINTER(ctypes.c_double)
+    hpos = np.ascontiguousarray(pos, dtype=np.float64)
+    hfrc, hen = np.zeros_like(hpos), np.zeros(6)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        m.api._check(lib.mbpol_context_evaluate(ctx._h, hpos.ctypes.data_as(dp), hfrc.ctypes.data_as(dp),
+                                                hen.ctypes.data_as(dp), ctypes.c_uint32(31)))
+    ms_host = (time.perf_counter() - t0) * 1000.0 / args.steps
+    line['host_buffer_path'] = {'value': 1000.0 / ms_host, 'ms_per_step': ms_host,
+                                'note': 'positions copied in and forces copied out on every evaluation'}
     if not args.no_fp64_path and args.precision != 'double':
         f_mixed = ctx.downloadForces()
         ctx64 = m.Context(sysm, precision='double')

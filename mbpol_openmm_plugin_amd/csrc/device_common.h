@@ -25,16 +25,20 @@ __device__ __forceinline__ void add_force(unsigned long long* f, int atom, doubl
     atomicAdd(&f[3 * atom + 2], to_fix(fz));
 }
 
-// Wave-level sum (64 lanes) followed by one atomic per wave.
+// Wave-level sum (64 lanes).
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
     return v;
 }
 
+// Energy of each lane to fixed point first, then an integer wave sum and one
+// atomic per wave: the total does not depend on which units share a wave.
 __device__ __forceinline__ void add_energy(unsigned long long* e, double v) {
-    v = wave_sum(v);
-    if ((threadIdx.x & 63) == 0) atomicAdd(e, to_fix(v));
+    unsigned long long f = to_fix(v);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) f += __shfl_xor(f, off, 64);
+    if ((threadIdx.x & 63) == 0) atomicAdd(e, f);
 }
 
 __device__ __forceinline__ d3 load_d3(const double* p, int i) { return d3{p[3 * i], p[3 * i + 1], p[3 * i + 2]}; }

@@ -6,6 +6,7 @@
 #include "../../include/mbpol_hip.h"
 #include "disp_kernels.h"
 #include "elec.hpp"
+#include "elec_kernels.h"
 
 namespace mbpol {
 
@@ -17,7 +18,7 @@ namespace mbpol {
 
 struct Disp {
     DispGeom g{};
-    DevBuf act, type, mol, c6, d6, cell_of, cell_count, cell_start, cell_fill, members, eblk, npairs;
+    DevBuf act, type, mol, c6, d6, cell_of, cell_count, cell_start, cell_fill, members, unsorted, eblk, npairs;
     int ncell = 1;
 };
 
@@ -73,6 +74,7 @@ int disp_create(mbpol_context* ctx) {
     if (int e = upv(D->d6, s.ds_d6)) return e;
     HIP_TRY(D->cell_of.ensure(sizeof(int) * std::max(1, D->g.na)));
     HIP_TRY(D->members.ensure(sizeof(int) * std::max(1, D->g.na)));
+    HIP_TRY(D->unsorted.ensure(sizeof(int) * std::max(1, D->g.na)));
     HIP_TRY(D->eblk.ensure(sizeof(double) * std::max(1, D->g.na)));
     HIP_TRY(D->npairs.ensure(sizeof(unsigned long long)));
     return disp_set_box(ctx);
@@ -96,7 +98,10 @@ int disp_evaluate(mbpol_context* ctx, u64* energy) {
         launch_disp_cells(st, D->g.na, D->act.as<int>(), pos, D->g.L, D->g.nc, D->cell_of.as<int>(), D->cell_count.as<int>());
         if (int e = scan_i32(ctx, D->cell_count.as<int>(), D->cell_start.as<int>(), D->ncell + 1)) return e;
         launch_disp_cell_fill(st, D->g.na, D->cell_of.as<int>(), D->cell_start.as<int>(), D->cell_fill.as<int>(),
-                              D->members.as<int>());
+                              D->unsorted.as<int>());
+        // members in index order: the pair sums, and so the forces, do not depend on scheduling
+        launch_rank_sort(st, D->g.na, D->cell_of.as<int>(), D->cell_start.as<int>(), D->unsorted.as<int>(),
+                         D->members.as<int>());
     }
     HIP_TRY(hipMemsetAsync(D->npairs.p, 0, sizeof(unsigned long long), st));
     launch_disp(st, ctx->precision == MBPOL_PRECISION_DOUBLE, D->g, D->act.as<int>(), pos, D->type.as<int>(),

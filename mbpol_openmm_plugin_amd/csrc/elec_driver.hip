@@ -38,7 +38,7 @@ struct Elec {
     DevBuf q0, q, damp, pol, mol, type, dq, polidx;
     std::vector<unsigned char> h_pol;  // host copy of (polarisability != 0)
     // cells for the neighbour rows
-    DevBuf cell_of, cell_count, cell_start, cell_fill, members;
+    DevBuf cell_of, cell_count, cell_start, cell_fill, members, unsorted;
     int3 nc{1, 1, 1};
     int reach = 1, use_cells = 0;
     // neighbour rows (ELL) and induced-field prefactors
@@ -283,6 +283,7 @@ int elec_create(mbpol_context* ctx) {
     HIP_TRY(E->eblk.ensure(sizeof(double) * (std::max(energy_blocks(n), sor_blocks(n)) + 1)));
     HIP_TRY(E->cell_of.ensure(sizeof(int) * std::max(n, 1)));
     HIP_TRY(E->members.ensure(sizeof(int) * std::max(n, 1)));
+    HIP_TRY(E->unsorted.ensure(sizeof(int) * std::max(n, 1)));
     HIP_TRY(E->cntP.ensure(sizeof(int) * std::max(n, 1)));
     HIP_TRY(E->cntQ.ensure(sizeof(int) * std::max(n, 1)));
     HIP_TRY(E->cell_count.ensure(sizeof(int) * 2));
@@ -373,8 +374,9 @@ static int build_rows(mbpol_context* ctx) {
         launch_site_cells(st, n, pos, E->P, E->nc, E->cell_of.as<int>(), E->cell_count.as<int>());
         if (int e = scan_i32(ctx, E->cell_count.as<int>(), E->cell_start.as<int>(), ncell + 1)) return e;
         launch_bucket_fill(st, n, E->cell_of.as<int>(), E->cell_start.as<int>(), E->cell_fill.as<int>(),
-                           E->members.as<int>());
-        launch_sort_segments(st, ncell, E->cell_start.as<int>(), E->members.as<int>());
+                           E->unsorted.as<int>());
+        launch_rank_sort(st, n, E->cell_of.as<int>(), E->cell_start.as<int>(), E->unsorted.as<int>(),
+                         E->members.as<int>());
     }
     int* h = reinterpret_cast<int*>(E->h_eps) + 8;
     for (int attempt = 0; attempt < 2; ++attempt) {
@@ -418,8 +420,8 @@ static int build_rows(mbpol_context* ctx) {
         HIP_TRY(hipMemsetAsync(b.fill.p, 0, sizeof(int) * (nbins + 1), st));
         launch_site_bins(st, n, pos, E->q.as<double>(), E->P, E->B, b.of.as<int>(), b.count.as<int>());
         if (int e = scan_i32(ctx, b.count.as<int>(), b.start.as<int>(), nbins + 1)) return e;
-        launch_bucket_fill(st, n, b.of.as<int>(), b.start.as<int>(), b.fill.as<int>(), b.members.as<int>());
-        launch_sort_segments(st, nbins, b.start.as<int>(), b.members.as<int>());
+        launch_bucket_fill(st, n, b.of.as<int>(), b.start.as<int>(), b.fill.as<int>(), E->unsorted.as<int>());
+        launch_rank_sort(st, n, b.of.as<int>(), b.start.as<int>(), E->unsorted.as<int>(), b.members.as<int>());
     }
     return MBPOL_OK;
 }

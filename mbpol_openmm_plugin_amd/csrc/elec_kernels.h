@@ -43,7 +43,9 @@ void launch_site_charges_copy(hipStream_t s, int n, const double* q0, double* q)
 void launch_site_cells(hipStream_t s, int n, const double* pos, ElecParams P, int3 nc, int* cell_of, int* cell_count);
 void launch_bucket_fill(hipStream_t s, int n, const int* bucket_of, const int* bucket_start, int* bucket_fill,
                         int* members);
-void launch_sort_segments(hipStream_t s, int nseg, const int* start, int* data);
+// members_out: every bucket's items in ascending order (members_in: same buckets, any order)
+void launch_rank_sort(hipStream_t s, int n, const int* bucket_of, const int* bucket_start, const int* members_in,
+                      int* members_out);
 void launch_site_lists(hipStream_t s, int n, const double* pos, const double* pol, ElecParams P, int3 nc, int reach,
                        int use_cells, const int* cell_of, const int* cell_start, const int* members, int capP, int capQ,
                        int* nbP, int* cntP, int* nbQ, int* cntQ, int* maxcnt);

@@ -174,21 +174,20 @@ __global__ void k_bucket_fill(int n, const int* __restrict__ bucket_of, const in
     members[bucket_start[c] + atomicAdd(&bucket_fill[c], 1)] = i;
 }
 
-// ascending insertion sort of each (short) segment, one lane per segment: the
-// order of every neighbour row and spreading sum then depends on the input only
-__global__ void k_sort_segments(int nseg, const int* __restrict__ start, int* __restrict__ data) {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= nseg) return;
-    const int b = start[s], e = start[s + 1];
-    for (int k = b + 1; k < e; ++k) {
-        const int v = data[k];
-        int m = k - 1;
-        while (m >= b && data[m] > v) {
-            data[m + 1] = data[m];
-            --m;
-        }
-        data[m + 1] = v;
-    }
+// Members of every bucket in ascending item order, one lane per item: the
+// item's slot is the number of smaller items in its bucket (members_in holds
+// each bucket's items in arbitrary order).  The order of every neighbour row
+// and spreading sum then depends on the input only.
+__global__ void k_rank_sort(int n, const int* __restrict__ bucket_of, const int* __restrict__ bucket_start,
+                            const int* __restrict__ members_in, int* __restrict__ members_out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int c = bucket_of[i];
+    if (c < 0) return;
+    const int b = bucket_start[c], e = bucket_start[c + 1];
+    int rank = 0;
+    for (int k = b; k < e; ++k) rank += members_in[k] < i;
+    members_out[b + rank] = i;
 }
 
 // Neighbour rows of every site, in fixed-capacity (ELL) layout: row i of the
@@ -1151,8 +1150,9 @@ void launch_bucket_fill(hipStream_t s, int n, const int* bucket_of, const int* b
                         int* members) {
     hipLaunchKernelGGL(k_bucket_fill, nblk(n, 256), 256, 0, s, n, bucket_of, bucket_start, bucket_fill, members);
 }
-void launch_sort_segments(hipStream_t s, int nseg, const int* start, int* data) {
-    hipLaunchKernelGGL(k_sort_segments, nblk(nseg, 64), 64, 0, s, nseg, start, data);
+void launch_rank_sort(hipStream_t s, int n, const int* bucket_of, const int* bucket_start, const int* members_in,
+                      int* members_out) {
+    hipLaunchKernelGGL(k_rank_sort, nblk(n, 256), 256, 0, s, n, bucket_of, bucket_start, members_in, members_out);
 }
 void launch_site_lists(hipStream_t s, int n, const double* pos, const double* pol, ElecParams P, int3 nc, int reach,
                        int use_cells, const int* cell_of, const int* cell_start, const int* members, int capP, int capQ,

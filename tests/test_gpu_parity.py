@@ -161,9 +161,10 @@ def test_config2_properties(mb):
         ctx.setPositions(pos)
         a = ctx.getState()
         b = ctx.getState()
-        # repeatable to round-off (PME grid spreading sums in FP64 atomics, in no fixed order)
-        assert abs(a.getPotentialEnergy() - b.getPotentialEnergy()) <= 1e-11 * abs(a.getPotentialEnergy())
-        assert rel_rms(a.getForces(), b.getForces()) <= 1e-10
+        # bitwise repeatable: every sum runs in an order fixed by the input (sorted rows,
+        # bins and spreading table, fixed-point force accumulation)
+        assert a.getPotentialEnergy() == b.getPotentialEnergy()
+        assert np.array_equal(a.getForces(), b.getForces())
         ctx.setPositions(pos + np.array([0.37, -0.21, 1.3]))
         c = ctx.getState()
         # a rigid translation moves the atoms relative to the PME grid: the energy changes by
