@@ -111,7 +111,8 @@ def cpu_baseline(gpu_iterations):
     1-body, the O-O lists and dispersion in full; 2-body and 3-body on the first
     2000 entries of their lists, scaled to the full lists; electrostatics run
     once with a single dipole iteration, its setup + energy time plus
-    `gpu_iterations` times the measured iteration time.  About 25 s."""
+    `gpu_iterations` (the SOR iteration count of this configuration, as the
+    reference iterates) times the measured iteration time.  About 25 s."""
     code = r'''
 import sys, time, json, numpy as np
 sys.path.insert(0, %(root)r); sys.path.insert(0, %(oracle)r)
@@ -226,6 +227,7 @@ def main():
                                % (alpha, grid[0], grid[1], grid[2], args.precision),
                    'waters': 4096, 'box_nm': L, 'precision': args.precision},
         'iterations': stats['iterations'],
+        'dipole_solver': 'sor' if args.precision == 'double' else 'pcg',
         'counts': {k: stats[k] for k in ('num_dimers', 'num_trimers', 'num_elec_pairs', 'num_pol_pairs')},
         'energies_kj_mol': dict(zip(['onebody', 'twobody', 'threebody', 'dispersion', 'electrostatics', 'total'],
                                     energies.tolist())),
@@ -246,6 +248,7 @@ def main():
     ms_host = (time.perf_counter() - t0) * 1000.0 / args.steps
     line['host_buffer_path'] = {'value': 1000.0 / ms_host, 'ms_per_step': ms_host,
                                 'note': 'positions copied in and forces copied out on every evaluation'}
+    sor_iterations = stats['iterations']
     if not args.no_fp64_path and args.precision != 'double':
         f_mixed = ctx.downloadForces()
         ctx64 = m.Context(sysm, precision='double')
@@ -254,16 +257,22 @@ def main():
             ctx64.evaluateResident()
         ms64, e64 = run_steps(ctx64, args.steps)
         f64 = ctx64.downloadForces()
+        sor_iterations = ctx64.getStats()['iterations']
         tot = abs(e64[5])
         line['fp64_path'] = {
-            'value': 1000.0 / ms64, 'ms_per_step': ms64,
+            'value': 1000.0 / ms64, 'ms_per_step': ms64, 'dipole_solver': 'sor', 'iterations': sor_iterations,
+            'speedup_of_default_path': ms64 / ms_per_step,
             'mixed_vs_fp64_rel_error_of_total_by_component': dict(
                 zip(['onebody', 'twobody', 'threebody', 'dispersion', 'electrostatics', 'total'],
                     (np.abs(energies - e64) / tot).tolist())),
             'mixed_vs_fp64_rel_rms_force': float(np.sqrt(((f_mixed - f64) ** 2).mean() / (f64 ** 2).mean()))}
         del ctx64
     if not args.no_cpu_baseline:
-        line['cpu_baseline'] = cpu_baseline(stats['iterations'])
+        if args.precision != 'double' and args.no_fp64_path:  # iterations of the reference's SOR solve
+            ctx.setDipoleSolver('sor')
+            ctx.evaluateResident()
+            sor_iterations = ctx.getStats()['iterations']
+        line['cpu_baseline'] = cpu_baseline(sor_iterations)
     print(json.dumps(line), flush=True)
 
 

@@ -75,11 +75,18 @@ extern "C" {
 #define MBPOL_GUESS_PREVIOUS 2          /* converged dipoles of the previous evaluate() */
 #define MBPOL_GUESS_USER 3              /* dipoles passed with mbpol_context_set_dipole_guess() */
 
+/* induced-dipole solver (mbpol_context_set_solver); both stop when the reference's convergence measure
+ * (:504-511, :588) of the current dipoles falls below the target epsilon */
+#define MBPOL_SOLVER_SOR 0              /* the reference's SOR iteration, omega 0.55 (:557-616); default of DOUBLE */
+#define MBPOL_SOLVER_PCG 1              /* conjugate gradients with a diagonal preconditioner on the symmetric
+                                           form of the same equations; default of MIXED and SINGLE_PIP */
+
 typedef struct mbpol_system mbpol_system;
 typedef struct mbpol_context mbpol_context;
 
 typedef struct {
-    int32_t iterations;        /* induced-dipole iterations of the last evaluate() */
+    int32_t iterations;        /* induced-field evaluations of the last evaluate()'s dipole solve (SOR: its
+                                  iterations; PCG: its steps plus the initial residual) */
     double epsilon;            /* final convergence measure (reference definition, :504-511, :588) */
     int32_t converged;
     int64_t num_dimers;        /* entries of the 2-body list */
@@ -153,6 +160,9 @@ int mbpol_context_get_dipoles(const mbpol_context* ctx, double* dipoles);
 /* charges used by the last evaluate (after redistribution) */
 int mbpol_context_get_charges(const mbpol_context* ctx, double* charges);
 int mbpol_context_set_dipole_guess(mbpol_context* ctx, int32_t mode, const double* dipoles);
+/* MBPOL_SOLVER_SOR or MBPOL_SOLVER_PCG for the following evaluations (no counterpart in the reference, which
+ * always iterates by SOR; the default depends on the precision mode given at creation) */
+int mbpol_context_set_solver(mbpol_context* ctx, int32_t solver);
 /* copyParametersToContext for the electrostatics (same count required) */
 int mbpol_context_update_electrostatics(mbpol_context* ctx, int32_t n, const double* charge, const int32_t* molecule,
                                         const int32_t* atom_type, const double* damping, const double* polarity);

@@ -440,6 +440,7 @@ def _build_native_system(system):
 
 
 PRECISIONS = {'mixed': 0, 'double': 1, 'single-pip': 2}
+SOLVERS = {'sor': 0, 'pcg': 1}
 
 
 class Context:
@@ -448,7 +449,11 @@ class Context:
     precision: 'mixed' (FP64 polynomials and electrostatics, FP32 dispersion
     pair arithmetic, FP64 accumulation), 'double' (FP64 throughout) or
     'single-pip' (mixed with FP32 2-/3-body polynomials; their energies then
-    carry relative errors of order 1e-3)."""
+    carry relative errors of order 1e-3).
+
+    The induced dipoles are solved by the reference's SOR iteration in
+    'double' mode and by preconditioned conjugate gradients otherwise; both
+    stop on the reference's convergence measure (setDipoleSolver changes it)."""
 
     def __init__(self, system, platform='HIP', precision='mixed', device=0):
         if platform != 'HIP':
@@ -494,6 +499,12 @@ class Context:
         modes = {'alpha_e': 0, 'zero': 1, 'previous': 2, 'user': 3}
         arr = _d(dipoles) if dipoles is not None else None
         _check(library().mbpol_context_set_dipole_guess(self._h, modes[mode], _pd(arr) if arr is not None else None))
+
+    def setDipoleSolver(self, solver):
+        """'sor' (the reference's iteration) or 'pcg' (conjugate gradients, same stopping rule)."""
+        if solver not in SOLVERS:
+            raise MBPolError(1, 'unknown dipole solver %r (one of %s)' % (solver, ', '.join(SOLVERS)))
+        _check(library().mbpol_context_set_solver(self._h, SOLVERS[solver]))
 
     def setListMode(self, full_chain_list):
         _check(library().mbpol_context_set_list_mode(self._h, int(bool(full_chain_list))))

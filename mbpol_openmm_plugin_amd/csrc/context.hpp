@@ -95,6 +95,9 @@ struct mbpol_context {
     int precision = 0;
     int device = 0;
     hipStream_t stream = nullptr;
+    // second stream for the short-range terms, forked from and joined to `stream` by events
+    hipStream_t side = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     int full_chain_list = 0;
     mbpol::EvalStats stats;
     mbpol::Timer timer;
@@ -104,7 +107,7 @@ struct mbpol_context {
     // pinned host staging
     void* h_counts = nullptr;
     // scan scratch
-    mbpol::DevBuf scan_tmp;
+    mbpol::DevBuf scan_tmp, scan_tmp_side;
     // 1B / 2B / 3B
     mbpol::DevBuf ob_idx, tb_idx, th_idx;
     mbpol::CellGrid grid2, grid3;

@@ -87,16 +87,15 @@ void disp_destroy(mbpol_context* ctx) {
 
 void launch_energy_finish(hipStream_t s, int nblocks, const double* eblk, u64* energy);
 
-int disp_evaluate(mbpol_context* ctx, u64* energy) {
+int disp_evaluate(mbpol_context* ctx, hipStream_t st, u64* energy) {
     Disp* D = ctx->disp;
-    hipStream_t st = ctx->stream;
     const double* pos = ctx->pos.as<double>();
     if (D->g.na == 0) return MBPOL_OK;
     if (D->g.use_cells) {
         HIP_TRY(hipMemsetAsync(D->cell_count.p, 0, sizeof(int) * (D->ncell + 1), st));
         HIP_TRY(hipMemsetAsync(D->cell_fill.p, 0, sizeof(int) * (D->ncell + 1), st));
         launch_disp_cells(st, D->g.na, D->act.as<int>(), pos, D->g.L, D->g.nc, D->cell_of.as<int>(), D->cell_count.as<int>());
-        if (int e = scan_i32(ctx, D->cell_count.as<int>(), D->cell_start.as<int>(), D->ncell + 1)) return e;
+        if (int e = scan_i32(ctx, D->cell_count.as<int>(), D->cell_start.as<int>(), D->ncell + 1, st)) return e;
         launch_disp_cell_fill(st, D->g.na, D->cell_of.as<int>(), D->cell_start.as<int>(), D->cell_fill.as<int>(),
                               D->unsorted.as<int>());
         // members in index order: the pair sums, and so the forces, do not depend on scheduling

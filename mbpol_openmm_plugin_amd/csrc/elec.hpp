@@ -8,8 +8,9 @@ namespace mbpol {
 constexpr double kFixInvHost = 1.0 / 4294967296.0;
 
 int set_error(int code, const char* fmt, ...);
-int scan_i32(mbpol_context* ctx, const int* in, int* out, int n);
-int scan_i64(mbpol_context* ctx, const long long* in, long long* out, int n);
+// exclusive scans on stream s (nullptr: the context's main stream)
+int scan_i32(mbpol_context* ctx, const int* in, int* out, int n, hipStream_t s = nullptr);
+int scan_i64(mbpol_context* ctx, const long long* in, long long* out, int n, hipStream_t s = nullptr);
 void timer_start(mbpol_context* ctx, int cat);
 void timer_stop(mbpol_context* ctx, int cat);
 // events for one timed kernel launch (null pair when timing is off) and their bookkeeping after the launch
@@ -26,6 +27,6 @@ int elec_evaluate(mbpol_context* ctx, u64* energy);
 int disp_create(mbpol_context* ctx);
 void disp_destroy(mbpol_context* ctx);
 int disp_set_box(mbpol_context* ctx);
-int disp_evaluate(mbpol_context* ctx, u64* energy);
+int disp_evaluate(mbpol_context* ctx, hipStream_t st, u64* energy);
 
 }  // namespace mbpol

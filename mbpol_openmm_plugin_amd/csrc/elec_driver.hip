@@ -33,9 +33,10 @@ struct Elec {
     ElecParams P{};
     int n = 0, nw = 0, npol = 0;
     int guess_mode = MBPOL_GUESS_ALPHA_E;
+    int solver = MBPOL_SOLVER_SOR;
     bool have_previous = false;
     // parameters; polidx lists the sites of non-zero polarisability
-    DevBuf q0, q, damp, pol, mol, type, dq, polidx;
+    DevBuf q0, q, damp6, pol, mol, type, dq, polidx;  // damp6: damping^(1/6)
     std::vector<unsigned char> h_pol;  // host copy of (polarisability != 0)
     // cells for the neighbour rows
     DevBuf cell_of, cell_count, cell_start, cell_fill, members, unsorted;
@@ -46,6 +47,8 @@ struct Elec {
     int capP = 0, capQ = 0;
     // fields and dipoles: the iteration alternates between mu_a and mu_b
     DevBuf efix, efix_pol, mu_a, mu_b, mu_prev, mu_user, partial, eps;
+    // conjugate-gradient vectors (3 n), block partials and the r.z history
+    DevBuf cg_r, cg_z, cg_p, cg_q, cg_partial, cg_rz;
     double* mu_cur = nullptr;  // dipoles of the last evaluate
     DevBuf phi_fix, phi_ind, fsite, V, eblk;
     // PME
@@ -123,7 +126,14 @@ static std::vector<double> bspline_moduli(int n) {
 }
 
 static SiteParams site_params(Elec* E) {
-    return SiteParams{E->q.as<double>(), E->damp.as<double>(), E->pol.as<double>(), E->mol.as<int>(), E->type.as<int>()};
+    return SiteParams{E->q.as<double>(), E->damp6.as<double>(), E->pol.as<double>(), E->mol.as<int>(), E->type.as<int>()};
+}
+
+// per-site damping^(1/6): the pair's Thole width (damp_i damp_j)^(1/6) is their product
+static std::vector<double> sixth_roots(const std::vector<double>& d) {
+    std::vector<double> out(d.size());
+    for (size_t k = 0; k < d.size(); ++k) out[k] = std::pow(d[k], 1.0 / 6.0);
+    return out;
 }
 
 template <typename T>
@@ -239,11 +249,14 @@ int elec_create(mbpol_context* ctx) {
     ctx->elec = E;
     E->n = s.ne;
     E->nw = s.el_redistribute ? s.ne / 4 : 0;
+    // the FP64 mode runs the reference's SOR iteration, the faster modes conjugate gradients
+    E->solver = ctx->precision == MBPOL_PRECISION_DOUBLE ? MBPOL_SOLVER_SOR : MBPOL_SOLVER_PCG;
     ElecParams& P = E->P;
     P.n = s.ne;
     P.pme = s.el_method == MBPOL_ELEC_PME;
     P.redistribute = s.el_redistribute;
     for (int k = 0; k < 5; ++k) P.thole[k] = s.el_thole[k];
+    P.tcc_q = std::pow(P.thole[0], 0.25);
     P.cutoff = P.pme ? s.el_cutoff : 1e30;
     P.alpha = s.el_alpha;
     int grid[3] = {s.el_grid[0], s.el_grid[1], s.el_grid[2]};
@@ -260,7 +273,7 @@ int elec_create(mbpol_context* ctx) {
     P.grid = make_int3(grid[0], grid[1], grid[2]);
     const int n = E->n;
     if (int e = up(E->q0, s.el_q)) return e;
-    if (int e = up(E->damp, s.el_damp)) return e;
+    if (int e = up(E->damp6, sixth_roots(s.el_damp))) return e;
     if (int e = up(E->pol, s.el_pol)) return e;
     if (int e = up(E->mol, s.el_mol)) return e;
     if (int e = up(E->type, s.el_type)) return e;
@@ -466,6 +479,78 @@ static int pme_grid(mbpol_context* ctx, int mode, const double* mu) {
     return MBPOL_OK;
 }
 
+// Preconditioned conjugate gradients on the symmetric form of the dipole
+// equations (kernels_elec.hip, k_cg_*), started from the guess in x.  Each
+// step costs one induced-field evaluation, as an SOR step does; the stopping
+// measure is the reference's epsilon of the current dipoles, tested after every
+// step against the same target.  `it` counts field evaluations (the initial
+// residual included).
+static int solve_pcg(mbpol_context* ctx, double* x, double self_term, int& it, double& eps, bool& conv) {
+    Elec* E = ctx->elec;
+    const mbpol_system& s = ctx->sys;
+    hipStream_t st = ctx->stream;
+    const ElecParams& P = E->P;
+    const ElecRows R = rows(E);
+    const double* pos = ctx->pos.as<double>();
+    const int n = E->n, npol = E->npol;
+    const size_t vbytes = sizeof(double) * 3 * std::max(n, 1);
+    for (DevBuf* b : {&E->cg_r, &E->cg_z, &E->cg_p, &E->cg_q}) HIP_TRY(b->ensure(vbytes));
+    HIP_TRY(E->cg_partial.ensure(sizeof(double) * 2 * (cg_blocks(npol) + 1)));
+    HIP_TRY(E->cg_rz.ensure(sizeof(double) * (s.el_max_iter + 2)));
+    HIP_TRY(E->partial.ensure(sizeof(double) * 2 * (iterate_blocks(npol) + 1)));
+    double *r = E->cg_r.as<double>(), *z = E->cg_z.as<double>(), *p = E->cg_p.as<double>(), *q = E->cg_q.as<double>();
+    double* rz = E->cg_rz.as<double>();
+    const double* pol = E->pol.as<double>();
+    const int* polidx = E->polidx.as<int>();
+    auto field = [&](const double* v, int mode, double* out, double* zout) -> int {
+        if (P.pme)
+            if (int e = pme_grid(ctx, 1, v)) return e;
+        const KEvents ev = timer_kernel(ctx, T_ELEC_ITERATE);
+        launch_cg_matvec(st, npol, polidx, pos, P, R, v, E->grid.as<double>(), self_term, pol, E->efix.as<double>(),
+                         mode, out, zout, E->partial.as<double>(), ev);
+        timer_kernel_done(ctx, T_ELEC_ITERATE, ev);
+        return MBPOL_OK;
+    };
+    auto read_eps = [&]() -> int {
+        HIP_TRY(hipMemcpyAsync(E->h_eps, E->eps.p, sizeof(double), hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+        eps = E->h_eps[0];
+        return MBPOL_OK;
+    };
+    conv = false;
+    it = 0;
+    if (npol == 0) {  // nothing to polarise
+        eps = 0.0;
+        conv = true;
+        return MBPOL_OK;
+    }
+    timer_start(ctx, T_ELEC_ITER);
+    if (int e = field(x, 0, r, z)) return e;
+    // r.z and the measure of the guess go through the same reduction as later steps
+    launch_cg_direction(st, npol, polidx, iterate_blocks(npol), E->partial.as<double>(), rz, 0, z, p, n,
+                        E->eps.as<double>());
+    timer_stop(ctx, T_ELEC_ITER);
+    ++it;
+    if (int e = read_eps()) return e;
+    for (int k = 0;; ++k) {
+        if (eps < s.el_eps) {
+            conv = true;
+            break;
+        }
+        if (it > s.el_max_iter) break;
+        timer_start(ctx, T_ELEC_ITER);
+        if (int e = field(p, 1, q, nullptr)) return e;
+        launch_cg_update(st, npol, polidx, pol, self_term, E->partial.as<double>(), rz, k, x, r, p, q, z,
+                         E->cg_partial.as<double>());
+        launch_cg_direction(st, npol, polidx, cg_blocks(npol), E->cg_partial.as<double>(), rz, k + 1, z, p, n,
+                            E->eps.as<double>());
+        timer_stop(ctx, T_ELEC_ITER);
+        ++it;
+        if (int e = read_eps()) return e;
+    }
+    return MBPOL_OK;
+}
+
 int elec_evaluate(mbpol_context* ctx, u64* energy) {
     Elec* E = ctx->elec;
     const mbpol_system& s = ctx->sys;
@@ -512,36 +597,43 @@ int elec_evaluate(mbpol_context* ctx, u64* energy) {
         default: HIP_TRY(hipMemcpyAsync(mu_in, E->efix_pol.p, mbytes, hipMemcpyDeviceToDevice, st));
     }
     timer_stop(ctx, T_ELEC_SETUP);
-    // SOR iteration (convergeInduceDipoles, :557-616)
     const double self_term = P.pme ? (4.0 / 3.0) * (P.alpha * P.alpha * P.alpha) / mbpol_params::el_sqrt_pi : 0.0;
     double cur = 1.0e+50;
     int it = 0;
     bool conv = false;
-    for (;;) {
-        timer_start(ctx, T_ELEC_ITER);
-        if (P.pme)
-            if (int e = pme_grid(ctx, 1, mu_in)) return e;
-        const KEvents ev = timer_kernel(ctx, T_ELEC_ITERATE);
-        launch_iterate(st, E->npol, E->polidx.as<int>(), pos, P, R, mu_in, mu_out, E->efix_pol.as<double>(),
-                       E->pol.as<double>(), E->grid.as<double>(), self_term, mbpol_params::el_polar_sor,
-                       E->partial.as<double>(), ev);
-        timer_kernel_done(ctx, T_ELEC_ITERATE, ev);
-        launch_eps_finish(st, iterate_blocks(E->npol), n, E->partial.as<double>(), E->eps.as<double>());
-        timer_stop(ctx, T_ELEC_ITER);
-        HIP_TRY(hipMemcpyAsync(E->h_eps, E->eps.p, sizeof(double), hipMemcpyDeviceToHost, st));
-        HIP_TRY(hipStreamSynchronize(st));
-        std::swap(mu_in, mu_out);  // mu_in: dipoles after the step, mu_out: before it
-        const double eps = E->h_eps[0];
-        bool done = false;
-        if (eps < s.el_eps) {
-            conv = true;
-            done = true;
-        } else if (cur < eps || it >= s.el_max_iter) {
-            done = true;
+    if (E->solver == MBPOL_SOLVER_PCG) {
+        // conjugate gradients from the guess in mu_in; the solution stays in mu_in, and
+        // the reciprocal induced potential of the energy is taken from it (mu_out = mu_in)
+        if (int e = solve_pcg(ctx, mu_in, self_term, it, cur, conv)) return e;
+        mu_out = mu_in;
+    } else {
+        // SOR iteration (convergeInduceDipoles, :557-616)
+        for (;;) {
+            timer_start(ctx, T_ELEC_ITER);
+            if (P.pme)
+                if (int e = pme_grid(ctx, 1, mu_in)) return e;
+            const KEvents ev = timer_kernel(ctx, T_ELEC_ITERATE);
+            launch_iterate(st, E->npol, E->polidx.as<int>(), pos, P, R, mu_in, mu_out, E->efix_pol.as<double>(),
+                           E->pol.as<double>(), E->grid.as<double>(), self_term, mbpol_params::el_polar_sor,
+                           E->partial.as<double>(), ev);
+            timer_kernel_done(ctx, T_ELEC_ITERATE, ev);
+            launch_eps_finish(st, iterate_blocks(E->npol), n, E->partial.as<double>(), E->eps.as<double>());
+            timer_stop(ctx, T_ELEC_ITER);
+            HIP_TRY(hipMemcpyAsync(E->h_eps, E->eps.p, sizeof(double), hipMemcpyDeviceToHost, st));
+            HIP_TRY(hipStreamSynchronize(st));
+            std::swap(mu_in, mu_out);  // mu_in: dipoles after the step, mu_out: before it
+            const double eps = E->h_eps[0];
+            bool done = false;
+            if (eps < s.el_eps) {
+                conv = true;
+                done = true;
+            } else if (cur < eps || it >= s.el_max_iter) {
+                done = true;
+            }
+            cur = eps;
+            ++it;
+            if (done) break;
         }
-        cur = eps;
-        ++it;
-        if (done) break;
     }
     ctx->stats.iterations = it;
     ctx->stats.epsilon = cur;
@@ -611,6 +703,16 @@ int mbpol_context_set_dipole_guess(mbpol_context* ctx, int32_t mode, const doubl
     return MBPOL_OK;
 }
 
+int mbpol_context_set_solver(mbpol_context* ctx, int32_t solver) {
+    using namespace mbpol;
+    if (!ctx) return set_error(MBPOL_ERR_INVALID_ARGUMENT, "null context");
+    if (!ctx->elec) return set_error(MBPOL_ERR_INVALID_ARGUMENT, "no electrostatics in this system");
+    if (solver != MBPOL_SOLVER_SOR && solver != MBPOL_SOLVER_PCG)
+        return set_error(MBPOL_ERR_INVALID_ARGUMENT, "unknown dipole solver %d", solver);
+    ctx->elec->solver = solver;
+    return MBPOL_OK;
+}
+
 int mbpol_context_get_electrostatic_potential(mbpol_context* ctx, int32_t npoints, const double* points,
                                               double* potential) {
     using namespace mbpol;
@@ -644,7 +746,7 @@ int mbpol_context_update_electrostatics(mbpol_context* ctx, int32_t n, const dou
     HIP_TRY(hipSetDevice(ctx->device));
     Elec* E = ctx->elec;
     if (int e = up(E->q0, s.el_q)) return e;
-    if (int e = up(E->damp, s.el_damp)) return e;
+    if (int e = up(E->damp6, sixth_roots(s.el_damp))) return e;
     if (int e = up(E->pol, s.el_pol)) return e;
     if (int e = up(E->mol, s.el_mol)) return e;
     if (int e = up(E->type, s.el_type)) return e;

@@ -12,12 +12,13 @@ struct ElecParams {
     d3 box, inv_box;
     int3 grid;
     double thole[5];  // TCC, TCD, TDD, TDDOH, TDDHH
+    double tcc_q;     // TCC^(1/4)
     int redistribute;
 };
 
 struct SiteParams {
     const double* q;
-    const double* damp;
+    const double* damp6;  // damping^(1/6)
     const double* pol;
     const int* mol;
     const int* type;
@@ -76,6 +77,19 @@ int iterate_blocks(int npol);
 void launch_iterate(hipStream_t s, int npol, const int* polidx, const double* pos, ElecParams P, const ElecRows& R,
                     const double* mu_in, double* mu_out, const double* efix_pol, const double* pol, const double* grid,
                     double self_term, double omega, double* partial, const KEvents& ev);
+// preconditioned conjugate gradients on A mu = E_fix, A = diag(1/alpha) - T (see kernels_elec.hip)
+int cg_blocks(int npol);
+// mode 0: out = E_fix - A v, zout = M^-1 out; mode 1: out = A v.  partial: 2 per iterate_blocks(npol)
+void launch_cg_matvec(hipStream_t s, int npol, const int* polidx, const double* pos, ElecParams P, const ElecRows& R,
+                      const double* v, const double* grid, double self_term, const double* pol, const double* efix,
+                      int mode, double* out, double* zout, double* partial, const KEvents& ev);
+// step k (uses rz[k]); partial: 2 per cg_blocks(npol)
+void launch_cg_update(hipStream_t s, int npol, const int* polidx, const double* pol, double self_term,
+                      const double* partial_mv, const double* rz, int k, double* x, double* r, const double* p,
+                      const double* q, double* z, double* partial);
+// direction k (writes rz[k], reads rz[k-1]) and the convergence measure from nb block partial pairs
+void launch_cg_direction(hipStream_t s, int npol, const int* polidx, int nb, const double* partial, double* rz, int k,
+                         const double* z, double* p, int n_all, double* eps);
 void launch_eps_finish(hipStream_t s, int nblocks, int n_all, const double* partial, double* eps);
 void launch_scale_copy(hipStream_t s, int n, const double* a, const double* pol, double* out, double* out2);
 int energy_blocks(int n);

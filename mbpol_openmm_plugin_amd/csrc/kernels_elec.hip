@@ -28,7 +28,7 @@ using namespace mbpol_params;
 // ---------------------------------------------------------------------------
 // regularised upper incomplete gamma Q(3/4, x): series below a+1, Lentz
 // continued fraction above (Press et al., Numerical Recipes 3rd ed. 6.2)
-__device__ double gammq34(double x) {
+__device__ __noinline__ double gammq34_eval(double x) {
     const double a = 0.75;
     const double eps = 2.220446049250313e-16, fpmin = 2.2250738585072014e-308 / 2.220446049250313e-16;
     const double lg = 0.20328095143129538;  // ln Gamma(3/4) (Gamma(3/4) = 1.2254...)
@@ -59,33 +59,39 @@ __device__ double gammq34(double x) {
     }
     return exp(-x + a * log(x) - lg) * h;
 }
+// out of line: most pairs of a cutoff sphere have x > 745, where Q is 0
+__device__ __forceinline__ double gammq34(double x) { return x > 745.0 ? 0.0 : gammq34_eval(x); }
 
 constexpr double kGamma34 = 1.2254167024651776451;  // Gamma(3/4)
 
 // Thole scale factors of one pair (justScale = true forms).  u = (r/A)^4 with
-// A = (damp_i damp_j)^(1/6); A == 0 means undamped (all scales 1).
+// A = (damp_i damp_j)^(1/6), formed as the product of the per-site sixth roots
+// a_i = damp_i^(1/6); A == 0 means undamped (all scales 1).  The scales of one
+// Thole parameter g share e = exp(-g u): ex(g) forms it once per pair.
 struct Thole {
-    double A, ratio;
-    __device__ __forceinline__ Thole(double di, double dj, double r) {
-        A = pow(di * dj, 1 / 6.);
-        ratio = (A != 0.0) ? pow(r / A, 4) : 0.0;
+    double A, ur, ratio;  // ur = r / A, ratio = ur^4
+    __device__ __forceinline__ Thole(double ai, double aj, double r) {
+        A = ai * aj;
+        ur = (A != 0.0) ? r / A : 0.0;
+        const double u2 = ur * ur;
+        ratio = u2 * u2;
     }
     __device__ __forceinline__ bool on() const { return A != 0.0 && A > -50.0; }
-    __device__ __forceinline__ double s1(double g, double r) const {
+    __device__ __forceinline__ double ex(double g) const { return exp(-g * ratio); }
+    // gq = g^(1/4)
+    __device__ __forceinline__ double s1(double g, double gq, double e) const {
         if (!on()) return 1.0;
-        const double x = g * ratio;
-        return 1.0 - exp(-x) + pow(g, 0.25) * (r / A) * kGamma34 * gammq34(x);
+        return 1.0 - e + gq * ur * kGamma34 * gammq34(g * ratio);
     }
-    __device__ __forceinline__ double s3(double g) const { return on() ? 1.0 - exp(-g * ratio) : 1.0; }
-    __device__ __forceinline__ double s5(double g) const {
+    __device__ __forceinline__ double s3(double e) const { return on() ? 1.0 - e : 1.0; }
+    __device__ __forceinline__ double s5(double g, double e) const {
         if (!on()) return 1.0;
-        const double e = exp(-g * ratio);
         return (1.0 - e) - (4. / 3.) * g * e * ratio;
     }
-    __device__ __forceinline__ double s7(double g, double r) const {
+    // (r/A)^4 stands for the reference's r^4 / A^4
+    __device__ __forceinline__ double s7(double g, double e) const {
         if (!on()) return 1.0;
-        const double e = exp(-g * ratio);
-        return ((1.0 - e) - (4. / 3.) * g * e * ratio) - (4. / 15.) * g * (4. * g * ratio - 1.) * e / pow(A, 4) * pow(r, 4);
+        return ((1.0 - e) - (4. / 3.) * g * e * ratio) - (4. / 15.) * g * (4. * g * ratio - 1.) * e * ratio;
     }
 };
 
@@ -285,9 +291,10 @@ __global__ void __launch_bounds__(256) k_prefactors(int npol, const int* __restr
         double dx = pos[3 * j] - xi, dy = pos[3 * j + 1] - yi, dz = pos[3 * j + 2] - zi;
         min_image(dx, dy, dz, P);
         const double r2 = dx * dx + dy * dy + dz * dz, r = sqrt(r2);
-        const Thole th(S.damp[i], S.damp[j], r);
+        const Thole th(S.damp6[i], S.damp6[j], r);
         const double gam = tdd_gamma(P, S.mol[i] == S.mol[j], S.type[i], S.type[j]);
-        const double sc3 = th.s3(gam), sc5 = th.s5(gam);
+        const double eg = th.ex(gam);
+        const double sc3 = th.s3(eg), sc5 = th.s5(gam, eg);
         double p1, p2;
         if (P.pme) {
             const Ewald ew(P.alpha, r, r2);
@@ -336,7 +343,8 @@ __global__ void __launch_bounds__(256) k_fixed_field(int npol, const int* __rest
         if (!P.pme) {
             if (same) return;
             const double r2 = dx * dx + dy * dy + dz * dz, r = sqrt(r2);
-            const double rr3 = Thole(S.damp[i], S.damp[j], r).s3(P.thole[0]) / (r2 * r);
+            const Thole th(S.damp6[i], S.damp6[j], r);
+            const double rr3 = th.s3(th.ex(P.thole[0])) / (r2 * r);
             const double f = -rr3 * S.q[j];
             ex += dx * f;
             ey += dy * f;
@@ -345,7 +353,8 @@ __global__ void __launch_bounds__(256) k_fixed_field(int npol, const int* __rest
             min_image(dx, dy, dz, P);
             const double r2 = dx * dx + dy * dy + dz * dz, r = sqrt(r2);
             const Ewald ew(P.alpha, r, r2);
-            double s3 = Thole(S.damp[i], S.damp[j], r).s3(P.thole[0]);
+            const Thole th(S.damp6[i], S.damp6[j], r);
+            double s3 = th.s3(th.ex(P.thole[0]));
             if (same) s3 = 2;  // reproduced as in the reference (:1382-1385)
             const double rr3 = (s3 - 1.) / (r2 * r);
             const double f = -(ew.bn1 - rr3) * S.q[j];
@@ -839,6 +848,78 @@ __global__ void k_record_field(int n, ElecParams P, const double* __restrict__ p
 // and the per-block convergence sums.  mu_in and mu_out are distinct buffers,
 // so every field is taken from the dipoles before the step, as in the reference.
 // ---------------------------------------------------------------------------
+// Field at polarisable site i from the dipoles v, summed over the G lanes of
+// its group: real space over the P row with the prefactors, and with PME minus
+// the gradient of the reciprocal potential on the grid (4 of the 10 gather
+// components, one lane per (ix, iy) column, lanes 0..24).  The self term is
+// left to the caller.
+template <int G>
+__device__ __forceinline__ void site_field(int i, int lane, const double* __restrict__ pos, const ElecParams& P,
+                                           int capP, const int* __restrict__ nbP, const int* __restrict__ cntP,
+                                           const double2* __restrict__ pf, const double* __restrict__ v,
+                                           const double* __restrict__ grid, double& fx, double& fy, double& fz) {
+    const double xi = pos[3 * i], yi = pos[3 * i + 1], zi = pos[3 * i + 2];
+    double ex = 0, ey = 0, ez = 0;
+    const int c = min(cntP[i], capP);
+    const int* row = nbP + (size_t)i * capP;
+    const double2* prow = pf + (size_t)i * capP;
+    for (int k = lane; k < c; k += G) {
+        const int j = row[k];
+        double dx = pos[3 * j] - xi, dy = pos[3 * j + 1] - yi, dz = pos[3 * j + 2] - zi;
+        min_image(dx, dy, dz, P);
+        const double2 p = prow[k];
+        const double mx = v[3 * j], my = v[3 * j + 1], mz = v[3 * j + 2];
+        const double dur = (mx * dx + my * dy + mz * dz) * p.y;
+        ex += dx * dur + mx * p.x;
+        ey += dy * dur + my * p.x;
+        ez += dz * dur + mz * p.x;
+    }
+    if (P.pme && lane < 25) {
+        int g0[3];
+        double th[3][5][4];
+        site_splines(pos, i, P, g0, th);
+        const int a = lane / 5, b = lane % 5;
+        const int ii = wrapg(g0[0] + a, P.grid.x), jj = wrapg(g0[1] + b, P.grid.y);
+        const double* line = grid + ((size_t)ii * P.grid.y + jj) * P.grid.z;
+        double t0 = 0, t1 = 0;
+#pragma unroll
+        for (int iz = 0; iz < 5; ++iz) {
+            const double gv = line[wrapg(g0[2] + iz, P.grid.z)];
+            t0 += gv * th[2][iz][0];
+            t1 += gv * th[2][iz][1];
+        }
+        const double x0 = pick5(th[0], a, 0), x1 = pick5(th[0], a, 1), y0 = pick5(th[1], b, 0), y1 = pick5(th[1], b, 1);
+        ex -= P.grid.x * P.inv_box.x * (x1 * y0 * t0);
+        ey -= P.grid.y * P.inv_box.y * (x0 * y1 * t0);
+        ez -= P.grid.z * P.inv_box.z * (x0 * y0 * t1);
+    }
+    fx = group_sum<G>(ex);
+    fy = group_sum<G>(ey);
+    fz = group_sum<G>(ez);
+}
+
+// per-block sum of one value per group (lane 0 of each group holds it)
+template <int G>
+__device__ __forceinline__ void block_partial(double v, int lane, double* __restrict__ out) {
+    __shared__ double red[256 / G];
+    if (lane == 0) red[threadIdx.x / G] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = 0.0;
+        for (int k = 0; k < 256 / G; ++k) s += red[k];
+        *out = s;
+    }
+    __syncthreads();  // red may be reused by the next call
+}
+
+// ---------------------------------------------------------------------------
+// one SOR step for the polarisable sites (convergeInduceDipoles and
+// calculateInducedDipoleField, MBPolReferenceElectrostaticsForce.cpp:557-616,
+// :516-531): field of mu_in, self term, SOR step
+// mu_out = mu_in + omega (alpha (E_fix + E_ind) - mu_in) and the per-block
+// convergence sums.  mu_in and mu_out are distinct buffers, so every field is
+// taken from the dipoles before the step, as in the reference.
+// ---------------------------------------------------------------------------
 template <int G>
 __global__ void __launch_bounds__(256) k_iterate(int npol, const int* __restrict__ polidx, const double* __restrict__ pos,
                                                  ElecParams P, int capP, const int* __restrict__ nbP,
@@ -847,50 +928,12 @@ __global__ void __launch_bounds__(256) k_iterate(int npol, const int* __restrict
                                                  const double* __restrict__ efix_pol, const double* __restrict__ pol,
                                                  const double* __restrict__ grid, double self_term, double omega,
                                                  double* __restrict__ partial) {
-    __shared__ double red[256 / G];
     const int g = (blockIdx.x * blockDim.x + threadIdx.x) / G, lane = threadIdx.x % G;
     double e2 = 0.0;
     if (g < npol) {
         const int i = polidx[g];
-        const double xi = pos[3 * i], yi = pos[3 * i + 1], zi = pos[3 * i + 2];
-        double ex = 0, ey = 0, ez = 0;
-        const int c = min(cntP[i], capP);
-        const int* row = nbP + (size_t)i * capP;
-        const double2* prow = pf + (size_t)i * capP;
-        for (int k = lane; k < c; k += G) {
-            const int j = row[k];
-            double dx = pos[3 * j] - xi, dy = pos[3 * j + 1] - yi, dz = pos[3 * j + 2] - zi;
-            min_image(dx, dy, dz, P);
-            const double2 p = prow[k];
-            const double mx = mu_in[3 * j], my = mu_in[3 * j + 1], mz = mu_in[3 * j + 2];
-            const double dur = (mx * dx + my * dy + mz * dz) * p.y;
-            ex += dx * dur + mx * p.x;
-            ey += dy * dur + my * p.x;
-            ez += dz * dur + mz * p.x;
-        }
-        if (P.pme && lane < 25) {
-            int g0[3];
-            double th[3][5][4];
-            site_splines(pos, i, P, g0, th);
-            const int a = lane / 5, b = lane % 5;
-            const int ii = wrapg(g0[0] + a, P.grid.x), jj = wrapg(g0[1] + b, P.grid.y);
-            const double* line = grid + ((size_t)ii * P.grid.y + jj) * P.grid.z;
-            double t0 = 0, t1 = 0;
-#pragma unroll
-            for (int iz = 0; iz < 5; ++iz) {
-                const double gv = line[wrapg(g0[2] + iz, P.grid.z)];
-                t0 += gv * th[2][iz][0];
-                t1 += gv * th[2][iz][1];
-            }
-            const double x0 = pick5(th[0], a, 0), x1 = pick5(th[0], a, 1), y0 = pick5(th[1], b, 0),
-                         y1 = pick5(th[1], b, 1);
-            ex -= P.grid.x * P.inv_box.x * (x1 * y0 * t0);
-            ey -= P.grid.y * P.inv_box.y * (x0 * y1 * t0);
-            ez -= P.grid.z * P.inv_box.z * (x0 * y0 * t1);
-        }
-        ex = group_sum<G>(ex);
-        ey = group_sum<G>(ey);
-        ez = group_sum<G>(ez);
+        double ex, ey, ez;
+        site_field<G>(i, lane, pos, P, capP, nbP, cntP, pf, mu_in, grid, ex, ey, ez);
         if (lane == 0) {
             const double m[3] = {mu_in[3 * i], mu_in[3 * i + 1], mu_in[3 * i + 2]};
             const double f[3] = {ex + m[0] * self_term, ey + m[1] * self_term, ez + m[2] * self_term};
@@ -903,12 +946,125 @@ __global__ void __launch_bounds__(256) k_iterate(int npol, const int* __restrict
             }
         }
     }
-    if (lane == 0) red[threadIdx.x / G] = e2;
+    block_partial<G>(e2, lane, partial + blockIdx.x);
+}
+
+// ---------------------------------------------------------------------------
+// Preconditioned conjugate gradients for the same linear system, written in
+// its symmetric form A mu = E_fix with A = diag(1/alpha) - T, where T mu is
+// the induced field (real space, reciprocal and self term) that the SOR step
+// uses; the preconditioner is the diagonal of A, 1/alpha - self.  At every
+// step the solver also forms the reference's convergence measure of the
+// current dipoles, omega * debye * rms(alpha (E_fix + T mu) - mu) = omega *
+// debye * rms(alpha r), so it stops by the same rule as the SOR iteration.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double cg_minv(double a, double self_term) { return a / (1.0 - a * self_term); }
+
+// mode 0: r = E_fix - A v (the initial residual), z = M^-1 r, partials (r.z, |alpha r|^2)
+// mode 1: q = A v, partials (v.q, 0)
+template <int G>
+__global__ void __launch_bounds__(256) k_cg_matvec(int npol, const int* __restrict__ polidx, const double* __restrict__ pos,
+                                                   ElecParams P, int capP, const int* __restrict__ nbP,
+                                                   const int* __restrict__ cntP, const double2* __restrict__ pf,
+                                                   const double* __restrict__ v, const double* __restrict__ grid,
+                                                   double self_term, const double* __restrict__ pol,
+                                                   const double* __restrict__ efix, int mode, double* __restrict__ out,
+                                                   double* __restrict__ zout, double* __restrict__ partial) {
+    const int g = (blockIdx.x * blockDim.x + threadIdx.x) / G, lane = threadIdx.x % G;
+    double s0 = 0.0, s1 = 0.0;
+    if (g < npol) {
+        const int i = polidx[g];
+        double ex, ey, ez;
+        site_field<G>(i, lane, pos, P, capP, nbP, cntP, pf, v, grid, ex, ey, ez);
+        if (lane == 0) {
+            const double a = pol[i], ia = 1.0 / a, mi = cg_minv(a, self_term);
+            const double f[3] = {ex, ey, ez};
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                const double vd = v[3 * i + d];
+                const double av = vd * ia - (f[d] + vd * self_term);  // (A v)_d
+                if (mode == 0) {
+                    const double r = efix[3 * i + d] - av;
+                    out[3 * i + d] = r;
+                    zout[3 * i + d] = r * mi;
+                    s0 += r * r * mi;
+                    s1 += (a * r) * (a * r);
+                } else {
+                    out[3 * i + d] = av;
+                    s0 += vd * av;
+                }
+            }
+        }
+    }
+    block_partial<G>(s0, lane, partial + 2 * blockIdx.x);
+    block_partial<G>(s1, lane, partial + 2 * blockIdx.x + 1);
+}
+
+// sum of the entries 2b + c (b < nb) of partial, in an order fixed by nb
+// alone, so that every block of a launch obtains the same value
+__device__ __forceinline__ double reduce_partials(int nb, int c, const double* __restrict__ partial) {
+    __shared__ double red[256];
+    double s = 0.0;
+    for (int b = threadIdx.x; b < nb; b += 256) s += partial[2 * b + c];
+    red[threadIdx.x] = s;
     __syncthreads();
-    if (threadIdx.x == 0) {
-        double s = 0.0;
-        for (int k = 0; k < 256 / G; ++k) s += red[k];
-        partial[blockIdx.x] = s;
+    for (int k = 128; k > 0; k >>= 1) {
+        if (threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+        __syncthreads();
+    }
+    const double r = red[0];
+    __syncthreads();
+    return r;
+}
+
+// step k: a = rz[k] / (p.q); x += a p; r -= a q; z = M^-1 r; partials (r.z, |alpha r|^2)
+__global__ void __launch_bounds__(256) k_cg_update(int npol, const int* __restrict__ polidx,
+                                                   const double* __restrict__ pol, double self_term, int nb_mv,
+                                                   const double* __restrict__ partial_mv,
+                                                   const double* __restrict__ rz, int k, double* __restrict__ x,
+                                                   double* __restrict__ r, const double* __restrict__ p,
+                                                   const double* __restrict__ q, double* __restrict__ z,
+                                                   double* __restrict__ partial) {
+    const double pq = reduce_partials(nb_mv, 0, partial_mv);
+    const double step = rz[k] / pq;
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    double s0 = 0.0, s1 = 0.0;
+    if (g < npol) {
+        const int i = polidx[g];
+        const double a = pol[i], mi = cg_minv(a, self_term);
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const int t = 3 * i + d;
+            x[t] += step * p[t];
+            const double rv = r[t] - step * q[t];
+            r[t] = rv;
+            z[t] = rv * mi;
+            s0 += rv * rv * mi;
+            s1 += (a * rv) * (a * rv);
+        }
+    }
+    block_partial<1>(s0, 0, partial + 2 * blockIdx.x);
+    block_partial<1>(s1, 0, partial + 2 * blockIdx.x + 1);
+}
+
+// k-th direction: rz[k] = r.z; p = z (k == 0) or z + (rz[k] / rz[k-1]) p;
+// eps_out = omega * debye * rms(alpha r) over all n sites
+__global__ void __launch_bounds__(256) k_cg_direction(int npol, const int* __restrict__ polidx, int nb,
+                                                      const double* __restrict__ partial, double* __restrict__ rz,
+                                                      int k, const double* __restrict__ z, double* __restrict__ p,
+                                                      int n_all, double* __restrict__ eps_out) {
+    const double s_rz = reduce_partials(nb, 0, partial);
+    const double s_ee = reduce_partials(nb, 1, partial);
+    const double beta = k == 0 ? 0.0 : s_rz / rz[k - 1];
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g < npol) {
+        const int i = polidx[g];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) p[3 * i + d] = z[3 * i + d] + beta * p[3 * i + d];
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        rz[k] = s_rz;
+        *eps_out = el_polar_sor * el_debye * sqrt(s_ee / (double)n_all);
     }
 }
 
@@ -962,7 +1118,7 @@ __global__ void __launch_bounds__(256) k_elec_energy(int n, const double* __rest
             const double ck = S.q[j];
             const double mjx = mu[3 * j], mjy = mu[3 * j + 1], mjz = mu[3 * j + 2];
             const int same = S.mol[i] == S.mol[j];
-            const Thole th(S.damp[i], S.damp[j], r);
+            const Thole th(S.damp6[i], S.damp6[j], r);
             const double gdd = tdd_gamma(P, same, S.type[i], S.type[j]);
             const double rr1 = 1.0 / r, rr3 = rr1 / r2, rr5 = 3.0 * rr3 / r2, rr7 = 5.0 * rr5 / r2;
             const double sci3 = mix * dx + miy * dy + miz * dz;  // mu_i . d
@@ -970,18 +1126,25 @@ __global__ void __launch_bounds__(256) k_elec_energy(int n, const double* __rest
             const double mm = mix * mjx + miy * mjy + miz * mjz;
             const double gl0 = ci * ck;
             const double gli1 = ck * sci3 - ci * sci4;
+            // the charge-charge (TCC) and charge-dipole (TCD) scales of a pair
+            // within one molecule are never used
+            const double ecc = same ? 0. : th.ex(P.thole[0]), ecd = same ? 0. : th.ex(P.thole[1]);
+            const double edd = th.ex(gdd);
+            const double s5DD = th.s5(gdd, edd), s7DD = th.s7(gdd, edd);
             double gx, gy, gz;  // (ftm2 + ftm2i): force on i is -f * this
             if (P.pme) {
                 const Ewald ew(P.alpha, r, r2);
-                double s1CC = th.s1(P.thole[0], r), s3CD = th.s3(P.thole[1]);
-                if (same) s1CC = s3CD = 0.;
+                double s1CC = 0., s3CD = 0., s3CC = 0., s5CD = 0.;
+                if (!same) {
+                    s1CC = th.s1(P.thole[0], P.tcc_q, ecc);
+                    s3CD = th.s3(ecd);
+                    s3CC = th.s3(ecc);
+                    s5CD = th.s5(P.thole[1], ecd);
+                }
                 const double e = ew.bn0 * gl0 - rr1 * gl0 * (1 - s1CC);
                 const double ei = 0.5 * (ew.bn1 * gli1) - 0.5 * (rr3 * gli1 * (1 - s3CD));
                 en += e + ei;
                 vi += ck * (ew.bn0 - rr1 * (1 - s1CC)) - sci4 * (ew.bn1 - rr3 * (1 - s3CD));
-                const double s5DD = th.s5(gdd), s7DD = th.s7(gdd, r);
-                const double s3CC = same ? 0. : th.s3(P.thole[0]);
-                const double s5CD = same ? 0. : th.s5(P.thole[1]);
                 const double gf1 = ew.bn1 * gl0 - (1 - s3CC) * rr3 * gl0;
                 const double gfi1 = ew.bn2 * (gli1 + mm) - ew.bn3 * sci3 * sci4;
                 const double gfri1 = rr5 * (gli1 * (1 - s5CD) + mm * (1 - s5DD)) - rr7 * sci3 * sci4 * (1 - s7DD);
@@ -992,20 +1155,17 @@ __global__ void __launch_bounds__(256) k_elec_energy(int n, const double* __rest
                 gy = dy * a + miy * bi + mjy * bj;
                 gz = dz * a + miz * bi + mjz * bj;
             } else {
-                const double s1CC = th.s1(P.thole[0], r), s3CD = th.s3(P.thole[1]);
-                const double s3CC = th.s3(P.thole[0]), s5CD = th.s5(P.thole[1]);
-                const double s5DD = th.s5(gdd), s7DD = th.s7(gdd, r);
-                const double g0 = same ? 0. : gl0, gi = same ? 0. : gli1;
-                en += rr1 * g0 * s1CC + 0.5 * (rr3 * gi * s3CD);
-                const double gf0 = rr3 * g0 * s3CC;
-                const double gfi0 = rr5 * gi * s5CD + rr5 * mm * s5DD - rr7 * sci3 * sci4 * s7DD;
-                const double a = gf0 + gfi0;
                 double bi = sci4 * rr5 * s5DD, bj = sci3 * rr5 * s5DD;
+                double a = rr5 * mm * s5DD - rr7 * sci3 * sci4 * s7DD;
                 if (!same) {
+                    const double s1CC = th.s1(P.thole[0], P.tcc_q, ecc), s3CD = th.s3(ecd);
+                    const double s3CC = th.s3(ecc), s5CD = th.s5(P.thole[1], ecd);
+                    en += rr1 * gl0 * s1CC + 0.5 * (rr3 * gli1 * s3CD);
+                    a += rr3 * gl0 * s3CC + rr5 * gli1 * s5CD;
                     bi += -ck * rr3 * s3CD;
                     bj += ci * rr3 * s3CD;
                     if (P.redistribute) {  // potential at i from the charge and dipole of j (TCC damping)
-                        vi += ck * rr1 * s1CC + th.s3(P.thole[0]) * (-sci4) * rr3;
+                        vi += ck * rr1 * s1CC + s3CC * (-sci4) * rr3;
                     }
                 }
                 gx = dx * a + mix * bi + mjx * bj;
@@ -1223,6 +1383,27 @@ void launch_iterate(hipStream_t s, int npol, const int* polidx, const double* po
                     double self_term, double omega, double* partial, const KEvents& ev) {
     MBPOL_LAUNCH(ev, k_iterate<kGI>, iterate_blocks(npol), 256, 0, s, npol, polidx, pos, P, R.capP, R.nbP, R.cntP, R.pf,
                  mu_in, mu_out, efix_pol, pol, grid, self_term, omega, partial);
+}
+int cg_blocks(int npol) { return std::max(1, nblk(npol, 256)); }
+void launch_cg_matvec(hipStream_t s, int npol, const int* polidx, const double* pos, ElecParams P, const ElecRows& R,
+                      const double* v, const double* grid, double self_term, const double* pol, const double* efix,
+                      int mode, double* out, double* zout, double* partial, const KEvents& ev) {
+    if (npol > 0)
+        MBPOL_LAUNCH(ev, k_cg_matvec<kGI>, iterate_blocks(npol), 256, 0, s, npol, polidx, pos, P, R.capP, R.nbP, R.cntP,
+                     R.pf, v, grid, self_term, pol, efix, mode, out, zout, partial);
+}
+void launch_cg_update(hipStream_t s, int npol, const int* polidx, const double* pol, double self_term,
+                      const double* partial_mv, const double* rz, int k, double* x, double* r, const double* p,
+                      const double* q, double* z, double* partial) {
+    if (npol > 0)
+        hipLaunchKernelGGL(k_cg_update, cg_blocks(npol), 256, 0, s, npol, polidx, pol, self_term, iterate_blocks(npol),
+                           partial_mv, rz, k, x, r, p, q, z, partial);
+}
+void launch_cg_direction(hipStream_t s, int npol, const int* polidx, int nb, const double* partial, double* rz, int k,
+                         const double* z, double* p, int n_all, double* eps) {
+    if (npol > 0)
+        hipLaunchKernelGGL(k_cg_direction, cg_blocks(npol), 256, 0, s, npol, polidx, nb, partial, rz, k, z, p, n_all,
+                           eps);
 }
 void launch_eps_finish(hipStream_t s, int nblocks, int n_all, const double* partial, double* eps) {
     hipLaunchKernelGGL(k_eps_finish, 1, 256, 0, s, nblocks, n_all, partial, eps);

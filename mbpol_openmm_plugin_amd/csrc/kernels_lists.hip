@@ -74,13 +74,24 @@ __global__ void k_cell_fill(int n, const int* __restrict__ cell_of, const int* _
     cell_members[cell_start[c] + slot] = m;
 }
 
-// Visit every candidate partner of molecule m: the 27 neighbouring cells when
-// a cell grid is in use, else every molecule.
-template <typename F>
-__device__ __forceinline__ void for_candidates(int m, int n, const ListGeom& g, const int* cell_of, const int* cell_start,
-                                               const int* cell_members, F&& f) {
+// Lanes of one G-lane group share molecule m; these helpers give the group's
+// lane bits and the bits of the lanes below this one, for ballot compaction.
+template <int G>
+__device__ __forceinline__ unsigned long long group_bits() {
+    const int lane = threadIdx.x & 63;
+    return G == 64 ? ~0ull : (((1ull << G) - 1ull) << (lane & ~(G - 1)));
+}
+__device__ __forceinline__ unsigned long long lanes_below() { return (1ull << (threadIdx.x & 63)) - 1ull; }
+
+// Visit every candidate partner of molecule m with the G lanes of its group:
+// the 27 neighbouring cells when a cell grid is in use, else every molecule.
+// f(j, active) is called by all lanes of the group in step (active = lane has
+// a candidate), so that it may use wavefront ballots.
+template <int G, typename F>
+__device__ __forceinline__ void for_candidates(int m, int lane, int n, const ListGeom& g, const int* cell_of,
+                                               const int* cell_start, const int* cell_members, F&& f) {
     if (!g.use_cells) {
-        for (int j = 0; j < n; ++j) f(j);
+        for (int b = 0; b < n; b += G) f(b + lane, b + lane < n);
         return;
     }
     const int c = cell_of[m];
@@ -91,28 +102,35 @@ __device__ __forceinline__ void for_candidates(int m, int n, const ListGeom& g, 
                 const int cc = (wrap_cell(cx + dx, g.nc.x) * g.nc.y + wrap_cell(cy + dy, g.nc.y)) * g.nc.z +
                                wrap_cell(cz + dz, g.nc.z);
                 const int b = cell_start[cc], e = cell_start[cc + 1];
-                for (int t = b; t < e; ++t) f(cell_members[t]);
+                for (int t = b; t < e; t += G) {
+                    const bool act = t + lane < e;
+                    f(act ? cell_members[t + lane] : -1, act);
+                }
             }
 }
 
-// ---- dimer list ----
-__global__ void k_dimer_pass(int n, const int* __restrict__ idx, const double* __restrict__ pos, ListGeom g,
-                             const int* __restrict__ cell_of, const int* __restrict__ cell_start,
-                             const int* __restrict__ cell_members, long long* __restrict__ count,
-                             const long long* __restrict__ offset, int2* __restrict__ pairs) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+constexpr int kGL = 16;  // lanes per molecule in the pair passes
+
+// ---- dimer list: count pass (pairs == nullptr) and fill pass ----
+__global__ void __launch_bounds__(256) k_dimer_pass(int n, const int* __restrict__ idx, const double* __restrict__ pos,
+                                                    ListGeom g, const int* __restrict__ cell_of,
+                                                    const int* __restrict__ cell_start,
+                                                    const int* __restrict__ cell_members, long long* __restrict__ count,
+                                                    const long long* __restrict__ offset, int2* __restrict__ pairs) {
+    const int i = (blockIdx.x * blockDim.x + threadIdx.x) / kGL, lane = threadIdx.x % kGL;
+    if (i >= n) return;  // uniform over the group
     const d3 pi = opos(pos, idx, i);
     const double rc2 = g.rc * g.rc;
+    const unsigned long long gb = group_bits<kGL>(), below = lanes_below();
     long long c = 0;
     const long long o = offset ? offset[i] : 0;
-    for_candidates(i, n, g, cell_of, cell_start, cell_members, [&](int j) {
-        if (j <= i) return;
-        if (dist2(pi, opos(pos, idx, j), g.L, g.periodic) > rc2) return;
-        if (pairs) pairs[o + c] = make_int2(i, j);
-        ++c;
+    for_candidates<kGL>(i, lane, n, g, cell_of, cell_start, cell_members, [&](int j, bool act) {
+        const bool ok = act && j > i && dist2(pi, opos(pos, idx, j), g.L, g.periodic) <= rc2;
+        const unsigned long long bits = __ballot(ok) & gb;
+        if (ok && pairs) pairs[o + c + __popcll(bits & below)] = make_int2(i, j);
+        c += __popcll(bits);
     });
-    if (count) count[i] = c;
+    if (count && lane == 0) count[i] = c;
 }
 
 // ---- chain trimer list ----
@@ -130,25 +148,30 @@ __device__ __forceinline__ bool phantom_voxel(const d3& p, const ListGeom& g) {
     return false;
 }
 
-__global__ void k_lower_pass(int n, const int* __restrict__ idx, const double* __restrict__ pos, ListGeom g,
-                             const int* __restrict__ cell_of, const int* __restrict__ cell_start,
-                             const int* __restrict__ cell_members, int* __restrict__ count,
-                             const int* __restrict__ offset, int* __restrict__ lower) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+// lo(i) = {j < i : d(i,j) <= rc, j not in a phantom voxel}: count pass (lower == nullptr) and fill pass
+__global__ void __launch_bounds__(256) k_lower_pass(int n, const int* __restrict__ idx, const double* __restrict__ pos,
+                                                    ListGeom g, const int* __restrict__ cell_of,
+                                                    const int* __restrict__ cell_start,
+                                                    const int* __restrict__ cell_members, int* __restrict__ count,
+                                                    const int* __restrict__ offset, int* __restrict__ lower) {
+    const int i = (blockIdx.x * blockDim.x + threadIdx.x) / kGL, lane = threadIdx.x % kGL;
     if (i >= n) return;
     const d3 pi = opos(pos, idx, i);
     const double rc2 = g.rc * g.rc;
+    const unsigned long long gb = group_bits<kGL>(), below = lanes_below();
     int c = 0;
     const int o = offset ? offset[i] : 0;
-    for_candidates(i, n, g, cell_of, cell_start, cell_members, [&](int j) {
-        if (j >= i) return;
-        const d3 pj = opos(pos, idx, j);
-        if (dist2(pi, pj, g.L, g.periodic) > rc2) return;
-        if (phantom_voxel(pj, g)) return;
-        if (lower) lower[o + c] = j;
-        ++c;
+    for_candidates<kGL>(i, lane, n, g, cell_of, cell_start, cell_members, [&](int j, bool act) {
+        bool ok = act && j < i;
+        if (ok) {
+            const d3 pj = opos(pos, idx, j);
+            ok = dist2(pi, pj, g.L, g.periodic) <= rc2 && !phantom_voxel(pj, g);
+        }
+        const unsigned long long bits = __ballot(ok) & gb;
+        if (ok && lower) lower[o + c + __popcll(bits & below)] = j;
+        c += __popcll(bits);
     });
-    if (count) count[i] = c;
+    if (count && lane == 0) count[i] = c;
 }
 
 // O-O distances of a triplet after the reference's imaging (every O imaged to Oa,
@@ -179,30 +202,40 @@ __device__ __forceinline__ bool trimer_active(const d3& a, const d3& b, const d3
     return on >= 2;
 }
 
-__global__ void k_trimer_pass(int n, const int* __restrict__ idx, const double* __restrict__ pos, ListGeom g,
-                              const int* __restrict__ lo_start, const int* __restrict__ lower, int full_list,
-                              long long* __restrict__ count, const long long* __restrict__ offset,
-                              int* __restrict__ trip) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+constexpr int kGT = 8;  // lanes per molecule in the trimer pass
+
+// trimers (i, j, k) with j in lo(i), k in lo(j): the lanes of i's group take
+// the k of one j at a time, compacting the active ones by ballot
+__global__ void __launch_bounds__(256) k_trimer_pass(int n, const int* __restrict__ idx, const double* __restrict__ pos,
+                                                     ListGeom g, const int* __restrict__ lo_start,
+                                                     const int* __restrict__ lower, int full_list,
+                                                     long long* __restrict__ count, const long long* __restrict__ offset,
+                                                     int* __restrict__ trip) {
+    const int i = (blockIdx.x * blockDim.x + threadIdx.x) / kGT, lane = threadIdx.x % kGT;
+    if (i >= n) return;  // uniform over the group
     const d3 pi = opos(pos, idx, i);
+    const unsigned long long gb = group_bits<kGT>(), below = lanes_below();
     long long c = 0;
     const long long o = offset ? offset[i] : 0;
     for (int a = lo_start[i]; a < lo_start[i + 1]; ++a) {
         const int j = lower[a];
         const d3 pj = opos(pos, idx, j);
-        for (int b = lo_start[j]; b < lo_start[j + 1]; ++b) {
-            const int k = lower[b];
-            if (!full_list && !trimer_active(pi, pj, opos(pos, idx, k), g)) continue;
-            if (trip) {
-                trip[3 * (o + c)] = i;
-                trip[3 * (o + c) + 1] = j;
-                trip[3 * (o + c) + 2] = k;
+        const int e = lo_start[j + 1];
+        for (int b = lo_start[j]; b < e; b += kGT) {
+            const bool act = b + lane < e;
+            const int k = act ? lower[b + lane] : 0;
+            const bool ok = act && (full_list || trimer_active(pi, pj, opos(pos, idx, k), g));
+            const unsigned long long bits = __ballot(ok) & gb;
+            if (ok && trip) {
+                const long long t = o + c + __popcll(bits & below);
+                trip[3 * t] = i;
+                trip[3 * t + 1] = j;
+                trip[3 * t + 2] = k;
             }
-            ++c;
+            c += __popcll(bits);
         }
     }
-    if (count) count[i] = c;
+    if (count && lane == 0) count[i] = c;
 }
 
 }  // namespace mbpol
@@ -221,19 +254,19 @@ void launch_dimer_pass(hipStream_t s, int n, const int* idx, const double* pos, 
                        const int* cell_start, const int* cell_members, long long* count, const long long* offset,
                        int2* pairs) {
     if (n > 0)
-        hipLaunchKernelGGL(k_dimer_pass, nblk(n, 64), 64, 0, s, n, idx, pos, g, cell_of, cell_start, cell_members, count,
-                           offset, pairs);
+        hipLaunchKernelGGL(k_dimer_pass, nblk((long long)n * kGL, 256), 256, 0, s, n, idx, pos, g, cell_of, cell_start,
+                           cell_members, count, offset, pairs);
 }
 void launch_lower_pass(hipStream_t s, int n, const int* idx, const double* pos, ListGeom g, const int* cell_of,
                        const int* cell_start, const int* cell_members, int* count, const int* offset, int* lower) {
     if (n > 0)
-        hipLaunchKernelGGL(k_lower_pass, nblk(n, 64), 64, 0, s, n, idx, pos, g, cell_of, cell_start, cell_members, count,
-                           offset, lower);
+        hipLaunchKernelGGL(k_lower_pass, nblk((long long)n * kGL, 256), 256, 0, s, n, idx, pos, g, cell_of, cell_start,
+                           cell_members, count, offset, lower);
 }
 void launch_trimer_pass(hipStream_t s, int n, const int* idx, const double* pos, ListGeom g, const int* lo_start,
                         const int* lower, int full_list, long long* count, const long long* offset, int* trip) {
     if (n > 0)
-        hipLaunchKernelGGL(k_trimer_pass, nblk(n, 64), 64, 0, s, n, idx, pos, g, lo_start, lower, full_list, count,
-                           offset, trip);
+        hipLaunchKernelGGL(k_trimer_pass, nblk((long long)n * kGT, 256), 256, 0, s, n, idx, pos, g, lo_start, lower,
+                           full_list, count, offset, trip);
 }
 }  // namespace mbpol

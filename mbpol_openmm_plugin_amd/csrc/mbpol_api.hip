@@ -55,17 +55,22 @@ hipError_t DevBuf::ensure(size_t nbytes) {
 // ---------------------------------------------------------------------------
 // scans (hipCUB)
 // ---------------------------------------------------------------------------
+// each stream has its own scratch, so that scans on the two streams may overlap
 template <typename In, typename Out>
-static int exclusive_scan(mbpol_context* ctx, const In* in, Out* out, int n) {
+static int exclusive_scan(mbpol_context* ctx, const In* in, Out* out, int n, hipStream_t s) {
+    if (!s) s = ctx->stream;
+    DevBuf& tmp = (s == ctx->stream) ? ctx->scan_tmp : ctx->scan_tmp_side;
     size_t bytes = 0;
-    HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, in, out, n, ctx->stream));
-    HIP_TRY(ctx->scan_tmp.ensure(bytes));
-    HIP_TRY(hipcub::DeviceScan::ExclusiveSum(ctx->scan_tmp.p, bytes, in, out, n, ctx->stream));
+    HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, in, out, n, s));
+    HIP_TRY(tmp.ensure(bytes));
+    HIP_TRY(hipcub::DeviceScan::ExclusiveSum(tmp.p, bytes, in, out, n, s));
     return MBPOL_OK;
 }
-int mbpol::scan_i32(mbpol_context* ctx, const int* in, int* out, int n) { return exclusive_scan(ctx, in, out, n); }
-int mbpol::scan_i64(mbpol_context* ctx, const long long* in, long long* out, int n) {
-    return exclusive_scan(ctx, in, out, n);
+int mbpol::scan_i32(mbpol_context* ctx, const int* in, int* out, int n, hipStream_t s) {
+    return exclusive_scan(ctx, in, out, n, s);
+}
+int mbpol::scan_i64(mbpol_context* ctx, const long long* in, long long* out, int n, hipStream_t s) {
+    return exclusive_scan(ctx, in, out, n, s);
 }
 
 // ---------------------------------------------------------------------------
@@ -327,14 +332,14 @@ static int setup_grid(mbpol_context* ctx, CellGrid& cg, int n, bool periodic, do
     return MBPOL_OK;
 }
 
-static int bin_cells(mbpol_context* ctx, CellGrid& cg, int n, const int* idx) {
+static int bin_cells(mbpol_context* ctx, hipStream_t st, CellGrid& cg, int n, const int* idx) {
     if (!cg.g.use_cells) return MBPOL_OK;
-    HIP_TRY(hipMemsetAsync(cg.cell_count.p, 0, sizeof(int) * (cg.ncell + 1), ctx->stream));
-    HIP_TRY(hipMemsetAsync(cg.cell_fill.p, 0, sizeof(int) * (cg.ncell + 1), ctx->stream));
-    launch_cell_count(ctx->stream, n, idx, ctx->pos.as<double>(), cg.g.L, cg.g.nc, cg.cell_of.as<int>(),
+    HIP_TRY(hipMemsetAsync(cg.cell_count.p, 0, sizeof(int) * (cg.ncell + 1), st));
+    HIP_TRY(hipMemsetAsync(cg.cell_fill.p, 0, sizeof(int) * (cg.ncell + 1), st));
+    launch_cell_count(st, n, idx, ctx->pos.as<double>(), cg.g.L, cg.g.nc, cg.cell_of.as<int>(),
                       cg.cell_count.as<int>());
-    if (int e = scan_i32(ctx, cg.cell_count.as<int>(), cg.cell_start.as<int>(), cg.ncell + 1)) return e;
-    launch_cell_fill(ctx->stream, n, cg.cell_of.as<int>(), cg.cell_start.as<int>(), cg.cell_fill.as<int>(),
+    if (int e = scan_i32(ctx, cg.cell_count.as<int>(), cg.cell_start.as<int>(), cg.ncell + 1, st)) return e;
+    launch_cell_fill(st, n, cg.cell_of.as<int>(), cg.cell_start.as<int>(), cg.cell_fill.as<int>(),
                      cg.members.as<int>());
     return MBPOL_OK;
 }
@@ -376,8 +381,12 @@ int mbpol_context_create(const mbpol_system* sys, int32_t precision, int32_t dev
         mbpol_context_destroy(ctx);
         return code;
     };
-    if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess)
+    if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking) != hipSuccess)
         return fail(set_error(MBPOL_ERR_HIP, "hipStreamCreate failed"));
+    if (hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming) != hipSuccess)
+        return fail(set_error(MBPOL_ERR_HIP, "hipEventCreate failed"));
     if (hipHostMalloc(&ctx->h_counts, 4096, hipHostMallocDefault) != hipSuccess)
         return fail(set_error(MBPOL_ERR_HIP, "hipHostMalloc failed"));
     const int n = sys->n;
@@ -415,11 +424,15 @@ void mbpol_context_destroy(mbpol_context* ctx) {
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->side) (void)hipStreamSynchronize(ctx->side);
     timer_collect(ctx);
     for (hipEvent_t e : ctx->timer.pool) (void)hipEventDestroy(e);
     elec_destroy(ctx);
     disp_destroy(ctx);
     if (ctx->h_counts) (void)hipHostFree(ctx->h_counts);
+    if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);
+    if (ctx->ev_join) (void)hipEventDestroy(ctx->ev_join);
+    if (ctx->side) (void)hipStreamDestroy(ctx->side);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -450,31 +463,31 @@ int mbpol_context_set_list_mode(mbpol_context* ctx, int32_t full_chain_list) {
 }  // extern "C"
 
 // Build the dimer list and the trimer work list; returns their sizes after one sync.
-static int build_lists(mbpol_context* ctx, bool want2, bool want3) {
+static int build_lists(mbpol_context* ctx, hipStream_t st, bool want2, bool want3) {
     const mbpol_system& s = ctx->sys;
     const int n2 = (int)s.tb_idx.size() / 3, n3 = (int)s.th_idx.size() / 3;
     const double* pos = ctx->pos.as<double>();
     long long* hc = static_cast<long long*>(ctx->h_counts);
     if (want2) {
         CellGrid& g = ctx->grid2;
-        if (int e = bin_cells(ctx, g, n2, ctx->tb_idx.as<int>())) return e;
-        HIP_TRY(hipMemsetAsync(ctx->dimer_count.p, 0, sizeof(long long) * (n2 + 1), ctx->stream));
-        launch_dimer_pass(ctx->stream, n2, ctx->tb_idx.as<int>(), pos, g.g, g.cell_of.as<int>(), g.cell_start.as<int>(),
+        if (int e = bin_cells(ctx, st, g, n2, ctx->tb_idx.as<int>())) return e;
+        HIP_TRY(hipMemsetAsync(ctx->dimer_count.p, 0, sizeof(long long) * (n2 + 1), st));
+        launch_dimer_pass(st, n2, ctx->tb_idx.as<int>(), pos, g.g, g.cell_of.as<int>(), g.cell_start.as<int>(),
                           g.members.as<int>(), ctx->dimer_count.as<long long>(), nullptr, nullptr);
-        if (int e = scan_i64(ctx, ctx->dimer_count.as<long long>(), ctx->dimer_off.as<long long>(), n2 + 1)) return e;
+        if (int e = scan_i64(ctx, ctx->dimer_count.as<long long>(), ctx->dimer_off.as<long long>(), n2 + 1, st)) return e;
         HIP_TRY(hipMemcpyAsync(&hc[0], ctx->dimer_off.as<long long>() + n2, sizeof(long long), hipMemcpyDeviceToHost,
-                               ctx->stream));
+                               st));
     }
     if (want3) {
         CellGrid& g = ctx->grid3;
-        if (int e = bin_cells(ctx, g, n3, ctx->th_idx.as<int>())) return e;
-        HIP_TRY(hipMemsetAsync(ctx->lo_count.p, 0, sizeof(int) * (n3 + 1), ctx->stream));
-        launch_lower_pass(ctx->stream, n3, ctx->th_idx.as<int>(), pos, g.g, g.cell_of.as<int>(), g.cell_start.as<int>(),
+        if (int e = bin_cells(ctx, st, g, n3, ctx->th_idx.as<int>())) return e;
+        HIP_TRY(hipMemsetAsync(ctx->lo_count.p, 0, sizeof(int) * (n3 + 1), st));
+        launch_lower_pass(st, n3, ctx->th_idx.as<int>(), pos, g.g, g.cell_of.as<int>(), g.cell_start.as<int>(),
                           g.members.as<int>(), ctx->lo_count.as<int>(), nullptr, nullptr);
-        if (int e = scan_i32(ctx, ctx->lo_count.as<int>(), ctx->lo_start.as<int>(), n3 + 1)) return e;
-        HIP_TRY(hipMemcpyAsync(&hc[1], ctx->lo_start.as<int>() + n3, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+        if (int e = scan_i32(ctx, ctx->lo_count.as<int>(), ctx->lo_start.as<int>(), n3 + 1, st)) return e;
+        HIP_TRY(hipMemcpyAsync(&hc[1], ctx->lo_start.as<int>() + n3, sizeof(int), hipMemcpyDeviceToHost, st));
     }
-    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    HIP_TRY(hipStreamSynchronize(st));
     if (want2) {
         const long long nd = hc[0];
         if (nd > ctx->dimer_cap) {
@@ -482,7 +495,7 @@ static int build_lists(mbpol_context* ctx, bool want2, bool want3) {
             ctx->dimer_cap = nd + nd / 4 + 64;
         }
         CellGrid& g = ctx->grid2;
-        launch_dimer_pass(ctx->stream, n2, ctx->tb_idx.as<int>(), pos, g.g, g.cell_of.as<int>(), g.cell_start.as<int>(),
+        launch_dimer_pass(st, n2, ctx->tb_idx.as<int>(), pos, g.g, g.cell_of.as<int>(), g.cell_start.as<int>(),
                           g.members.as<int>(), nullptr, ctx->dimer_off.as<long long>(), ctx->dimers.as<int2>());
         ctx->stats.dimers = nd;
     }
@@ -493,21 +506,21 @@ static int build_lists(mbpol_context* ctx, bool want2, bool want3) {
             ctx->lower_cap = nl + nl / 4 + 64;
         }
         CellGrid& g = ctx->grid3;
-        launch_lower_pass(ctx->stream, n3, ctx->th_idx.as<int>(), pos, g.g, g.cell_of.as<int>(), g.cell_start.as<int>(),
+        launch_lower_pass(st, n3, ctx->th_idx.as<int>(), pos, g.g, g.cell_of.as<int>(), g.cell_start.as<int>(),
                           g.members.as<int>(), nullptr, ctx->lo_start.as<int>(), ctx->lower.as<int>());
-        HIP_TRY(hipMemsetAsync(ctx->trip_count.p, 0, sizeof(long long) * (n3 + 1), ctx->stream));
-        launch_trimer_pass(ctx->stream, n3, ctx->th_idx.as<int>(), pos, g.g, ctx->lo_start.as<int>(), ctx->lower.as<int>(),
+        HIP_TRY(hipMemsetAsync(ctx->trip_count.p, 0, sizeof(long long) * (n3 + 1), st));
+        launch_trimer_pass(st, n3, ctx->th_idx.as<int>(), pos, g.g, ctx->lo_start.as<int>(), ctx->lower.as<int>(),
                            ctx->full_chain_list, ctx->trip_count.as<long long>(), nullptr, nullptr);
-        if (int e = scan_i64(ctx, ctx->trip_count.as<long long>(), ctx->trip_off.as<long long>(), n3 + 1)) return e;
+        if (int e = scan_i64(ctx, ctx->trip_count.as<long long>(), ctx->trip_off.as<long long>(), n3 + 1, st)) return e;
         HIP_TRY(hipMemcpyAsync(&hc[2], ctx->trip_off.as<long long>() + n3, sizeof(long long), hipMemcpyDeviceToHost,
-                               ctx->stream));
-        HIP_TRY(hipStreamSynchronize(ctx->stream));
+                               st));
+        HIP_TRY(hipStreamSynchronize(st));
         const long long nt = hc[2];
         if (nt > ctx->trip_cap) {
             HIP_TRY(ctx->trips.ensure(sizeof(int) * 3 * (nt + nt / 4 + 64)));
             ctx->trip_cap = nt + nt / 4 + 64;
         }
-        launch_trimer_pass(ctx->stream, n3, ctx->th_idx.as<int>(), pos, g.g, ctx->lo_start.as<int>(), ctx->lower.as<int>(),
+        launch_trimer_pass(st, n3, ctx->th_idx.as<int>(), pos, g.g, ctx->lo_start.as<int>(), ctx->lower.as<int>(),
                            ctx->full_chain_list, nullptr, ctx->trip_off.as<long long>(), ctx->trips.as<int>());
         ctx->stats.trimers = nt;
     }
@@ -524,18 +537,28 @@ static int evaluate(mbpol_context* ctx, double* forces_dev_out, double* energies
     HIP_TRY(hipMemsetAsync(F, 0, sizeof(u64) * 3 * s.n, st));
     HIP_TRY(hipMemsetAsync(E, 0, sizeof(u64) * 8, st));
     launch_place_vsites(st, (int)s.vs_idx.size() / 4, ctx->vs_idx.as<int>(), ctx->vs_w.as<double>(), pos);
+    // The 1-, 2- and 3-body terms and the dispersion go to a second stream and
+    // overlap the electrostatics, which waits on its host-side convergence
+    // tests; with timing on, everything stays on one stream so that every
+    // span is attributable.
+    const bool fork = !ctx->timer.enabled;
+    hipStream_t sr = fork ? ctx->side : st;
+    if (fork) {
+        HIP_TRY(hipEventRecord(ctx->ev_fork, st));
+        HIP_TRY(hipStreamWaitEvent(sr, ctx->ev_fork, 0));
+    }
     const bool do1 = (comp & MBPOL_ONEBODY) && !s.ob_idx.empty();
     const bool do2 = (comp & MBPOL_TWOBODY) && !s.tb_idx.empty();
     const bool do3 = (comp & MBPOL_THREEBODY) && !s.th_idx.empty();
     if (do1) {
         timer_start(ctx, T_ONEBODY);
-        launch_onebody(st, (int)s.ob_idx.size() / 3, ctx->ob_idx.as<int>(), pos, s.ob_method == MBPOL_ONEBODY_PERIODIC,
+        launch_onebody(sr, (int)s.ob_idx.size() / 3, ctx->ob_idx.as<int>(), pos, s.ob_method == MBPOL_ONEBODY_PERIODIC,
                        box, F, E + 0);
         timer_stop(ctx, T_ONEBODY);
     }
     if (do2 || do3) {
         timer_start(ctx, T_LISTS);
-        if (int e = build_lists(ctx, do2, do3)) return e;
+        if (int e = build_lists(ctx, sr, do2, do3)) return e;
         timer_stop(ctx, T_LISTS);
     }
     // The polynomials run in FP64 unless FP32 was asked for explicitly: their
@@ -546,10 +569,10 @@ static int evaluate(mbpol_context* ctx, double* forces_dev_out, double* energies
         const int per = s.tb_method == MBPOL_CUTOFF_PERIODIC;
         const KEvents ev = timer_kernel(ctx, T_TWOBODY);
         if (dp)
-            launch_twobody_f64(st, ctx->stats.dimers, ctx->dimers.as<int2>(), ctx->tb_idx.as<int>(), pos, per, box, F, E + 1,
+            launch_twobody_f64(sr, ctx->stats.dimers, ctx->dimers.as<int2>(), ctx->tb_idx.as<int>(), pos, per, box, F, E + 1,
                                ev);
         else
-            launch_twobody_f32(st, ctx->stats.dimers, ctx->dimers.as<int2>(), ctx->tb_idx.as<int>(), pos, per, box, F, E + 1,
+            launch_twobody_f32(sr, ctx->stats.dimers, ctx->dimers.as<int2>(), ctx->tb_idx.as<int>(), pos, per, box, F, E + 1,
                                ev);
         timer_kernel_done(ctx, T_TWOBODY, ev);
     }
@@ -557,20 +580,22 @@ static int evaluate(mbpol_context* ctx, double* forces_dev_out, double* energies
         const int per = s.th_method == MBPOL_CUTOFF_PERIODIC;
         const KEvents ev = timer_kernel(ctx, T_THREEBODY);
         if (dp)
-            launch_threebody_f64(st, ctx->stats.trimers, ctx->trips.as<int>(), ctx->th_idx.as<int>(), pos, per, box, F,
+            launch_threebody_f64(sr, ctx->stats.trimers, ctx->trips.as<int>(), ctx->th_idx.as<int>(), pos, per, box, F,
                                  E + 2, ev);
         else
-            launch_threebody_f32(st, ctx->stats.trimers, ctx->trips.as<int>(), ctx->th_idx.as<int>(), pos, per, box, F,
+            launch_threebody_f32(sr, ctx->stats.trimers, ctx->trips.as<int>(), ctx->th_idx.as<int>(), pos, per, box, F,
                                  E + 2, ev);
         timer_kernel_done(ctx, T_THREEBODY, ev);
     }
     if ((comp & MBPOL_DISPERSION) && ctx->disp) {
         timer_start(ctx, T_DISPERSION);
-        if (int e = disp_evaluate(ctx, E + 3)) return e;
+        if (int e = disp_evaluate(ctx, sr, E + 3)) return e;
         timer_stop(ctx, T_DISPERSION);
     }
+    if (fork) HIP_TRY(hipEventRecord(ctx->ev_join, sr));
     if ((comp & MBPOL_ELECTROSTATICS) && ctx->elec)
         if (int e = elec_evaluate(ctx, E + 4)) return e;
+    if (fork) HIP_TRY(hipStreamWaitEvent(st, ctx->ev_join, 0));
     double* fout = forces_dev_out ? forces_dev_out : ctx->force.as<double>();
     launch_forces_to_double(st, 3 * s.n, F, fout);
     launch_fold_vsites(st, (int)s.vs_idx.size() / 4, ctx->vs_idx.as<int>(), ctx->vs_w.as<double>(), fout);
