@@ -7,6 +7,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -49,6 +50,9 @@ struct Elec {
     DevBuf efix, efix_pol, mu_a, mu_b, mu_prev, mu_user, partial, eps;
     // conjugate-gradient vectors (3 n), block partials and the r.z history
     DevBuf cg_r, cg_z, cg_p, cg_q, cg_partial, cg_rz;
+    // block-Jacobi preconditioner: polarisable sites grouped by molecule, and the blocks' inverses
+    DevBuf blk_start, blk_site, minv;
+    int nblk = 0;
     double* mu_cur = nullptr;  // dipoles of the last evaluate
     DevBuf phi_fix, phi_ind, fsite, V, eblk;
     // PME
@@ -66,7 +70,9 @@ struct Elec {
     bool lds_fft = false;
     hipfftHandle plan_f = 0, plan_b = 0;
     bool plans = false;
-    double* h_eps = nullptr;  // pinned: [0] epsilon, ints from [8] on
+    double* h_eps = nullptr;  // pinned, 128 bytes: [0] epsilon, ints from byte 32, a SolveCtl from byte 64
+    DevBuf ctl;               // SolveCtl of the running dipole solve
+    int last_iters = 0;       // field evaluations of the previous solve (sizes the first batch of steps)
 };
 
 // B-spline moduli with the reference's small-modulus fix-up and order-5 zeta
@@ -189,15 +195,34 @@ static void brick_axis(int K, int& s, int& nb) {
 }
 
 // polarisable-site index list and its host mirror
-static int set_polarisable(Elec* E, const std::vector<double>& pol) {
+// the polarisable sites, and their grouping by molecule into preconditioner
+// blocks of at most 4 sites (molecules in order of their first site)
+static int set_polarisable(Elec* E, const std::vector<double>& pol, const std::vector<int>& mol) {
     std::vector<int> idx;
     E->h_pol.assign(pol.size(), 0);
+    std::vector<int> order;                      // molecules by first polarisable site
+    std::map<int, std::vector<int>> by_mol;
     for (size_t i = 0; i < pol.size(); ++i)
         if (pol[i] != 0.0) {
             idx.push_back((int)i);
             E->h_pol[i] = 1;
+            auto& v = by_mol[mol[i]];
+            if (v.empty()) order.push_back(mol[i]);
+            v.push_back((int)i);
         }
     E->npol = (int)idx.size();
+    std::vector<int> start{0}, site;
+    for (int m : order) {
+        const std::vector<int>& v = by_mol[m];
+        for (size_t a = 0; a < v.size(); a += 4) {
+            for (size_t b = a; b < std::min(v.size(), a + 4); ++b) site.push_back(v[b]);
+            start.push_back((int)site.size());
+        }
+    }
+    E->nblk = (int)start.size() - 1;
+    if (int e = up(E->blk_start, start)) return e;
+    if (int e = up(E->blk_site, site)) return e;
+    HIP_TRY(E->minv.ensure(sizeof(double) * block_ld() * block_ld() * std::max(E->nblk, 1)));
     return up(E->polidx, idx);
 }
 
@@ -277,7 +302,7 @@ int elec_create(mbpol_context* ctx) {
     if (int e = up(E->pol, s.el_pol)) return e;
     if (int e = up(E->mol, s.el_mol)) return e;
     if (int e = up(E->type, s.el_type)) return e;
-    if (int e = set_polarisable(E, s.el_pol)) return e;
+    if (int e = set_polarisable(E, s.el_pol, s.el_mol)) return e;
     HIP_TRY(E->q.ensure(sizeof(double) * n));
     HIP_TRY(E->dq.ensure(sizeof(double) * 27 * std::max(E->nw, 1)));
     for (DevBuf* b : {&E->efix, &E->efix_pol, &E->mu_a, &E->mu_b, &E->mu_prev, &E->mu_user, &E->fsite})
@@ -302,7 +327,8 @@ int elec_create(mbpol_context* ctx) {
     HIP_TRY(E->cell_count.ensure(sizeof(int) * 2));
     HIP_TRY(E->cell_start.ensure(sizeof(int) * 2));
     HIP_TRY(E->cell_fill.ensure(sizeof(int) * 2));
-    HIP_TRY(hipHostMalloc((void**)&E->h_eps, 64, hipHostMallocDefault));
+    HIP_TRY(hipHostMalloc((void**)&E->h_eps, 128, hipHostMallocDefault));
+    HIP_TRY(E->ctl.ensure(sizeof(SolveCtl)));
     if (!P.pme) {  // direct sum: rows hold every other site
         E->capP = std::max(E->npol, 1);
         E->capQ = std::max(n - E->npol, 1);
@@ -440,7 +466,7 @@ static int build_rows(mbpol_context* ctx) {
 }
 
 // reciprocal-space potential on the grid of the charges (mode 0) or of the dipoles mu (mode 1)
-static int pme_grid(mbpol_context* ctx, int mode, const double* mu) {
+static int pme_grid(mbpol_context* ctx, int mode, const double* mu, const int* stop = nullptr) {
     Elec* E = ctx->elec;
     hipStream_t st = ctx->stream;
     const ElecParams& P = E->P;
@@ -451,25 +477,25 @@ static int pme_grid(mbpol_context* ctx, int mode, const double* mu) {
     } else {
         const KEvents ev = timer_kernel(ctx, T_PME_SPREAD);
         launch_spread_csr(st, (size_t)P.grid.x * P.grid.y * P.grid.z, E->rowstart.as<int>(), E->esite.as<int>(),
-                          E->ecoef.as<double>(), mu, E->grid.as<double>(), ev);
+                          E->ecoef.as<double>(), mu, E->grid.as<double>(), ev, stop);
         timer_kernel_done(ctx, T_PME_SPREAD, ev);
     }
     if (E->lds_fft) {
         double2* H = E->spec.as<double2>();
         KEvents ev = timer_kernel(ctx, T_FFT_Z_R2C);
-        launch_fft_z_r2c(st, E->fft, E->grid.as<double>(), H, ev);
+        launch_fft_z_r2c(st, E->fft, E->grid.as<double>(), H, ev, stop);
         timer_kernel_done(ctx, T_FFT_Z_R2C, ev);
         ev = timer_kernel(ctx, T_FFT_Y);
-        launch_fft_y(st, E->fft, H, 0, ev);
+        launch_fft_y(st, E->fft, H, 0, ev, stop);
         timer_kernel_done(ctx, T_FFT_Y, ev);
         ev = timer_kernel(ctx, T_FFT_X_CONV);
-        launch_fft_x_conv(st, E->fft, H, E->influence.as<double>(), ev);
+        launch_fft_x_conv(st, E->fft, H, E->influence.as<double>(), ev, stop);
         timer_kernel_done(ctx, T_FFT_X_CONV, ev);
         ev = timer_kernel(ctx, T_FFT_Y);
-        launch_fft_y(st, E->fft, H, 1, ev);
+        launch_fft_y(st, E->fft, H, 1, ev, stop);
         timer_kernel_done(ctx, T_FFT_Y, ev);
         ev = timer_kernel(ctx, T_FFT_Z_C2R);
-        launch_fft_z_c2r(st, E->fft, H, E->grid.as<double>(), ev);
+        launch_fft_z_c2r(st, E->fft, H, E->grid.as<double>(), ev, stop);
         timer_kernel_done(ctx, T_FFT_Z_C2R, ev);
     } else {
         FFT_TRY(hipfftExecD2Z(E->plan_f, E->grid.as<double>(), reinterpret_cast<hipfftDoubleComplex*>(E->spec.p)));
@@ -479,13 +505,75 @@ static int pme_grid(mbpol_context* ctx, int mode, const double* mu) {
     return MBPOL_OK;
 }
 
+// The dipole solves enqueue their steps in batches without waiting for the
+// convergence test: each step's last kernel applies the stopping rules on the
+// device (SolveCtl), and every kernel of a later step returns at once when the
+// solve has stopped.  The host reads the state once per batch.  The first batch
+// covers the previous solve's length less one step, later batches two steps;
+// with timing on, or with hipFFT transforms (which cannot skip), one step.
+static int solve_batch(const Elec* E, const mbpol_context* ctx, int done) {
+    if (ctx->timer.enabled || !E->lds_fft) return 1;
+    if (done == 0) return std::max(1, E->last_iters - 1);
+    return 2;
+}
+
+static int read_ctl(mbpol_context* ctx, SolveCtl& out) {
+    Elec* E = ctx->elec;
+    SolveCtl* h = reinterpret_cast<SolveCtl*>(reinterpret_cast<char*>(E->h_eps) + 64);
+    HIP_TRY(hipMemcpyAsync(h, E->ctl.p, sizeof(SolveCtl), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    out = *h;
+    return MBPOL_OK;
+}
+
+// SOR iteration (convergeInduceDipoles, :557-616) from the guess in buf[0];
+// step t reads buf[t % 2] and writes buf[(t + 1) % 2].  On return mu_in holds
+// the dipoles after the last step and mu_out those before it.
+static int solve_sor(mbpol_context* ctx, double* buf0, double* buf1, double self_term, double*& mu_in,
+                     double*& mu_out, SolveCtl& c) {
+    Elec* E = ctx->elec;
+    const mbpol_system& s = ctx->sys;
+    hipStream_t st = ctx->stream;
+    const ElecParams& P = E->P;
+    const ElecRows R = rows(E);
+    const double* pos = ctx->pos.as<double>();
+    SolveCtl* ctl = E->ctl.as<SolveCtl>();
+    const int* stop = &ctl->stop;
+    double* buf[2] = {buf0, buf1};
+    HIP_TRY(hipMemsetAsync(ctl, 0, sizeof(SolveCtl), st));
+    int t = 0;
+    c = SolveCtl{};
+    while (t <= s.el_max_iter) {
+        const int nb = std::min(solve_batch(E, ctx, t), s.el_max_iter + 1 - t);
+        for (int b = 0; b < nb; ++b, ++t) {
+            timer_start(ctx, T_ELEC_ITER);
+            if (P.pme)
+                if (int e = pme_grid(ctx, 1, buf[t % 2], stop)) return e;
+            const KEvents ev = timer_kernel(ctx, T_ELEC_ITERATE);
+            launch_iterate(st, E->npol, E->polidx.as<int>(), pos, P, R, buf[t % 2], buf[(t + 1) % 2],
+                           E->efix_pol.as<double>(), E->pol.as<double>(), E->grid.as<double>(), self_term,
+                           mbpol_params::el_polar_sor, E->partial.as<double>(), ctl, ev);
+            timer_kernel_done(ctx, T_ELEC_ITERATE, ev);
+            launch_sor_finish(st, iterate_blocks(E->npol), E->n, E->partial.as<double>(), t, s.el_eps, s.el_max_iter,
+                              ctl);
+            timer_stop(ctx, T_ELEC_ITER);
+        }
+        if (int e = read_ctl(ctx, c)) return e;
+        if (c.stop) break;
+    }
+    const int it = c.iterations;
+    mu_in = buf[it % 2];
+    mu_out = buf[(it + 1) % 2];
+    return MBPOL_OK;
+}
+
 // Preconditioned conjugate gradients on the symmetric form of the dipole
 // equations (kernels_elec.hip, k_cg_*), started from the guess in x.  Each
 // step costs one induced-field evaluation, as an SOR step does; the stopping
 // measure is the reference's epsilon of the current dipoles, tested after every
-// step against the same target.  `it` counts field evaluations (the initial
-// residual included).
-static int solve_pcg(mbpol_context* ctx, double* x, double self_term, int& it, double& eps, bool& conv) {
+// step against the same target.  Iterations count field evaluations (the
+// initial residual included).
+static int solve_pcg(mbpol_context* ctx, double* x, double self_term, SolveCtl& c) {
     Elec* E = ctx->elec;
     const mbpol_system& s = ctx->sys;
     hipStream_t st = ctx->stream;
@@ -493,60 +581,59 @@ static int solve_pcg(mbpol_context* ctx, double* x, double self_term, int& it, d
     const ElecRows R = rows(E);
     const double* pos = ctx->pos.as<double>();
     const int n = E->n, npol = E->npol;
+    c = SolveCtl{};
+    if (npol == 0) {  // nothing to polarise
+        c.converged = 1;
+        return MBPOL_OK;
+    }
     const size_t vbytes = sizeof(double) * 3 * std::max(n, 1);
     for (DevBuf* b : {&E->cg_r, &E->cg_z, &E->cg_p, &E->cg_q}) HIP_TRY(b->ensure(vbytes));
-    HIP_TRY(E->cg_partial.ensure(sizeof(double) * 2 * (cg_blocks(npol) + 1)));
+    HIP_TRY(E->cg_partial.ensure(sizeof(double) * 2 * (cg_blocks(E->nblk) + 1)));
     HIP_TRY(E->cg_rz.ensure(sizeof(double) * (s.el_max_iter + 2)));
     HIP_TRY(E->partial.ensure(sizeof(double) * 2 * (iterate_blocks(npol) + 1)));
     double *r = E->cg_r.as<double>(), *z = E->cg_z.as<double>(), *p = E->cg_p.as<double>(), *q = E->cg_q.as<double>();
     double* rz = E->cg_rz.as<double>();
     const double* pol = E->pol.as<double>();
     const int* polidx = E->polidx.as<int>();
-    auto field = [&](const double* v, int mode, double* out, double* zout) -> int {
+    SolveCtl* ctl = E->ctl.as<SolveCtl>();
+    const int* stop = &ctl->stop;
+    const PrecondBlocks B{E->nblk, E->blk_start.as<int>(), E->blk_site.as<int>()};
+    double* minv = E->minv.as<double>();
+    auto field = [&](const double* v, int mode, double* out) -> int {
         if (P.pme)
-            if (int e = pme_grid(ctx, 1, v)) return e;
+            if (int e = pme_grid(ctx, 1, v, stop)) return e;
         const KEvents ev = timer_kernel(ctx, T_ELEC_ITERATE);
         launch_cg_matvec(st, npol, polidx, pos, P, R, v, E->grid.as<double>(), self_term, pol, E->efix.as<double>(),
-                         mode, out, zout, E->partial.as<double>(), ev);
+                         mode, out, E->partial.as<double>(), ctl, ev);
         timer_kernel_done(ctx, T_ELEC_ITERATE, ev);
         return MBPOL_OK;
     };
-    auto read_eps = [&]() -> int {
-        HIP_TRY(hipMemcpyAsync(E->h_eps, E->eps.p, sizeof(double), hipMemcpyDeviceToHost, st));
-        HIP_TRY(hipStreamSynchronize(st));
-        eps = E->h_eps[0];
-        return MBPOL_OK;
-    };
-    conv = false;
-    it = 0;
-    if (npol == 0) {  // nothing to polarise
-        eps = 0.0;
-        conv = true;
-        return MBPOL_OK;
-    }
-    timer_start(ctx, T_ELEC_ITER);
-    if (int e = field(x, 0, r, z)) return e;
-    // r.z and the measure of the guess go through the same reduction as later steps
-    launch_cg_direction(st, npol, polidx, iterate_blocks(npol), E->partial.as<double>(), rz, 0, z, p, n,
-                        E->eps.as<double>());
-    timer_stop(ctx, T_ELEC_ITER);
-    ++it;
-    if (int e = read_eps()) return e;
-    for (int k = 0;; ++k) {
-        if (eps < s.el_eps) {
-            conv = true;
-            break;
+    launch_block_factor(st, B, pos, site_params(E), P, minv);
+    HIP_TRY(hipMemsetAsync(ctl, 0, sizeof(SolveCtl), st));
+    // field evaluation f (0-based): f = 0 forms the initial residual and direction 0,
+    // f = k + 1 takes step k and forms direction k + 1
+    int f = 0;
+    while (f <= s.el_max_iter) {
+        const int nb = std::min(solve_batch(E, ctx, f), s.el_max_iter + 1 - f);
+        for (int b = 0; b < nb; ++b, ++f) {
+            timer_start(ctx, T_ELEC_ITER);
+            if (f == 0) {
+                if (int e = field(x, 0, r)) return e;
+                launch_cg_precond(st, B, minv, pol, r, z, E->cg_partial.as<double>(), ctl);
+                launch_cg_direction(st, npol, polidx, cg_blocks(E->nblk), E->cg_partial.as<double>(), rz, 0, z, p, n,
+                                    s.el_eps, s.el_max_iter, ctl);
+            } else {
+                const int k = f - 1;
+                if (int e = field(p, 1, q)) return e;
+                launch_cg_update(st, B, minv, pol, npol, E->partial.as<double>(), rz, k, x, r, p, q, z,
+                                 E->cg_partial.as<double>(), ctl);
+                launch_cg_direction(st, npol, polidx, cg_blocks(E->nblk), E->cg_partial.as<double>(), rz, k + 1, z, p,
+                                    n, s.el_eps, s.el_max_iter, ctl);
+            }
+            timer_stop(ctx, T_ELEC_ITER);
         }
-        if (it > s.el_max_iter) break;
-        timer_start(ctx, T_ELEC_ITER);
-        if (int e = field(p, 1, q, nullptr)) return e;
-        launch_cg_update(st, npol, polidx, pol, self_term, E->partial.as<double>(), rz, k, x, r, p, q, z,
-                         E->cg_partial.as<double>());
-        launch_cg_direction(st, npol, polidx, cg_blocks(npol), E->cg_partial.as<double>(), rz, k + 1, z, p, n,
-                            E->eps.as<double>());
-        timer_stop(ctx, T_ELEC_ITER);
-        ++it;
-        if (int e = read_eps()) return e;
+        if (int e = read_ctl(ctx, c)) return e;
+        if (c.stop) break;
     }
     return MBPOL_OK;
 }
@@ -598,43 +685,19 @@ int elec_evaluate(mbpol_context* ctx, u64* energy) {
     }
     timer_stop(ctx, T_ELEC_SETUP);
     const double self_term = P.pme ? (4.0 / 3.0) * (P.alpha * P.alpha * P.alpha) / mbpol_params::el_sqrt_pi : 0.0;
-    double cur = 1.0e+50;
-    int it = 0;
-    bool conv = false;
+    SolveCtl c{};
     if (E->solver == MBPOL_SOLVER_PCG) {
         // conjugate gradients from the guess in mu_in; the solution stays in mu_in, and
         // the reciprocal induced potential of the energy is taken from it (mu_out = mu_in)
-        if (int e = solve_pcg(ctx, mu_in, self_term, it, cur, conv)) return e;
+        if (int e = solve_pcg(ctx, mu_in, self_term, c)) return e;
         mu_out = mu_in;
     } else {
-        // SOR iteration (convergeInduceDipoles, :557-616)
-        for (;;) {
-            timer_start(ctx, T_ELEC_ITER);
-            if (P.pme)
-                if (int e = pme_grid(ctx, 1, mu_in)) return e;
-            const KEvents ev = timer_kernel(ctx, T_ELEC_ITERATE);
-            launch_iterate(st, E->npol, E->polidx.as<int>(), pos, P, R, mu_in, mu_out, E->efix_pol.as<double>(),
-                           E->pol.as<double>(), E->grid.as<double>(), self_term, mbpol_params::el_polar_sor,
-                           E->partial.as<double>(), ev);
-            timer_kernel_done(ctx, T_ELEC_ITERATE, ev);
-            launch_eps_finish(st, iterate_blocks(E->npol), n, E->partial.as<double>(), E->eps.as<double>());
-            timer_stop(ctx, T_ELEC_ITER);
-            HIP_TRY(hipMemcpyAsync(E->h_eps, E->eps.p, sizeof(double), hipMemcpyDeviceToHost, st));
-            HIP_TRY(hipStreamSynchronize(st));
-            std::swap(mu_in, mu_out);  // mu_in: dipoles after the step, mu_out: before it
-            const double eps = E->h_eps[0];
-            bool done = false;
-            if (eps < s.el_eps) {
-                conv = true;
-                done = true;
-            } else if (cur < eps || it >= s.el_max_iter) {
-                done = true;
-            }
-            cur = eps;
-            ++it;
-            if (done) break;
-        }
+        if (int e = solve_sor(ctx, E->mu_a.as<double>(), E->mu_b.as<double>(), self_term, mu_in, mu_out, c)) return e;
     }
+    const int it = c.iterations;
+    const double cur = c.eps;
+    const bool conv = c.converged != 0;
+    E->last_iters = it;
     ctx->stats.iterations = it;
     ctx->stats.epsilon = cur;
     ctx->stats.converged = conv ? 1 : 0;
@@ -750,7 +813,7 @@ int mbpol_context_update_electrostatics(mbpol_context* ctx, int32_t n, const dou
     if (int e = up(E->pol, s.el_pol)) return e;
     if (int e = up(E->mol, s.el_mol)) return e;
     if (int e = up(E->type, s.el_type)) return e;
-    if (int e = set_polarisable(E, s.el_pol)) return e;
+    if (int e = set_polarisable(E, s.el_pol, s.el_mol)) return e;
     HIP_TRY(E->partial.ensure(sizeof(double) * (iterate_blocks(E->npol) + 1)));
     // the polarisable set may have changed: clear the dipoles the iteration no longer visits
     HIP_TRY(hipMemset(E->mu_a.p, 0, sizeof(double) * 3 * std::max(n, 1)));

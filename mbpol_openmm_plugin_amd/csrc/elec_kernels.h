@@ -33,6 +33,16 @@ struct ElecRows {
     const double2* pf;
 };
 
+// state of one dipole solve, written on the device by its convergence test;
+// every later kernel of the solve returns at once when stop is set
+struct SolveCtl {
+    int stop;
+    int converged;
+    int iterations;  // field evaluations so far
+    int pad;
+    double eps;      // the reference's measure at the last test
+};
+
 // PME grid bricks: s[d] points per brick along axis d (every brick, the last
 // one included, at least 4 points wide), nb[d] bricks
 struct BrickGeom {
@@ -65,7 +75,7 @@ void launch_csr_fill(hipStream_t s, int npol, const int* polidx, const int* g0ta
                      const int* rowstart, int* fill, int* esite, double* ecoef);
 void launch_csr_sort(hipStream_t s, size_t ngrid, const int* rowstart, int* esite, double* ecoef);
 void launch_spread_csr(hipStream_t s, size_t ngrid, const int* rowstart, const int* esite, const double* ecoef,
-                       const double* mu, double* grid, const KEvents& ev);
+                       const double* mu, double* grid, const KEvents& ev, const int* stop = nullptr);
 void launch_convolve(hipStream_t s, size_t nhalf, const double* influence, double2* spec);
 // influence function on the half grid, rows of khp entries (entries kz >= K3/2 + 1 are zero)
 void launch_influence(hipStream_t s, ElecParams P, int khp, const double* bx, const double* by, const double* bz,
@@ -76,21 +86,35 @@ void launch_record_field(hipStream_t s, int n, ElecParams P, const double* phi, 
 int iterate_blocks(int npol);
 void launch_iterate(hipStream_t s, int npol, const int* polidx, const double* pos, ElecParams P, const ElecRows& R,
                     const double* mu_in, double* mu_out, const double* efix_pol, const double* pol, const double* grid,
-                    double self_term, double omega, double* partial, const KEvents& ev);
+                    double self_term, double omega, double* partial, const SolveCtl* ctl, const KEvents& ev);
 // preconditioned conjugate gradients on A mu = E_fix, A = diag(1/alpha) - T (see kernels_elec.hip)
-int cg_blocks(int npol);
-// mode 0: out = E_fix - A v, zout = M^-1 out; mode 1: out = A v.  partial: 2 per iterate_blocks(npol)
+// preconditioner blocks: the polarisable sites of one molecule (at most 4 per block)
+struct PrecondBlocks {
+    int n;             // blocks
+    const int* start;  // n + 1 offsets into site
+    const int* site;   // site indices, block by block
+};
+int cg_blocks(int nitems);
+int block_ld();  // leading dimension of a block's inverse (block_ld()^2 doubles per block)
+void launch_block_factor(hipStream_t s, const PrecondBlocks& B, const double* pos, SiteParams S, ElecParams P,
+                         double* minv);
+// mode 0: out = E_fix - A v; mode 1: out = A v with partials (v.out), 2 per iterate_blocks(npol)
 void launch_cg_matvec(hipStream_t s, int npol, const int* polidx, const double* pos, ElecParams P, const ElecRows& R,
                       const double* v, const double* grid, double self_term, const double* pol, const double* efix,
-                      int mode, double* out, double* zout, double* partial, const KEvents& ev);
-// step k (uses rz[k]); partial: 2 per cg_blocks(npol)
-void launch_cg_update(hipStream_t s, int npol, const int* polidx, const double* pol, double self_term,
+                      int mode, double* out, double* partial, const SolveCtl* ctl, const KEvents& ev);
+// z = M^-1 r with partials (r.z, |alpha r|^2), 2 per cg_blocks(B.n)
+void launch_cg_precond(hipStream_t s, const PrecondBlocks& B, const double* minv, const double* pol, const double* r,
+                       double* z, double* partial, const SolveCtl* ctl);
+// step k (uses rz[k] and the matvec partials); partials as launch_cg_precond
+void launch_cg_update(hipStream_t s, const PrecondBlocks& B, const double* minv, const double* pol, int npol,
                       const double* partial_mv, const double* rz, int k, double* x, double* r, const double* p,
-                      const double* q, double* z, double* partial);
+                      const double* q, double* z, double* partial, const SolveCtl* ctl);
 // direction k (writes rz[k], reads rz[k-1]) and the convergence measure from nb block partial pairs
 void launch_cg_direction(hipStream_t s, int npol, const int* polidx, int nb, const double* partial, double* rz, int k,
-                         const double* z, double* p, int n_all, double* eps);
-void launch_eps_finish(hipStream_t s, int nblocks, int n_all, const double* partial, double* eps);
+                         const double* z, double* p, int n_all, double tol, int max_iter, SolveCtl* ctl);
+// SOR step t: the measure and the stopping rules into ctl
+void launch_sor_finish(hipStream_t s, int nblocks, int n_all, const double* partial, int t, double tol, int max_iter,
+                       SolveCtl* ctl);
 void launch_scale_copy(hipStream_t s, int n, const double* a, const double* pol, double* out, double* out2);
 int energy_blocks(int n);
 int sor_blocks(int n);

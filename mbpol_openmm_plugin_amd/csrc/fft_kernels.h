@@ -28,9 +28,14 @@ struct PmeFft {
 constexpr int kFftMaxLength = 1024;
 // real grid -> half spectrum H -> (x pass) times influence -> real grid, as five passes:
 // z_r2c, y (forward), x_conv, y (inverse), z_c2r
-void launch_fft_z_r2c(hipStream_t s, const PmeFft& F, const double* real, double2* H, const KEvents& ev);
-void launch_fft_y(hipStream_t s, const PmeFft& F, double2* H, int inverse, const KEvents& ev);
-void launch_fft_x_conv(hipStream_t s, const PmeFft& F, double2* H, const double* influence, const KEvents& ev);
-void launch_fft_z_c2r(hipStream_t s, const PmeFft& F, const double2* H, double* real, const KEvents& ev);
+// stop: when non-null and *stop != 0 at launch, the pass does nothing (a dipole solve that has converged)
+void launch_fft_z_r2c(hipStream_t s, const PmeFft& F, const double* real, double2* H, const KEvents& ev,
+                      const int* stop = nullptr);
+void launch_fft_y(hipStream_t s, const PmeFft& F, double2* H, int inverse, const KEvents& ev,
+                  const int* stop = nullptr);
+void launch_fft_x_conv(hipStream_t s, const PmeFft& F, double2* H, const double* influence, const KEvents& ev,
+                       const int* stop = nullptr);
+void launch_fft_z_c2r(hipStream_t s, const PmeFft& F, const double2* H, double* real, const KEvents& ev,
+                      const int* stop = nullptr);
 
 }  // namespace mbpol

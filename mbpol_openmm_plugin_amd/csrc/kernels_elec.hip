@@ -729,7 +729,9 @@ __global__ void k_csr_sort(size_t ngrid, const int* __restrict__ rowstart, int* 
 
 __global__ void __launch_bounds__(256) k_spread_csr(size_t ngrid, const int* __restrict__ rowstart,
                                                     const int* __restrict__ esite, const double* __restrict__ ecoef,
-                                                    const double* __restrict__ mu, double* __restrict__ grid) {
+                                                    const double* __restrict__ mu, double* __restrict__ grid,
+                                                    const int* __restrict__ stop) {
+    if (stop && *stop) return;
     const size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= ngrid) return;
     double v = 0.0;
@@ -927,7 +929,8 @@ __global__ void __launch_bounds__(256) k_iterate(int npol, const int* __restrict
                                                  const double* __restrict__ mu_in, double* __restrict__ mu_out,
                                                  const double* __restrict__ efix_pol, const double* __restrict__ pol,
                                                  const double* __restrict__ grid, double self_term, double omega,
-                                                 double* __restrict__ partial) {
+                                                 double* __restrict__ partial, const SolveCtl* __restrict__ ctl) {
+    if (ctl->stop) return;
     const int g = (blockIdx.x * blockDim.x + threadIdx.x) / G, lane = threadIdx.x % G;
     double e2 = 0.0;
     if (g < npol) {
@@ -958,10 +961,7 @@ __global__ void __launch_bounds__(256) k_iterate(int npol, const int* __restrict
 // current dipoles, omega * debye * rms(alpha (E_fix + T mu) - mu) = omega *
 // debye * rms(alpha r), so it stops by the same rule as the SOR iteration.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ double cg_minv(double a, double self_term) { return a / (1.0 - a * self_term); }
-
-// mode 0: r = E_fix - A v (the initial residual), z = M^-1 r, partials (r.z, |alpha r|^2)
-// mode 1: q = A v, partials (v.q, 0)
+// mode 0: r = E_fix - A v (the initial residual); mode 1: q = A v, partials (v.q)
 template <int G>
 __global__ void __launch_bounds__(256) k_cg_matvec(int npol, const int* __restrict__ polidx, const double* __restrict__ pos,
                                                    ElecParams P, int capP, const int* __restrict__ nbP,
@@ -969,7 +969,8 @@ __global__ void __launch_bounds__(256) k_cg_matvec(int npol, const int* __restri
                                                    const double* __restrict__ v, const double* __restrict__ grid,
                                                    double self_term, const double* __restrict__ pol,
                                                    const double* __restrict__ efix, int mode, double* __restrict__ out,
-                                                   double* __restrict__ zout, double* __restrict__ partial) {
+                                                   double* __restrict__ partial, const SolveCtl* __restrict__ ctl) {
+    if (ctl->stop) return;
     const int g = (blockIdx.x * blockDim.x + threadIdx.x) / G, lane = threadIdx.x % G;
     double s0 = 0.0, s1 = 0.0;
     if (g < npol) {
@@ -977,18 +978,14 @@ __global__ void __launch_bounds__(256) k_cg_matvec(int npol, const int* __restri
         double ex, ey, ez;
         site_field<G>(i, lane, pos, P, capP, nbP, cntP, pf, v, grid, ex, ey, ez);
         if (lane == 0) {
-            const double a = pol[i], ia = 1.0 / a, mi = cg_minv(a, self_term);
+            const double ia = 1.0 / pol[i];
             const double f[3] = {ex, ey, ez};
 #pragma unroll
             for (int d = 0; d < 3; ++d) {
                 const double vd = v[3 * i + d];
                 const double av = vd * ia - (f[d] + vd * self_term);  // (A v)_d
                 if (mode == 0) {
-                    const double r = efix[3 * i + d] - av;
-                    out[3 * i + d] = r;
-                    zout[3 * i + d] = r * mi;
-                    s0 += r * r * mi;
-                    s1 += (a * r) * (a * r);
+                    out[3 * i + d] = efix[3 * i + d] - av;
                 } else {
                     out[3 * i + d] = av;
                     s0 += vd * av;
@@ -998,6 +995,110 @@ __global__ void __launch_bounds__(256) k_cg_matvec(int npol, const int* __restri
     }
     block_partial<G>(s0, lane, partial + 2 * blockIdx.x);
     block_partial<G>(s1, lane, partial + 2 * blockIdx.x + 1);
+}
+
+// Block-Jacobi preconditioner: the polarisable sites of one molecule (at most
+// kBlockSites, larger molecules split into several blocks) form a block of
+// A restricted to the molecule, taken as the isolated molecule's polarisation
+// matrix diag(1/alpha) - T_Thole (the Thole-damped dipole tensor of the
+// molecule's own pairs, without the Ewald split: in PME the real-space and
+// reciprocal parts of an intramolecular pair add up to it, and the self term
+// cancels the reciprocal self-interaction).  One thread inverts one block in
+// place (Gauss-Jordan; the block is symmetric positive definite).
+constexpr int kBlockSites = 4, kBlockLd = 3 * kBlockSites;
+
+__global__ void __launch_bounds__(256) k_block_factor(int nblocks, const int* __restrict__ blk_start,
+                                                      const int* __restrict__ blk_site, const double* __restrict__ pos,
+                                                      const SiteParams S, ElecParams P, double* __restrict__ minv) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nblocks) return;
+    const int s0 = blk_start[b], m = blk_start[b + 1] - s0, d = 3 * m;
+    double* M = minv + (size_t)b * kBlockLd * kBlockLd;
+    for (int u = 0; u < m; ++u) {
+        const int i = blk_site[s0 + u];
+        for (int w = 0; w < m; ++w) {
+            const int j = blk_site[s0 + w];
+            double t[3][3];
+            if (u == w) {
+                for (int x = 0; x < 3; ++x)
+                    for (int y = 0; y < 3; ++y) t[x][y] = x == y ? 1.0 / S.pol[i] : 0.0;
+            } else {
+                double dx = pos[3 * j] - pos[3 * i], dy = pos[3 * j + 1] - pos[3 * i + 1],
+                       dz = pos[3 * j + 2] - pos[3 * i + 2];
+                min_image(dx, dy, dz, P);
+                const double r2 = dx * dx + dy * dy + dz * dz, r = sqrt(r2);
+                const Thole th(S.damp6[i], S.damp6[j], r);
+                const double g = tdd_gamma(P, 1, S.type[i], S.type[j]);
+                const double e = th.ex(g);
+                const double p1 = th.s3(e) / (r2 * r), p2 = -3.0 * th.s5(g, e) / (r2 * r2 * r);
+                const double dv[3] = {dx, dy, dz};
+                for (int x = 0; x < 3; ++x)
+                    for (int y = 0; y < 3; ++y) t[x][y] = (x == y ? p1 : 0.0) + p2 * dv[x] * dv[y];
+            }
+            for (int x = 0; x < 3; ++x)
+                for (int y = 0; y < 3; ++y) M[(3 * u + x) * kBlockLd + 3 * w + y] = t[x][y];
+        }
+    }
+    for (int k = 0; k < d; ++k) {
+        const double piv = 1.0 / M[k * kBlockLd + k];
+        M[k * kBlockLd + k] = 1.0;
+        for (int j = 0; j < d; ++j) M[k * kBlockLd + j] *= piv;
+        for (int i = 0; i < d; ++i) {
+            if (i == k) continue;
+            const double f = M[i * kBlockLd + k];
+            M[i * kBlockLd + k] = 0.0;
+            for (int j = 0; j < d; ++j) M[i * kBlockLd + j] -= f * M[k * kBlockLd + j];
+        }
+    }
+}
+
+// z = M^-1 r on one block, and its contributions to r.z and |alpha r|^2
+__device__ __forceinline__ void block_apply(int b, const int* __restrict__ blk_start, const int* __restrict__ blk_site,
+                                            const double* __restrict__ minv, const double* __restrict__ pol,
+                                            const double* __restrict__ r, double* __restrict__ z, double& s0,
+                                            double& s1) {
+    const int st = blk_start[b], m = blk_start[b + 1] - st, d = 3 * m;
+    double rv[kBlockLd];
+#pragma unroll
+    for (int u = 0; u < kBlockSites; ++u)
+        if (u < m) {
+            const int i = blk_site[st + u];
+            const double a = pol[i];
+#pragma unroll
+            for (int x = 0; x < 3; ++x) {
+                rv[3 * u + x] = r[3 * i + x];
+                s1 += (a * rv[3 * u + x]) * (a * rv[3 * u + x]);
+            }
+        }
+    const double* M = minv + (size_t)b * kBlockLd * kBlockLd;
+#pragma unroll
+    for (int u = 0; u < kBlockSites; ++u)
+        if (u < m) {
+            const int i = blk_site[st + u];
+#pragma unroll
+            for (int x = 0; x < 3; ++x) {
+                double acc = 0.0;
+#pragma unroll
+                for (int c = 0; c < kBlockLd; ++c)
+                    if (c < d) acc += M[(3 * u + x) * kBlockLd + c] * rv[c];
+                z[3 * i + x] = acc;
+                s0 += rv[3 * u + x] * acc;
+            }
+        }
+}
+
+// the initial residual's preconditioned vector and partials (r.z, |alpha r|^2)
+__global__ void __launch_bounds__(256) k_cg_precond(int nblocks, const int* __restrict__ blk_start,
+                                                    const int* __restrict__ blk_site, const double* __restrict__ minv,
+                                                    const double* __restrict__ pol, const double* __restrict__ r,
+                                                    double* __restrict__ z, double* __restrict__ partial,
+                                                    const SolveCtl* __restrict__ ctl) {
+    if (ctl->stop) return;
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    double s0 = 0.0, s1 = 0.0;
+    if (b < nblocks) block_apply(b, blk_start, blk_site, minv, pol, r, z, s0, s1);
+    block_partial<1>(s0, 0, partial + 2 * blockIdx.x);
+    block_partial<1>(s1, 0, partial + 2 * blockIdx.x + 1);
 }
 
 // sum of the entries 2b + c (b < nb) of partial, in an order fixed by nb
@@ -1017,42 +1118,48 @@ __device__ __forceinline__ double reduce_partials(int nb, int c, const double* _
     return r;
 }
 
-// step k: a = rz[k] / (p.q); x += a p; r -= a q; z = M^-1 r; partials (r.z, |alpha r|^2)
-__global__ void __launch_bounds__(256) k_cg_update(int npol, const int* __restrict__ polidx,
-                                                   const double* __restrict__ pol, double self_term, int nb_mv,
+// step k: a = rz[k] / (p.q); x += a p; r -= a q; z = M^-1 r; partials (r.z, |alpha r|^2);
+// one thread per preconditioner block
+__global__ void __launch_bounds__(256) k_cg_update(int nblocks, const int* __restrict__ blk_start,
+                                                   const int* __restrict__ blk_site, const double* __restrict__ minv,
+                                                   const double* __restrict__ pol, int nb_mv,
                                                    const double* __restrict__ partial_mv,
                                                    const double* __restrict__ rz, int k, double* __restrict__ x,
                                                    double* __restrict__ r, const double* __restrict__ p,
                                                    const double* __restrict__ q, double* __restrict__ z,
-                                                   double* __restrict__ partial) {
+                                                   double* __restrict__ partial, const SolveCtl* __restrict__ ctl) {
+    if (ctl->stop) return;
     const double pq = reduce_partials(nb_mv, 0, partial_mv);
     const double step = rz[k] / pq;
-    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
     double s0 = 0.0, s1 = 0.0;
-    if (g < npol) {
-        const int i = polidx[g];
-        const double a = pol[i], mi = cg_minv(a, self_term);
+    if (b < nblocks) {
+        const int st = blk_start[b], en = blk_start[b + 1];
+        for (int e = st; e < en; ++e) {
+            const int i = blk_site[e];
 #pragma unroll
-        for (int d = 0; d < 3; ++d) {
-            const int t = 3 * i + d;
-            x[t] += step * p[t];
-            const double rv = r[t] - step * q[t];
-            r[t] = rv;
-            z[t] = rv * mi;
-            s0 += rv * rv * mi;
-            s1 += (a * rv) * (a * rv);
+            for (int d = 0; d < 3; ++d) {
+                const int t = 3 * i + d;
+                x[t] += step * p[t];
+                r[t] -= step * q[t];
+            }
         }
+        block_apply(b, blk_start, blk_site, minv, pol, r, z, s0, s1);
     }
     block_partial<1>(s0, 0, partial + 2 * blockIdx.x);
     block_partial<1>(s1, 0, partial + 2 * blockIdx.x + 1);
 }
 
 // k-th direction: rz[k] = r.z; p = z (k == 0) or z + (rz[k] / rz[k-1]) p;
-// eps_out = omega * debye * rms(alpha r) over all n sites
+// the measure omega * debye * rms(alpha r) over all n sites of the current
+// dipoles (k + 1 field evaluations so far) and the stopping rules: converged
+// below tol, not converged after max_iter + 1 field evaluations
 __global__ void __launch_bounds__(256) k_cg_direction(int npol, const int* __restrict__ polidx, int nb,
                                                       const double* __restrict__ partial, double* __restrict__ rz,
                                                       int k, const double* __restrict__ z, double* __restrict__ p,
-                                                      int n_all, double* __restrict__ eps_out) {
+                                                      int n_all, double tol, int max_iter,
+                                                      SolveCtl* __restrict__ ctl) {
+    if (ctl->stop) return;
     const double s_rz = reduce_partials(nb, 0, partial);
     const double s_ee = reduce_partials(nb, 1, partial);
     const double beta = k == 0 ? 0.0 : s_rz / rz[k - 1];
@@ -1064,12 +1171,25 @@ __global__ void __launch_bounds__(256) k_cg_direction(int npol, const int* __res
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         rz[k] = s_rz;
-        *eps_out = el_polar_sor * el_debye * sqrt(s_ee / (double)n_all);
+        const double eps = el_polar_sor * el_debye * sqrt(s_ee / (double)n_all);
+        ctl->eps = eps;
+        ctl->iterations = k + 1;
+        if (eps < tol) {
+            ctl->converged = 1;
+            ctl->stop = 1;
+        } else if (k + 1 > max_iter) {
+            ctl->stop = 1;
+        }
     }
 }
 
-// epsilon = omega * debye * sqrt(sum / n) from the block partials, in block order (:504-511, :588)
-__global__ void k_eps_finish(int nblocks, int n, const double* __restrict__ partial, double* __restrict__ eps_out) {
+// SOR step t: epsilon = omega * debye * sqrt(sum / n) from the block partials,
+// in block order (:504-511, :588), and the reference's stopping rules
+// (:596-612): converged below tol; stop without convergence when epsilon grew
+// or after max_iter + 1 steps
+__global__ void k_sor_finish(int nblocks, int n, const double* __restrict__ partial, int t, double tol, int max_iter,
+                             SolveCtl* __restrict__ ctl) {
+    if (ctl->stop) return;
     __shared__ double red[256];
     double s = 0.0;
     for (int b = threadIdx.x; b < nblocks; b += blockDim.x) s += partial[b];
@@ -1079,7 +1199,18 @@ __global__ void k_eps_finish(int nblocks, int n, const double* __restrict__ part
         if (threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
         __syncthreads();
     }
-    if (threadIdx.x == 0) *eps_out = el_polar_sor * el_debye * sqrt(red[0] / (double)n);
+    if (threadIdx.x == 0) {
+        const double eps = el_polar_sor * el_debye * sqrt(red[0] / (double)n);
+        const double cur = t == 0 ? 1.0e+50 : ctl->eps;
+        ctl->eps = eps;
+        ctl->iterations = t + 1;
+        if (eps < tol) {
+            ctl->converged = 1;
+            ctl->stop = 1;
+        } else if (cur < eps || t >= max_iter) {
+            ctl->stop = 1;
+        }
+    }
 }
 
 __global__ void k_scale_copy(int n3, const double* __restrict__ a, const double* __restrict__ pol, double* __restrict__ out,
@@ -1359,8 +1490,8 @@ void launch_csr_sort(hipStream_t s, size_t ngrid, const int* rowstart, int* esit
     hipLaunchKernelGGL(k_csr_sort, nblk(ngrid, 256), 256, 0, s, ngrid, rowstart, esite, ecoef);
 }
 void launch_spread_csr(hipStream_t s, size_t ngrid, const int* rowstart, const int* esite, const double* ecoef,
-                       const double* mu, double* grid, const KEvents& ev) {
-    MBPOL_LAUNCH(ev, k_spread_csr, nblk(ngrid, 256), 256, 0, s, ngrid, rowstart, esite, ecoef, mu, grid);
+                       const double* mu, double* grid, const KEvents& ev, const int* stop) {
+    MBPOL_LAUNCH(ev, k_spread_csr, nblk(ngrid, 256), 256, 0, s, ngrid, rowstart, esite, ecoef, mu, grid, stop);
 }
 void launch_convolve(hipStream_t s, size_t nhalf, const double* influence, double2* spec) {
     hipLaunchKernelGGL(k_convolve, nblk(nhalf, 256), 256, 0, s, nhalf, influence, spec);
@@ -1380,33 +1511,46 @@ void launch_record_field(hipStream_t s, int n, ElecParams P, const double* phi, 
 int iterate_blocks(int npol) { return std::max(1, nblk((long long)npol * kGI, 256)); }
 void launch_iterate(hipStream_t s, int npol, const int* polidx, const double* pos, ElecParams P, const ElecRows& R,
                     const double* mu_in, double* mu_out, const double* efix_pol, const double* pol, const double* grid,
-                    double self_term, double omega, double* partial, const KEvents& ev) {
+                    double self_term, double omega, double* partial, const SolveCtl* ctl, const KEvents& ev) {
     MBPOL_LAUNCH(ev, k_iterate<kGI>, iterate_blocks(npol), 256, 0, s, npol, polidx, pos, P, R.capP, R.nbP, R.cntP, R.pf,
-                 mu_in, mu_out, efix_pol, pol, grid, self_term, omega, partial);
+                 mu_in, mu_out, efix_pol, pol, grid, self_term, omega, partial, ctl);
 }
-int cg_blocks(int npol) { return std::max(1, nblk(npol, 256)); }
+int cg_blocks(int nitems) { return std::max(1, nblk(nitems, 256)); }
+int block_ld() { return kBlockLd; }
+void launch_block_factor(hipStream_t s, const PrecondBlocks& B, const double* pos, SiteParams S, ElecParams P,
+                         double* minv) {
+    if (B.n > 0)
+        hipLaunchKernelGGL(k_block_factor, cg_blocks(B.n), 256, 0, s, B.n, B.start, B.site, pos, S, P, minv);
+}
 void launch_cg_matvec(hipStream_t s, int npol, const int* polidx, const double* pos, ElecParams P, const ElecRows& R,
                       const double* v, const double* grid, double self_term, const double* pol, const double* efix,
-                      int mode, double* out, double* zout, double* partial, const KEvents& ev) {
+                      int mode, double* out, double* partial, const SolveCtl* ctl, const KEvents& ev) {
     if (npol > 0)
         MBPOL_LAUNCH(ev, k_cg_matvec<kGI>, iterate_blocks(npol), 256, 0, s, npol, polidx, pos, P, R.capP, R.nbP, R.cntP,
-                     R.pf, v, grid, self_term, pol, efix, mode, out, zout, partial);
+                     R.pf, v, grid, self_term, pol, efix, mode, out, partial, ctl);
 }
-void launch_cg_update(hipStream_t s, int npol, const int* polidx, const double* pol, double self_term,
+void launch_cg_precond(hipStream_t s, const PrecondBlocks& B, const double* minv, const double* pol, const double* r,
+                       double* z, double* partial, const SolveCtl* ctl) {
+    if (B.n > 0)
+        hipLaunchKernelGGL(k_cg_precond, cg_blocks(B.n), 256, 0, s, B.n, B.start, B.site, minv, pol, r, z, partial,
+                           ctl);
+}
+void launch_cg_update(hipStream_t s, const PrecondBlocks& B, const double* minv, const double* pol, int npol,
                       const double* partial_mv, const double* rz, int k, double* x, double* r, const double* p,
-                      const double* q, double* z, double* partial) {
-    if (npol > 0)
-        hipLaunchKernelGGL(k_cg_update, cg_blocks(npol), 256, 0, s, npol, polidx, pol, self_term, iterate_blocks(npol),
-                           partial_mv, rz, k, x, r, p, q, z, partial);
+                      const double* q, double* z, double* partial, const SolveCtl* ctl) {
+    if (B.n > 0)
+        hipLaunchKernelGGL(k_cg_update, cg_blocks(B.n), 256, 0, s, B.n, B.start, B.site, minv, pol,
+                           iterate_blocks(npol), partial_mv, rz, k, x, r, p, q, z, partial, ctl);
 }
 void launch_cg_direction(hipStream_t s, int npol, const int* polidx, int nb, const double* partial, double* rz, int k,
-                         const double* z, double* p, int n_all, double* eps) {
+                         const double* z, double* p, int n_all, double tol, int max_iter, SolveCtl* ctl) {
     if (npol > 0)
         hipLaunchKernelGGL(k_cg_direction, cg_blocks(npol), 256, 0, s, npol, polidx, nb, partial, rz, k, z, p, n_all,
-                           eps);
+                           tol, max_iter, ctl);
 }
-void launch_eps_finish(hipStream_t s, int nblocks, int n_all, const double* partial, double* eps) {
-    hipLaunchKernelGGL(k_eps_finish, 1, 256, 0, s, nblocks, n_all, partial, eps);
+void launch_sor_finish(hipStream_t s, int nblocks, int n_all, const double* partial, int t, double tol, int max_iter,
+                       SolveCtl* ctl) {
+    hipLaunchKernelGGL(k_sor_finish, 1, 256, 0, s, nblocks, n_all, partial, t, tol, max_iter, ctl);
 }
 void launch_scale_copy(hipStream_t s, int n, const double* a, const double* pol, double* out, double* out2) {
     hipLaunchKernelGGL(k_scale_copy, nblk(3 * n, 256), 256, 0, s, 3 * n, a, pol, out, out2);

@@ -184,8 +184,10 @@ __device__ double2* fft_line(double2* a, double2* b, bool active, const FftAxis&
 // z pass, forward: lines of the real grid -> half spectrum
 template <int C>
 __global__ void __launch_bounds__(kFftThreads) k_fft_z_r2c(const double* __restrict__ real, double2* __restrict__ H,
-                                                           int nlines, FftAxis az, int Kh, int Khp) {
+                                                           int nlines, FftAxis az, int Kh, int Khp,
+                                                           const int* __restrict__ stop) {
     extern __shared__ double2 lds[];
+    if (stop && *stop) return;
     const int N = az.n, M = kFftThreads >> az.lgP, P = 1 << az.lgP;
     const int L0 = blockIdx.x * M, m = min(M, nlines - L0);
     const int l = threadIdx.x >> az.lgP, lane = threadIdx.x & (P - 1);
@@ -219,8 +221,9 @@ __global__ void __launch_bounds__(kFftThreads) k_fft_z_r2c(const double* __restr
 // y pass (either direction): block = (x, tile of M consecutive kz)
 template <int C>
 __global__ void __launch_bounds__(kFftThreads) k_fft_y(double2* __restrict__ H, int K1, int Kh, int Khp, FftAxis ay,
-                                                       int inv) {
+                                                       int inv, const int* __restrict__ stop) {
     extern __shared__ double2 lds[];
+    if (stop && *stop) return;
     const int N = ay.n, lgM = 7 - ay.lgP, M = 1 << lgM;
     const int ntile = (Kh + M - 1) >> lgM;
     const int x = blockIdx.x / ntile, kz0 = (blockIdx.x - x * ntile) * M, m = min(M, Kh - kz0);
@@ -266,8 +269,10 @@ __global__ void __launch_bounds__(kFftThreads) k_fft_y(double2* __restrict__ H, 
 // x pass: forward, influence function, inverse; block = (y, tile of M consecutive kz)
 template <int C>
 __global__ void __launch_bounds__(kFftThreads) k_fft_x_conv(double2* __restrict__ H, const double* __restrict__ influence,
-                                                            int K1, int Kh, int Khp, FftAxis ax) {
+                                                            int K1, int Kh, int Khp, FftAxis ax,
+                                                            const int* __restrict__ stop) {
     extern __shared__ double2 lds[];
+    if (stop && *stop) return;
     const int N = ax.n, lgM = 7 - ax.lgP, M = 1 << lgM, P = 1 << ax.lgP;
     const int ntile = (Kh + M - 1) >> lgM;
     const int y = blockIdx.x / ntile, kz0 = (blockIdx.x - y * ntile) * M, m = min(M, Kh - kz0);
@@ -333,8 +338,10 @@ __global__ void __launch_bounds__(kFftThreads) k_fft_x_conv(double2* __restrict_
 // z pass, inverse: half spectrum -> real lines (kz >= Kh from conj(H[N - kz]))
 template <int C>
 __global__ void __launch_bounds__(kFftThreads) k_fft_z_c2r(const double2* __restrict__ H, double* __restrict__ real,
-                                                           int nlines, FftAxis az, int Kh, int Khp) {
+                                                           int nlines, FftAxis az, int Kh, int Khp,
+                                                           const int* __restrict__ stop) {
     extern __shared__ double2 lds[];
+    if (stop && *stop) return;
     const int N = az.n, M = kFftThreads >> az.lgP, P = 1 << az.lgP;
     const int L0 = blockIdx.x * M, m = min(M, nlines - L0);
     const int l = threadIdx.x >> az.lgP, lane = threadIdx.x & (P - 1);
@@ -388,25 +395,28 @@ static int copies(const FftAxis& a) {
 
 static size_t lds_bytes(const FftAxis& a) { return 2 * sizeof(double2) * (size_t)(kFftThreads >> a.lgP) * a.n; }
 
-void launch_fft_z_r2c(hipStream_t s, const PmeFft& F, const double* real, double2* H, const KEvents& ev) {
+void launch_fft_z_r2c(hipStream_t s, const PmeFft& F, const double* real, double2* H, const KEvents& ev,
+                      const int* stop) {
     const int nl = F.ax[0].n * F.ax[1].n, M = kFftThreads >> F.ax[2].lgP, Kh = F.ax[2].n / 2 + 1;
     MBPOL_FFT_LAUNCH(k_fft_z_r2c, F.ax[2], nblk(nl, M), kFftThreads, lds_bytes(F.ax[2]), s, real, H, nl, F.ax[2], Kh,
-                     F.khp)
+                     F.khp, stop)
 }
-void launch_fft_y(hipStream_t s, const PmeFft& F, double2* H, int inverse, const KEvents& ev) {
+void launch_fft_y(hipStream_t s, const PmeFft& F, double2* H, int inverse, const KEvents& ev, const int* stop) {
     const int K0 = F.ax[0].n, K1 = F.ax[1].n, Kh = F.ax[2].n / 2 + 1, M = kFftThreads >> F.ax[1].lgP;
     MBPOL_FFT_LAUNCH(k_fft_y, F.ax[1], K0 * nblk(Kh, M), kFftThreads, lds_bytes(F.ax[1]), s, H, K1, Kh, F.khp, F.ax[1],
-                     inverse)
+                     inverse, stop)
 }
-void launch_fft_x_conv(hipStream_t s, const PmeFft& F, double2* H, const double* influence, const KEvents& ev) {
+void launch_fft_x_conv(hipStream_t s, const PmeFft& F, double2* H, const double* influence, const KEvents& ev,
+                       const int* stop) {
     const int K1 = F.ax[1].n, Kh = F.ax[2].n / 2 + 1, M = kFftThreads >> F.ax[0].lgP;
     MBPOL_FFT_LAUNCH(k_fft_x_conv, F.ax[0], K1 * nblk(Kh, M), kFftThreads, lds_bytes(F.ax[0]), s, H, influence, K1, Kh,
-                     F.khp, F.ax[0])
+                     F.khp, F.ax[0], stop)
 }
-void launch_fft_z_c2r(hipStream_t s, const PmeFft& F, const double2* H, double* real, const KEvents& ev) {
+void launch_fft_z_c2r(hipStream_t s, const PmeFft& F, const double2* H, double* real, const KEvents& ev,
+                      const int* stop) {
     const int nl = F.ax[0].n * F.ax[1].n, M = kFftThreads >> F.ax[2].lgP, Kh = F.ax[2].n / 2 + 1;
     MBPOL_FFT_LAUNCH(k_fft_z_c2r, F.ax[2], nblk(nl, M), kFftThreads, lds_bytes(F.ax[2]), s, H, real, nl, F.ax[2], Kh,
-                     F.khp)
+                     F.khp, stop)
 }
 #undef MBPOL_FFT_LAUNCH
 
