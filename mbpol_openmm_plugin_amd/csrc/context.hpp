@@ -3,8 +3,12 @@
 // Reference kernels keep between calls, MBPolReferenceKernels.h:45-244).
 #pragma once
 #include <hip/hip_runtime.h>
+#include <condition_variable>
 #include <cstdint>
+#include <functional>
+#include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "kernels.h"
@@ -88,6 +92,24 @@ struct EvalStats {
     long long dimers = 0, trimers = 0, elec_pairs = 0, disp_pairs = 0, pol_pairs = 0;
 };
 
+// One host thread that runs one posted job at a time; wait() returns the
+// job's status and, on failure, its error message.
+class HostWorker {
+public:
+    ~HostWorker();
+    void post(std::function<int()> job);
+    int wait(std::string& msg);
+
+private:
+    std::thread th_;
+    std::mutex m_;
+    std::condition_variable cv_;
+    std::function<int()> job_;
+    bool has_job_ = false, done_ = true, quit_ = false;
+    int rc_ = 0;
+    std::string msg_;
+};
+
 }  // namespace mbpol
 
 struct mbpol_context {
@@ -98,6 +120,7 @@ struct mbpol_context {
     // second stream for the short-range terms, forked from and joined to `stream` by events
     hipStream_t side = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    mbpol::HostWorker worker;  // enqueues the short-range terms on `side`
     int full_chain_list = 0;
     mbpol::EvalStats stats;
     mbpol::Timer timer;

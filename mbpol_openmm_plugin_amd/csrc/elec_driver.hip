@@ -588,7 +588,7 @@ static int solve_pcg(mbpol_context* ctx, double* x, double self_term, SolveCtl& 
     }
     const size_t vbytes = sizeof(double) * 3 * std::max(n, 1);
     for (DevBuf* b : {&E->cg_r, &E->cg_z, &E->cg_p, &E->cg_q}) HIP_TRY(b->ensure(vbytes));
-    HIP_TRY(E->cg_partial.ensure(sizeof(double) * 2 * (cg_blocks(E->nblk) + 1)));
+    HIP_TRY(E->cg_partial.ensure(sizeof(double) * 2 * (precond_wgs(E->nblk) + 1)));
     HIP_TRY(E->cg_rz.ensure(sizeof(double) * (s.el_max_iter + 2)));
     HIP_TRY(E->partial.ensure(sizeof(double) * 2 * (iterate_blocks(npol) + 1)));
     double *r = E->cg_r.as<double>(), *z = E->cg_z.as<double>(), *p = E->cg_p.as<double>(), *q = E->cg_q.as<double>();
@@ -620,14 +620,14 @@ static int solve_pcg(mbpol_context* ctx, double* x, double self_term, SolveCtl& 
             if (f == 0) {
                 if (int e = field(x, 0, r)) return e;
                 launch_cg_precond(st, B, minv, pol, r, z, E->cg_partial.as<double>(), ctl);
-                launch_cg_direction(st, npol, polidx, cg_blocks(E->nblk), E->cg_partial.as<double>(), rz, 0, z, p, n,
+                launch_cg_direction(st, npol, polidx, precond_wgs(E->nblk), E->cg_partial.as<double>(), rz, 0, z, p, n,
                                     s.el_eps, s.el_max_iter, ctl);
             } else {
                 const int k = f - 1;
                 if (int e = field(p, 1, q)) return e;
                 launch_cg_update(st, B, minv, pol, npol, E->partial.as<double>(), rz, k, x, r, p, q, z,
                                  E->cg_partial.as<double>(), ctl);
-                launch_cg_direction(st, npol, polidx, cg_blocks(E->nblk), E->cg_partial.as<double>(), rz, k + 1, z, p,
+                launch_cg_direction(st, npol, polidx, precond_wgs(E->nblk), E->cg_partial.as<double>(), rz, k + 1, z, p,
                                     n, s.el_eps, s.el_max_iter, ctl);
             }
             timer_stop(ctx, T_ELEC_ITER);

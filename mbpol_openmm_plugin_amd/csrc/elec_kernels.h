@@ -95,6 +95,7 @@ struct PrecondBlocks {
     const int* site;   // site indices, block by block
 };
 int cg_blocks(int nitems);
+int precond_wgs(int nblocks);  // workgroups (and partial pairs) of launch_cg_precond / launch_cg_update
 int block_ld();  // leading dimension of a block's inverse (block_ld()^2 doubles per block)
 void launch_block_factor(hipStream_t s, const PrecondBlocks& B, const double* pos, SiteParams S, ElecParams P,
                          double* minv);
@@ -102,7 +103,7 @@ void launch_block_factor(hipStream_t s, const PrecondBlocks& B, const double* po
 void launch_cg_matvec(hipStream_t s, int npol, const int* polidx, const double* pos, ElecParams P, const ElecRows& R,
                       const double* v, const double* grid, double self_term, const double* pol, const double* efix,
                       int mode, double* out, double* partial, const SolveCtl* ctl, const KEvents& ev);
-// z = M^-1 r with partials (r.z, |alpha r|^2), 2 per cg_blocks(B.n)
+// z = M^-1 r with partials (r.z, |alpha r|^2), 2 per precond_wgs(B.n)
 void launch_cg_precond(hipStream_t s, const PrecondBlocks& B, const double* minv, const double* pol, const double* r,
                        double* z, double* partial, const SolveCtl* ctl);
 // step k (uses rz[k] and the matvec partials); partials as launch_cg_precond

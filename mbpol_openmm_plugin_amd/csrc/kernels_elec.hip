@@ -1006,85 +1006,85 @@ __global__ void __launch_bounds__(256) k_cg_matvec(int npol, const int* __restri
 // cancels the reciprocal self-interaction).  One thread inverts one block in
 // place (Gauss-Jordan; the block is symmetric positive definite).
 constexpr int kBlockSites = 4, kBlockLd = 3 * kBlockSites;
+constexpr int kGB = 16;  // lanes per block when applying the preconditioner (>= kBlockLd)
 
+// kGB lanes per block, lane l < 3m holding row l of the block (site l / 3,
+// component l % 3) in registers: assembly, then in-place Gauss-Jordan with the
+// pivot row broadcast by shuffles.  The stored inverse is kBlockLd x kBlockLd,
+// zero outside the block's 3m x 3m.
 __global__ void __launch_bounds__(256) k_block_factor(int nblocks, const int* __restrict__ blk_start,
                                                       const int* __restrict__ blk_site, const double* __restrict__ pos,
                                                       const SiteParams S, ElecParams P, double* __restrict__ minv) {
-    const int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= nblocks) return;
+    const int b = (blockIdx.x * blockDim.x + threadIdx.x) / kGB, l = threadIdx.x % kGB;
+    if (b >= nblocks) return;  // uniform over the lanes of a block
     const int s0 = blk_start[b], m = blk_start[b + 1] - s0, d = 3 * m;
-    double* M = minv + (size_t)b * kBlockLd * kBlockLd;
-    for (int u = 0; u < m; ++u) {
-        const int i = blk_site[s0 + u];
-        for (int w = 0; w < m; ++w) {
-            const int j = blk_site[s0 + w];
-            double t[3][3];
-            if (u == w) {
-                for (int x = 0; x < 3; ++x)
-                    for (int y = 0; y < 3; ++y) t[x][y] = x == y ? 1.0 / S.pol[i] : 0.0;
-            } else {
-                double dx = pos[3 * j] - pos[3 * i], dy = pos[3 * j + 1] - pos[3 * i + 1],
-                       dz = pos[3 * j + 2] - pos[3 * i + 2];
-                min_image(dx, dy, dz, P);
-                const double r2 = dx * dx + dy * dy + dz * dz, r = sqrt(r2);
-                const Thole th(S.damp6[i], S.damp6[j], r);
-                const double g = tdd_gamma(P, 1, S.type[i], S.type[j]);
-                const double e = th.ex(g);
-                const double p1 = th.s3(e) / (r2 * r), p2 = -3.0 * th.s5(g, e) / (r2 * r2 * r);
-                const double dv[3] = {dx, dy, dz};
-                for (int x = 0; x < 3; ++x)
-                    for (int y = 0; y < 3; ++y) t[x][y] = (x == y ? p1 : 0.0) + p2 * dv[x] * dv[y];
+    double a[kBlockLd];
+#pragma unroll
+    for (int c = 0; c < kBlockLd; ++c) a[c] = 0.0;
+    if (l < d) {
+        const int u = l / 3, x = l % 3, i = blk_site[s0 + u];
+#pragma unroll
+        for (int w = 0; w < kBlockSites; ++w) {
+            if (w >= m) continue;
+            if (w == u) {
+                const double ia = 1.0 / S.pol[i];
+#pragma unroll
+                for (int y = 0; y < 3; ++y)
+                    if (y == x) a[3 * w + y] = ia;
+                continue;
             }
-            for (int x = 0; x < 3; ++x)
-                for (int y = 0; y < 3; ++y) M[(3 * u + x) * kBlockLd + 3 * w + y] = t[x][y];
+            const int j = blk_site[s0 + w];
+            double dx = pos[3 * j] - pos[3 * i], dy = pos[3 * j + 1] - pos[3 * i + 1],
+                   dz = pos[3 * j + 2] - pos[3 * i + 2];
+            min_image(dx, dy, dz, P);
+            const double r2 = dx * dx + dy * dy + dz * dz, r = sqrt(r2);
+            const Thole th(S.damp6[i], S.damp6[j], r);
+            const double g = tdd_gamma(P, 1, S.type[i], S.type[j]);
+            const double e = th.ex(g);
+            const double p1 = th.s3(e) / (r2 * r), p2 = -3.0 * th.s5(g, e) / (r2 * r2 * r);
+            const double dv[3] = {dx, dy, dz};
+            const double dxl = x == 0 ? dx : (x == 1 ? dy : dz);
+#pragma unroll
+            for (int y = 0; y < 3; ++y) a[3 * w + y] = (y == x ? p1 : 0.0) + p2 * dxl * dv[y];
         }
     }
-    for (int k = 0; k < d; ++k) {
-        const double piv = 1.0 / M[k * kBlockLd + k];
-        M[k * kBlockLd + k] = 1.0;
-        for (int j = 0; j < d; ++j) M[k * kBlockLd + j] *= piv;
-        for (int i = 0; i < d; ++i) {
-            if (i == k) continue;
-            const double f = M[i * kBlockLd + k];
-            M[i * kBlockLd + k] = 0.0;
-            for (int j = 0; j < d; ++j) M[i * kBlockLd + j] -= f * M[k * kBlockLd + j];
+    for (int k = 0; k < d; ++k) {  // d is uniform over the group
+        double rk[kBlockLd];
+#pragma unroll
+        for (int c = 0; c < kBlockLd; ++c) rk[c] = __shfl(a[c], k, kGB);
+        double akk = 0.0, aik = 0.0;
+#pragma unroll
+        for (int c = 0; c < kBlockLd; ++c)
+            if (c == k) {
+                akk = rk[c];
+                aik = a[c];
+            }
+        const double piv = 1.0 / akk;
+#pragma unroll
+        for (int c = 0; c < kBlockLd; ++c) rk[c] = (c == k ? 1.0 : rk[c]) * piv;
+        if (l == k) {
+#pragma unroll
+            for (int c = 0; c < kBlockLd; ++c) a[c] = rk[c];
+        } else {
+#pragma unroll
+            for (int c = 0; c < kBlockLd; ++c) a[c] = (c == k ? 0.0 : a[c]) - aik * rk[c];
         }
+    }
+    if (l < kBlockLd) {
+        double* M = minv + ((size_t)b * kBlockLd + l) * kBlockLd;
+#pragma unroll
+        for (int c = 0; c < kBlockLd; ++c) M[c] = (l < d && c < d) ? a[c] : 0.0;
     }
 }
 
-// z = M^-1 r on one block, and its contributions to r.z and |alpha r|^2
-__device__ __forceinline__ void block_apply(int b, const int* __restrict__ blk_start, const int* __restrict__ blk_site,
-                                            const double* __restrict__ minv, const double* __restrict__ pol,
-                                            const double* __restrict__ r, double* __restrict__ z, double& s0,
-                                            double& s1) {
-    const int st = blk_start[b], m = blk_start[b + 1] - st, d = 3 * m;
-    double rv[kBlockLd];
+// z = M^-1 r on one block with kGB lanes, lane l < 3m owning row l (site l / 3,
+// component l % 3, global index t); rl is this lane's r (0 beyond the block)
+__device__ __forceinline__ double block_row_apply(int b, int lane, double rl, const double* __restrict__ minv) {
+    const double* M = minv + (size_t)b * kBlockLd * kBlockLd + (size_t)min(lane, kBlockLd - 1) * kBlockLd;
+    double acc = 0.0;
 #pragma unroll
-    for (int u = 0; u < kBlockSites; ++u)
-        if (u < m) {
-            const int i = blk_site[st + u];
-            const double a = pol[i];
-#pragma unroll
-            for (int x = 0; x < 3; ++x) {
-                rv[3 * u + x] = r[3 * i + x];
-                s1 += (a * rv[3 * u + x]) * (a * rv[3 * u + x]);
-            }
-        }
-    const double* M = minv + (size_t)b * kBlockLd * kBlockLd;
-#pragma unroll
-    for (int u = 0; u < kBlockSites; ++u)
-        if (u < m) {
-            const int i = blk_site[st + u];
-#pragma unroll
-            for (int x = 0; x < 3; ++x) {
-                double acc = 0.0;
-#pragma unroll
-                for (int c = 0; c < kBlockLd; ++c)
-                    if (c < d) acc += M[(3 * u + x) * kBlockLd + c] * rv[c];
-                z[3 * i + x] = acc;
-                s0 += rv[3 * u + x] * acc;
-            }
-        }
+    for (int c = 0; c < kBlockLd; ++c) acc += M[c] * __shfl(rl, c, kGB);
+    return acc;
 }
 
 // the initial residual's preconditioned vector and partials (r.z, |alpha r|^2)
@@ -1094,9 +1094,25 @@ __global__ void __launch_bounds__(256) k_cg_precond(int nblocks, const int* __re
                                                     double* __restrict__ z, double* __restrict__ partial,
                                                     const SolveCtl* __restrict__ ctl) {
     if (ctl->stop) return;
-    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    const int b = (blockIdx.x * blockDim.x + threadIdx.x) / kGB, lane = threadIdx.x % kGB;
     double s0 = 0.0, s1 = 0.0;
-    if (b < nblocks) block_apply(b, blk_start, blk_site, minv, pol, r, z, s0, s1);
+    if (b < nblocks) {  // uniform over the lanes of a block
+        const int st = blk_start[b], d = 3 * (blk_start[b + 1] - st);
+        int t = 0;
+        double rl = 0.0, a = 0.0;
+        if (lane < d) {
+            const int i = blk_site[st + lane / 3];
+            t = 3 * i + lane % 3;
+            rl = r[t];
+            a = pol[i];
+        }
+        const double zl = block_row_apply(b, lane, rl, minv);
+        if (lane < d) {
+            z[t] = zl;
+            s0 = rl * zl;
+            s1 = (a * rl) * (a * rl);
+        }
+    }
     block_partial<1>(s0, 0, partial + 2 * blockIdx.x);
     block_partial<1>(s1, 0, partial + 2 * blockIdx.x + 1);
 }
@@ -1119,7 +1135,7 @@ __device__ __forceinline__ double reduce_partials(int nb, int c, const double* _
 }
 
 // step k: a = rz[k] / (p.q); x += a p; r -= a q; z = M^-1 r; partials (r.z, |alpha r|^2);
-// one thread per preconditioner block
+// kGB lanes per preconditioner block
 __global__ void __launch_bounds__(256) k_cg_update(int nblocks, const int* __restrict__ blk_start,
                                                    const int* __restrict__ blk_site, const double* __restrict__ minv,
                                                    const double* __restrict__ pol, int nb_mv,
@@ -1131,20 +1147,26 @@ __global__ void __launch_bounds__(256) k_cg_update(int nblocks, const int* __res
     if (ctl->stop) return;
     const double pq = reduce_partials(nb_mv, 0, partial_mv);
     const double step = rz[k] / pq;
-    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    const int b = (blockIdx.x * blockDim.x + threadIdx.x) / kGB, lane = threadIdx.x % kGB;
     double s0 = 0.0, s1 = 0.0;
-    if (b < nblocks) {
-        const int st = blk_start[b], en = blk_start[b + 1];
-        for (int e = st; e < en; ++e) {
-            const int i = blk_site[e];
-#pragma unroll
-            for (int d = 0; d < 3; ++d) {
-                const int t = 3 * i + d;
-                x[t] += step * p[t];
-                r[t] -= step * q[t];
-            }
+    if (b < nblocks) {  // uniform over the lanes of a block
+        const int st = blk_start[b], d = 3 * (blk_start[b + 1] - st);
+        int t = 0;
+        double rl = 0.0, a = 0.0;
+        if (lane < d) {
+            const int i = blk_site[st + lane / 3];
+            t = 3 * i + lane % 3;
+            x[t] += step * p[t];
+            rl = r[t] - step * q[t];
+            r[t] = rl;
+            a = pol[i];
         }
-        block_apply(b, blk_start, blk_site, minv, pol, r, z, s0, s1);
+        const double zl = block_row_apply(b, lane, rl, minv);
+        if (lane < d) {
+            z[t] = zl;
+            s0 = rl * zl;
+            s1 = (a * rl) * (a * rl);
+        }
     }
     block_partial<1>(s0, 0, partial + 2 * blockIdx.x);
     block_partial<1>(s1, 0, partial + 2 * blockIdx.x + 1);
@@ -1516,11 +1538,12 @@ void launch_iterate(hipStream_t s, int npol, const int* polidx, const double* po
                  mu_in, mu_out, efix_pol, pol, grid, self_term, omega, partial, ctl);
 }
 int cg_blocks(int nitems) { return std::max(1, nblk(nitems, 256)); }
+int precond_wgs(int nblocks) { return cg_blocks(nblocks * kGB); }
 int block_ld() { return kBlockLd; }
 void launch_block_factor(hipStream_t s, const PrecondBlocks& B, const double* pos, SiteParams S, ElecParams P,
                          double* minv) {
     if (B.n > 0)
-        hipLaunchKernelGGL(k_block_factor, cg_blocks(B.n), 256, 0, s, B.n, B.start, B.site, pos, S, P, minv);
+        hipLaunchKernelGGL(k_block_factor, precond_wgs(B.n), 256, 0, s, B.n, B.start, B.site, pos, S, P, minv);
 }
 void launch_cg_matvec(hipStream_t s, int npol, const int* polidx, const double* pos, ElecParams P, const ElecRows& R,
                       const double* v, const double* grid, double self_term, const double* pol, const double* efix,
@@ -1532,14 +1555,14 @@ void launch_cg_matvec(hipStream_t s, int npol, const int* polidx, const double* 
 void launch_cg_precond(hipStream_t s, const PrecondBlocks& B, const double* minv, const double* pol, const double* r,
                        double* z, double* partial, const SolveCtl* ctl) {
     if (B.n > 0)
-        hipLaunchKernelGGL(k_cg_precond, cg_blocks(B.n), 256, 0, s, B.n, B.start, B.site, minv, pol, r, z, partial,
+        hipLaunchKernelGGL(k_cg_precond, precond_wgs(B.n), 256, 0, s, B.n, B.start, B.site, minv, pol, r, z, partial,
                            ctl);
 }
 void launch_cg_update(hipStream_t s, const PrecondBlocks& B, const double* minv, const double* pol, int npol,
                       const double* partial_mv, const double* rz, int k, double* x, double* r, const double* p,
                       const double* q, double* z, double* partial, const SolveCtl* ctl) {
     if (B.n > 0)
-        hipLaunchKernelGGL(k_cg_update, cg_blocks(B.n), 256, 0, s, B.n, B.start, B.site, minv, pol,
+        hipLaunchKernelGGL(k_cg_update, precond_wgs(B.n), 256, 0, s, B.n, B.start, B.site, minv, pol,
                            iterate_blocks(npol), partial_mv, rz, k, x, r, p, q, z, partial, ctl);
 }
 void launch_cg_direction(hipStream_t s, int npol, const int* polidx, int nb, const double* partial, double* rz, int k,

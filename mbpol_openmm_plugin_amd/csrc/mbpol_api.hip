@@ -74,6 +74,52 @@ int mbpol::scan_i64(mbpol_context* ctx, const long long* in, long long* out, int
 }
 
 // ---------------------------------------------------------------------------
+// host worker thread
+// ---------------------------------------------------------------------------
+mbpol::HostWorker::~HostWorker() {
+    if (!th_.joinable()) return;
+    {
+        std::lock_guard<std::mutex> g(m_);
+        quit_ = true;
+    }
+    cv_.notify_all();
+    th_.join();
+}
+
+void mbpol::HostWorker::post(std::function<int()> job) {
+    std::unique_lock<std::mutex> g(m_);
+    if (!th_.joinable())
+        th_ = std::thread([this] {
+            std::unique_lock<std::mutex> l(m_);
+            for (;;) {
+                cv_.wait(l, [this] { return quit_ || has_job_; });
+                if (quit_) return;
+                std::function<int()> f = std::move(job_);
+                has_job_ = false;
+                l.unlock();
+                const int rc = f();
+                std::string msg = rc ? std::string(mbpol_last_error()) : std::string();
+                l.lock();
+                rc_ = rc;
+                msg_ = std::move(msg);
+                done_ = true;
+                cv_.notify_all();
+            }
+        });
+    job_ = std::move(job);
+    has_job_ = true;
+    done_ = false;
+    cv_.notify_all();
+}
+
+int mbpol::HostWorker::wait(std::string& msg) {
+    std::unique_lock<std::mutex> g(m_);
+    cv_.wait(g, [this] { return done_; });
+    msg = msg_;
+    return rc_;
+}
+
+// ---------------------------------------------------------------------------
 // HIP-event timing
 // ---------------------------------------------------------------------------
 static hipEvent_t take_event(mbpol_context* ctx) {
@@ -527,26 +573,13 @@ static int build_lists(mbpol_context* ctx, hipStream_t st, bool want2, bool want
     return MBPOL_OK;
 }
 
-static int evaluate(mbpol_context* ctx, double* forces_dev_out, double* energies_out, uint32_t comp) {
+// 1-, 2- and 3-body terms and the dispersion, enqueued on stream sr
+static int short_range(mbpol_context* ctx, hipStream_t sr, uint32_t comp) {
     const mbpol_system& s = ctx->sys;
-    hipStream_t st = ctx->stream;
     u64* F = ctx->force_fx.as<u64>();
     u64* E = ctx->energy_fx.as<u64>();
-    double* pos = ctx->pos.as<double>();
+    const double* pos = ctx->pos.as<double>();
     const d3 box{s.box[0], s.box[1], s.box[2]};
-    HIP_TRY(hipMemsetAsync(F, 0, sizeof(u64) * 3 * s.n, st));
-    HIP_TRY(hipMemsetAsync(E, 0, sizeof(u64) * 8, st));
-    launch_place_vsites(st, (int)s.vs_idx.size() / 4, ctx->vs_idx.as<int>(), ctx->vs_w.as<double>(), pos);
-    // The 1-, 2- and 3-body terms and the dispersion go to a second stream and
-    // overlap the electrostatics, which waits on its host-side convergence
-    // tests; with timing on, everything stays on one stream so that every
-    // span is attributable.
-    const bool fork = !ctx->timer.enabled;
-    hipStream_t sr = fork ? ctx->side : st;
-    if (fork) {
-        HIP_TRY(hipEventRecord(ctx->ev_fork, st));
-        HIP_TRY(hipStreamWaitEvent(sr, ctx->ev_fork, 0));
-    }
     const bool do1 = (comp & MBPOL_ONEBODY) && !s.ob_idx.empty();
     const bool do2 = (comp & MBPOL_TWOBODY) && !s.tb_idx.empty();
     const bool do3 = (comp & MBPOL_THREEBODY) && !s.th_idx.empty();
@@ -569,11 +602,11 @@ static int evaluate(mbpol_context* ctx, double* forces_dev_out, double* energies
         const int per = s.tb_method == MBPOL_CUTOFF_PERIODIC;
         const KEvents ev = timer_kernel(ctx, T_TWOBODY);
         if (dp)
-            launch_twobody_f64(sr, ctx->stats.dimers, ctx->dimers.as<int2>(), ctx->tb_idx.as<int>(), pos, per, box, F, E + 1,
-                               ev);
+            launch_twobody_f64(sr, ctx->stats.dimers, ctx->dimers.as<int2>(), ctx->tb_idx.as<int>(), pos, per, box, F,
+                               E + 1, ev);
         else
-            launch_twobody_f32(sr, ctx->stats.dimers, ctx->dimers.as<int2>(), ctx->tb_idx.as<int>(), pos, per, box, F, E + 1,
-                               ev);
+            launch_twobody_f32(sr, ctx->stats.dimers, ctx->dimers.as<int2>(), ctx->tb_idx.as<int>(), pos, per, box, F,
+                               E + 1, ev);
         timer_kernel_done(ctx, T_TWOBODY, ev);
     }
     if (do3) {
@@ -592,10 +625,49 @@ static int evaluate(mbpol_context* ctx, double* forces_dev_out, double* energies
         if (int e = disp_evaluate(ctx, sr, E + 3)) return e;
         timer_stop(ctx, T_DISPERSION);
     }
-    if (fork) HIP_TRY(hipEventRecord(ctx->ev_join, sr));
-    if ((comp & MBPOL_ELECTROSTATICS) && ctx->elec)
-        if (int e = elec_evaluate(ctx, E + 4)) return e;
-    if (fork) HIP_TRY(hipStreamWaitEvent(st, ctx->ev_join, 0));
+    return MBPOL_OK;
+}
+
+static int evaluate(mbpol_context* ctx, double* forces_dev_out, double* energies_out, uint32_t comp) {
+    const mbpol_system& s = ctx->sys;
+    hipStream_t st = ctx->stream;
+    u64* F = ctx->force_fx.as<u64>();
+    u64* E = ctx->energy_fx.as<u64>();
+    double* pos = ctx->pos.as<double>();
+    HIP_TRY(hipMemsetAsync(F, 0, sizeof(u64) * 3 * s.n, st));
+    HIP_TRY(hipMemsetAsync(E, 0, sizeof(u64) * 8, st));
+    launch_place_vsites(st, (int)s.vs_idx.size() / 4, ctx->vs_idx.as<int>(), ctx->vs_w.as<double>(), pos);
+    // The short-range terms go to a second stream, enqueued by the context's
+    // worker thread: their host-side waits (list sizes) and their kernels
+    // overlap the electrostatics, which the calling thread drives.  With
+    // timing on, everything runs on one stream from this thread, so that every
+    // timed span is attributable.
+    const bool fork = !ctx->timer.enabled;
+    if (!fork) {
+        if (int e = short_range(ctx, st, comp)) return e;
+    } else {
+        HIP_TRY(hipEventRecord(ctx->ev_fork, st));
+        const int device = ctx->device;
+        ctx->worker.post([ctx, comp, device]() -> int {
+            HIP_TRY(hipSetDevice(device));
+            HIP_TRY(hipStreamWaitEvent(ctx->side, ctx->ev_fork, 0));
+            if (int e = short_range(ctx, ctx->side, comp)) return e;
+            HIP_TRY(hipEventRecord(ctx->ev_join, ctx->side));
+            return MBPOL_OK;
+        });
+    }
+    int rc = MBPOL_OK;
+    if ((comp & MBPOL_ELECTROSTATICS) && ctx->elec) rc = elec_evaluate(ctx, E + 4);
+    if (fork) {
+        // always wait for the worker: it uses the context until its job ends
+        std::string msg;
+        const int wrc = ctx->worker.wait(msg);
+        if (rc) return rc;
+        if (wrc) return set_error(wrc, "%s", msg.c_str());
+        HIP_TRY(hipStreamWaitEvent(st, ctx->ev_join, 0));
+    } else if (rc) {
+        return rc;
+    }
     double* fout = forces_dev_out ? forces_dev_out : ctx->force.as<double>();
     launch_forces_to_double(st, 3 * s.n, F, fout);
     launch_fold_vsites(st, (int)s.vs_idx.size() / 4, ctx->vs_idx.as<int>(), ctx->vs_w.as<double>(), fout);
