@@ -74,6 +74,7 @@ extern "C" {
 #define MBPOL_GUESS_ZERO 1
 #define MBPOL_GUESS_PREVIOUS 2          /* converged dipoles of the previous evaluate() */
 #define MBPOL_GUESS_USER 3              /* dipoles passed with mbpol_context_set_dipole_guess() */
+#define MBPOL_GUESS_LEARNED 4           /* the network attached with mbpol_guess_attach() */
 
 /* induced-dipole solver (mbpol_context_set_solver); both stop when the reference's convergence measure
  * (:504-511, :588) of the current dipoles falls below the target epsilon */
@@ -160,6 +161,9 @@ int mbpol_context_get_dipoles(const mbpol_context* ctx, double* dipoles);
 /* charges used by the last evaluate (after redistribution) */
 int mbpol_context_get_charges(const mbpol_context* ctx, double* charges);
 int mbpol_context_set_dipole_guess(mbpol_context* ctx, int32_t mode, const double* dipoles);
+/* setMutualInducedTargetEpsilon / setMutualInducedMaxIterations on a live context (the reference sets them on the
+ * Force before the context exists, MBPolElectrostaticsForce.h:233-262) */
+int mbpol_context_set_dipole_convergence(mbpol_context* ctx, double target_eps, int32_t max_iter);
 /* MBPOL_SOLVER_SOR or MBPOL_SOLVER_PCG for the following evaluations (no counterpart in the reference, which
  * always iterates by SOR; the default depends on the precision mode given at creation) */
 int mbpol_context_set_solver(mbpol_context* ctx, int32_t solver);
@@ -177,6 +181,51 @@ int mbpol_context_set_list_mode(mbpol_context* ctx, int32_t full_chain_list);
  * induced dipoles of the last evaluate (MBPolReferenceElectrostaticsForce.cpp:1023-1086) */
 int mbpol_context_get_electrostatic_potential(mbpol_context* ctx, int32_t npoints, const double* points,
                                               double* potential);
+
+/* ---- learned initial guess of the induced dipoles (no counterpart in the reference) ----
+ * A per-atom network (48 radial features of the neighbourhood within 0.6 nm -> 64 -> 64 -> 3, SiLU, BF16 MFMA with
+ * FP32 accumulation) predicts (mu - alpha E_fixed) / |alpha E_fixed| in a frame of the atom anchored on its
+ * alpha E_fixed.  It is trained on the GPU from snapshots of
+ * converged dipoles taken from a context's evaluations.  Water systems with the O H H M site layout only. */
+typedef struct mbpol_guess mbpol_guess;
+#define MBPOL_GUESS_PARAMS 7491         /* W1[48][64] b1[64] W2[64][64] b2[64] W3[64][3] b3[3], matrices [in][out] */
+#define MBPOL_GUESS_FEATURES 48
+int mbpol_guess_create(mbpol_context* ctx, int32_t max_snapshots, uint64_t seed, mbpol_guess** out);
+void mbpol_guess_destroy(mbpol_guess* g);
+/* features and targets (converged dipoles) of the context's last evaluate */
+int mbpol_guess_add_snapshot(mbpol_guess* g);
+int mbpol_guess_num_snapshots(const mbpol_guess* g, int32_t* count);
+/* stored snapshot k: features (3 per water x 48) and targets (3 per water x 3, frame components of
+ * (mu - alpha E_fixed) / |alpha E_fixed|) */
+int mbpol_guess_get_snapshot(const mbpol_guess* g, int32_t k, float* features, float* targets);
+/* Adam on the stored snapshots (input normalisation and output scale set from them on the first call);
+ * loss_per_epoch[epochs]: mean squared error in units of the output scale */
+int mbpol_guess_train(mbpol_guess* g, int32_t epochs, int32_t batch, double learning_rate, float* loss_per_epoch);
+/* attach (1) or detach (0) the network as the context's MBPOL_GUESS_LEARNED guess (the mode is set too) */
+int mbpol_guess_attach(mbpol_guess* g, int32_t on);
+/* parameters (MBPOL_GUESS_PARAMS floats) and normalisation (48 means, 48 inverse deviations, output scale) */
+int mbpol_guess_get_params(const mbpol_guess* g, float* params, float* norm);
+int mbpol_guess_set_params(mbpol_guess* g, const float* params, const float* norm);
+/* gradient of (1/n) sum |net(x_i) - t_i / scale|^2 over n raw feature rows x (n x 48) and targets t (n x 3), as
+ * the training step computes it; grad[MBPOL_GUESS_PARAMS] */
+int mbpol_guess_gradient(mbpol_guess* g, int32_t n, const float* x, const float* t, float* grad, float* loss);
+/* network outputs (n x 3, unscaled) for n raw feature rows */
+int mbpol_guess_forward(mbpol_guess* g, int32_t n, const float* x, float* y);
+/* features (3 per water x 48) and frames (3 x 3 per atom, rows e1 e2 e3: e1 along the atom's alpha E_fixed, e2
+ * from the HOH bisector (O) or O -> H (H) orthogonalised, e3 = e1 x e2) of the context's last evaluate */
+int mbpol_guess_features(mbpol_guess* g, float* features, float* frames);
+
+/* ---- Langevin dynamics on the resident positions (BAOAB; no counterpart in the reference, whose examples drive
+ * OpenMM integrators, python/water14.py:36-126) ---- */
+typedef struct mbpol_md mbpol_md;
+/* masses in amu per particle (0: virtual site); velocities drawn at temperature_K; forces of the components
+ * (MBPOL_ALL or a subset) evaluated once */
+int mbpol_md_create(mbpol_context* ctx, const double* masses, double temperature_K, double friction_per_ps,
+                    double dt_ps, uint64_t seed, uint32_t components, mbpol_md** out);
+void mbpol_md_destroy(mbpol_md* md);
+/* nsteps steps; energies_out[6] of the last force evaluation and the kinetic energy (kJ/mol) after it, or NULL */
+int mbpol_md_step(mbpol_md* md, int32_t nsteps, double* energies_out, double* kinetic_out);
+int mbpol_md_get_positions(const mbpol_md* md, double* positions);
 
 #ifdef __cplusplus
 }

@@ -4,7 +4,7 @@ The product is ``libmbpol_hip.so`` (C ABI: ``include/mbpol_hip.h``); this
 package holds its sources (``csrc/``), the force-field data (``params/``) and a
 Python mirror of the reference plugin's Force / Context API (``api``).
 """
-from .api import (Context, MBPolDispersionForce, MBPolElectrostaticsForce, MBPolError,  # noqa: F401
-                  MBPolOneBodyForce, MBPolThreeBodyForce, MBPolTwoBodyForce, State, System,
-                  ThreeParticleAverageSite, device_count, library)
+from .api import (Context, GuessModel, LangevinIntegrator, MBPolDispersionForce,  # noqa: F401
+                  MBPolElectrostaticsForce, MBPolError, MBPolOneBodyForce, MBPolThreeBodyForce, MBPolTwoBodyForce,
+                  State, System, ThreeParticleAverageSite, device_count, library)
 from . import forcefield, systems  # noqa: F401

@@ -496,7 +496,7 @@ class Context:
         self._mask = int(mask)
 
     def setDipoleGuess(self, mode, dipoles=None):
-        modes = {'alpha_e': 0, 'zero': 1, 'previous': 2, 'user': 3}
+        modes = {'alpha_e': 0, 'zero': 1, 'previous': 2, 'user': 3, 'learned': 4}
         arr = _d(dipoles) if dipoles is not None else None
         _check(library().mbpol_context_set_dipole_guess(self._h, modes[mode], _pd(arr) if arr is not None else None))
 
@@ -599,3 +599,158 @@ def _context_get_potential(self, points):
 
 
 Context.getElectrostaticPotential = _context_get_potential
+
+
+def _context_set_dipole_convergence(self, target_epsilon, max_iterations=200):
+    """setMutualInducedTargetEpsilon / setMutualInducedMaxIterations on the live context."""
+    L = library()
+    L.mbpol_context_set_dipole_convergence.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_int32]
+    _check(L.mbpol_context_set_dipole_convergence(self._h, float(target_epsilon), int(max_iterations)))
+
+
+Context.setDipoleConvergence = _context_set_dipole_convergence
+
+_fp = ctypes.POINTER(ctypes.c_float)
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def _pf(a):
+    return a.ctypes.data_as(_fp)
+
+
+GUESS_PARAMS = 7491
+GUESS_FEATURES = 48
+
+
+class GuessModel:
+    """Learned initial guess of the induced dipoles (HIP extension; the
+    reference always starts from alpha * E_fixed).
+
+    A per-atom network (48 radial features -> 64 -> 64 -> 3, SiLU; BF16 MFMA
+    with FP32 accumulation) predicts (mu - alpha E_fixed) / |alpha E_fixed| in
+    a frame of the atom anchored on its alpha E_fixed.
+    Snapshots of converged dipoles are taken from the context's evaluations
+    (addSnapshot after getState), the network is trained on the GPU (train),
+    and attach() makes it the context's initial guess.  The solve still runs to
+    the same tolerance, so the guess changes the iteration count only."""
+
+    def __init__(self, context, max_snapshots, seed=1):
+        L = library()
+        L.mbpol_guess_create.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_uint64, ctypes.c_void_p]
+        self.context = context
+        self._h = ctypes.c_void_p()
+        _check(L.mbpol_guess_create(context._h, int(max_snapshots), int(seed), ctypes.byref(self._h)))
+        self.natoms = 3 * (context.system.getNumParticles() // 4)
+
+    def __del__(self):
+        h = getattr(self, '_h', None)
+        if h is not None and h.value:
+            L = library()
+            L.mbpol_guess_destroy.argtypes = [ctypes.c_void_p]
+            L.mbpol_guess_destroy(h)
+            self._h = None
+
+    def addSnapshot(self):
+        _check(library().mbpol_guess_add_snapshot(self._h))
+
+    def numSnapshots(self):
+        c = ctypes.c_int32(0)
+        _check(library().mbpol_guess_num_snapshots(self._h, ctypes.byref(c)))
+        return c.value
+
+    def getSnapshot(self, k):
+        """(features [3 W, 48], targets [3 W, 3]) of stored snapshot k."""
+        f = np.zeros((self.natoms, GUESS_FEATURES), dtype=np.float32)
+        t = np.zeros((self.natoms, 3), dtype=np.float32)
+        _check(library().mbpol_guess_get_snapshot(self._h, int(k), _pf(f), _pf(t)))
+        return f, t
+
+    def train(self, epochs, batch=4096, learning_rate=1e-3):
+        """Adam epochs over the stored snapshots; returns the mean loss of each epoch."""
+        L = library()
+        L.mbpol_guess_train.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_double, _fp]
+        out = np.zeros(max(int(epochs), 1), dtype=np.float32)
+        _check(L.mbpol_guess_train(self._h, int(epochs), int(batch), float(learning_rate), _pf(out)))
+        return out[:int(epochs)]
+
+    def attach(self, on=True):
+        _check(library().mbpol_guess_attach(self._h, int(bool(on))))
+
+    def getParameters(self):
+        p = np.zeros(GUESS_PARAMS, dtype=np.float32)
+        n = np.zeros(2 * GUESS_FEATURES + 1, dtype=np.float32)
+        _check(library().mbpol_guess_get_params(self._h, _pf(p), _pf(n)))
+        return p, n
+
+    def setParameters(self, params, norm=None):
+        p = _f32(params)
+        n = _f32(norm) if norm is not None else None
+        _check(library().mbpol_guess_set_params(self._h, _pf(p), _pf(n) if n is not None else None))
+
+    def gradient(self, x, t):
+        """(gradient, loss) of (1/n) sum |net(x_i) - t_i / scale|^2 as one training step computes it."""
+        x, t = _f32(np.asarray(x).reshape(-1, GUESS_FEATURES)), _f32(np.asarray(t).reshape(-1, 3))
+        g = np.zeros(GUESS_PARAMS, dtype=np.float32)
+        loss = ctypes.c_float(0)
+        _check(library().mbpol_guess_gradient(self._h, len(x), _pf(x), _pf(t), _pf(g), ctypes.byref(loss)))
+        return g, loss.value
+
+    def forward(self, x):
+        x = _f32(np.asarray(x).reshape(-1, GUESS_FEATURES))
+        y = np.zeros((len(x), 3), dtype=np.float32)
+        _check(library().mbpol_guess_forward(self._h, len(x), _pf(x), _pf(y)))
+        return y
+
+    def features(self):
+        """(features [3 W, 48], frames [3 W, 3, 3]) of the context's last evaluation."""
+        f = np.zeros((self.natoms, GUESS_FEATURES), dtype=np.float32)
+        r = np.zeros((self.natoms, 3, 3), dtype=np.float32)
+        _check(library().mbpol_guess_features(self._h, _pf(f), _pf(r)))
+        return f, r
+
+
+class LangevinIntegrator:
+    """Langevin dynamics (BAOAB splitting) on the context's device-resident
+    positions, with forces from the same context (HIP extension; mirrors
+    OpenMM's LangevinIntegrator(temperature K, friction 1/ps, step ps)).
+    Virtual sites (mass 0) follow their parents.  `components` restricts the
+    forces (and energies) to a subset of the Forces."""
+
+    def __init__(self, context, temperature, friction, step_size, seed=1, components=MBPOL_ALL):
+        L = library()
+        L.mbpol_md_create.argtypes = [ctypes.c_void_p, _dp, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                      ctypes.c_uint64, ctypes.c_uint32, ctypes.c_void_p]
+        self.context = context
+        sysm = context.system
+        masses = _d([0.0 if sysm.isVirtualSite(i) else sysm.getParticleMass(i)
+                     for i in range(sysm.getNumParticles())])
+        if context.getPositions() is None:
+            raise MBPolError(1, 'positions have not been set')
+        context.uploadPositions(context.getPositions())
+        self._h = ctypes.c_void_p()
+        _check(L.mbpol_md_create(context._h, _pd(masses), float(temperature), float(friction), float(step_size),
+                                 int(seed), int(components), ctypes.byref(self._h)))
+        self._n = len(masses)
+
+    def __del__(self):
+        h = getattr(self, '_h', None)
+        if h is not None and h.value:
+            L = library()
+            L.mbpol_md_destroy.argtypes = [ctypes.c_void_p]
+            L.mbpol_md_destroy(h)
+            self._h = None
+
+    def step(self, steps):
+        """Run steps; returns (energies[6] of the last force evaluation, kinetic energy), kJ/mol."""
+        e = np.zeros(6)
+        ke = ctypes.c_double(0)
+        _check(library().mbpol_md_step(self._h, int(steps), _pd(e), ctypes.byref(ke)))
+        return e, ke.value
+
+    def getPositions(self):
+        out = np.zeros((self._n, 3))
+        _check(library().mbpol_md_get_positions(self._h, _pd(out)))
+        return out

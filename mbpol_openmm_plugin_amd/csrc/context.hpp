@@ -112,6 +112,8 @@ private:
 
 }  // namespace mbpol
 
+struct mbpol_guess;
+
 struct mbpol_context {
     mbpol_system sys;
     int precision = 0;
@@ -143,5 +145,6 @@ struct mbpol_context {
 
     // electrostatics and dispersion live in their own structures
     mbpol::Elec* elec = nullptr;
+    mbpol_guess* guess = nullptr;  // learned dipole guess in use (not owned)
     mbpol::Disp* disp = nullptr;
 };

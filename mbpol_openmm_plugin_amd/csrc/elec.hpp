@@ -23,6 +23,24 @@ void elec_destroy(mbpol_context* ctx);
 int elec_set_box(mbpol_context* ctx);
 int elec_evaluate(mbpol_context* ctx, u64* energy);
 
+// what the learned dipole guess reads from the electrostatics of the last
+// set-up (positions in the context's buffer)
+struct ElecView {
+    int n = 0, nw = 0;
+    bool water_layout = false;  // 4 sites per water, O H H M, types 0 1 1 2
+    const double *efix_pol = nullptr, *mu = nullptr;
+    const int* type = nullptr;
+    const int *cell_of = nullptr, *cell_start = nullptr, *members = nullptr;
+    int use_cells = 0, reach = 0;
+    int3 nc{1, 1, 1};
+    int periodic = 0;
+    d3 box{0, 0, 0}, inv_box{0, 0, 0};
+};
+int elec_view(mbpol_context* ctx, ElecView& v);
+// learned guess (guess_driver.hip): mu0 for the polarisable sites from the
+// context's attached model, after the electrostatics set-up of this evaluation
+int guess_predict(mbpol_context* ctx, double* mu0);
+
 // dispersion (disp_driver.hip)
 int disp_create(mbpol_context* ctx);
 void disp_destroy(mbpol_context* ctx);

@@ -638,6 +638,37 @@ static int solve_pcg(mbpol_context* ctx, double* x, double self_term, SolveCtl& 
     return MBPOL_OK;
 }
 
+int elec_view(mbpol_context* ctx, ElecView& v) {
+    Elec* E = ctx->elec;
+    if (!E) return set_error(MBPOL_ERR_INVALID_ARGUMENT, "no electrostatics in this system");
+    const mbpol_system& s = ctx->sys;
+    v.n = E->n;
+    v.nw = E->n / 4;
+    v.water_layout = s.el_redistribute && E->n % 4 == 0;
+    for (int i = 0; v.water_layout && i < E->n; ++i)
+        v.water_layout = s.el_type[i] == (i % 4 == 0 ? 0 : (i % 4 == 3 ? 2 : 1));
+    v.efix_pol = E->efix_pol.as<double>();
+    v.mu = E->mu_cur;
+    v.type = E->type.as<int>();
+    v.cell_of = E->cell_of.as<int>();
+    v.cell_start = E->cell_start.as<int>();
+    v.members = E->members.as<int>();
+    v.use_cells = E->use_cells;
+    v.nc = E->nc;
+    v.periodic = E->P.pme;
+    v.box = E->P.box;
+    v.inv_box = E->P.inv_box;
+    // cells reach the feature cutoff (0.6 nm) at reach ceil(0.6 / cell) without wrapping onto themselves
+    v.reach = 0;
+    if (E->use_cells) {
+        const double cell = std::min({E->P.box.x / E->nc.x, E->P.box.y / E->nc.y, E->P.box.z / E->nc.z});
+        const int r = (int)std::ceil(0.6 / cell);
+        if (2 * r + 1 <= std::min({E->nc.x, E->nc.y, E->nc.z})) v.reach = r;
+        else v.use_cells = 0;
+    }
+    return MBPOL_OK;
+}
+
 int elec_evaluate(mbpol_context* ctx, u64* energy) {
     Elec* E = ctx->elec;
     const mbpol_system& s = ctx->sys;
@@ -668,6 +699,9 @@ int elec_evaluate(mbpol_context* ctx, u64* energy) {
     double* mu_out = E->mu_b.as<double>();
     const size_t mbytes = sizeof(double) * 3 * n;
     switch (E->guess_mode) {
+        case MBPOL_GUESS_LEARNED:
+            if (int e = guess_predict(ctx, mu_in)) return e;
+            break;
         case MBPOL_GUESS_ZERO: HIP_TRY(hipMemsetAsync(mu_in, 0, mbytes, st)); break;
         case MBPOL_GUESS_PREVIOUS:
             if (E->have_previous) {
@@ -751,7 +785,10 @@ int mbpol_context_get_charges(const mbpol_context* ctx, double* charges) {
 int mbpol_context_set_dipole_guess(mbpol_context* ctx, int32_t mode, const double* dipoles) {
     using namespace mbpol;
     if (!ctx->elec) return set_error(MBPOL_ERR_INVALID_ARGUMENT, "no electrostatics in this system");
-    if (mode < MBPOL_GUESS_ALPHA_E || mode > MBPOL_GUESS_USER) return set_error(MBPOL_ERR_INVALID_ARGUMENT, "bad guess mode");
+    if (mode < MBPOL_GUESS_ALPHA_E || mode > MBPOL_GUESS_LEARNED)
+        return set_error(MBPOL_ERR_INVALID_ARGUMENT, "bad guess mode");
+    if (mode == MBPOL_GUESS_LEARNED && !ctx->guess)
+        return set_error(MBPOL_ERR_INVALID_ARGUMENT, "no learned guess attached (mbpol_guess_attach)");
     if (mode == MBPOL_GUESS_USER) {
         if (!dipoles) return set_error(MBPOL_ERR_INVALID_ARGUMENT, "user guess needs dipoles");
         // sites of zero polarisability keep zero dipoles (their guess entries are ignored)
@@ -763,6 +800,17 @@ int mbpol_context_set_dipole_guess(mbpol_context* ctx, int32_t mode, const doubl
         HIP_TRY(hipMemcpy(E->mu_user.p, g.data(), sizeof(double) * 3 * (size_t)E->n, hipMemcpyHostToDevice));
     }
     ctx->elec->guess_mode = mode;
+    return MBPOL_OK;
+}
+
+int mbpol_context_set_dipole_convergence(mbpol_context* ctx, double target_eps, int32_t max_iter) {
+    using namespace mbpol;
+    if (!ctx) return set_error(MBPOL_ERR_INVALID_ARGUMENT, "null context");
+    if (!ctx->elec) return set_error(MBPOL_ERR_INVALID_ARGUMENT, "no electrostatics in this system");
+    if (!(target_eps > 0.0) || max_iter < 1)
+        return set_error(MBPOL_ERR_INVALID_ARGUMENT, "target epsilon must be > 0 and max iterations >= 1");
+    ctx->sys.el_eps = target_eps;
+    ctx->sys.el_max_iter = max_iter;
     return MBPOL_OK;
 }
 

@@ -1,0 +1,205 @@
+"""Learned dipole guess and Langevin dynamics on the GPU (HIP extensions with no
+counterpart in the reference).
+
+* the network's gradient (one training step of the MFMA kernels) and its
+  forward pass against the CPU checker oracle/mlp_oracle.py, which rounds the
+  matrix-product operands to bfloat16 as the kernels do: the remaining
+  difference is FP32 accumulation order, checked at 1e-2 relative per
+  parameter block (the float64 network without bfloat16 differs by more, and
+  that difference is printed);
+* the features and frames against a brute-force numpy evaluation;
+* energy conservation of the integrator at zero friction;
+* the trained guess on a 256-water trajectory: same converged energy as the
+  reference guess, and fewer solver iterations.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+sys.path.insert(0, os.path.join(os.path.dirname(__file__), '..', 'oracle'))
+import mlp_oracle as M  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope='module')
+def mb():
+    import mbpol_openmm_plugin_amd as m
+    if m.device_count() < 1:
+        pytest.fail('no GPU visible to libmbpol_hip.so')
+    return m
+
+
+def config1_context(mb, precision='mixed', **kw):
+    waters, L = mb.systems.config1()
+    sysm = mb.forcefield.create_system(256, nonbonded_method='PME', box=L, ewald_error_tolerance=1e-5,
+                                       dipole_epsilon=1e-8, **kw)
+    ctx = mb.Context(sysm, precision=precision)
+    ctx.setPositions(mb.forcefield.water_positions_with_m(waters))
+    return ctx, L
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30))
+
+
+def test_network_gradient_and_forward_match_oracle(mb):
+    ctx, _ = config1_context(mb)
+    ctx.getState()
+    gm = mb.GuessModel(ctx, max_snapshots=1, seed=5)
+    rng = np.random.default_rng(11)
+    p = (rng.normal(size=M.NPARAM) * 0.25).astype(np.float32)
+    mean = rng.normal(size=M.NF).astype(np.float32)
+    inv_std = (1.0 / (0.5 + rng.random(M.NF))).astype(np.float32)
+    scale = np.float32(1.3)
+    gm.setParameters(p, np.concatenate([mean, inv_std, [scale]]).astype(np.float32))
+    for n in (100, 32, 257):  # partial waves and blocks
+        x = (rng.normal(size=(n, M.NF)) * 1.5 + mean).astype(np.float32)
+        t = rng.normal(size=(n, 3)).astype(np.float32)
+        g, loss = gm.gradient(x, t)
+        lo, go = M.loss_and_grad(x, t, p, mean.astype(np.float64), inv_std.astype(np.float64), float(scale), True)
+        l64, g64 = M.loss_and_grad(x, t, p, mean.astype(np.float64), inv_std.astype(np.float64), float(scale), False)
+        assert abs(loss - lo) <= 1e-4 * abs(lo), (n, loss, lo)
+        blocks = {'W1': (M.OFF_W1, M.OFF_B1), 'b1': (M.OFF_B1, M.OFF_W2), 'W2': (M.OFF_W2, M.OFF_B2),
+                  'b2': (M.OFF_B2, M.OFF_W3), 'W3': (M.OFF_W3, M.OFF_B3), 'b3': (M.OFF_B3, M.NPARAM)}
+        for name, (a, b) in blocks.items():
+            e = rel(g[a:b], go[a:b])
+            print('n %d %s rel err vs bf16 oracle %.2e, vs float64 network %.2e' % (n, name, e, rel(g[a:b], g64[a:b])),
+                  flush=True)
+            assert e <= 1e-2, (n, name, e)
+        y = gm.forward(x)
+        yo = M.forward(x, p, mean.astype(np.float64), inv_std.astype(np.float64), True)
+        print('n %d forward rel err %.2e (float64 network %.2e)' % (n, rel(y, yo),
+              rel(y, M.forward(x, p, mean.astype(np.float64), inv_std.astype(np.float64), False))), flush=True)
+        assert rel(y, yo) <= 1e-3
+
+
+def brute_features(pos, L, nw):
+    """48 features of the O, H1, H2 of every water, and their reference
+    directions (HOH bisector for O, O -> H for H), by brute force."""
+    lo, hi = 0.08, 0.6
+    step = (hi - lo) / 15
+    eta = 1.0 / (2 * step * step)
+    types = np.tile([0, 1, 1, 2], nw)
+    feats = np.zeros((3 * nw, 48))
+    refs = np.zeros((3 * nw, 3))
+    for a in range(3 * nw):
+        w, k = divmod(a, 3)
+        s = 4 * w + k
+        d = pos - pos[s]
+        d -= np.floor(d / L + 0.5) * L
+        r = np.sqrt((d ** 2).sum(1))
+        m = (r < hi) & (np.arange(len(pos)) != s)
+        rr, tt = r[m], types[m]
+        fc = 0.5 * (1 + np.cos(np.pi * rr / hi))
+        g = fc[:, None] * np.exp(-eta * (rr[:, None] - (lo + step * np.arange(16))[None, :]) ** 2)
+        for c in range(3):
+            feats[a, 16 * c:16 * c + 16] = g[tt == c].sum(0)
+        o = 4 * w
+        h = [pos[o + 1] - pos[o], pos[o + 2] - pos[o]]
+        h = [v - np.floor(v / L + 0.5) * L for v in h]
+        refs[a] = h[0] + h[1] if k == 0 else h[k - 1]
+    return feats, refs
+
+
+def test_features_match_brute_force(mb):
+    ctx, L = config1_context(mb)
+    ctx.getState()
+    gm = mb.GuessModel(ctx, max_snapshots=1)
+    f, R = gm.features()
+    fo, refs = brute_features(ctx.getPositions(), L, 256)
+    err = np.abs(f - fo).max() / np.abs(fo).max()
+    # frames: orthonormal, right-handed, e2 on the side of the reference direction, e3 normal to it
+    RRt = np.einsum('aij,akj->aik', R, R)
+    orth = np.abs(RRt - np.eye(3)).max()
+    det = np.linalg.det(R.astype(np.float64))
+    e2r = np.einsum('ai,ai->a', R[:, 1], refs)
+    e3r = np.abs(np.einsum('ai,ai->a', R[:, 2], refs)) / np.linalg.norm(refs, axis=1)
+    print('feature max rel err %.2e; frames: orthonormality %.1e, det min %.6f, e3.ref %.1e' %
+          (err, orth, det.min(), e3r.max()), flush=True)
+    assert err <= 1e-4
+    assert orth <= 1e-5 and np.abs(det - 1).max() <= 1e-5
+    assert np.all(e2r > 0) and e3r.max() <= 1e-5
+
+
+def nve_deviation(mb, ctx, dt, comp, seed=5, span=0.02):
+    md = mb.LangevinIntegrator(ctx, 300.0, 0.0, dt, seed=seed, components=comp)
+    e0, k0 = md.step(0)
+    n = int(round(span / dt))
+    devs = []
+    for _ in range(10):
+        e, k = md.step(n // 10)
+        devs.append(abs(e[5] + k - e0[5] - k0))
+    return max(devs), k0
+
+
+def test_langevin_zero_friction_conserves_energy(mb):
+    """At zero friction the integrator is velocity Verlet.  On the smooth
+    1-body term alone the energy error falls as dt^2 (halving the step cuts it
+    at least 2.5 times); with the 2-body, dispersion and electrostatic terms
+    it stays below 1e-3 of the kinetic energy over 20 fs at 0.2 fs.  Those
+    terms are truncated at their cutoffs as in the reference (no switch for the
+    dispersion and the real-space Ewald terms), so a small part of the error
+    does not fall with dt.  The 3-body term is left out: the reference's chain
+    trimer list drops a triplet when its middle O-O distance crosses the cutoff
+    while the triplet's energy is not zero there."""
+    from mbpol_openmm_plugin_amd.api import MBPOL_ALL, MBPOL_ONEBODY, MBPOL_THREEBODY
+    ctx, _ = config1_context(mb, precision='double')
+    warm = mb.LangevinIntegrator(ctx, 300.0, 10.0, 0.0005, seed=3)
+    warm.step(400)  # away from the lattice start
+    ctx.setPositions(warm.getPositions())
+    d1 = {dt: nve_deviation(mb, ctx, dt, MBPOL_ONEBODY) for dt in (0.0004, 0.0002)}
+    print('1-body only: max |E + K - start| over 20 fs: dt 0.4 fs %.5f, 0.2 fs %.5f kJ/mol' %
+          (d1[0.0004][0], d1[0.0002][0]), flush=True)
+    assert d1[0.0002][0] <= d1[0.0004][0] / 2.5
+    dev, k0 = nve_deviation(mb, ctx, 0.0002, MBPOL_ALL & ~MBPOL_THREEBODY)
+    print('all but 3-body, dt 0.2 fs: max |E + K - start| over 20 fs %.4f kJ/mol (K %.1f)' % (dev, k0), flush=True)
+    assert dev <= 1e-3 * k0
+
+
+def test_learned_guess_on_a_trajectory(mb):
+    """Train on 60 snapshots of a 300 K trajectory of 256 waters, then solve
+    later snapshots from the trained guess and from alpha * E_fixed (the
+    reference's start), by SOR.  Checked: the training loss falls well below
+    that of a zero output (3 in units of the output scale, one per component);
+    both starts reach the same converged energy; the trained start needs no
+    more iterations than alpha * E_fixed (one of slack).  The iteration counts
+    and the epsilon of each start are printed: the gain is small (about one
+    SOR iteration of ~58; none for the conjugate-gradient solver), because
+    radial features leave most of the direction of the correction open."""
+    ctx, _ = config1_context(mb)
+    ctx.setDipoleGuess('previous')
+    md = mb.LangevinIntegrator(ctx, 300.0, 10.0, 0.0005, seed=7)
+    md.step(400)
+    gm = mb.GuessModel(ctx, max_snapshots=60, seed=2)
+    for _ in range(60):
+        md.step(5)
+        gm.addSnapshot()
+    losses = gm.train(40, batch=1024, learning_rate=1e-3)
+    print('training loss', losses[0], '->', losses[-1], '(zero output: 3)', flush=True)
+    assert losses[-1] < 2.0
+    its = {'alpha_e': [], 'learned': []}
+    eps0 = {'alpha_e': [], 'learned': []}
+    for _ in range(4):
+        md.step(10)
+        e = {}
+        ctx.setDipoleSolver('sor')
+        for mode in ('alpha_e', 'learned'):
+            if mode == 'learned':
+                gm.attach(True)
+            else:
+                ctx.setDipoleGuess(mode)
+            e[mode] = ctx.evaluateResident()
+            its[mode].append(ctx.getStats()['iterations'])
+            ctx.setDipoleConvergence(1e3, 1)  # one step: the epsilon of the start
+            ctx.evaluateResident()
+            eps0[mode].append(ctx.getStats()['epsilon'])
+            ctx.setDipoleConvergence(1e-8, 200)
+        ctx.setDipoleSolver('pcg')
+        ctx.setDipoleGuess('previous')
+        assert abs(e['learned'][5] - e['alpha_e'][5]) <= 1e-7 * abs(e['alpha_e'][5])
+    print('SOR iterations alpha_e', its['alpha_e'], 'learned', its['learned'], flush=True)
+    print('epsilon of the start alpha_e', eps0['alpha_e'], 'learned', eps0['learned'], flush=True)
+    assert np.mean(its['learned']) <= np.mean(its['alpha_e']) + 1
