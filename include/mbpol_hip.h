@@ -124,8 +124,9 @@ int mbpol_system_set_electrostatics(mbpol_system* sys, int32_t n, const double* 
                                     int32_t method, double cutoff_nm, double ewald_alpha, const int32_t grid[3],
                                     double ewald_tol, int32_t max_iter, double target_eps,
                                     int32_t charge_redistribution, const double thole[5]);
-/* damped-C6 dispersion (python/mbpol.xml:37-108): per-particle type and molecule; pairs within one molecule and
- * particles whose type has an all-zero C6 row are excluded; c6/d6 are num_types x num_types
+/* damped-C6 dispersion (python/mbpol.xml:37-108): per-particle type (num_types <= 16) and molecule (0 <= m < 2^26);
+ * pairs within one molecule and particles whose type has an all-zero C6 row are excluded; c6/d6 are
+ * num_types x num_types
  * (kJ nm^6/mol, nm^-1); method as the two-body enum */
 int mbpol_system_set_dispersion(mbpol_system* sys, int32_t n, const int32_t* type, const int32_t* molecule,
                                 int32_t num_types, const double* c6, const double* d6, int32_t method,
@@ -167,6 +168,10 @@ int mbpol_context_set_dipole_convergence(mbpol_context* ctx, double target_eps, 
 /* MBPOL_SOLVER_SOR or MBPOL_SOLVER_PCG for the following evaluations (no counterpart in the reference, which
  * always iterates by SOR; the default depends on the precision mode given at creation) */
 int mbpol_context_set_solver(mbpol_context* ctx, int32_t solver);
+/* launch shapes the context picked by timing candidates on its first evaluation (no counterpart in the
+ * reference): "site_lanes" = lanes per site of the electrostatics row build (8, 16 or 32; 0 before the first
+ * evaluation) */
+int mbpol_context_get_tuning(const mbpol_context* ctx, const char* name, int32_t* value);
 /* copyParametersToContext for the electrostatics (same count required) */
 int mbpol_context_update_electrostatics(mbpol_context* ctx, int32_t n, const double* charge, const int32_t* molecule,
                                         const int32_t* atom_type, const double* damping, const double* polarity);

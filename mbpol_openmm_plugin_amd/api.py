@@ -506,6 +506,12 @@ class Context:
             raise MBPolError(1, 'unknown dipole solver %r (one of %s)' % (solver, ', '.join(SOLVERS)))
         _check(library().mbpol_context_set_solver(self._h, SOLVERS[solver]))
 
+    def getTuning(self, name):
+        """Launch shape picked by the context's autotuning (e.g. 'site_lanes')."""
+        v = ctypes.c_int32(0)
+        _check(library().mbpol_context_get_tuning(self._h, name.encode(), ctypes.byref(v)))
+        return v.value
+
     def setListMode(self, full_chain_list):
         _check(library().mbpol_context_set_list_mode(self._h, int(bool(full_chain_list))))
 

@@ -123,6 +123,7 @@ struct mbpol_context {
     hipStream_t side = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     mbpol::HostWorker worker;  // enqueues the short-range terms on `side`
+    bool evaluated = false;    // an evaluation has completed (launch shapes are tuned)
     int full_chain_list = 0;
     mbpol::EvalStats stats;
     mbpol::Timer timer;

@@ -19,6 +19,7 @@ namespace mbpol {
 struct Disp {
     DispGeom g{};
     DevBuf act, type, mol, c6, d6, cell_of, cell_count, cell_start, cell_fill, members, unsorted, eblk, npairs;
+    DevBuf cpos;  // positions, molecule and type of the active atoms in cell order
     int ncell = 1;
 };
 
@@ -74,6 +75,7 @@ int disp_create(mbpol_context* ctx) {
     if (int e = upv(D->d6, s.ds_d6)) return e;
     HIP_TRY(D->cell_of.ensure(sizeof(int) * std::max(1, D->g.na)));
     HIP_TRY(D->members.ensure(sizeof(int) * std::max(1, D->g.na)));
+    HIP_TRY(D->cpos.ensure(sizeof(double4) * std::max(1, D->g.na)));
     HIP_TRY(D->unsorted.ensure(sizeof(int) * std::max(1, D->g.na)));
     HIP_TRY(D->eblk.ensure(sizeof(double) * std::max(1, D->g.na)));
     HIP_TRY(D->npairs.ensure(sizeof(unsigned long long)));
@@ -101,11 +103,13 @@ int disp_evaluate(mbpol_context* ctx, hipStream_t st, u64* energy) {
         // members in index order: the pair sums, and so the forces, do not depend on scheduling
         launch_rank_sort(st, D->g.na, D->cell_of.as<int>(), D->cell_start.as<int>(), D->unsorted.as<int>(),
                          D->members.as<int>());
+        launch_disp_pack(st, D->g.na, D->members.as<int>(), D->act.as<int>(), pos, D->type.as<int>(),
+                         D->mol.as<int>(), D->cpos.as<double4>());
     }
     HIP_TRY(hipMemsetAsync(D->npairs.p, 0, sizeof(unsigned long long), st));
     launch_disp(st, ctx->precision == MBPOL_PRECISION_DOUBLE, D->g, D->act.as<int>(), pos, D->type.as<int>(),
                 D->mol.as<int>(), D->c6.as<double>(), D->d6.as<double>(), D->cell_of.as<int>(), D->cell_start.as<int>(),
-                D->members.as<int>(), ctx->force_fx.as<u64>(), D->eblk.as<double>(),
+                D->members.as<int>(), D->cpos.as<double4>(), ctx->force_fx.as<u64>(), D->eblk.as<double>(),
                 D->npairs.as<unsigned long long>());
     launch_energy_finish(st, D->g.na, D->eblk.as<double>(), energy);
     return MBPOL_OK;

@@ -18,8 +18,11 @@ struct DispGeom {
 void launch_disp_cells(hipStream_t s, int na, const int* act, const double* pos, d3 L, int3 nc, int* cell_of,
                        int* cell_count);
 void launch_disp_cell_fill(hipStream_t s, int na, const int* cell_of, const int* cell_start, int* cell_fill, int* members);
+// cpos[t] = (position, molecule * 16 + type) of atom act[members[t]] (types < 16)
+void launch_disp_pack(hipStream_t s, int na, const int* members, const int* act, const double* pos, const int* type,
+                      const int* mol, double4* cpos);
 void launch_disp(hipStream_t s, int precision_double, DispGeom g, const int* act, const double* pos, const int* type,
                  const int* mol, const double* c6, const double* d6, const int* cell_of, const int* cell_start,
-                 const int* members, u64* force, double* eblk, unsigned long long* npairs);
+                 const int* members, const double4* cpos, u64* force, double* eblk, unsigned long long* npairs);
 
 }  // namespace mbpol

@@ -7,6 +7,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
+#include <cstring>
 #include <map>
 #include <string>
 #include <vector>
@@ -41,8 +42,10 @@ struct Elec {
     std::vector<unsigned char> h_pol;  // host copy of (polarisability != 0)
     // cells for the neighbour rows
     DevBuf cell_of, cell_count, cell_start, cell_fill, members, unsorted;
+    DevBuf cpos;  // positions and polarisable flags in cell order
     int3 nc{1, 1, 1};
     int reach = 1, use_cells = 0;
+    int site_lanes = 0;  // lanes per site of the row build: 0 until tuned on the first evaluation
     // neighbour rows (ELL) and induced-field prefactors
     DevBuf nbP, cntP, nbQ, cntQ, pf, maxcnt;
     int capP = 0, capQ = 0;
@@ -321,6 +324,7 @@ int elec_create(mbpol_context* ctx) {
     HIP_TRY(E->eblk.ensure(sizeof(double) * (std::max(energy_blocks(n), sor_blocks(n)) + 1)));
     HIP_TRY(E->cell_of.ensure(sizeof(int) * std::max(n, 1)));
     HIP_TRY(E->members.ensure(sizeof(int) * std::max(n, 1)));
+    HIP_TRY(E->cpos.ensure(sizeof(double4) * std::max(n, 1)));
     HIP_TRY(E->unsorted.ensure(sizeof(int) * std::max(n, 1)));
     HIP_TRY(E->cntP.ensure(sizeof(int) * std::max(n, 1)));
     HIP_TRY(E->cntQ.ensure(sizeof(int) * std::max(n, 1)));
@@ -401,6 +405,58 @@ static ElecRows rows(const Elec* E) {
 
 // neighbour rows of every site (PME: within the cutoff; direct sum: every
 // other site), the induced-field prefactors, and the PME bins
+// Autotuning of the row build: the lane count per site (8, 16, 32) is timed on
+// this context's own first configuration, each candidate twice (the second
+// launch counts), and the fastest is kept.  Every candidate writes the same
+// rows.  MBPOL_HIP_SITE_LANES=8|16|32 fixes the choice instead.
+static int tune_site_lists(mbpol_context* ctx) {
+    Elec* E = ctx->elec;
+    if (const char* env = std::getenv("MBPOL_HIP_SITE_LANES")) {
+        const int v = std::atoi(env);
+        E->site_lanes = (v == 8 || v == 32) ? v : 16;
+        return MBPOL_OK;
+    }
+    if (!E->use_cells) {  // direct sum: rows of every site, no search to tune
+        E->site_lanes = 16;
+        return MBPOL_OK;
+    }
+    hipStream_t st = ctx->stream;
+    hipEvent_t a = nullptr, b = nullptr;
+    HIP_TRY(hipEventCreate(&a));
+    HIP_TRY(hipEventCreate(&b));
+    float best = 1e30f;
+    int choice = 16;
+    for (int lanes : {8, 16, 32}) {
+        float ms = 0.f;
+        for (int rep = 0; rep < 2; ++rep) {
+            HIP_TRY(hipMemsetAsync(E->maxcnt.p, 0, sizeof(int) * 4, st));
+            HIP_TRY(hipEventRecord(a, st));
+            launch_site_lists(st, lanes, E->n, ctx->pos.as<double>(), E->pol.as<double>(), E->P, E->nc, E->reach,
+                              E->use_cells, E->cell_of.as<int>(), E->cell_start.as<int>(), E->members.as<int>(),
+                              E->cpos.as<double4>(), E->capP, E->capQ, E->nbP.as<int>(), E->cntP.as<int>(),
+                              E->nbQ.as<int>(), E->cntQ.as<int>(), E->maxcnt.as<int>());
+            HIP_TRY(hipEventRecord(b, st));
+            HIP_TRY(hipEventSynchronize(b));
+            HIP_TRY(hipEventElapsedTime(&ms, a, b));
+        }
+        if (ms < best) {
+            best = ms;
+            choice = lanes;
+        }
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    E->site_lanes = choice;
+    return MBPOL_OK;
+}
+
+int elec_tuning(const mbpol_context* ctx, const char* name, int* value) {
+    const Elec* E = ctx->elec;
+    if (!E || std::strcmp(name, "site_lanes") != 0) return set_error(MBPOL_ERR_INVALID_ARGUMENT, "no tuning '%s'", name);
+    *value = E->site_lanes;
+    return MBPOL_OK;
+}
+
 static int build_rows(mbpol_context* ctx) {
     Elec* E = ctx->elec;
     hipStream_t st = ctx->stream;
@@ -416,6 +472,7 @@ static int build_rows(mbpol_context* ctx) {
                            E->unsorted.as<int>());
         launch_rank_sort(st, n, E->cell_of.as<int>(), E->cell_start.as<int>(), E->unsorted.as<int>(),
                          E->members.as<int>());
+        launch_site_pack(st, n, E->members.as<int>(), pos, E->pol.as<double>(), E->cpos.as<double4>());
     }
     int* h = reinterpret_cast<int*>(E->h_eps) + 8;
     for (int attempt = 0; attempt < 2; ++attempt) {
@@ -424,9 +481,10 @@ static int build_rows(mbpol_context* ctx) {
         HIP_TRY(E->pf.ensure(sizeof(double2) * std::max<size_t>(rp, 1)));
         HIP_TRY(E->nbQ.ensure(sizeof(int) * std::max<size_t>(rq, 1)));
         HIP_TRY(hipMemsetAsync(E->maxcnt.p, 0, sizeof(int) * 4, st));
-        launch_site_lists(st, n, pos, E->pol.as<double>(), E->P, E->nc, E->reach, E->use_cells, E->cell_of.as<int>(),
-                          E->cell_start.as<int>(), E->members.as<int>(), E->capP, E->capQ, E->nbP.as<int>(),
-                          E->cntP.as<int>(), E->nbQ.as<int>(), E->cntQ.as<int>(), E->maxcnt.as<int>());
+        launch_site_lists(st, E->site_lanes > 0 ? E->site_lanes : 16, n, pos, E->pol.as<double>(), E->P, E->nc, E->reach,
+                          E->use_cells, E->cell_of.as<int>(), E->cell_start.as<int>(), E->members.as<int>(),
+                          E->cpos.as<double4>(), E->capP, E->capQ, E->nbP.as<int>(), E->cntP.as<int>(),
+                          E->nbQ.as<int>(), E->cntQ.as<int>(), E->maxcnt.as<int>());
         HIP_TRY(hipMemcpyAsync(h, E->maxcnt.p, sizeof(int) * 4, hipMemcpyDeviceToHost, st));
         HIP_TRY(hipStreamSynchronize(st));
         if (h[0] <= E->capP && h[1] <= E->capQ) break;
@@ -434,6 +492,8 @@ static int build_rows(mbpol_context* ctx) {
         E->capP = std::max(E->capP, (h[0] + h[0] / 4 + 63) / 32 * 32);
         E->capQ = std::max(E->capQ, (h[1] + h[1] / 4 + 63) / 32 * 32);
     }
+    if (E->site_lanes == 0)
+        if (int e = tune_site_lists(ctx)) return e;
     ctx->stats.elec_pairs = h[2] / 2;
     ctx->stats.pol_pairs = h[3];
     launch_prefactors(st, E->npol, E->polidx.as<int>(), pos, site_params(E), E->P, E->capP, E->nbP.as<int>(),
@@ -811,6 +871,15 @@ int mbpol_context_set_dipole_convergence(mbpol_context* ctx, double target_eps, 
         return set_error(MBPOL_ERR_INVALID_ARGUMENT, "target epsilon must be > 0 and max iterations >= 1");
     ctx->sys.el_eps = target_eps;
     ctx->sys.el_max_iter = max_iter;
+    return MBPOL_OK;
+}
+
+int mbpol_context_get_tuning(const mbpol_context* ctx, const char* name, int32_t* value) {
+    using namespace mbpol;
+    if (!ctx || !name || !value) return set_error(MBPOL_ERR_INVALID_ARGUMENT, "null argument");
+    int v = 0;
+    if (int e = elec_tuning(ctx, name, &v)) return e;
+    *value = v;
     return MBPOL_OK;
 }
 

@@ -57,9 +57,13 @@ void launch_bucket_fill(hipStream_t s, int n, const int* bucket_of, const int* b
 // members_out: every bucket's items in ascending order (members_in: same buckets, any order)
 void launch_rank_sort(hipStream_t s, int n, const int* bucket_of, const int* bucket_start, const int* members_in,
                       int* members_out);
-void launch_site_lists(hipStream_t s, int n, const double* pos, const double* pol, ElecParams P, int3 nc, int reach,
-                       int use_cells, const int* cell_of, const int* cell_start, const int* members, int capP, int capQ,
-                       int* nbP, int* cntP, int* nbQ, int* cntQ, int* maxcnt);
+// cpos[t] = (position, polarisable ? 1 : 0) of site members[t]
+void launch_site_pack(hipStream_t s, int n, const int* members, const double* pos, const double* pol, double4* cpos);
+// lanes per site: 8, 16 or 32 (the rows are the same for every choice); cpos used when use_cells
+void launch_site_lists(hipStream_t s, int lanes, int n, const double* pos, const double* pol, ElecParams P, int3 nc,
+                       int reach, int use_cells, const int* cell_of, const int* cell_start, const int* members,
+                       const double4* cpos, int capP, int capQ, int* nbP, int* cntP, int* nbQ, int* cntQ,
+                       int* maxcnt);
 void launch_prefactors(hipStream_t s, int npol, const int* polidx, const double* pos, SiteParams S, ElecParams P,
                        int capP, const int* nbP, const int* cntP, double2* pf);
 void launch_fixed_field(hipStream_t s, int npol, const int* polidx, const double* pos, SiteParams S, ElecParams P,

@@ -36,13 +36,24 @@ __global__ void k_disp_cell_fill(int na, const int* __restrict__ cell_of, const 
     members[cell_start[c] + atomicAdd(&cell_fill[c], 1)] = a;
 }
 
+// positions and molecule * 16 + type of the active atoms in cell order (members[t] indexes act)
+__global__ void k_disp_pack(int na, const int* __restrict__ members, const int* __restrict__ act,
+                            const double* __restrict__ pos, const int* __restrict__ type, const int* __restrict__ mol,
+                            double4* __restrict__ cpos) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= na) return;
+    const int j = act[members[t]];
+    cpos[t] = make_double4(pos[3 * j], pos[3 * j + 1], pos[3 * j + 2], (double)(mol[j] * 16 + type[j]));
+}
+
 template <typename T>
 __global__ void __launch_bounds__(64) k_disp(DispGeom g, const int* __restrict__ act, const double* __restrict__ pos,
                                              const int* __restrict__ type, const int* __restrict__ mol,
                                              const double* __restrict__ c6, const double* __restrict__ d6,
                                              const int* __restrict__ cell_of, const int* __restrict__ cell_start,
-                                             const int* __restrict__ members, u64* __restrict__ force,
-                                             double* __restrict__ eblk, unsigned long long* __restrict__ npairs) {
+                                             const int* __restrict__ members, const double4* __restrict__ cpos,
+                                             u64* __restrict__ force, double* __restrict__ eblk,
+                                             unsigned long long* __restrict__ npairs) {
     const int a = blockIdx.x, lane = threadIdx.x;
     const int i = act[a];
     const double xi = pos[3 * i], yi = pos[3 * i + 1], zi = pos[3 * i + 2];
@@ -50,10 +61,11 @@ __global__ void __launch_bounds__(64) k_disp(DispGeom g, const int* __restrict__
     const double rc2 = g.cutoff * g.cutoff;
     double fx = 0, fy = 0, fz = 0, en = 0;
     unsigned long long cnt = 0;
-    auto visit = [&](int b) {
-        const int j = act[b];
-        if (j == i || mol[j] == mi) return;
-        double dx = xi - pos[3 * j], dy = yi - pos[3 * j + 1], dz = zi - pos[3 * j + 2];
+    // q: position of the candidate and its molecule * 16 + type (cell-ordered copy when cells are used)
+    auto visit = [&](const double4& q) {
+        const int key = (int)q.w, mj = key >> 4, tj = key & 15;
+        if (mj == mi) return;  // same molecule, the atom itself included
+        double dx = xi - q.x, dy = yi - q.y, dz = zi - q.z;
         if (g.periodic) {
             dx -= floor(dx / g.L.x + 0.5) * g.L.x;
             dy -= floor(dy / g.L.y + 0.5) * g.L.y;
@@ -61,7 +73,6 @@ __global__ void __launch_bounds__(64) k_disp(DispGeom g, const int* __restrict__
         }
         const double r2d = dx * dx + dy * dy + dz * dz;
         if (g.method != 0 && r2d >= rc2) return;
-        const int tj = type[j];
         const T C = (T)c6[ti + g.ntypes * tj], D = (T)d6[ti + g.ntypes * tj];
         const T r2 = (T)r2d, r = sqrt(r2), x = D * r;
         const T s = (T)1 + ((T)1 + ((T)1 + ((T)1 + ((T)1 + ((T)1 + x / (T)6) * x / (T)5) * x / (T)4) * x / (T)3) * x / (T)2) * x;
@@ -78,20 +89,35 @@ __global__ void __launch_bounds__(64) k_disp(DispGeom g, const int* __restrict__
         ++cnt;
     };
     if (!g.use_cells) {
-        for (int b = lane; b < g.na; b += 64) visit(b);
+        for (int b = lane; b < g.na; b += 64) {
+            const int j = act[b];
+            visit(make_double4(pos[3 * j], pos[3 * j + 1], pos[3 * j + 2], (double)(mol[j] * 16 + type[j])));
+        }
     } else {
         const int cell = cell_of[a];
         const int cz = cell % g.nc.z, cy = (cell / g.nc.z) % g.nc.y, cx = cell / (g.nc.z * g.nc.y);
+        // the three cells along z of each (X, Y) column are consecutive in the cell order:
+        // one or two contiguous member ranges
+        auto scan = [&](int base, int za, int zb) {
+            const int e = cell_start[base + zb + 1];
+            for (int t = cell_start[base + za] + lane; t < e; t += 64) visit(cpos[t]);
+        };
         for (int ax = -1; ax <= 1; ++ax)
-            for (int ay = -1; ay <= 1; ++ay)
-                for (int az = -1; az <= 1; ++az) {
-                    int X = cx + ax, Y = cy + ay, Z = cz + az;
-                    X = X < 0 ? X + g.nc.x : (X >= g.nc.x ? X - g.nc.x : X);
-                    Y = Y < 0 ? Y + g.nc.y : (Y >= g.nc.y ? Y - g.nc.y : Y);
-                    Z = Z < 0 ? Z + g.nc.z : (Z >= g.nc.z ? Z - g.nc.z : Z);
-                    const int cc = (X * g.nc.y + Y) * g.nc.z + Z;
-                    for (int t = cell_start[cc] + lane; t < cell_start[cc + 1]; t += 64) visit(members[t]);
+            for (int ay = -1; ay <= 1; ++ay) {
+                int X = cx + ax, Y = cy + ay;
+                X = X < 0 ? X + g.nc.x : (X >= g.nc.x ? X - g.nc.x : X);
+                Y = Y < 0 ? Y + g.nc.y : (Y >= g.nc.y ? Y - g.nc.y : Y);
+                const int base = (X * g.nc.y + Y) * g.nc.z, z0 = cz - 1, z1 = cz + 1;
+                if (z0 < 0) {
+                    scan(base, z0 + g.nc.z, g.nc.z - 1);
+                    scan(base, 0, z1);
+                } else if (z1 >= g.nc.z) {
+                    scan(base, z0, g.nc.z - 1);
+                    scan(base, 0, z1 - g.nc.z);
+                } else {
+                    scan(base, z0, z1);
                 }
+            }
     }
     fx = wave_sum(fx);
     fy = wave_sum(fy);
@@ -115,16 +141,20 @@ void launch_disp_cells(hipStream_t s, int na, const int* act, const double* pos,
 void launch_disp_cell_fill(hipStream_t s, int na, const int* cell_of, const int* cell_start, int* cell_fill, int* members) {
     hipLaunchKernelGGL(k_disp_cell_fill, nblk(na, 256), 256, 0, s, na, cell_of, cell_start, cell_fill, members);
 }
+void launch_disp_pack(hipStream_t s, int na, const int* members, const int* act, const double* pos, const int* type,
+                      const int* mol, double4* cpos) {
+    if (na > 0) hipLaunchKernelGGL(k_disp_pack, nblk(na, 256), 256, 0, s, na, members, act, pos, type, mol, cpos);
+}
 void launch_disp(hipStream_t s, int precision_double, DispGeom g, const int* act, const double* pos, const int* type,
                  const int* mol, const double* c6, const double* d6, const int* cell_of, const int* cell_start,
-                 const int* members, u64* force, double* eblk, unsigned long long* npairs) {
+                 const int* members, const double4* cpos, u64* force, double* eblk, unsigned long long* npairs) {
     if (g.na <= 0) return;
     if (precision_double)
         hipLaunchKernelGGL(k_disp<double>, g.na, 64, 0, s, g, act, pos, type, mol, c6, d6, cell_of, cell_start, members,
-                           force, eblk, npairs);
+                           cpos, force, eblk, npairs);
     else
         hipLaunchKernelGGL(k_disp<float>, g.na, 64, 0, s, g, act, pos, type, mol, c6, d6, cell_of, cell_start, members,
-                           force, eblk, npairs);
+                           cpos, force, eblk, npairs);
 }
 
 }  // namespace mbpol

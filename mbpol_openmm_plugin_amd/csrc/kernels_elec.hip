@@ -204,11 +204,22 @@ __global__ void k_rank_sort(int n, const int* __restrict__ bucket_of, const int*
 // detect an overflow (maxcnt[0..1]) and rebuild with larger rows; maxcnt[2]
 // accumulates the number of directed pairs, maxcnt[3] the P entries of the
 // polarisable sites (the rows the dipole iteration streams).
+// positions and polarisable flags in cell order: the row build reads its
+// candidates from consecutive entries instead of gathering them by site index
+__global__ void k_site_pack(int n, const int* __restrict__ members, const double* __restrict__ pos,
+                            const double* __restrict__ pol, double4* __restrict__ cpos) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int j = members[t];
+    cpos[t] = make_double4(pos[3 * j], pos[3 * j + 1], pos[3 * j + 2], pol[j] != 0.0 ? 1.0 : 0.0);
+}
+
 template <int G>
 __global__ void __launch_bounds__(256) k_site_lists(int n, const double* __restrict__ pos, const double* __restrict__ pol,
                                                     ElecParams P, int3 nc, int reach, int use_cells,
                                                     const int* __restrict__ cell_of, const int* __restrict__ cell_start,
-                                                    const int* __restrict__ members, int capP, int capQ,
+                                                    const int* __restrict__ members,
+                                                    const double4* __restrict__ cpos, int capP, int capQ,
                                                     int* __restrict__ nbP, int* __restrict__ cntP,
                                                     int* __restrict__ nbQ, int* __restrict__ cntQ,
                                                     int* __restrict__ maxcnt) {
@@ -220,13 +231,14 @@ __global__ void __launch_bounds__(256) k_site_lists(int n, const double* __restr
     int cP = 0, cQ = 0;
     int* rowP = nbP + (size_t)i * capP;
     int* rowQ = nbQ + (size_t)i * capQ;
-    auto consider = [&](int j, bool active) {
+    // q: position of j and 1 if j is polarisable (from the cell-ordered copy when cells are used)
+    auto consider = [&](int j, bool active, const double4& q) {
         bool ok = false, pj = false;
         if (active && j != i) {
-            double dx = pos[3 * j] - xi, dy = pos[3 * j + 1] - yi, dz = pos[3 * j + 2] - zi;
+            double dx = q.x - xi, dy = q.y - yi, dz = q.z - zi;
             min_image(dx, dy, dz, P);
             ok = dx * dx + dy * dy + dz * dz <= rc2;
-            pj = pol[j] != 0.0;
+            pj = q.w != 0.0;
         }
         const bool inP = ok && pj, inQ = ok && !pj;
         const unsigned long long bP = __ballot(inP) & gb, bQ = __ballot(inQ) & gb;
@@ -242,24 +254,43 @@ __global__ void __launch_bounds__(256) k_site_lists(int n, const double* __restr
         cQ += __popcll(bQ);
     };
     if (!use_cells) {
-        for (int b = 0; b < n; b += G) consider(b + lane, b + lane < n);
+        for (int b = 0; b < n; b += G) {
+            const int j = b + lane;
+            const bool act = j < n;
+            const double4 q = act ? make_double4(pos[3 * j], pos[3 * j + 1], pos[3 * j + 2], pol[j] != 0.0 ? 1.0 : 0.0)
+                                  : make_double4(0.0, 0.0, 0.0, 0.0);
+            consider(j, act, q);
+        }
     } else {
+        // cells (X, Y, Z - reach .. Z + reach) are consecutive in the cell order, so each
+        // (X, Y) column is one or two contiguous member ranges, visited in the same order
+        // as cell by cell
         const int cell = cell_of[i];
         const int cz = cell % nc.z, cy = (cell / nc.z) % nc.y, cx = cell / (nc.z * nc.y);
+        auto scan = [&](int base, int za, int zb) {
+            const int b0 = cell_start[base + za], b1 = cell_start[base + zb + 1];
+            for (int t = b0; t < b1; t += G) {
+                const int tt = t + lane;
+                const bool act = tt < b1;
+                consider(act ? members[tt] : -1, act, act ? cpos[tt] : make_double4(0.0, 0.0, 0.0, 0.0));
+            }
+        };
         for (int ax = -reach; ax <= reach; ++ax)
-            for (int ay = -reach; ay <= reach; ++ay)
-                for (int az = -reach; az <= reach; ++az) {
-                    int X = cx + ax, Y = cy + ay, Z = cz + az;
-                    X = X < 0 ? X + nc.x : (X >= nc.x ? X - nc.x : X);
-                    Y = Y < 0 ? Y + nc.y : (Y >= nc.y ? Y - nc.y : Y);
-                    Z = Z < 0 ? Z + nc.z : (Z >= nc.z ? Z - nc.z : Z);
-                    const int cc = (X * nc.y + Y) * nc.z + Z;
-                    const int b0 = cell_start[cc], b1 = cell_start[cc + 1];
-                    for (int t = b0; t < b1; t += G) {
-                        const int tt = t + lane;
-                        consider(tt < b1 ? members[tt] : -1, tt < b1);
-                    }
+            for (int ay = -reach; ay <= reach; ++ay) {
+                int X = cx + ax, Y = cy + ay;
+                X = X < 0 ? X + nc.x : (X >= nc.x ? X - nc.x : X);
+                Y = Y < 0 ? Y + nc.y : (Y >= nc.y ? Y - nc.y : Y);
+                const int base = (X * nc.y + Y) * nc.z, z0 = cz - reach, z1 = cz + reach;
+                if (z0 < 0) {
+                    scan(base, z0 + nc.z, nc.z - 1);
+                    scan(base, 0, z1);
+                } else if (z1 >= nc.z) {
+                    scan(base, z0, nc.z - 1);
+                    scan(base, 0, z1 - nc.z);
+                } else {
+                    scan(base, z0, z1);
                 }
+            }
     }
     if (lane == 0) {
         cntP[i] = cP;
@@ -1450,7 +1481,7 @@ __global__ void k_potential_at_points(int npts, const double* __restrict__ pts, 
 // launchers
 // ---------------------------------------------------------------------------
 static inline int nblk(long long n, int b) { return (int)((n + b - 1) / b); }
-constexpr int kG = 16;   // lanes per site: list build, fixed field, energy
+constexpr int kG = 16;   // lanes per site: fixed field, energy
 constexpr int kGI = 32;  // lanes per site in the iteration (>= 25 for the PME gather)
 
 void launch_site_charges_copy(hipStream_t s, int n, const double* q0, double* q) {
@@ -1467,11 +1498,22 @@ void launch_rank_sort(hipStream_t s, int n, const int* bucket_of, const int* buc
                       int* members_out) {
     hipLaunchKernelGGL(k_rank_sort, nblk(n, 256), 256, 0, s, n, bucket_of, bucket_start, members_in, members_out);
 }
-void launch_site_lists(hipStream_t s, int n, const double* pos, const double* pol, ElecParams P, int3 nc, int reach,
-                       int use_cells, const int* cell_of, const int* cell_start, const int* members, int capP, int capQ,
-                       int* nbP, int* cntP, int* nbQ, int* cntQ, int* maxcnt) {
-    hipLaunchKernelGGL(k_site_lists<kG>, nblk((long long)n * kG, 256), 256, 0, s, n, pos, pol, P, nc, reach, use_cells,
-                       cell_of, cell_start, members, capP, capQ, nbP, cntP, nbQ, cntQ, maxcnt);
+void launch_site_pack(hipStream_t s, int n, const int* members, const double* pos, const double* pol, double4* cpos) {
+    if (n > 0) hipLaunchKernelGGL(k_site_pack, nblk(n, 256), 256, 0, s, n, members, pos, pol, cpos);
+}
+void launch_site_lists(hipStream_t s, int lanes, int n, const double* pos, const double* pol, ElecParams P, int3 nc,
+                       int reach, int use_cells, const int* cell_of, const int* cell_start, const int* members,
+                       const double4* cpos, int capP, int capQ, int* nbP, int* cntP, int* nbQ, int* cntQ,
+                       int* maxcnt) {
+#define MBPOL_SITE_LISTS(G)                                                                                          \
+    hipLaunchKernelGGL(k_site_lists<G>, nblk((long long)n * G, 256), 256, 0, s, n, pos, pol, P, nc, reach, use_cells, \
+                       cell_of, cell_start, members, cpos, capP, capQ, nbP, cntP, nbQ, cntQ, maxcnt)
+    switch (lanes) {
+        case 8: MBPOL_SITE_LISTS(8); break;
+        case 32: MBPOL_SITE_LISTS(32); break;
+        default: MBPOL_SITE_LISTS(16); break;
+    }
+#undef MBPOL_SITE_LISTS
 }
 void launch_prefactors(hipStream_t s, int npol, const int* polidx, const double* pos, SiteParams S, ElecParams P,
                        int capP, const int* nbP, const int* cntP, double2* pf) {

@@ -94,19 +94,31 @@ __device__ __forceinline__ void for_candidates(int m, int lane, int n, const Lis
         for (int b = 0; b < n; b += G) f(b + lane, b + lane < n);
         return;
     }
+    // the three cells along z of each (X, Y) column are consecutive in the cell order:
+    // one or two contiguous member ranges, visited in the same order as cell by cell
     const int c = cell_of[m];
     const int cz = c % g.nc.z, cy = (c / g.nc.z) % g.nc.y, cx = c / (g.nc.z * g.nc.y);
+    auto scan = [&](int base, int za, int zb) {
+        const int b = cell_start[base + za], e = cell_start[base + zb + 1];
+        for (int t = b; t < e; t += G) {
+            const bool act = t + lane < e;
+            f(act ? cell_members[t + lane] : -1, act);
+        }
+    };
     for (int dx = -1; dx <= 1; ++dx)
-        for (int dy = -1; dy <= 1; ++dy)
-            for (int dz = -1; dz <= 1; ++dz) {
-                const int cc = (wrap_cell(cx + dx, g.nc.x) * g.nc.y + wrap_cell(cy + dy, g.nc.y)) * g.nc.z +
-                               wrap_cell(cz + dz, g.nc.z);
-                const int b = cell_start[cc], e = cell_start[cc + 1];
-                for (int t = b; t < e; t += G) {
-                    const bool act = t + lane < e;
-                    f(act ? cell_members[t + lane] : -1, act);
-                }
+        for (int dy = -1; dy <= 1; ++dy) {
+            const int base = (wrap_cell(cx + dx, g.nc.x) * g.nc.y + wrap_cell(cy + dy, g.nc.y)) * g.nc.z;
+            const int z0 = cz - 1, z1 = cz + 1;
+            if (z0 < 0) {
+                scan(base, z0 + g.nc.z, g.nc.z - 1);
+                scan(base, 0, z1);
+            } else if (z1 >= g.nc.z) {
+                scan(base, z0, g.nc.z - 1);
+                scan(base, 0, z1 - g.nc.z);
+            } else {
+                scan(base, z0, z1);
             }
+        }
 }
 
 constexpr int kGL = 16;  // lanes per molecule in the pair passes
