@@ -19,6 +19,7 @@ namespace mbpol {
 struct Disp {
     DispGeom g{};
     DevBuf act, type, mol, c6, d6, cell_of, cell_count, cell_start, cell_fill, members, unsorted, eblk, npairs;
+    DevBuf cnt;  // pairs within the cutoff per active atom
     DevBuf cpos;  // positions, molecule and type of the active atoms in cell order
     int ncell = 1;
 };
@@ -79,6 +80,7 @@ int disp_create(mbpol_context* ctx) {
     HIP_TRY(D->unsorted.ensure(sizeof(int) * std::max(1, D->g.na)));
     HIP_TRY(D->eblk.ensure(sizeof(double) * std::max(1, D->g.na)));
     HIP_TRY(D->npairs.ensure(sizeof(unsigned long long)));
+    HIP_TRY(D->cnt.ensure(sizeof(unsigned long long) * std::max(1, D->g.na)));
     return disp_set_box(ctx);
 }
 
@@ -106,12 +108,22 @@ int disp_evaluate(mbpol_context* ctx, hipStream_t st, u64* energy) {
         launch_disp_pack(st, D->g.na, D->members.as<int>(), D->act.as<int>(), pos, D->type.as<int>(),
                          D->mol.as<int>(), D->cpos.as<double4>());
     }
-    HIP_TRY(hipMemsetAsync(D->npairs.p, 0, sizeof(unsigned long long), st));
     launch_disp(st, ctx->precision == MBPOL_PRECISION_DOUBLE, D->g, D->act.as<int>(), pos, D->type.as<int>(),
                 D->mol.as<int>(), D->c6.as<double>(), D->d6.as<double>(), D->cell_of.as<int>(), D->cell_start.as<int>(),
                 D->members.as<int>(), D->cpos.as<double4>(), ctx->force_fx.as<u64>(), D->eblk.as<double>(),
-                D->npairs.as<unsigned long long>());
+                D->cnt.as<unsigned long long>());
+    launch_count_total(st, D->g.na, D->cnt.as<unsigned long long>(), D->npairs.as<unsigned long long>());
     launch_energy_finish(st, D->g.na, D->eblk.as<double>(), energy);
+    return MBPOL_OK;
+}
+
+int disp_pair_count(const mbpol_context* ctx, long long* pairs) {
+    const Disp* D = ctx->disp;
+    *pairs = 0;
+    if (!D || D->g.na == 0) return MBPOL_OK;
+    unsigned long long v = 0;
+    HIP_TRY(hipMemcpy(&v, D->npairs.p, sizeof(v), hipMemcpyDeviceToHost));
+    *pairs = (long long)(v / 2);  // each pair is counted from both atoms
     return MBPOL_OK;
 }
 

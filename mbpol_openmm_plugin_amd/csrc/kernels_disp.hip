@@ -53,7 +53,7 @@ __global__ void __launch_bounds__(64) k_disp(DispGeom g, const int* __restrict__
                                              const int* __restrict__ cell_of, const int* __restrict__ cell_start,
                                              const int* __restrict__ members, const double4* __restrict__ cpos,
                                              u64* __restrict__ force, double* __restrict__ eblk,
-                                             unsigned long long* __restrict__ npairs) {
+                                             unsigned long long* __restrict__ cnt_out) {
     const int a = blockIdx.x, lane = threadIdx.x;
     const int i = act[a];
     const double xi = pos[3 * i], yi = pos[3 * i + 1], zi = pos[3 * i + 2];
@@ -128,7 +128,7 @@ __global__ void __launch_bounds__(64) k_disp(DispGeom g, const int* __restrict__
     if (lane == 0) {
         add_force(force, i, fx, fy, fz);
         eblk[a] = 0.5 * en;
-        atomicAdd(npairs, cnt);
+        cnt_out[a] = cnt;
     }
 }
 
@@ -141,20 +141,37 @@ void launch_disp_cells(hipStream_t s, int na, const int* act, const double* pos,
 void launch_disp_cell_fill(hipStream_t s, int na, const int* cell_of, const int* cell_start, int* cell_fill, int* members) {
     hipLaunchKernelGGL(k_disp_cell_fill, nblk(na, 256), 256, 0, s, na, cell_of, cell_start, cell_fill, members);
 }
+// one block: total of the per-atom pair counts (each pair counted from both atoms)
+__global__ void k_count_total(int na, const unsigned long long* __restrict__ cnt, unsigned long long* __restrict__ out) {
+    __shared__ unsigned long long red[256];
+    unsigned long long s = 0;
+    for (int a = threadIdx.x; a < na; a += 256) s += cnt[a];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int k = 128; k > 0; k >>= 1) {
+        if (threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = red[0];
+}
+
+void launch_count_total(hipStream_t s, int na, const unsigned long long* cnt, unsigned long long* out) {
+    hipLaunchKernelGGL(k_count_total, 1, 256, 0, s, na, cnt, out);
+}
 void launch_disp_pack(hipStream_t s, int na, const int* members, const int* act, const double* pos, const int* type,
                       const int* mol, double4* cpos) {
     if (na > 0) hipLaunchKernelGGL(k_disp_pack, nblk(na, 256), 256, 0, s, na, members, act, pos, type, mol, cpos);
 }
 void launch_disp(hipStream_t s, int precision_double, DispGeom g, const int* act, const double* pos, const int* type,
                  const int* mol, const double* c6, const double* d6, const int* cell_of, const int* cell_start,
-                 const int* members, const double4* cpos, u64* force, double* eblk, unsigned long long* npairs) {
+                 const int* members, const double4* cpos, u64* force, double* eblk, unsigned long long* cnt) {
     if (g.na <= 0) return;
     if (precision_double)
         hipLaunchKernelGGL(k_disp<double>, g.na, 64, 0, s, g, act, pos, type, mol, c6, d6, cell_of, cell_start, members,
-                           cpos, force, eblk, npairs);
+                           cpos, force, eblk, cnt);
     else
         hipLaunchKernelGGL(k_disp<float>, g.na, 64, 0, s, g, act, pos, type, mol, c6, d6, cell_of, cell_start, members,
-                           cpos, force, eblk, npairs);
+                           cpos, force, eblk, cnt);
 }
 
 }  // namespace mbpol

@@ -224,11 +224,11 @@ __global__ void __launch_bounds__(256) k_site_lists(int n, const double* __restr
                                                     int* __restrict__ nbQ, int* __restrict__ cntQ,
                                                     int* __restrict__ maxcnt) {
     const int i = (blockIdx.x * blockDim.x + threadIdx.x) / G, lane = threadIdx.x % G;
-    if (i >= n) return;  // uniform over the group
+    int cP = 0, cQ = 0;
+    if (i < n) {  // uniform over the group
     const double xi = pos[3 * i], yi = pos[3 * i + 1], zi = pos[3 * i + 2];
     const double rc2 = P.cutoff * P.cutoff;
     const unsigned long long gb = group_bits<G>(), below = lanes_below();
-    int cP = 0, cQ = 0;
     int* rowP = nbP + (size_t)i * capP;
     int* rowQ = nbQ + (size_t)i * capQ;
     // q: position of j and 1 if j is polarisable (from the cell-ordered copy when cells are used)
@@ -295,10 +295,24 @@ __global__ void __launch_bounds__(256) k_site_lists(int n, const double* __restr
     if (lane == 0) {
         cntP[i] = cP;
         cntQ[i] = cQ;
-        atomicMax(&maxcnt[0], cP);
-        atomicMax(&maxcnt[1], cQ);
-        atomicAdd(&maxcnt[2], cP + cQ);
-        if (pol[i] != 0.0) atomicAdd(&maxcnt[3], cP);
+    }
+    }
+    // row maxima and totals: combined in LDS, one set of global atomics per block
+    __shared__ int red[4];
+    if (threadIdx.x < 4) red[threadIdx.x] = 0;
+    __syncthreads();
+    if (i < n && lane == 0) {
+        atomicMax(&red[0], cP);
+        atomicMax(&red[1], cQ);
+        atomicAdd(&red[2], cP + cQ);
+        if (pol[i] != 0.0) atomicAdd(&red[3], cP);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        atomicMax(&maxcnt[0], red[0]);
+        atomicMax(&maxcnt[1], red[1]);
+        atomicAdd(&maxcnt[2], red[2]);
+        atomicAdd(&maxcnt[3], red[3]);
     }
 }
 

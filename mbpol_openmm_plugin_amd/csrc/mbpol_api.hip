@@ -774,7 +774,12 @@ int mbpol_context_get_stats(const mbpol_context* ctx, mbpol_eval_stats* out) {
     out->num_dimers = ctx->stats.dimers;
     out->num_trimers = ctx->stats.trimers;
     out->num_elec_pairs = ctx->stats.elec_pairs;
-    out->num_disp_pairs = ctx->stats.disp_pairs;
+    long long dp = 0;
+    if (ctx->disp) {
+        (void)hipSetDevice(ctx->device);
+        if (int e = disp_pair_count(ctx, &dp)) return e;
+    }
+    out->num_disp_pairs = dp;
     out->num_pol_pairs = ctx->stats.pol_pairs;
     return MBPOL_OK;
 }
