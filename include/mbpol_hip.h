@@ -132,6 +132,13 @@ int mbpol_system_set_dispersion(mbpol_system* sys, int32_t n, const int32_t* typ
                                 int32_t num_types, const double* c6, const double* d6, int32_t method,
                                 double cutoff_nm);
 
+/* independent groups of waters evaluated together (no counterpart in the reference, which evaluates one system per
+ * Context): waters [water_start[g], water_start[g+1]) form group g (water w = particles 4w..4w+3, O H1 H2 M; the
+ * 1-, 2- and 3-body molecule lists are the waters in order; non-periodic methods only).  No interaction crosses a
+ * group; the dipole solve (conjugate gradients) runs until every group meets the target; energies are also
+ * reported per group (mbpol_context_get_group_energies).  ngroups = 0 removes the groups. */
+int mbpol_system_set_groups(mbpol_system* sys, int32_t ngroups, const int32_t* water_start);
+
 /* ---- context (device side) ---- */
 int mbpol_context_create(const mbpol_system* sys, int32_t precision, int32_t device, mbpol_context** out);
 void mbpol_context_destroy(mbpol_context* ctx);
@@ -157,6 +164,8 @@ int mbpol_context_set_timing(mbpol_context* ctx, int32_t enabled);
 int mbpol_context_get_timing(mbpol_context* ctx, const char* name, double* total_ms, int64_t* count);
 int mbpol_context_set_box(mbpol_context* ctx, const double box_nm[3]);
 int mbpol_context_get_stats(const mbpol_context* ctx, mbpol_eval_stats* stats);
+/* energies of every group of the last evaluate: ngroups x {1B, 2B, 3B, dispersion, electrostatics, total} */
+int mbpol_context_get_group_energies(const mbpol_context* ctx, double* energies);
 /* converged induced dipoles of the last evaluate (electrostatics particles x 3, e nm) */
 int mbpol_context_get_dipoles(const mbpol_context* ctx, double* dipoles);
 /* charges used by the last evaluate (after redistribution) */

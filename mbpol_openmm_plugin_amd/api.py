@@ -335,6 +335,7 @@ class System:
         self._vsites = {}
         self._forces = []
         self._box = None
+        self._groups = None
 
     def addParticle(self, mass):
         self._masses.append(float(mass))
@@ -374,6 +375,12 @@ class System:
 
     def removeForce(self, i):
         del self._forces[i]
+
+    def setWaterGroups(self, water_start):
+        """Independent groups (HIP extension): waters [water_start[g], water_start[g+1])
+        form group g; nothing interacts across groups and energies are also reported per
+        group (Context.getGroupEnergies).  None removes the groups."""
+        self._groups = None if water_start is None else [int(v) for v in water_start]
 
     def computeVirtualSites(self, positions):
         pos = np.array(positions, dtype=np.float64).reshape(-1, 3)
@@ -433,6 +440,9 @@ def _build_native_system(system):
                                                      ctypes.c_double(f._cutoff)))
             else:
                 raise MBPolError(1, 'unsupported force type %s' % type(f).__name__)
+        if system._groups is not None:
+            ws = _i(system._groups)
+            _check(L.mbpol_system_set_groups(h, len(ws) - 1, _pi(ws)))
     except Exception:
         L.mbpol_system_destroy(h)
         raise
@@ -553,6 +563,14 @@ class Context:
         ms, cnt = ctypes.c_double(0), ctypes.c_int64(0)
         _check(library().mbpol_context_get_timing(self._h, name.encode(), ctypes.byref(ms), ctypes.byref(cnt)))
         return ms.value, cnt.value
+
+    def getGroupEnergies(self):
+        """(groups, 6) energies of the last evaluation, kJ/mol: 1B, 2B, 3B, dispersion,
+        electrostatics, total of every group."""
+        ng = len(self.system._groups) - 1
+        out = np.zeros((ng, 6))
+        _check(library().mbpol_context_get_group_energies(self._h, _pd(out)))
+        return out
 
     def getStats(self):
         st = _EvalStats()

@@ -47,6 +47,8 @@ struct mbpol_system {
     std::vector<double> ds_c6, ds_d6;
     int ds_method = 0;
     double ds_cutoff = 1e10;
+    // independent groups of waters (clusters), [grp_wstart[g], grp_wstart[g+1]); empty = one system
+    std::vector<int> grp_wstart;
 };
 
 namespace mbpol {
@@ -124,6 +126,9 @@ struct mbpol_context {
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     mbpol::HostWorker worker;  // enqueues the short-range terms on `side`
     bool evaluated = false;    // an evaluation has completed (launch shapes are tuned)
+    // independent groups: water ranges, per-molecule candidate ranges, per-unit energies and per-group sums
+    int ngroups = 0;
+    mbpol::DevBuf g_wstart, g_mol_s, g_mol_e, g_eu1, g_eu2, g_eu3, g_energy;
     int full_chain_list = 0;
     mbpol::EvalStats stats;
     mbpol::Timer timer;

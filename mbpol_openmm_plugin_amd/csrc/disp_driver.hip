@@ -20,6 +20,8 @@ struct Disp {
     DispGeom g{};
     DevBuf act, type, mol, c6, d6, cell_of, cell_count, cell_start, cell_fill, members, unsorted, eblk, npairs;
     DevBuf cnt;  // pairs within the cutoff per active atom
+    DevBuf gs, ge;  // independent groups: candidate range (in active-atom order) of every active atom
+    DevBuf gact;    // independent groups: first active atom of every group (ngroups + 1)
     DevBuf cpos;  // positions, molecule and type of the active atoms in cell order
     int ncell = 1;
 };
@@ -81,7 +83,35 @@ int disp_create(mbpol_context* ctx) {
     HIP_TRY(D->eblk.ensure(sizeof(double) * std::max(1, D->g.na)));
     HIP_TRY(D->npairs.ensure(sizeof(unsigned long long)));
     HIP_TRY(D->cnt.ensure(sizeof(unsigned long long) * std::max(1, D->g.na)));
+    if (!s.grp_wstart.empty()) {
+        // active atoms are in particle order, so every group's atoms are consecutive
+        const std::vector<int>& ws = s.grp_wstart;
+        const int G = (int)ws.size() - 1;
+        std::vector<int> gact(G + 1, 0), gs(D->g.na), ge(D->g.na);
+        for (int g = 0, a = 0; g <= G; ++g) {
+            while (a < D->g.na && g < G && act[a] / 4 < ws[g]) ++a;
+            gact[g] = g < G ? a : D->g.na;
+        }
+        for (int g = 0; g < G; ++g)
+            for (int a = gact[g]; a < gact[g + 1]; ++a) {
+                gs[a] = gact[g];
+                ge[a] = gact[g + 1];
+            }
+        if (int e = upv(D->gs, gs)) return e;
+        if (int e = upv(D->ge, ge)) return e;
+        if (int e = upv(D->gact, gact)) return e;
+        D->g.gs = D->gs.as<int>();
+        D->g.ge = D->ge.as<int>();
+    }
     return disp_set_box(ctx);
+}
+
+int disp_group_sums(mbpol_context* ctx, hipStream_t st, u64* out) {
+    const Disp* D = ctx->disp;
+    if (!D->g.gs) return MBPOL_OK;
+    // eblk[a] holds half of atom a's pair energies; every pair is seen from both atoms
+    launch_group_sums(st, ctx->ngroups, D->gact.as<int>(), 1, nullptr, D->eblk.as<double>(), out, 3);
+    return MBPOL_OK;
 }
 
 void disp_destroy(mbpol_context* ctx) {

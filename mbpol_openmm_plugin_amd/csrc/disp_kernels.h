@@ -13,6 +13,8 @@ struct DispGeom {
     d3 L;
     int use_cells;
     int3 nc;
+    // independent groups (non-periodic): candidates of active atom a are [gs[a], ge[a]); null = all
+    const int *gs, *ge;
 };
 
 void launch_disp_cells(hipStream_t s, int na, const int* act, const double* pos, d3 L, int3 nc, int* cell_of,

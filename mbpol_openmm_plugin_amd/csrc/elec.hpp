@@ -46,6 +46,8 @@ int disp_create(mbpol_context* ctx);
 void disp_destroy(mbpol_context* ctx);
 int disp_set_box(mbpol_context* ctx);
 int disp_evaluate(mbpol_context* ctx, hipStream_t st, u64* energy);
+// per-group dispersion energies into out[g * 8 + 3]
+int disp_group_sums(mbpol_context* ctx, hipStream_t st, u64* out);
 // dispersion pairs within the cutoff at the last evaluation
 int disp_pair_count(const mbpol_context* ctx, long long* pairs);
 

@@ -56,6 +56,10 @@ struct Elec {
     // block-Jacobi preconditioner: polarisable sites grouped by molecule, and the blocks' inverses
     DevBuf blk_start, blk_site, minv;
     int nblk = 0;
+    // independent groups: site ranges, per-group preconditioner blocks and site counts, per-block
+    // |alpha r|^2, per-site energies
+    int ngroups = 0, cap_grp_pol = 0, cap_grp_other = 0;
+    DevBuf g_site_s, g_site_e, g_blk_start, g_nsites, g_blk_ee, g_esite;
     double* mu_cur = nullptr;  // dipoles of the last evaluate
     DevBuf phi_fix, phi_ind, fsite, V, eblk;
     // PME
@@ -271,6 +275,52 @@ int elec_set_box(mbpol_context* ctx) {
     return MBPOL_OK;
 }
 
+// rows of the direct sum: every other site, or every other site of the group
+static void set_direct_caps(Elec* E) {
+    if (E->ngroups) {
+        E->capP = std::max(E->cap_grp_pol, 1);
+        E->capQ = std::max(E->cap_grp_other, 1);
+    } else {
+        E->capP = std::max(E->npol, 1);
+        E->capQ = std::max(E->n - E->npol, 1);
+    }
+}
+
+// independent groups of waters: site ranges (4 sites per water), one preconditioner block per water
+static int set_groups(mbpol_context* ctx) {
+    Elec* E = ctx->elec;
+    const std::vector<int>& ws = ctx->sys.grp_wstart;
+    E->ngroups = ws.empty() ? 0 : (int)ws.size() - 1;
+    if (!E->ngroups) return MBPOL_OK;
+    const int G = E->ngroups;
+    std::vector<int> ss(E->n), se(E->n), bs(G + 1), ns(G);
+    E->cap_grp_pol = E->cap_grp_other = 0;
+    for (int g = 0; g < G; ++g) {
+        int np = 0;
+        for (int i = 4 * ws[g]; i < 4 * ws[g + 1]; ++i) {
+            ss[i] = 4 * ws[g];
+            se[i] = 4 * ws[g + 1];
+            np += E->h_pol[i] ? 1 : 0;
+        }
+        ns[g] = 4 * (ws[g + 1] - ws[g]);
+        bs[g] = ws[g];
+        E->cap_grp_pol = std::max(E->cap_grp_pol, np);
+        E->cap_grp_other = std::max(E->cap_grp_other, ns[g] - np);
+    }
+    bs[G] = ws[G];
+    if (E->nblk != ws[G])
+        return set_error(MBPOL_ERR_INVALID_ARGUMENT, "groups: every water needs its polarisable sites");
+    if (int e = up(E->g_site_s, ss)) return e;
+    if (int e = up(E->g_site_e, se)) return e;
+    if (int e = up(E->g_blk_start, bs)) return e;
+    if (int e = up(E->g_nsites, ns)) return e;
+    HIP_TRY(E->g_blk_ee.ensure(sizeof(double) * std::max(E->nblk, 1)));
+    HIP_TRY(E->g_esite.ensure(sizeof(double) * std::max(E->n, 1)));
+    E->P.gs = E->g_site_s.as<int>();
+    E->P.ge = E->g_site_e.as<int>();
+    return MBPOL_OK;
+}
+
 int elec_create(mbpol_context* ctx) {
     const mbpol_system& s = ctx->sys;
     Elec* E = new Elec();
@@ -333,10 +383,8 @@ int elec_create(mbpol_context* ctx) {
     HIP_TRY(E->cell_fill.ensure(sizeof(int) * 2));
     HIP_TRY(hipHostMalloc((void**)&E->h_eps, 128, hipHostMallocDefault));
     HIP_TRY(E->ctl.ensure(sizeof(SolveCtl)));
-    if (!P.pme) {  // direct sum: rows hold every other site
-        E->capP = std::max(E->npol, 1);
-        E->capQ = std::max(n - E->npol, 1);
-    }
+    if (int e = set_groups(ctx)) return e;
+    if (!P.pme) set_direct_caps(E);  // direct sum: rows hold every other site (of the group)
     if (P.pme) {
         const size_t nreal = (size_t)grid[0] * grid[1] * grid[2];
         bool lds = true;
@@ -659,6 +707,9 @@ static int solve_pcg(mbpol_context* ctx, double* x, double self_term, SolveCtl& 
     const int* stop = &ctl->stop;
     const PrecondBlocks B{E->nblk, E->blk_start.as<int>(), E->blk_site.as<int>()};
     double* minv = E->minv.as<double>();
+    // independent groups: convergence of the worst group
+    const GroupConv gc{E->ngroups, E->g_blk_start.as<int>(), E->g_nsites.as<int>(), E->g_blk_ee.as<double>()};
+    double* blk_ee = E->ngroups ? E->g_blk_ee.as<double>() : nullptr;
     auto field = [&](const double* v, int mode, double* out) -> int {
         if (P.pme)
             if (int e = pme_grid(ctx, 1, v, stop)) return e;
@@ -679,16 +730,16 @@ static int solve_pcg(mbpol_context* ctx, double* x, double self_term, SolveCtl& 
             timer_start(ctx, T_ELEC_ITER);
             if (f == 0) {
                 if (int e = field(x, 0, r)) return e;
-                launch_cg_precond(st, B, minv, pol, r, z, E->cg_partial.as<double>(), ctl);
+                launch_cg_precond(st, B, minv, pol, r, z, E->cg_partial.as<double>(), blk_ee, ctl);
                 launch_cg_direction(st, npol, polidx, precond_wgs(E->nblk), E->cg_partial.as<double>(), rz, 0, z, p, n,
-                                    s.el_eps, s.el_max_iter, ctl);
+                                    s.el_eps, s.el_max_iter, gc, ctl);
             } else {
                 const int k = f - 1;
                 if (int e = field(p, 1, q)) return e;
                 launch_cg_update(st, B, minv, pol, npol, E->partial.as<double>(), rz, k, x, r, p, q, z,
-                                 E->cg_partial.as<double>(), ctl);
+                                 E->cg_partial.as<double>(), blk_ee, ctl);
                 launch_cg_direction(st, npol, polidx, precond_wgs(E->nblk), E->cg_partial.as<double>(), rz, k + 1, z, p,
-                                    n, s.el_eps, s.el_max_iter, ctl);
+                                    n, s.el_eps, s.el_max_iter, gc, ctl);
             }
             timer_stop(ctx, T_ELEC_ITER);
         }
@@ -786,6 +837,8 @@ int elec_evaluate(mbpol_context* ctx, u64* energy) {
         if (int e = solve_pcg(ctx, mu_in, self_term, c)) return e;
         mu_out = mu_in;
     } else {
+        if (E->ngroups)
+            return set_error(MBPOL_ERR_INVALID_ARGUMENT, "independent groups are solved by conjugate gradients only");
         if (int e = solve_sor(ctx, E->mu_a.as<double>(), E->mu_b.as<double>(), self_term, mu_in, mu_out, c)) return e;
     }
     const int it = c.iterations;
@@ -804,7 +857,11 @@ int elec_evaluate(mbpol_context* ctx, u64* energy) {
     timer_start(ctx, T_ELEC_ENERGY);
     HIP_TRY(hipMemsetAsync(E->fsite.p, 0, sizeof(double) * 3 * n, st));
     HIP_TRY(hipMemsetAsync(E->V.p, 0, sizeof(double) * n, st));
-    launch_elec_energy(st, n, pos, S, P, R, mu_in, E->fsite.as<double>(), E->V.as<double>(), E->eblk.as<double>());
+    launch_elec_energy(st, n, pos, S, P, R, mu_in, E->fsite.as<double>(), E->V.as<double>(), E->eblk.as<double>(),
+                       E->ngroups ? E->g_esite.as<double>() : nullptr);
+    if (E->ngroups)
+        launch_group_sums(st, E->ngroups, ctx->g_wstart.as<int>(), 4, nullptr, E->g_esite.as<double>(),
+                          ctx->g_energy.as<u64>(), 4);
     launch_energy_finish(st, energy_blocks(n), E->eblk.as<double>(), energy);
     if (P.pme) {
         // the reciprocal induced potential of the reference is the one of the last
@@ -938,10 +995,8 @@ int mbpol_context_update_electrostatics(mbpol_context* ctx, int32_t n, const dou
     HIP_TRY(hipMemset(E->mu_prev.p, 0, sizeof(double) * 3 * std::max(n, 1)));
     E->mu_cur = E->mu_a.as<double>();
     E->have_previous = false;
-    if (!E->P.pme) {
-        E->capP = std::max(E->npol, 1);
-        E->capQ = std::max(n - E->npol, 1);
-    }
+    if (int e = set_groups(ctx)) return e;
+    if (!E->P.pme) set_direct_caps(E);
     return MBPOL_OK;
 }
 

@@ -14,6 +14,8 @@ struct ElecParams {
     double thole[5];  // TCC, TCD, TDD, TDDOH, TDDHH
     double tcc_q;     // TCC^(1/4)
     int redistribute;
+    // independent groups (direct sum only): site i interacts with sites [gs[i], ge[i]); null = all
+    const int *gs, *ge;
 };
 
 struct SiteParams {
@@ -91,6 +93,15 @@ int iterate_blocks(int npol);
 void launch_iterate(hipStream_t s, int npol, const int* polidx, const double* pos, ElecParams P, const ElecRows& R,
                     const double* mu_in, double* mu_out, const double* efix_pol, const double* pol, const double* grid,
                     double self_term, double omega, double* partial, const SolveCtl* ctl, const KEvents& ev);
+// independent groups in one solve: the convergence measure is that of the worst group, from the
+// |alpha r|^2 of every preconditioner block (blocks of group g: [blk_start[g], blk_start[g+1]))
+struct GroupConv {
+    int ngroups;  // 0: one system
+    const int* blk_start;
+    const int* nsites;  // sites of each group (all of them, as the reference's measure counts)
+    const double* blk_ee;
+};
+
 // preconditioned conjugate gradients on A mu = E_fix, A = diag(1/alpha) - T (see kernels_elec.hip)
 // preconditioner blocks: the polarisable sites of one molecule (at most 4 per block)
 struct PrecondBlocks {
@@ -108,23 +119,26 @@ void launch_cg_matvec(hipStream_t s, int npol, const int* polidx, const double* 
                       const double* v, const double* grid, double self_term, const double* pol, const double* efix,
                       int mode, double* out, double* partial, const SolveCtl* ctl, const KEvents& ev);
 // z = M^-1 r with partials (r.z, |alpha r|^2), 2 per precond_wgs(B.n)
+// blk_ee (optional): |alpha r|^2 of every block
 void launch_cg_precond(hipStream_t s, const PrecondBlocks& B, const double* minv, const double* pol, const double* r,
-                       double* z, double* partial, const SolveCtl* ctl);
+                       double* z, double* partial, double* blk_ee, const SolveCtl* ctl);
 // step k (uses rz[k] and the matvec partials); partials as launch_cg_precond
 void launch_cg_update(hipStream_t s, const PrecondBlocks& B, const double* minv, const double* pol, int npol,
                       const double* partial_mv, const double* rz, int k, double* x, double* r, const double* p,
-                      const double* q, double* z, double* partial, const SolveCtl* ctl);
+                      const double* q, double* z, double* partial, double* blk_ee, const SolveCtl* ctl);
 // direction k (writes rz[k], reads rz[k-1]) and the convergence measure from nb block partial pairs
 void launch_cg_direction(hipStream_t s, int npol, const int* polidx, int nb, const double* partial, double* rz, int k,
-                         const double* z, double* p, int n_all, double tol, int max_iter, SolveCtl* ctl);
+                         const double* z, double* p, int n_all, double tol, int max_iter, const GroupConv& gc,
+                         SolveCtl* ctl);
 // SOR step t: the measure and the stopping rules into ctl
 void launch_sor_finish(hipStream_t s, int nblocks, int n_all, const double* partial, int t, double tol, int max_iter,
                        SolveCtl* ctl);
 void launch_scale_copy(hipStream_t s, int n, const double* a, const double* pol, double* out, double* out2);
 int energy_blocks(int n);
 int sor_blocks(int n);
+// esite (optional): energy of every site's pairs (half of each pair), for per-group sums
 void launch_elec_energy(hipStream_t s, int n, const double* pos, SiteParams S, ElecParams P, const ElecRows& R,
-                        const double* mu, double* force, double* V, double* eblk);
+                        const double* mu, double* force, double* V, double* eblk, double* esite = nullptr);
 void launch_recip_site(hipStream_t s, int n, SiteParams S, ElecParams P, const double* phi_fix, const double* phi_ind,
                        const double* mu, double* force, double* V, double* eblk);
 void launch_charge_deriv_forces(hipStream_t s, int nw, const double* dq, const double* V, double* force);

@@ -36,12 +36,19 @@ struct ListGeom {
     int use_cells; // 27-cell search over an nc grid (periodic, nc >= 3), else all pairs
     int3 nc;
     double rc;     // nm
+    // independent groups (non-periodic): candidates of molecule m are [gs[m], ge[m]); null = all molecules
+    const int *gs, *ge;
 };
 
 // ---- kernels_water.hip ----
+// per-group sums of unit energies into out[g * 8 + comp] (fixed point): group g covers the units
+// [off[start[g]], off[start[g+1]]) when off is given, else [mult * start[g], mult * start[g+1])
+void launch_group_sums(hipStream_t s, int ngroups, const int* start, int mult, const long long* off,
+                       const double* eunit, u64* out, int comp);
+// eunit (optional): energy of every unit (water, dimer, trimer) in kJ/mol, for per-group sums
 void launch_place_vsites(hipStream_t s, int n, const int* vs_idx, const double* vs_w, double* pos);
 void launch_onebody(hipStream_t s, int nw, const int* idx, const double* pos, int periodic, d3 box, u64* force,
-                    u64* energy);
+                    u64* energy, double* eunit = nullptr);
 void launch_water_charges(hipStream_t s, int nw, const int* sites, const double* pos, double* q, double* dq);
 void launch_forces_to_double(hipStream_t s, int n3, const u64* fx, double* f);
 void launch_fold_vsites(hipStream_t s, int n, const int* vs_idx, const double* vs_w, double* f);
@@ -61,12 +68,12 @@ void launch_trimer_pass(hipStream_t s, int n, const int* idx, const double* pos,
 
 // ---- kernels_pip.hip (one object per body order and precision) ----
 void launch_twobody_f32(hipStream_t s, long long npairs, const int2* pairs, const int* idx, const double* pos,
-                        int periodic, d3 box, u64* force, u64* energy, const KEvents& ev = KEvents());
+                        int periodic, d3 box, u64* force, u64* energy, const KEvents& ev = KEvents(), double* eunit = nullptr);
 void launch_twobody_f64(hipStream_t s, long long npairs, const int2* pairs, const int* idx, const double* pos,
-                        int periodic, d3 box, u64* force, u64* energy, const KEvents& ev = KEvents());
+                        int periodic, d3 box, u64* force, u64* energy, const KEvents& ev = KEvents(), double* eunit = nullptr);
 void launch_threebody_f32(hipStream_t s, long long ntrip, const int* trip, const int* idx, const double* pos,
-                          int periodic, d3 box, u64* force, u64* energy, const KEvents& ev = KEvents());
+                          int periodic, d3 box, u64* force, u64* energy, const KEvents& ev = KEvents(), double* eunit = nullptr);
 void launch_threebody_f64(hipStream_t s, long long ntrip, const int* trip, const int* idx, const double* pos,
-                          int periodic, d3 box, u64* force, u64* energy, const KEvents& ev = KEvents());
+                          int periodic, d3 box, u64* force, u64* energy, const KEvents& ev = KEvents(), double* eunit = nullptr);
 
 }  // namespace mbpol

@@ -89,7 +89,8 @@ __global__ void __launch_bounds__(64) k_disp(DispGeom g, const int* __restrict__
         ++cnt;
     };
     if (!g.use_cells) {
-        for (int b = lane; b < g.na; b += 64) {
+        const int b0 = g.gs ? g.gs[a] : 0, b1 = g.gs ? g.ge[a] : g.na;
+        for (int b = b0 + lane; b < b1; b += 64) {
             const int j = act[b];
             visit(make_double4(pos[3 * j], pos[3 * j + 1], pos[3 * j + 2], (double)(mol[j] * 16 + type[j])));
         }

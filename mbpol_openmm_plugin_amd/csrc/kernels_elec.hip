@@ -254,9 +254,10 @@ __global__ void __launch_bounds__(256) k_site_lists(int n, const double* __restr
         cQ += __popcll(bQ);
     };
     if (!use_cells) {
-        for (int b = 0; b < n; b += G) {
+        const int b0 = P.gs ? P.gs[i] : 0, b1 = P.gs ? P.ge[i] : n;
+        for (int b = b0; b < b1; b += G) {
             const int j = b + lane;
-            const bool act = j < n;
+            const bool act = j < b1;
             const double4 q = act ? make_double4(pos[3 * j], pos[3 * j + 1], pos[3 * j + 2], pol[j] != 0.0 ? 1.0 : 0.0)
                                   : make_double4(0.0, 0.0, 0.0, 0.0);
             consider(j, act, q);
@@ -1137,7 +1138,7 @@ __global__ void __launch_bounds__(256) k_cg_precond(int nblocks, const int* __re
                                                     const int* __restrict__ blk_site, const double* __restrict__ minv,
                                                     const double* __restrict__ pol, const double* __restrict__ r,
                                                     double* __restrict__ z, double* __restrict__ partial,
-                                                    const SolveCtl* __restrict__ ctl) {
+                                                    double* __restrict__ blk_ee, const SolveCtl* __restrict__ ctl) {
     if (ctl->stop) return;
     const int b = (blockIdx.x * blockDim.x + threadIdx.x) / kGB, lane = threadIdx.x % kGB;
     double s0 = 0.0, s1 = 0.0;
@@ -1156,6 +1157,12 @@ __global__ void __launch_bounds__(256) k_cg_precond(int nblocks, const int* __re
             z[t] = zl;
             s0 = rl * zl;
             s1 = (a * rl) * (a * rl);
+        }
+        if (blk_ee) {  // per block, for the per-group convergence measure
+            double e = s1;
+#pragma unroll
+            for (int off = kGB / 2; off > 0; off >>= 1) e += __shfl_xor(e, off, kGB);
+            if (lane == 0) blk_ee[b] = e;
         }
     }
     block_partial<1>(s0, 0, partial + 2 * blockIdx.x);
@@ -1188,7 +1195,8 @@ __global__ void __launch_bounds__(256) k_cg_update(int nblocks, const int* __res
                                                    const double* __restrict__ rz, int k, double* __restrict__ x,
                                                    double* __restrict__ r, const double* __restrict__ p,
                                                    const double* __restrict__ q, double* __restrict__ z,
-                                                   double* __restrict__ partial, const SolveCtl* __restrict__ ctl) {
+                                                   double* __restrict__ partial, double* __restrict__ blk_ee,
+                                                   const SolveCtl* __restrict__ ctl) {
     if (ctl->stop) return;
     const double pq = reduce_partials(nb_mv, 0, partial_mv);
     const double step = rz[k] / pq;
@@ -1212,6 +1220,12 @@ __global__ void __launch_bounds__(256) k_cg_update(int nblocks, const int* __res
             s0 = rl * zl;
             s1 = (a * rl) * (a * rl);
         }
+        if (blk_ee) {  // per block, for the per-group convergence measure
+            double e = s1;
+#pragma unroll
+            for (int off = kGB / 2; off > 0; off >>= 1) e += __shfl_xor(e, off, kGB);
+            if (lane == 0) blk_ee[b] = e;
+        }
     }
     block_partial<1>(s0, 0, partial + 2 * blockIdx.x);
     block_partial<1>(s1, 0, partial + 2 * blockIdx.x + 1);
@@ -1224,11 +1238,29 @@ __global__ void __launch_bounds__(256) k_cg_update(int nblocks, const int* __res
 __global__ void __launch_bounds__(256) k_cg_direction(int npol, const int* __restrict__ polidx, int nb,
                                                       const double* __restrict__ partial, double* __restrict__ rz,
                                                       int k, const double* __restrict__ z, double* __restrict__ p,
-                                                      int n_all, double tol, int max_iter,
+                                                      int n_all, double tol, int max_iter, GroupConv gc,
                                                       SolveCtl* __restrict__ ctl) {
     if (ctl->stop) return;
     const double s_rz = reduce_partials(nb, 0, partial);
     const double s_ee = reduce_partials(nb, 1, partial);
+    // independent groups: the measure of the worst group (block 0 only needs it)
+    double g_eps = 0.0;
+    if (gc.ngroups > 0 && blockIdx.x == 0) {
+        __shared__ double gmax[256];
+        double m = 0.0;
+        for (int g = threadIdx.x; g < gc.ngroups; g += 256) {
+            double e = 0.0;
+            for (int b = gc.blk_start[g]; b < gc.blk_start[g + 1]; ++b) e += gc.blk_ee[b];
+            m = fmax(m, el_polar_sor * el_debye * sqrt(e / (double)gc.nsites[g]));
+        }
+        gmax[threadIdx.x] = m;
+        __syncthreads();
+        for (int k = 128; k > 0; k >>= 1) {
+            if (threadIdx.x < k) gmax[threadIdx.x] = fmax(gmax[threadIdx.x], gmax[threadIdx.x + k]);
+            __syncthreads();
+        }
+        g_eps = gmax[0];
+    }
     const double beta = k == 0 ? 0.0 : s_rz / rz[k - 1];
     const int g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g < npol) {
@@ -1238,7 +1270,7 @@ __global__ void __launch_bounds__(256) k_cg_direction(int npol, const int* __res
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         rz[k] = s_rz;
-        const double eps = el_polar_sor * el_debye * sqrt(s_ee / (double)n_all);
+        const double eps = gc.ngroups > 0 ? g_eps : el_polar_sor * el_debye * sqrt(s_ee / (double)n_all);
         ctl->eps = eps;
         ctl->iterations = k + 1;
         if (eps < tol) {
@@ -1302,7 +1334,8 @@ __global__ void __launch_bounds__(256) k_elec_energy(int n, const double* __rest
                                                      const int* __restrict__ cntP, int capQ,
                                                      const int* __restrict__ nbQ, const int* __restrict__ cntQ,
                                                      const double* __restrict__ mu, double* __restrict__ force,
-                                                     double* __restrict__ V, double* __restrict__ eblk) {
+                                                     double* __restrict__ V, double* __restrict__ eblk,
+                                                     double* __restrict__ esite) {
     const int i = (blockIdx.x * blockDim.x + threadIdx.x) / G, lane = threadIdx.x % G;
     double en = 0, fx = 0, fy = 0, fz = 0, vi = 0;
     if (i < n) {
@@ -1378,6 +1411,10 @@ __global__ void __launch_bounds__(256) k_elec_energy(int n, const double* __rest
         fy = group_sum<G>(fy);
         fz = group_sum<G>(fz);
         vi = group_sum<G>(vi);
+        if (esite) {
+            const double es = group_sum<G>(en);
+            if (lane == 0) esite[i] = 0.5 * es * el_electric;
+        }
         if (lane == 0) {
             const double f = el_electric;
             force[3 * i] += fx * f;
@@ -1609,23 +1646,24 @@ void launch_cg_matvec(hipStream_t s, int npol, const int* polidx, const double* 
                      R.pf, v, grid, self_term, pol, efix, mode, out, partial, ctl);
 }
 void launch_cg_precond(hipStream_t s, const PrecondBlocks& B, const double* minv, const double* pol, const double* r,
-                       double* z, double* partial, const SolveCtl* ctl) {
+                       double* z, double* partial, double* blk_ee, const SolveCtl* ctl) {
     if (B.n > 0)
         hipLaunchKernelGGL(k_cg_precond, precond_wgs(B.n), 256, 0, s, B.n, B.start, B.site, minv, pol, r, z, partial,
-                           ctl);
+                           blk_ee, ctl);
 }
 void launch_cg_update(hipStream_t s, const PrecondBlocks& B, const double* minv, const double* pol, int npol,
                       const double* partial_mv, const double* rz, int k, double* x, double* r, const double* p,
-                      const double* q, double* z, double* partial, const SolveCtl* ctl) {
+                      const double* q, double* z, double* partial, double* blk_ee, const SolveCtl* ctl) {
     if (B.n > 0)
         hipLaunchKernelGGL(k_cg_update, precond_wgs(B.n), 256, 0, s, B.n, B.start, B.site, minv, pol,
-                           iterate_blocks(npol), partial_mv, rz, k, x, r, p, q, z, partial, ctl);
+                           iterate_blocks(npol), partial_mv, rz, k, x, r, p, q, z, partial, blk_ee, ctl);
 }
 void launch_cg_direction(hipStream_t s, int npol, const int* polidx, int nb, const double* partial, double* rz, int k,
-                         const double* z, double* p, int n_all, double tol, int max_iter, SolveCtl* ctl) {
+                         const double* z, double* p, int n_all, double tol, int max_iter, const GroupConv& gc,
+                         SolveCtl* ctl) {
     if (npol > 0)
         hipLaunchKernelGGL(k_cg_direction, cg_blocks(npol), 256, 0, s, npol, polidx, nb, partial, rz, k, z, p, n_all,
-                           tol, max_iter, ctl);
+                           tol, max_iter, gc, ctl);
 }
 void launch_sor_finish(hipStream_t s, int nblocks, int n_all, const double* partial, int t, double tol, int max_iter,
                        SolveCtl* ctl) {
@@ -1636,9 +1674,9 @@ void launch_scale_copy(hipStream_t s, int n, const double* a, const double* pol,
 }
 int energy_blocks(int n) { return nblk((long long)n * kG, 256); }
 void launch_elec_energy(hipStream_t s, int n, const double* pos, SiteParams S, ElecParams P, const ElecRows& R,
-                        const double* mu, double* force, double* V, double* eblk) {
+                        const double* mu, double* force, double* V, double* eblk, double* esite) {
     hipLaunchKernelGGL(k_elec_energy<kG>, energy_blocks(n), 256, 0, s, n, pos, S, P, R.capP, R.nbP, R.cntP, R.capQ,
-                       R.nbQ, R.cntQ, mu, force, V, eblk);
+                       R.nbQ, R.cntQ, mu, force, V, eblk, esite);
 }
 int sor_blocks(int n) { return nblk(n, 256); }
 void launch_recip_site(hipStream_t s, int n, SiteParams S, ElecParams P, const double* phi_fix, const double* phi_ind,

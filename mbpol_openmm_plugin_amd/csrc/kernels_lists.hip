@@ -91,7 +91,8 @@ template <int G, typename F>
 __device__ __forceinline__ void for_candidates(int m, int lane, int n, const ListGeom& g, const int* cell_of,
                                                const int* cell_start, const int* cell_members, F&& f) {
     if (!g.use_cells) {
-        for (int b = 0; b < n; b += G) f(b + lane, b + lane < n);
+        const int b0 = g.gs ? g.gs[m] : 0, b1 = g.gs ? g.ge[m] : n;
+        for (int b = b0; b < b1; b += G) f(b + lane, b + lane < b1);
         return;
     }
     // the three cells along z of each (X, Y) column are consecutive in the cell order:

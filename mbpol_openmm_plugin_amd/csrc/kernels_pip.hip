@@ -139,7 +139,8 @@ __device__ __forceinline__ bool dimer_geometry(long long t, const int2* __restri
 
 __global__ void __launch_bounds__(64) k_twobody(long long np, const int2* __restrict__ pairs, const int* __restrict__ idx,
                                                 const double* __restrict__ pos, int periodic, d3 boxA,
-                                                u64* __restrict__ force, u64* __restrict__ energy) {
+                                                u64* __restrict__ force, u64* __restrict__ energy,
+                                                double* __restrict__ eunit) {
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     double e_out = 0.0;
     int at[6];
@@ -211,6 +212,7 @@ __global__ void __launch_bounds__(64) k_twobody(long long np, const int2* __rest
         }
         e_out = sw * (double)Ep * cal2joule;
     }
+    if (eunit && t < np) eunit[t] = e_out;
     add_energy(energy, e_out);
 }
 
@@ -320,7 +322,8 @@ __device__ __forceinline__ bool trimer_geometry(long long t, const int* __restri
 
 __global__ void __launch_bounds__(64) k_threebody(long long nt, const int* __restrict__ trip, const int* __restrict__ idx,
                                                   const double* __restrict__ pos, int periodic, d3 box,
-                                                  u64* __restrict__ force, u64* __restrict__ energy) {
+                                                  u64* __restrict__ force, u64* __restrict__ energy,
+                                                  double* __restrict__ eunit) {
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     double e_out = 0.0;
     int at[9];
@@ -389,6 +392,7 @@ __global__ void __launch_bounds__(64) k_threebody(long long nt, const int* __res
         }
         e_out = E * s * cal2joule;
     }
+    if (eunit && t < nt) eunit[t] = e_out;
     add_energy(energy, e_out);
 }
 #endif
@@ -403,17 +407,19 @@ static inline int nblk(long long n, int b) { return (int)((n + b - 1) / b); }
 #if PIP_BODY == 2
 void PIP_CAT(launch_twobody_, PIP_SUFFIX)(hipStream_t s, long long npairs, const int2* pairs, const int* idx,
                                           const double* pos, int periodic, d3 box, u64* force, u64* energy,
-                                          const KEvents& ev) {
+                                          const KEvents& ev, double* eunit) {
     const d3 boxA{box.x * 10., box.y * 10., box.z * 10.};
     if (npairs > 0)
-        MBPOL_LAUNCH(ev, k_twobody, nblk(npairs, 64), 64, 0, s, npairs, pairs, idx, pos, periodic, boxA, force, energy);
+        MBPOL_LAUNCH(ev, k_twobody, nblk(npairs, 64), 64, 0, s, npairs, pairs, idx, pos, periodic, boxA, force, energy,
+                     eunit);
 }
 #else
 void PIP_CAT(launch_threebody_, PIP_SUFFIX)(hipStream_t s, long long ntrip, const int* trip, const int* idx,
                                             const double* pos, int periodic, d3 box, u64* force, u64* energy,
-                                            const KEvents& ev) {
+                                            const KEvents& ev, double* eunit) {
     if (ntrip > 0)
-        MBPOL_LAUNCH(ev, k_threebody, nblk(ntrip, 64), 64, 0, s, ntrip, trip, idx, pos, periodic, box, force, energy);
+        MBPOL_LAUNCH(ev, k_threebody, nblk(ntrip, 64), 64, 0, s, ntrip, trip, idx, pos, periodic, box, force, energy,
+                     eunit);
 }
 #endif
 }  // namespace mbpol

@@ -20,7 +20,8 @@ __global__ void k_place_vsites(int n, const int* __restrict__ vs_idx, const doub
 
 // 1-body energy/forces of one water (MBPolReferenceOneBodyForce.cpp:69-221).
 __global__ void k_onebody(int nw, const int* __restrict__ idx, const double* __restrict__ pos, int periodic, d3 box,
-                          unsigned long long* __restrict__ force, unsigned long long* __restrict__ energy) {
+                          unsigned long long* __restrict__ force, unsigned long long* __restrict__ energy,
+                          double* __restrict__ eunit) {
     const int w = blockIdx.x * blockDim.x + threadIdx.x;
     double e_out = 0.0;
     if (w < nw) {
@@ -84,6 +85,7 @@ __global__ void k_onebody(int nw, const int* __restrict__ idx, const double* __r
         add_force(force, iH2, -g2[0], -g2[1], -g2[2]);
         add_force(force, iO, g1[0] + g2[0], g1[1] + g2[1], g1[2] + g2[2]);
         e_out = e * cal2joule;
+        if (eunit) eunit[w] = e_out;
     }
     add_energy(energy, e_out);
 }
@@ -210,9 +212,32 @@ static inline int nblk(long long n, int b) { return (int)((n + b - 1) / b); }
 void launch_place_vsites(hipStream_t s, int n, const int* vs_idx, const double* vs_w, double* pos) {
     if (n > 0) hipLaunchKernelGGL(k_place_vsites, nblk(n, 256), 256, 0, s, n, vs_idx, vs_w, pos);
 }
+__global__ void k_group_sums(int ngroups, const int* __restrict__ start, int mult, const long long* __restrict__ off,
+                             const double* __restrict__ eunit, u64* __restrict__ out, int comp) {
+    __shared__ u64 red[256];
+    const int g = blockIdx.x;
+    const long long b0 = off ? off[start[g]] : (long long)mult * start[g];
+    const long long b1 = off ? off[start[g + 1]] : (long long)mult * start[g + 1];
+    u64 s = 0;
+    for (long long u = b0 + threadIdx.x; u < b1; u += 256) s += to_fix(eunit[u]);
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int k = 128; k > 0; k >>= 1) {
+        if (threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[(size_t)g * 8 + comp] = red[0];
+}
+
+void launch_group_sums(hipStream_t s, int ngroups, const int* start, int mult, const long long* off,
+                       const double* eunit, u64* out, int comp) {
+    if (ngroups > 0) hipLaunchKernelGGL(k_group_sums, ngroups, 256, 0, s, ngroups, start, mult, off, eunit, out, comp);
+}
+
 void launch_onebody(hipStream_t s, int nw, const int* idx, const double* pos, int periodic, d3 box, u64* force,
-                    u64* energy) {
-    if (nw > 0) hipLaunchKernelGGL(k_onebody, nblk(nw, 64), 64, 0, s, nw, idx, pos, periodic, box, force, energy);
+                    u64* energy, double* eunit) {
+    if (nw > 0)
+        hipLaunchKernelGGL(k_onebody, nblk(nw, 64), 64, 0, s, nw, idx, pos, periodic, box, force, energy, eunit);
 }
 void launch_water_charges(hipStream_t s, int nw, const int* sites, const double* pos, double* q, double* dq) {
     if (nw > 0) hipLaunchKernelGGL(k_water_charges, nblk(nw, 64), 64, 0, s, nw, sites, pos, q, dq);

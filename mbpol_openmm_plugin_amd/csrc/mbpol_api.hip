@@ -214,6 +214,18 @@ int mbpol_system_set_box(mbpol_system* sys, const double box_nm[3]) {
     return MBPOL_OK;
 }
 
+int mbpol_system_set_groups(mbpol_system* sys, int32_t ngroups, const int32_t* water_start) {
+    if (!sys || ngroups < 0 || (ngroups > 0 && !water_start)) return set_error(MBPOL_ERR_INVALID_ARGUMENT, "bad argument");
+    sys->grp_wstart.clear();
+    if (ngroups == 0) return MBPOL_OK;
+    if (water_start[0] != 0) return set_error(MBPOL_ERR_INVALID_ARGUMENT, "groups: the first group starts at water 0");
+    for (int g = 0; g < ngroups; ++g)
+        if (water_start[g + 1] <= water_start[g])
+            return set_error(MBPOL_ERR_INVALID_ARGUMENT, "groups: group %d is empty or out of order", g);
+    sys->grp_wstart.assign(water_start, water_start + ngroups + 1);
+    return MBPOL_OK;
+}
+
 static bool valid_particle(const mbpol_system* s, int i) { return i >= 0 && i < s->n; }
 
 int mbpol_system_add_virtual_site(mbpol_system* sys, int32_t site, int32_t p1, int32_t p2, int32_t p3, double w1,
@@ -381,6 +393,61 @@ static int setup_grid(mbpol_context* ctx, CellGrid& cg, int n, bool periodic, do
     return MBPOL_OK;
 }
 
+// Independent groups (clusters) of waters in one non-periodic system: every
+// water occupies particles 4w..4w+3 (O, H1, H2, M), the 1-, 2- and 3-body
+// molecule lists are the waters in order, and no Force uses a periodic method.
+static int check_groups(const mbpol_system& s) {
+    const int W = s.grp_wstart.back();
+    auto bad = [&](const char* what) { return set_error(MBPOL_ERR_INVALID_ARGUMENT, "groups: %s", what); };
+    if (s.n != 4 * W) return bad("the system must hold exactly 4 particles (O, H1, H2, M) per water of the groups");
+    if (periodic_box(s)) return bad("groups need a non-periodic system");
+    auto waters = [&](const std::vector<int>& idx) {
+        if (idx.empty()) return true;
+        if ((int)idx.size() != 3 * W) return false;
+        for (int w = 0; w < W; ++w)
+            if (idx[3 * w] != 4 * w || idx[3 * w + 1] != 4 * w + 1 || idx[3 * w + 2] != 4 * w + 2) return false;
+        return true;
+    };
+    if (!waters(s.ob_idx) || !waters(s.tb_idx) || !waters(s.th_idx))
+        return bad("the 1-, 2- and 3-body molecule lists must be the waters in order");
+    if ((!s.tb_idx.empty() && s.tb_method == MBPOL_CUTOFF_PERIODIC) ||
+        (!s.th_idx.empty() && s.th_method == MBPOL_CUTOFF_PERIODIC) ||
+        (s.nd > 0 && s.ds_method == MBPOL_CUTOFF_PERIODIC) || (s.ne > 0 && s.el_method == MBPOL_ELEC_PME))
+        return bad("periodic methods cannot be combined with groups");
+    if (s.ne > 0 && (s.ne != 4 * W || !s.el_redistribute)) return bad("electrostatics must cover the O H H M sites of every water");
+    if (s.nd > 0)
+        for (int i = 0; i < s.nd; ++i)
+            if (s.ds_mol[i] != i / 4) return bad("dispersion molecules must be the waters");
+    return MBPOL_OK;
+}
+
+// device arrays of the groups and the candidate ranges of the molecule lists
+static int groups_create(mbpol_context* ctx) {
+    const std::vector<int>& ws = ctx->sys.grp_wstart;
+    ctx->ngroups = ws.empty() ? 0 : (int)ws.size() - 1;
+    if (!ctx->ngroups) return MBPOL_OK;
+    const int W = ws.back();
+    std::vector<int> ms(W), me(W);
+    for (int g = 0; g < ctx->ngroups; ++g)
+        for (int w = ws[g]; w < ws[g + 1]; ++w) {
+            ms[w] = ws[g];
+            me[w] = ws[g + 1];
+        }
+    if (int e = upload(ctx, ctx->g_wstart, ws)) return e;
+    if (int e = upload(ctx, ctx->g_mol_s, ms)) return e;
+    if (int e = upload(ctx, ctx->g_mol_e, me)) return e;
+    HIP_TRY(ctx->g_eu1.ensure(sizeof(double) * std::max(W, 1)));
+    HIP_TRY(ctx->g_energy.ensure(sizeof(u64) * 8 * ctx->ngroups));
+    return MBPOL_OK;
+}
+
+static void apply_list_groups(mbpol_context* ctx) {
+    for (CellGrid* cg : {&ctx->grid2, &ctx->grid3}) {
+        cg->g.gs = ctx->ngroups ? ctx->g_mol_s.as<int>() : nullptr;
+        cg->g.ge = ctx->ngroups ? ctx->g_mol_e.as<int>() : nullptr;
+    }
+}
+
 static int bin_cells(mbpol_context* ctx, hipStream_t st, CellGrid& cg, int n, const int* idx) {
     if (!cg.g.use_cells) return MBPOL_OK;
     HIP_TRY(hipMemsetAsync(cg.cell_count.p, 0, sizeof(int) * (cg.ncell + 1), st));
@@ -396,6 +463,8 @@ static int bin_cells(mbpol_context* ctx, hipStream_t st, CellGrid& cg, int n, co
 extern "C" {
 
 int mbpol_context_create(const mbpol_system* sys, int32_t precision, int32_t device, mbpol_context** out) {
+    if (sys && !sys->grp_wstart.empty())
+        if (int e = check_groups(*sys)) return e;
     if (!sys || !out) return set_error(MBPOL_ERR_INVALID_ARGUMENT, "null argument");
     if (precision != MBPOL_PRECISION_MIXED && precision != MBPOL_PRECISION_DOUBLE &&
         precision != MBPOL_PRECISION_SINGLE_PIP)
@@ -447,6 +516,7 @@ int mbpol_context_create(const mbpol_system* sys, int32_t precision, int32_t dev
     if (int e = upload(ctx, ctx->ob_idx, sys->ob_idx)) return fail(e);
     if (int e = upload(ctx, ctx->tb_idx, sys->tb_idx)) return fail(e);
     if (int e = upload(ctx, ctx->th_idx, sys->th_idx)) return fail(e);
+    if (int e = groups_create(ctx)) return fail(e);
     const int n2 = (int)sys->tb_idx.size() / 3, n3 = (int)sys->th_idx.size() / 3;
     if (n2 > 0) {
         if (int e = setup_grid(ctx, ctx->grid2, n2, sys->tb_method == MBPOL_CUTOFF_PERIODIC, sys->tb_cutoff)) return fail(e);
@@ -459,6 +529,7 @@ int mbpol_context_create(const mbpol_system* sys, int32_t precision, int32_t dev
             ctx->trip_count.ensure(sizeof(long long) * (n3 + 1)) || ctx->trip_off.ensure(sizeof(long long) * (n3 + 1)))
             return fail(set_error(MBPOL_ERR_HIP, "device allocation failed"));
     }
+    apply_list_groups(ctx);
     if (sys->ne > 0) {
         if (int e = elec_create(ctx)) return fail(e);
     }
@@ -497,6 +568,7 @@ int mbpol_context_set_box(mbpol_context* ctx, const double box_nm[3]) {
         if (int e = setup_grid(ctx, ctx->grid2, n2, ctx->sys.tb_method == MBPOL_CUTOFF_PERIODIC, ctx->sys.tb_cutoff)) return e;
     if (n3 > 0)
         if (int e = setup_grid(ctx, ctx->grid3, n3, ctx->sys.th_method == MBPOL_CUTOFF_PERIODIC, ctx->sys.th_cutoff)) return e;
+    apply_list_groups(ctx);
     if (ctx->elec)
         if (int e = elec_set_box(ctx)) return e;
     if (ctx->disp)
@@ -586,10 +658,14 @@ static int short_range(mbpol_context* ctx, hipStream_t sr, uint32_t comp) {
     const bool do1 = (comp & MBPOL_ONEBODY) && !s.ob_idx.empty();
     const bool do2 = (comp & MBPOL_TWOBODY) && !s.tb_idx.empty();
     const bool do3 = (comp & MBPOL_THREEBODY) && !s.th_idx.empty();
+    // independent groups: per-unit energies, summed per group after each term
+    const int G = ctx->ngroups;
+    u64* EG = ctx->g_energy.as<u64>();
     if (do1) {
         timer_start(ctx, T_ONEBODY);
         launch_onebody(sr, (int)s.ob_idx.size() / 3, ctx->ob_idx.as<int>(), pos, s.ob_method == MBPOL_ONEBODY_PERIODIC,
-                       box, F, E + 0);
+                       box, F, E + 0, G ? ctx->g_eu1.as<double>() : nullptr);
+        if (G) launch_group_sums(sr, G, ctx->g_wstart.as<int>(), 1, nullptr, ctx->g_eu1.as<double>(), EG, 0);
         timer_stop(ctx, T_ONEBODY);
     }
     if (do2 || do3) {
@@ -597,6 +673,10 @@ static int short_range(mbpol_context* ctx, hipStream_t sr, uint32_t comp) {
         if (int e = build_lists(ctx, sr, do2, do3)) return e;
         timer_stop(ctx, T_LISTS);
     }
+    if (G && do2) HIP_TRY(ctx->g_eu2.ensure(sizeof(double) * std::max<long long>(ctx->stats.dimers, 1)));
+    if (G && do3) HIP_TRY(ctx->g_eu3.ensure(sizeof(double) * std::max<long long>(ctx->stats.trimers, 1)));
+    double* eu2 = G ? ctx->g_eu2.as<double>() : nullptr;
+    double* eu3 = G ? ctx->g_eu3.as<double>() : nullptr;
     // The polynomials run in FP64 unless FP32 was asked for explicitly: their
     // terms cancel to ~1e-3 of their magnitudes, so FP32 evaluation loses three
     // digits of the 2- and 3-body energies.
@@ -606,26 +686,31 @@ static int short_range(mbpol_context* ctx, hipStream_t sr, uint32_t comp) {
         const KEvents ev = timer_kernel(ctx, T_TWOBODY);
         if (dp)
             launch_twobody_f64(sr, ctx->stats.dimers, ctx->dimers.as<int2>(), ctx->tb_idx.as<int>(), pos, per, box, F,
-                               E + 1, ev);
+                               E + 1, ev, eu2);
         else
             launch_twobody_f32(sr, ctx->stats.dimers, ctx->dimers.as<int2>(), ctx->tb_idx.as<int>(), pos, per, box, F,
-                               E + 1, ev);
+                               E + 1, ev, eu2);
         timer_kernel_done(ctx, T_TWOBODY, ev);
+        // dimers are listed by their first molecule, so a group's dimers are consecutive
+        if (G) launch_group_sums(sr, G, ctx->g_wstart.as<int>(), 0, ctx->dimer_off.as<long long>(), eu2, EG, 1);
     }
     if (do3) {
         const int per = s.th_method == MBPOL_CUTOFF_PERIODIC;
         const KEvents ev = timer_kernel(ctx, T_THREEBODY);
         if (dp)
             launch_threebody_f64(sr, ctx->stats.trimers, ctx->trips.as<int>(), ctx->th_idx.as<int>(), pos, per, box, F,
-                                 E + 2, ev);
+                                 E + 2, ev, eu3);
         else
             launch_threebody_f32(sr, ctx->stats.trimers, ctx->trips.as<int>(), ctx->th_idx.as<int>(), pos, per, box, F,
-                                 E + 2, ev);
+                                 E + 2, ev, eu3);
         timer_kernel_done(ctx, T_THREEBODY, ev);
+        if (G) launch_group_sums(sr, G, ctx->g_wstart.as<int>(), 0, ctx->trip_off.as<long long>(), eu3, EG, 2);
     }
     if ((comp & MBPOL_DISPERSION) && ctx->disp) {
         timer_start(ctx, T_DISPERSION);
         if (int e = disp_evaluate(ctx, sr, E + 3)) return e;
+        if (G)
+            if (int e = disp_group_sums(ctx, sr, EG)) return e;
         timer_stop(ctx, T_DISPERSION);
     }
     return MBPOL_OK;
@@ -639,6 +724,7 @@ static int evaluate(mbpol_context* ctx, double* forces_dev_out, double* energies
     double* pos = ctx->pos.as<double>();
     HIP_TRY(hipMemsetAsync(F, 0, sizeof(u64) * 3 * s.n, st));
     HIP_TRY(hipMemsetAsync(E, 0, sizeof(u64) * 8, st));
+    if (ctx->ngroups) HIP_TRY(hipMemsetAsync(ctx->g_energy.p, 0, sizeof(u64) * 8 * ctx->ngroups, st));
     launch_place_vsites(st, (int)s.vs_idx.size() / 4, ctx->vs_idx.as<int>(), ctx->vs_w.as<double>(), pos);
     // The short-range terms go to a second stream, enqueued by the context's
     // worker thread: their host-side waits (list sizes) and their kernels
@@ -765,6 +851,23 @@ int mbpol_context_get_timing(mbpol_context* ctx, const char* name, double* total
             return MBPOL_OK;
         }
     return set_error(MBPOL_ERR_INVALID_ARGUMENT, "unknown timer '%s'", name);
+}
+
+int mbpol_context_get_group_energies(const mbpol_context* ctx, double* energies) {
+    if (!ctx || !energies) return set_error(MBPOL_ERR_INVALID_ARGUMENT, "null argument");
+    if (!ctx->ngroups) return set_error(MBPOL_ERR_INVALID_ARGUMENT, "the system has no groups");
+    HIP_TRY(hipSetDevice(ctx->device));
+    std::vector<u64> h((size_t)8 * ctx->ngroups);
+    HIP_TRY(hipMemcpy(h.data(), ctx->g_energy.p, sizeof(u64) * h.size(), hipMemcpyDeviceToHost));
+    for (int g = 0; g < ctx->ngroups; ++g) {
+        double tot = 0;
+        for (int k = 0; k < 5; ++k) {
+            energies[6 * g + k] = (double)(long long)h[8 * (size_t)g + k] * kFixInvHost;
+            tot += energies[6 * g + k];
+        }
+        energies[6 * g + 5] = tot;
+    }
+    return MBPOL_OK;
 }
 
 int mbpol_context_get_stats(const mbpol_context* ctx, mbpol_eval_stats* out) {

@@ -141,3 +141,19 @@ def read_pdb(path):
                 res.append(line[17:20].strip())
                 pos.append([float(line[30:38]) / 10, float(line[38:46]) / 10, float(line[46:54]) / 10])
     return np.array(pos), names, res, box
+
+
+def create_cluster_batch(clusters, dipole_epsilon=1e-8, components=api.MBPOL_ALL):
+    """One System holding independent water clusters (lists of (n, 3, 3) water
+    coordinates in nm) as groups: the settings of a single cluster (no cutoffs
+    on the 2- and 3-body lists, direct-sum electrostatics, dispersion without a
+    cutoff) with no interaction between clusters.  Returns (system, positions
+    with M sites, water offsets of the clusters)."""
+    sizes = [len(c) for c in clusters]
+    start = np.concatenate([[0], np.cumsum(sizes)]).astype(int)
+    sys_ = create_system(int(start[-1]), nonbonded_method='CutoffNonPeriodic', twobody_cutoff=1e10,
+                         threebody_cutoff=1e10, nonbonded_cutoff=1e3, dipole_epsilon=dipole_epsilon,
+                         components=components)
+    sys_.setWaterGroups(start)
+    pos = water_positions_with_m(np.concatenate(clusters))
+    return sys_, pos, start

@@ -64,3 +64,23 @@ def test_context_without_device_fails_loudly():
     with pytest.raises(m.MBPolError) as e:
         m.Context(sys_)
     assert e.value.code == 7
+
+
+def test_group_validation_without_gpu():
+    """Independent groups: malformed offsets are refused by the system call, and a
+    system that does not have the water layout is refused before any GPU work."""
+    import mbpol_openmm_plugin_amd as m
+    L = m.library()
+    h = ctypes.c_void_p()
+    assert L.mbpol_system_create(ctypes.c_int32(8), ctypes.byref(h)) == 0
+    for bad in ((1, 2), (0, 0), (0, 2, 1)):
+        arr = (ctypes.c_int32 * len(bad))(*bad)
+        assert L.mbpol_system_set_groups(h, len(bad) - 1, arr) == 1, bad
+    L.mbpol_system_destroy(h)
+    # two waters, groups of 1 + 1, but a periodic box: refused (the check runs before the device query)
+    sysm = m.forcefield.create_system(2, nonbonded_method='CutoffNonPeriodic', twobody_cutoff=1e10,
+                                      threebody_cutoff=1e10, nonbonded_cutoff=1e3)
+    sysm.setWaterGroups([0, 1, 3])  # three waters claimed, two present
+    with pytest.raises(m.MBPolError) as e:
+        m.Context(sysm)
+    assert 'groups' in str(e.value)
