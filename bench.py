@@ -44,14 +44,14 @@ FLOPS_PER_DIMER = 31.0e3
 # x 1.0; profiles/memory_counters_4096w_eval.md holds the runs); kernels not
 # listed report null
 MEASURED_TRAFFIC = {
-    'elec_iterate': 105088023,
-    'pme_spread': 70124034,
-    'fft_z_r2c': 28975953,
-    'fft_y': 29725692,
-    'fft_x_conv': 44848900,
-    'fft_z_c2r': 29341622,
-    'threebody': 230665234,
-    'twobody': 88443131,
+    'elec_iterate': 97323924,
+    'pme_spread': 65648313,
+    'fft_z_r2c': 27277445,
+    'fft_y': 27978192,
+    'fft_x_conv': 42220929,
+    'fft_z_c2r': 27629240,
+    'threebody': 228101166,
+    'twobody': 88313151,
 }
 
 
@@ -63,6 +63,7 @@ def parse():
     ap.add_argument('--precision', default='mixed', choices=['mixed', 'double', 'single-pip'])
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--no-fp64-path', action='store_true')
+    ap.add_argument('--no-dipole-iterations', action='store_true')
     return ap.parse_args()
 
 
@@ -81,9 +82,10 @@ def kernel_models(stats, grid, nsites, npol):
     return {
         'threebody': ('flops', FLOPS_PER_TRIMER * stats['num_trimers'], 'mfma', PEAK_FP64_TFLOPS, 'TFLOP/s'),
         'twobody': ('flops', FLOPS_PER_DIMER * stats['num_dimers'], 'mfma', PEAK_FP64_TFLOPS, 'TFLOP/s'),
-        # rows of the polarisable sites (int index + 2 FP64 prefactors per entry), the
-        # potential grid once, positions and dipoles of every site once, the site's own
-        # fixed field / polarisability / row length and the new dipole
+        # the induced-field kernel (the conjugate-gradient matrix-vector product; the SOR
+        # step in the FP64 path): rows of the polarisable sites (int index + 2 FP64
+        # prefactors per entry), the potential grid once, positions and dipoles of every
+        # site once, the site's own vector / polarisability / row length and its result
         'elec_iterate': ('bytes', stats['num_pol_pairs'] * 20 + ngrid * 8 + nsites * 48 + npol * (24 + 24 + 8 + 4 + 4),
                          'hbm', PEAK_HBM_GBS, 'GB/s'),
         # row starts, entries (site + 3 FP64 coefficients), dipoles, grid write
@@ -284,7 +286,7 @@ def main():
     ms_host = (time.perf_counter() - t0) * 1000.0 / args.steps
     line['host_buffer_path'] = {'value': 1000.0 / ms_host, 'ms_per_step': ms_host,
                                 'note': 'positions copied in and forces copied out on every evaluation'}
-    if args.precision != 'double':
+    if args.precision != 'double' and not args.no_dipole_iterations:
         line['dipole_iterations'] = dipole_iterations(ctx)
     sor_iterations = stats['iterations']
     if not args.no_fp64_path and args.precision != 'double':
@@ -307,7 +309,13 @@ def main():
         del ctx64
     if not args.no_cpu_baseline:
         if args.precision != 'double' and args.no_fp64_path:  # iterations of the reference's SOR solve
-            sor_iterations = line['dipole_iterations']['plain_sor_from_alpha_e']['evaluations_to_eps_1e-8']
+            if 'dipole_iterations' in line:
+                sor_iterations = line['dipole_iterations']['plain_sor_from_alpha_e']['evaluations_to_eps_1e-8']
+            else:
+                ctx.setDipoleSolver('sor')
+                ctx.evaluateResident()
+                sor_iterations = ctx.getStats()['iterations']
+                ctx.setDipoleSolver('pcg')
         line['cpu_baseline'] = cpu_baseline(sor_iterations)
     print(json.dumps(line), flush=True)
 
