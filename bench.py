@@ -94,6 +94,10 @@ def kernel_models(stats, grid, nsites, npol):
         'fft_y': ('bytes', nhalf * 32, 'hbm', PEAK_HBM_GBS, 'GB/s'),
         'fft_x_conv': ('bytes', nhalf * 40, 'hbm', PEAK_HBM_GBS, 'GB/s'),
         'fft_z_c2r': ('bytes', nhalf * 16 + ngrid * 8, 'hbm', PEAK_HBM_GBS, 'GB/s'),
+        # z and y passes fused per x plane (when a plane fits in LDS): the real grid and
+        # the half spectrum once each
+        'fft_plane_fwd': ('bytes', ngrid * 8 + nhalf * 16, 'hbm', PEAK_HBM_GBS, 'GB/s'),
+        'fft_plane_inv': ('bytes', nhalf * 16 + ngrid * 8, 'hbm', PEAK_HBM_GBS, 'GB/s'),
     }
 
 

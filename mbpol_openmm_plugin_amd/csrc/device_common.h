@@ -41,6 +41,18 @@ __device__ __forceinline__ void add_energy(unsigned long long* e, double v) {
     if ((threadIdx.x & 63) == 0) atomicAdd(e, f);
 }
 
+// Waves of the electrostatics kernels (the evaluation's critical path) raise
+// their issue priority: the short-range kernels run beside them on a second
+// stream and would otherwise take turns with them at every SIMD's arbiter.
+#ifndef MBPOL_WAVE_PRIO
+#define MBPOL_WAVE_PRIO 1
+#endif
+__device__ __forceinline__ void critical_path_wave() {
+#if MBPOL_WAVE_PRIO
+    __builtin_amdgcn_s_setprio(2);
+#endif
+}
+
 __device__ __forceinline__ d3 load_d3(const double* p, int i) { return d3{p[3 * i], p[3 * i + 1], p[3 * i + 2]}; }
 
 // Minimum image of p relative to ref, component-wise: p += L * floor((ref - p)/L + 0.5)

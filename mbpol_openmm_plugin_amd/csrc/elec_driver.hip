@@ -75,6 +75,7 @@ struct Elec {
     PmeFft fft{};
     DevBuf tw[3];
     bool lds_fft = false;
+    bool plane_fft = false;  // z and y passes fused per x plane
     hipfftHandle plan_f = 0, plan_b = 0;
     bool plans = false;
     double* h_eps = nullptr;  // pinned, 128 bytes: [0] epsilon, ints from byte 32, a SolveCtl from byte 64
@@ -392,6 +393,11 @@ int elec_create(mbpol_context* ctx) {
         const char* force = std::getenv("MBPOL_HIP_PME_FFT");  // "hipfft" selects the library transforms
         if (force && std::string(force) == "hipfft") lds = false;
         E->lds_fft = lds;
+        // z and y passes fused per x plane: faster alone, but a plane workgroup takes a
+        // whole CU and stalls beside the short-range kernels on the second stream,
+        // so the five single-axis passes are the default (MBPOL_HIP_PME_PLANES=1: planes)
+        const char* planes = std::getenv("MBPOL_HIP_PME_PLANES");
+        E->plane_fft = lds && planes && std::string(planes) == "1" && fft_plane_ok(E->fft);
         // half-spectrum rows: padded to 128 bytes for the LDS transforms, packed for hipFFT
         const int kh = grid[2] / 2 + 1;
         E->fft.khp = lds ? (kh + 7) / 8 * 8 : kh;
@@ -590,21 +596,35 @@ static int pme_grid(mbpol_context* ctx, int mode, const double* mu, const int* s
     }
     if (E->lds_fft) {
         double2* H = E->spec.as<double2>();
-        KEvents ev = timer_kernel(ctx, T_FFT_Z_R2C);
-        launch_fft_z_r2c(st, E->fft, E->grid.as<double>(), H, ev, stop);
-        timer_kernel_done(ctx, T_FFT_Z_R2C, ev);
-        ev = timer_kernel(ctx, T_FFT_Y);
-        launch_fft_y(st, E->fft, H, 0, ev, stop);
-        timer_kernel_done(ctx, T_FFT_Y, ev);
+        const bool planes = E->plane_fft;
+        KEvents ev;
+        if (planes) {
+            ev = timer_kernel(ctx, T_FFT_PLANE_FWD);
+            launch_fft_plane_fwd(st, E->fft, E->grid.as<double>(), H, ev, stop);
+            timer_kernel_done(ctx, T_FFT_PLANE_FWD, ev);
+        } else {
+            ev = timer_kernel(ctx, T_FFT_Z_R2C);
+            launch_fft_z_r2c(st, E->fft, E->grid.as<double>(), H, ev, stop);
+            timer_kernel_done(ctx, T_FFT_Z_R2C, ev);
+            ev = timer_kernel(ctx, T_FFT_Y);
+            launch_fft_y(st, E->fft, H, 0, ev, stop);
+            timer_kernel_done(ctx, T_FFT_Y, ev);
+        }
         ev = timer_kernel(ctx, T_FFT_X_CONV);
         launch_fft_x_conv(st, E->fft, H, E->influence.as<double>(), ev, stop);
         timer_kernel_done(ctx, T_FFT_X_CONV, ev);
-        ev = timer_kernel(ctx, T_FFT_Y);
-        launch_fft_y(st, E->fft, H, 1, ev, stop);
-        timer_kernel_done(ctx, T_FFT_Y, ev);
-        ev = timer_kernel(ctx, T_FFT_Z_C2R);
-        launch_fft_z_c2r(st, E->fft, H, E->grid.as<double>(), ev, stop);
-        timer_kernel_done(ctx, T_FFT_Z_C2R, ev);
+        if (planes) {
+            ev = timer_kernel(ctx, T_FFT_PLANE_INV);
+            launch_fft_plane_inv(st, E->fft, H, E->grid.as<double>(), ev, stop);
+            timer_kernel_done(ctx, T_FFT_PLANE_INV, ev);
+        } else {
+            ev = timer_kernel(ctx, T_FFT_Y);
+            launch_fft_y(st, E->fft, H, 1, ev, stop);
+            timer_kernel_done(ctx, T_FFT_Y, ev);
+            ev = timer_kernel(ctx, T_FFT_Z_C2R);
+            launch_fft_z_c2r(st, E->fft, H, E->grid.as<double>(), ev, stop);
+            timer_kernel_done(ctx, T_FFT_Z_C2R, ev);
+        }
     } else {
         FFT_TRY(hipfftExecD2Z(E->plan_f, E->grid.as<double>(), reinterpret_cast<hipfftDoubleComplex*>(E->spec.p)));
         launch_convolve(st, nhalf, E->influence.as<double>(), E->spec.as<double2>());

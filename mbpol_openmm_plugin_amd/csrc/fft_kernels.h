@@ -37,5 +37,12 @@ void launch_fft_x_conv(hipStream_t s, const PmeFft& F, double2* H, const double*
                        const int* stop = nullptr);
 void launch_fft_z_c2r(hipStream_t s, const PmeFft& F, const double2* H, double* real, const KEvents& ev,
                       const int* stop = nullptr);
+// z and y passes of one direction as one kernel per x plane (plane in LDS);
+// fft_plane_ok: the plane fits in LDS and every stage has at most one butterfly per thread
+bool fft_plane_ok(const PmeFft& F);
+void launch_fft_plane_fwd(hipStream_t s, const PmeFft& F, const double* real, double2* H, const KEvents& ev,
+                          const int* stop = nullptr);
+void launch_fft_plane_inv(hipStream_t s, const PmeFft& F, const double2* H, double* real, const KEvents& ev,
+                          const int* stop = nullptr);
 
 }  // namespace mbpol

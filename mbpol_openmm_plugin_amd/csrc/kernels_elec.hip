@@ -147,12 +147,14 @@ __device__ __forceinline__ unsigned long long lanes_below() { return (1ull << (t
 // site set-up and neighbour rows
 // ---------------------------------------------------------------------------
 __global__ void k_site_charges_copy(int n, const double* __restrict__ q0, double* __restrict__ q) {
+    critical_path_wave();
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) q[i] = q0[i];
 }
 
 __global__ void k_site_cells(int n, const double* __restrict__ pos, ElecParams P, int3 nc, int* __restrict__ cell_of,
                              int* __restrict__ cell_count) {
+    critical_path_wave();
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const double x = pos[3 * i], y = pos[3 * i + 1], z = pos[3 * i + 2];
@@ -173,6 +175,7 @@ __global__ void k_site_cells(int n, const double* __restrict__ pos, ElecParams P
 // scatter item i into its bucket (slot order is arbitrary; k_sort_segments restores index order)
 __global__ void k_bucket_fill(int n, const int* __restrict__ bucket_of, const int* __restrict__ bucket_start,
                               int* __restrict__ bucket_fill, int* __restrict__ members) {
+    critical_path_wave();
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int c = bucket_of[i];
@@ -186,6 +189,7 @@ __global__ void k_bucket_fill(int n, const int* __restrict__ bucket_of, const in
 // and spreading sum then depends on the input only.
 __global__ void k_rank_sort(int n, const int* __restrict__ bucket_of, const int* __restrict__ bucket_start,
                             const int* __restrict__ members_in, int* __restrict__ members_out) {
+    critical_path_wave();
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int c = bucket_of[i];
@@ -208,6 +212,7 @@ __global__ void k_rank_sort(int n, const int* __restrict__ bucket_of, const int*
 // candidates from consecutive entries instead of gathering them by site index
 __global__ void k_site_pack(int n, const int* __restrict__ members, const double* __restrict__ pos,
                             const double* __restrict__ pol, double4* __restrict__ cpos) {
+    critical_path_wave();
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
     const int j = members[t];
@@ -223,6 +228,7 @@ __global__ void __launch_bounds__(256) k_site_lists(int n, const double* __restr
                                                     int* __restrict__ nbP, int* __restrict__ cntP,
                                                     int* __restrict__ nbQ, int* __restrict__ cntQ,
                                                     int* __restrict__ maxcnt) {
+    critical_path_wave();
     const int i = (blockIdx.x * blockDim.x + threadIdx.x) / G, lane = threadIdx.x % G;
     int cP = 0, cQ = 0;
     if (i < n) {  // uniform over the group
@@ -325,6 +331,7 @@ __global__ void __launch_bounds__(256) k_prefactors(int npol, const int* __restr
                                                     const SiteParams S, ElecParams P, int capP,
                                                     const int* __restrict__ nbP, const int* __restrict__ cntP,
                                                     double2* __restrict__ pf) {
+    critical_path_wave();
     const int g = (blockIdx.x * blockDim.x + threadIdx.x) / G, lane = threadIdx.x % G;
     if (g >= npol) return;
     const int i = polidx[g];
@@ -378,6 +385,7 @@ __global__ void __launch_bounds__(256) k_fixed_field(int npol, const int* __rest
                                                      int capP, const int* __restrict__ nbP, const int* __restrict__ cntP,
                                                      int capQ, const int* __restrict__ nbQ,
                                                      const int* __restrict__ cntQ, double* __restrict__ field) {
+    critical_path_wave();
     const int g = (blockIdx.x * blockDim.x + threadIdx.x) / G, lane = threadIdx.x % G;
     if (g >= npol) return;
     const int i = polidx[g];
@@ -524,6 +532,7 @@ __device__ __forceinline__ double pick5(const double (&th)[5][4], int k, int c) 
 // sites whose weight (charge or polarisability) is zero add nothing to the grid and get no bin
 __global__ void k_site_bins(int n, const double* __restrict__ pos, const double* __restrict__ weight, ElecParams P,
                             BrickGeom B, int* __restrict__ bin_of, int* __restrict__ bin_count) {
+    critical_path_wave();
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     if (weight[i] == 0.0) {
@@ -545,6 +554,7 @@ __global__ void k_site_bins(int n, const double* __restrict__ pos, const double*
 // derivative (k >= 5) of the order-5 B-spline along axis d.
 __global__ void k_site_spline_table(int n, const double* __restrict__ pos, ElecParams P, int* __restrict__ g0tab,
                                     double* __restrict__ sptab) {
+    critical_path_wave();
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     int g0[3];
@@ -577,6 +587,7 @@ __global__ void __launch_bounds__(kBrickThreads) k_spread_bricks(int mode, const
                                                                  const int* __restrict__ g0tab,
                                                                  const double* __restrict__ sptab,
                                                                  double* __restrict__ grid) {
+    critical_path_wave();
     // per candidate: contribution at support offset (ix, iy, iz) is
     //   u0[iy] v0[iz] A[ix] + T0[ix] (U1[iy] v0[iz] + u0[iy] V1[iz])
     __shared__ double cs[kBrickCand][30];  // A, T0, u0, U1, v0, V1 (5 each)
@@ -723,6 +734,7 @@ __device__ __forceinline__ size_t support_point(const int* g0, int k, const Elec
 
 __global__ void k_csr_count(int npol, const int* __restrict__ polidx, const int* __restrict__ g0tab, ElecParams P,
                             int* __restrict__ rowcount) {
+    critical_path_wave();
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (long long)npol * 125) return;
     const int s = polidx[t / 125];
@@ -734,6 +746,7 @@ __global__ void k_csr_count(int npol, const int* __restrict__ polidx, const int*
 __global__ void k_csr_fill(int npol, const int* __restrict__ polidx, const int* __restrict__ g0tab,
                            const double* __restrict__ sptab, ElecParams P, const int* __restrict__ rowstart,
                            int* __restrict__ fill, int* __restrict__ esite, double* __restrict__ ecoef) {
+    critical_path_wave();
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (long long)npol * 125) return;
     const int s = polidx[t / 125];
@@ -752,6 +765,7 @@ __global__ void k_csr_fill(int npol, const int* __restrict__ polidx, const int* 
 // rows ascending by site (a row holds a handful of entries)
 __global__ void k_csr_sort(size_t ngrid, const int* __restrict__ rowstart, int* __restrict__ esite,
                            double* __restrict__ ecoef) {
+    critical_path_wave();
     const size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= ngrid) return;
     const int b = rowstart[g], e = rowstart[g + 1];
@@ -777,6 +791,7 @@ __global__ void __launch_bounds__(256) k_spread_csr(size_t ngrid, const int* __r
                                                     const int* __restrict__ esite, const double* __restrict__ ecoef,
                                                     const double* __restrict__ mu, double* __restrict__ grid,
                                                     const int* __restrict__ stop) {
+    critical_path_wave();
     if (stop && *stop) return;
     const size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= ngrid) return;
@@ -792,6 +807,7 @@ __global__ void __launch_bounds__(256) k_spread_csr(size_t ngrid, const int* __r
 
 // multiply the half-complex spectrum by the reciprocal-space influence function (:1676-1713)
 __global__ void k_convolve(size_t nhalf, const double* __restrict__ influence, double2* __restrict__ spec) {
+    critical_path_wave();
     const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nhalf) return;
     const double e = influence[t];
@@ -804,6 +820,7 @@ __global__ void k_convolve(size_t nhalf, const double* __restrict__ influence, d
 // influence function on the half grid (kz <= K3/2)
 __global__ void k_influence(ElecParams P, int khp, const double* __restrict__ bx, const double* __restrict__ by,
                             const double* __restrict__ bz, double* __restrict__ out) {
+    critical_path_wave();
     const size_t total = (size_t)P.grid.x * P.grid.y * khp;
     const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= total) return;
@@ -827,6 +844,7 @@ __global__ void k_influence(ElecParams P, int khp, const double* __restrict__ bx
 // order 000 100 010 001 200 020 002 110 101 011), :1715-1821
 __global__ void __launch_bounds__(64) k_gather(int n, const double* __restrict__ pos, ElecParams P, const double* __restrict__ grid,
                          double* __restrict__ phi) {
+    critical_path_wave();
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     int g0[3];
@@ -876,6 +894,7 @@ __global__ void __launch_bounds__(64) k_gather(int n, const double* __restrict__
 // field = -scale * grad(phi) (+= for the induced field, = for the fixed field); :2267-2278, :2300-2324
 __global__ void k_record_field(int n, ElecParams P, const double* __restrict__ phi, int accumulate, double self_term,
                                const double* __restrict__ mu, double* __restrict__ field) {
+    critical_path_wave();
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const double sc[3] = {P.grid.x * P.inv_box.x, P.grid.y * P.inv_box.y, P.grid.z * P.inv_box.z};
@@ -976,6 +995,7 @@ __global__ void __launch_bounds__(256) k_iterate(int npol, const int* __restrict
                                                  const double* __restrict__ efix_pol, const double* __restrict__ pol,
                                                  const double* __restrict__ grid, double self_term, double omega,
                                                  double* __restrict__ partial, const SolveCtl* __restrict__ ctl) {
+    critical_path_wave();
     if (ctl->stop) return;
     const int g = (blockIdx.x * blockDim.x + threadIdx.x) / G, lane = threadIdx.x % G;
     double e2 = 0.0;
@@ -1016,6 +1036,7 @@ __global__ void __launch_bounds__(256) k_cg_matvec(int npol, const int* __restri
                                                    double self_term, const double* __restrict__ pol,
                                                    const double* __restrict__ efix, int mode, double* __restrict__ out,
                                                    double* __restrict__ partial, const SolveCtl* __restrict__ ctl) {
+    critical_path_wave();
     if (ctl->stop) return;
     const int g = (blockIdx.x * blockDim.x + threadIdx.x) / G, lane = threadIdx.x % G;
     double s0 = 0.0, s1 = 0.0;
@@ -1061,6 +1082,7 @@ constexpr int kGB = 16;  // lanes per block when applying the preconditioner (>=
 __global__ void __launch_bounds__(256) k_block_factor(int nblocks, const int* __restrict__ blk_start,
                                                       const int* __restrict__ blk_site, const double* __restrict__ pos,
                                                       const SiteParams S, ElecParams P, double* __restrict__ minv) {
+    critical_path_wave();
     const int b = (blockIdx.x * blockDim.x + threadIdx.x) / kGB, l = threadIdx.x % kGB;
     if (b >= nblocks) return;  // uniform over the lanes of a block
     const int s0 = blk_start[b], m = blk_start[b + 1] - s0, d = 3 * m;
@@ -1139,6 +1161,7 @@ __global__ void __launch_bounds__(256) k_cg_precond(int nblocks, const int* __re
                                                     const double* __restrict__ pol, const double* __restrict__ r,
                                                     double* __restrict__ z, double* __restrict__ partial,
                                                     double* __restrict__ blk_ee, const SolveCtl* __restrict__ ctl) {
+    critical_path_wave();
     if (ctl->stop) return;
     const int b = (blockIdx.x * blockDim.x + threadIdx.x) / kGB, lane = threadIdx.x % kGB;
     double s0 = 0.0, s1 = 0.0;
@@ -1197,6 +1220,7 @@ __global__ void __launch_bounds__(256) k_cg_update(int nblocks, const int* __res
                                                    const double* __restrict__ q, double* __restrict__ z,
                                                    double* __restrict__ partial, double* __restrict__ blk_ee,
                                                    const SolveCtl* __restrict__ ctl) {
+    critical_path_wave();
     if (ctl->stop) return;
     const double pq = reduce_partials(nb_mv, 0, partial_mv);
     const double step = rz[k] / pq;
@@ -1240,6 +1264,7 @@ __global__ void __launch_bounds__(256) k_cg_direction(int npol, const int* __res
                                                       int k, const double* __restrict__ z, double* __restrict__ p,
                                                       int n_all, double tol, int max_iter, GroupConv gc,
                                                       SolveCtl* __restrict__ ctl) {
+    critical_path_wave();
     if (ctl->stop) return;
     const double s_rz = reduce_partials(nb, 0, partial);
     const double s_ee = reduce_partials(nb, 1, partial);
@@ -1288,6 +1313,7 @@ __global__ void __launch_bounds__(256) k_cg_direction(int npol, const int* __res
 // or after max_iter + 1 steps
 __global__ void k_sor_finish(int nblocks, int n, const double* __restrict__ partial, int t, double tol, int max_iter,
                              SolveCtl* __restrict__ ctl) {
+    critical_path_wave();
     if (ctl->stop) return;
     __shared__ double red[256];
     double s = 0.0;
@@ -1314,6 +1340,7 @@ __global__ void k_sor_finish(int nblocks, int n, const double* __restrict__ part
 
 __global__ void k_scale_copy(int n3, const double* __restrict__ a, const double* __restrict__ pol, double* __restrict__ out,
                              double* __restrict__ out2) {
+    critical_path_wave();
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n3) return;
     const double v = a[t] * pol[t / 3];
@@ -1336,6 +1363,7 @@ __global__ void __launch_bounds__(256) k_elec_energy(int n, const double* __rest
                                                      const double* __restrict__ mu, double* __restrict__ force,
                                                      double* __restrict__ V, double* __restrict__ eblk,
                                                      double* __restrict__ esite) {
+    critical_path_wave();
     const int i = (blockIdx.x * blockDim.x + threadIdx.x) / G, lane = threadIdx.x % G;
     double en = 0, fx = 0, fy = 0, fz = 0, vi = 0;
     if (i < n) {
@@ -1439,6 +1467,7 @@ __global__ void __launch_bounds__(256) k_elec_energy(int n, const double* __rest
 __global__ void k_recip_site(int n, const SiteParams S, ElecParams P, const double* __restrict__ phi_fix,
                              const double* __restrict__ phi_ind, const double* __restrict__ mu, double* __restrict__ force,
                              double* __restrict__ V, double* __restrict__ eblk) {
+    critical_path_wave();
     __shared__ double red[256];
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     double en = 0.0;
@@ -1476,6 +1505,7 @@ __global__ void k_recip_site(int n, const SiteParams S, ElecParams P, const doub
 // charge-derivative chain rule (:2767-2773): F_a -= f sum_s dq_s/dr_a V_s, a in O,H1,H2
 __global__ void k_charge_deriv_forces(int nw, const double* __restrict__ dq, const double* __restrict__ V,
                                       double* __restrict__ force) {
+    critical_path_wave();
     const int w = blockIdx.x * blockDim.x + threadIdx.x;
     if (w >= nw) return;
     const int base = 4 * w;
@@ -1490,6 +1520,7 @@ __global__ void k_charge_deriv_forces(int nw, const double* __restrict__ dq, con
 
 // sum block partials into the fixed-point energy slot (fixed order)
 __global__ void k_energy_finish(int nblocks, const double* __restrict__ eblk, u64* __restrict__ energy) {
+    critical_path_wave();
     __shared__ double red[256];
     double s = 0.0;
     for (int b = threadIdx.x; b < nblocks; b += blockDim.x) s += eblk[b];
@@ -1504,6 +1535,7 @@ __global__ void k_energy_finish(int nblocks, const double* __restrict__ eblk, u6
 
 // double forces of the electrostatics particles -> fixed-point particle forces
 __global__ void k_add_forces_fix(int n, const double* __restrict__ f, u64* __restrict__ fx) {
+    critical_path_wave();
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t < 3 * n) atomicAdd(&fx[t], to_fix(f[t]));
 }
@@ -1514,6 +1546,7 @@ __global__ void k_add_forces_fix(int n, const double* __restrict__ f, u64* __res
 __global__ void k_potential_at_points(int npts, const double* __restrict__ pts, int n, const double* __restrict__ pos,
                                       const double* __restrict__ q, const double* __restrict__ mu, ElecParams P,
                                       double* __restrict__ out) {
+    critical_path_wave();
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= npts) return;
     double v = 0.0;

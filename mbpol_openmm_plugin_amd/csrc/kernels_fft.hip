@@ -186,6 +186,7 @@ template <int C>
 __global__ void __launch_bounds__(kFftThreads) k_fft_z_r2c(const double* __restrict__ real, double2* __restrict__ H,
                                                            int nlines, FftAxis az, int Kh, int Khp,
                                                            const int* __restrict__ stop) {
+    critical_path_wave();
     extern __shared__ double2 lds[];
     if (stop && *stop) return;
     const int N = az.n, M = kFftThreads >> az.lgP, P = 1 << az.lgP;
@@ -222,6 +223,7 @@ __global__ void __launch_bounds__(kFftThreads) k_fft_z_r2c(const double* __restr
 template <int C>
 __global__ void __launch_bounds__(kFftThreads) k_fft_y(double2* __restrict__ H, int K1, int Kh, int Khp, FftAxis ay,
                                                        int inv, const int* __restrict__ stop) {
+    critical_path_wave();
     extern __shared__ double2 lds[];
     if (stop && *stop) return;
     const int N = ay.n, lgM = 7 - ay.lgP, M = 1 << lgM;
@@ -271,6 +273,7 @@ template <int C>
 __global__ void __launch_bounds__(kFftThreads) k_fft_x_conv(double2* __restrict__ H, const double* __restrict__ influence,
                                                             int K1, int Kh, int Khp, FftAxis ax,
                                                             const int* __restrict__ stop) {
+    critical_path_wave();
     extern __shared__ double2 lds[];
     if (stop && *stop) return;
     const int N = ax.n, lgM = 7 - ax.lgP, M = 1 << lgM, P = 1 << ax.lgP;
@@ -340,6 +343,7 @@ template <int C>
 __global__ void __launch_bounds__(kFftThreads) k_fft_z_c2r(const double2* __restrict__ H, double* __restrict__ real,
                                                            int nlines, FftAxis az, int Kh, int Khp,
                                                            const int* __restrict__ stop) {
+    critical_path_wave();
     extern __shared__ double2 lds[];
     if (stop && *stop) return;
     const int N = az.n, M = kFftThreads >> az.lgP, P = 1 << az.lgP;
@@ -377,9 +381,328 @@ __global__ void __launch_bounds__(kFftThreads) k_fft_z_c2r(const double2* __rest
     }
 }
 
+// ---------------------------------------------------------------------------
+// Plane passes: when one x plane of the grid fits in LDS, the z and y passes
+// of each direction run as one kernel per plane (one workgroup per x), so the
+// half spectrum makes one round trip through HBM less per direction.  Two real
+// z lines (y = 2p, 2p + 1) share one complex transform: Z = r_2p + i r_2p+1,
+// separated afterwards with X_2p[k] = (Z[k] + conj Z[-k]) / 2 and
+// X_2p+1[k] = (Z[k] - conj Z[-k]) / 2i (and the reverse on the way back).
+// Every stage works in place: each thread holds its butterflies in registers
+// across a barrier, so the plane needs a single LDS buffer.
+// ---------------------------------------------------------------------------
+// A plane workgroup takes a whole CU (16 waves, ~118 KB of LDS at 121 x 121),
+// so the plane passes only pay when nothing else runs beside them.
+constexpr int kPlaneThreads = 1024;
+constexpr int kPlaneBpt = 1;     // butterflies per thread and stage (<= 1024 per stage)
+constexpr int kPlaneCopies = 8;  // plane entries per thread in the copy phases (<= 8192)
+constexpr int kPlanePairs = 4;   // (k, -k) pairs per thread in the separation (<= 4096)
+
+// DFT of one radix-R butterfly held in vx, vy (inputs in r order, outputs in q
+// order), twiddled for column k of the stage as in butterfly<R>.
+template <int R>
+__device__ __forceinline__ void bf_compute(double (&vx)[R], double (&vy)[R], int k, int Ns, int step,
+                                           const double2* __restrict__ tw, bool inv) {
+    if (Ns > 1) {
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+            const double wx = tw[k * r * step].x, wy0 = tw[k * r * step].y;
+            const double wy = inv ? -wy0 : wy0;
+            const double x = vx[r] * wx - vy[r] * wy, y = vx[r] * wy + vy[r] * wx;
+            vx[r] = x;
+            vy[r] = y;
+        }
+    }
+    if constexpr (R % 2 == 0) {
+        double ox[R], oy[R];
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+            double ax = vx[0], ay = vy[0];
+#pragma unroll
+            for (int r = 1; r < R; ++r) {
+                const int t = (r * q) % R;
+                const double c = Roots<R>::c[t], sn = inv ? Roots<R>::s[t] : -Roots<R>::s[t];
+                ax += vx[r] * c - vy[r] * sn;
+                ay += vx[r] * sn + vy[r] * c;
+            }
+            ox[q] = ax;
+            oy[q] = ay;
+        }
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+            vx[q] = ox[q];
+            vy[q] = oy[q];
+        }
+    } else {
+        constexpr int H = (R - 1) / 2;
+        double sx[H], sy[H], dx[H], dy[H];
+        double x0 = vx[0], y0 = vy[0];
+#pragma unroll
+        for (int r = 1; r <= H; ++r) {
+            sx[r - 1] = vx[r] + vx[R - r];
+            sy[r - 1] = vy[r] + vy[R - r];
+            dx[r - 1] = vx[r] - vx[R - r];
+            dy[r - 1] = vy[r] - vy[R - r];
+            x0 += sx[r - 1];
+            y0 += sy[r - 1];
+        }
+        const double v0x = vx[0], v0y = vy[0];
+        vx[0] = x0;
+        vy[0] = y0;
+        const double sgn = inv ? -1.0 : 1.0;
+#pragma unroll
+        for (int q = 1; q <= H; ++q) {
+            double ax = v0x, ay = v0y, bx = 0.0, by = 0.0;
+#pragma unroll
+            for (int r = 1; r <= H; ++r) {
+                const int t = (r * q) % R;
+                const double c = Roots<R>::c[t], sn = Roots<R>::s[t];
+                ax += sx[r - 1] * c;
+                ay += sy[r - 1] * c;
+                bx += dx[r - 1] * sn;
+                by += dy[r - 1] * sn;
+            }
+            vx[q] = ax + sgn * by;
+            vy[q] = ay - sgn * bx;
+            vx[R - q] = ax - sgn * by;
+            vy[R - q] = ay + sgn * bx;
+        }
+    }
+}
+
+// Stage s of the lines buf[l * ld + (0..n-1)], l < L, in place: thread t owns
+// butterflies t, t + blockDim.x, ... (L * nb <= kPlaneBpt * blockDim.x), held in
+// registers across the barrier.  Called by every thread of the block.
+template <int R>
+__device__ __forceinline__ void stage_inplace(double2* buf, int L, int ld, const FftAxis& ax, int s, bool inv) {
+    const int Ns = ax.ns[s], nb = ax.nb[s], step = ax.step[s];
+    const float inv_nb = 1.0f / (float)nb;
+    double vx[kPlaneBpt][R], vy[kPlaneBpt][R];
+    int base[kPlaneBpt];
+    bool act[kPlaneBpt];
+#pragma unroll
+    for (int t = 0; t < kPlaneBpt; ++t) {
+        const int b = threadIdx.x + t * blockDim.x;
+        act[t] = b < L * nb;
+        base[t] = 0;
+        if (act[t]) {
+            const int l = div_small(b, nb, inv_nb), j = b - l * nb;
+            const int jq = Ns == 1 ? j : div_small(j, Ns, ax.inv_ns[s]);
+            const double2* line = buf + l * ld;
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                vx[t][r] = line[j + r * nb].x;
+                vy[t][r] = line[j + r * nb].y;
+            }
+            bf_compute<R>(vx[t], vy[t], j - jq * Ns, Ns, step, ax.tw, inv);
+            base[t] = l * ld + jq * Ns * R + (j - jq * Ns);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int t = 0; t < kPlaneBpt; ++t)
+        if (act[t]) {
+#pragma unroll
+            for (int q = 0; q < R; ++q) {
+                buf[base[t] + q * Ns].x = vx[t][q];
+                buf[base[t] + q * Ns].y = vy[t][q];
+            }
+        }
+    __syncthreads();
+}
+
+__device__ void fft_lines_inplace(double2* buf, int L, int ld, const FftAxis& ax, bool inv) {
+    for (int s = 0; s < ax.nfac; ++s) {
+        switch (ax.fac[s]) {  // uniform over the block: the barriers inside are reached by all
+            case 2: stage_inplace<2>(buf, L, ld, ax, s, inv); break;
+            case 3: stage_inplace<3>(buf, L, ld, ax, s, inv); break;
+            case 4: stage_inplace<4>(buf, L, ld, ax, s, inv); break;
+            case 5: stage_inplace<5>(buf, L, ld, ax, s, inv); break;
+            case 7: stage_inplace<7>(buf, L, ld, ax, s, inv); break;
+            case 11: stage_inplace<11>(buf, L, ld, ax, s, inv); break;
+            default: stage_inplace<13>(buf, L, ld, ax, s, inv); break;
+        }
+    }
+}
+
+// Forward, plane x = blockIdx.x: real[x][y][z] -> z transform (two lines per
+// complex transform) -> y transform -> H[x][ky][kz], kz < Kh.
+// LDS: packed lines A[p][z] (p < NP = ceil(K1/2)), then B[kz][y].
+__global__ void __launch_bounds__(kPlaneThreads) k_fft_plane_fwd(const double* __restrict__ real, double2* __restrict__ H,
+                                                                 int K1, int Kh, int Khp, FftAxis ay, FftAxis az,
+                                                                 const int* __restrict__ stop) {
+    critical_path_wave();
+    extern __shared__ double2 lds[];
+    if (stop && *stop) return;
+    const int K2 = az.n, NP = (K1 + 1) >> 1, x = blockIdx.x;
+    const double* src = real + (size_t)x * K1 * K2;
+    {  // packed lines, all loads in flight before the LDS stores
+        double re[kPlaneCopies], im[kPlaneCopies];
+#pragma unroll
+        for (int u = 0; u < kPlaneCopies; ++u) {
+            const int e = threadIdx.x + u * kPlaneThreads;
+            const int p = e / K2, z = e - p * K2;
+            re[u] = e < NP * K2 ? src[(size_t)(2 * p) * K2 + z] : 0.0;
+            im[u] = e < NP * K2 && 2 * p + 1 < K1 ? src[(size_t)(2 * p + 1) * K2 + z] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < kPlaneCopies; ++u) {
+            const int e = threadIdx.x + u * kPlaneThreads;
+            if (e < NP * K2) lds[e] = make_double2(re[u], im[u]);
+        }
+    }
+    __syncthreads();
+    fft_lines_inplace(lds, NP, K2, az, false);
+    {  // separate the two lines of each transform into B[kz][y]
+        double zx[kPlanePairs], zy[kPlanePairs], cx[kPlanePairs], cy[kPlanePairs];
+#pragma unroll
+        for (int u = 0; u < kPlanePairs; ++u) {
+            const int e = threadIdx.x + u * kPlaneThreads;
+            if (e < NP * Kh) {
+                const int p = e / Kh, k = e - p * Kh;
+                const double2 a = lds[p * K2 + k], b = lds[p * K2 + (k == 0 ? 0 : K2 - k)];
+                zx[u] = a.x;
+                zy[u] = a.y;
+                cx[u] = b.x;
+                cy[u] = -b.y;  // conj Z[-k]
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < kPlanePairs; ++u) {
+            const int e = threadIdx.x + u * kPlaneThreads;
+            if (e < NP * Kh) {
+                const int p = e / Kh, k = e - p * Kh;
+                // X_2p = (Z + conj Z-) / 2; X_2p+1 = (Z - conj Z-) / 2i = (Im, -Re) of (Z - conj Z-) / 2
+                lds[k * K1 + 2 * p] = make_double2(0.5 * (zx[u] + cx[u]), 0.5 * (zy[u] + cy[u]));
+                if (2 * p + 1 < K1) lds[k * K1 + 2 * p + 1] = make_double2(0.5 * (zy[u] - cy[u]), -0.5 * (zx[u] - cx[u]));
+            }
+        }
+    }
+    __syncthreads();
+    fft_lines_inplace(lds, Kh, K1, ay, false);
+    double2* dst = H + (size_t)x * K1 * Khp;
+#pragma unroll
+    for (int u = 0; u < kPlaneCopies; ++u) {
+        const int e = threadIdx.x + u * kPlaneThreads;
+        if (e < K1 * Kh) {
+            const int y = e / Kh, k = e - y * Kh;
+            const double2 v = lds[k * K1 + y];
+            dst[(size_t)y * Khp + k].x = v.x;
+            dst[(size_t)y * Khp + k].y = v.y;
+        }
+    }
+}
+
+// Inverse, plane x: H[x][ky][kz] -> y transform -> z transform of the
+// Hermitian-extended lines (two per complex transform) -> real[x][y][z].
+__global__ void __launch_bounds__(kPlaneThreads) k_fft_plane_inv(const double2* __restrict__ H, double* __restrict__ real,
+                                                                 int K1, int Kh, int Khp, FftAxis ay, FftAxis az,
+                                                                 const int* __restrict__ stop) {
+    critical_path_wave();
+    extern __shared__ double2 lds[];
+    if (stop && *stop) return;
+    const int K2 = az.n, NP = (K1 + 1) >> 1, x = blockIdx.x;
+    const double2* src = H + (size_t)x * K1 * Khp;
+    {
+        double vx[kPlaneCopies], vy[kPlaneCopies];
+#pragma unroll
+        for (int u = 0; u < kPlaneCopies; ++u) {
+            const int e = threadIdx.x + u * kPlaneThreads;
+            const int y = e / Kh, k = e - y * Kh;
+            if (e < K1 * Kh) {
+                vx[u] = src[(size_t)y * Khp + k].x;
+                vy[u] = src[(size_t)y * Khp + k].y;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kPlaneCopies; ++u) {
+            const int e = threadIdx.x + u * kPlaneThreads;
+            const int y = e / Kh, k = e - y * Kh;
+            if (e < K1 * Kh) lds[k * K1 + y] = make_double2(vx[u], vy[u]);
+        }
+    }
+    __syncthreads();
+    fft_lines_inplace(lds, Kh, K1, ay, true);
+    {  // Z[p][k] = S_2p[k] + i S_2p+1[k], S_y[k] = B[k][y] (k < Kh) or conj B[K2-k][y]
+        double zx[kPlaneCopies], zy[kPlaneCopies];
+#pragma unroll
+        for (int u = 0; u < kPlaneCopies; ++u) {
+            const int e = threadIdx.x + u * kPlaneThreads;
+            if (e < NP * K2) {
+                const int p = e / K2, k = e - p * K2;
+                const bool lo = k < Kh;
+                const int kk = lo ? k : K2 - k;
+                const double2 a = lds[kk * K1 + 2 * p];
+                const double2 b = 2 * p + 1 < K1 ? lds[kk * K1 + 2 * p + 1] : make_double2(0.0, 0.0);
+                const double ay_ = lo ? a.y : -a.y, bx = b.x, by = lo ? b.y : -b.y;
+                zx[u] = a.x - by;
+                zy[u] = ay_ + bx;
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < kPlaneCopies; ++u) {
+            const int e = threadIdx.x + u * kPlaneThreads;
+            if (e < NP * K2) lds[e] = make_double2(zx[u], zy[u]);
+        }
+    }
+    __syncthreads();
+    fft_lines_inplace(lds, NP, K2, az, true);
+    double* dst = real + (size_t)x * K1 * K2;
+#pragma unroll
+    for (int u = 0; u < kPlaneCopies; ++u) {
+        const int e = threadIdx.x + u * kPlaneThreads;
+        if (e < NP * K2) {
+            const int p = e / K2, z = e - p * K2;
+            const double2 v = lds[e];
+            dst[(size_t)(2 * p) * K2 + z] = v.x;
+            if (2 * p + 1 < K1) dst[(size_t)(2 * p + 1) * K2 + z] = v.y;
+        }
+    }
+}
+
 }  // namespace
 
 static inline int nblk(long long n, int b) { return (int)((n + b - 1) / b); }
+
+// LDS entries of one plane: packed z lines and y lines of the half spectrum
+static size_t plane_entries(const PmeFft& F) {
+    const size_t K1 = F.ax[1].n, K2 = F.ax[2].n, Kh = K2 / 2 + 1, NP = (K1 + 1) / 2;
+    return std::max(NP * K2, Kh * K1);
+}
+
+bool fft_plane_ok(const PmeFft& F) {
+    const int K1 = F.ax[1].n, K2 = F.ax[2].n, Kh = K2 / 2 + 1, NP = (K1 + 1) / 2;
+    if (plane_entries(F) > (size_t)kPlaneCopies * kPlaneThreads) return false;
+    if ((size_t)NP * Kh > (size_t)kPlanePairs * kPlaneThreads) return false;
+    if (plane_entries(F) * sizeof(double2) > 160 * 1024) return false;
+    // every stage of both axes within kPlaneBpt butterflies per thread
+    for (int s = 0; s < F.ax[2].nfac; ++s)
+        if (NP * F.ax[2].nb[s] > kPlaneBpt * kPlaneThreads) return false;
+    for (int s = 0; s < F.ax[1].nfac; ++s)
+        if (Kh * F.ax[1].nb[s] > kPlaneBpt * kPlaneThreads) return false;
+    return true;
+}
+
+void launch_fft_plane_fwd(hipStream_t s, const PmeFft& F, const double* real, double2* H, const KEvents& ev,
+                          const int* stop) {
+    const int K1 = F.ax[1].n, Kh = F.ax[2].n / 2 + 1;
+    static const hipError_t attr = hipFuncSetAttribute((const void*)k_fft_plane_fwd,
+                                                       hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    (void)attr;  // a failed launch shows as a launch error
+    MBPOL_LAUNCH(ev, k_fft_plane_fwd, F.ax[0].n, kPlaneThreads, plane_entries(F) * sizeof(double2), s, real, H, K1, Kh,
+                 F.khp, F.ax[1], F.ax[2], stop);
+}
+void launch_fft_plane_inv(hipStream_t s, const PmeFft& F, const double2* H, double* real, const KEvents& ev,
+                          const int* stop) {
+    const int K1 = F.ax[1].n, Kh = F.ax[2].n / 2 + 1;
+    static const hipError_t attr = hipFuncSetAttribute((const void*)k_fft_plane_inv,
+                                                       hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    (void)attr;
+    MBPOL_LAUNCH(ev, k_fft_plane_inv, F.ax[0].n, kPlaneThreads, plane_entries(F) * sizeof(double2), s, H, real, K1, Kh,
+                 F.khp, F.ax[1], F.ax[2], stop);
+}
 
 // C: elements a thread moves per copy phase, >= ceil(n / P) (at most 13 for the supported lengths)
 static int copies(const FftAxis& a) {
