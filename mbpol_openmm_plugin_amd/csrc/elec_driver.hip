@@ -637,12 +637,14 @@ static int pme_grid(mbpol_context* ctx, int mode, const double* mu, const int* s
 // convergence test: each step's last kernel applies the stopping rules on the
 // device (SolveCtl), and every kernel of a later step returns at once when the
 // solve has stopped.  The host reads the state once per batch.  The first batch
-// covers the previous solve's length less one step, later batches two steps;
-// with timing on, or with hipFFT transforms (which cannot skip), one step.
-static int solve_batch(const Elec* E, const mbpol_context* ctx, int done) {
+// covers the previous solve's length (a solve as long as the last one ends with
+// one read and no step that returns at once); later batches take 1, 2, 4, ...
+// steps (a step that only returns costs ~8 launches, a read ~35 us).  With
+// timing on, or with hipFFT transforms (which cannot skip), one step per batch.
+static int solve_batch(const Elec* E, const mbpol_context* ctx, int batch) {
     if (ctx->timer.enabled || !E->lds_fft) return 1;
-    if (done == 0) return std::max(1, E->last_iters - 1);
-    return 2;
+    if (batch == 0) return std::max(1, E->last_iters);
+    return 1 << std::min(batch - 1, 4);
 }
 
 static int read_ctl(mbpol_context* ctx, SolveCtl& out) {
@@ -669,10 +671,10 @@ static int solve_sor(mbpol_context* ctx, double* buf0, double* buf1, double self
     const int* stop = &ctl->stop;
     double* buf[2] = {buf0, buf1};
     HIP_TRY(hipMemsetAsync(ctl, 0, sizeof(SolveCtl), st));
-    int t = 0;
+    int t = 0, batch = 0;
     c = SolveCtl{};
     while (t <= s.el_max_iter) {
-        const int nb = std::min(solve_batch(E, ctx, t), s.el_max_iter + 1 - t);
+        const int nb = std::min(solve_batch(E, ctx, batch++), s.el_max_iter + 1 - t);
         for (int b = 0; b < nb; ++b, ++t) {
             timer_start(ctx, T_ELEC_ITER);
             if (P.pme)
@@ -743,9 +745,9 @@ static int solve_pcg(mbpol_context* ctx, double* x, double self_term, SolveCtl& 
     HIP_TRY(hipMemsetAsync(ctl, 0, sizeof(SolveCtl), st));
     // field evaluation f (0-based): f = 0 forms the initial residual and direction 0,
     // f = k + 1 takes step k and forms direction k + 1
-    int f = 0;
+    int f = 0, batch = 0;
     while (f <= s.el_max_iter) {
-        const int nb = std::min(solve_batch(E, ctx, f), s.el_max_iter + 1 - f);
+        const int nb = std::min(solve_batch(E, ctx, batch++), s.el_max_iter + 1 - f);
         for (int b = 0; b < nb; ++b, ++f) {
             timer_start(ctx, T_ELEC_ITER);
             if (f == 0) {
