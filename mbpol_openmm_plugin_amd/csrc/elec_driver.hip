@@ -179,6 +179,7 @@ static bool fft_factor(int n, FftAxis& a) {
     }
     a.lgP = 0;
     while ((1 << a.lgP) < maxnb && a.lgP < 7) ++a.lgP;
+    a.inplace = (1 << a.lgP) >= maxnb;
     return true;
 }
 

@@ -16,6 +16,7 @@ struct FftAxis {
     int fac[12], ns[12], nb[12], step[12];
     float inv_ns[12];
     int lgP;
+    int inplace;  // every stage has at most 2^lgP butterflies: stages run in place (one LDS buffer)
     const double2* tw;
 };
 

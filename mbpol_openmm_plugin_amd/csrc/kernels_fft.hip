@@ -181,222 +181,11 @@ __device__ double2* fft_line(double2* a, double2* b, bool active, const FftAxis&
     return a;
 }
 
-// z pass, forward: lines of the real grid -> half spectrum
-template <int C>
-__global__ void __launch_bounds__(kFftThreads) k_fft_z_r2c(const double* __restrict__ real, double2* __restrict__ H,
-                                                           int nlines, FftAxis az, int Kh, int Khp,
-                                                           const int* __restrict__ stop) {
-    critical_path_wave();
-    extern __shared__ double2 lds[];
-    if (stop && *stop) return;
-    const int N = az.n, M = kFftThreads >> az.lgP, P = 1 << az.lgP;
-    const int L0 = blockIdx.x * M, m = min(M, nlines - L0);
-    const int l = threadIdx.x >> az.lgP, lane = threadIdx.x & (P - 1);
-    {  // all loads in flight before the LDS stores
-        double v[C];
-#pragma unroll
-        for (int u = 0; u < C; ++u) {
-            const int e = threadIdx.x + u * kFftThreads;
-            if (e < m * N) v[u] = real[(size_t)L0 * N + e];
-        }
-#pragma unroll
-        for (int u = 0; u < C; ++u) {
-            const int e = threadIdx.x + u * kFftThreads;
-            if (e < m * N) lds[e] = make_double2(v[u], 0.0);
-        }
-    }
-    __syncthreads();
-    const double2* r = fft_line(lds + l * N, lds + (M + l) * N, l < m, az, false);
-    if (l < m) {
-#pragma unroll
-        for (int u = 0; u < C; ++u) {
-            const int k = lane + u * P;
-            if (k < Kh) {
-                H[(size_t)(L0 + l) * Khp + k].x = r[k].x;
-                H[(size_t)(L0 + l) * Khp + k].y = r[k].y;
-            }
-        }
-    }
-}
-
-// y pass (either direction): block = (x, tile of M consecutive kz)
-template <int C>
-__global__ void __launch_bounds__(kFftThreads) k_fft_y(double2* __restrict__ H, int K1, int Kh, int Khp, FftAxis ay,
-                                                       int inv, const int* __restrict__ stop) {
-    critical_path_wave();
-    extern __shared__ double2 lds[];
-    if (stop && *stop) return;
-    const int N = ay.n, lgM = 7 - ay.lgP, M = 1 << lgM;
-    const int ntile = (Kh + M - 1) >> lgM;
-    const int x = blockIdx.x / ntile, kz0 = (blockIdx.x - x * ntile) * M, m = min(M, Kh - kz0);
-    const int cl = threadIdx.x & (M - 1), cy = threadIdx.x >> lgM;  // copy mapping: consecutive kz per row
-    const int ystep = kFftThreads >> lgM;
-    if (cl < m) {
-        double vx[C], vy[C];
-#pragma unroll
-        for (int u = 0; u < C; ++u) {
-            const int y = cy + u * ystep;
-            if (y < N) {
-                const double2* src = H + ((size_t)x * K1 + y) * Khp + kz0 + cl;
-                vx[u] = src->x;
-                vy[u] = src->y;
-            }
-        }
-#pragma unroll
-        for (int u = 0; u < C; ++u) {
-            const int y = cy + u * ystep;
-            if (y < N) {
-                lds[cl * N + y].x = vx[u];
-                lds[cl * N + y].y = vy[u];
-            }
-        }
-    }
-    __syncthreads();
-    const int l = threadIdx.x >> ay.lgP;
-    const double2* r = fft_line(lds + l * N, lds + (M + l) * N, l < m, ay, inv != 0);
-    const int off = (int)(r - (lds + l * N));  // 0 or M N: which buffer holds the result
-    if (cl < m) {
-#pragma unroll
-        for (int u = 0; u < C; ++u) {
-            const int y = cy + u * ystep;
-            if (y < N) {
-                double2* dst = H + ((size_t)x * K1 + y) * Khp + kz0 + cl;
-                dst->x = lds[off + cl * N + y].x;
-                dst->y = lds[off + cl * N + y].y;
-            }
-        }
-    }
-}
-
-// x pass: forward, influence function, inverse; block = (y, tile of M consecutive kz)
-template <int C>
-__global__ void __launch_bounds__(kFftThreads) k_fft_x_conv(double2* __restrict__ H, const double* __restrict__ influence,
-                                                            int K1, int Kh, int Khp, FftAxis ax,
-                                                            const int* __restrict__ stop) {
-    critical_path_wave();
-    extern __shared__ double2 lds[];
-    if (stop && *stop) return;
-    const int N = ax.n, lgM = 7 - ax.lgP, M = 1 << lgM, P = 1 << ax.lgP;
-    const int ntile = (Kh + M - 1) >> lgM;
-    const int y = blockIdx.x / ntile, kz0 = (blockIdx.x - y * ntile) * M, m = min(M, Kh - kz0);
-    const int cl = threadIdx.x & (M - 1), cx = threadIdx.x >> lgM;
-    const int xstep = kFftThreads >> lgM;
-    const int l = threadIdx.x >> ax.lgP, lane = threadIdx.x & (P - 1);
-    double f[C];  // influence factors of this thread's k-space points, fetched early
-#pragma unroll
-    for (int u = 0; u < C; ++u) {
-        const int kx = lane + u * P;
-        f[u] = (l < m && kx < N) ? influence[((size_t)kx * K1 + y) * Khp + kz0 + l] : 0.0;
-    }
-    if (cl < m) {
-        double vx[C], vy[C];
-#pragma unroll
-        for (int u = 0; u < C; ++u) {
-            const int x = cx + u * xstep;
-            if (x < N) {
-                const double2* src = H + ((size_t)x * K1 + y) * Khp + kz0 + cl;
-                vx[u] = src->x;
-                vy[u] = src->y;
-            }
-        }
-#pragma unroll
-        for (int u = 0; u < C; ++u) {
-            const int x = cx + u * xstep;
-            if (x < N) {
-                lds[cl * N + x].x = vx[u];
-                lds[cl * N + x].y = vy[u];
-            }
-        }
-    }
-    __syncthreads();
-    double2* a = lds + l * N;
-    double2* b = lds + (M + l) * N;
-    double2* r = fft_line(a, b, l < m, ax, false);
-    if (l < m) {
-#pragma unroll
-        for (int u = 0; u < C; ++u) {
-            const int kx = lane + u * P;
-            if (kx < N) {
-                r[kx].x = r[kx].x * f[u];
-                r[kx].y = r[kx].y * f[u];
-            }
-        }
-    }
-    __syncthreads();
-    r = fft_line(r, r == a ? b : a, l < m, ax, true);
-    const int off = (int)(r - (lds + l * N));
-    if (cl < m) {
-#pragma unroll
-        for (int u = 0; u < C; ++u) {
-            const int x = cx + u * xstep;
-            if (x < N) {
-                double2* dst = H + ((size_t)x * K1 + y) * Khp + kz0 + cl;
-                dst->x = lds[off + cl * N + x].x;
-                dst->y = lds[off + cl * N + x].y;
-            }
-        }
-    }
-}
-
-// z pass, inverse: half spectrum -> real lines (kz >= Kh from conj(H[N - kz]))
-template <int C>
-__global__ void __launch_bounds__(kFftThreads) k_fft_z_c2r(const double2* __restrict__ H, double* __restrict__ real,
-                                                           int nlines, FftAxis az, int Kh, int Khp,
-                                                           const int* __restrict__ stop) {
-    critical_path_wave();
-    extern __shared__ double2 lds[];
-    if (stop && *stop) return;
-    const int N = az.n, M = kFftThreads >> az.lgP, P = 1 << az.lgP;
-    const int L0 = blockIdx.x * M, m = min(M, nlines - L0);
-    const int l = threadIdx.x >> az.lgP, lane = threadIdx.x & (P - 1);
-    double2* a = lds + l * N;
-    if (l < m) {
-        const size_t row = (size_t)(L0 + l) * Khp;
-        double vx[C], vy[C];
-#pragma unroll
-        for (int u = 0; u < C; ++u) {
-            const int k = lane + u * P;
-            if (k < N) {
-                const double2* src = H + row + (k < Kh ? k : N - k);
-                vx[u] = src->x;
-                vy[u] = src->y;
-            }
-        }
-#pragma unroll
-        for (int u = 0; u < C; ++u) {
-            const int k = lane + u * P;
-            if (k < N) {
-                a[k].x = vx[u];
-                a[k].y = k < Kh ? vy[u] : -vy[u];
-            }
-        }
-    }
-    __syncthreads();
-    const double2* r = fft_line(a, lds + (M + l) * N, l < m, az, true);
-    const int off = (int)(r - a);
-#pragma unroll
-    for (int u = 0; u < C; ++u) {
-        const int e = threadIdx.x + u * kFftThreads;
-        if (e < m * N) real[(size_t)L0 * N + e] = lds[off + e].x;
-    }
-}
-
-// ---------------------------------------------------------------------------
-// Plane passes: when one x plane of the grid fits in LDS, the z and y passes
-// of each direction run as one kernel per plane (one workgroup per x), so the
-// half spectrum makes one round trip through HBM less per direction.  Two real
-// z lines (y = 2p, 2p + 1) share one complex transform: Z = r_2p + i r_2p+1,
-// separated afterwards with X_2p[k] = (Z[k] + conj Z[-k]) / 2 and
-// X_2p+1[k] = (Z[k] - conj Z[-k]) / 2i (and the reverse on the way back).
-// Every stage works in place: each thread holds its butterflies in registers
-// across a barrier, so the plane needs a single LDS buffer.
-// ---------------------------------------------------------------------------
-// A plane workgroup takes a whole CU (16 waves, ~118 KB of LDS at 121 x 121),
-// so the plane passes only pay when nothing else runs beside them.
-constexpr int kPlaneThreads = 1024;
-constexpr int kPlaneBpt = 1;     // butterflies per thread and stage (<= 1024 per stage)
-constexpr int kPlaneCopies = 8;  // plane entries per thread in the copy phases (<= 8192)
-constexpr int kPlanePairs = 4;   // (k, -k) pairs per thread in the separation (<= 4096)
+// In-place stages: every thread holds its butterflies in registers across a
+// barrier, so a batch of lines needs a single LDS buffer (the ping-pong of
+// fft_line needs two).  Used when every stage of an axis has at most
+// kInplaceBpt butterflies per thread.
+constexpr int kInplaceBpt = 1;
 
 // DFT of one radix-R butterfly held in vx, vy (inputs in r order, outputs in q
 // order), twiddled for column k of the stage as in butterfly<R>.
@@ -471,17 +260,17 @@ __device__ __forceinline__ void bf_compute(double (&vx)[R], double (&vy)[R], int
 }
 
 // Stage s of the lines buf[l * ld + (0..n-1)], l < L, in place: thread t owns
-// butterflies t, t + blockDim.x, ... (L * nb <= kPlaneBpt * blockDim.x), held in
+// butterflies t, t + blockDim.x, ... (L * nb <= kInplaceBpt * blockDim.x), held in
 // registers across the barrier.  Called by every thread of the block.
 template <int R>
 __device__ __forceinline__ void stage_inplace(double2* buf, int L, int ld, const FftAxis& ax, int s, bool inv) {
     const int Ns = ax.ns[s], nb = ax.nb[s], step = ax.step[s];
     const float inv_nb = 1.0f / (float)nb;
-    double vx[kPlaneBpt][R], vy[kPlaneBpt][R];
-    int base[kPlaneBpt];
-    bool act[kPlaneBpt];
+    double vx[kInplaceBpt][R], vy[kInplaceBpt][R];
+    int base[kInplaceBpt];
+    bool act[kInplaceBpt];
 #pragma unroll
-    for (int t = 0; t < kPlaneBpt; ++t) {
+    for (int t = 0; t < kInplaceBpt; ++t) {
         const int b = threadIdx.x + t * blockDim.x;
         act[t] = b < L * nb;
         base[t] = 0;
@@ -500,7 +289,7 @@ __device__ __forceinline__ void stage_inplace(double2* buf, int L, int ld, const
     }
     __syncthreads();
 #pragma unroll
-    for (int t = 0; t < kPlaneBpt; ++t)
+    for (int t = 0; t < kInplaceBpt; ++t)
         if (act[t]) {
 #pragma unroll
             for (int q = 0; q < R; ++q) {
@@ -524,6 +313,239 @@ __device__ void fft_lines_inplace(double2* buf, int L, int ld, const FftAxis& ax
         }
     }
 }
+
+// z pass, forward: lines of the real grid -> half spectrum
+template <int C>
+__global__ void __launch_bounds__(kFftThreads) k_fft_z_r2c(const double* __restrict__ real, double2* __restrict__ H,
+                                                           int nlines, FftAxis az, int Kh, int Khp,
+                                                           const int* __restrict__ stop) {
+    critical_path_wave();
+    extern __shared__ double2 lds[];
+    if (stop && *stop) return;
+    const int N = az.n, M = kFftThreads >> az.lgP, P = 1 << az.lgP;
+    const int L0 = blockIdx.x * M, m = min(M, nlines - L0);
+    const int l = threadIdx.x >> az.lgP, lane = threadIdx.x & (P - 1);
+    {  // all loads in flight before the LDS stores
+        double v[C];
+#pragma unroll
+        for (int u = 0; u < C; ++u) {
+            const int e = threadIdx.x + u * kFftThreads;
+            if (e < m * N) v[u] = real[(size_t)L0 * N + e];
+        }
+#pragma unroll
+        for (int u = 0; u < C; ++u) {
+            const int e = threadIdx.x + u * kFftThreads;
+            if (e < m * N) lds[e] = make_double2(v[u], 0.0);
+        }
+    }
+    __syncthreads();
+    const double2* r = lds + l * N;
+    if (az.inplace)
+        fft_lines_inplace(lds, m, N, az, false);
+    else
+        r = fft_line(lds + l * N, lds + (M + l) * N, l < m, az, false);
+    if (l < m) {
+#pragma unroll
+        for (int u = 0; u < C; ++u) {
+            const int k = lane + u * P;
+            if (k < Kh) {
+                H[(size_t)(L0 + l) * Khp + k].x = r[k].x;
+                H[(size_t)(L0 + l) * Khp + k].y = r[k].y;
+            }
+        }
+    }
+}
+
+// y pass (either direction): block = (x, tile of M consecutive kz)
+template <int C>
+__global__ void __launch_bounds__(kFftThreads) k_fft_y(double2* __restrict__ H, int K1, int Kh, int Khp, FftAxis ay,
+                                                       int inv, const int* __restrict__ stop) {
+    critical_path_wave();
+    extern __shared__ double2 lds[];
+    if (stop && *stop) return;
+    const int N = ay.n, lgM = 7 - ay.lgP, M = 1 << lgM;
+    const int ntile = (Kh + M - 1) >> lgM;
+    const int x = blockIdx.x / ntile, kz0 = (blockIdx.x - x * ntile) * M, m = min(M, Kh - kz0);
+    const int cl = threadIdx.x & (M - 1), cy = threadIdx.x >> lgM;  // copy mapping: consecutive kz per row
+    const int ystep = kFftThreads >> lgM;
+    if (cl < m) {
+        double vx[C], vy[C];
+#pragma unroll
+        for (int u = 0; u < C; ++u) {
+            const int y = cy + u * ystep;
+            if (y < N) {
+                const double2* src = H + ((size_t)x * K1 + y) * Khp + kz0 + cl;
+                vx[u] = src->x;
+                vy[u] = src->y;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < C; ++u) {
+            const int y = cy + u * ystep;
+            if (y < N) {
+                lds[cl * N + y].x = vx[u];
+                lds[cl * N + y].y = vy[u];
+            }
+        }
+    }
+    __syncthreads();
+    const int l = threadIdx.x >> ay.lgP;
+    int off = 0;  // 0 or M N: which buffer holds the result
+    if (ay.inplace)
+        fft_lines_inplace(lds, m, N, ay, inv != 0);
+    else
+        off = (int)(fft_line(lds + l * N, lds + (M + l) * N, l < m, ay, inv != 0) - (lds + l * N));
+    if (cl < m) {
+#pragma unroll
+        for (int u = 0; u < C; ++u) {
+            const int y = cy + u * ystep;
+            if (y < N) {
+                double2* dst = H + ((size_t)x * K1 + y) * Khp + kz0 + cl;
+                dst->x = lds[off + cl * N + y].x;
+                dst->y = lds[off + cl * N + y].y;
+            }
+        }
+    }
+}
+
+// x pass: forward, influence function, inverse; block = (y, tile of M consecutive kz)
+template <int C>
+__global__ void __launch_bounds__(kFftThreads) k_fft_x_conv(double2* __restrict__ H, const double* __restrict__ influence,
+                                                            int K1, int Kh, int Khp, FftAxis ax,
+                                                            const int* __restrict__ stop) {
+    critical_path_wave();
+    extern __shared__ double2 lds[];
+    if (stop && *stop) return;
+    const int N = ax.n, lgM = 7 - ax.lgP, M = 1 << lgM, P = 1 << ax.lgP;
+    const int ntile = (Kh + M - 1) >> lgM;
+    const int y = blockIdx.x / ntile, kz0 = (blockIdx.x - y * ntile) * M, m = min(M, Kh - kz0);
+    const int cl = threadIdx.x & (M - 1), cx = threadIdx.x >> lgM;
+    const int xstep = kFftThreads >> lgM;
+    const int l = threadIdx.x >> ax.lgP, lane = threadIdx.x & (P - 1);
+    double f[C];  // influence factors of this thread's k-space points, fetched early
+#pragma unroll
+    for (int u = 0; u < C; ++u) {
+        const int kx = lane + u * P;
+        f[u] = (l < m && kx < N) ? influence[((size_t)kx * K1 + y) * Khp + kz0 + l] : 0.0;
+    }
+    if (cl < m) {
+        double vx[C], vy[C];
+#pragma unroll
+        for (int u = 0; u < C; ++u) {
+            const int x = cx + u * xstep;
+            if (x < N) {
+                const double2* src = H + ((size_t)x * K1 + y) * Khp + kz0 + cl;
+                vx[u] = src->x;
+                vy[u] = src->y;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < C; ++u) {
+            const int x = cx + u * xstep;
+            if (x < N) {
+                lds[cl * N + x].x = vx[u];
+                lds[cl * N + x].y = vy[u];
+            }
+        }
+    }
+    __syncthreads();
+    double2* a = lds + l * N;
+    double2* b = lds + (M + l) * N;
+    double2* r = a;
+    if (ax.inplace)
+        fft_lines_inplace(lds, m, N, ax, false);
+    else
+        r = fft_line(a, b, l < m, ax, false);
+    if (l < m) {
+#pragma unroll
+        for (int u = 0; u < C; ++u) {
+            const int kx = lane + u * P;
+            if (kx < N) {
+                r[kx].x = r[kx].x * f[u];
+                r[kx].y = r[kx].y * f[u];
+            }
+        }
+    }
+    __syncthreads();
+    if (ax.inplace)
+        fft_lines_inplace(lds, m, N, ax, true);
+    else
+        r = fft_line(r, r == a ? b : a, l < m, ax, true);
+    const int off = (int)(r - (lds + l * N));
+    if (cl < m) {
+#pragma unroll
+        for (int u = 0; u < C; ++u) {
+            const int x = cx + u * xstep;
+            if (x < N) {
+                double2* dst = H + ((size_t)x * K1 + y) * Khp + kz0 + cl;
+                dst->x = lds[off + cl * N + x].x;
+                dst->y = lds[off + cl * N + x].y;
+            }
+        }
+    }
+}
+
+// z pass, inverse: half spectrum -> real lines (kz >= Kh from conj(H[N - kz]))
+template <int C>
+__global__ void __launch_bounds__(kFftThreads) k_fft_z_c2r(const double2* __restrict__ H, double* __restrict__ real,
+                                                           int nlines, FftAxis az, int Kh, int Khp,
+                                                           const int* __restrict__ stop) {
+    critical_path_wave();
+    extern __shared__ double2 lds[];
+    if (stop && *stop) return;
+    const int N = az.n, M = kFftThreads >> az.lgP, P = 1 << az.lgP;
+    const int L0 = blockIdx.x * M, m = min(M, nlines - L0);
+    const int l = threadIdx.x >> az.lgP, lane = threadIdx.x & (P - 1);
+    double2* a = lds + l * N;
+    if (l < m) {
+        const size_t row = (size_t)(L0 + l) * Khp;
+        double vx[C], vy[C];
+#pragma unroll
+        for (int u = 0; u < C; ++u) {
+            const int k = lane + u * P;
+            if (k < N) {
+                const double2* src = H + row + (k < Kh ? k : N - k);
+                vx[u] = src->x;
+                vy[u] = src->y;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < C; ++u) {
+            const int k = lane + u * P;
+            if (k < N) {
+                a[k].x = vx[u];
+                a[k].y = k < Kh ? vy[u] : -vy[u];
+            }
+        }
+    }
+    __syncthreads();
+    const double2* r = a;
+    if (az.inplace)
+        fft_lines_inplace(lds, m, N, az, true);
+    else
+        r = fft_line(a, lds + (M + l) * N, l < m, az, true);
+    const int off = (int)(r - a);
+#pragma unroll
+    for (int u = 0; u < C; ++u) {
+        const int e = threadIdx.x + u * kFftThreads;
+        if (e < m * N) real[(size_t)L0 * N + e] = lds[off + e].x;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Plane passes: when one x plane of the grid fits in LDS, the z and y passes
+// of each direction run as one kernel per plane (one workgroup per x), so the
+// half spectrum makes one round trip through HBM less per direction.  Two real
+// z lines (y = 2p, 2p + 1) share one complex transform: Z = r_2p + i r_2p+1,
+// separated afterwards with X_2p[k] = (Z[k] + conj Z[-k]) / 2 and
+// X_2p+1[k] = (Z[k] - conj Z[-k]) / 2i (and the reverse on the way back).
+// Every stage works in place, so the plane needs a single LDS buffer.
+// ---------------------------------------------------------------------------
+// A plane workgroup takes a whole CU (16 waves, ~118 KB of LDS at 121 x 121),
+// so the plane passes only pay when nothing else runs beside them.
+constexpr int kPlaneThreads = 1024;
+constexpr int kPlaneCopies = 8;  // plane entries per thread in the copy phases (<= 8192)
+constexpr int kPlanePairs = 4;   // (k, -k) pairs per thread in the separation (<= 4096)
 
 // Forward, plane x = blockIdx.x: real[x][y][z] -> z transform (two lines per
 // complex transform) -> y transform -> H[x][ky][kz], kz < Kh.
@@ -677,11 +699,11 @@ bool fft_plane_ok(const PmeFft& F) {
     if (plane_entries(F) > (size_t)kPlaneCopies * kPlaneThreads) return false;
     if ((size_t)NP * Kh > (size_t)kPlanePairs * kPlaneThreads) return false;
     if (plane_entries(F) * sizeof(double2) > 160 * 1024) return false;
-    // every stage of both axes within kPlaneBpt butterflies per thread
+    // every stage of both axes within kInplaceBpt butterflies per thread
     for (int s = 0; s < F.ax[2].nfac; ++s)
-        if (NP * F.ax[2].nb[s] > kPlaneBpt * kPlaneThreads) return false;
+        if (NP * F.ax[2].nb[s] > kInplaceBpt * kPlaneThreads) return false;
     for (int s = 0; s < F.ax[1].nfac; ++s)
-        if (Kh * F.ax[1].nb[s] > kPlaneBpt * kPlaneThreads) return false;
+        if (Kh * F.ax[1].nb[s] > kInplaceBpt * kPlaneThreads) return false;
     return true;
 }
 
@@ -716,7 +738,10 @@ static int copies(const FftAxis& a) {
         default: MBPOL_LAUNCH(ev, kern<16>, __VA_ARGS__); break; \
     }
 
-static size_t lds_bytes(const FftAxis& a) { return 2 * sizeof(double2) * (size_t)(kFftThreads >> a.lgP) * a.n; }
+// one buffer of M lines for the in-place stages, two for the ping-pong ones
+static size_t lds_bytes(const FftAxis& a) {
+    return (a.inplace ? 1 : 2) * sizeof(double2) * (size_t)(kFftThreads >> a.lgP) * a.n;
+}
 
 void launch_fft_z_r2c(hipStream_t s, const PmeFft& F, const double* real, double2* H, const KEvents& ev,
                       const int* stop) {
