@@ -67,13 +67,18 @@ void launch_trimer_pass(hipStream_t s, int n, const int* idx, const double* pos,
                         const int* lower, int full_list, long long* count, const long long* offset, int* trip);
 
 // ---- kernels_pip.hip (one object per body order and precision) ----
+// max_blocks > 0 caps the grid (64-lane blocks); the kernels then stride over the list
 void launch_twobody_f32(hipStream_t s, long long npairs, const int2* pairs, const int* idx, const double* pos,
-                        int periodic, d3 box, u64* force, u64* energy, const KEvents& ev = KEvents(), double* eunit = nullptr);
+                        int periodic, d3 box, u64* force, u64* energy, const KEvents& ev = KEvents(), double* eunit = nullptr,
+                        int max_blocks = 0);
 void launch_twobody_f64(hipStream_t s, long long npairs, const int2* pairs, const int* idx, const double* pos,
-                        int periodic, d3 box, u64* force, u64* energy, const KEvents& ev = KEvents(), double* eunit = nullptr);
+                        int periodic, d3 box, u64* force, u64* energy, const KEvents& ev = KEvents(), double* eunit = nullptr,
+                        int max_blocks = 0);
 void launch_threebody_f32(hipStream_t s, long long ntrip, const int* trip, const int* idx, const double* pos,
-                          int periodic, d3 box, u64* force, u64* energy, const KEvents& ev = KEvents(), double* eunit = nullptr);
+                          int periodic, d3 box, u64* force, u64* energy, const KEvents& ev = KEvents(), double* eunit = nullptr,
+                        int max_blocks = 0);
 void launch_threebody_f64(hipStream_t s, long long ntrip, const int* trip, const int* idx, const double* pos,
-                          int periodic, d3 box, u64* force, u64* energy, const KEvents& ev = KEvents(), double* eunit = nullptr);
+                          int periodic, d3 box, u64* force, u64* energy, const KEvents& ev = KEvents(), double* eunit = nullptr,
+                        int max_blocks = 0);
 
 }  // namespace mbpol

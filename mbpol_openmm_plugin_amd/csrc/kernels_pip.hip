@@ -141,79 +141,82 @@ __global__ void __launch_bounds__(64) k_twobody(long long np, const int2* __rest
                                                 const double* __restrict__ pos, int periodic, d3 boxA,
                                                 u64* __restrict__ force, u64* __restrict__ energy,
                                                 double* __restrict__ eunit) {
-    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    double e_out = 0.0;
-    int at[6];
-    vr R[10];
-    d3 dOO;
-    double rOO;
-    if (t < np && dimer_geometry(t, pairs, idx, pos, periodic, boxA, at, R, dOO, rOO)) {
-        real x[31], g[31];
+    // a capped grid strides over the list in whole waves
+    for (long long t0 = (long long)blockIdx.x * blockDim.x; t0 < np; t0 += (long long)gridDim.x * blockDim.x) {
+        const long long t = t0 + threadIdx.x;
+        double e_out = 0.0;
+        int at[6];
+        vr R[10];
+        d3 dOO;
+        double rOO;
+        if (t < np && dimer_geometry(t, pairs, idx, pos, periodic, boxA, at, R, dOO, rOO)) {
+            real x[31], g[31];
 #pragma unroll
-        for (int n = 0; n < 31; ++n) {
-            const Var2 v = var2(n);
-            const vr d = R[v.a] - R[v.b];
-            const real r = sqrt(d.dot(d));
-            const real e = exp_t<real>((real)v.k * ((real)v.r0 - r));
-            x[n] = v.coul ? e / r : e;
-        }
-        real Ep = pip2b_eval<real>(x, g);
-        // Only t, Ep and g stay live across the polynomial: the geometry is
-        // reloaded from memory (the opaque copy of t keeps the compiler from
-        // holding the first loads in registers through the polynomial).
-        long long tt = t;
-        asm volatile("" : "+v"(tt));
-        asm volatile("" : "+v"(Ep));
-#pragma unroll
-        for (int n = 0; n < 31; ++n) asm volatile("" : "+v"(g[n]));
-        dimer_geometry(tt, pairs, idx, pos, periodic, boxA, at, R, dOO, rOO);
-        // chain rule through the variables
-        vr F[10];
-#pragma unroll
-        for (int q = 0; q < 10; ++q) F[q] = vr(0, 0, 0);
-#pragma unroll
-        for (int n = 0; n < 31; ++n) {
-            const Var2 v = var2(n);
-            const vr d = R[v.a] - R[v.b];
-            const real r = sqrt(d.dot(d));
-            const real e = exp_t<real>((real)v.k * ((real)v.r0 - r));
-            real gg;
-            if (v.coul) {
-                const real ri = (real)1 / r, val = e * ri;
-                gg = -((real)v.k + ri) * val * ri;
-            } else {
-                gg = -(real)v.k * e / r;
+            for (int n = 0; n < 31; ++n) {
+                const Var2 v = var2(n);
+                const vr d = R[v.a] - R[v.b];
+                const real r = sqrt(d.dot(d));
+                const real e = exp_t<real>((real)v.k * ((real)v.r0 - r));
+                x[n] = v.coul ? e / r : e;
             }
-            const vr dd = d * (gg * g[n]);
-            F[v.a] += dd;
-            F[v.b] -= dd;
-        }
-        extra_grads(R[0], R[1], R[2], F[6], F[7], F[0], F[1], F[2]);
-        extra_grads(R[3], R[4], R[5], F[8], F[9], F[3], F[4], F[5]);
-        // switch (mbpol_2body_constants.cpp:98-111), FP64
-        double sw, gsw;
-        if (rOO > tb_r2i) {
-            const double t1 = M_PI / (tb_r2f - tb_r2i);
-            const double xs = (rOO - tb_r2i) * t1;
-            gsw = -sin(xs) * t1 / 2.0;
-            sw = (1.0 + cos(xs)) / 2.0;
-        } else {
-            sw = 1.0;
-            gsw = 0.0;
-        }
-        const double s = sw * cal2joule * -10.;
-        const double gs = gsw * (double)Ep / rOO * cal2joule * -10.;
+            real Ep = pip2b_eval<real>(x, g);
+            // Only t, Ep and g stay live across the polynomial: the geometry is
+            // reloaded from memory (the opaque copy of t keeps the compiler from
+            // holding the first loads in registers through the polynomial).
+            long long tt = t;
+            asm volatile("" : "+v"(tt));
+            asm volatile("" : "+v"(Ep));
 #pragma unroll
-        for (int q = 0; q < 6; ++q) {
-            double fx = (double)F[q].x * s, fy = (double)F[q].y * s, fz = (double)F[q].z * s;
-            if (q == 0) { fx += gs * dOO.x; fy += gs * dOO.y; fz += gs * dOO.z; }
-            if (q == 3) { fx -= gs * dOO.x; fy -= gs * dOO.y; fz -= gs * dOO.z; }
-            add_force(force, at[q], fx, fy, fz);
+            for (int n = 0; n < 31; ++n) asm volatile("" : "+v"(g[n]));
+            dimer_geometry(tt, pairs, idx, pos, periodic, boxA, at, R, dOO, rOO);
+            // chain rule through the variables
+            vr F[10];
+#pragma unroll
+            for (int q = 0; q < 10; ++q) F[q] = vr(0, 0, 0);
+#pragma unroll
+            for (int n = 0; n < 31; ++n) {
+                const Var2 v = var2(n);
+                const vr d = R[v.a] - R[v.b];
+                const real r = sqrt(d.dot(d));
+                const real e = exp_t<real>((real)v.k * ((real)v.r0 - r));
+                real gg;
+                if (v.coul) {
+                    const real ri = (real)1 / r, val = e * ri;
+                    gg = -((real)v.k + ri) * val * ri;
+                } else {
+                    gg = -(real)v.k * e / r;
+                }
+                const vr dd = d * (gg * g[n]);
+                F[v.a] += dd;
+                F[v.b] -= dd;
+            }
+            extra_grads(R[0], R[1], R[2], F[6], F[7], F[0], F[1], F[2]);
+            extra_grads(R[3], R[4], R[5], F[8], F[9], F[3], F[4], F[5]);
+            // switch (mbpol_2body_constants.cpp:98-111), FP64
+            double sw, gsw;
+            if (rOO > tb_r2i) {
+                const double t1 = M_PI / (tb_r2f - tb_r2i);
+                const double xs = (rOO - tb_r2i) * t1;
+                gsw = -sin(xs) * t1 / 2.0;
+                sw = (1.0 + cos(xs)) / 2.0;
+            } else {
+                sw = 1.0;
+                gsw = 0.0;
+            }
+            const double s = sw * cal2joule * -10.;
+            const double gs = gsw * (double)Ep / rOO * cal2joule * -10.;
+#pragma unroll
+            for (int q = 0; q < 6; ++q) {
+                double fx = (double)F[q].x * s, fy = (double)F[q].y * s, fz = (double)F[q].z * s;
+                if (q == 0) { fx += gs * dOO.x; fy += gs * dOO.y; fz += gs * dOO.z; }
+                if (q == 3) { fx -= gs * dOO.x; fy -= gs * dOO.y; fz -= gs * dOO.z; }
+                add_force(force, at[q], fx, fy, fz);
+            }
+            e_out = sw * (double)Ep * cal2joule;
         }
-        e_out = sw * (double)Ep * cal2joule;
+        if (eunit && t < np) eunit[t] = e_out;
+        add_energy(energy, e_out);
     }
-    if (eunit && t < np) eunit[t] = e_out;
-    add_energy(energy, e_out);
 }
 
 #else
@@ -324,76 +327,79 @@ __global__ void __launch_bounds__(64) k_threebody(long long nt, const int* __res
                                                   const double* __restrict__ pos, int periodic, d3 box,
                                                   u64* __restrict__ force, u64* __restrict__ energy,
                                                   double* __restrict__ eunit) {
-    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    double e_out = 0.0;
-    int at[9];
-    vr R[9];
-    d3 roo[3];
-    double doo[3];
-    if (t < nt && trimer_geometry(t, trip, idx, pos, periodic, box, at, R, roo, doo)) {
-        real x[36], g[36];
+    // a capped grid strides over the list in whole waves
+    for (long long t0 = (long long)blockIdx.x * blockDim.x; t0 < nt; t0 += (long long)gridDim.x * blockDim.x) {
+        const long long t = t0 + threadIdx.x;
+        double e_out = 0.0;
+        int at[9];
+        vr R[9];
+        d3 roo[3];
+        double doo[3];
+        if (t < nt && trimer_geometry(t, trip, idx, pos, periodic, box, at, R, roo, doo)) {
+            real x[36], g[36];
 #pragma unroll
-        for (int n = 0; n < 36; ++n) {
-            const Var3 v = var3(n);
-            const vr d = R[v.a] - R[v.b];
-            const real r = sqrt(d.dot(d));
-            x[n] = exp_t<real>(-(real)v.k * (r - (real)v.r0));
-        }
-        real Ep = pip3b_eval<real>(x, g);
-        // Only t, Ep and g stay live across the polynomial; the geometry is
-        // reloaded (through an opaque copy of t, so that the first loads are
-        // not kept in registers).
-        long long tt = t;
-        asm volatile("" : "+v"(tt));
-        asm volatile("" : "+v"(Ep));
-#pragma unroll
-        for (int n = 0; n < 36; ++n) asm volatile("" : "+v"(g[n]));
-        trimer_geometry(tt, trip, idx, pos, periodic, box, at, R, roo, doo);
-        double gab, gac, gbc;
-        const double sab = switch3(doo[0], gab), sac = switch3(doo[1], gac), sbc = switch3(doo[2], gbc);
-        const double s = sab * sac + sab * sbc + sac * sbc;
-        // g_var (:80-101): forces of the variables, times s; units kJ/mol/nm
-        vr F[9];
-#pragma unroll
-        for (int q = 0; q < 9; ++q) F[q] = vr(0, 0, 0);
-#pragma unroll
-        for (int n = 0; n < 36; ++n) {
-            const Var3 v = var3(n);
-            const vr d = R[v.a] - R[v.b];
-            const real r = sqrt(d.dot(d));
-            const real gg = -(real)v.k * g[n] * exp_t<real>(-(real)v.k * (r - (real)v.r0)) / r;
-            const vr dd = d * gg;
-            F[v.a] += dd;
-            F[v.b] -= dd;
-        }
-        const double fs = s * cal2joule * 10. * -1;
-        const double E = (double)Ep;
-        gab *= (sac + sbc) * E / doo[0];
-        gac *= (sab + sbc) * E / doo[1];
-        gbc *= (sab + sac) * E / doo[2];
-        const double u = cal2joule * -nm_to_A;
-#pragma unroll
-        for (int q = 0; q < 9; ++q) {
-            double fx = (double)F[q].x * fs, fy = (double)F[q].y * fs, fz = (double)F[q].z * fs;
-            if (q == 0) {
-                fx += (gab * roo[0].x + gac * roo[1].x) * u;
-                fy += (gab * roo[0].y + gac * roo[1].y) * u;
-                fz += (gab * roo[0].z + gac * roo[1].z) * u;
-            } else if (q == 3) {
-                fx += (gbc * roo[2].x - gab * roo[0].x) * u;
-                fy += (gbc * roo[2].y - gab * roo[0].y) * u;
-                fz += (gbc * roo[2].z - gab * roo[0].z) * u;
-            } else if (q == 6) {
-                fx -= (gac * roo[1].x + gbc * roo[2].x) * u;
-                fy -= (gac * roo[1].y + gbc * roo[2].y) * u;
-                fz -= (gac * roo[1].z + gbc * roo[2].z) * u;
+            for (int n = 0; n < 36; ++n) {
+                const Var3 v = var3(n);
+                const vr d = R[v.a] - R[v.b];
+                const real r = sqrt(d.dot(d));
+                x[n] = exp_t<real>(-(real)v.k * (r - (real)v.r0));
             }
-            add_force(force, at[q], fx, fy, fz);
+            real Ep = pip3b_eval<real>(x, g);
+            // Only t, Ep and g stay live across the polynomial; the geometry is
+            // reloaded (through an opaque copy of t, so that the first loads are
+            // not kept in registers).
+            long long tt = t;
+            asm volatile("" : "+v"(tt));
+            asm volatile("" : "+v"(Ep));
+#pragma unroll
+            for (int n = 0; n < 36; ++n) asm volatile("" : "+v"(g[n]));
+            trimer_geometry(tt, trip, idx, pos, periodic, box, at, R, roo, doo);
+            double gab, gac, gbc;
+            const double sab = switch3(doo[0], gab), sac = switch3(doo[1], gac), sbc = switch3(doo[2], gbc);
+            const double s = sab * sac + sab * sbc + sac * sbc;
+            // g_var (:80-101): forces of the variables, times s; units kJ/mol/nm
+            vr F[9];
+#pragma unroll
+            for (int q = 0; q < 9; ++q) F[q] = vr(0, 0, 0);
+#pragma unroll
+            for (int n = 0; n < 36; ++n) {
+                const Var3 v = var3(n);
+                const vr d = R[v.a] - R[v.b];
+                const real r = sqrt(d.dot(d));
+                const real gg = -(real)v.k * g[n] * exp_t<real>(-(real)v.k * (r - (real)v.r0)) / r;
+                const vr dd = d * gg;
+                F[v.a] += dd;
+                F[v.b] -= dd;
+            }
+            const double fs = s * cal2joule * 10. * -1;
+            const double E = (double)Ep;
+            gab *= (sac + sbc) * E / doo[0];
+            gac *= (sab + sbc) * E / doo[1];
+            gbc *= (sab + sac) * E / doo[2];
+            const double u = cal2joule * -nm_to_A;
+#pragma unroll
+            for (int q = 0; q < 9; ++q) {
+                double fx = (double)F[q].x * fs, fy = (double)F[q].y * fs, fz = (double)F[q].z * fs;
+                if (q == 0) {
+                    fx += (gab * roo[0].x + gac * roo[1].x) * u;
+                    fy += (gab * roo[0].y + gac * roo[1].y) * u;
+                    fz += (gab * roo[0].z + gac * roo[1].z) * u;
+                } else if (q == 3) {
+                    fx += (gbc * roo[2].x - gab * roo[0].x) * u;
+                    fy += (gbc * roo[2].y - gab * roo[0].y) * u;
+                    fz += (gbc * roo[2].z - gab * roo[0].z) * u;
+                } else if (q == 6) {
+                    fx -= (gac * roo[1].x + gbc * roo[2].x) * u;
+                    fy -= (gac * roo[1].y + gbc * roo[2].y) * u;
+                    fz -= (gac * roo[1].z + gbc * roo[2].z) * u;
+                }
+                add_force(force, at[q], fx, fy, fz);
+            }
+            e_out = E * s * cal2joule;
         }
-        e_out = E * s * cal2joule;
+        if (eunit && t < nt) eunit[t] = e_out;
+        add_energy(energy, e_out);
     }
-    if (eunit && t < nt) eunit[t] = e_out;
-    add_energy(energy, e_out);
 }
 #endif
 
@@ -403,22 +409,26 @@ __global__ void __launch_bounds__(64) k_threebody(long long nt, const int* __res
 #define PIP_CAT2(a, b) a##b
 #define PIP_CAT(a, b) PIP_CAT2(a, b)
 namespace mbpol {
-static inline int nblk(long long n, int b) { return (int)((n + b - 1) / b); }
+// one 64-lane block per 64 units, at most max_blocks (when positive)
+static inline int grid(long long n, int max_blocks) {
+    const long long b = (n + 63) / 64;
+    return (int)(max_blocks > 0 && b > max_blocks ? max_blocks : b);
+}
 #if PIP_BODY == 2
 void PIP_CAT(launch_twobody_, PIP_SUFFIX)(hipStream_t s, long long npairs, const int2* pairs, const int* idx,
                                           const double* pos, int periodic, d3 box, u64* force, u64* energy,
-                                          const KEvents& ev, double* eunit) {
+                                          const KEvents& ev, double* eunit, int max_blocks) {
     const d3 boxA{box.x * 10., box.y * 10., box.z * 10.};
     if (npairs > 0)
-        MBPOL_LAUNCH(ev, k_twobody, nblk(npairs, 64), 64, 0, s, npairs, pairs, idx, pos, periodic, boxA, force, energy,
+        MBPOL_LAUNCH(ev, k_twobody, grid(npairs, max_blocks), 64, 0, s, npairs, pairs, idx, pos, periodic, boxA, force, energy,
                      eunit);
 }
 #else
 void PIP_CAT(launch_threebody_, PIP_SUFFIX)(hipStream_t s, long long ntrip, const int* trip, const int* idx,
                                             const double* pos, int periodic, d3 box, u64* force, u64* energy,
-                                            const KEvents& ev, double* eunit) {
+                                            const KEvents& ev, double* eunit, int max_blocks) {
     if (ntrip > 0)
-        MBPOL_LAUNCH(ev, k_threebody, nblk(ntrip, 64), 64, 0, s, ntrip, trip, idx, pos, periodic, box, force, energy,
+        MBPOL_LAUNCH(ev, k_threebody, grid(ntrip, max_blocks), 64, 0, s, ntrip, trip, idx, pos, periodic, box, force, energy,
                      eunit);
 }
 #endif

@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 
@@ -507,6 +508,19 @@ int mbpol_context_create(const mbpol_system* sys, int32_t precision, int32_t dev
         return fail(set_error(MBPOL_ERR_HIP, "hipEventCreate failed"));
     if (hipHostMalloc(&ctx->h_counts, 4096, hipHostMallocDefault) != hipSuccess)
         return fail(set_error(MBPOL_ERR_HIP, "hipHostMalloc failed"));
+    {
+        // Grid cap of the 2-body and 3-body kernels: one wave per SIMD of the device
+        // (MBPOL_HIP_PIP_WAVES: waves per SIMD, 0 = one block per 64 units).  The
+        // whole capped grid is resident at once, so the dispatcher holds no queue of
+        // their blocks and the electrostatics kernels on the other stream are
+        // dispatched as soon as they are launched.
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, device) != hipSuccess)
+            return fail(set_error(MBPOL_ERR_HIP, "hipGetDeviceProperties failed"));
+        int waves = 1;
+        if (const char* env = std::getenv("MBPOL_HIP_PIP_WAVES")) waves = std::max(0, std::atoi(env));
+        ctx->pip_blocks = waves * 4 * prop.multiProcessorCount;
+    }
     const int n = sys->n;
     if (ctx->pos.ensure(sizeof(double) * 3 * n) || ctx->force_fx.ensure(sizeof(u64) * 3 * n) ||
         ctx->force.ensure(sizeof(double) * 3 * n) || ctx->energy_fx.ensure(sizeof(u64) * 8))
@@ -686,10 +700,10 @@ static int short_range(mbpol_context* ctx, hipStream_t sr, uint32_t comp) {
         const KEvents ev = timer_kernel(ctx, T_TWOBODY);
         if (dp)
             launch_twobody_f64(sr, ctx->stats.dimers, ctx->dimers.as<int2>(), ctx->tb_idx.as<int>(), pos, per, box, F,
-                               E + 1, ev, eu2);
+                               E + 1, ev, eu2, ctx->pip_blocks);
         else
             launch_twobody_f32(sr, ctx->stats.dimers, ctx->dimers.as<int2>(), ctx->tb_idx.as<int>(), pos, per, box, F,
-                               E + 1, ev, eu2);
+                               E + 1, ev, eu2, ctx->pip_blocks);
         timer_kernel_done(ctx, T_TWOBODY, ev);
         // dimers are listed by their first molecule, so a group's dimers are consecutive
         if (G) launch_group_sums(sr, G, ctx->g_wstart.as<int>(), 0, ctx->dimer_off.as<long long>(), eu2, EG, 1);
@@ -699,10 +713,10 @@ static int short_range(mbpol_context* ctx, hipStream_t sr, uint32_t comp) {
         const KEvents ev = timer_kernel(ctx, T_THREEBODY);
         if (dp)
             launch_threebody_f64(sr, ctx->stats.trimers, ctx->trips.as<int>(), ctx->th_idx.as<int>(), pos, per, box, F,
-                                 E + 2, ev, eu3);
+                                 E + 2, ev, eu3, ctx->pip_blocks);
         else
             launch_threebody_f32(sr, ctx->stats.trimers, ctx->trips.as<int>(), ctx->th_idx.as<int>(), pos, per, box, F,
-                                 E + 2, ev, eu3);
+                                 E + 2, ev, eu3, ctx->pip_blocks);
         timer_kernel_done(ctx, T_THREEBODY, ev);
         if (G) launch_group_sums(sr, G, ctx->g_wstart.as<int>(), 0, ctx->trip_off.as<long long>(), eu3, EG, 2);
     }
