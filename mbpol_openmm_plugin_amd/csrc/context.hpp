@@ -127,6 +127,7 @@ struct mbpol_context {
     mbpol::HostWorker worker;  // enqueues the short-range terms on `side`
     bool evaluated = false;    // an evaluation has completed (launch shapes are tuned)
     int pip_blocks = 0;        // grid cap of the 2-body and 3-body kernels (0: one block per 64 units)
+    int disp_blocks = 0;       // grid cap of the dispersion kernel (0: one wave per atom)
     // independent groups: water ranges, per-molecule candidate ranges, per-unit energies and per-group sums
     int ngroups = 0;
     mbpol::DevBuf g_wstart, g_mol_s, g_mol_e, g_eu1, g_eu2, g_eu3, g_energy;

@@ -141,7 +141,7 @@ int disp_evaluate(mbpol_context* ctx, hipStream_t st, u64* energy) {
     launch_disp(st, ctx->precision == MBPOL_PRECISION_DOUBLE, D->g, D->act.as<int>(), pos, D->type.as<int>(),
                 D->mol.as<int>(), D->c6.as<double>(), D->d6.as<double>(), D->cell_of.as<int>(), D->cell_start.as<int>(),
                 D->members.as<int>(), D->cpos.as<double4>(), ctx->force_fx.as<u64>(), D->eblk.as<double>(),
-                D->cnt.as<unsigned long long>());
+                D->cnt.as<unsigned long long>(), ctx->disp_blocks);
     launch_count_total(st, D->g.na, D->cnt.as<unsigned long long>(), D->npairs.as<unsigned long long>());
     launch_energy_finish(st, D->g.na, D->eblk.as<double>(), energy);
     return MBPOL_OK;

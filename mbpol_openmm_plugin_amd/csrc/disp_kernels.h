@@ -25,7 +25,8 @@ void launch_disp_pack(hipStream_t s, int na, const int* members, const int* act,
                       const int* mol, double4* cpos);
 void launch_disp(hipStream_t s, int precision_double, DispGeom g, const int* act, const double* pos, const int* type,
                  const int* mol, const double* c6, const double* d6, const int* cell_of, const int* cell_start,
-                 const int* members, const double4* cpos, u64* force, double* eblk, unsigned long long* cnt);
+                 const int* members, const double4* cpos, u64* force, double* eblk, unsigned long long* cnt,
+                 int max_blocks = 0);  // max_blocks > 0: capped grid, each wave strides over the atoms
 // cnt: per active atom; launch_count_total sums it into *out
 void launch_count_total(hipStream_t s, int na, const unsigned long long* cnt, unsigned long long* out);
 

@@ -54,82 +54,85 @@ __global__ void __launch_bounds__(64) k_disp(DispGeom g, const int* __restrict__
                                              const int* __restrict__ members, const double4* __restrict__ cpos,
                                              u64* __restrict__ force, double* __restrict__ eblk,
                                              unsigned long long* __restrict__ cnt_out) {
-    const int a = blockIdx.x, lane = threadIdx.x;
-    const int i = act[a];
-    const double xi = pos[3 * i], yi = pos[3 * i + 1], zi = pos[3 * i + 2];
-    const int ti = type[i], mi = mol[i];
-    const double rc2 = g.cutoff * g.cutoff;
-    double fx = 0, fy = 0, fz = 0, en = 0;
-    unsigned long long cnt = 0;
-    // q: position of the candidate and its molecule * 16 + type (cell-ordered copy when cells are used)
-    auto visit = [&](const double4& q) {
-        const int key = (int)q.w, mj = key >> 4, tj = key & 15;
-        if (mj == mi) return;  // same molecule, the atom itself included
-        double dx = xi - q.x, dy = yi - q.y, dz = zi - q.z;
-        if (g.periodic) {
-            dx -= floor(dx / g.L.x + 0.5) * g.L.x;
-            dy -= floor(dy / g.L.y + 0.5) * g.L.y;
-            dz -= floor(dz / g.L.z + 0.5) * g.L.z;
-        }
-        const double r2d = dx * dx + dy * dy + dz * dz;
-        if (g.method != 0 && r2d >= rc2) return;
-        const T C = (T)c6[ti + g.ntypes * tj], D = (T)d6[ti + g.ntypes * tj];
-        const T r2 = (T)r2d, r = sqrt(r2), x = D * r;
-        const T s = (T)1 + ((T)1 + ((T)1 + ((T)1 + ((T)1 + ((T)1 + x / (T)6) * x / (T)5) * x / (T)4) * x / (T)3) * x / (T)2) * x;
-        const T ex = exp_t<T>(-x);
-        const T tt6 = (T)1 - s * ex;
-        const T r6 = r2 * r2 * r2;
-        const T x3 = x * x * x;
-        const T dEdr = (T)6 * C * tt6 / (r6 * r) - C * D * (x3 * x3) * ex / ((T)720 * r6);
-        const double fr = -(double)(dEdr / r);
-        fx += fr * dx;
-        fy += fr * dy;
-        fz += fr * dz;
-        en += -(double)(C * tt6 / r6);
-        ++cnt;
-    };
-    if (!g.use_cells) {
-        const int b0 = g.gs ? g.gs[a] : 0, b1 = g.gs ? g.ge[a] : g.na;
-        for (int b = b0 + lane; b < b1; b += 64) {
-            const int j = act[b];
-            visit(make_double4(pos[3 * j], pos[3 * j + 1], pos[3 * j + 2], (double)(mol[j] * 16 + type[j])));
-        }
-    } else {
-        const int cell = cell_of[a];
-        const int cz = cell % g.nc.z, cy = (cell / g.nc.z) % g.nc.y, cx = cell / (g.nc.z * g.nc.y);
-        // the three cells along z of each (X, Y) column are consecutive in the cell order:
-        // one or two contiguous member ranges
-        auto scan = [&](int base, int za, int zb) {
-            const int e = cell_start[base + zb + 1];
-            for (int t = cell_start[base + za] + lane; t < e; t += 64) visit(cpos[t]);
-        };
-        for (int ax = -1; ax <= 1; ++ax)
-            for (int ay = -1; ay <= 1; ++ay) {
-                int X = cx + ax, Y = cy + ay;
-                X = X < 0 ? X + g.nc.x : (X >= g.nc.x ? X - g.nc.x : X);
-                Y = Y < 0 ? Y + g.nc.y : (Y >= g.nc.y ? Y - g.nc.y : Y);
-                const int base = (X * g.nc.y + Y) * g.nc.z, z0 = cz - 1, z1 = cz + 1;
-                if (z0 < 0) {
-                    scan(base, z0 + g.nc.z, g.nc.z - 1);
-                    scan(base, 0, z1);
-                } else if (z1 >= g.nc.z) {
-                    scan(base, z0, g.nc.z - 1);
-                    scan(base, 0, z1 - g.nc.z);
-                } else {
-                    scan(base, z0, z1);
-                }
+    const int lane = threadIdx.x;
+    // one atom per wave; a capped grid strides over the atoms
+    for (int a = blockIdx.x; a < g.na; a += gridDim.x) {
+        const int i = act[a];
+        const double xi = pos[3 * i], yi = pos[3 * i + 1], zi = pos[3 * i + 2];
+        const int ti = type[i], mi = mol[i];
+        const double rc2 = g.cutoff * g.cutoff;
+        double fx = 0, fy = 0, fz = 0, en = 0;
+        unsigned long long cnt = 0;
+        // q: position of the candidate and its molecule * 16 + type (cell-ordered copy when cells are used)
+        auto visit = [&](const double4& q) {
+            const int key = (int)q.w, mj = key >> 4, tj = key & 15;
+            if (mj == mi) return;  // same molecule, the atom itself included
+            double dx = xi - q.x, dy = yi - q.y, dz = zi - q.z;
+            if (g.periodic) {
+                dx -= floor(dx / g.L.x + 0.5) * g.L.x;
+                dy -= floor(dy / g.L.y + 0.5) * g.L.y;
+                dz -= floor(dz / g.L.z + 0.5) * g.L.z;
             }
-    }
-    fx = wave_sum(fx);
-    fy = wave_sum(fy);
-    fz = wave_sum(fz);
-    en = wave_sum(en);
+            const double r2d = dx * dx + dy * dy + dz * dz;
+            if (g.method != 0 && r2d >= rc2) return;
+            const T C = (T)c6[ti + g.ntypes * tj], D = (T)d6[ti + g.ntypes * tj];
+            const T r2 = (T)r2d, r = sqrt(r2), x = D * r;
+            const T s = (T)1 + ((T)1 + ((T)1 + ((T)1 + ((T)1 + ((T)1 + x / (T)6) * x / (T)5) * x / (T)4) * x / (T)3) * x / (T)2) * x;
+            const T ex = exp_t<T>(-x);
+            const T tt6 = (T)1 - s * ex;
+            const T r6 = r2 * r2 * r2;
+            const T x3 = x * x * x;
+            const T dEdr = (T)6 * C * tt6 / (r6 * r) - C * D * (x3 * x3) * ex / ((T)720 * r6);
+            const double fr = -(double)(dEdr / r);
+            fx += fr * dx;
+            fy += fr * dy;
+            fz += fr * dz;
+            en += -(double)(C * tt6 / r6);
+            ++cnt;
+        };
+        if (!g.use_cells) {
+            const int b0 = g.gs ? g.gs[a] : 0, b1 = g.gs ? g.ge[a] : g.na;
+            for (int b = b0 + lane; b < b1; b += 64) {
+                const int j = act[b];
+                visit(make_double4(pos[3 * j], pos[3 * j + 1], pos[3 * j + 2], (double)(mol[j] * 16 + type[j])));
+            }
+        } else {
+            const int cell = cell_of[a];
+            const int cz = cell % g.nc.z, cy = (cell / g.nc.z) % g.nc.y, cx = cell / (g.nc.z * g.nc.y);
+            // the three cells along z of each (X, Y) column are consecutive in the cell order:
+            // one or two contiguous member ranges
+            auto scan = [&](int base, int za, int zb) {
+                const int e = cell_start[base + zb + 1];
+                for (int t = cell_start[base + za] + lane; t < e; t += 64) visit(cpos[t]);
+            };
+            for (int ax = -1; ax <= 1; ++ax)
+                for (int ay = -1; ay <= 1; ++ay) {
+                    int X = cx + ax, Y = cy + ay;
+                    X = X < 0 ? X + g.nc.x : (X >= g.nc.x ? X - g.nc.x : X);
+                    Y = Y < 0 ? Y + g.nc.y : (Y >= g.nc.y ? Y - g.nc.y : Y);
+                    const int base = (X * g.nc.y + Y) * g.nc.z, z0 = cz - 1, z1 = cz + 1;
+                    if (z0 < 0) {
+                        scan(base, z0 + g.nc.z, g.nc.z - 1);
+                        scan(base, 0, z1);
+                    } else if (z1 >= g.nc.z) {
+                        scan(base, z0, g.nc.z - 1);
+                        scan(base, 0, z1 - g.nc.z);
+                    } else {
+                        scan(base, z0, z1);
+                    }
+                }
+        }
+        fx = wave_sum(fx);
+        fy = wave_sum(fy);
+        fz = wave_sum(fz);
+        en = wave_sum(en);
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) cnt += __shfl_xor(cnt, off, 64);
-    if (lane == 0) {
-        add_force(force, i, fx, fy, fz);
-        eblk[a] = 0.5 * en;
-        cnt_out[a] = cnt;
+        for (int off = 32; off > 0; off >>= 1) cnt += __shfl_xor(cnt, off, 64);
+        if (lane == 0) {
+            add_force(force, i, fx, fy, fz);
+            eblk[a] = 0.5 * en;
+            cnt_out[a] = cnt;
+        }
     }
 }
 
@@ -165,13 +168,15 @@ void launch_disp_pack(hipStream_t s, int na, const int* members, const int* act,
 }
 void launch_disp(hipStream_t s, int precision_double, DispGeom g, const int* act, const double* pos, const int* type,
                  const int* mol, const double* c6, const double* d6, const int* cell_of, const int* cell_start,
-                 const int* members, const double4* cpos, u64* force, double* eblk, unsigned long long* cnt) {
+                 const int* members, const double4* cpos, u64* force, double* eblk, unsigned long long* cnt,
+                 int max_blocks) {
     if (g.na <= 0) return;
+    const int grid = max_blocks > 0 && g.na > max_blocks ? max_blocks : g.na;
     if (precision_double)
-        hipLaunchKernelGGL(k_disp<double>, g.na, 64, 0, s, g, act, pos, type, mol, c6, d6, cell_of, cell_start, members,
+        hipLaunchKernelGGL(k_disp<double>, grid, 64, 0, s, g, act, pos, type, mol, c6, d6, cell_of, cell_start, members,
                            cpos, force, eblk, cnt);
     else
-        hipLaunchKernelGGL(k_disp<float>, g.na, 64, 0, s, g, act, pos, type, mol, c6, d6, cell_of, cell_start, members,
+        hipLaunchKernelGGL(k_disp<float>, grid, 64, 0, s, g, act, pos, type, mol, c6, d6, cell_of, cell_start, members,
                            cpos, force, eblk, cnt);
 }
 

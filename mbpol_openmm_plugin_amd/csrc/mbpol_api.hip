@@ -520,6 +520,9 @@ int mbpol_context_create(const mbpol_system* sys, int32_t precision, int32_t dev
         int waves = 1;
         if (const char* env = std::getenv("MBPOL_HIP_PIP_WAVES")) waves = std::max(0, std::atoi(env));
         ctx->pip_blocks = waves * 4 * prop.multiProcessorCount;
+        int dwaves = 2;  // the dispersion kernel: two waves per SIMD (MBPOL_HIP_DISP_WAVES)
+        if (const char* env = std::getenv("MBPOL_HIP_DISP_WAVES")) dwaves = std::max(0, std::atoi(env));
+        ctx->disp_blocks = dwaves * 4 * prop.multiProcessorCount;
     }
     const int n = sys->n;
     if (ctx->pos.ensure(sizeof(double) * 3 * n) || ctx->force_fx.ensure(sizeof(u64) * 3 * n) ||
