@@ -44,14 +44,14 @@ FLOPS_PER_DIMER = 31.0e3
 # x 1.0; profiles/memory_counters_4096w_eval.md holds the runs); kernels not
 # listed report null
 MEASURED_TRAFFIC = {
-    'elec_iterate': 97323924,
-    'pme_spread': 65648313,
-    'fft_z_r2c': 27277445,
-    'fft_y': 27978192,
-    'fft_x_conv': 42220929,
-    'fft_z_c2r': 27629240,
-    'threebody': 228101166,
-    'twobody': 88313151,
+    'elec_iterate': 90976788,
+    'pme_spread': 61711794,
+    'fft_z_r2c': 25781628,
+    'fft_y': 26429641,
+    'fft_x_conv': 39895296,
+    'fft_z_c2r': 26099163,
+    'threebody': 227674289,
+    'twobody': 86748268,
 }
 
 
