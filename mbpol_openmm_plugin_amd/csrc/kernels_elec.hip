@@ -379,7 +379,7 @@ __device__ __forceinline__ void for_rows(int i, int lane, int capP, const int* _
 // fixed field (direct part) at the polarisable sites, :361-420 direct sum,
 // :1342-1407 PME
 // ---------------------------------------------------------------------------
-template <int G>
+template <int G, bool PME>  // PME: Ewald real-space terms (compile-time branch)
 __global__ void __launch_bounds__(256) k_fixed_field(int npol, const int* __restrict__ polidx,
                                                      const double* __restrict__ pos, const SiteParams S, ElecParams P,
                                                      int capP, const int* __restrict__ nbP, const int* __restrict__ cntP,
@@ -394,7 +394,7 @@ __global__ void __launch_bounds__(256) k_fixed_field(int npol, const int* __rest
     for_rows<G>(i, lane, capP, nbP, cntP, capQ, nbQ, cntQ, [&](int j) {
         const int same = S.mol[i] == S.mol[j];
         double dx = pos[3 * j] - xi, dy = pos[3 * j + 1] - yi, dz = pos[3 * j + 2] - zi;
-        if (!P.pme) {
+        if constexpr (!PME) {
             if (same) return;
             const double r2 = dx * dx + dy * dy + dz * dz, r = sqrt(r2);
             const Thole th(S.damp6[i], S.damp6[j], r);
@@ -1355,8 +1355,8 @@ __global__ void k_scale_copy(int n3, const double* __restrict__ a, const double*
 // terms (:791-827) gathered as a potential V (same sums, see the driver)
 // Each directed pair (i, j) adds its share to site i only; energy is halved.
 // ---------------------------------------------------------------------------
-template <int G>
-__global__ void __launch_bounds__(256) k_elec_energy(int n, const double* __restrict__ pos, const SiteParams S,
+template <int G, bool PME>  // PME: Ewald real-space terms (compile-time, so only one branch holds registers)
+__global__ void __launch_bounds__(256, 3) k_elec_energy(int n, const double* __restrict__ pos, const SiteParams S,
                                                      ElecParams P, int capP, const int* __restrict__ nbP,
                                                      const int* __restrict__ cntP, int capQ,
                                                      const int* __restrict__ nbQ, const int* __restrict__ cntQ,
@@ -1391,7 +1391,7 @@ __global__ void __launch_bounds__(256) k_elec_energy(int n, const double* __rest
             const double edd = th.ex(gdd);
             const double s5DD = th.s5(gdd, edd), s7DD = th.s7(gdd, edd);
             double gx, gy, gz;  // (ftm2 + ftm2i): force on i is -f * this
-            if (P.pme) {
+            if constexpr (PME) {
                 const Ewald ew(P.alpha, r, r2);
                 double s1CC = 0., s3CD = 0., s3CC = 0., s5CD = 0.;
                 if (!same) {
@@ -1608,8 +1608,14 @@ void launch_prefactors(hipStream_t s, int npol, const int* polidx, const double*
 void launch_fixed_field(hipStream_t s, int npol, const int* polidx, const double* pos, SiteParams S, ElecParams P,
                         const ElecRows& R, double* field) {
     if (npol > 0)
-        hipLaunchKernelGGL(k_fixed_field<kG>, nblk((long long)npol * kG, 256), 256, 0, s, npol, polidx, pos, S, P,
-                           R.capP, R.nbP, R.cntP, R.capQ, R.nbQ, R.cntQ, field);
+    {
+        if (P.pme)
+            hipLaunchKernelGGL((k_fixed_field<kG, true>), nblk((long long)npol * kG, 256), 256, 0, s, npol, polidx, pos,
+                               S, P, R.capP, R.nbP, R.cntP, R.capQ, R.nbQ, R.cntQ, field);
+        else
+            hipLaunchKernelGGL((k_fixed_field<kG, false>), nblk((long long)npol * kG, 256), 256, 0, s, npol, polidx, pos,
+                               S, P, R.capP, R.nbP, R.cntP, R.capQ, R.nbQ, R.cntQ, field);
+    }
 }
 void launch_site_bins(hipStream_t s, int n, const double* pos, const double* weight, ElecParams P, BrickGeom B,
                       int* bin_of, int* bin_count) {
@@ -1708,8 +1714,12 @@ void launch_scale_copy(hipStream_t s, int n, const double* a, const double* pol,
 int energy_blocks(int n) { return nblk((long long)n * kG, 256); }
 void launch_elec_energy(hipStream_t s, int n, const double* pos, SiteParams S, ElecParams P, const ElecRows& R,
                         const double* mu, double* force, double* V, double* eblk, double* esite) {
-    hipLaunchKernelGGL(k_elec_energy<kG>, energy_blocks(n), 256, 0, s, n, pos, S, P, R.capP, R.nbP, R.cntP, R.capQ,
-                       R.nbQ, R.cntQ, mu, force, V, eblk, esite);
+    if (P.pme)
+        hipLaunchKernelGGL((k_elec_energy<kG, true>), energy_blocks(n), 256, 0, s, n, pos, S, P, R.capP, R.nbP, R.cntP,
+                           R.capQ, R.nbQ, R.cntQ, mu, force, V, eblk, esite);
+    else
+        hipLaunchKernelGGL((k_elec_energy<kG, false>), energy_blocks(n), 256, 0, s, n, pos, S, P, R.capP, R.nbP, R.cntP,
+                           R.capQ, R.nbQ, R.cntQ, mu, force, V, eblk, esite);
 }
 int sor_blocks(int n) { return nblk(n, 256); }
 void launch_recip_site(hipStream_t s, int n, SiteParams S, ElecParams P, const double* phi_fix, const double* phi_ind,
