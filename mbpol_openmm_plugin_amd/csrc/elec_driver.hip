@@ -551,8 +551,11 @@ static int build_rows(mbpol_context* ctx) {
         if (int e = tune_site_lists(ctx)) return e;
     ctx->stats.elec_pairs = h[2] / 2;
     ctx->stats.pol_pairs = h[3];
-    launch_prefactors(st, E->npol, E->polidx.as<int>(), pos, site_params(E), E->P, E->capP, E->nbP.as<int>(),
-                      E->cntP.as<int>(), E->pf.as<double2>());
+    // prefactors of the P rows and the real-space part of the fixed field (the
+    // reciprocal part is added to it after the charge grid)
+    HIP_TRY(hipMemsetAsync(E->efix.p, 0, sizeof(double) * 3 * n, st));
+    launch_prefactors_field(st, E->npol, E->polidx.as<int>(), pos, site_params(E), E->P, rows(E), E->pf.as<double2>(),
+                            E->efix.as<double>());
     if (E->P.pme) {
         const int nbins = E->B.nb[0] * E->B.nb[1] * E->B.nb[2];
         launch_site_spline_table(st, n, pos, E->P, E->g0tab.as<int>(), E->sptab.as<double>());
@@ -818,15 +821,13 @@ int elec_evaluate(mbpol_context* ctx, u64* energy) {
     if (P.redistribute) launch_water_charges(st, E->nw, nullptr, pos, E->q.as<double>(), E->dq.as<double>());
     if (int e = build_rows(ctx)) return e;
     const ElecRows R = rows(E);
-    // fixed field: reciprocal part at every site, real-space part at the polarisable ones
+    // fixed field: real-space part at the polarisable sites (with the rows,
+    // build_rows), plus the reciprocal part at every site
     if (P.pme) {
         if (int e = pme_grid(ctx, 0, nullptr)) return e;
         launch_gather(st, n, pos, P, E->grid.as<double>(), E->phi_fix.as<double>());
-        launch_record_field(st, n, P, E->phi_fix.as<double>(), 0, 0.0, nullptr, E->efix.as<double>());
-    } else {
-        HIP_TRY(hipMemsetAsync(E->efix.p, 0, sizeof(double) * 3 * n, st));
+        launch_record_field(st, n, P, E->phi_fix.as<double>(), 1, 0.0, nullptr, E->efix.as<double>());
     }
-    launch_fixed_field(st, E->npol, E->polidx.as<int>(), pos, S, P, R, E->efix.as<double>());
     // alpha * E_fixed, and the initial guess (into mu_a)
     launch_scale_copy(st, n, E->efix.as<double>(), E->pol.as<double>(), E->efix_pol.as<double>(), nullptr);
     double* mu_in = E->mu_a.as<double>();

@@ -66,10 +66,9 @@ void launch_site_lists(hipStream_t s, int lanes, int n, const double* pos, const
                        int reach, int use_cells, const int* cell_of, const int* cell_start, const int* members,
                        const double4* cpos, int capP, int capQ, int* nbP, int* cntP, int* nbQ, int* cntQ,
                        int* maxcnt);
-void launch_prefactors(hipStream_t s, int npol, const int* polidx, const double* pos, SiteParams S, ElecParams P,
-                       int capP, const int* nbP, const int* cntP, double2* pf);
-void launch_fixed_field(hipStream_t s, int npol, const int* polidx, const double* pos, SiteParams S, ElecParams P,
-                        const ElecRows& R, double* field);
+// prefactors of the P rows (into pf) and the real-space fixed field added to field, in one pass
+void launch_prefactors_field(hipStream_t s, int npol, const int* polidx, const double* pos, SiteParams S, ElecParams P,
+                             const ElecRows& R, double2* pf, double* field);
 void launch_site_bins(hipStream_t s, int n, const double* pos, const double* weight, ElecParams P, BrickGeom B,
                       int* bin_of, int* bin_count);
 void launch_site_spline_table(hipStream_t s, int n, const double* pos, ElecParams P, int* g0tab, double* sptab);

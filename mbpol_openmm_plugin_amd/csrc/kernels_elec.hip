@@ -323,46 +323,6 @@ __global__ void __launch_bounds__(256) k_site_lists(int n, const double* __restr
     }
 }
 
-// induced-field prefactors of every (polarisable i, polarisable j) entry:
-// field_i += p1 mu_j + p2 (mu_j.d) d (PME :2374-2390 with the precomputed
-// justScale factors of :2280-2298; direct sum :438-486 with :534-555)
-template <int G>
-__global__ void __launch_bounds__(256) k_prefactors(int npol, const int* __restrict__ polidx, const double* __restrict__ pos,
-                                                    const SiteParams S, ElecParams P, int capP,
-                                                    const int* __restrict__ nbP, const int* __restrict__ cntP,
-                                                    double2* __restrict__ pf) {
-    critical_path_wave();
-    const int g = (blockIdx.x * blockDim.x + threadIdx.x) / G, lane = threadIdx.x % G;
-    if (g >= npol) return;
-    const int i = polidx[g];
-    const double xi = pos[3 * i], yi = pos[3 * i + 1], zi = pos[3 * i + 2];
-    const int c = min(cntP[i], capP);
-    const int* row = nbP + (size_t)i * capP;
-    double2* prow = pf + (size_t)i * capP;
-    for (int k = lane; k < c; k += G) {
-        const int j = row[k];
-        double dx = pos[3 * j] - xi, dy = pos[3 * j + 1] - yi, dz = pos[3 * j + 2] - zi;
-        min_image(dx, dy, dz, P);
-        const double r2 = dx * dx + dy * dy + dz * dz, r = sqrt(r2);
-        const Thole th(S.damp6[i], S.damp6[j], r);
-        const double gam = tdd_gamma(P, S.mol[i] == S.mol[j], S.type[i], S.type[j]);
-        const double eg = th.ex(gam);
-        const double sc3 = th.s3(eg), sc5 = th.s5(gam, eg);
-        double p1, p2;
-        if (P.pme) {
-            const Ewald ew(P.alpha, r, r2);
-            const double r3 = r * r2, r5 = r3 * r2;
-            p1 = (1.0 - sc3) / r3 - ew.bn1;
-            p2 = ew.bn2 - 3.0 * (1.0 - sc5) / r5;
-        } else {
-            const double ri = 1.0 / r, r2i = ri * ri;
-            p1 = -(ri * r2i) * sc3;
-            p2 = (ri * r2i * 3.0 * r2i) * sc5;
-        }
-        prow[k] = make_double2(p1, p2);
-    }
-}
-
 // visit every neighbour of site i (P row, then Q row) with G lanes
 template <int G, typename F>
 __device__ __forceinline__ void for_rows(int i, int lane, int capP, const int* __restrict__ nbP,
@@ -375,48 +335,91 @@ __device__ __forceinline__ void for_rows(int i, int lane, int capP, const int* _
     for (int k = lane; k < cq; k += G) f(rq[k]);
 }
 
-// ---------------------------------------------------------------------------
-// fixed field (direct part) at the polarisable sites, :361-420 direct sum,
-// :1342-1407 PME
-// ---------------------------------------------------------------------------
-template <int G, bool PME>  // PME: Ewald real-space terms (compile-time branch)
-__global__ void __launch_bounds__(256) k_fixed_field(int npol, const int* __restrict__ polidx,
-                                                     const double* __restrict__ pos, const SiteParams S, ElecParams P,
-                                                     int capP, const int* __restrict__ nbP, const int* __restrict__ cntP,
-                                                     int capQ, const int* __restrict__ nbQ,
-                                                     const int* __restrict__ cntQ, double* __restrict__ field) {
+// One pass over the rows of polarisable site i:
+// - induced-field prefactors of every (polarisable i, polarisable j) entry of
+//   the P row, field_i += p1 mu_j + p2 (mu_j.d) d (PME :2374-2390 with the
+//   precomputed justScale factors of :2280-2298; direct sum :438-486 with
+//   :534-555), stored per entry;
+// - the real-space fixed field at i over the P and Q rows (:361-420 direct
+//   sum, :1342-1407 PME), summed per lane in row order and added to field[i].
+// The P-row pairs share the distance, the Ewald factors and the Thole sixth root.
+template <int G, bool PME>
+__global__ void __launch_bounds__(256) k_prefactors_field(int npol, const int* __restrict__ polidx,
+                                                          const double* __restrict__ pos, const SiteParams S,
+                                                          ElecParams P, int capP, const int* __restrict__ nbP,
+                                                          const int* __restrict__ cntP, int capQ,
+                                                          const int* __restrict__ nbQ, const int* __restrict__ cntQ,
+                                                          double2* __restrict__ pf, double* __restrict__ field) {
     critical_path_wave();
     const int g = (blockIdx.x * blockDim.x + threadIdx.x) / G, lane = threadIdx.x % G;
     if (g >= npol) return;
     const int i = polidx[g];
     const double xi = pos[3 * i], yi = pos[3 * i + 1], zi = pos[3 * i + 2];
     double ex = 0, ey = 0, ez = 0;
-    for_rows<G>(i, lane, capP, nbP, cntP, capQ, nbQ, cntQ, [&](int j) {
-        const int same = S.mol[i] == S.mol[j];
-        double dx = pos[3 * j] - xi, dy = pos[3 * j + 1] - yi, dz = pos[3 * j + 2] - zi;
+    // fixed-field term of pair (i, j) at distance r
+    auto field_term = [&](int j, double dx, double dy, double dz, double r2, double r, const Thole& th, bool same,
+                          double bn1) {
         if constexpr (!PME) {
             if (same) return;
-            const double r2 = dx * dx + dy * dy + dz * dz, r = sqrt(r2);
-            const Thole th(S.damp6[i], S.damp6[j], r);
             const double rr3 = th.s3(th.ex(P.thole[0])) / (r2 * r);
             const double f = -rr3 * S.q[j];
             ex += dx * f;
             ey += dy * f;
             ez += dz * f;
         } else {
-            min_image(dx, dy, dz, P);
-            const double r2 = dx * dx + dy * dy + dz * dz, r = sqrt(r2);
-            const Ewald ew(P.alpha, r, r2);
-            const Thole th(S.damp6[i], S.damp6[j], r);
             double s3 = th.s3(th.ex(P.thole[0]));
             if (same) s3 = 2;  // reproduced as in the reference (:1382-1385)
             const double rr3 = (s3 - 1.) / (r2 * r);
-            const double f = -(ew.bn1 - rr3) * S.q[j];
+            const double f = -(bn1 - rr3) * S.q[j];
             ex += dx * f;
             ey += dy * f;
             ez += dz * f;
         }
-    });
+    };
+    {
+        const int c = min(cntP[i], capP);
+        const int* row = nbP + (size_t)i * capP;
+        double2* prow = pf + (size_t)i * capP;
+        for (int k = lane; k < c; k += G) {
+            const int j = row[k];
+            double dx = pos[3 * j] - xi, dy = pos[3 * j + 1] - yi, dz = pos[3 * j + 2] - zi;
+            if constexpr (PME) min_image(dx, dy, dz, P);
+            const double r2 = dx * dx + dy * dy + dz * dz, r = sqrt(r2);
+            const Thole th(S.damp6[i], S.damp6[j], r);
+            const bool same = S.mol[i] == S.mol[j];
+            const double gam = tdd_gamma(P, same, S.type[i], S.type[j]);
+            const double eg = th.ex(gam);
+            const double sc3 = th.s3(eg), sc5 = th.s5(gam, eg);
+            double p1, p2, bn1 = 0.0;
+            if constexpr (PME) {
+                const Ewald ew(P.alpha, r, r2);
+                const double r3 = r * r2, r5 = r3 * r2;
+                p1 = (1.0 - sc3) / r3 - ew.bn1;
+                p2 = ew.bn2 - 3.0 * (1.0 - sc5) / r5;
+                bn1 = ew.bn1;
+            } else {
+                const double ri = 1.0 / r, r2i = ri * ri;
+                p1 = -(ri * r2i) * sc3;
+                p2 = (ri * r2i * 3.0 * r2i) * sc5;
+            }
+            prow[k] = make_double2(p1, p2);
+            field_term(j, dx, dy, dz, r2, r, th, same, bn1);
+        }
+    }
+    {
+        const int c = min(cntQ[i], capQ);
+        const int* row = nbQ + (size_t)i * capQ;
+        for (int k = lane; k < c; k += G) {
+            const int j = row[k];
+            double dx = pos[3 * j] - xi, dy = pos[3 * j + 1] - yi, dz = pos[3 * j + 2] - zi;
+            if constexpr (PME) min_image(dx, dy, dz, P);
+            const double r2 = dx * dx + dy * dy + dz * dz, r = sqrt(r2);
+            const Thole th(S.damp6[i], S.damp6[j], r);
+            double bn1 = 0.0;
+            if constexpr (PME) bn1 = Ewald(P.alpha, r, r2).bn1;
+            field_term(j, dx, dy, dz, r2, r, th, S.mol[i] == S.mol[j], bn1);
+        }
+    }
     ex = group_sum<G>(ex);
     ey = group_sum<G>(ey);
     ez = group_sum<G>(ez);
@@ -1565,7 +1568,7 @@ __global__ void k_potential_at_points(int npts, const double* __restrict__ pts, 
 // launchers
 // ---------------------------------------------------------------------------
 static inline int nblk(long long n, int b) { return (int)((n + b - 1) / b); }
-constexpr int kG = 16;   // lanes per site: fixed field, energy
+constexpr int kG = 16;   // lanes per site: prefactors and fixed field, energy
 constexpr int kGI = 32;  // lanes per site in the iteration (>= 25 for the PME gather)
 
 void launch_site_charges_copy(hipStream_t s, int n, const double* q0, double* q) {
@@ -1599,23 +1602,15 @@ void launch_site_lists(hipStream_t s, int lanes, int n, const double* pos, const
     }
 #undef MBPOL_SITE_LISTS
 }
-void launch_prefactors(hipStream_t s, int npol, const int* polidx, const double* pos, SiteParams S, ElecParams P,
-                       int capP, const int* nbP, const int* cntP, double2* pf) {
-    if (npol > 0)
-        hipLaunchKernelGGL(k_prefactors<kG>, nblk((long long)npol * kG, 256), 256, 0, s, npol, polidx, pos, S, P, capP,
-                           nbP, cntP, pf);
-}
-void launch_fixed_field(hipStream_t s, int npol, const int* polidx, const double* pos, SiteParams S, ElecParams P,
-                        const ElecRows& R, double* field) {
-    if (npol > 0)
-    {
-        if (P.pme)
-            hipLaunchKernelGGL((k_fixed_field<kG, true>), nblk((long long)npol * kG, 256), 256, 0, s, npol, polidx, pos,
-                               S, P, R.capP, R.nbP, R.cntP, R.capQ, R.nbQ, R.cntQ, field);
-        else
-            hipLaunchKernelGGL((k_fixed_field<kG, false>), nblk((long long)npol * kG, 256), 256, 0, s, npol, polidx, pos,
-                               S, P, R.capP, R.nbP, R.cntP, R.capQ, R.nbQ, R.cntQ, field);
-    }
+void launch_prefactors_field(hipStream_t s, int npol, const int* polidx, const double* pos, SiteParams S, ElecParams P,
+                             const ElecRows& R, double2* pf, double* field) {
+    if (npol <= 0) return;
+    if (P.pme)
+        hipLaunchKernelGGL((k_prefactors_field<kG, true>), nblk((long long)npol * kG, 256), 256, 0, s, npol, polidx,
+                           pos, S, P, R.capP, R.nbP, R.cntP, R.capQ, R.nbQ, R.cntQ, pf, field);
+    else
+        hipLaunchKernelGGL((k_prefactors_field<kG, false>), nblk((long long)npol * kG, 256), 256, 0, s, npol, polidx,
+                           pos, S, P, R.capP, R.nbP, R.cntP, R.capQ, R.nbQ, R.cntQ, pf, field);
 }
 void launch_site_bins(hipStream_t s, int n, const double* pos, const double* weight, ElecParams P, BrickGeom B,
                       int* bin_of, int* bin_count) {
