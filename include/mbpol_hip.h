@@ -179,7 +179,9 @@ int mbpol_context_set_dipole_convergence(mbpol_context* ctx, double target_eps, 
 int mbpol_context_set_solver(mbpol_context* ctx, int32_t solver);
 /* launch shapes the context picked by timing candidates on its first evaluation (no counterpart in the
  * reference): "site_lanes" = lanes per site of the electrostatics row build (8, 16 or 32; 0 before the first
- * evaluation) */
+ * evaluation); "row_capacity_pol" / "row_capacity_other" = entries per electrostatics neighbour row (they
+ * grow when a configuration needs more: a row that overflows is detected at the end of the evaluation,
+ * which is then run again) */
 int mbpol_context_get_tuning(const mbpol_context* ctx, const char* name, int32_t* value);
 /* copyParametersToContext for the electrostatics (same count required) */
 int mbpol_context_update_electrostatics(mbpol_context* ctx, int32_t n, const double* charge, const int32_t* molecule,

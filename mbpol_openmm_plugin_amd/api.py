@@ -517,7 +517,8 @@ class Context:
         _check(library().mbpol_context_set_solver(self._h, SOLVERS[solver]))
 
     def getTuning(self, name):
-        """Launch shape picked by the context's autotuning (e.g. 'site_lanes')."""
+        """Launch shape picked by the context's autotuning ('site_lanes'), or a row
+        capacity of the electrostatics ('row_capacity_pol', 'row_capacity_other')."""
         v = ctypes.c_int32(0)
         _check(library().mbpol_context_get_tuning(self._h, name.encode(), ctypes.byref(v)))
         return v.value

@@ -22,6 +22,9 @@ int elec_create(mbpol_context* ctx);
 void elec_destroy(mbpol_context* ctx);
 int elec_set_box(mbpol_context* ctx);
 int elec_evaluate(mbpol_context* ctx, u64* energy);
+// after an evaluation: whether a neighbour row overflowed its capacity (the
+// capacities have then grown and the evaluation must be run again)
+int elec_rows_check(mbpol_context* ctx, bool& redo);
 
 // what the learned dipole guess reads from the electrostatics of the last
 // set-up (positions in the context's buffer)

@@ -81,6 +81,9 @@ struct Elec {
     double* h_eps = nullptr;  // pinned, 128 bytes: [0] epsilon, ints from byte 32, a SolveCtl from byte 64
     DevBuf ctl;               // SolveCtl of the running dipole solve
     int last_iters = 0;       // field evaluations of the previous solve (sizes the first batch of steps)
+    // row capacities are checked at the end of the evaluation (rows_pending) once
+    // the launch shapes are tuned; after an overflow the next build waits for the check
+    bool rows_pending = false, rows_sync = true;
 };
 
 // B-spline moduli with the reference's small-modulus fix-up and order-5 zeta
@@ -507,8 +510,14 @@ static int tune_site_lists(mbpol_context* ctx) {
 
 int elec_tuning(const mbpol_context* ctx, const char* name, int* value) {
     const Elec* E = ctx->elec;
-    if (!E || std::strcmp(name, "site_lanes") != 0) return set_error(MBPOL_ERR_INVALID_ARGUMENT, "no tuning '%s'", name);
-    *value = E->site_lanes;
+    if (E && std::strcmp(name, "site_lanes") == 0)
+        *value = E->site_lanes;
+    else if (E && std::strcmp(name, "row_capacity_pol") == 0)  // entries per polarisable-neighbour row
+        *value = E->capP;
+    else if (E && std::strcmp(name, "row_capacity_other") == 0)  // entries per row of the other neighbours
+        *value = E->capQ;
+    else
+        return set_error(MBPOL_ERR_INVALID_ARGUMENT, "no tuning '%s'", name);
     return MBPOL_OK;
 }
 
@@ -530,7 +539,24 @@ static int build_rows(mbpol_context* ctx) {
         launch_site_pack(st, n, E->members.as<int>(), pos, E->pol.as<double>(), E->cpos.as<double4>());
     }
     int* h = reinterpret_cast<int*>(E->h_eps) + 8;
-    for (int attempt = 0; attempt < 2; ++attempt) {
+    E->rows_pending = false;
+    if (!E->rows_sync && E->site_lanes > 0) {
+        // The rows' maxima reach the host with the energies; elec_rows_check
+        // compares them with the capacities after the evaluation and has it
+        // redone when a row overflowed (the kernels clamp to the capacity, so an
+        // overflowing row loses neighbours but nothing is written out of bounds).
+        const size_t rp = (size_t)n * E->capP, rq = (size_t)n * E->capQ;
+        HIP_TRY(E->nbP.ensure(sizeof(int) * std::max<size_t>(rp, 1)));
+        HIP_TRY(E->pf.ensure(sizeof(double2) * std::max<size_t>(rp, 1)));
+        HIP_TRY(E->nbQ.ensure(sizeof(int) * std::max<size_t>(rq, 1)));
+        HIP_TRY(hipMemsetAsync(E->maxcnt.p, 0, sizeof(int) * 4, st));
+        launch_site_lists(st, E->site_lanes, n, pos, E->pol.as<double>(), E->P, E->nc, E->reach, E->use_cells,
+                          E->cell_of.as<int>(), E->cell_start.as<int>(), E->members.as<int>(), E->cpos.as<double4>(),
+                          E->capP, E->capQ, E->nbP.as<int>(), E->cntP.as<int>(), E->nbQ.as<int>(), E->cntQ.as<int>(),
+                          E->maxcnt.as<int>());
+        HIP_TRY(hipMemcpyAsync(h, E->maxcnt.p, sizeof(int) * 4, hipMemcpyDeviceToHost, st));
+        E->rows_pending = true;
+    } else for (int attempt = 0; attempt < 2; ++attempt) {
         const size_t rp = (size_t)n * E->capP, rq = (size_t)n * E->capQ;
         HIP_TRY(E->nbP.ensure(sizeof(int) * std::max<size_t>(rp, 1)));
         HIP_TRY(E->pf.ensure(sizeof(double2) * std::max<size_t>(rp, 1)));
@@ -549,8 +575,11 @@ static int build_rows(mbpol_context* ctx) {
     }
     if (E->site_lanes == 0)
         if (int e = tune_site_lists(ctx)) return e;
-    ctx->stats.elec_pairs = h[2] / 2;
-    ctx->stats.pol_pairs = h[3];
+    if (!E->rows_pending) {
+        E->rows_sync = false;
+        ctx->stats.elec_pairs = h[2] / 2;
+        ctx->stats.pol_pairs = h[3];
+    }
     // prefactors of the P rows and the real-space part of the fixed field (the
     // reciprocal part is added to it after the charge grid)
     HIP_TRY(hipMemsetAsync(E->efix.p, 0, sizeof(double) * 3 * n, st));
@@ -772,6 +801,25 @@ static int solve_pcg(mbpol_context* ctx, double* x, double self_term, SolveCtl& 
         if (int e = read_ctl(ctx, c)) return e;
         if (c.stop) break;
     }
+    return MBPOL_OK;
+}
+
+int elec_rows_check(mbpol_context* ctx, bool& redo) {
+    Elec* E = ctx->elec;
+    redo = false;
+    if (!E || !E->rows_pending) return MBPOL_OK;
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    E->rows_pending = false;
+    const int* h = reinterpret_cast<const int*>(E->h_eps) + 8;
+    if (h[0] > E->capP || h[1] > E->capQ) {
+        E->capP = std::max(E->capP, (h[0] + h[0] / 4 + 63) / 32 * 32);
+        E->capQ = std::max(E->capQ, (h[1] + h[1] / 4 + 63) / 32 * 32);
+        E->rows_sync = true;
+        redo = true;
+        return MBPOL_OK;
+    }
+    ctx->stats.elec_pairs = h[2] / 2;
+    ctx->stats.pol_pairs = h[3];
     return MBPOL_OK;
 }
 

@@ -765,22 +765,30 @@ static int evaluate(mbpol_context* ctx, double* forces_dev_out, double* energies
     }
     int rc = MBPOL_OK;
     if ((comp & MBPOL_ELECTROSTATICS) && ctx->elec) rc = elec_evaluate(ctx, E + 4);
-    if (fork) {
-        // always wait for the worker: it uses the context until its job ends
-        std::string msg;
-        const int wrc = ctx->worker.wait(msg);
+    int wrc = MBPOL_OK;
+    std::string msg;
+    if (fork) wrc = ctx->worker.wait(msg);  // always wait for the worker: it uses the context until its job ends
+    if (rc || wrc) {
+        // a solve that failed on rows that overflowed is run again with rows that fit
+        bool redo = false;
+        if (ctx->elec)
+            if (int e = elec_rows_check(ctx, redo)) return e;
+        if (redo) return evaluate(ctx, forces_dev_out, energies_out, comp);
         if (rc) return rc;
-        if (wrc) return set_error(wrc, "%s", msg.c_str());
-        HIP_TRY(hipStreamWaitEvent(st, ctx->ev_join, 0));
-    } else if (rc) {
-        return rc;
+        return set_error(wrc, "%s", msg.c_str());
     }
+    if (fork) HIP_TRY(hipStreamWaitEvent(st, ctx->ev_join, 0));
     double* fout = forces_dev_out ? forces_dev_out : ctx->force.as<double>();
     launch_forces_to_double(st, 3 * s.n, F, fout);
     launch_fold_vsites(st, (int)s.vs_idx.size() / 4, ctx->vs_idx.as<int>(), ctx->vs_w.as<double>(), fout);
     u64* he = static_cast<u64*>(ctx->h_counts) + 8;
     HIP_TRY(hipMemcpyAsync(he, E, sizeof(u64) * 8, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
+    if (ctx->elec) {
+        bool redo = false;
+        if (int e = elec_rows_check(ctx, redo)) return e;
+        if (redo) return evaluate(ctx, forces_dev_out, energies_out, comp);  // once: the next build checks first
+    }
     ctx->evaluated = true;
     if (energies_out) {
         double tot = 0;
