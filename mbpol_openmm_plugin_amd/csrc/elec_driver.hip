@@ -900,6 +900,10 @@ int elec_evaluate(mbpol_context* ctx, u64* energy) {
             [[fallthrough]];
         default: HIP_TRY(hipMemcpyAsync(mu_in, E->efix_pol.p, mbytes, hipMemcpyDeviceToDevice, st));
     }
+    // accumulators of the energy phase, cleared before the solve: after the
+    // solve's last host read the energy kernels follow without other work first
+    HIP_TRY(hipMemsetAsync(E->fsite.p, 0, sizeof(double) * 3 * n, st));
+    HIP_TRY(hipMemsetAsync(E->V.p, 0, sizeof(double) * n, st));
     timer_stop(ctx, T_ELEC_SETUP);
     const double self_term = P.pme ? (4.0 / 3.0) * (P.alpha * P.alpha * P.alpha) / mbpol_params::el_sqrt_pi : 0.0;
     SolveCtl c{};
@@ -923,12 +927,8 @@ int elec_evaluate(mbpol_context* ctx, u64* energy) {
     E->mu_cur = mu_in;
     if (!conv)
         return set_error(MBPOL_ERR_NOT_CONVERGED, "Induced dipoles did not converge:  iterations=%d eps=%g", it, cur);
-    HIP_TRY(hipMemcpyAsync(E->mu_prev.p, mu_in, mbytes, hipMemcpyDeviceToDevice, st));
-    E->have_previous = true;
     // energy and forces (calculateElectrostatic, :838-855 direct sum, :2718-2777 PME)
     timer_start(ctx, T_ELEC_ENERGY);
-    HIP_TRY(hipMemsetAsync(E->fsite.p, 0, sizeof(double) * 3 * n, st));
-    HIP_TRY(hipMemsetAsync(E->V.p, 0, sizeof(double) * n, st));
     launch_elec_energy(st, n, pos, S, P, R, mu_in, E->fsite.as<double>(), E->V.as<double>(), E->eblk.as<double>(),
                        E->ngroups ? E->g_esite.as<double>() : nullptr);
     if (E->ngroups)
@@ -947,6 +947,9 @@ int elec_evaluate(mbpol_context* ctx, u64* energy) {
     if (P.redistribute) launch_charge_deriv_forces(st, E->nw, E->dq.as<double>(), E->V.as<double>(), E->fsite.as<double>());
     launch_add_forces_fix(st, n, E->fsite.as<double>(), ctx->force_fx.as<u64>());
     timer_stop(ctx, T_ELEC_ENERGY);
+    // converged dipoles kept for the next evaluation's "previous" start
+    HIP_TRY(hipMemcpyAsync(E->mu_prev.p, mu_in, mbytes, hipMemcpyDeviceToDevice, st));
+    E->have_previous = true;
     return MBPOL_OK;
 }
 
