@@ -44,14 +44,14 @@ FLOPS_PER_DIMER = 31.0e3
 # x 1.0; profiles/memory_counters_4096w_eval.md holds the runs); kernels not
 # listed report null
 MEASURED_TRAFFIC = {
-    'elec_iterate': 90976788,
-    'pme_spread': 61711794,
-    'fft_z_r2c': 25781628,
-    'fft_y': 26429641,
-    'fft_x_conv': 39895296,
-    'fft_z_c2r': 26099163,
-    'threebody': 227674289,
-    'twobody': 86748268,
+    'elec_iterate': 90978071,
+    'pme_spread': 61716016,
+    'fft_z_r2c': 25782308,
+    'fft_y': 26430314,
+    'fft_x_conv': 39895966,
+    'fft_z_c2r': 26099820,
+    'threebody': 227600009,
+    'twobody': 86314272,
 }
 
 
