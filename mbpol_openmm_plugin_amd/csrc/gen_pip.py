@@ -24,7 +24,14 @@ register count by the block count).  The generated function is a template on
 the arithmetic type, instantiated for float (mixed-precision path) and double
 (FP64 path).
 
-Usage: gen_pip.py --out DIR  (writes DIR/pip_eval_gen.h)
+The 3-body polynomial is invariant under the 24 relabellings of a trimer
+that permute its waters and swap the hydrogens of an even number of them.
+pip_sym_gen.h writes it as E(x) = R(x) + sum_h Q(y_h): Q holds one
+representative per full orbit (1288 monomials), R the monomials of smaller
+orbits (2613), so the code evaluated 24 times is small enough to stay in the
+instruction cache.
+
+Usage: gen_pip.py --out DIR  (writes DIR/pip_eval_gen.h and DIR/pip_sym_gen.h)
 """
 import argparse
 import json
@@ -142,6 +149,55 @@ def emit(name, nvar, tree):
     return '\n'.join(out), dict(stats)
 
 
+# Points of the 3-body variables (reference order, kernels_pip.hip:var3):
+# 0 Oa 1 Ha1 2 Ha2 3 Ob 4 Hb1 5 Hb2 6 Oc 7 Hc1 8 Hc2.
+VARS3 = [(1, 2), (4, 5), (7, 8), (0, 1), (0, 2), (3, 4), (3, 5), (6, 7), (6, 8), (1, 4), (1, 5), (1, 7), (1, 8),
+         (2, 4), (2, 5), (2, 7), (2, 8), (4, 7), (4, 8), (5, 7), (5, 8), (0, 4), (0, 5), (0, 7), (0, 8), (3, 1),
+         (3, 2), (3, 7), (3, 8), (6, 1), (6, 2), (6, 4), (6, 5), (0, 3), (0, 6), (3, 6)]
+
+
+def trimer_group():
+    """Permutations of the 36 variables induced by the 24 relabellings of a
+    trimer that permute the waters and swap the hydrogens of an even number
+    of them: h[m] = the variable that relabelling h puts in place of m."""
+    idx = {frozenset(p): n for n, p in enumerate(VARS3)}
+    out = set()
+    for wp in ((0, 1, 2), (0, 2, 1), (1, 0, 2), (1, 2, 0), (2, 0, 1), (2, 1, 0)):
+        for sw in ((0, 0, 0), (1, 1, 0), (1, 0, 1), (0, 1, 1)):
+            pi = [0] * 9
+            for w in range(3):
+                t = wp[w]
+                pi[3 * w] = 3 * t
+                pi[3 * w + 1], pi[3 * w + 2] = (3 * t + 2, 3 * t + 1) if sw[w] else (3 * t + 1, 3 * t + 2)
+            out.add(tuple(idx[frozenset((pi[a], pi[b]))] for a, b in VARS3))
+    return sorted(out)
+
+
+def split_by_orbits(rows, group):
+    """The polynomial as E(x) = R(x) + sum_h Q(y_h), y_h[m] = x[h[m]]: Q holds one
+    representative (the smallest) of every orbit of full size |group| with its
+    coefficient; R holds the monomials of the smaller orbits.  Checks that every
+    coefficient's monomials are closed under the group."""
+    mono = {}
+    for r in rows:
+        m = tuple(sorted(int(v) for v in r[1:] if v >= 0))
+        assert m not in mono, 'repeated monomial'
+        mono[m] = int(r[0])
+    orbit_rows, rest_rows, seen = [], [], set()
+    for m in sorted(mono):
+        if m in seen:
+            continue
+        orb = {tuple(sorted(h[v] for v in m)) for h in group}
+        assert all(mono.get(o) == mono[m] for o in orb), 'polynomial not invariant under the group'
+        seen |= orb
+        pad = lambda t: list(t) + [-1] * (4 - len(t))
+        if len(orb) == len(group):
+            orbit_rows.append([mono[m]] + pad(min(orb)))
+        else:
+            rest_rows.extend([mono[o]] + pad(o) for o in sorted(orb))
+    return np.array(orbit_rows), np.array(rest_rows)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--out', required=True)
@@ -161,6 +217,26 @@ def main():
     os.makedirs(args.out, exist_ok=True)
     text = '\n'.join(parts) + '\n'
     path = os.path.join(args.out, 'pip_eval_gen.h')
+    if not (os.path.exists(path) and open(path).read() == text):
+        with open(path, 'w') as fh:
+            fh.write(text)
+    # 3-body polynomial split by the trimer's relabelling group (pip_sym_gen.h)
+    group = trimer_group()
+    orows, rrows = split_by_orbits(data['poly3b'], group)
+    coefs = t['threebody']['thefit']
+    parts = ['// Generated by gen_pip.py from pip_monomials.npz -- do not edit.', '#pragma once', '',
+             '// E(x) = pip3b_rest(x) + sum over the %d relabellings h of pip3b_orbit(y_h), y_h[m] = x[pip3b_perm[h][m]]'
+             % len(group)]
+    for name, rows in (('pip3b_orbit', orows), ('pip3b_rest', rrows)):
+        code, stats = emit(name, 36, build_tree(rows, coefs))
+        parts += ['// %s: %s' % (name, stats), code, '']
+        summary[name] = stats
+    parts.append('constexpr int pip3b_ngroup = %d;' % len(group))
+    parts.append('__constant__ int pip3b_perm[%d][36] = {' % len(group))
+    parts += ['    {%s},' % ', '.join(str(v) for v in h) for h in group]
+    parts.append('};')
+    text = '\n'.join(parts) + '\n'
+    path = os.path.join(args.out, 'pip_sym_gen.h')
     if not (os.path.exists(path) and open(path).read() == text):
         with open(path, 'w') as fh:
             fh.write(text)

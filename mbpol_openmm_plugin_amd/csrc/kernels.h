@@ -80,5 +80,9 @@ void launch_threebody_f32(hipStream_t s, long long ntrip, const int* trip, const
 void launch_threebody_f64(hipStream_t s, long long ntrip, const int* trip, const int* idx, const double* pos,
                           int periodic, d3 box, u64* force, u64* energy, const KEvents& ev = KEvents(), double* eunit = nullptr,
                         int max_blocks = 0);
+// FP64, the polynomial through its relabelling group (36 KB of LDS per 64-lane block)
+void launch_threebody_sym_f64(hipStream_t s, long long ntrip, const int* trip, const int* idx, const double* pos,
+                          int periodic, d3 box, u64* force, u64* energy, const KEvents& ev = KEvents(), double* eunit = nullptr,
+                        int max_blocks = 0);
 
 }  // namespace mbpol

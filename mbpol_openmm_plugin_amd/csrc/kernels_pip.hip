@@ -9,8 +9,14 @@
 //
 // This file is compiled once per body order and arithmetic type:
 //   -DPIP_BODY=2|3  -DPIP_REAL=float|double
+// and once more for the 3-body term with -DPIP_SYM=1: the polynomial evaluated
+// through its relabelling group (pip_sym_gen.h) instead of term by term.
 #include "device_common.h"
+#if PIP_SYM
+#include "pip_sym_gen.h"
+#else
 #include "pip_eval_gen.h"
+#endif
 
 #ifndef PIP_BODY
 #error "PIP_BODY must be 2 or 3"
@@ -323,10 +329,43 @@ __device__ __forceinline__ bool trimer_geometry(long long t, const int* __restri
     return true;
 }
 
+#if PIP_SYM
+// E(x) and dE/dx of the 3-body polynomial as R(x) + sum_h Q(y_h), y_h[m] =
+// x[perm[h][m]] (pip_sym_gen.h): the relabelled variables are read from, and
+// the gradients scattered back into, this lane's LDS column, so the loop over
+// the 24 relabellings runs the same small block of code (it stays in the
+// instruction cache, where the term-by-term polynomial is 733 KB of code).
+// Each lane touches only its own column: no barrier.
+__device__ __forceinline__ real pip3b_sym_eval(real (&x)[36], real (&g)[36], real* xs, real* gs) {
+    const int lane = threadIdx.x & 63;
+    real e = pip3b_rest<real>(x, g);
+#pragma unroll
+    for (int n = 0; n < 36; ++n) {
+        xs[n * 64 + lane] = x[n];
+        gs[n * 64 + lane] = g[n];
+    }
+#pragma nounroll
+    for (int h = 0; h < pip3b_ngroup; ++h) {
+        real y[36], q[36];
+#pragma unroll
+        for (int m = 0; m < 36; ++m) y[m] = xs[pip3b_perm[h][m] * 64 + lane];
+        e += pip3b_orbit<real>(y, q);
+#pragma unroll
+        for (int m = 0; m < 36; ++m) gs[pip3b_perm[h][m] * 64 + lane] += q[m];
+    }
+#pragma unroll
+    for (int n = 0; n < 36; ++n) g[n] = gs[n * 64 + lane];
+    return e;
+}
+#endif
+
 __global__ void __launch_bounds__(64) k_threebody(long long nt, const int* __restrict__ trip, const int* __restrict__ idx,
                                                   const double* __restrict__ pos, int periodic, d3 box,
                                                   u64* __restrict__ force, u64* __restrict__ energy,
                                                   double* __restrict__ eunit) {
+#if PIP_SYM
+    __shared__ real sym_x[36 * 64], sym_g[36 * 64];  // this wave's variables and gradient, [variable][lane]
+#endif
     // a capped grid strides over the list in whole waves
     for (long long t0 = (long long)blockIdx.x * blockDim.x; t0 < nt; t0 += (long long)gridDim.x * blockDim.x) {
         const long long t = t0 + threadIdx.x;
@@ -344,7 +383,11 @@ __global__ void __launch_bounds__(64) k_threebody(long long nt, const int* __res
                 const real r = sqrt(d.dot(d));
                 x[n] = exp_t<real>(-(real)v.k * (r - (real)v.r0));
             }
+#if PIP_SYM
+            real Ep = pip3b_sym_eval(x, g, sym_x, sym_g);
+#else
             real Ep = pip3b_eval<real>(x, g);
+#endif
             // Only t, Ep and g stay live across the polynomial; the geometry is
             // reloaded (through an opaque copy of t, so that the first loads are
             // not kept in registers).

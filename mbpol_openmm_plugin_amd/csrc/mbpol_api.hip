@@ -17,6 +17,11 @@
 
 using namespace mbpol;
 
+// 3-body polynomial term by term by default; through its relabelling group with
+// MBPOL_HIP_3B_SYM=1 (less instruction traffic, but 36 KB of LDS per wave and
+// slower: 3.1-4.9 against 2.87 ms per 4096-water evaluation)
+static constexpr bool kPip3SymDefault = false;
+
 static thread_local std::string g_last_error;
 
 int mbpol::set_error(int code, const char* fmt, ...) {
@@ -520,6 +525,13 @@ int mbpol_context_create(const mbpol_system* sys, int32_t precision, int32_t dev
         int waves = 1;
         if (const char* env = std::getenv("MBPOL_HIP_PIP_WAVES")) waves = std::max(0, std::atoi(env));
         ctx->pip_blocks = waves * 4 * prop.multiProcessorCount;
+        // the 3-body polynomial through its relabelling group (MBPOL_HIP_3B_SYM=0|1): one
+        // 64-lane block per CU holds 36 KB of LDS; MBPOL_HIP_3B_SYM_WAVES blocks per CU
+        const char* sym = std::getenv("MBPOL_HIP_3B_SYM");
+        ctx->pip3_sym = sym ? std::atoi(sym) != 0 : kPip3SymDefault;
+        int swaves = 2;
+        if (const char* env = std::getenv("MBPOL_HIP_3B_SYM_WAVES")) swaves = std::max(1, std::atoi(env));
+        ctx->sym_blocks = swaves * prop.multiProcessorCount;
         int dwaves = 2;  // the dispersion kernel: two waves per SIMD (MBPOL_HIP_DISP_WAVES)
         if (const char* env = std::getenv("MBPOL_HIP_DISP_WAVES")) dwaves = std::max(0, std::atoi(env));
         ctx->disp_blocks = dwaves * 4 * prop.multiProcessorCount;
@@ -714,7 +726,10 @@ static int short_range(mbpol_context* ctx, hipStream_t sr, uint32_t comp) {
     if (do3) {
         const int per = s.th_method == MBPOL_CUTOFF_PERIODIC;
         const KEvents ev = timer_kernel(ctx, T_THREEBODY);
-        if (dp)
+        if (dp && ctx->pip3_sym)
+            launch_threebody_sym_f64(sr, ctx->stats.trimers, ctx->trips.as<int>(), ctx->th_idx.as<int>(), pos, per,
+                                     box, F, E + 2, ev, eu3, ctx->sym_blocks);
+        else if (dp)
             launch_threebody_f64(sr, ctx->stats.trimers, ctx->trips.as<int>(), ctx->th_idx.as<int>(), pos, per, box, F,
                                  E + 2, ev, eu3, ctx->pip_blocks);
         else
