@@ -1,8 +1,12 @@
 """The 3-body polynomial evaluated through its relabelling group (opt-in,
 MBPOL_HIP_3B_SYM=1: E(x) = R(x) + sum over 24 relabellings of Q(y_h), see
 csrc/gen_pip.py) against the term-by-term kernel, which the parity tests pin
-to the FP64 oracle: the same 3-body energy and forces on the 256-water box to
-1e-12 relative (both FP64; they differ in summation order only).  The CPU test
+to the FP64 oracle: the same 3-body energy and forces on the 256-water box.
+Both are FP64 and differ in summation order only (~1e-15 per trimer), but
+every trimer's energy and forces are rounded to 2^-32 fixed point before they
+are summed, so a rounding that flips moves the totals by 2^-32 kJ/mol (per
+trimer): the bounds are 1e-11 relative for the energy (5e-9 kJ/mol here)
+and 1e-10 for the RMS force.  The CPU test
 tests/test_pip_symmetry.py checks the decomposition itself on random
 variables.
 """
@@ -44,5 +48,5 @@ def test_relabelling_group_matches_term_by_term(mb):
     e1, f1 = threebody(mb, True)
     err = np.sqrt(np.mean((f1 - f0) ** 2)) / np.sqrt(np.mean(f0 ** 2))
     print('3-body energy %.12f vs %.12f, force rel rms %.2e' % (e1, e0, err), flush=True)
-    assert abs(e1 - e0) <= 1e-12 * abs(e0)
-    assert err <= 1e-12
+    assert abs(e1 - e0) <= 1e-11 * abs(e0)
+    assert err <= 1e-10
