@@ -141,6 +141,9 @@ struct mbpol_context {
     mbpol::DevBuf pos, pos_resident, force_fx, force, energy_fx, vs_idx, vs_w;
     // pinned host staging
     void* h_counts = nullptr;
+    // pinned staging of positions in and forces out for the host-buffer entry point
+    double* h_pos = nullptr;
+    double* h_force = nullptr;
     // scan scratch
     mbpol::DevBuf scan_tmp, scan_tmp_side;
     // 1B / 2B / 3B

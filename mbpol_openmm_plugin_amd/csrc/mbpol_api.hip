@@ -511,7 +511,9 @@ int mbpol_context_create(const mbpol_system* sys, int32_t precision, int32_t dev
     if (hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming) != hipSuccess)
         return fail(set_error(MBPOL_ERR_HIP, "hipEventCreate failed"));
-    if (hipHostMalloc(&ctx->h_counts, 4096, hipHostMallocDefault) != hipSuccess)
+    if (hipHostMalloc(&ctx->h_counts, 4096, hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void**)&ctx->h_pos, sizeof(double) * 3 * std::max(sys->n, 1), hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void**)&ctx->h_force, sizeof(double) * 3 * std::max(sys->n, 1), hipHostMallocDefault) != hipSuccess)
         return fail(set_error(MBPOL_ERR_HIP, "hipHostMalloc failed"));
     {
         // Grid cap of the 2-body and 3-body kernels: one wave per SIMD of the device
@@ -579,6 +581,8 @@ void mbpol_context_destroy(mbpol_context* ctx) {
     elec_destroy(ctx);
     disp_destroy(ctx);
     if (ctx->h_counts) (void)hipHostFree(ctx->h_counts);
+    if (ctx->h_pos) (void)hipHostFree(ctx->h_pos);
+    if (ctx->h_force) (void)hipHostFree(ctx->h_force);
     if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);
     if (ctx->ev_join) (void)hipEventDestroy(ctx->ev_join);
     if (ctx->side) (void)hipStreamDestroy(ctx->side);
@@ -822,14 +826,19 @@ int mbpol_context_evaluate(mbpol_context* ctx, const double* positions, double* 
                            uint32_t components) {
     if (!ctx || !positions) return set_error(MBPOL_ERR_INVALID_ARGUMENT, "null argument");
     HIP_TRY(hipSetDevice(ctx->device));
-    HIP_TRY(hipMemcpyAsync(ctx->pos.p, positions, sizeof(double) * 3 * ctx->sys.n, hipMemcpyHostToDevice, ctx->stream));
+    // through the context's pinned buffers: copies from and to pageable memory
+    // would be staged by the runtime in small synchronous pieces
+    const size_t bytes = sizeof(double) * 3 * ctx->sys.n;
+    HIP_TRY(hipStreamSynchronize(ctx->stream));  // no earlier copy still reads the staging buffer
+    std::memcpy(ctx->h_pos, positions, bytes);
+    HIP_TRY(hipMemcpyAsync(ctx->pos.p, ctx->h_pos, bytes, hipMemcpyHostToDevice, ctx->stream));
     timer_start(ctx, T_EVALUATE);
     if (int e = evaluate(ctx, nullptr, energies_out, components)) return e;
     timer_stop(ctx, T_EVALUATE);
     if (forces_out) {
-        HIP_TRY(hipMemcpyAsync(forces_out, ctx->force.p, sizeof(double) * 3 * ctx->sys.n, hipMemcpyDeviceToHost,
-                               ctx->stream));
+        HIP_TRY(hipMemcpyAsync(ctx->h_force, ctx->force.p, bytes, hipMemcpyDeviceToHost, ctx->stream));
         HIP_TRY(hipStreamSynchronize(ctx->stream));
+        std::memcpy(forces_out, ctx->h_force, bytes);
     }
     return MBPOL_OK;
 }
