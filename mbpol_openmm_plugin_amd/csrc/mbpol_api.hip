@@ -752,7 +752,10 @@ static int short_range(mbpol_context* ctx, hipStream_t sr, uint32_t comp) {
     return MBPOL_OK;
 }
 
-static int evaluate(mbpol_context* ctx, double* forces_dev_out, double* energies_out, uint32_t comp) {
+// forces_host_out: pinned host buffer that receives the forces with the same
+// final synchronisation as the energies (nullptr: forces stay on the device)
+static int evaluate(mbpol_context* ctx, double* forces_dev_out, double* energies_out, uint32_t comp,
+                    double* forces_host_out = nullptr) {
     const mbpol_system& s = ctx->sys;
     hipStream_t st = ctx->stream;
     u64* F = ctx->force_fx.as<u64>();
@@ -792,7 +795,7 @@ static int evaluate(mbpol_context* ctx, double* forces_dev_out, double* energies
         bool redo = false;
         if (ctx->elec)
             if (int e = elec_rows_check(ctx, redo)) return e;
-        if (redo) return evaluate(ctx, forces_dev_out, energies_out, comp);
+        if (redo) return evaluate(ctx, forces_dev_out, energies_out, comp, forces_host_out);
         if (rc) return rc;
         return set_error(wrc, "%s", msg.c_str());
     }
@@ -802,11 +805,13 @@ static int evaluate(mbpol_context* ctx, double* forces_dev_out, double* energies
     launch_fold_vsites(st, (int)s.vs_idx.size() / 4, ctx->vs_idx.as<int>(), ctx->vs_w.as<double>(), fout);
     u64* he = static_cast<u64*>(ctx->h_counts) + 8;
     HIP_TRY(hipMemcpyAsync(he, E, sizeof(u64) * 8, hipMemcpyDeviceToHost, st));
+    if (forces_host_out)
+        HIP_TRY(hipMemcpyAsync(forces_host_out, fout, sizeof(double) * 3 * s.n, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
     if (ctx->elec) {
         bool redo = false;
         if (int e = elec_rows_check(ctx, redo)) return e;
-        if (redo) return evaluate(ctx, forces_dev_out, energies_out, comp);  // once: the next build checks first
+        if (redo) return evaluate(ctx, forces_dev_out, energies_out, comp, forces_host_out);  // once: the next build checks first
     }
     ctx->evaluated = true;
     if (energies_out) {
@@ -833,13 +838,9 @@ int mbpol_context_evaluate(mbpol_context* ctx, const double* positions, double* 
     std::memcpy(ctx->h_pos, positions, bytes);
     HIP_TRY(hipMemcpyAsync(ctx->pos.p, ctx->h_pos, bytes, hipMemcpyHostToDevice, ctx->stream));
     timer_start(ctx, T_EVALUATE);
-    if (int e = evaluate(ctx, nullptr, energies_out, components)) return e;
+    if (int e = evaluate(ctx, nullptr, energies_out, components, forces_out ? ctx->h_force : nullptr)) return e;
     timer_stop(ctx, T_EVALUATE);
-    if (forces_out) {
-        HIP_TRY(hipMemcpyAsync(ctx->h_force, ctx->force.p, bytes, hipMemcpyDeviceToHost, ctx->stream));
-        HIP_TRY(hipStreamSynchronize(ctx->stream));
-        std::memcpy(forces_out, ctx->h_force, bytes);
-    }
+    if (forces_out) std::memcpy(forces_out, ctx->h_force, bytes);
     return MBPOL_OK;
 }
 
