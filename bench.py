@@ -80,8 +80,8 @@ def kernel_models(stats, grid, nsites, npol):
     K0, K1, K2 = grid
     ngrid, nhalf = K0 * K1 * K2, K0 * K1 * (K2 // 2 + 1)
     return {
-        'threebody': ('flops', FLOPS_PER_TRIMER * stats['num_trimers'], 'mfma', PEAK_FP64_TFLOPS, 'TFLOP/s'),
-        'twobody': ('flops', FLOPS_PER_DIMER * stats['num_dimers'], 'mfma', PEAK_FP64_TFLOPS, 'TFLOP/s'),
+        'threebody': ('flops', FLOPS_PER_TRIMER * stats['num_trimers'], 'valu', PEAK_FP64_TFLOPS, 'TFLOP/s'),
+        'twobody': ('flops', FLOPS_PER_DIMER * stats['num_dimers'], 'valu', PEAK_FP64_TFLOPS, 'TFLOP/s'),
         # the induced-field kernel (the conjugate-gradient matrix-vector product; the SOR
         # step in the FP64 path): rows of the polarisable sites (int index + 2 FP64
         # prefactors per entry), the potential grid once, positions and dipoles of every

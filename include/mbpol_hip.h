@@ -22,8 +22,11 @@
  * kJ/mol, forces in kJ/mol/nm, charges in e, polarisabilities in nm^3.
  * Plain pointers and sizes only; errors are returned as status codes with a
  * per-thread message (mbpol_last_error) -- no exceptions cross this ABI.
- * A context binds one GPU and one HIP stream; calls on one context must not
- * overlap (the reference kernels are not re-entrant either).
+ * A context binds one GPU and two HIP streams: the calling thread drives the
+ * electrostatics on the main stream, and a worker thread owned by the context
+ * enqueues the 1-, 2- and 3-body terms and the dispersion on the second one;
+ * the worker is joined before every call returns.  Calls on one context must
+ * not overlap (the reference kernels are not re-entrant either).
  */
 #ifndef MBPOL_HIP_H
 #define MBPOL_HIP_H
@@ -79,8 +82,9 @@ extern "C" {
 /* induced-dipole solver (mbpol_context_set_solver); both stop when the reference's convergence measure
  * (:504-511, :588) of the current dipoles falls below the target epsilon */
 #define MBPOL_SOLVER_SOR 0              /* the reference's SOR iteration, omega 0.55 (:557-616); default of DOUBLE */
-#define MBPOL_SOLVER_PCG 1              /* conjugate gradients with a diagonal preconditioner on the symmetric
-                                           form of the same equations; default of MIXED and SINGLE_PIP */
+#define MBPOL_SOLVER_PCG 1              /* conjugate gradients with a block-Jacobi preconditioner (the inverse of
+                                           each water's 9x9 polarisation block) on the symmetric form of the same
+                                           equations; default of MIXED and SINGLE_PIP */
 
 typedef struct mbpol_system mbpol_system;
 typedef struct mbpol_context mbpol_context;

@@ -791,6 +791,10 @@ static int evaluate(mbpol_context* ctx, double* forces_dev_out, double* energies
     std::string msg;
     if (fork) wrc = ctx->worker.wait(msg);  // always wait for the worker: it uses the context until its job ends
     if (rc || wrc) {
+        // The worker only enqueued the short-range kernels: they may still be
+        // adding into the force/energy accumulators and reading the positions,
+        // which a re-run (or the next call) clears and overwrites.
+        if (fork) HIP_TRY(hipStreamSynchronize(ctx->side));
         // a solve that failed on rows that overflowed is run again with rows that fit
         bool redo = false;
         if (ctx->elec)
