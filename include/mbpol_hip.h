@@ -17,6 +17,7 @@
  *                          (MBPolReferenceKernels.cpp:101-114,246-259,362-399,451-483)
  *   update parameters   <- copyParametersToContext (MBPolReferenceKernels.cpp:116-127,305-325,401-414,485-498)
  *   electrostatic potential <- getElectrostaticPotential (MBPolReferenceKernels.cpp:260-283)
+ *   system moments      <- getSystemElectrostaticsMoments (MBPolReferenceKernels.cpp:285-303)
  *
  * Units follow the reference API: positions and lengths in nm, energies in
  * kJ/mol, forces in kJ/mol/nm, charges in e, polarisabilities in nm^3.
@@ -187,6 +188,12 @@ int mbpol_context_set_solver(mbpol_context* ctx, int32_t solver);
  * grow when a configuration needs more: a row that overflows is detected at the end of the evaluation,
  * which is then run again) */
 int mbpol_context_get_tuning(const mbpol_context* ctx, const char* name, int32_t* value);
+/* copyParametersToContext of MBPolOneBodyForce / MBPolTwoBodyForce / MBPolThreeBodyForce
+ * (MBPolReferenceKernels.cpp:116-127, 401-414, 485-498): new {O,H1,H2} index lists for the same number of
+ * molecules (MBPOL_ERR_COUNT_CHANGED otherwise) */
+int mbpol_context_update_onebody(mbpol_context* ctx, int32_t n, const int32_t* idx);
+int mbpol_context_update_twobody(mbpol_context* ctx, int32_t n, const int32_t* idx);
+int mbpol_context_update_threebody(mbpol_context* ctx, int32_t n, const int32_t* idx);
 /* copyParametersToContext for the electrostatics (same count required) */
 int mbpol_context_update_electrostatics(mbpol_context* ctx, int32_t n, const double* charge, const int32_t* molecule,
                                         const int32_t* atom_type, const double* damping, const double* polarity);
@@ -201,6 +208,11 @@ int mbpol_context_set_list_mode(mbpol_context* ctx, int32_t full_chain_list);
  * induced dipoles of the last evaluate (MBPolReferenceElectrostaticsForce.cpp:1023-1086) */
 int mbpol_context_get_electrostatic_potential(mbpol_context* ctx, int32_t npoints, const double* points,
                                               double* potential);
+/* MBPolElectrostaticsForce::getSystemElectrostaticsMoments (mbpolKernels.h:120; Reference glue
+ * MBPolReferenceKernels.cpp:285-303, sums MBPolReferenceElectrostaticsForce.cpp:923-1021): moments[13] = net
+ * charge (e), dipole (3, Debye), traceless quadrupole (3x3 row-major, Buckingham) about the centre of mass of
+ * masses[n] (amu, one per particle), from the positions, charges and induced dipoles of the last evaluate */
+int mbpol_context_get_system_electrostatics_moments(mbpol_context* ctx, const double* masses, double* moments);
 
 /* ---- learned initial guess of the induced dipoles (no counterpart in the reference) ----
  * A per-atom network (48 radial features of the neighbourhood within 0.6 nm -> 64 -> 64 -> 3, SiLU, BF16 MFMA with

@@ -609,21 +609,46 @@ class Context:
             q, mol, typ, damp, pol = force._arrays()
             _check(library().mbpol_context_update_electrostatics(self._h, len(q), _pd(_d(q)), _pi(_i(mol)),
                                                                  _pi(_i(typ)), _pd(_d(damp)), _pd(_d(pol))))
+        elif isinstance(force, MBPolOneBodyForce):
+            idx = _i([force.getOneBodyParameters(i) for i in range(force.getNumOneBodys())]).reshape(-1)
+            _check(library().mbpol_context_update_onebody(self._h, force.getNumOneBodys(), _pi(idx)))
+        elif isinstance(force, (MBPolTwoBodyForce, MBPolThreeBodyForce)):
+            n = force.getNumMolecules()
+            idx = _i([force.getParticleParameters(i) for i in range(n)]).reshape(-1)
+            fn = (library().mbpol_context_update_twobody if isinstance(force, MBPolTwoBodyForce)
+                  else library().mbpol_context_update_threebody)
+            _check(fn(self._h, n, _pi(idx)))
         else:
-            # molecule lists are rebuilt with a fresh context (counts must not change, as in the reference)
-            raise MBPolError(6, 'updateParametersInContext is supported for MBPolElectrostaticsForce only')
+            raise MBPolError(1, 'updateParametersInContext: unknown Force %s' % type(force).__name__)
 
 
 def _context_get_potential(self, points):
     """MBPolElectrostaticsForce::getElectrostaticPotential: potential (kJ/mol/e) at
-    points (nm) from the charges and induced dipoles of the last getState()."""
+    points (nm) from the site charges and induced dipoles at the current positions.
+    As the reference's kernel (MBPolReferenceKernels.cpp:260-283), it first runs the
+    electrostatics set-up and dipole solve, here as an electrostatics-only evaluation."""
     pts = _d(np.asarray(points, dtype=np.float64).reshape(-1, 3))
     out = np.zeros(len(pts))
+    self.getState(getForces=False, components=MBPOL_ELECTROSTATICS)
     _check(library().mbpol_context_get_electrostatic_potential(self._h, len(pts), _pd(pts), _pd(out)))
     return out
 
 
+def _context_get_moments(self):
+    """MBPolElectrostaticsForce::getSystemElectrostaticsMoments: [net charge (e),
+    dipole (3, Debye), traceless quadrupole (3 x 3 row-major, Buckingham)] about
+    the System's centre of mass, at the current positions (the electrostatics are
+    evaluated first, as the reference's kernel does, MBPolReferenceKernels.cpp:285-303)."""
+    sysm = self.system
+    masses = _d([sysm.getParticleMass(i) for i in range(sysm.getNumParticles())])
+    out = np.zeros(13)
+    self.getState(getForces=False, components=MBPOL_ELECTROSTATICS)
+    _check(library().mbpol_context_get_system_electrostatics_moments(self._h, _pd(masses), _pd(out)))
+    return out
+
+
 Context.getElectrostaticPotential = _context_get_potential
+Context.getSystemElectrostaticsMoments = _context_get_moments
 
 
 def _context_set_dipole_convergence(self, target_epsilon, max_iterations=200):

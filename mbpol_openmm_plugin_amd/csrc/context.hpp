@@ -128,8 +128,12 @@ struct mbpol_context {
     bool evaluated = false;    // an evaluation has completed (launch shapes are tuned)
     int pip_blocks = 0;        // grid cap of the 2-body and 3-body kernels (0: one block per 64 units)
     int disp_blocks = 0;       // grid cap of the dispersion kernel (0: one wave per atom)
-    bool pip3_sym = false;     // FP64 3-body polynomial through its relabelling group
-    int sym_blocks = 0;        // grid of that kernel (64-lane blocks, 36 KB of LDS each)
+    int pip3_kernel = 0;       // 3-body kernel: 0 the polynomial through its relabelling group, 1 term by term
+    int pip3_blocks = 0;       // grid cap of the 3-body kernel (0: one block per 64 units)
+    // the 1-body waters are the electrostatics waters (sites 4w..4w+3): the 1-body term
+    // runs in the per-water kernel that computes the charges
+    bool water_fused = false;
+    bool onebody_with_charges = false;  // this evaluation's 1-body term runs with the charges
     // independent groups: water ranges, per-molecule candidate ranges, per-unit energies and per-group sums
     int ngroups = 0;
     mbpol::DevBuf g_wstart, g_mol_s, g_mol_e, g_eu1, g_eu2, g_eu3, g_energy;

@@ -25,6 +25,32 @@ __device__ __forceinline__ void add_force(unsigned long long* f, int atom, doubl
     atomicAdd(&f[3 * atom + 2], to_fix(fz));
 }
 
+// Sum of N fixed-point values over each run of consecutive lanes with equal
+// key (the dimer and trimer lists are ordered by first molecule, so a wave's
+// units mostly share a few first molecules).  Integer sums are exact, so one
+// atomic per run adds the same bits as one atomic per lane.  Every lane of the
+// wave must call it (uniform control flow); returns true on the last lane of
+// each run, which then holds the run's sums in v.
+template <int N>
+__device__ __forceinline__ bool wave_run_sum(long long key, unsigned long long (&v)[N]) {
+    const int lane = threadIdx.x & 63;
+    const long long prev = __shfl_up(key, 1, 64);
+    const unsigned long long heads = __ballot(lane == 0 || prev != key);
+    // first lane of this lane's run: the highest run head at or below this lane
+    const unsigned long long below = heads & (lane == 63 ? ~0ull : ((2ull << lane) - 1));
+    const int start = 63 - __clzll(below);
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const bool take = lane - off >= start;
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+            const unsigned long long o = __shfl_up(v[i], off, 64);
+            if (take) v[i] += o;
+        }
+    }
+    return lane == 63 || ((heads >> (lane + 1)) & 1ull);
+}
+
 // Wave-level sum (64 lanes).
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll

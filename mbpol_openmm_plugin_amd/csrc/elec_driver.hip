@@ -866,7 +866,17 @@ int elec_evaluate(mbpol_context* ctx, u64* energy) {
     timer_start(ctx, T_ELEC_SETUP);
     launch_site_charges_copy(st, n, E->q0.as<double>(), E->q.as<double>());
     // sites of water w are 4w..4w+3 (the reference's ii += 4 loop, :881)
-    if (P.redistribute) launch_water_charges(st, E->nw, nullptr, pos, E->q.as<double>(), E->dq.as<double>());
+    if (P.redistribute) {
+        // with the 1-body term of the same waters when the evaluation runs it here
+        const bool one = ctx->onebody_with_charges;
+        const int G = ctx->ngroups;
+        launch_water_terms(st, E->nw, one, true, ctx->ob_idx.as<int>(), pos, s.ob_method == MBPOL_ONEBODY_PERIODIC,
+                           d3{s.box[0], s.box[1], s.box[2]}, ctx->force_fx.as<u64>(), ctx->energy_fx.as<u64>() + 0,
+                           one && G ? ctx->g_eu1.as<double>() : nullptr, E->q.as<double>(), E->dq.as<double>());
+        if (one && G)
+            launch_group_sums(st, G, ctx->g_wstart.as<int>(), 1, nullptr, ctx->g_eu1.as<double>(),
+                              ctx->g_energy.as<u64>(), 0);
+    }
     if (int e = build_rows(ctx)) return e;
     const ElecRows R = rows(E);
     // fixed field: real-space part at the polarisable sites (with the rows,
@@ -1040,6 +1050,23 @@ int mbpol_context_get_electrostatic_potential(mbpol_context* ctx, int32_t npoint
                                E->mu_cur, E->P, out.as<double>());
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     HIP_TRY(hipMemcpy(potential, out.p, sizeof(double) * npoints, hipMemcpyDeviceToHost));
+    return MBPOL_OK;
+}
+
+int mbpol_context_get_system_electrostatics_moments(mbpol_context* ctx, const double* masses, double* moments) {
+    using namespace mbpol;
+    if (!ctx || !masses || !moments) return set_error(MBPOL_ERR_INVALID_ARGUMENT, "null argument");
+    if (!ctx->elec) return set_error(MBPOL_ERR_INVALID_ARGUMENT, "no electrostatics in this system");
+    HIP_TRY(hipSetDevice(ctx->device));
+    Elec* E = ctx->elec;
+    DevBuf m, out;
+    HIP_TRY(m.ensure(sizeof(double) * E->n));
+    HIP_TRY(out.ensure(sizeof(double) * 13));
+    HIP_TRY(hipMemcpy(m.p, masses, sizeof(double) * E->n, hipMemcpyHostToDevice));
+    launch_system_moments(ctx->stream, E->n, ctx->pos.as<double>(), E->q.as<double>(), E->mu_cur, m.as<double>(),
+                          out.as<double>());
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    HIP_TRY(hipMemcpy(moments, out.p, sizeof(double) * 13, hipMemcpyDeviceToHost));
     return MBPOL_OK;
 }
 

@@ -145,5 +145,8 @@ void launch_energy_finish(hipStream_t s, int nblocks, const double* eblk, u64* e
 void launch_add_forces_fix(hipStream_t s, int n, const double* f, u64* fx);
 void launch_potential_at_points(hipStream_t s, int npts, const double* pts, int n, const double* pos, const double* q,
                                 const double* mu, ElecParams P, double* out);
+// system multipole moments about the centre of mass of `mass` (out[13], one block)
+void launch_system_moments(hipStream_t s, int n, const double* pos, const double* q, const double* mu,
+                           const double* mass, double* out);
 
 }  // namespace mbpol

@@ -26,12 +26,17 @@ the arithmetic type, instantiated for float (mixed-precision path) and double
 
 The 3-body polynomial is invariant under the 24 relabellings of a trimer
 that permute its waters and swap the hydrogens of an even number of them.
-pip_sym_gen.h writes it as E(x) = R(x) + sum_h Q(y_h): Q holds one
-representative per full orbit (1288 monomials), R the monomials of smaller
-orbits (2613), so the code evaluated 24 times is small enough to stay in the
-instruction cache.
+pip_orbit_gen.h writes it as E(x) = R(x) + sum_h Q(y_h): Q holds one
+representative of every orbit whose stabiliser order is a power of two (1518
+of 1526 orbits), with its coefficient divided by that order, and R the 33
+monomials of the other orbits.  The kernel runs Q in a loop of 24 passes,
+permuting the variable and gradient registers between passes along a
+Hamiltonian cycle of the group, so the code of one pass (a few thousand
+instructions) stays in the instruction cache, and Q reads its coefficients
+from a table in order (scalar loads) instead of carrying them as
+instruction literals.
 
-Usage: gen_pip.py --out DIR  (writes DIR/pip_eval_gen.h and DIR/pip_sym_gen.h)
+Usage: gen_pip.py --out DIR  (writes DIR/pip_eval_gen.h and DIR/pip_orbit_gen.h)
 """
 import argparse
 import json
@@ -59,23 +64,34 @@ def build_tree(rows, coef):
     return tree
 
 
-def lit(v):
+def _lit(v):
     r = repr(float(v))
     if 'e' not in r and '.' not in r:
         r += '.0'
     return 'T(%s)' % r
 
 
-def emit(name, nvar, tree):
+def emit(name, nvar, tree, table=None, accumulate=False):
     """Straight-line Horner code.  Each accumulator starts from its first
     product and adds its own constant last, so no instruction needs two
     literal operands (two literals would force the compiler to keep the
     coefficient in a register, and identical coefficients -- every one recurs
     about 29 times by symmetry -- would then be merged into long-lived
-    registers)."""
+    registers).
+
+    table: None -- coefficients are literals; a list -- every coefficient use
+    appends its value to the list and reads it as cf[k] (k = its position), so
+    the function takes the table as an argument and reads it in order.
+    accumulate: add the gradient into g instead of overwriting it."""
     out = []
     w = out.append
     stats = defaultdict(int)
+
+    def lit(v):
+        if table is None:
+            return _lit(v)
+        table.append(float(v))
+        return 'cf[%d]' % (len(table) - 1)
 
     def acc(var, first, a, b):
         if first:
@@ -90,10 +106,15 @@ def emit(name, nvar, tree):
             w('%s += %s;' % (var, lit(c)))
             stats['add'] += 1
 
-    w('template <typename T>')
-    w('__device__ __forceinline__ T %s(T (&x)[%d], T (&g)[%d]) {' % (name, nvar, nvar))
+    if table is None:
+        w('template <typename T>')
+        w('__device__ __forceinline__ T %s(T (&x)[%d], T (&g)[%d]) {' % (name, nvar, nvar))
+    else:  # cf: any pointer type (e.g. one into the constant address space)
+        w('template <typename T, typename P>')
+        w('__host__ __device__ __forceinline__ T %s(T (&x)[%d], T (&g)[%d], P cf) {' % (name, nvar, nvar))
     w('#pragma clang fp contract(fast)')
-    w('for (int i = 0; i < %d; ++i) g[i] = T(0);' % nvar)
+    if not accumulate:
+        w('for (int i = 0; i < %d; ++i) g[i] = T(0);' % nvar)
     w('T e, s1, s2, s3, p2, p3;')
     first_e = True
     for v0 in sorted(tree):
@@ -107,7 +128,7 @@ def emit(name, nvar, tree):
             # top of the function, where they would all stay live at once.
             used = sorted(set([v0, v1] + list(n2['kids']) +
                               [v3 for n3 in n2['kids'].values() for v3 in n3['kids']]))
-            w(' '.join('asm volatile("" : "+v"(x[%d]));' % v for v in used))
+            w(' '.join('PIP_FENCE(x[%d]);' % v for v in used))
             stats['fence'] += 1
             acc('p2', True, 'x[%d]' % v0, 'x[%d]' % v1)
             first2 = True
@@ -117,17 +138,18 @@ def emit(name, nvar, tree):
                     acc('p3', True, 'p2', 'x[%d]' % v2)
                     first3 = True
                     for v3 in sorted(n3['kids']):
-                        c = n3['kids'][v3]['c']
-                        acc('s3', first3, lit(c), 'x[%d]' % v3)
+                        c = lit(n3['kids'][v3]['c'])
+                        acc('s3', first3, c, 'x[%d]' % v3)
                         first3 = False
-                        acc('g[%d]' % v3, False, lit(c), 'p3')
+                        acc('g[%d]' % v3, False, c, 'p3')
                         stats['leaf'] += 1
                     add_const('s3', n3['c'])
                     acc('s2', first2, 'x[%d]' % v2, 's3')
                     acc('g[%d]' % v2, False, 'p2', 's3')
                 else:
-                    acc('s2', first2, lit(n3['c']), 'x[%d]' % v2)
-                    acc('g[%d]' % v2, False, lit(n3['c']), 'p2')
+                    c = lit(n3['c'])
+                    acc('s2', first2, c, 'x[%d]' % v2)
+                    acc('g[%d]' % v2, False, c, 'p2')
                 first2 = False
             if first2:  # level-2 node with no children: s2 is just its constant
                 w('s2 = %s;' % lit(n2['c']))
@@ -146,7 +168,47 @@ def emit(name, nvar, tree):
         stats['add'] += 1
     w('return e;')
     w('}')
+    if table is not None:
+        out = prefetch_table_reads(out, len(table))
     return '\n'.join(out), dict(stats)
+
+
+PREFETCH_BATCH = 16
+
+
+def prefetch_table_reads(lines, ncoef, batch=PREFETCH_BATCH):
+    """Rewrite the cf[k] reads of a table-mode function as explicit loads into
+    named constants, batch by batch, with batch j+1 loaded right after the
+    first instruction that uses batch j.  Scalar loads of the coefficient table
+    may complete out of order, so a use waits for every load in flight: issued
+    after that first use, the next batch's loads are covered by the rest of the
+    current batch's arithmetic instead of stalling the next use.  Scheduling
+    barriers keep the compiler from moving the loads to their uses."""
+    import re
+    pat = re.compile(r'cf\[(\d+)\]')
+    nb = (ncoef + batch - 1) // batch
+
+    def loads(j):
+        if j >= nb:
+            return []
+        ks = range(j * batch, min(ncoef, (j + 1) * batch))
+        return ([' '.join('const T c%d = cf[%d];' % (k, k) for k in ks), '__builtin_amdgcn_sched_barrier(0);'])
+
+    out, started = [], set()
+    for ln in lines:
+        ks = [int(m) for m in pat.findall(ln)]
+        if not ks:
+            out.append(ln)
+            continue
+        ln = pat.sub(lambda m: 'c' + m.group(1), ln)
+        j = ks[0] // batch
+        if j == 0 and 0 not in started:
+            out += loads(0)
+        out.append(ln)
+        if j not in started:
+            started.add(j)
+            out += ['__builtin_amdgcn_sched_barrier(0);'] + loads(j + 1)
+    return out
 
 
 # Points of the 3-body variables (reference order, kernels_pip.hip:var3):
@@ -173,29 +235,107 @@ def trimer_group():
     return sorted(out)
 
 
-def split_by_orbits(rows, group):
-    """The polynomial as E(x) = R(x) + sum_h Q(y_h), y_h[m] = x[h[m]]: Q holds one
-    representative (the smallest) of every orbit of full size |group| with its
-    coefficient; R holds the monomials of the smaller orbits.  Checks that every
-    coefficient's monomials are closed under the group."""
+def compose(h, s):
+    """The relabelling 'h, then s' as a variable map: (h o s)[m] = h[s[m]]."""
+    return tuple(h[s[m]] for m in range(len(s)))
+
+
+def group_cycle(group):
+    """A Hamiltonian cycle of the group's right Cayley graph over two
+    generators a, b: steps s_1..s_n in {a, b} such that the prefixes
+    h_k = s_1 o ... o s_k visit every element once and h_n = identity.  A loop
+    that applies s_k to the registers after its k-th pass therefore sees every
+    relabelling once and ends in the original labelling.  Among all generator
+    pairs with such a cycle, the one with the fewest register moves (36 minus
+    fixed points plus cycles, per permuted vector) is taken.
+    Returns (a, b, mask) with bit k of mask set when step k+1 is a."""
+    ident = tuple(range(len(group[0])))
+
+    def moves(p):
+        seen, cyc = set(), 0
+        for i in range(len(p)):
+            if i in seen or p[i] == i:
+                continue
+            cyc += 1
+            while i not in seen:
+                seen.add(i)
+                i = p[i]
+        return sum(1 for i in range(len(p)) if p[i] != i) + cyc
+
+    def hamilton(a, b):
+        path, used, steps = [ident], {ident}, []
+
+        def dfs():
+            if len(path) == len(group):
+                for st, bit in ((a, 1), (b, 0)):
+                    if compose(path[-1], st) == ident:
+                        steps.append(bit)
+                        return True
+                return False
+            for st, bit in ((a, 1), (b, 0)):
+                nx = compose(path[-1], st)
+                if nx in used:
+                    continue
+                used.add(nx)
+                path.append(nx)
+                steps.append(bit)
+                if dfs():
+                    return True
+                used.discard(nx)
+                path.pop()
+                steps.pop()
+            return False
+        return steps if dfs() else None
+
+    best = None
+    for a in group:
+        for b in group:
+            if ident in (a, b) or a >= b:
+                continue
+            steps = hamilton(a, b)
+            if steps is None:
+                continue
+            cost = sum(moves(a) if bit else moves(b) for bit in steps)
+            if best is None or cost < best[0]:
+                best = (cost, a, b, steps)
+    _, a, b, steps = best
+    return a, b, sum(bit << k for k, bit in enumerate(steps))
+
+
+def orbit_loop_split(rows, coefs, group):
+    """The polynomial as E(x) = R(x) + sum_h Q(y_h), y_h[m] = x[h[m]].
+
+    Summing a monomial m over all relabellings gives |Stab(m)| times the sum of
+    its orbit, so every orbit is represented in Q by its smallest member with
+    coefficient c/|Stab(m)|.  That division is exact for the stabiliser orders
+    1, 2, 4, 8; the few orbits with stabilisers of order 3, 6 or 24 are written
+    out monomial by monomial in R with their own coefficients.  Checks that
+    every coefficient's monomials are closed under the group.  Returns
+    (Q rows, Q coefficients, R rows, R coefficients); rows are {k, v0..v3} with
+    k indexing the returned coefficient list."""
     mono = {}
     for r in rows:
         m = tuple(sorted(int(v) for v in r[1:] if v >= 0))
         assert m not in mono, 'repeated monomial'
         mono[m] = int(r[0])
-    orbit_rows, rest_rows, seen = [], [], set()
+    pad = lambda t: list(t) + [-1] * (4 - len(t))
+    qrows, qcoef, rrows, rcoef, seen = [], [], [], [], set()
     for m in sorted(mono):
         if m in seen:
             continue
         orb = {tuple(sorted(h[v] for v in m)) for h in group}
         assert all(mono.get(o) == mono[m] for o in orb), 'polynomial not invariant under the group'
         seen |= orb
-        pad = lambda t: list(t) + [-1] * (4 - len(t))
-        if len(orb) == len(group):
-            orbit_rows.append([mono[m]] + pad(min(orb)))
+        stab = len(group) // len(orb)
+        c = coefs[mono[m]]
+        if stab & (stab - 1) == 0:
+            qrows.append([len(qcoef)] + pad(min(orb)))
+            qcoef.append(c / stab)
         else:
-            rest_rows.extend([mono[o]] + pad(o) for o in sorted(orb))
-    return np.array(orbit_rows), np.array(rest_rows)
+            for o in sorted(orb):
+                rrows.append([len(rcoef)] + pad(o))
+                rcoef.append(c)
+    return np.array(qrows), qcoef, np.array(rrows), rcoef
 
 
 def main():
@@ -205,7 +345,8 @@ def main():
     with open(os.path.join(PARAMS, 'mbpol_tables.json')) as fh:
         t = json.load(fh)
     data = np.load(os.path.join(PARAMS, 'pip_monomials.npz'))
-    parts = ['// Generated by gen_pip.py from pip_monomials.npz -- do not edit.', '#pragma once', '']
+    parts = ['// Generated by gen_pip.py from pip_monomials.npz -- do not edit.', '#pragma once', '',
+             '// PIP_FENCE(v): an empty asm that re-defines v in a register (the includer defines it)', '']
     summary = {}
     for name, key, nvar, coefs in (('pip2b_eval', 'poly2b', 31, t['twobody']['thefit']),
                                    ('pip3b_eval', 'poly3b', 36, t['threebody']['thefit'])):
@@ -220,23 +361,36 @@ def main():
     if not (os.path.exists(path) and open(path).read() == text):
         with open(path, 'w') as fh:
             fh.write(text)
-    # 3-body polynomial split by the trimer's relabelling group (pip_sym_gen.h)
+    # 3-body polynomial as a loop over the trimer's relabelling group (pip_orbit_gen.h)
     group = trimer_group()
-    orows, rrows = split_by_orbits(data['poly3b'], group)
-    coefs = t['threebody']['thefit']
+    qrows, qcoef, rrows, rcoef = orbit_loop_split(data['poly3b'], t['threebody']['thefit'], group)
+    a, b, mask = group_cycle(group)
+    table = []
+    qcode, qstats = emit('pip3b_orbit_pass', 36, build_tree(qrows, qcoef), table=table, accumulate=True)
+    rcode, rstats = emit('pip3b_rest', 36, build_tree(rrows, rcoef))
+    summary['pip3b_orbit_pass'] = qstats
+    summary['pip3b_rest'] = rstats
     parts = ['// Generated by gen_pip.py from pip_monomials.npz -- do not edit.', '#pragma once', '',
-             '// E(x) = pip3b_rest(x) + sum over the %d relabellings h of pip3b_orbit(y_h), y_h[m] = x[pip3b_perm[h][m]]'
-             % len(group)]
-    for name, rows in (('pip3b_orbit', orows), ('pip3b_rest', rrows)):
-        code, stats = emit(name, 36, build_tree(rows, coefs))
-        parts += ['// %s: %s' % (name, stats), code, '']
-        summary[name] = stats
-    parts.append('constexpr int pip3b_ngroup = %d;' % len(group))
-    parts.append('__constant__ int pip3b_perm[%d][36] = {' % len(group))
-    parts += ['    {%s},' % ', '.join(str(v) for v in h) for h in group]
-    parts.append('};')
+             '// E(x) = pip3b_rest(x) + sum over the %d relabellings h of pip3b_orbit_pass(y_h), y_h[m] = x[h[m]].'
+             % len(group),
+             '// The relabellings are visited by applying pip3b_step_a or pip3b_step_b to the registers after',
+             '// each pass (pass k is followed by step a when bit k of pip3b_step_mask is set); the last step',
+             '// returns to the original labelling.  pip3b_orbit_pass reads its %d coefficients in order from'
+             % len(table),
+             '// pip3b_orbit_coef (double or float table) and adds its gradient into g.',
+             'constexpr int pip3b_npass = %d;' % len(group),
+             'constexpr unsigned pip3b_step_mask = 0x%xu;' % mask,
+             'constexpr int pip3b_step_a[36] = {%s};' % ', '.join(str(v) for v in a),
+             'constexpr int pip3b_step_b[36] = {%s};' % ', '.join(str(v) for v in b),
+             'constexpr int pip3b_orbit_ncoef = %d;' % len(table)]
+    for typ, suf in (('double', 'f64'), ('float', 'f32')):
+        parts.append('__constant__ %s pip3b_orbit_coef_%s[%d] = {' % (typ, suf, len(table)))
+        for i in range(0, len(table), 4):
+            parts.append('    ' + ', '.join(repr(float(v)) for v in table[i:i + 4]) + ',')
+        parts.append('};')
+    parts += ['', '// %s' % qstats, qcode, '', '// %s' % rstats, rcode.replace('__device__', '__host__ __device__'), '']
     text = '\n'.join(parts) + '\n'
-    path = os.path.join(args.out, 'pip_sym_gen.h')
+    path = os.path.join(args.out, 'pip_orbit_gen.h')
     if not (os.path.exists(path) and open(path).read() == text):
         with open(path, 'w') as fh:
             fh.write(text)

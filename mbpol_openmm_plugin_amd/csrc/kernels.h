@@ -47,9 +47,10 @@ void launch_group_sums(hipStream_t s, int ngroups, const int* start, int mult, c
                        const double* eunit, u64* out, int comp);
 // eunit (optional): energy of every unit (water, dimer, trimer) in kJ/mol, for per-group sums
 void launch_place_vsites(hipStream_t s, int n, const int* vs_idx, const double* vs_w, double* pos);
-void launch_onebody(hipStream_t s, int nw, const int* idx, const double* pos, int periodic, d3 box, u64* force,
-                    u64* energy, double* eunit = nullptr);
-void launch_water_charges(hipStream_t s, int nw, const int* sites, const double* pos, double* q, double* dq);
+// per-water terms: the 1-body energy/forces of waters ob[3w..3w+2] (into force/energy, eunit optional)
+// and/or the charges of sites 4w..4w+3 with their derivatives (q, dq); with both, the waters coincide
+void launch_water_terms(hipStream_t s, int nw, bool onebody, bool charges, const int* ob, const double* pos,
+                        int periodic, d3 box, u64* force, u64* energy, double* eunit, double* q, double* dq);
 void launch_forces_to_double(hipStream_t s, int n3, const u64* fx, double* f);
 void launch_fold_vsites(hipStream_t s, int n, const int* vs_idx, const double* vs_w, double* f);
 
@@ -74,15 +75,20 @@ void launch_twobody_f32(hipStream_t s, long long npairs, const int2* pairs, cons
 void launch_twobody_f64(hipStream_t s, long long npairs, const int2* pairs, const int* idx, const double* pos,
                         int periodic, d3 box, u64* force, u64* energy, const KEvents& ev = KEvents(), double* eunit = nullptr,
                         int max_blocks = 0);
-void launch_threebody_f32(hipStream_t s, long long ntrip, const int* trip, const int* idx, const double* pos,
-                          int periodic, d3 box, u64* force, u64* energy, const KEvents& ev = KEvents(), double* eunit = nullptr,
-                        int max_blocks = 0);
+// FP64, term by term (the coefficients as instruction literals)
 void launch_threebody_f64(hipStream_t s, long long ntrip, const int* trip, const int* idx, const double* pos,
-                          int periodic, d3 box, u64* force, u64* energy, const KEvents& ev = KEvents(), double* eunit = nullptr,
-                        int max_blocks = 0);
-// FP64, the polynomial through its relabelling group (36 KB of LDS per 64-lane block)
-void launch_threebody_sym_f64(hipStream_t s, long long ntrip, const int* trip, const int* idx, const double* pos,
-                          int periodic, d3 box, u64* force, u64* energy, const KEvents& ev = KEvents(), double* eunit = nullptr,
-                        int max_blocks = 0);
+                          int periodic, d3 box, u64* force, u64* energy, const KEvents& ev = KEvents(),
+                          double* eunit = nullptr, int max_blocks = 0);
+// the polynomial through its relabelling group (pip_orbit_loop.h): FP32, FP64 with the register budget of
+// two waves per SIMD, FP64 with that of three
+void launch_threebody_orbit_f32(hipStream_t s, long long ntrip, const int* trip, const int* idx, const double* pos,
+                                int periodic, d3 box, u64* force, u64* energy, const KEvents& ev = KEvents(),
+                                double* eunit = nullptr, int max_blocks = 0);
+void launch_threebody_orbit_f64(hipStream_t s, long long ntrip, const int* trip, const int* idx, const double* pos,
+                                int periodic, d3 box, u64* force, u64* energy, const KEvents& ev = KEvents(),
+                                double* eunit = nullptr, int max_blocks = 0);
+void launch_threebody_orbit3_f64(hipStream_t s, long long ntrip, const int* trip, const int* idx, const double* pos,
+                                 int periodic, d3 box, u64* force, u64* energy, const KEvents& ev = KEvents(),
+                                 double* eunit = nullptr, int max_blocks = 0);
 
 }  // namespace mbpol

@@ -1563,6 +1563,71 @@ __global__ void k_potential_at_points(int npts, const double* __restrict__ pts, 
     out[k] = v * el_electric;
 }
 
+// System multipole moments about the centre of mass (MBPolReferenceElectrostaticsForce.cpp:923-1021),
+// one block of 256 lanes: per-lane partial sums over a fixed stride, then a fixed-order tree
+// reduction in LDS, so the result does not depend on scheduling.  out[13]: net charge, dipole
+// (Debye), traceless quadrupole (Buckingham, row-major 3x3).
+__global__ void __launch_bounds__(256) k_system_moments(int n, const double* __restrict__ pos,
+                                                        const double* __restrict__ q, const double* __restrict__ mu,
+                                                        const double* __restrict__ mass, double* __restrict__ out) {
+    __shared__ double red[10][256];
+    const int t = threadIdx.x;
+    // centre of mass
+    double m = 0, cx = 0, cy = 0, cz = 0;
+    for (int i = t; i < n; i += 256) {
+        m += mass[i];
+        cx += pos[3 * i] * mass[i];
+        cy += pos[3 * i + 1] * mass[i];
+        cz += pos[3 * i + 2] * mass[i];
+    }
+    red[0][t] = m; red[1][t] = cx; red[2][t] = cy; red[3][t] = cz;
+    __syncthreads();
+    for (int k = 128; k > 0; k >>= 1) {
+        if (t < k)
+            for (int c = 0; c < 4; ++c) red[c][t] += red[c][t + k];
+        __syncthreads();
+    }
+    const double mt = red[0][0];
+    const double s = mt > 0.0 ? 1.0 / mt : 1.0;
+    const double com[3] = {mt > 0.0 ? red[1][0] * s : 0.0, mt > 0.0 ? red[2][0] * s : 0.0, mt > 0.0 ? red[3][0] * s : 0.0};
+    __syncthreads();
+    // net charge, dipole and second moments
+    double a[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int i = t; i < n; i += 256) {
+        const double r[3] = {pos[3 * i] - com[0], pos[3 * i + 1] - com[1], pos[3 * i + 2] - com[2]};
+        const double qi = q[i], m0 = mu[3 * i], m1 = mu[3 * i + 1], m2 = mu[3 * i + 2];
+        a[0] += qi;
+        a[1] += r[0] * qi + m0;
+        a[2] += r[1] * qi + m1;
+        a[3] += r[2] * qi + m2;
+        a[4] += r[0] * r[0] * qi + 2.0 * r[0] * m0;                 // xx
+        a[5] += r[0] * r[1] * qi + r[0] * m1 + r[1] * m0;            // xy
+        a[6] += r[0] * r[2] * qi + r[0] * m2 + r[2] * m0;            // xz
+        a[7] += r[1] * r[1] * qi + 2.0 * r[1] * m1;                 // yy
+        a[8] += r[1] * r[2] * qi + r[1] * m2 + r[2] * m1;            // yz
+        a[9] += r[2] * r[2] * qi + 2.0 * r[2] * m2;                 // zz
+    }
+    for (int c = 0; c < 10; ++c) red[c][t] = a[c];
+    __syncthreads();
+    for (int k = 128; k > 0; k >>= 1) {
+        if (t < k)
+            for (int c = 0; c < 10; ++c) red[c][t] += red[c][t + k];
+        __syncthreads();
+    }
+    if (t == 0) {
+        const double debye = 4.80321;
+        const double xx = red[4][0], xy = red[5][0], xz = red[6][0], yy = red[7][0], yz = red[8][0], zz = red[9][0];
+        const double tr = (xx + yy + zz) / 3.0, qs = 100.0 * 3.0 * debye;
+        out[0] = red[0][0];
+        for (int c = 0; c < 3; ++c) out[1 + c] = red[1 + c][0] * 10.0 * debye;
+        out[4] = 0.5 * (xx - tr) * qs;
+        out[5] = out[7] = 0.5 * xy * qs;
+        out[6] = out[10] = 0.5 * xz * qs;
+        out[8] = 0.5 * (yy - tr) * qs;
+        out[9] = out[11] = 0.5 * yz * qs;
+        out[12] = 0.5 * (zz - tr) * qs;
+    }
+}
 
 // ---------------------------------------------------------------------------
 // launchers
@@ -1730,6 +1795,10 @@ void launch_energy_finish(hipStream_t s, int nblocks, const double* eblk, u64* e
 void launch_potential_at_points(hipStream_t s, int npts, const double* pts, int n, const double* pos, const double* q,
                                 const double* mu, ElecParams P, double* out) {
     hipLaunchKernelGGL(k_potential_at_points, nblk(npts, 64), 64, 0, s, npts, pts, n, pos, q, mu, P, out);
+}
+void launch_system_moments(hipStream_t s, int n, const double* pos, const double* q, const double* mu,
+                           const double* mass, double* out) {
+    hipLaunchKernelGGL(k_system_moments, 1, 256, 0, s, n, pos, q, mu, mass, out);
 }
 void launch_add_forces_fix(hipStream_t s, int n, const double* f, u64* fx) {
     hipLaunchKernelGGL(k_add_forces_fix, nblk(3 * n, 256), 256, 0, s, n, f, fx);

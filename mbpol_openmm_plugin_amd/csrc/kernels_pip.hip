@@ -5,17 +5,32 @@
 // (float on the mixed-precision path, double on the FP64 path) only ever sees
 // small coordinates.  The polynomial itself (generated, pip_eval_gen.h) keeps
 // variables, gradients and coefficients in registers.  Per-atom forces are
-// accumulated in T inside the lane and scattered once as 64-bit fixed point.
+// accumulated in T inside the lane and scattered once as 64-bit fixed point;
+// the first molecule's are first summed over the lanes that share it (the
+// lists are ordered by first molecule), one atomic per run of lanes.
 //
 // This file is compiled once per body order and arithmetic type:
 //   -DPIP_BODY=2|3  -DPIP_REAL=float|double
-// and once more for the 3-body term with -DPIP_SYM=1: the polynomial evaluated
-// through its relabelling group (pip_sym_gen.h) instead of term by term.
+// and once more per type for the 3-body term with -DPIP_ORBIT=1: the
+// polynomial evaluated through its relabelling group (pip_orbit_loop.h)
+// instead of term by term.
 #include "device_common.h"
-#if PIP_SYM
-#include "pip_sym_gen.h"
+// an empty asm that re-defines a variable in its register: no instruction
+#define PIP_FENCE(v) asm volatile("" : "+v"(v))
+// the same for a uniform pointer in scalar registers
+#define PIP_OPAQUE_PTR(p) asm volatile("" : "+s"(p))
+// tables read through the constant address space: scalar loads
+#define PIP_CONST_PTR(T) const __attribute__((address_space(4))) T*
+#if PIP_ORBIT
+#include "pip_orbit_loop.h"
 #else
 #include "pip_eval_gen.h"
+#endif
+// Register budget of the 3-body kernel, as waves per SIMD (-DPIP3_WAVES=n)
+#if defined(PIP3_WAVES) && PIP3_WAVES > 1
+#define PIP3_OCCUPANCY __attribute__((amdgpu_waves_per_eu(PIP3_WAVES)))
+#else
+#define PIP3_OCCUPANCY
 #endif
 
 #ifndef PIP_BODY
@@ -155,6 +170,9 @@ __global__ void __launch_bounds__(64) k_twobody(long long np, const int2* __rest
         vr R[10];
         d3 dOO;
         double rOO;
+        // forces on the first molecule, summed over the wave's runs of dimers that share it
+        u64 fa[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        const long long key = t < np ? (long long)pairs[t].x : -1;
         if (t < np && dimer_geometry(t, pairs, idx, pos, periodic, boxA, at, R, dOO, rOO)) {
             real x[31], g[31];
 #pragma unroll
@@ -216,9 +234,23 @@ __global__ void __launch_bounds__(64) k_twobody(long long np, const int2* __rest
                 double fx = (double)F[q].x * s, fy = (double)F[q].y * s, fz = (double)F[q].z * s;
                 if (q == 0) { fx += gs * dOO.x; fy += gs * dOO.y; fz += gs * dOO.z; }
                 if (q == 3) { fx -= gs * dOO.x; fy -= gs * dOO.y; fz -= gs * dOO.z; }
-                add_force(force, at[q], fx, fy, fz);
+                if (q < 3) {
+                    fa[3 * q] = to_fix(fx);
+                    fa[3 * q + 1] = to_fix(fy);
+                    fa[3 * q + 2] = to_fix(fz);
+                } else {
+                    add_force(force, at[q], fx, fy, fz);
+                }
             }
             e_out = sw * (double)Ep * cal2joule;
+        }
+        if (wave_run_sum(key, fa) && key >= 0) {
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                const int a = idx[3 * key + q];
+#pragma unroll
+                for (int d = 0; d < 3; ++d) atomicAdd(&force[3 * a + d], fa[3 * q + d]);
+            }
         }
         if (eunit && t < np) eunit[t] = e_out;
         add_energy(energy, e_out);
@@ -329,43 +361,10 @@ __device__ __forceinline__ bool trimer_geometry(long long t, const int* __restri
     return true;
 }
 
-#if PIP_SYM
-// E(x) and dE/dx of the 3-body polynomial as R(x) + sum_h Q(y_h), y_h[m] =
-// x[perm[h][m]] (pip_sym_gen.h): the relabelled variables are read from, and
-// the gradients scattered back into, this lane's LDS column, so the loop over
-// the 24 relabellings runs the same small block of code (it stays in the
-// instruction cache, where the term-by-term polynomial is 733 KB of code).
-// Each lane touches only its own column: no barrier.
-__device__ __forceinline__ real pip3b_sym_eval(real (&x)[36], real (&g)[36], real* xs, real* gs) {
-    const int lane = threadIdx.x & 63;
-    real e = pip3b_rest<real>(x, g);
-#pragma unroll
-    for (int n = 0; n < 36; ++n) {
-        xs[n * 64 + lane] = x[n];
-        gs[n * 64 + lane] = g[n];
-    }
-#pragma nounroll
-    for (int h = 0; h < pip3b_ngroup; ++h) {
-        real y[36], q[36];
-#pragma unroll
-        for (int m = 0; m < 36; ++m) y[m] = xs[pip3b_perm[h][m] * 64 + lane];
-        e += pip3b_orbit<real>(y, q);
-#pragma unroll
-        for (int m = 0; m < 36; ++m) gs[pip3b_perm[h][m] * 64 + lane] += q[m];
-    }
-#pragma unroll
-    for (int n = 0; n < 36; ++n) g[n] = gs[n * 64 + lane];
-    return e;
-}
-#endif
-
-__global__ void __launch_bounds__(64) k_threebody(long long nt, const int* __restrict__ trip, const int* __restrict__ idx,
+__global__ void __launch_bounds__(64) PIP3_OCCUPANCY k_threebody(long long nt, const int* __restrict__ trip, const int* __restrict__ idx,
                                                   const double* __restrict__ pos, int periodic, d3 box,
                                                   u64* __restrict__ force, u64* __restrict__ energy,
                                                   double* __restrict__ eunit) {
-#if PIP_SYM
-    __shared__ real sym_x[36 * 64], sym_g[36 * 64];  // this wave's variables and gradient, [variable][lane]
-#endif
     // a capped grid strides over the list in whole waves
     for (long long t0 = (long long)blockIdx.x * blockDim.x; t0 < nt; t0 += (long long)gridDim.x * blockDim.x) {
         const long long t = t0 + threadIdx.x;
@@ -374,6 +373,9 @@ __global__ void __launch_bounds__(64) k_threebody(long long nt, const int* __res
         vr R[9];
         d3 roo[3];
         double doo[3];
+        // forces on the first molecule, summed over the wave's runs of trimers that share it
+        u64 fa[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        const long long key = t < nt ? (long long)trip[3 * t] : -1;
         if (t < nt && trimer_geometry(t, trip, idx, pos, periodic, box, at, R, roo, doo)) {
             real x[36], g[36];
 #pragma unroll
@@ -383,8 +385,8 @@ __global__ void __launch_bounds__(64) k_threebody(long long nt, const int* __res
                 const real r = sqrt(d.dot(d));
                 x[n] = exp_t<real>(-(real)v.k * (r - (real)v.r0));
             }
-#if PIP_SYM
-            real Ep = pip3b_sym_eval(x, g, sym_x, sym_g);
+#if PIP_ORBIT
+            real Ep = pip3b_orbit_eval<real>(x, g);
 #else
             real Ep = pip3b_eval<real>(x, g);
 #endif
@@ -436,9 +438,23 @@ __global__ void __launch_bounds__(64) k_threebody(long long nt, const int* __res
                     fy -= (gac * roo[1].y + gbc * roo[2].y) * u;
                     fz -= (gac * roo[1].z + gbc * roo[2].z) * u;
                 }
-                add_force(force, at[q], fx, fy, fz);
+                if (q < 3) {
+                    fa[3 * q] = to_fix(fx);
+                    fa[3 * q + 1] = to_fix(fy);
+                    fa[3 * q + 2] = to_fix(fz);
+                } else {
+                    add_force(force, at[q], fx, fy, fz);
+                }
             }
             e_out = E * s * cal2joule;
+        }
+        if (wave_run_sum(key, fa) && key >= 0) {
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                const int a = idx[3 * key + q];
+#pragma unroll
+                for (int d = 0; d < 3; ++d) atomicAdd(&force[3 * a + d], fa[3 * q + d]);
+            }
         }
         if (eunit && t < nt) eunit[t] = e_out;
         add_energy(energy, e_out);

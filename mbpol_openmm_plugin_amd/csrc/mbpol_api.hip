@@ -17,10 +17,17 @@
 
 using namespace mbpol;
 
-// 3-body polynomial term by term by default; through its relabelling group with
-// MBPOL_HIP_3B_SYM=1 (less instruction traffic, but 36 KB of LDS per wave and
-// slower: 3.1-4.9 against 2.87 ms per 4096-water evaluation)
-static constexpr bool kPip3SymDefault = false;
+// 3-body kernel (MBPOL_HIP_3B_KERNEL): "orbit" (default), the polynomial as 24
+// passes of one orbit-representative polynomial over the trimer's relabelling
+// group; "orbit3", the same built for three waves per SIMD; "terms", term by
+// term with the coefficients as instruction literals.
+static int pip3_kernel_choice() {
+    const char* env = std::getenv("MBPOL_HIP_3B_KERNEL");
+    if (!env) return 0;
+    if (!std::strcmp(env, "terms")) return 1;
+    if (!std::strcmp(env, "orbit3")) return 2;
+    return 0;
+}
 
 static thread_local std::string g_last_error;
 
@@ -527,13 +534,11 @@ int mbpol_context_create(const mbpol_system* sys, int32_t precision, int32_t dev
         int waves = 1;
         if (const char* env = std::getenv("MBPOL_HIP_PIP_WAVES")) waves = std::max(0, std::atoi(env));
         ctx->pip_blocks = waves * 4 * prop.multiProcessorCount;
-        // the 3-body polynomial through its relabelling group (MBPOL_HIP_3B_SYM=0|1): one
-        // 64-lane block per CU holds 36 KB of LDS; MBPOL_HIP_3B_SYM_WAVES blocks per CU
-        const char* sym = std::getenv("MBPOL_HIP_3B_SYM");
-        ctx->pip3_sym = sym ? std::atoi(sym) != 0 : kPip3SymDefault;
-        int swaves = 2;
-        if (const char* env = std::getenv("MBPOL_HIP_3B_SYM_WAVES")) swaves = std::max(1, std::atoi(env));
-        ctx->sym_blocks = swaves * prop.multiProcessorCount;
+        // the 3-body kernel and its grid cap (MBPOL_HIP_3B_WAVES waves per SIMD)
+        ctx->pip3_kernel = pip3_kernel_choice();
+        int waves3 = waves;
+        if (const char* env = std::getenv("MBPOL_HIP_3B_WAVES")) waves3 = std::max(0, std::atoi(env));
+        ctx->pip3_blocks = waves3 * 4 * prop.multiProcessorCount;
         int dwaves = 2;  // the dispersion kernel: two waves per SIMD (MBPOL_HIP_DISP_WAVES)
         if (const char* env = std::getenv("MBPOL_HIP_DISP_WAVES")) dwaves = std::max(0, std::atoi(env));
         ctx->disp_blocks = dwaves * 4 * prop.multiProcessorCount;
@@ -563,6 +568,13 @@ int mbpol_context_create(const mbpol_system* sys, int32_t precision, int32_t dev
     apply_list_groups(ctx);
     if (sys->ne > 0) {
         if (int e = elec_create(ctx)) return fail(e);
+        // the per-water kernel computes the 1-body term with the charges when the
+        // 1-body waters are the electrostatics waters, in order
+        const int nw1 = (int)sys->ob_idx.size() / 3;
+        bool same = sys->el_redistribute && nw1 > 0 && 4 * nw1 == sys->ne;
+        for (int w = 0; same && w < nw1; ++w)
+            same = sys->ob_idx[3 * w] == 4 * w && sys->ob_idx[3 * w + 1] == 4 * w + 1 && sys->ob_idx[3 * w + 2] == 4 * w + 2;
+        ctx->water_fused = same;
     }
     if (sys->nd > 0) {
         if (int e = disp_create(ctx)) return fail(e);
@@ -607,6 +619,52 @@ int mbpol_context_set_box(mbpol_context* ctx, const double box_nm[3]) {
     if (ctx->disp)
         if (int e = disp_set_box(ctx)) return e;
     return MBPOL_OK;
+}
+
+// copyParametersToContext of the 1-, 2- and 3-body Forces
+// (MBPolReferenceKernels.cpp:116-127, 401-414, 485-498): same molecule count, new index lists
+static int update_molecules(mbpol_context* ctx, int32_t n, const int32_t* idx, std::vector<int>& host, DevBuf& dev,
+                            const char* what, const char* count_msg) {
+    if (!ctx) return set_error(MBPOL_ERR_INVALID_ARGUMENT, "null context");
+    if (n != (int32_t)(host.size() / 3)) return set_error(MBPOL_ERR_COUNT_CHANGED, "updateParametersInContext: %s", count_msg);
+    if (int e = check_molecules(&ctx->sys, n, idx, what)) return e;
+    const std::vector<int> old = host;
+    host.assign(idx, idx + 3 * n);
+    if (!ctx->sys.grp_wstart.empty())
+        if (int e = check_groups(ctx->sys)) {
+            host = old;
+            return e;
+        }
+    HIP_TRY(hipSetDevice(ctx->device));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if (ctx->side) HIP_TRY(hipStreamSynchronize(ctx->side));
+    if (n > 0) HIP_TRY(hipMemcpy(dev.p, host.data(), sizeof(int) * host.size(), hipMemcpyHostToDevice));
+    return MBPOL_OK;
+}
+
+int mbpol_context_update_onebody(mbpol_context* ctx, int32_t n, const int32_t* idx) {
+    if (!ctx) return set_error(MBPOL_ERR_INVALID_ARGUMENT, "null context");
+    if (int e = update_molecules(ctx, n, idx, ctx->sys.ob_idx, ctx->ob_idx, "MBPolOneBodyForce",
+                                 "The number of stretch-bends has changed"))
+        return e;
+    const mbpol_system& s = ctx->sys;
+    bool same = ctx->elec && s.el_redistribute && 4 * n == s.ne;
+    for (int w = 0; same && w < n; ++w)
+        same = s.ob_idx[3 * w] == 4 * w && s.ob_idx[3 * w + 1] == 4 * w + 1 && s.ob_idx[3 * w + 2] == 4 * w + 2;
+    ctx->water_fused = same;
+    return MBPOL_OK;
+}
+
+int mbpol_context_update_twobody(mbpol_context* ctx, int32_t n, const int32_t* idx) {
+    if (!ctx) return set_error(MBPOL_ERR_INVALID_ARGUMENT, "null context");
+    return update_molecules(ctx, n, idx, ctx->sys.tb_idx, ctx->tb_idx, "MBPolTwoBodyForce",
+                            "The number of particles has changed");
+}
+
+int mbpol_context_update_threebody(mbpol_context* ctx, int32_t n, const int32_t* idx) {
+    if (!ctx) return set_error(MBPOL_ERR_INVALID_ARGUMENT, "null context");
+    return update_molecules(ctx, n, idx, ctx->sys.th_idx, ctx->th_idx, "MBPolThreeBodyForce",
+                            "The number of particles has changed");
 }
 
 int mbpol_context_set_list_mode(mbpol_context* ctx, int32_t full_chain_list) {
@@ -694,10 +752,11 @@ static int short_range(mbpol_context* ctx, hipStream_t sr, uint32_t comp) {
     // independent groups: per-unit energies, summed per group after each term
     const int G = ctx->ngroups;
     u64* EG = ctx->g_energy.as<u64>();
-    if (do1) {
+    if (do1 && !ctx->onebody_with_charges) {
         timer_start(ctx, T_ONEBODY);
-        launch_onebody(sr, (int)s.ob_idx.size() / 3, ctx->ob_idx.as<int>(), pos, s.ob_method == MBPOL_ONEBODY_PERIODIC,
-                       box, F, E + 0, G ? ctx->g_eu1.as<double>() : nullptr);
+        launch_water_terms(sr, (int)s.ob_idx.size() / 3, true, false, ctx->ob_idx.as<int>(), pos,
+                           s.ob_method == MBPOL_ONEBODY_PERIODIC, box, F, E + 0, G ? ctx->g_eu1.as<double>() : nullptr,
+                           nullptr, nullptr);
         if (G) launch_group_sums(sr, G, ctx->g_wstart.as<int>(), 1, nullptr, ctx->g_eu1.as<double>(), EG, 0);
         timer_stop(ctx, T_ONEBODY);
     }
@@ -730,15 +789,18 @@ static int short_range(mbpol_context* ctx, hipStream_t sr, uint32_t comp) {
     if (do3) {
         const int per = s.th_method == MBPOL_CUTOFF_PERIODIC;
         const KEvents ev = timer_kernel(ctx, T_THREEBODY);
-        if (dp && ctx->pip3_sym)
-            launch_threebody_sym_f64(sr, ctx->stats.trimers, ctx->trips.as<int>(), ctx->th_idx.as<int>(), pos, per,
-                                     box, F, E + 2, ev, eu3, ctx->sym_blocks);
-        else if (dp)
-            launch_threebody_f64(sr, ctx->stats.trimers, ctx->trips.as<int>(), ctx->th_idx.as<int>(), pos, per, box, F,
-                                 E + 2, ev, eu3, ctx->pip_blocks);
+        const long long nt = ctx->stats.trimers;
+        const int* tr = ctx->trips.as<int>();
+        const int* ti = ctx->th_idx.as<int>();
+        const int cap = ctx->pip3_blocks;
+        if (!dp)
+            launch_threebody_orbit_f32(sr, nt, tr, ti, pos, per, box, F, E + 2, ev, eu3, cap);
+        else if (ctx->pip3_kernel == 1)
+            launch_threebody_f64(sr, nt, tr, ti, pos, per, box, F, E + 2, ev, eu3, cap);
+        else if (ctx->pip3_kernel == 2)
+            launch_threebody_orbit3_f64(sr, nt, tr, ti, pos, per, box, F, E + 2, ev, eu3, cap);
         else
-            launch_threebody_f32(sr, ctx->stats.trimers, ctx->trips.as<int>(), ctx->th_idx.as<int>(), pos, per, box, F,
-                                 E + 2, ev, eu3, ctx->pip_blocks);
+            launch_threebody_orbit_f64(sr, nt, tr, ti, pos, per, box, F, E + 2, ev, eu3, cap);
         timer_kernel_done(ctx, T_THREEBODY, ev);
         if (G) launch_group_sums(sr, G, ctx->g_wstart.as<int>(), 0, ctx->trip_off.as<long long>(), eu3, EG, 2);
     }
@@ -765,6 +827,9 @@ static int evaluate(mbpol_context* ctx, double* forces_dev_out, double* energies
     HIP_TRY(hipMemsetAsync(E, 0, sizeof(u64) * 8, st));
     if (ctx->ngroups) HIP_TRY(hipMemsetAsync(ctx->g_energy.p, 0, sizeof(u64) * 8 * ctx->ngroups, st));
     launch_place_vsites(st, (int)s.vs_idx.size() / 4, ctx->vs_idx.as<int>(), ctx->vs_w.as<double>(), pos);
+    // the 1-body term runs with the charges when both are computed for the same waters
+    ctx->onebody_with_charges = ctx->water_fused && (comp & MBPOL_ONEBODY) && (comp & MBPOL_ELECTROSTATICS) &&
+                                ctx->elec && !s.ob_idx.empty();
     // The short-range terms go to a second stream, enqueued by the context's
     // worker thread: their host-side waits (list sizes) and their kernels
     // overlap the electrostatics, which the calling thread drives.  With

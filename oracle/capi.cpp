@@ -17,6 +17,11 @@ double threebody_all(int64_t ntrip, const int* trip, const double* pos, const in
 double electrostatics(const oracle_elec_params& P, int n, const double* pos, const double* charge, const int* mol,
                       const int* type, const double* damp, const double* pol, double* forces, oracle_elec_stats* stats,
                       double* dipoles_out, double* fixed_field_out);
+int electrostatic_potential(const oracle_elec_params& P, int n, const double* pos, const double* charge,
+                            const int* mol, const int* type, const double* damp, const double* pol, int npts,
+                            const double* pts, double* out);
+int system_moments(const oracle_elec_params& P, int n, const double* pos, const double* charge, const int* mol,
+                   const int* type, const double* damp, const double* pol, const double* masses, double* out);
 double thole_scale(const oracle_elec_params& P, double damp_i, double damp_j, int mol_i, int mol_j, int type_i, int type_j,
                    double r, int just_scale, int order, int itype);
 void water_charges(const double* pos4, double* q4, double* dq);
@@ -64,6 +69,15 @@ int oracle_electrostatics(const oracle_elec_params* p, int n, const double* pos,
         }
         return 1;
     }
+}
+int oracle_electrostatic_potential(const oracle_elec_params* p, int n, const double* pos, const double* charge,
+                                   const int* mol, const int* type, const double* damp, const double* pol, int npts,
+                                   const double* pts, double* out) {
+    return oracle::electrostatic_potential(*p, n, pos, charge, mol, type, damp, pol, npts, pts, out);
+}
+int oracle_system_moments(const oracle_elec_params* p, int n, const double* pos, const double* charge, const int* mol,
+                          const int* type, const double* damp, const double* pol, const double* masses, double* out) {
+    return oracle::system_moments(*p, n, pos, charge, mol, type, damp, pol, masses, out);
 }
 double oracle_thole_scale(const oracle_elec_params* p, double damp_i, double damp_j, int mol_i, int mol_j, int type_i,
                           int type_j, double r, int just_scale, int order, int itype) {

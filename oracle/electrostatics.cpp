@@ -885,6 +885,105 @@ double electrostatics(const oracle_elec_params& P, int n, const double* pos, con
     return e.run(s, forces, stats, dipoles_out, fixed_field_out);
 }
 
+// The site charges and converged induced dipoles of the reference's setup()
+// (MBPolReferenceElectrostaticsForce.cpp:857-897 and the dipole solve), as
+// getElectrostaticPotential and getSystemElectrostaticsMoments compute them
+// before their sums; returns false when the dipoles did not converge.
+static bool setup_charges_dipoles(const oracle_elec_params& P, int n, const double* pos, const double* charge,
+                                  const int* mol, const int* type, const double* damp, const double* pol,
+                                  std::vector<double>& q, std::vector<double>& mu) {
+    std::vector<Site> s(n);
+    for (int i = 0; i < n; ++i) {
+        s[i].pos = load3(pos, i);
+        s[i].q = charge[i];
+        s[i].mol = mol[i];
+        s[i].type = type[i];
+        s[i].damp = damp[i];
+        s[i].pol = pol[i];
+        s[i].other[0] = s[i].other[1] = s[i].other[2] = i;
+    }
+    std::vector<double> forces(3 * (size_t)n, 0.0);
+    mu.assign(3 * (size_t)n, 0.0);
+    Elec e(P, n);
+    try {
+        e.run(s, forces.data(), nullptr, mu.data(), nullptr);
+    } catch (const std::exception&) {
+        return false;
+    }
+    q.resize(n);
+    for (int i = 0; i < n; ++i) q[i] = s[i].q;
+    return true;
+}
+
+// calculateElectrostaticPotential (:1023-1086): at each grid point, the sum over
+// all sites of q/r - mu.(r_site - r_point)/r^3, minimum image with PME
+// (getPeriodicDelta :1234-1239, no Ewald split), times the electric constant.
+int electrostatic_potential(const oracle_elec_params& P, int n, const double* pos, const double* charge,
+                            const int* mol, const int* type, const double* damp, const double* pol, int npts,
+                            const double* pts, double* out) {
+    std::vector<double> q, mu;
+    if (!setup_charges_dipoles(P, n, pos, charge, mol, type, damp, pol, q, mu)) return 1;
+    for (int k = 0; k < npts; ++k) out[k] = 0.0;
+    for (int i = 0; i < n; ++i)
+        for (int k = 0; k < npts; ++k) {
+            double d[3];
+            for (int c = 0; c < 3; ++c) d[c] = pos[3 * i + c] - pts[3 * k + c];
+            if (P.method == 1)
+                for (int c = 0; c < 3; ++c) d[c] -= std::floor(d[c] * (1.0 / P.box[c]) + 0.5) * P.box[c];
+            const double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+            const double r = std::sqrt(r2);
+            const double rr1 = 1.0 / r, rr3 = rr1 * rr1 * rr1;
+            out[k] += q[i] * rr1 - (mu[3 * i] * d[0] + mu[3 * i + 1] * d[1] + mu[3 * i + 2] * d[2]) * rr3;
+        }
+    for (int k = 0; k < npts; ++k) out[k] *= el_electric;
+    return 0;
+}
+
+// calculateMBPolSystemElectrostaticsMoments (:923-1021): positions relative to
+// the centre of mass; out[0] net charge, out[1..3] dipole (Debye), out[4..12]
+// the traceless quadrupole (row-major 3x3, Buckingham units).
+int system_moments(const oracle_elec_params& P, int n, const double* pos, const double* charge, const int* mol,
+                   const int* type, const double* damp, const double* pol, const double* masses, double* out) {
+    std::vector<double> q, mu;
+    if (!setup_charges_dipoles(P, n, pos, charge, mol, type, damp, pol, q, mu)) return 1;
+    double mtot = 0.0, com[3] = {0, 0, 0};
+    for (int i = 0; i < n; ++i) {
+        mtot += masses[i];
+        for (int c = 0; c < 3; ++c) com[c] += pos[3 * i + c] * masses[i];
+    }
+    if (mtot > 0.0)
+        for (int c = 0; c < 3; ++c) com[c] *= 1.0 / mtot;
+    double net = 0, dpl[3] = {0, 0, 0}, xx = 0, xy = 0, xz = 0, yy = 0, yz = 0, zz = 0;
+    for (int i = 0; i < n; ++i) {
+        double r[3];
+        for (int c = 0; c < 3; ++c) r[c] = pos[3 * i + c] - com[c];
+        const double* m = &mu[3 * i];
+        net += q[i];
+        for (int c = 0; c < 3; ++c) dpl[c] += r[c] * q[i] + m[c];
+        xx += r[0] * r[0] * q[i] + 2.0 * r[0] * m[0];
+        xy += r[0] * r[1] * q[i] + r[0] * m[1] + r[1] * m[0];
+        xz += r[0] * r[2] * q[i] + r[0] * m[2] + r[2] * m[0];
+        yy += r[1] * r[1] * q[i] + 2.0 * r[1] * m[1];
+        yz += r[1] * r[2] * q[i] + r[1] * m[2] + r[2] * m[1];
+        zz += r[2] * r[2] * q[i] + 2.0 * r[2] * m[2];
+    }
+    const double qave = (xx + yy + zz) / 3.0;
+    out[4] = 0.5 * (xx - qave);
+    out[5] = 0.5 * xy;
+    out[6] = 0.5 * xz;
+    out[8] = 0.5 * (yy - qave);
+    out[9] = 0.5 * yz;
+    out[12] = 0.5 * (zz - qave);
+    out[7] = out[5];
+    out[10] = out[6];
+    out[11] = out[9];
+    const double debye = 4.80321;
+    out[0] = net;
+    for (int c = 0; c < 3; ++c) out[1 + c] = dpl[c] * 10.0 * debye;
+    for (int k = 4; k < 13; ++k) out[k] *= 100.0 * 3.0 * debye;
+    return 0;
+}
+
 // Thole scale of one pair (white-box known answers).
 double thole_scale(const oracle_elec_params& P, double damp_i, double damp_j, int mol_i, int mol_j, int type_i, int type_j,
                    double r, int just_scale, int order, int itype) {

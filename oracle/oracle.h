@@ -47,6 +47,15 @@ double oracle_threebody(int64_t ntrip, const int* trip, const double* pos, const
 int oracle_electrostatics(const oracle_elec_params* p, int n, const double* pos, const double* charge, const int* mol,
                           const int* type, const double* damp, const double* pol, double* forces, double* energy,
                           oracle_elec_stats* stats, double* dipoles_out, double* fixed_field_out, char* err, int errlen);
+/* getElectrostaticPotential: potential (kJ/mol/e) at npts points (nm) from the site charges and converged
+ * induced dipoles; returns 0, or 1 when the dipoles did not converge */
+int oracle_electrostatic_potential(const oracle_elec_params* p, int n, const double* pos, const double* charge,
+                                   const int* mol, const int* type, const double* damp, const double* pol, int npts,
+                                   const double* pts, double* out);
+/* getSystemElectrostaticsMoments: out[13] = net charge, dipole (3, Debye), traceless quadrupole (3x3, Buckingham)
+ * about the centre of mass of `masses`; returns 0, or 1 when the dipoles did not converge */
+int oracle_system_moments(const oracle_elec_params* p, int n, const double* pos, const double* charge, const int* mol,
+                          const int* type, const double* damp, const double* pol, const double* masses, double* out);
 double oracle_thole_scale(const oracle_elec_params* p, double damp_i, double damp_j, int mol_i, int mol_j, int type_i,
                           int type_j, double r, int just_scale, int order, int itype);
 void oracle_water_charges(const double* pos4, double* q4, double* dq);

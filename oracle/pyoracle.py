@@ -70,6 +70,8 @@ def lib():
                                          ctypes.c_int, ctypes.c_int]
         L.oracle_gammq.restype = ctypes.c_double
         L.oracle_gammq.argtypes = [ctypes.c_double, ctypes.c_double]
+        L.oracle_electrostatic_potential.restype = ctypes.c_int
+        L.oracle_system_moments.restype = ctypes.c_int
         L.oracle_dispersion.restype = ctypes.c_double
         L.oracle_dispersion.argtypes = [ctypes.c_int, _dp, _ip, _ip, ctypes.c_int, _dp, _dp, ctypes.c_int,
                                         ctypes.c_double, _dp, _dp]
@@ -178,6 +180,29 @@ def electrostatics(params, pos, charge, mol, atype, damp, pol, want_field=False)
     return out
 
 
+def electrostatic_potential(params, pos, charge, mol, atype, damp, pol, points):
+    """getElectrostaticPotential (MBPolReferenceElectrostaticsForce.cpp:1023-1086), kJ/mol/e."""
+    pos, pts = _d(pos), _d(np.asarray(points).reshape(-1, 3))
+    out = np.zeros(len(pts))
+    rc = lib().oracle_electrostatic_potential(ctypes.byref(params), len(pos), _pd(pos), _pd(_d(charge)),
+                                              _pi(_i(mol)), _pi(_i(atype)), _pd(_d(damp)), _pd(_d(pol)),
+                                              len(pts), _pd(pts), _pd(out))
+    if rc != 0:
+        raise RuntimeError('induced dipoles did not converge')
+    return out
+
+
+def system_moments(params, pos, charge, mol, atype, damp, pol, masses):
+    """calculateMBPolSystemElectrostaticsMoments (MBPolReferenceElectrostaticsForce.cpp:923-1021)."""
+    pos = _d(pos)
+    out = np.zeros(13)
+    rc = lib().oracle_system_moments(ctypes.byref(params), len(pos), _pd(pos), _pd(_d(charge)), _pi(_i(mol)),
+                                     _pi(_i(atype)), _pd(_d(damp)), _pd(_d(pol)), _pd(_d(masses)), _pd(out))
+    if rc != 0:
+        raise RuntimeError('induced dipoles did not converge')
+    return out
+
+
 def thole_scale(params, damp_i, damp_j, r, order, itype, mol=(0, 1), types=(0, 0), just_scale=False):
     return lib().oracle_thole_scale(ctypes.byref(params), damp_i, damp_j, mol[0], mol[1], types[0], types[1], r,
                                     int(just_scale), order, itype)
@@ -236,6 +261,35 @@ def pme_parameters(cutoff, tol, box):
     alpha = (1.0 / cutoff) * math.sqrt(-math.log(2.0 * tol))
     grid = [max(int(math.ceil(2 * alpha * L / (3 * tol ** 0.2))), 6) for L in box]
     return alpha, grid
+
+
+def _elec_force_params(system):
+    """The System's electrostatics Force and the oracle parameters it describes."""
+    f = [f for f in system._forces if type(f).__name__ == 'MBPolElectrostaticsForce'][0]
+    box = np.array(system._box) if system._box is not None else None
+    method = f.getNonbondedMethod()
+    alpha, grid = f.getAEwald(), f.getPmeGridDimensions()
+    if method == 1 and (grid[0] == 0 or alpha == 0.0):
+        alpha, grid = pme_parameters(f.getCutoffDistance(), f.getEwaldErrorTolerance(), box)
+    P = elec_params(method=method, cutoff=f.getCutoffDistance(), alpha=alpha, grid=grid,
+                    thole=f.getTholeParameters(), max_iter=f.getMutualInducedMaxIterations(),
+                    target_eps=f.getMutualInducedTargetEpsilon(),
+                    charge_redistribution=f.getIncludeChargeRedistribution(),
+                    box=box if box is not None else (0, 0, 0))
+    return f, P
+
+
+def system_electrostatic_potential(system, positions, points):
+    f, P = _elec_force_params(system)
+    q, mol, typ, damp, pol = f._arrays()
+    return electrostatic_potential(P, system.computeVirtualSites(positions), q, mol, typ, damp, pol, points)
+
+
+def system_electrostatics_moments(system, positions):
+    f, P = _elec_force_params(system)
+    q, mol, typ, damp, pol = f._arrays()
+    masses = [system.getParticleMass(i) for i in range(system.getNumParticles())]
+    return system_moments(P, system.computeVirtualSites(positions), q, mol, typ, damp, pol, masses)
 
 
 def evaluate_system(system, positions, components=31, return_lists=False):

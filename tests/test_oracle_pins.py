@@ -213,3 +213,41 @@ def test_pme_matches_direct_ewald_sum(oracle):
     E = f * (e_real + e_rec + e_self)
     assert abs(r['energy'] - E) <= 2e-6 * abs(E), (r['energy'], E)
     np.testing.assert_allclose(r['forces'], f * F, atol=2e-5 * np.abs(f * F).max())
+
+
+def test_potential_and_moments_far_field(mb, oracle):
+    """getElectrostaticPotential and getSystemElectrostaticsMoments against the
+    multipole expansion they must agree with: far from a neutral 14-water cluster,
+    phi(R) = k [p.R/R^3 + sum_ij Theta_ij R_i R_j / R^5 + O(R^-4)] about the centre
+    of mass, with p the moments' dipole (Debye -> e nm) and Theta the traceless
+    quadrupole (1/2) sum [q (3 r_i r_j - r^2 delta_ij) + 3 (r_i mu_j + r_j mu_i) - 2 r.mu delta_ij],
+    which the moments report as Theta / 3 in units of 100 x 3 x 4.80321.  The
+    charge term's own expansion fixes the signs, so a sign or factor slip in the
+    dipole or quadrupole terms of either function fails.  Also: zero net charge, and
+    a symmetric quadrupole block."""
+    waters = mb.systems.config0()
+    sysm = mb.forcefield.create_system(14, nonbonded_method='CutoffNonPeriodic', twobody_cutoff=1e10,
+                                       threebody_cutoff=1e10, nonbonded_cutoff=1e3, dipole_epsilon=1e-12)
+    pos = mb.forcefield.water_positions_with_m(waters)
+    mom = oracle.system_electrostatics_moments(sysm, pos)
+    assert abs(mom[0]) < 1e-12
+    Q = mom[4:].reshape(3, 3)
+    np.testing.assert_allclose(Q, Q.T, atol=1e-12)
+    assert abs(np.trace(Q)) < 1e-9 * np.abs(Q).max()
+    debye = 4.80321
+    p = mom[1:4] / (10.0 * debye)                # e nm
+    theta = Q * 3.0 / (100.0 * 3.0 * debye)     # e nm^2
+    k = 138.9354558456
+    masses = np.array([sysm.getParticleMass(i) for i in range(sysm.getNumParticles())])
+    com = (pos * masses[:, None]).sum(axis=0) / masses.sum()
+    rng = np.random.default_rng(6)
+    dirs = rng.normal(size=(6, 3))
+    dirs /= np.linalg.norm(dirs, axis=1)[:, None]
+    for R in (40.0, 80.0):
+        pts = com + R * dirs
+        phi = oracle.system_electrostatic_potential(sysm, pos, pts)
+        dip = k * (dirs @ p) / R ** 2
+        quad = k * np.einsum('ki,ij,kj->k', dirs, theta, dirs) / R ** 3
+        # the dipole term dominates; what remains is the quadrupole term to O(size/R)
+        np.testing.assert_allclose(phi, dip + quad, rtol=0, atol=0.05 * np.abs(quad).max() + 1e-6 * np.abs(dip).max())
+        np.testing.assert_allclose(phi - dip, quad, rtol=0, atol=0.1 * np.abs(quad).max() * 40.0 / R)
