@@ -64,6 +64,8 @@ def parse():
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--no-fp64-path', action='store_true')
     ap.add_argument('--no-dipole-iterations', action='store_true')
+    ap.add_argument('--cpu-config1-evals', type=int, default=1,
+                    help='complete oracle evaluations of the 256-water box timed beside the sampled estimate')
     return ap.parse_args()
 
 
@@ -196,6 +198,46 @@ print(json.dumps(dict(parts=res, n2=k2, n3=k3, ndimers=len(pairs), ntrimers=len(
             'seconds_per_eval_by_component': r['parts'], 'electrostatics_phases_s': r['elec']}
 
 
+def cpu_full_config1(evals):
+    """The FP64 oracle's complete evaluation of the 256-water box (all five
+    components, PME, SOR from alpha E_fixed as the reference iterates, its O-O
+    lists built in full), `evals` times on one core (taskset core 0): median
+    seconds per evaluation, with the CPU model."""
+    code = r'''
+import sys, time, json, statistics
+sys.path.insert(0, %(root)r); sys.path.insert(0, %(oracle)r)
+import pyoracle as po, mbpol_openmm_plugin_amd as m
+waters, L = m.systems.config1()
+s = m.forcefield.create_system(256, nonbonded_method='PME', box=L, ewald_error_tolerance=1e-5, dipole_epsilon=1e-8)
+pos = m.forcefield.water_positions_with_m(waters)
+ts = []
+for _ in range(%(evals)d):
+    t = time.perf_counter(); E, F, Ls = po.evaluate_system(s, pos, return_lists=True); ts.append(time.perf_counter() - t)
+model = ''
+try:
+    for ln in open('/proc/cpuinfo'):
+        if ln.startswith('model name'):
+            model = ln.split(':', 1)[1].strip(); break
+except OSError:
+    pass
+print(json.dumps(dict(seconds=ts, iterations=Ls['elec']['iterations'], cpu_model=model)))
+''' % dict(root=ROOT, oracle=os.path.join(ROOT, 'oracle'), evals=evals)
+    cmd = [sys.executable, '-c', code]
+    if os.path.exists('/usr/bin/taskset'):
+        cmd = ['/usr/bin/taskset', '-c', '0'] + cmd
+    out = subprocess.run(cmd, capture_output=True, text=True, env=dict(os.environ, OMP_NUM_THREADS='1'),
+                         timeout=120 * evals + 60)
+    if out.returncode != 0:
+        return {'error': out.stderr[-400:]}
+    r = json.loads(out.stdout.strip().splitlines()[-1])
+    med = float(np.median(r['seconds']))
+    return {'workload': '256 waters, PBC 19.71 A, PME tol 1e-5, all five components, SOR from alpha E_fixed',
+            'evaluations': len(r['seconds']), 'seconds_per_eval_median': med, 'evals_per_s': 1.0 / med,
+            'seconds': r['seconds'], 'dipole_iterations': r['iterations'], 'cores': 1, 'cpu_model': r['cpu_model'],
+            'oracle_mode': 'oracle/: the reference loop structure (all-pairs electrostatics with the cutoff '
+                           'test inside, PME reciprocal part, SOR omega 0.55), polynomials term by term'}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -310,6 +352,16 @@ def main():
                 zip(['onebody', 'twobody', 'threebody', 'dispersion', 'electrostatics', 'total'],
                     (np.abs(energies - e64) / tot).tolist())),
             'mixed_vs_fp64_rel_rms_force': float(np.sqrt(((f_mixed - f64) ** 2).mean() / (f64 ** 2).mean()))}
+        # FP64 everywhere with the conjugate-gradient solve: separates the solver's gain from precision's
+        ctx64.setDipoleSolver('pcg')
+        for _ in range(args.warmup):
+            ctx64.evaluateResident()
+        ms64p, e64p = run_steps(ctx64, args.steps)
+        line['fp64_pcg_path'] = {
+            'value': 1000.0 / ms64p, 'ms_per_step': ms64p, 'dipole_solver': 'pcg',
+            'iterations': ctx64.getStats()['iterations'],
+            'speedup_over_fp64_sor': ms64 / ms64p, 'speedup_of_default_path_over_it': ms64p / ms_per_step,
+            'rel_error_of_total_vs_fp64_sor': float(abs(e64p[5] - e64[5]) / tot)}
         del ctx64
     if not args.no_cpu_baseline:
         if args.precision != 'double' and args.no_fp64_path:  # iterations of the reference's SOR solve
@@ -321,6 +373,8 @@ def main():
                 sor_iterations = ctx.getStats()['iterations']
                 ctx.setDipoleSolver('pcg')
         line['cpu_baseline'] = cpu_baseline(sor_iterations)
+        if args.cpu_config1_evals > 0:
+            line['cpu_baseline']['full_config1'] = cpu_full_config1(args.cpu_config1_evals)
     print(json.dumps(line), flush=True)
 
 

@@ -79,16 +79,12 @@ void launch_twobody_f64(hipStream_t s, long long npairs, const int2* pairs, cons
 void launch_threebody_f64(hipStream_t s, long long ntrip, const int* trip, const int* idx, const double* pos,
                           int periodic, d3 box, u64* force, u64* energy, const KEvents& ev = KEvents(),
                           double* eunit = nullptr, int max_blocks = 0);
-// the polynomial through its relabelling group (pip_orbit_loop.h): FP32, FP64 with the register budget of
-// two waves per SIMD, FP64 with that of three
+// the polynomial through its relabelling group (pip_orbit_loop.h), FP32 and FP64
 void launch_threebody_orbit_f32(hipStream_t s, long long ntrip, const int* trip, const int* idx, const double* pos,
                                 int periodic, d3 box, u64* force, u64* energy, const KEvents& ev = KEvents(),
                                 double* eunit = nullptr, int max_blocks = 0);
 void launch_threebody_orbit_f64(hipStream_t s, long long ntrip, const int* trip, const int* idx, const double* pos,
                                 int periodic, d3 box, u64* force, u64* energy, const KEvents& ev = KEvents(),
                                 double* eunit = nullptr, int max_blocks = 0);
-void launch_threebody_orbit3_f64(hipStream_t s, long long ntrip, const int* trip, const int* idx, const double* pos,
-                                 int periodic, d3 box, u64* force, u64* energy, const KEvents& ev = KEvents(),
-                                 double* eunit = nullptr, int max_blocks = 0);
 
 }  // namespace mbpol

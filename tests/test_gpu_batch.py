@@ -6,6 +6,8 @@ reference, which evaluates one cluster per Context).
   each cluster alone: per-cluster energies of every component within 1e-6 of
   the cluster's total energy, per-cluster forces within 1e-4 relative RMS
   (the mixed-precision bar);
+* every one of the 2048 clusters of 3..64 waters against the oracle fixture
+  tests/golden/config4_oracle.npz (energies and force checksums);
 * the 2048-cluster batch of 3..64 waters: group energies add up to the
   system's energies, and a cluster's energies do not depend on the other
   clusters in the batch (same cluster alone and at another place in a
@@ -83,3 +85,45 @@ def test_full_batch_is_independent_per_cluster(mb):
     e1 = ctx1.getGroupEnergies()[0]
     print('largest cluster (%d waters) alone %s, in batch %s' % (len(clusters[k]), e1, ge[k]), flush=True)
     assert np.all(np.abs(e1 - ge[k]) <= 1e-7 * (abs(ge[k, 5]) + 1.0))
+
+
+def test_full_batch_matches_oracle_per_cluster(mb):
+    """All 2048 clusters of the batch against the FP64 oracle run on each
+    cluster alone (tests/golden/config4_oracle.npz, written by
+    tests/golden/make_config4_fixture.py): every component's energy within
+    1e-6 of the cluster's total energy, and two force checksums per cluster --
+    the RMS force within 1e-4 relative, and the projection on a seeded normal
+    field within 1e-4 of |F| |u| (the mixed-precision bar)."""
+    import hashlib
+    import os
+    ref = np.load(os.path.join(os.path.dirname(__file__), 'golden', 'config4_oracle.npz'))
+    clusters = mb.systems.config4()
+    h = hashlib.sha256()
+    for c in clusters:
+        h.update(np.ascontiguousarray(c, dtype=np.float64).tobytes())
+    assert h.hexdigest() == str(ref['positions_sha256'])
+    sysb, pos, start = mb.forcefield.create_cluster_batch(clusters)
+    ctx = mb.Context(sysb, precision='mixed')
+    ctx.setPositions(pos)
+    F = ctx.getState().getForces()
+    ge = ctx.getGroupEnergies()
+    E = ref['energies']
+    tot = np.maximum(np.abs(E[:, 5]), 1.0)
+    err_e = np.abs(ge - E).max(axis=1) / tot
+    frms = np.zeros(len(clusters))
+    fdot = np.zeros(len(clusters))
+    for k in range(len(clusters)):
+        Fk = F[4 * start[k]:4 * start[k + 1]]
+        u = np.random.default_rng(1_000_003 + k).standard_normal(Fk.shape)
+        frms[k] = np.sqrt(np.mean(Fk ** 2))
+        fdot[k] = np.sum(Fk * u)
+        scale = np.sqrt(np.sum(Fk ** 2)) * np.sqrt(np.sum(u ** 2))
+        fdot[k] = abs(fdot[k] - ref['force_dot'][k]) / scale
+    err_rms = np.abs(frms - ref['force_rms']) / ref['force_rms']
+    worst = int(np.argmax(err_e))
+    print('2048 clusters: worst energy error %.2e (cluster %d, %d waters), worst force rms error %.2e, '
+          'worst projection error %.2e' % (err_e.max(), worst, len(clusters[worst]), err_rms.max(), fdot.max()),
+          flush=True)
+    assert err_e.max() <= 1e-6
+    assert err_rms.max() <= 1e-4
+    assert fdot.max() <= 1e-4

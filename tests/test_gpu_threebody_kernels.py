@@ -1,7 +1,7 @@
 """The 3-body kernels against each other and against the oracle on the
 256-water box: the default kernel (the polynomial as 24 passes over the
-trimer's relabelling group, csrc/pip_orbit_loop.h), its three-waves-per-SIMD
-build, and the term-by-term kernel (MBPOL_HIP_3B_KERNEL=terms), all FP64.
+trimer's relabelling group, csrc/pip_orbit_loop.h) and the term-by-term
+kernel (MBPOL_HIP_3B_KERNEL=terms), both FP64.
 They differ in summation order only (~1e-15 per trimer), but every trimer's
 energy and forces are rounded to 2^-32 fixed point before they are summed, so
 a rounding that flips moves the totals by 2^-32 kJ/mol per trimer: the bounds
@@ -50,7 +50,7 @@ def test_threebody_kernels_agree_and_match_oracle(mb, oracle):
     print('orbit: 3-body %.12f vs oracle %.12f, force rel rms %.2e' % (e0, E['threebody'], err), flush=True)
     assert abs(e0 - E['threebody']) <= 1e-9 * abs(E['threebody'])
     assert err <= 1e-8
-    for kernel in ('orbit3', 'terms'):
+    for kernel in ('terms',):
         _, _, e1, f1 = threebody(mb, kernel)
         err = np.sqrt(np.mean((f1 - f0) ** 2)) / np.sqrt(np.mean(f0 ** 2))
         print('%s: 3-body %.12f vs orbit %.12f, force rel rms %.2e' % (kernel, e1, e0, err), flush=True)
