@@ -208,6 +208,13 @@ int mbpol_context_set_list_mode(mbpol_context* ctx, int32_t full_chain_list);
  * induced dipoles of the last evaluate (MBPolReferenceElectrostaticsForce.cpp:1023-1086) */
 int mbpol_context_get_electrostatic_potential(mbpol_context* ctx, int32_t npoints, const double* points,
                                               double* potential);
+/* No counterpart in the reference: mbpol_context_get_electrostatic_potential without any allocation, for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight serving loops).  The points and
+ * results are staged through a fixed 2 MiB device array of the library (one per GPU, held until the process
+ * ends): at most 65536 points per call (MBPOL_ERR_INVALID_ARGUMENT beyond).  Calls that overlap on one GPU
+ * share that array and race. */
+int mbpol_context_get_electrostatic_potential_noalloc(mbpol_context* ctx, int32_t npoints, const double* points,
+                                                      double* potential);
 /* MBPolElectrostaticsForce::getSystemElectrostaticsMoments (mbpolKernels.h:120; Reference glue
  * MBPolReferenceKernels.cpp:285-303, sums MBPolReferenceElectrostaticsForce.cpp:923-1021): moments[13] = net
  * charge (e), dipole (3, Debye), traceless quadrupole (3x3 row-major, Buckingham) about the centre of mass of

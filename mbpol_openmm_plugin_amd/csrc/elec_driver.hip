@@ -9,6 +9,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -30,6 +31,35 @@ namespace mbpol {
         hipfftResult _r = (expr);                                                            \
         if (_r != HIPFFT_SUCCESS) return set_error(MBPOL_ERR_HIP, "%s: hipFFT error %d", #expr, (int)_r); \
     } while (0)
+
+// ---------------------------------------------------------------------------
+// Potential at points without allocation (mbpol_context_get_electrostatic_potential_noalloc):
+// the points and the results are staged through `scratch`, a fixed device array
+// of this library (2 MiB; every GPU holds its own copy until the process ends).
+// Its device address is looked up once per device, with that device current,
+// when the first context with electrostatics is created on it.  Calls that
+// overlap on one GPU share the array and race.
+// ---------------------------------------------------------------------------
+namespace {
+constexpr int kScratchPoints = 1 << 16;                          // at most this many points per call
+static __device__ __align__(256) double scratch[4 * kScratchPoints];  // 3 coordinates per point, then the potentials
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+// the calling thread's current device must be `device`
+static int scratch_address(int device, double** out) {
+    if (device < 0 || device >= kMaxDevices) return set_error(MBPOL_ERR_INVALID_ARGUMENT, "device %d out of range", device);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void* p = nullptr;
+        HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_scratch_ptr[device] = static_cast<double*>(p);
+    }
+    *out = g_scratch_ptr[device];
+    return MBPOL_OK;
+}
 
 struct Elec {
     ElecParams P{};
@@ -330,6 +360,11 @@ int elec_create(mbpol_context* ctx) {
     const mbpol_system& s = ctx->sys;
     Elec* E = new Elec();
     ctx->elec = E;
+    // device address of the no-allocation entry point's scratch array, once per device
+    {
+        double* sc = nullptr;
+        if (int e = scratch_address(ctx->device, &sc)) return e;
+    }
     E->n = s.ne;
     E->nw = s.el_redistribute ? s.ne / 4 : 0;
     // the FP64 mode runs the reference's SOR iteration, the faster modes conjugate gradients
@@ -1050,6 +1085,33 @@ int mbpol_context_get_electrostatic_potential(mbpol_context* ctx, int32_t npoint
                                E->mu_cur, E->P, out.as<double>());
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     HIP_TRY(hipMemcpy(potential, out.p, sizeof(double) * npoints, hipMemcpyDeviceToHost));
+    return MBPOL_OK;
+}
+
+int mbpol_context_get_electrostatic_potential_noalloc(mbpol_context* ctx, int32_t npoints, const double* points,
+                                                      double* potential) {
+    using namespace mbpol;
+    if (!ctx || (npoints > 0 && (!points || !potential))) return set_error(MBPOL_ERR_INVALID_ARGUMENT, "null argument");
+    if (!ctx->elec) return set_error(MBPOL_ERR_INVALID_ARGUMENT, "no electrostatics in this system");
+    if (npoints < 0 || npoints > kScratchPoints)
+        return set_error(MBPOL_ERR_INVALID_ARGUMENT, "potential without allocation: at most %d points per call (%d asked)",
+                         kScratchPoints, npoints);
+    if (npoints == 0) return MBPOL_OK;
+    HIP_TRY(hipSetDevice(ctx->device));
+    double* sc = nullptr;
+    {
+        std::lock_guard<std::mutex> lock(g_scratch_mu);
+        sc = (ctx->device >= 0 && ctx->device < kMaxDevices) ? g_scratch_ptr[ctx->device] : nullptr;
+    }
+    if (!sc) return set_error(MBPOL_ERR_INVALID_ARGUMENT, "scratch not set up on device %d", ctx->device);
+    double* pts = sc;
+    double* out = sc + 3 * (size_t)kScratchPoints;
+    Elec* E = ctx->elec;
+    HIP_TRY(hipMemcpyAsync(pts, points, sizeof(double) * 3 * npoints, hipMemcpyHostToDevice, ctx->stream));
+    launch_potential_at_points(ctx->stream, npoints, pts, E->n, ctx->pos.as<double>(), E->q.as<double>(), E->mu_cur,
+                               E->P, out);
+    HIP_TRY(hipMemcpyAsync(potential, out, sizeof(double) * npoints, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
     return MBPOL_OK;
 }
 
