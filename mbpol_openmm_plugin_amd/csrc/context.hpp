@@ -3,12 +3,9 @@
 // Reference kernels keep between calls, MBPolReferenceKernels.h:45-244).
 #pragma once
 #include <hip/hip_runtime.h>
-#include <condition_variable>
 #include <cstdint>
-#include <functional>
 #include <mutex>
 #include <string>
-#include <thread>
 #include <vector>
 
 #include "kernels.h"
@@ -94,23 +91,6 @@ struct EvalStats {
     long long dimers = 0, trimers = 0, elec_pairs = 0, disp_pairs = 0, pol_pairs = 0;
 };
 
-// One host thread that runs one posted job at a time; wait() returns the
-// job's status and, on failure, its error message.
-class HostWorker {
-public:
-    ~HostWorker();
-    void post(std::function<int()> job);
-    int wait(std::string& msg);
-
-private:
-    std::thread th_;
-    std::mutex m_;
-    std::condition_variable cv_;
-    std::function<int()> job_;
-    bool has_job_ = false, done_ = true, quit_ = false;
-    int rc_ = 0;
-    std::string msg_;
-};
 
 }  // namespace mbpol
 
@@ -124,7 +104,6 @@ struct mbpol_context {
     // second stream for the short-range terms, forked from and joined to `stream` by events
     hipStream_t side = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
-    mbpol::HostWorker worker;  // enqueues the short-range terms on `side`
     bool evaluated = false;    // an evaluation has completed (launch shapes are tuned)
     int pip_blocks = 0;        // grid cap of the 2-body and 3-body kernels (0: one block per 64 units)
     int disp_blocks = 0;       // grid cap of the dispersion kernel (0: one wave per atom)
@@ -159,6 +138,8 @@ struct mbpol_context {
     int lower_cap = 0;
     mbpol::DevBuf trip_count, trip_off, trips;
     long long trip_cap = 0;
+    int lists_built = 0;        // lists built by this evaluation: 1 dimers, 2 trimers
+    mbpol::DevBuf list_len;     // the lower list's length widened to 64 bits, for the read-back
 
     // electrostatics and dispersion live in their own structures
     mbpol::Elec* elec = nullptr;

@@ -126,8 +126,8 @@ int disp_evaluate(mbpol_context* ctx, hipStream_t st, u64* energy) {
     const double* pos = ctx->pos.as<double>();
     if (D->g.na == 0) return MBPOL_OK;
     if (D->g.use_cells) {
-        HIP_TRY(hipMemsetAsync(D->cell_count.p, 0, sizeof(int) * (D->ncell + 1), st));
-        HIP_TRY(hipMemsetAsync(D->cell_fill.p, 0, sizeof(int) * (D->ncell + 1), st));
+        launch_fill_zero(st, D->cell_count.p, sizeof(int) * (D->ncell + 1));
+        launch_fill_zero(st, D->cell_fill.p, sizeof(int) * (D->ncell + 1));
         launch_disp_cells(st, D->g.na, D->act.as<int>(), pos, D->g.L, D->g.nc, D->cell_of.as<int>(), D->cell_count.as<int>());
         if (int e = scan_i32(ctx, D->cell_count.as<int>(), D->cell_start.as<int>(), D->ncell + 1, st)) return e;
         launch_disp_cell_fill(st, D->g.na, D->cell_of.as<int>(), D->cell_start.as<int>(), D->cell_fill.as<int>(),

@@ -44,7 +44,7 @@ struct ListGeom {
 // per-group sums of unit energies into out[g * 8 + comp] (fixed point): group g covers the units
 // [off[start[g]], off[start[g+1]]) when off is given, else [mult * start[g], mult * start[g+1])
 void launch_group_sums(hipStream_t s, int ngroups, const int* start, int mult, const long long* off,
-                       const double* eunit, u64* out, int comp);
+                       const double* eunit, u64* out, int comp, long long limit = 0x7fffffffffffffffLL);
 // eunit (optional): energy of every unit (water, dimer, trimer) in kJ/mol, for per-group sums
 void launch_place_vsites(hipStream_t s, int n, const int* vs_idx, const double* vs_w, double* pos);
 // per-water terms: the 1-body energy/forces of waters ob[3w..3w+2] (into force/energy, eunit optional)
@@ -52,7 +52,16 @@ void launch_place_vsites(hipStream_t s, int n, const int* vs_idx, const double* 
 void launch_water_terms(hipStream_t s, int nw, bool onebody, bool charges, const int* ob, const double* pos,
                         int periodic, d3 box, u64* force, u64* energy, double* eunit, double* q, double* dq);
 void launch_forces_to_double(hipStream_t s, int n3, const u64* fx, double* f);
+// zero `bytes` (a multiple of 4) with a kernel (no memset node in a captured graph)
+void launch_fill_zero(hipStream_t s, void* p, size_t bytes);
+// *out = (long long)*in
+void launch_widen_i32(hipStream_t s, const int* in, long long* out);
 void launch_fold_vsites(hipStream_t s, int n, const int* vs_idx, const double* vs_w, double* f);
+
+// ---- kernels_scan.hip: exclusive prefix sums (scratch of scan_scratch_bytes(n)) ----
+size_t scan_scratch_bytes(long long n);
+void launch_scan_i32(hipStream_t s, const int* in, int* out, long long n, void* scratch);
+void launch_scan_i64(hipStream_t s, const long long* in, long long* out, long long n, void* scratch);
 
 // ---- kernels_lists.hip ----
 void launch_cell_count(hipStream_t s, int n, const int* idx, const double* pos, d3 L, int3 nc, int* cell_of,
@@ -61,30 +70,28 @@ void launch_cell_fill(hipStream_t s, int n, const int* cell_of, const int* cell_
                       int* cell_members);
 void launch_dimer_pass(hipStream_t s, int n, const int* idx, const double* pos, ListGeom g, const int* cell_of,
                        const int* cell_start, const int* cell_members, long long* count, const long long* offset,
-                       int2* pairs);
+                       int2* pairs, long long cap);
 void launch_lower_pass(hipStream_t s, int n, const int* idx, const double* pos, ListGeom g, const int* cell_of,
-                       const int* cell_start, const int* cell_members, int* count, const int* offset, int* lower);
+                       const int* cell_start, const int* cell_members, int* count, const int* offset, int* lower,
+                       int cap);
 void launch_trimer_pass(hipStream_t s, int n, const int* idx, const double* pos, ListGeom g, const int* lo_start,
-                        const int* lower, int full_list, long long* count, const long long* offset, int* trip);
+                        const int* lower, int full_list, long long* count, const long long* offset, int* trip,
+                        int lower_cap, long long cap);
 
 // ---- kernels_pip.hip (one object per body order and precision) ----
-// max_blocks > 0 caps the grid (64-lane blocks); the kernels then stride over the list
-void launch_twobody_f32(hipStream_t s, long long npairs, const int2* pairs, const int* idx, const double* pos,
-                        int periodic, d3 box, u64* force, u64* energy, const KEvents& ev = KEvents(), double* eunit = nullptr,
-                        int max_blocks = 0);
-void launch_twobody_f64(hipStream_t s, long long npairs, const int2* pairs, const int* idx, const double* pos,
-                        int periodic, d3 box, u64* force, u64* energy, const KEvents& ev = KEvents(), double* eunit = nullptr,
-                        int max_blocks = 0);
+// n: the list's capacity (grid size); ndev (optional): its length on the device (the kernels
+// stop there); max_blocks > 0 caps the grid (64-lane blocks), the kernels then stride over the list
+#define MBPOL_PIP_LAUNCHER(name, T)                                                                             \
+    void name(hipStream_t s, long long n, const long long* ndev, const T* list, const int* idx, const double* pos, \
+              int periodic, d3 box, u64* force, u64* energy, const KEvents& ev = KEvents(), double* eunit = nullptr, \
+              int max_blocks = 0)
+MBPOL_PIP_LAUNCHER(launch_twobody_f32, int2);
+MBPOL_PIP_LAUNCHER(launch_twobody_f64, int2);
 // FP64, term by term (the coefficients as instruction literals)
-void launch_threebody_f64(hipStream_t s, long long ntrip, const int* trip, const int* idx, const double* pos,
-                          int periodic, d3 box, u64* force, u64* energy, const KEvents& ev = KEvents(),
-                          double* eunit = nullptr, int max_blocks = 0);
+MBPOL_PIP_LAUNCHER(launch_threebody_f64, int);
 // the polynomial through its relabelling group (pip_orbit_loop.h), FP32 and FP64
-void launch_threebody_orbit_f32(hipStream_t s, long long ntrip, const int* trip, const int* idx, const double* pos,
-                                int periodic, d3 box, u64* force, u64* energy, const KEvents& ev = KEvents(),
-                                double* eunit = nullptr, int max_blocks = 0);
-void launch_threebody_orbit_f64(hipStream_t s, long long ntrip, const int* trip, const int* idx, const double* pos,
-                                int periodic, d3 box, u64* force, u64* energy, const KEvents& ev = KEvents(),
-                                double* eunit = nullptr, int max_blocks = 0);
+MBPOL_PIP_LAUNCHER(launch_threebody_orbit_f32, int);
+MBPOL_PIP_LAUNCHER(launch_threebody_orbit_f64, int);
+#undef MBPOL_PIP_LAUNCHER
 
 }  // namespace mbpol

@@ -12,7 +12,10 @@
 //    voxel index nv is invisible to later molecules there (:86-97, :140-146),
 //    which is reproduced.
 // Lists are built in two passes (count, exclusive scan, fill); the fill pass
-// recomputes the same predicate, so it writes exactly `count` entries.
+// recomputes the same predicate, so it writes exactly `count` entries -- those
+// below the list's capacity: the host learns the totals after the evaluation
+// and runs it again with larger lists when one did not fit, so that no host
+// wait sits between the passes.
 #include "device_common.h"
 #include "kernels.h"
 
@@ -129,7 +132,8 @@ __global__ void __launch_bounds__(256) k_dimer_pass(int n, const int* __restrict
                                                     ListGeom g, const int* __restrict__ cell_of,
                                                     const int* __restrict__ cell_start,
                                                     const int* __restrict__ cell_members, long long* __restrict__ count,
-                                                    const long long* __restrict__ offset, int2* __restrict__ pairs) {
+                                                    const long long* __restrict__ offset, int2* __restrict__ pairs,
+                                                    long long cap) {
     const int i = (blockIdx.x * blockDim.x + threadIdx.x) / kGL, lane = threadIdx.x % kGL;
     if (i >= n) return;  // uniform over the group
     const d3 pi = opos(pos, idx, i);
@@ -140,7 +144,10 @@ __global__ void __launch_bounds__(256) k_dimer_pass(int n, const int* __restrict
     for_candidates<kGL>(i, lane, n, g, cell_of, cell_start, cell_members, [&](int j, bool act) {
         const bool ok = act && j > i && dist2(pi, opos(pos, idx, j), g.L, g.periodic) <= rc2;
         const unsigned long long bits = __ballot(ok) & gb;
-        if (ok && pairs) pairs[o + c + __popcll(bits & below)] = make_int2(i, j);
+        if (ok && pairs) {
+            const long long t = o + c + __popcll(bits & below);
+            if (t < cap) pairs[t] = make_int2(i, j);  // beyond the capacity: counted, reported, not written
+        }
         c += __popcll(bits);
     });
     if (count && lane == 0) count[i] = c;
@@ -166,7 +173,7 @@ __global__ void __launch_bounds__(256) k_lower_pass(int n, const int* __restrict
                                                     ListGeom g, const int* __restrict__ cell_of,
                                                     const int* __restrict__ cell_start,
                                                     const int* __restrict__ cell_members, int* __restrict__ count,
-                                                    const int* __restrict__ offset, int* __restrict__ lower) {
+                                                    const int* __restrict__ offset, int* __restrict__ lower, int cap) {
     const int i = (blockIdx.x * blockDim.x + threadIdx.x) / kGL, lane = threadIdx.x % kGL;
     if (i >= n) return;
     const d3 pi = opos(pos, idx, i);
@@ -181,7 +188,10 @@ __global__ void __launch_bounds__(256) k_lower_pass(int n, const int* __restrict
             ok = dist2(pi, pj, g.L, g.periodic) <= rc2 && !phantom_voxel(pj, g);
         }
         const unsigned long long bits = __ballot(ok) & gb;
-        if (ok && lower) lower[o + c + __popcll(bits & below)] = j;
+        if (ok && lower) {
+            const int t = o + c + __popcll(bits & below);
+            if (t < cap) lower[t] = j;
+        }
         c += __popcll(bits);
     });
     if (count && lane == 0) count[i] = c;
@@ -223,9 +233,15 @@ __global__ void __launch_bounds__(256) k_trimer_pass(int n, const int* __restric
                                                      ListGeom g, const int* __restrict__ lo_start,
                                                      const int* __restrict__ lower, int full_list,
                                                      long long* __restrict__ count, const long long* __restrict__ offset,
-                                                     int* __restrict__ trip) {
+                                                     int* __restrict__ trip, int lower_cap, long long cap) {
     const int i = (blockIdx.x * blockDim.x + threadIdx.x) / kGT, lane = threadIdx.x % kGT;
     if (i >= n) return;  // uniform over the group
+    // an incomplete lower list (it overflowed its capacity): nothing to read; the
+    // evaluation is run again with a larger one
+    if (lo_start[n] > lower_cap) {
+        if (count && lane == 0) count[i] = 0;
+        return;
+    }
     const d3 pi = opos(pos, idx, i);
     const unsigned long long gb = group_bits<kGT>(), below = lanes_below();
     long long c = 0;
@@ -241,9 +257,11 @@ __global__ void __launch_bounds__(256) k_trimer_pass(int n, const int* __restric
             const unsigned long long bits = __ballot(ok) & gb;
             if (ok && trip) {
                 const long long t = o + c + __popcll(bits & below);
-                trip[3 * t] = i;
-                trip[3 * t + 1] = j;
-                trip[3 * t + 2] = k;
+                if (t < cap) {
+                    trip[3 * t] = i;
+                    trip[3 * t + 1] = j;
+                    trip[3 * t + 2] = k;
+                }
             }
             c += __popcll(bits);
         }
@@ -265,21 +283,23 @@ void launch_cell_fill(hipStream_t s, int n, const int* cell_of, const int* cell_
 }
 void launch_dimer_pass(hipStream_t s, int n, const int* idx, const double* pos, ListGeom g, const int* cell_of,
                        const int* cell_start, const int* cell_members, long long* count, const long long* offset,
-                       int2* pairs) {
+                       int2* pairs, long long cap) {
     if (n > 0)
         hipLaunchKernelGGL(k_dimer_pass, nblk((long long)n * kGL, 256), 256, 0, s, n, idx, pos, g, cell_of, cell_start,
-                           cell_members, count, offset, pairs);
+                           cell_members, count, offset, pairs, cap);
 }
 void launch_lower_pass(hipStream_t s, int n, const int* idx, const double* pos, ListGeom g, const int* cell_of,
-                       const int* cell_start, const int* cell_members, int* count, const int* offset, int* lower) {
+                       const int* cell_start, const int* cell_members, int* count, const int* offset, int* lower,
+                       int cap) {
     if (n > 0)
         hipLaunchKernelGGL(k_lower_pass, nblk((long long)n * kGL, 256), 256, 0, s, n, idx, pos, g, cell_of, cell_start,
-                           cell_members, count, offset, lower);
+                           cell_members, count, offset, lower, cap);
 }
 void launch_trimer_pass(hipStream_t s, int n, const int* idx, const double* pos, ListGeom g, const int* lo_start,
-                        const int* lower, int full_list, long long* count, const long long* offset, int* trip) {
+                        const int* lower, int full_list, long long* count, const long long* offset, int* trip,
+                        int lower_cap, long long cap) {
     if (n > 0)
         hipLaunchKernelGGL(k_trimer_pass, nblk((long long)n * kGT, 256), 256, 0, s, n, idx, pos, g, lo_start, lower,
-                           full_list, count, offset, trip);
+                           full_list, count, offset, trip, lower_cap, cap);
 }
 }  // namespace mbpol

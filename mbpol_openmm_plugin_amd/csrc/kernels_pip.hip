@@ -158,10 +158,13 @@ __device__ __forceinline__ bool dimer_geometry(long long t, const int2* __restri
     return true;
 }
 
-__global__ void __launch_bounds__(64) k_twobody(long long np, const int2* __restrict__ pairs, const int* __restrict__ idx,
+__global__ void __launch_bounds__(64) k_twobody(long long np, const long long* __restrict__ ndev,
+                                                const int2* __restrict__ pairs, const int* __restrict__ idx,
                                                 const double* __restrict__ pos, int periodic, d3 boxA,
                                                 u64* __restrict__ force, u64* __restrict__ energy,
                                                 double* __restrict__ eunit) {
+    // np: the list's capacity; ndev (when given): its length, known on the device only
+    if (ndev) np = min(np, *ndev);
     // a capped grid strides over the list in whole waves
     for (long long t0 = (long long)blockIdx.x * blockDim.x; t0 < np; t0 += (long long)gridDim.x * blockDim.x) {
         const long long t = t0 + threadIdx.x;
@@ -361,10 +364,14 @@ __device__ __forceinline__ bool trimer_geometry(long long t, const int* __restri
     return true;
 }
 
-__global__ void __launch_bounds__(64) PIP3_OCCUPANCY k_threebody(long long nt, const int* __restrict__ trip, const int* __restrict__ idx,
-                                                  const double* __restrict__ pos, int periodic, d3 box,
-                                                  u64* __restrict__ force, u64* __restrict__ energy,
-                                                  double* __restrict__ eunit) {
+__global__ void __launch_bounds__(64) PIP3_OCCUPANCY k_threebody(long long nt, const long long* __restrict__ ndev,
+                                                                 const int* __restrict__ trip,
+                                                                 const int* __restrict__ idx,
+                                                                 const double* __restrict__ pos, int periodic, d3 box,
+                                                                 u64* __restrict__ force, u64* __restrict__ energy,
+                                                                 double* __restrict__ eunit) {
+    // nt: the list's capacity; ndev (when given): its length, known on the device only
+    if (ndev) nt = min(nt, *ndev);
     // a capped grid strides over the list in whole waves
     for (long long t0 = (long long)blockIdx.x * blockDim.x; t0 < nt; t0 += (long long)gridDim.x * blockDim.x) {
         const long long t = t0 + threadIdx.x;
@@ -474,21 +481,21 @@ static inline int grid(long long n, int max_blocks) {
     return (int)(max_blocks > 0 && b > max_blocks ? max_blocks : b);
 }
 #if PIP_BODY == 2
-void PIP_CAT(launch_twobody_, PIP_SUFFIX)(hipStream_t s, long long npairs, const int2* pairs, const int* idx,
-                                          const double* pos, int periodic, d3 box, u64* force, u64* energy,
-                                          const KEvents& ev, double* eunit, int max_blocks) {
+void PIP_CAT(launch_twobody_, PIP_SUFFIX)(hipStream_t s, long long npairs, const long long* ndev, const int2* pairs,
+                                          const int* idx, const double* pos, int periodic, d3 box, u64* force,
+                                          u64* energy, const KEvents& ev, double* eunit, int max_blocks) {
     const d3 boxA{box.x * 10., box.y * 10., box.z * 10.};
     if (npairs > 0)
-        MBPOL_LAUNCH(ev, k_twobody, grid(npairs, max_blocks), 64, 0, s, npairs, pairs, idx, pos, periodic, boxA, force, energy,
-                     eunit);
+        MBPOL_LAUNCH(ev, k_twobody, grid(npairs, max_blocks), 64, 0, s, npairs, ndev, pairs, idx, pos, periodic, boxA,
+                     force, energy, eunit);
 }
 #else
-void PIP_CAT(launch_threebody_, PIP_SUFFIX)(hipStream_t s, long long ntrip, const int* trip, const int* idx,
-                                            const double* pos, int periodic, d3 box, u64* force, u64* energy,
-                                            const KEvents& ev, double* eunit, int max_blocks) {
+void PIP_CAT(launch_threebody_, PIP_SUFFIX)(hipStream_t s, long long ntrip, const long long* ndev, const int* trip,
+                                            const int* idx, const double* pos, int periodic, d3 box, u64* force,
+                                            u64* energy, const KEvents& ev, double* eunit, int max_blocks) {
     if (ntrip > 0)
-        MBPOL_LAUNCH(ev, k_threebody, grid(ntrip, max_blocks), 64, 0, s, ntrip, trip, idx, pos, periodic, box, force, energy,
-                     eunit);
+        MBPOL_LAUNCH(ev, k_threebody, grid(ntrip, max_blocks), 64, 0, s, ntrip, ndev, trip, idx, pos, periodic, box,
+                     force, energy, eunit);
 }
 #endif
 }  // namespace mbpol

@@ -3,6 +3,8 @@
 // water_terms.h), and the fixed-point -> double force/energy finalisation.
 // All FP64 (one lane per water; a few thousand flops per water, negligible
 // next to the PIPs).
+#include <algorithm>
+
 #include "device_common.h"
 #include "kernels.h"
 #include "water_terms.h"
@@ -76,6 +78,17 @@ __global__ void k_water_terms(int nw, const int* __restrict__ ob, const double* 
     if (ONE) add_energy(energy, e_out);
 }
 
+// Zero `bytes` (a multiple of 4) of device memory with plain stores: a kernel,
+// so that it may be captured in a HIP graph.
+__global__ void k_fill_zero(unsigned int* __restrict__ p, long long nwords) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < nwords;
+         i += (long long)gridDim.x * blockDim.x)
+        p[i] = 0u;
+}
+
+// one int widened to 64 bits (for a read-back next to 64-bit lengths)
+__global__ void k_widen_i32(const int* __restrict__ in, long long* __restrict__ out) { *out = *in; }
+
 // fixed point -> double for every particle
 __global__ void k_forces_to_double(int n3, const unsigned long long* __restrict__ fx, double* __restrict__ f) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -105,11 +118,13 @@ void launch_place_vsites(hipStream_t s, int n, const int* vs_idx, const double* 
     if (n > 0) hipLaunchKernelGGL(k_place_vsites, nblk(n, 256), 256, 0, s, n, vs_idx, vs_w, pos);
 }
 __global__ void k_group_sums(int ngroups, const int* __restrict__ start, int mult, const long long* __restrict__ off,
-                             const double* __restrict__ eunit, u64* __restrict__ out, int comp) {
+                             const double* __restrict__ eunit, u64* __restrict__ out, int comp, long long limit) {
     __shared__ u64 red[256];
     const int g = blockIdx.x;
-    const long long b0 = off ? off[start[g]] : (long long)mult * start[g];
-    const long long b1 = off ? off[start[g + 1]] : (long long)mult * start[g + 1];
+    // limit: units written (a list's capacity: offsets past it belong to a list that did not fit, and the
+    // evaluation is run again)
+    const long long b0 = min(limit, off ? off[start[g]] : (long long)mult * start[g]);
+    const long long b1 = min(limit, off ? off[start[g + 1]] : (long long)mult * start[g + 1]);
     u64 s = 0;
     for (long long u = b0 + threadIdx.x; u < b1; u += 256) s += to_fix(eunit[u]);
     red[threadIdx.x] = s;
@@ -122,8 +137,9 @@ __global__ void k_group_sums(int ngroups, const int* __restrict__ start, int mul
 }
 
 void launch_group_sums(hipStream_t s, int ngroups, const int* start, int mult, const long long* off,
-                       const double* eunit, u64* out, int comp) {
-    if (ngroups > 0) hipLaunchKernelGGL(k_group_sums, ngroups, 256, 0, s, ngroups, start, mult, off, eunit, out, comp);
+                       const double* eunit, u64* out, int comp, long long limit) {
+    if (ngroups > 0)
+        hipLaunchKernelGGL(k_group_sums, ngroups, 256, 0, s, ngroups, start, mult, off, eunit, out, comp, limit);
 }
 
 void launch_water_terms(hipStream_t s, int nw, bool onebody, bool charges, const int* ob, const double* pos,
@@ -139,6 +155,13 @@ void launch_water_terms(hipStream_t s, int nw, bool onebody, bool charges, const
         hipLaunchKernelGGL((k_water_terms<false, true>), nblk(nw, 64), 64, 0, s, nw, ob, pos, periodic, box, force,
                            energy, eunit, q, dq);
 }
+void launch_fill_zero(hipStream_t s, void* p, size_t bytes) {
+    const long long nw = (long long)(bytes / 4);
+    if (nw > 0)
+        hipLaunchKernelGGL(k_fill_zero, (int)std::min<long long>(nblk(nw, 256), 2048), 256, 0, s,
+                           static_cast<unsigned int*>(p), nw);
+}
+void launch_widen_i32(hipStream_t s, const int* in, long long* out) { hipLaunchKernelGGL(k_widen_i32, 1, 1, 0, s, in, out); }
 void launch_forces_to_double(hipStream_t s, int n3, const u64* fx, double* f) {
     if (n3 > 0) hipLaunchKernelGGL(k_forces_to_double, nblk(n3, 256), 256, 0, s, n3, fx, f);
 }

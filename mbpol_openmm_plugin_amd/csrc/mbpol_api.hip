@@ -1,8 +1,6 @@
 // C ABI of the MB-pol HIP platform (include/mbpol_hip.h): system description,
 // context set-up and the evaluation driver that sequences the kernels of one
 // energy+force evaluation on the context's stream.
-#include <hipcub/hipcub.hpp>
-
 #include <algorithm>
 #include <cmath>
 #include <cstdarg>
@@ -64,70 +62,25 @@ hipError_t DevBuf::ensure(size_t nbytes) {
 }
 
 // ---------------------------------------------------------------------------
-// scans (hipCUB)
+// scans (kernels_scan.hip); each stream has its own scratch, so that scans on
+// the two streams may overlap
 // ---------------------------------------------------------------------------
-// each stream has its own scratch, so that scans on the two streams may overlap
-template <typename In, typename Out>
-static int exclusive_scan(mbpol_context* ctx, const In* in, Out* out, int n, hipStream_t s) {
-    if (!s) s = ctx->stream;
-    DevBuf& tmp = (s == ctx->stream) ? ctx->scan_tmp : ctx->scan_tmp_side;
-    size_t bytes = 0;
-    HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, in, out, n, s));
-    HIP_TRY(tmp.ensure(bytes));
-    HIP_TRY(hipcub::DeviceScan::ExclusiveSum(tmp.p, bytes, in, out, n, s));
-    return MBPOL_OK;
+static DevBuf& scan_scratch(mbpol_context* ctx, hipStream_t s) {
+    return (s == ctx->stream) ? ctx->scan_tmp : ctx->scan_tmp_side;
 }
 int mbpol::scan_i32(mbpol_context* ctx, const int* in, int* out, int n, hipStream_t s) {
-    return exclusive_scan(ctx, in, out, n, s);
+    if (!s) s = ctx->stream;
+    DevBuf& tmp = scan_scratch(ctx, s);
+    HIP_TRY(tmp.ensure(scan_scratch_bytes(n)));
+    launch_scan_i32(s, in, out, n, tmp.p);
+    return MBPOL_OK;
 }
 int mbpol::scan_i64(mbpol_context* ctx, const long long* in, long long* out, int n, hipStream_t s) {
-    return exclusive_scan(ctx, in, out, n, s);
-}
-
-// ---------------------------------------------------------------------------
-// host worker thread
-// ---------------------------------------------------------------------------
-mbpol::HostWorker::~HostWorker() {
-    if (!th_.joinable()) return;
-    {
-        std::lock_guard<std::mutex> g(m_);
-        quit_ = true;
-    }
-    cv_.notify_all();
-    th_.join();
-}
-
-void mbpol::HostWorker::post(std::function<int()> job) {
-    std::unique_lock<std::mutex> g(m_);
-    if (!th_.joinable())
-        th_ = std::thread([this] {
-            std::unique_lock<std::mutex> l(m_);
-            for (;;) {
-                cv_.wait(l, [this] { return quit_ || has_job_; });
-                if (quit_) return;
-                std::function<int()> f = std::move(job_);
-                has_job_ = false;
-                l.unlock();
-                const int rc = f();
-                std::string msg = rc ? std::string(mbpol_last_error()) : std::string();
-                l.lock();
-                rc_ = rc;
-                msg_ = std::move(msg);
-                done_ = true;
-                cv_.notify_all();
-            }
-        });
-    job_ = std::move(job);
-    has_job_ = true;
-    done_ = false;
-    cv_.notify_all();
-}
-
-int mbpol::HostWorker::wait(std::string& msg) {
-    std::unique_lock<std::mutex> g(m_);
-    cv_.wait(g, [this] { return done_; });
-    msg = msg_;
-    return rc_;
+    if (!s) s = ctx->stream;
+    DevBuf& tmp = scan_scratch(ctx, s);
+    HIP_TRY(tmp.ensure(scan_scratch_bytes(n)));
+    launch_scan_i64(s, in, out, n, tmp.p);
+    return MBPOL_OK;
 }
 
 // ---------------------------------------------------------------------------
@@ -461,8 +414,8 @@ static void apply_list_groups(mbpol_context* ctx) {
 
 static int bin_cells(mbpol_context* ctx, hipStream_t st, CellGrid& cg, int n, const int* idx) {
     if (!cg.g.use_cells) return MBPOL_OK;
-    HIP_TRY(hipMemsetAsync(cg.cell_count.p, 0, sizeof(int) * (cg.ncell + 1), st));
-    HIP_TRY(hipMemsetAsync(cg.cell_fill.p, 0, sizeof(int) * (cg.ncell + 1), st));
+    launch_fill_zero(st, cg.cell_count.p, sizeof(int) * (cg.ncell + 1));
+    launch_fill_zero(st, cg.cell_fill.p, sizeof(int) * (cg.ncell + 1));
     launch_cell_count(st, n, idx, ctx->pos.as<double>(), cg.g.L, cg.g.nc, cg.cell_of.as<int>(),
                       cg.cell_count.as<int>());
     if (int e = scan_i32(ctx, cg.cell_count.as<int>(), cg.cell_start.as<int>(), cg.ncell + 1, st)) return e;
@@ -563,7 +516,8 @@ int mbpol_context_create(const mbpol_system* sys, int32_t precision, int32_t dev
     if (n3 > 0) {
         if (int e = setup_grid(ctx, ctx->grid3, n3, sys->th_method == MBPOL_CUTOFF_PERIODIC, sys->th_cutoff)) return fail(e);
         if (ctx->lo_count.ensure(sizeof(int) * (n3 + 1)) || ctx->lo_start.ensure(sizeof(int) * (n3 + 1)) ||
-            ctx->trip_count.ensure(sizeof(long long) * (n3 + 1)) || ctx->trip_off.ensure(sizeof(long long) * (n3 + 1)))
+            ctx->trip_count.ensure(sizeof(long long) * (n3 + 1)) || ctx->trip_off.ensure(sizeof(long long) * (n3 + 1)) ||
+            ctx->list_len.ensure(sizeof(long long)))
             return fail(set_error(MBPOL_ERR_HIP, "device allocation failed"));
     }
     apply_list_groups(ctx);
@@ -675,67 +629,123 @@ int mbpol_context_set_list_mode(mbpol_context* ctx, int32_t full_chain_list) {
 
 }  // extern "C"
 
-// Build the dimer list and the trimer work list; returns their sizes after one sync.
+// Build the dimer list and the trimer work list.  The list lengths stay on the
+// device (dimer_off[n2], lo_start[n3], trip_off[n3]): the fill passes write
+// what fits in the current capacities, the PIP kernels read the lengths, and
+// the host compares lengths and capacities after the evaluation (list_check),
+// growing the lists and running the evaluation again when one did not fit.
+// Before the first evaluation (no capacities yet) the lengths are read back in
+// between the passes, which sizes the lists.
 static int build_lists(mbpol_context* ctx, hipStream_t st, bool want2, bool want3) {
     const mbpol_system& s = ctx->sys;
     const int n2 = (int)s.tb_idx.size() / 3, n3 = (int)s.th_idx.size() / 3;
     const double* pos = ctx->pos.as<double>();
     long long* hc = static_cast<long long*>(ctx->h_counts);
-    if (want2) {
-        CellGrid& g = ctx->grid2;
-        if (int e = bin_cells(ctx, st, g, n2, ctx->tb_idx.as<int>())) return e;
-        HIP_TRY(hipMemsetAsync(ctx->dimer_count.p, 0, sizeof(long long) * (n2 + 1), st));
-        launch_dimer_pass(st, n2, ctx->tb_idx.as<int>(), pos, g.g, g.cell_of.as<int>(), g.cell_start.as<int>(),
-                          g.members.as<int>(), ctx->dimer_count.as<long long>(), nullptr, nullptr);
-        if (int e = scan_i64(ctx, ctx->dimer_count.as<long long>(), ctx->dimer_off.as<long long>(), n2 + 1, st)) return e;
-        HIP_TRY(hipMemcpyAsync(&hc[0], ctx->dimer_off.as<long long>() + n2, sizeof(long long), hipMemcpyDeviceToHost,
-                               st));
-    }
-    if (want3) {
-        CellGrid& g = ctx->grid3;
-        if (int e = bin_cells(ctx, st, g, n3, ctx->th_idx.as<int>())) return e;
-        HIP_TRY(hipMemsetAsync(ctx->lo_count.p, 0, sizeof(int) * (n3 + 1), st));
-        launch_lower_pass(st, n3, ctx->th_idx.as<int>(), pos, g.g, g.cell_of.as<int>(), g.cell_start.as<int>(),
-                          g.members.as<int>(), ctx->lo_count.as<int>(), nullptr, nullptr);
-        if (int e = scan_i32(ctx, ctx->lo_count.as<int>(), ctx->lo_start.as<int>(), n3 + 1, st)) return e;
-        HIP_TRY(hipMemcpyAsync(&hc[1], ctx->lo_start.as<int>() + n3, sizeof(int), hipMemcpyDeviceToHost, st));
-    }
-    HIP_TRY(hipStreamSynchronize(st));
-    if (want2) {
-        const long long nd = hc[0];
+    auto grow2 = [&](long long nd) -> int {
         if (nd > ctx->dimer_cap) {
             HIP_TRY(ctx->dimers.ensure(sizeof(int2) * (nd + nd / 4 + 64)));
             ctx->dimer_cap = nd + nd / 4 + 64;
         }
+        return MBPOL_OK;
+    };
+    if (want2) {
         CellGrid& g = ctx->grid2;
+        if (int e = bin_cells(ctx, st, g, n2, ctx->tb_idx.as<int>())) return e;
+        launch_fill_zero(st, ctx->dimer_count.p, sizeof(long long) * (n2 + 1));
         launch_dimer_pass(st, n2, ctx->tb_idx.as<int>(), pos, g.g, g.cell_of.as<int>(), g.cell_start.as<int>(),
-                          g.members.as<int>(), nullptr, ctx->dimer_off.as<long long>(), ctx->dimers.as<int2>());
-        ctx->stats.dimers = nd;
+                          g.members.as<int>(), ctx->dimer_count.as<long long>(), nullptr, nullptr, 0);
+        if (int e = scan_i64(ctx, ctx->dimer_count.as<long long>(), ctx->dimer_off.as<long long>(), n2 + 1, st)) return e;
+        if (ctx->dimer_cap == 0) {  // first build: size the list
+            HIP_TRY(hipMemcpyAsync(&hc[0], ctx->dimer_off.as<long long>() + n2, sizeof(long long),
+                                   hipMemcpyDeviceToHost, st));
+            HIP_TRY(hipStreamSynchronize(st));
+            if (int e = grow2(hc[0])) return e;
+        }
+        launch_dimer_pass(st, n2, ctx->tb_idx.as<int>(), pos, g.g, g.cell_of.as<int>(), g.cell_start.as<int>(),
+                          g.members.as<int>(), nullptr, ctx->dimer_off.as<long long>(), ctx->dimers.as<int2>(),
+                          ctx->dimer_cap);
     }
     if (want3) {
-        const int nl = (int)(reinterpret_cast<int*>(&hc[1])[0]);
-        if (nl > ctx->lower_cap) {
+        CellGrid& g = ctx->grid3;
+        if (int e = bin_cells(ctx, st, g, n3, ctx->th_idx.as<int>())) return e;
+        launch_fill_zero(st, ctx->lo_count.p, sizeof(int) * (n3 + 1));
+        launch_lower_pass(st, n3, ctx->th_idx.as<int>(), pos, g.g, g.cell_of.as<int>(), g.cell_start.as<int>(),
+                          g.members.as<int>(), ctx->lo_count.as<int>(), nullptr, nullptr, 0);
+        if (int e = scan_i32(ctx, ctx->lo_count.as<int>(), ctx->lo_start.as<int>(), n3 + 1, st)) return e;
+        if (ctx->lower_cap == 0) {
+            HIP_TRY(hipMemcpyAsync(&hc[1], ctx->lo_start.as<int>() + n3, sizeof(int), hipMemcpyDeviceToHost, st));
+            HIP_TRY(hipStreamSynchronize(st));
+            const int nl = (int)(reinterpret_cast<int*>(&hc[1])[0]);
             HIP_TRY(ctx->lower.ensure(sizeof(int) * (nl + nl / 4 + 64)));
             ctx->lower_cap = nl + nl / 4 + 64;
         }
-        CellGrid& g = ctx->grid3;
         launch_lower_pass(st, n3, ctx->th_idx.as<int>(), pos, g.g, g.cell_of.as<int>(), g.cell_start.as<int>(),
-                          g.members.as<int>(), nullptr, ctx->lo_start.as<int>(), ctx->lower.as<int>());
-        HIP_TRY(hipMemsetAsync(ctx->trip_count.p, 0, sizeof(long long) * (n3 + 1), st));
+                          g.members.as<int>(), nullptr, ctx->lo_start.as<int>(), ctx->lower.as<int>(), ctx->lower_cap);
+        launch_fill_zero(st, ctx->trip_count.p, sizeof(long long) * (n3 + 1));
         launch_trimer_pass(st, n3, ctx->th_idx.as<int>(), pos, g.g, ctx->lo_start.as<int>(), ctx->lower.as<int>(),
-                           ctx->full_chain_list, ctx->trip_count.as<long long>(), nullptr, nullptr);
+                           ctx->full_chain_list, ctx->trip_count.as<long long>(), nullptr, nullptr, ctx->lower_cap, 0);
         if (int e = scan_i64(ctx, ctx->trip_count.as<long long>(), ctx->trip_off.as<long long>(), n3 + 1, st)) return e;
-        HIP_TRY(hipMemcpyAsync(&hc[2], ctx->trip_off.as<long long>() + n3, sizeof(long long), hipMemcpyDeviceToHost,
-                               st));
-        HIP_TRY(hipStreamSynchronize(st));
-        const long long nt = hc[2];
-        if (nt > ctx->trip_cap) {
+        if (ctx->trip_cap == 0) {
+            HIP_TRY(hipMemcpyAsync(&hc[2], ctx->trip_off.as<long long>() + n3, sizeof(long long), hipMemcpyDeviceToHost,
+                                   st));
+            HIP_TRY(hipStreamSynchronize(st));
+            const long long nt = hc[2];
             HIP_TRY(ctx->trips.ensure(sizeof(int) * 3 * (nt + nt / 4 + 64)));
             ctx->trip_cap = nt + nt / 4 + 64;
         }
         launch_trimer_pass(st, n3, ctx->th_idx.as<int>(), pos, g.g, ctx->lo_start.as<int>(), ctx->lower.as<int>(),
-                           ctx->full_chain_list, nullptr, ctx->trip_off.as<long long>(), ctx->trips.as<int>());
+                           ctx->full_chain_list, nullptr, ctx->trip_off.as<long long>(), ctx->trips.as<int>(),
+                           ctx->lower_cap, ctx->trip_cap);
+    }
+    return MBPOL_OK;
+}
+
+// After an evaluation: the list lengths (read back with the energies into
+// h_counts[16..18]) against the capacities; when one did not fit, the lists
+// grow and `redo` asks for the evaluation to be run again.
+static int list_check(mbpol_context* ctx, bool& redo) {
+    const mbpol_system& s = ctx->sys;
+    const long long* hc = static_cast<const long long*>(ctx->h_counts);
+    redo = false;
+    if (ctx->lists_built & 1) {
+        const long long nd = hc[16];
+        if (nd > ctx->dimer_cap) {
+            HIP_TRY(ctx->dimers.ensure(sizeof(int2) * (nd + nd / 4 + 64)));
+            ctx->dimer_cap = nd + nd / 4 + 64;
+            redo = true;
+        }
+        ctx->stats.dimers = nd;
+    }
+    if (ctx->lists_built & 2) {
+        const long long nl = hc[17], nt = hc[18];
+        if (nl > ctx->lower_cap) {
+            HIP_TRY(ctx->lower.ensure(sizeof(int) * (nl + nl / 4 + 64)));
+            ctx->lower_cap = (int)(nl + nl / 4 + 64);
+            redo = true;  // the trimer list could not be built: its length is unknown
+        } else if (nt > ctx->trip_cap) {
+            HIP_TRY(ctx->trips.ensure(sizeof(int) * 3 * (nt + nt / 4 + 64)));
+            ctx->trip_cap = nt + nt / 4 + 64;
+            redo = true;
+        }
         ctx->stats.trimers = nt;
+    }
+    (void)s;
+    return MBPOL_OK;
+}
+
+// device -> host copies of the three list lengths into h_counts[16..18]
+static int list_lengths_to_host(mbpol_context* ctx, hipStream_t st) {
+    const mbpol_system& s = ctx->sys;
+    const int n2 = (int)s.tb_idx.size() / 3, n3 = (int)s.th_idx.size() / 3;
+    long long* hc = static_cast<long long*>(ctx->h_counts);
+    if (ctx->lists_built & 1)
+        HIP_TRY(hipMemcpyAsync(&hc[16], ctx->dimer_off.as<long long>() + n2, sizeof(long long), hipMemcpyDeviceToHost,
+                               st));
+    if (ctx->lists_built & 2) {
+        launch_widen_i32(st, ctx->lo_start.as<int>() + n3, ctx->list_len.as<long long>());
+        HIP_TRY(hipMemcpyAsync(&hc[17], ctx->list_len.p, sizeof(long long), hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipMemcpyAsync(&hc[18], ctx->trip_off.as<long long>() + n3, sizeof(long long), hipMemcpyDeviceToHost,
+                               st));
     }
     return MBPOL_OK;
 }
@@ -766,8 +776,9 @@ static int short_range(mbpol_context* ctx, hipStream_t sr, uint32_t comp) {
         if (int e = build_lists(ctx, sr, do2, do3)) return e;
         timer_stop(ctx, T_LISTS);
     }
-    if (G && do2) HIP_TRY(ctx->g_eu2.ensure(sizeof(double) * std::max<long long>(ctx->stats.dimers, 1)));
-    if (G && do3) HIP_TRY(ctx->g_eu3.ensure(sizeof(double) * std::max<long long>(ctx->stats.trimers, 1)));
+    ctx->lists_built = (do2 ? 1 : 0) | (do3 ? 2 : 0);
+    if (G && do2) HIP_TRY(ctx->g_eu2.ensure(sizeof(double) * std::max<long long>(ctx->dimer_cap, 1)));
+    if (G && do3) HIP_TRY(ctx->g_eu3.ensure(sizeof(double) * std::max<long long>(ctx->trip_cap, 1)));
     double* eu2 = G ? ctx->g_eu2.as<double>() : nullptr;
     double* eu3 = G ? ctx->g_eu3.as<double>() : nullptr;
     // The polynomials run in FP64 unless FP32 was asked for explicitly: their
@@ -777,31 +788,37 @@ static int short_range(mbpol_context* ctx, hipStream_t sr, uint32_t comp) {
     if (do2) {
         const int per = s.tb_method == MBPOL_CUTOFF_PERIODIC;
         const KEvents ev = timer_kernel(ctx, T_TWOBODY);
+        const long long* nd = ctx->dimer_off.as<long long>() + (int)s.tb_idx.size() / 3;  // list length (device)
         if (dp)
-            launch_twobody_f64(sr, ctx->stats.dimers, ctx->dimers.as<int2>(), ctx->tb_idx.as<int>(), pos, per, box, F,
+            launch_twobody_f64(sr, ctx->dimer_cap, nd, ctx->dimers.as<int2>(), ctx->tb_idx.as<int>(), pos, per, box, F,
                                E + 1, ev, eu2, ctx->pip_blocks);
         else
-            launch_twobody_f32(sr, ctx->stats.dimers, ctx->dimers.as<int2>(), ctx->tb_idx.as<int>(), pos, per, box, F,
+            launch_twobody_f32(sr, ctx->dimer_cap, nd, ctx->dimers.as<int2>(), ctx->tb_idx.as<int>(), pos, per, box, F,
                                E + 1, ev, eu2, ctx->pip_blocks);
         timer_kernel_done(ctx, T_TWOBODY, ev);
         // dimers are listed by their first molecule, so a group's dimers are consecutive
-        if (G) launch_group_sums(sr, G, ctx->g_wstart.as<int>(), 0, ctx->dimer_off.as<long long>(), eu2, EG, 1);
+        if (G)
+            launch_group_sums(sr, G, ctx->g_wstart.as<int>(), 0, ctx->dimer_off.as<long long>(), eu2, EG, 1,
+                              ctx->dimer_cap);
     }
     if (do3) {
         const int per = s.th_method == MBPOL_CUTOFF_PERIODIC;
         const KEvents ev = timer_kernel(ctx, T_THREEBODY);
-        const long long nt = ctx->stats.trimers;
+        const long long nt = ctx->trip_cap;
+        const long long* ntd = ctx->trip_off.as<long long>() + (int)s.th_idx.size() / 3;  // list length (device)
         const int* tr = ctx->trips.as<int>();
         const int* ti = ctx->th_idx.as<int>();
         const int cap = ctx->pip3_blocks;
         if (!dp)
-            launch_threebody_orbit_f32(sr, nt, tr, ti, pos, per, box, F, E + 2, ev, eu3, cap);
+            launch_threebody_orbit_f32(sr, nt, ntd, tr, ti, pos, per, box, F, E + 2, ev, eu3, cap);
         else if (ctx->pip3_kernel == 1)
-            launch_threebody_f64(sr, nt, tr, ti, pos, per, box, F, E + 2, ev, eu3, cap);
+            launch_threebody_f64(sr, nt, ntd, tr, ti, pos, per, box, F, E + 2, ev, eu3, cap);
         else
-            launch_threebody_orbit_f64(sr, nt, tr, ti, pos, per, box, F, E + 2, ev, eu3, cap);
+            launch_threebody_orbit_f64(sr, nt, ntd, tr, ti, pos, per, box, F, E + 2, ev, eu3, cap);
         timer_kernel_done(ctx, T_THREEBODY, ev);
-        if (G) launch_group_sums(sr, G, ctx->g_wstart.as<int>(), 0, ctx->trip_off.as<long long>(), eu3, EG, 2);
+        if (G)
+            launch_group_sums(sr, G, ctx->g_wstart.as<int>(), 0, ctx->trip_off.as<long long>(), eu3, EG, 2,
+                              ctx->trip_cap);
     }
     if ((comp & MBPOL_DISPERSION) && ctx->disp) {
         timer_start(ctx, T_DISPERSION);
@@ -822,64 +839,63 @@ static int evaluate(mbpol_context* ctx, double* forces_dev_out, double* energies
     u64* F = ctx->force_fx.as<u64>();
     u64* E = ctx->energy_fx.as<u64>();
     double* pos = ctx->pos.as<double>();
-    HIP_TRY(hipMemsetAsync(F, 0, sizeof(u64) * 3 * s.n, st));
-    HIP_TRY(hipMemsetAsync(E, 0, sizeof(u64) * 8, st));
-    if (ctx->ngroups) HIP_TRY(hipMemsetAsync(ctx->g_energy.p, 0, sizeof(u64) * 8 * ctx->ngroups, st));
+    launch_fill_zero(st, F, sizeof(u64) * 3 * s.n);
+    launch_fill_zero(st, E, sizeof(u64) * 8);
+    if (ctx->ngroups) launch_fill_zero(st, ctx->g_energy.p, sizeof(u64) * 8 * ctx->ngroups);
+    ctx->lists_built = 0;
     launch_place_vsites(st, (int)s.vs_idx.size() / 4, ctx->vs_idx.as<int>(), ctx->vs_w.as<double>(), pos);
     // the 1-body term runs with the charges when both are computed for the same waters
     ctx->onebody_with_charges = ctx->water_fused && (comp & MBPOL_ONEBODY) && (comp & MBPOL_ELECTROSTATICS) &&
                                 ctx->elec && !s.ob_idx.empty();
-    // The short-range terms go to a second stream, enqueued by the context's
-    // worker thread: their host-side waits (list sizes) and their kernels
-    // overlap the electrostatics, which the calling thread drives.  With
-    // timing on, everything runs on one stream from this thread, so that every
-    // timed span is attributable.
-    // the first evaluation also runs the launch-shape tuning, which times kernels: it runs unforked
+    // The short-range terms go to a second stream, forked from and joined to the
+    // electrostatics' stream by events, so that the two overlap; no host wait sits
+    // in either (list lengths and row counts stay on the device and are checked
+    // after the evaluation).  With timing on, everything runs on one stream, so
+    // that every timed span is attributable; the first evaluation also runs the
+    // launch-shape tuning, which times kernels, and sizes the lists: unforked.
     const bool fork = !ctx->timer.enabled && ctx->evaluated;
+    int rc = MBPOL_OK;
     if (!fork) {
-        if (int e = short_range(ctx, st, comp)) return e;
+        rc = short_range(ctx, st, comp);
     } else {
         HIP_TRY(hipEventRecord(ctx->ev_fork, st));
-        const int device = ctx->device;
-        ctx->worker.post([ctx, comp, device]() -> int {
-            HIP_TRY(hipSetDevice(device));
-            HIP_TRY(hipStreamWaitEvent(ctx->side, ctx->ev_fork, 0));
-            if (int e = short_range(ctx, ctx->side, comp)) return e;
-            HIP_TRY(hipEventRecord(ctx->ev_join, ctx->side));
-            return MBPOL_OK;
-        });
+        HIP_TRY(hipStreamWaitEvent(ctx->side, ctx->ev_fork, 0));
+        rc = short_range(ctx, ctx->side, comp);
+        HIP_TRY(hipEventRecord(ctx->ev_join, ctx->side));
     }
-    int rc = MBPOL_OK;
-    if ((comp & MBPOL_ELECTROSTATICS) && ctx->elec) rc = elec_evaluate(ctx, E + 4);
-    int wrc = MBPOL_OK;
-    std::string msg;
-    if (fork) wrc = ctx->worker.wait(msg);  // always wait for the worker: it uses the context until its job ends
-    if (rc || wrc) {
-        // The worker only enqueued the short-range kernels: they may still be
-        // adding into the force/energy accumulators and reading the positions,
-        // which a re-run (or the next call) clears and overwrites.
-        if (fork) HIP_TRY(hipStreamSynchronize(ctx->side));
-        // a solve that failed on rows that overflowed is run again with rows that fit
-        bool redo = false;
+    if (!rc && (comp & MBPOL_ELECTROSTATICS) && ctx->elec) rc = elec_evaluate(ctx, E + 4);
+    if (fork) HIP_TRY(hipStreamWaitEvent(st, ctx->ev_join, 0));
+    if (rc) {
+        // the short-range kernels may still be adding into the accumulators and
+        // reading the positions, which a re-run (or the next call) clears and overwrites
+        HIP_TRY(hipStreamSynchronize(st));
+        // a solve that failed on rows or lists that overflowed is run again with ones that fit
+        bool redo = false, redo_lists = false;
         if (ctx->elec)
             if (int e = elec_rows_check(ctx, redo)) return e;
-        if (redo) return evaluate(ctx, forces_dev_out, energies_out, comp, forces_host_out);
-        if (rc) return rc;
-        return set_error(wrc, "%s", msg.c_str());
+        if (int e = list_lengths_to_host(ctx, st)) return e;
+        HIP_TRY(hipStreamSynchronize(st));
+        if (int e = list_check(ctx, redo_lists)) return e;
+        if (redo || redo_lists) return evaluate(ctx, forces_dev_out, energies_out, comp, forces_host_out);
+        return rc;
     }
-    if (fork) HIP_TRY(hipStreamWaitEvent(st, ctx->ev_join, 0));
     double* fout = forces_dev_out ? forces_dev_out : ctx->force.as<double>();
     launch_forces_to_double(st, 3 * s.n, F, fout);
     launch_fold_vsites(st, (int)s.vs_idx.size() / 4, ctx->vs_idx.as<int>(), ctx->vs_w.as<double>(), fout);
     u64* he = static_cast<u64*>(ctx->h_counts) + 8;
     HIP_TRY(hipMemcpyAsync(he, E, sizeof(u64) * 8, hipMemcpyDeviceToHost, st));
+    if (int e = list_lengths_to_host(ctx, st)) return e;
     if (forces_host_out)
         HIP_TRY(hipMemcpyAsync(forces_host_out, fout, sizeof(double) * 3 * s.n, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
-    if (ctx->elec) {
-        bool redo = false;
-        if (int e = elec_rows_check(ctx, redo)) return e;
-        if (redo) return evaluate(ctx, forces_dev_out, energies_out, comp, forces_host_out);  // once: the next build checks first
+    {
+        // rows or lists that did not fit: grown, and the evaluation runs again (once:
+        // the capacities then hold the lengths just measured)
+        bool redo = false, redo_lists = false;
+        if (ctx->elec)
+            if (int e = elec_rows_check(ctx, redo)) return e;
+        if (int e = list_check(ctx, redo_lists)) return e;
+        if (redo || redo_lists) return evaluate(ctx, forces_dev_out, energies_out, comp, forces_host_out);
     }
     ctx->evaluated = true;
     if (energies_out) {
