@@ -310,7 +310,8 @@ def main():
                                % (alpha, grid[0], grid[1], grid[2], args.precision),
                    'waters': 4096, 'box_nm': L, 'precision': args.precision},
         'iterations': stats['iterations'],
-        'autotuned': {'site_lanes': ctx.getTuning('site_lanes')},
+        'autotuned': {n: ctx.getTuning(n) for n in ('site_lanes', 'pip_waves', 'pip3_waves', 'disp_waves',
+                                                     'tune_evaluations')},
         'dipole_solver': 'sor' if args.precision == 'double' else 'pcg',
         'counts': {k: stats[k] for k in ('num_dimers', 'num_trimers', 'num_elec_pairs', 'num_pol_pairs')},
         'energies_kj_mol': dict(zip(['onebody', 'twobody', 'threebody', 'dispersion', 'electrostatics', 'total'],
@@ -318,6 +319,18 @@ def main():
         'breakdown_ms_per_step': {k: timing[k]['ms_per_step'] for k in PHASES},
         'roofline': roofline,
     }
+    # the same evaluations replayed from a HIP graph (mbpol_context_set_graph): captured
+    # once, checked after every replay; the energies must equal the eager ones bit for bit
+    ctx.setGraph(True)
+    for _ in range(max(args.warmup, 2)):
+        ctx.evaluateResident()
+    ms_graph, e_graph = run_steps(ctx, args.steps)
+    line['graph_path'] = {'value': 1000.0 / ms_graph, 'ms_per_step': ms_graph,
+                          'energies_equal_eager': bool(np.array_equal(e_graph, energies)),
+                          'captured_field_evaluations': ctx.getTuning('graph_steps'),
+                          'captures': ctx.getTuning('graph_captures'), 'replays': ctx.getTuning('graph_replays'),
+                          'fallbacks': ctx.getTuning('graph_fallbacks')}
+    ctx.setGraph(False)
     # the same evaluation through the host-buffer entry point mbpol_context_evaluate
     # (positions copied in, virtual sites placed on the device, forces copied out)
     import ctypes

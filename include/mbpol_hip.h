@@ -182,11 +182,15 @@ int mbpol_context_set_dipole_convergence(mbpol_context* ctx, double target_eps, 
 /* MBPOL_SOLVER_SOR or MBPOL_SOLVER_PCG for the following evaluations (no counterpart in the reference, which
  * always iterates by SOR; the default depends on the precision mode given at creation) */
 int mbpol_context_set_solver(mbpol_context* ctx, int32_t solver);
-/* launch shapes the context picked by timing candidates on its first evaluation (no counterpart in the
- * reference): "site_lanes" = lanes per site of the electrostatics row build (8, 16 or 32; 0 before the first
- * evaluation); "row_capacity_pol" / "row_capacity_other" = entries per electrostatics neighbour row (they
- * grow when a configuration needs more: a row that overflows is detected at the end of the evaluation,
- * which is then run again) */
+/* launch shapes, capacities and counters of the context (no counterpart in the reference): "site_lanes" =
+ * lanes per site of the electrostatics row build, picked by timing 8, 16 and 32 on the first evaluation (0
+ * before it); "row_capacity_pol" / "row_capacity_other" = entries per electrostatics neighbour row;
+ * "dimer_capacity", "lower_capacity", "trimer_capacity" = entries of the 2- and 3-body work lists (rows and
+ * lists grow when a configuration needs more: an overflow is detected at the end of the evaluation, which
+ * is then run again); "pip_blocks", "pip3_blocks", "disp_blocks" = grid caps of the 2-body, 3-body and
+ * dispersion kernels (0 = one block per 64 units); "pip3_kernel" = 3-body kernel (0 relabelling-group loop,
+ * 1 term by term); "graph_steps", "graph_captures", "graph_replays", "graph_fallbacks" = field evaluations
+ * of the captured dipole solve and the graph mode's counters (mbpol_context_set_graph) */
 int mbpol_context_get_tuning(const mbpol_context* ctx, const char* name, int32_t* value);
 /* copyParametersToContext of MBPolOneBodyForce / MBPolTwoBodyForce / MBPolThreeBodyForce
  * (MBPolReferenceKernels.cpp:116-127, 401-414, 485-498): new {O,H1,H2} index lists for the same number of
@@ -204,6 +208,16 @@ int mbpol_context_get_dimer_list(const mbpol_context* ctx, int32_t* pairs, int64
 int mbpol_context_get_trimer_list(const mbpol_context* ctx, int32_t* triplets, int64_t capacity, int64_t* count);
 /* 1 = keep every chain triplet (reference list); 0 (default) = drop triplets whose switch product is exactly 0 */
 int mbpol_context_set_list_mode(mbpol_context* ctx, int32_t full_chain_list);
+/* Graph mode (no counterpart in the reference; OpenMM's CUDA platform launches kernel by kernel): 1 =
+ * evaluations are captured in a HIP graph once and replayed while the host state they were captured from
+ * holds -- the same arguments, parameters, list and row capacities and guess start; every setter above
+ * marks a captured graph stale.  The dipole solve is captured as a fixed number of conjugate-gradient steps
+ * (the last solve's + 3; steps after the stopping rule return at once); after each replay the host checks
+ * the stopping rule, the lists and the rows, and runs the evaluation again eagerly when one of them did not
+ * fit (the next call captures anew).  Evaluations a graph cannot hold run eagerly: timing on, the SOR
+ * solver, a learned guess or its history, hipFFT transforms, and the first evaluation of a context, which
+ * sizes every buffer.  Results equal the eager path's bit for bit.  0 (default) = eager. */
+int mbpol_context_set_graph(mbpol_context* ctx, int32_t enabled);
 /* MBPolElectrostaticsForce::getElectrostaticPotential at npoints points (nm), kJ/mol/e, from the charges and
  * induced dipoles of the last evaluate (MBPolReferenceElectrostaticsForce.cpp:1023-1086) */
 int mbpol_context_get_electrostatic_potential(mbpol_context* ctx, int32_t npoints, const double* points,
@@ -229,17 +243,33 @@ int mbpol_context_get_system_electrostatics_moments(mbpol_context* ctx, const do
 typedef struct mbpol_guess mbpol_guess;
 #define MBPOL_GUESS_PARAMS 7491         /* W1[48][64] b1[64] W2[64][64] b2[64] W3[64][3] b3[3], matrices [in][out] */
 #define MBPOL_GUESS_FEATURES 48
+/* what the network corrects: alpha E_fixed, anchored on it (the default), or the previous evaluation's
+ * converged dipoles, anchored on the change of alpha E_fixed since that evaluation (the context then keeps
+ * the last two evaluations' dipoles and fields; a snapshot needs two earlier evaluations) */
+#define MBPOL_GUESS_KIND_ALPHA_E 0
+#define MBPOL_GUESS_KIND_PREVIOUS 1
 int mbpol_guess_create(mbpol_context* ctx, int32_t max_snapshots, uint64_t seed, mbpol_guess** out);
+int mbpol_guess_create_kind(mbpol_context* ctx, int32_t max_snapshots, uint64_t seed, int32_t kind,
+                            mbpol_guess** out);
 void mbpol_guess_destroy(mbpol_guess* g);
 /* features and targets (converged dipoles) of the context's last evaluate */
 int mbpol_guess_add_snapshot(mbpol_guess* g);
 int mbpol_guess_num_snapshots(const mbpol_guess* g, int32_t* count);
+/* the network's guess (dipoles, particles x 3) for the context's last evaluated configuration, from alpha
+ * E_fixed or (previous-step kind) from the evaluation before it -- as a snapshot pairs them */
+int mbpol_guess_predict_last(mbpol_guess* g, double* dipoles);
 /* stored snapshot k: features (3 per water x 48) and targets (3 per water x 3, frame components of
  * (mu - alpha E_fixed) / |alpha E_fixed|) */
 int mbpol_guess_get_snapshot(const mbpol_guess* g, int32_t k, float* features, float* targets);
 /* Adam on the stored snapshots (input normalisation and output scale set from them on the first call);
  * loss_per_epoch[epochs]: mean squared error in units of the output scale */
 int mbpol_guess_train(mbpol_guess* g, int32_t epochs, int32_t batch, double learning_rate, float* loss_per_epoch);
+/* the same on the first ntrain stored snapshots only (the rest held out) */
+int mbpol_guess_train_range(mbpol_guess* g, int32_t ntrain, int32_t epochs, int32_t batch, double learning_rate,
+                            float* loss_per_epoch);
+/* mean squared error (units of the output scale) of the trained network on stored snapshots
+ * [first, first + count), in batches of `batch` rows */
+int mbpol_guess_validation_loss(mbpol_guess* g, int32_t first, int32_t count, int32_t batch, double* loss);
 /* attach (1) or detach (0) the network as the context's MBPOL_GUESS_LEARNED guess (the mode is set too) */
 int mbpol_guess_attach(mbpol_guess* g, int32_t on);
 /* parameters (MBPOL_GUESS_PARAMS floats) and normalisation (48 means, 48 inverse deviations, output scale) */

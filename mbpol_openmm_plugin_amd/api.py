@@ -517,14 +517,21 @@ class Context:
         _check(library().mbpol_context_set_solver(self._h, SOLVERS[solver]))
 
     def getTuning(self, name):
-        """Launch shape picked by the context's autotuning ('site_lanes'), or a row
-        capacity of the electrostatics ('row_capacity_pol', 'row_capacity_other')."""
+        """Launch shape picked by the context's autotuning ('site_lanes'), a grid cap
+        ('pip_blocks', 'pip3_blocks', 'disp_blocks'), the 3-body kernel ('pip3_kernel'),
+        a capacity ('row_capacity_pol', 'row_capacity_other', 'dimer_capacity',
+        'lower_capacity', 'trimer_capacity') or a graph-mode counter ('graph_steps',
+        'graph_captures', 'graph_replays', 'graph_fallbacks')."""
         v = ctypes.c_int32(0)
         _check(library().mbpol_context_get_tuning(self._h, name.encode(), ctypes.byref(v)))
         return v.value
 
     def setListMode(self, full_chain_list):
         _check(library().mbpol_context_set_list_mode(self._h, int(bool(full_chain_list))))
+
+    def setGraph(self, enabled):
+        """Graph mode (HIP extension): evaluations captured in a HIP graph and replayed."""
+        _check(library().mbpol_context_set_graph(self._h, int(bool(enabled))))
 
     def getState(self, getEnergy=True, getForces=True, components=None):
         if self._positions is None:
@@ -687,12 +694,23 @@ class GuessModel:
     and attach() makes it the context's initial guess.  The solve still runs to
     the same tolerance, so the guess changes the iteration count only."""
 
-    def __init__(self, context, max_snapshots, seed=1):
+    KINDS = {'alpha_e': 0, 'previous': 1}
+
+    def __init__(self, context, max_snapshots, seed=1, kind='alpha_e'):
+        """kind: 'alpha_e' -- the network corrects alpha E_fixed (anchored on it);
+        'previous' -- it corrects the previous evaluation's converged dipoles,
+        anchored on the change of alpha E_fixed since then (snapshots need two
+        earlier evaluations of the context)."""
+        if kind not in self.KINDS:
+            raise MBPolError(1, 'unknown guess kind %r (one of %s)' % (kind, ', '.join(self.KINDS)))
         L = library()
-        L.mbpol_guess_create.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_uint64, ctypes.c_void_p]
+        L.mbpol_guess_create_kind.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_uint64, ctypes.c_int32,
+                                              ctypes.c_void_p]
         self.context = context
+        self.kind = kind
         self._h = ctypes.c_void_p()
-        _check(L.mbpol_guess_create(context._h, int(max_snapshots), int(seed), ctypes.byref(self._h)))
+        _check(L.mbpol_guess_create_kind(context._h, int(max_snapshots), int(seed), self.KINDS[kind],
+                                         ctypes.byref(self._h)))
         self.natoms = 3 * (context.system.getNumParticles() // 4)
 
     def __del__(self):
@@ -718,13 +736,32 @@ class GuessModel:
         _check(library().mbpol_guess_get_snapshot(self._h, int(k), _pf(f), _pf(t)))
         return f, t
 
-    def train(self, epochs, batch=4096, learning_rate=1e-3):
-        """Adam epochs over the stored snapshots; returns the mean loss of each epoch."""
+    def train(self, epochs, batch=4096, learning_rate=1e-3, ntrain=None):
+        """Adam epochs over the first `ntrain` stored snapshots (all by default);
+        returns the mean loss of each epoch."""
         L = library()
-        L.mbpol_guess_train.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_double, _fp]
+        L.mbpol_guess_train_range.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                              ctypes.c_double, _fp]
         out = np.zeros(max(int(epochs), 1), dtype=np.float32)
-        _check(L.mbpol_guess_train(self._h, int(epochs), int(batch), float(learning_rate), _pf(out)))
+        n = self.numSnapshots() if ntrain is None else int(ntrain)
+        _check(L.mbpol_guess_train_range(self._h, n, int(epochs), int(batch), float(learning_rate), _pf(out)))
         return out[:int(epochs)]
+
+    def predictLast(self):
+        """The network's guess (particles x 3) for the last evaluated configuration: from its
+        alpha E_fixed, or (kind 'previous') from the evaluation before it."""
+        out = np.zeros((self.context.system.getNumParticles(), 3))
+        _check(library().mbpol_guess_predict_last(self._h, _pd(out)))
+        return out
+
+    def validationLoss(self, first, count, batch=4096):
+        """Mean squared error (units of the output scale) on stored snapshots [first, first + count)."""
+        L = library()
+        L.mbpol_guess_validation_loss.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                                  ctypes.POINTER(ctypes.c_double)]
+        v = ctypes.c_double(0)
+        _check(L.mbpol_guess_validation_loss(self._h, int(first), int(count), int(batch), ctypes.byref(v)))
+        return v.value
 
     def attach(self, on=True):
         _check(library().mbpol_guess_attach(self._h, int(bool(on))))

@@ -54,6 +54,20 @@ struct Elec;
 struct Disp;
 
 // Grow-only device allocation.
+// set while this thread captures an evaluation into a HIP graph: a buffer that
+// would have to grow then fails instead (the capture is dropped and the
+// evaluation runs eagerly)
+extern thread_local bool g_capturing;
+
+// A copy ordered with the context's work: enqueued on stream s and waited for.
+// A plain hipMemcpy runs on the null stream, which the context's non-blocking
+// streams do not wait for, and a host-to-device (pageable source) or
+// device-to-device hipMemcpy may return before its data has landed.
+inline hipError_t copy_on(hipStream_t s, void* dst, const void* src, size_t bytes) {
+    const hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyDefault, s);
+    return e != hipSuccess ? e : hipStreamSynchronize(s);
+}
+
 struct DevBuf {
     void* p = nullptr;
     size_t bytes = 0;
@@ -109,6 +123,10 @@ struct mbpol_context {
     int disp_blocks = 0;       // grid cap of the dispersion kernel (0: one wave per atom)
     int pip3_kernel = 0;       // 3-body kernel: 0 the polynomial through its relabelling group, 1 term by term
     int pip3_blocks = 0;       // grid cap of the 3-body kernel (0: one block per 64 units)
+    // the caps in waves per SIMD (blocks = waves x 4 x CUs), tuned on the first forked evaluation
+    int pip_waves = 1, pip3_waves = 0, disp_waves = 2, cus = 0;
+    bool tuned = false;
+    int tune_evaluations = 0;  // trial evaluations the tuning ran
     // the 1-body waters are the electrostatics waters (sites 4w..4w+3): the 1-body term
     // runs in the per-water kernel that computes the charges
     bool water_fused = false;
@@ -140,6 +158,19 @@ struct mbpol_context {
     long long trip_cap = 0;
     int lists_built = 0;        // lists built by this evaluation: 1 dimers, 2 trimers
     mbpol::DevBuf list_len;     // the lower list's length widened to 64 bits, for the read-back
+
+    // HIP graph of the whole evaluation (mbpol_context_set_graph): captured on first
+    // use, replayed while the host state it was captured from is unchanged
+    // (graph_key; graph_gen counts the calls that change parameters)
+    bool graph_on = false;
+    hipGraphExec_t graph_exec = nullptr;
+    long long graph_key[16] = {};
+    long long graph_bad_key[16] = {};  // a key whose capture failed (evaluated eagerly)
+    long long graph_gen = 0;
+    int graph_steps = 0;               // field evaluations of the captured solve (0: from the last solve)
+    int graph_lists_built = 0;
+    bool fork_ready = false;           // a forked evaluation has run (every buffer of one is allocated)
+    long long graph_captures = 0, graph_replays = 0, graph_fallbacks = 0;
 
     // electrostatics and dispersion live in their own structures
     mbpol::Elec* elec = nullptr;

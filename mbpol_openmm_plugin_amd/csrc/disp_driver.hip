@@ -27,9 +27,9 @@ struct Disp {
 };
 
 template <typename T>
-static int upv(DevBuf& b, const std::vector<T>& v) {
+static int upv(hipStream_t stream, DevBuf& b, const std::vector<T>& v) {
     HIP_TRY(b.ensure(std::max<size_t>(1, v.size()) * sizeof(T)));
-    if (!v.empty()) HIP_TRY(hipMemcpy(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    if (!v.empty()) HIP_TRY(copy_on(stream, b.p, v.data(), v.size() * sizeof(T)));
     return MBPOL_OK;
 }
 
@@ -71,11 +71,11 @@ int disp_create(mbpol_context* ctx) {
     D->g.ntypes = s.ds_ntypes;
     D->g.method = s.ds_method;
     D->g.cutoff = s.ds_cutoff;
-    if (int e = upv(D->act, act)) return e;
-    if (int e = upv(D->type, s.ds_type)) return e;
-    if (int e = upv(D->mol, s.ds_mol)) return e;
-    if (int e = upv(D->c6, s.ds_c6)) return e;
-    if (int e = upv(D->d6, s.ds_d6)) return e;
+    if (int e = upv(ctx->stream, D->act, act)) return e;
+    if (int e = upv(ctx->stream, D->type, s.ds_type)) return e;
+    if (int e = upv(ctx->stream, D->mol, s.ds_mol)) return e;
+    if (int e = upv(ctx->stream, D->c6, s.ds_c6)) return e;
+    if (int e = upv(ctx->stream, D->d6, s.ds_d6)) return e;
     HIP_TRY(D->cell_of.ensure(sizeof(int) * std::max(1, D->g.na)));
     HIP_TRY(D->members.ensure(sizeof(int) * std::max(1, D->g.na)));
     HIP_TRY(D->cpos.ensure(sizeof(double4) * std::max(1, D->g.na)));
@@ -97,9 +97,9 @@ int disp_create(mbpol_context* ctx) {
                 gs[a] = gact[g];
                 ge[a] = gact[g + 1];
             }
-        if (int e = upv(D->gs, gs)) return e;
-        if (int e = upv(D->ge, ge)) return e;
-        if (int e = upv(D->gact, gact)) return e;
+        if (int e = upv(ctx->stream, D->gs, gs)) return e;
+        if (int e = upv(ctx->stream, D->ge, ge)) return e;
+        if (int e = upv(ctx->stream, D->gact, gact)) return e;
         D->g.gs = D->gs.as<int>();
         D->g.ge = D->ge.as<int>();
     }
@@ -152,7 +152,7 @@ int disp_pair_count(const mbpol_context* ctx, long long* pairs) {
     *pairs = 0;
     if (!D || D->g.na == 0) return MBPOL_OK;
     unsigned long long v = 0;
-    HIP_TRY(hipMemcpy(&v, D->npairs.p, sizeof(v), hipMemcpyDeviceToHost));
+    HIP_TRY(copy_on(ctx->stream, &v, D->npairs.p, sizeof(v)));
     *pairs = (long long)(v / 2);  // each pair is counted from both atoms
     return MBPOL_OK;
 }

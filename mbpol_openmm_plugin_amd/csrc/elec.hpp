@@ -1,6 +1,7 @@
 // Declarations shared between the evaluation driver (mbpol_api.hip) and the
 // electrostatics / dispersion drivers.
 #pragma once
+#include <functional>
 #include "context.hpp"
 
 namespace mbpol {
@@ -21,7 +22,27 @@ void timer_kernel_done(mbpol_context* ctx, int cat, const KEvents& ev);
 int elec_create(mbpol_context* ctx);
 void elec_destroy(mbpol_context* ctx);
 int elec_set_box(mbpol_context* ctx);
-int elec_evaluate(mbpol_context* ctx, u64* energy);
+// after_setup (optional) runs once the set-up phase is enqueued, before the dipole solve.
+// fixed_steps > 0: a conjugate-gradient solve of that many field evaluations with no
+// host read (the form a HIP graph captures); elec_graph_finish reads its outcome.
+int elec_evaluate(mbpol_context* ctx, u64* energy, const std::function<int()>& after_setup = nullptr,
+                  int fixed_steps = 0);
+// whether the context's electrostatics can run as a fixed-length solve: conjugate
+// gradients, rows checked after the evaluation, transforms by the library's own
+// kernels, and no learned guess or guess history
+bool elec_graph_ok(const mbpol_context* ctx);
+// field evaluations for a fixed-length solve: the last solve's plus `extra`, at most max_iter + 1
+int elec_graph_steps(const mbpol_context* ctx, int extra);
+// whether the evaluation-level tuning may run trial evaluations (no learned guess or history)
+bool elec_tune_ok(const mbpol_context* ctx);
+// hold = true: keep the previous dipoles and solver state aside and start trial solves
+// from alpha E; false: put them back, so the next evaluation runs as if no trial had
+int elec_tune_hold(mbpol_context* ctx, bool hold);
+// the host state a captured solve depends on (guess start, row capacities)
+void elec_graph_key(const mbpol_context* ctx, long long key[4]);
+// after a fixed-length solve has run: its statistics; `more` when it stopped before the
+// stopping rule did (the evaluation is then run again with the host-driven solve)
+int elec_graph_finish(mbpol_context* ctx, bool& more);
 // after an evaluation: whether a neighbour row overflowed its capacity (the
 // capacities have then grown and the evaluation must be run again)
 int elec_rows_check(mbpol_context* ctx, bool& redo);
@@ -32,6 +53,10 @@ struct ElecView {
     int n = 0, nw = 0;
     bool water_layout = false;  // 4 sites per water, O H H M, types 0 1 1 2
     const double *efix_pol = nullptr, *mu = nullptr;
+    // history (kept while a learned guess of the previous-step kind exists): the converged
+    // dipoles and alpha E_fixed of the last evaluation (prev) and of the one before it (prev2)
+    const double *mu_prev = nullptr, *efix_prev = nullptr, *mu_prev2 = nullptr, *efix_prev2 = nullptr;
+    bool have_previous = false, have_previous2 = false;
     const int* type = nullptr;
     const int *cell_of = nullptr, *cell_start = nullptr, *members = nullptr;
     int use_cells = 0, reach = 0;
@@ -40,6 +65,8 @@ struct ElecView {
     d3 box{0, 0, 0}, inv_box{0, 0, 0};
 };
 int elec_view(mbpol_context* ctx, ElecView& v);
+// keep (on) or stop keeping the history of ElecView (counted: one per user)
+void elec_keep_history(mbpol_context* ctx, bool on);
 // learned guess (guess_driver.hip): mu0 for the polarisable sites from the
 // context's attached model, after the electrostatics set-up of this evaluation
 int guess_predict(mbpol_context* ctx, double* mu0);

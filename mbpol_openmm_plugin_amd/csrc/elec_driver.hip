@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <mutex>
 #include <string>
@@ -81,6 +82,15 @@ struct Elec {
     int capP = 0, capQ = 0;
     // fields and dipoles: the iteration alternates between mu_a and mu_b
     DevBuf efix, efix_pol, mu_a, mu_b, mu_prev, mu_user, partial, eps;
+    // history for the learned guess of the previous-step kind (kept while history > 0): alpha E_fixed
+    // of the last evaluation, and the dipoles and alpha E_fixed of the one before it
+    int history = 0;
+    // state held while the evaluation-level tuning runs (elec_tune_hold)
+    DevBuf mu_held;
+    bool held_have_previous = false;
+    int held_last_iters = 0, held_guess = MBPOL_GUESS_ALPHA_E;
+    bool have_previous2 = false;
+    DevBuf efix_prev, mu_prev2, efix_prev2;
     // conjugate-gradient vectors (3 n), block partials and the r.z history
     DevBuf cg_r, cg_z, cg_p, cg_q, cg_partial, cg_rz;
     // block-Jacobi preconditioner: polarisable sites grouped by molecule, and the blocks' inverses
@@ -184,9 +194,9 @@ static std::vector<double> sixth_roots(const std::vector<double>& d) {
 }
 
 template <typename T>
-static int up(DevBuf& b, const std::vector<T>& v) {
+static int up(hipStream_t stream, DevBuf& b, const std::vector<T>& v) {
     HIP_TRY(b.ensure(std::max<size_t>(1, v.size()) * sizeof(T)));
-    if (!v.empty()) HIP_TRY(hipMemcpy(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    if (!v.empty()) HIP_TRY(copy_on(stream, b.p, v.data(), v.size() * sizeof(T)));
     return MBPOL_OK;
 }
 
@@ -239,7 +249,7 @@ static void brick_axis(int K, int& s, int& nb) {
 // polarisable-site index list and its host mirror
 // the polarisable sites, and their grouping by molecule into preconditioner
 // blocks of at most 4 sites (molecules in order of their first site)
-static int set_polarisable(Elec* E, const std::vector<double>& pol, const std::vector<int>& mol) {
+static int set_polarisable(mbpol_context* ctx, Elec* E, const std::vector<double>& pol, const std::vector<int>& mol) {
     std::vector<int> idx;
     E->h_pol.assign(pol.size(), 0);
     std::vector<int> order;                      // molecules by first polarisable site
@@ -262,10 +272,10 @@ static int set_polarisable(Elec* E, const std::vector<double>& pol, const std::v
         }
     }
     E->nblk = (int)start.size() - 1;
-    if (int e = up(E->blk_start, start)) return e;
-    if (int e = up(E->blk_site, site)) return e;
+    if (int e = up(ctx->stream, E->blk_start, start)) return e;
+    if (int e = up(ctx->stream, E->blk_site, site)) return e;
     HIP_TRY(E->minv.ensure(sizeof(double) * block_ld() * block_ld() * std::max(E->nblk, 1)));
-    return up(E->polidx, idx);
+    return up(ctx->stream, E->polidx, idx);
 }
 
 int elec_set_box(mbpol_context* ctx) {
@@ -345,10 +355,10 @@ static int set_groups(mbpol_context* ctx) {
     bs[G] = ws[G];
     if (E->nblk != ws[G])
         return set_error(MBPOL_ERR_INVALID_ARGUMENT, "groups: every water needs its polarisable sites");
-    if (int e = up(E->g_site_s, ss)) return e;
-    if (int e = up(E->g_site_e, se)) return e;
-    if (int e = up(E->g_blk_start, bs)) return e;
-    if (int e = up(E->g_nsites, ns)) return e;
+    if (int e = up(ctx->stream, E->g_site_s, ss)) return e;
+    if (int e = up(ctx->stream, E->g_site_e, se)) return e;
+    if (int e = up(ctx->stream, E->g_blk_start, bs)) return e;
+    if (int e = up(ctx->stream, E->g_nsites, ns)) return e;
     HIP_TRY(E->g_blk_ee.ensure(sizeof(double) * std::max(E->nblk, 1)));
     HIP_TRY(E->g_esite.ensure(sizeof(double) * std::max(E->n, 1)));
     E->P.gs = E->g_site_s.as<int>();
@@ -390,20 +400,20 @@ int elec_create(mbpol_context* ctx) {
                          grid[0], grid[1], grid[2]);
     P.grid = make_int3(grid[0], grid[1], grid[2]);
     const int n = E->n;
-    if (int e = up(E->q0, s.el_q)) return e;
-    if (int e = up(E->damp6, sixth_roots(s.el_damp))) return e;
-    if (int e = up(E->pol, s.el_pol)) return e;
-    if (int e = up(E->mol, s.el_mol)) return e;
-    if (int e = up(E->type, s.el_type)) return e;
-    if (int e = set_polarisable(E, s.el_pol, s.el_mol)) return e;
+    if (int e = up(ctx->stream, E->q0, s.el_q)) return e;
+    if (int e = up(ctx->stream, E->damp6, sixth_roots(s.el_damp))) return e;
+    if (int e = up(ctx->stream, E->pol, s.el_pol)) return e;
+    if (int e = up(ctx->stream, E->mol, s.el_mol)) return e;
+    if (int e = up(ctx->stream, E->type, s.el_type)) return e;
+    if (int e = set_polarisable(ctx, E, s.el_pol, s.el_mol)) return e;
     HIP_TRY(E->q.ensure(sizeof(double) * n));
     HIP_TRY(E->dq.ensure(sizeof(double) * 27 * std::max(E->nw, 1)));
     for (DevBuf* b : {&E->efix, &E->efix_pol, &E->mu_a, &E->mu_b, &E->mu_prev, &E->mu_user, &E->fsite})
         HIP_TRY(b->ensure(sizeof(double) * 3 * std::max(n, 1)));
     // sites the iteration never visits keep zero dipoles in both buffers
-    HIP_TRY(hipMemset(E->mu_a.p, 0, sizeof(double) * 3 * std::max(n, 1)));
-    HIP_TRY(hipMemset(E->mu_b.p, 0, sizeof(double) * 3 * std::max(n, 1)));
-    HIP_TRY(hipMemset(E->mu_prev.p, 0, sizeof(double) * 3 * std::max(n, 1)));
+    launch_fill_zero(ctx->stream, E->mu_a.p, sizeof(double) * 3 * std::max(n, 1));
+    launch_fill_zero(ctx->stream, E->mu_b.p, sizeof(double) * 3 * std::max(n, 1));
+    launch_fill_zero(ctx->stream, E->mu_prev.p, sizeof(double) * 3 * std::max(n, 1));
     E->mu_cur = E->mu_a.as<double>();
     HIP_TRY(E->V.ensure(sizeof(double) * std::max(n, 1)));
     HIP_TRY(E->phi_fix.ensure(sizeof(double) * 10 * std::max(n, 1)));
@@ -444,9 +454,9 @@ int elec_create(mbpol_context* ctx) {
         HIP_TRY(E->grid.ensure(sizeof(double) * nreal));
         HIP_TRY(E->spec.ensure(sizeof(double2) * nhalf));
         HIP_TRY(E->influence.ensure(sizeof(double) * nhalf));
-        if (int e = up(E->bx, bspline_moduli(grid[0]))) return e;
-        if (int e = up(E->by, bspline_moduli(grid[1]))) return e;
-        if (int e = up(E->bz, bspline_moduli(grid[2]))) return e;
+        if (int e = up(ctx->stream, E->bx, bspline_moduli(grid[0]))) return e;
+        if (int e = up(ctx->stream, E->by, bspline_moduli(grid[1]))) return e;
+        if (int e = up(ctx->stream, E->bz, bspline_moduli(grid[2]))) return e;
         for (int d = 0; d < 3; ++d) brick_axis(grid[d], E->B.s[d], E->B.nb[d]);
         const int nbins = E->B.nb[0] * E->B.nb[1] * E->B.nb[2];
         HIP_TRY(E->binq.of.ensure(sizeof(int) * std::max(n, 1)));
@@ -465,7 +475,7 @@ int elec_create(mbpol_context* ctx) {
                     const long double a = 2.0L * 3.141592653589793238462643383279502884L * t / m;
                     tw[t] = make_double2((double)cosl(a), (double)-sinl(a));
                 }
-                if (int e = up(E->tw[d], tw)) return e;
+                if (int e = up(ctx->stream, E->tw[d], tw)) return e;
                 E->fft.ax[d].tw = E->tw[d].as<double2>();
             }
         } else {
@@ -522,7 +532,7 @@ static int tune_site_lists(mbpol_context* ctx) {
     for (int lanes : {8, 16, 32}) {
         float ms = 0.f;
         for (int rep = 0; rep < 2; ++rep) {
-            HIP_TRY(hipMemsetAsync(E->maxcnt.p, 0, sizeof(int) * 4, st));
+            launch_fill_zero(st, E->maxcnt.p, sizeof(int) * 4);
             HIP_TRY(hipEventRecord(a, st));
             launch_site_lists(st, lanes, E->n, ctx->pos.as<double>(), E->pol.as<double>(), E->P, E->nc, E->reach,
                               E->use_cells, E->cell_of.as<int>(), E->cell_start.as<int>(), E->members.as<int>(),
@@ -563,8 +573,8 @@ static int build_rows(mbpol_context* ctx) {
     const double* pos = ctx->pos.as<double>();
     const int ncell = E->nc.x * E->nc.y * E->nc.z;
     if (E->use_cells) {
-        HIP_TRY(hipMemsetAsync(E->cell_count.p, 0, sizeof(int) * (ncell + 1), st));
-        HIP_TRY(hipMemsetAsync(E->cell_fill.p, 0, sizeof(int) * (ncell + 1), st));
+        launch_fill_zero(st, E->cell_count.p, sizeof(int) * (ncell + 1));
+        launch_fill_zero(st, E->cell_fill.p, sizeof(int) * (ncell + 1));
         launch_site_cells(st, n, pos, E->P, E->nc, E->cell_of.as<int>(), E->cell_count.as<int>());
         if (int e = scan_i32(ctx, E->cell_count.as<int>(), E->cell_start.as<int>(), ncell + 1)) return e;
         launch_bucket_fill(st, n, E->cell_of.as<int>(), E->cell_start.as<int>(), E->cell_fill.as<int>(),
@@ -584,7 +594,7 @@ static int build_rows(mbpol_context* ctx) {
         HIP_TRY(E->nbP.ensure(sizeof(int) * std::max<size_t>(rp, 1)));
         HIP_TRY(E->pf.ensure(sizeof(double2) * std::max<size_t>(rp, 1)));
         HIP_TRY(E->nbQ.ensure(sizeof(int) * std::max<size_t>(rq, 1)));
-        HIP_TRY(hipMemsetAsync(E->maxcnt.p, 0, sizeof(int) * 4, st));
+        launch_fill_zero(st, E->maxcnt.p, sizeof(int) * 4);
         launch_site_lists(st, E->site_lanes, n, pos, E->pol.as<double>(), E->P, E->nc, E->reach, E->use_cells,
                           E->cell_of.as<int>(), E->cell_start.as<int>(), E->members.as<int>(), E->cpos.as<double4>(),
                           E->capP, E->capQ, E->nbP.as<int>(), E->cntP.as<int>(), E->nbQ.as<int>(), E->cntQ.as<int>(),
@@ -596,7 +606,7 @@ static int build_rows(mbpol_context* ctx) {
         HIP_TRY(E->nbP.ensure(sizeof(int) * std::max<size_t>(rp, 1)));
         HIP_TRY(E->pf.ensure(sizeof(double2) * std::max<size_t>(rp, 1)));
         HIP_TRY(E->nbQ.ensure(sizeof(int) * std::max<size_t>(rq, 1)));
-        HIP_TRY(hipMemsetAsync(E->maxcnt.p, 0, sizeof(int) * 4, st));
+        launch_fill_zero(st, E->maxcnt.p, sizeof(int) * 4);
         launch_site_lists(st, E->site_lanes > 0 ? E->site_lanes : 16, n, pos, E->pol.as<double>(), E->P, E->nc, E->reach,
                           E->use_cells, E->cell_of.as<int>(), E->cell_start.as<int>(), E->members.as<int>(),
                           E->cpos.as<double4>(), E->capP, E->capQ, E->nbP.as<int>(), E->cntP.as<int>(),
@@ -617,7 +627,7 @@ static int build_rows(mbpol_context* ctx) {
     }
     // prefactors of the P rows and the real-space part of the fixed field (the
     // reciprocal part is added to it after the charge grid)
-    HIP_TRY(hipMemsetAsync(E->efix.p, 0, sizeof(double) * 3 * n, st));
+    launch_fill_zero(st, E->efix.p, sizeof(double) * 3 * n);
     launch_prefactors_field(st, E->npol, E->polidx.as<int>(), pos, site_params(E), E->P, rows(E), E->pf.as<double2>(),
                             E->efix.as<double>());
     if (E->P.pme) {
@@ -628,8 +638,8 @@ static int build_rows(mbpol_context* ctx) {
         const size_t nent = (size_t)E->npol * 125;
         HIP_TRY(E->esite.ensure(sizeof(int) * std::max<size_t>(nent, 1)));
         HIP_TRY(E->ecoef.ensure(sizeof(double) * 3 * std::max<size_t>(nent, 1)));
-        HIP_TRY(hipMemsetAsync(E->rowcount.p, 0, sizeof(int) * (ngrid + 1), st));
-        HIP_TRY(hipMemsetAsync(E->rowfill.p, 0, sizeof(int) * (ngrid + 1), st));
+        launch_fill_zero(st, E->rowcount.p, sizeof(int) * (ngrid + 1));
+        launch_fill_zero(st, E->rowfill.p, sizeof(int) * (ngrid + 1));
         launch_csr_count(st, E->npol, E->polidx.as<int>(), E->g0tab.as<int>(), E->P, E->rowcount.as<int>());
         if (int e = scan_i32(ctx, E->rowcount.as<int>(), E->rowstart.as<int>(), (int)ngrid + 1)) return e;
         launch_csr_fill(st, E->npol, E->polidx.as<int>(), E->g0tab.as<int>(), E->sptab.as<double>(), E->P,
@@ -637,8 +647,8 @@ static int build_rows(mbpol_context* ctx) {
         launch_csr_sort(st, ngrid, E->rowstart.as<int>(), E->esite.as<int>(), E->ecoef.as<double>());
         // charged sites by brick, for the charge spreading
         Elec::Bins& b = E->binq;
-        HIP_TRY(hipMemsetAsync(b.count.p, 0, sizeof(int) * (nbins + 1), st));
-        HIP_TRY(hipMemsetAsync(b.fill.p, 0, sizeof(int) * (nbins + 1), st));
+        launch_fill_zero(st, b.count.p, sizeof(int) * (nbins + 1));
+        launch_fill_zero(st, b.fill.p, sizeof(int) * (nbins + 1));
         launch_site_bins(st, n, pos, E->q.as<double>(), E->P, E->B, b.of.as<int>(), b.count.as<int>());
         if (int e = scan_i32(ctx, b.count.as<int>(), b.start.as<int>(), nbins + 1)) return e;
         launch_bucket_fill(st, n, b.of.as<int>(), b.start.as<int>(), b.fill.as<int>(), E->unsorted.as<int>());
@@ -738,7 +748,7 @@ static int solve_sor(mbpol_context* ctx, double* buf0, double* buf1, double self
     SolveCtl* ctl = E->ctl.as<SolveCtl>();
     const int* stop = &ctl->stop;
     double* buf[2] = {buf0, buf1};
-    HIP_TRY(hipMemsetAsync(ctl, 0, sizeof(SolveCtl), st));
+    launch_fill_zero(st, ctl, sizeof(SolveCtl));
     int t = 0, batch = 0;
     c = SolveCtl{};
     while (t <= s.el_max_iter) {
@@ -771,7 +781,11 @@ static int solve_sor(mbpol_context* ctx, double* buf0, double* buf1, double self
 // measure is the reference's epsilon of the current dipoles, tested after every
 // step against the same target.  Iterations count field evaluations (the
 // initial residual included).
-static int solve_pcg(mbpol_context* ctx, double* x, double self_term, SolveCtl& c) {
+// fixed_steps > 0: exactly that many field evaluations (or max_iter + 1), with
+// no host read; the stopping rule on the device makes the steps after the stop
+// return at once, and the state is copied to the host for reading after the
+// work has run (elec_graph_finish).  This form can be captured in a HIP graph.
+static int solve_pcg(mbpol_context* ctx, double* x, double self_term, SolveCtl& c, int fixed_steps = 0) {
     Elec* E = ctx->elec;
     const mbpol_system& s = ctx->sys;
     hipStream_t st = ctx->stream;
@@ -810,12 +824,13 @@ static int solve_pcg(mbpol_context* ctx, double* x, double self_term, SolveCtl& 
         return MBPOL_OK;
     };
     launch_block_factor(st, B, pos, site_params(E), P, minv);
-    HIP_TRY(hipMemsetAsync(ctl, 0, sizeof(SolveCtl), st));
+    launch_fill_zero(st, ctl, sizeof(SolveCtl));
     // field evaluation f (0-based): f = 0 forms the initial residual and direction 0,
     // f = k + 1 takes step k and forms direction k + 1
     int f = 0, batch = 0;
     while (f <= s.el_max_iter) {
-        const int nb = std::min(solve_batch(E, ctx, batch++), s.el_max_iter + 1 - f);
+        const int nb = fixed_steps > 0 ? std::min(fixed_steps, s.el_max_iter + 1)
+                                       : std::min(solve_batch(E, ctx, batch++), s.el_max_iter + 1 - f);
         for (int b = 0; b < nb; ++b, ++f) {
             timer_start(ctx, T_ELEC_ITER);
             if (f == 0) {
@@ -833,10 +848,100 @@ static int solve_pcg(mbpol_context* ctx, double* x, double self_term, SolveCtl& 
             }
             timer_stop(ctx, T_ELEC_ITER);
         }
+        if (fixed_steps > 0) {
+            HIP_TRY(hipMemcpyAsync(reinterpret_cast<char*>(E->h_eps) + 64, E->ctl.p, sizeof(SolveCtl),
+                                   hipMemcpyDeviceToHost, st));
+            return MBPOL_OK;
+        }
         if (int e = read_ctl(ctx, c)) return e;
         if (c.stop) break;
     }
     return MBPOL_OK;
+}
+
+bool elec_graph_ok(const mbpol_context* ctx) {
+    const Elec* E = ctx->elec;
+    return E->solver == MBPOL_SOLVER_PCG && !E->rows_sync && E->site_lanes > 0 && (!E->P.pme || E->lds_fft) &&
+           E->guess_mode != MBPOL_GUESS_LEARNED && E->history == 0;
+}
+
+int elec_graph_steps(const mbpol_context* ctx, int extra) {
+    const Elec* E = ctx->elec;
+    return std::min(ctx->sys.el_max_iter + 1, std::max(E->last_iters, 1) + extra);
+}
+
+bool elec_tune_ok(const mbpol_context* ctx) {
+    const Elec* E = ctx->elec;
+    return E->history == 0 && E->guess_mode != MBPOL_GUESS_LEARNED;
+}
+
+int elec_tune_hold(mbpol_context* ctx, bool hold) {
+    Elec* E = ctx->elec;
+    hipStream_t st = ctx->stream;
+    const size_t bytes = sizeof(double) * 3 * std::max(E->n, 1);
+    if (hold) {
+        HIP_TRY(E->mu_held.ensure(bytes));
+        HIP_TRY(hipMemcpyAsync(E->mu_held.p, E->mu_prev.p, bytes, hipMemcpyDeviceToDevice, st));
+        E->held_have_previous = E->have_previous;
+        E->held_last_iters = E->last_iters;
+        E->held_guess = E->guess_mode;
+        E->guess_mode = MBPOL_GUESS_ALPHA_E;  // every trial solve starts alike
+    } else {
+        HIP_TRY(hipMemcpyAsync(E->mu_prev.p, E->mu_held.p, bytes, hipMemcpyDeviceToDevice, st));
+        E->have_previous = E->held_have_previous;
+        E->last_iters = E->held_last_iters;
+        E->guess_mode = E->held_guess;
+    }
+    return MBPOL_OK;
+}
+
+void elec_graph_key(const mbpol_context* ctx, long long key[4]) {
+    const Elec* E = ctx->elec;
+    key[0] = E->have_previous;
+    key[1] = E->guess_mode;
+    key[2] = E->capP;
+    key[3] = E->capQ;
+}
+
+int elec_graph_finish(mbpol_context* ctx, bool& more) {
+    Elec* E = ctx->elec;
+    more = false;
+    const SolveCtl c = *reinterpret_cast<const SolveCtl*>(reinterpret_cast<const char*>(E->h_eps) + 64);
+    E->rows_pending = true;  // the row maxima were copied to the host with the rest
+    E->mu_cur = E->mu_a.as<double>();  // where the conjugate-gradient solve leaves the dipoles
+    if (!c.stop) {  // the fixed steps did not reach the stopping rule
+        more = true;
+        return MBPOL_OK;
+    }
+    E->last_iters = c.iterations;
+    ctx->stats.iterations = c.iterations;
+    ctx->stats.epsilon = c.eps;
+    ctx->stats.converged = c.converged ? 1 : 0;
+    if (!c.converged)
+        return set_error(MBPOL_ERR_NOT_CONVERGED, "Induced dipoles did not converge:  iterations=%d eps=%g",
+                         c.iterations, c.eps);
+    E->have_previous = true;
+    return MBPOL_OK;
+}
+
+void elec_keep_history(mbpol_context* ctx, bool on) {
+    ++ctx->graph_gen;
+    Elec* E = ctx->elec;
+    if (!E) return;
+    if (on) {
+        if (E->history++ == 0) {
+            const size_t bytes = sizeof(double) * 3 * std::max(E->n, 1);
+            if (E->efix_prev.ensure(bytes) || E->mu_prev2.ensure(bytes) || E->efix_prev2.ensure(bytes)) {
+                --E->history;
+                return;
+            }
+            // the history starts with the next evaluation
+            E->have_previous = false;
+            E->have_previous2 = false;
+        }
+    } else if (E->history > 0) {
+        --E->history;
+    }
 }
 
 int elec_rows_check(mbpol_context* ctx, bool& redo) {
@@ -868,6 +973,16 @@ int elec_view(mbpol_context* ctx, ElecView& v) {
     for (int i = 0; v.water_layout && i < E->n; ++i)
         v.water_layout = s.el_type[i] == (i % 4 == 0 ? 0 : (i % 4 == 3 ? 2 : 1));
     v.efix_pol = E->efix_pol.as<double>();
+    v.mu_prev = E->mu_prev.as<double>();
+    v.have_previous = E->have_previous;
+    if (E->history) {
+        v.efix_prev = E->efix_prev.as<double>();
+        v.mu_prev2 = E->mu_prev2.as<double>();
+        v.efix_prev2 = E->efix_prev2.as<double>();
+        v.have_previous2 = E->have_previous2;
+    } else {
+        v.have_previous = false;  // no alpha E_fixed of the last evaluation without the history
+    }
     v.mu = E->mu_cur;
     v.type = E->type.as<int>();
     v.cell_of = E->cell_of.as<int>();
@@ -889,7 +1004,7 @@ int elec_view(mbpol_context* ctx, ElecView& v) {
     return MBPOL_OK;
 }
 
-int elec_evaluate(mbpol_context* ctx, u64* energy) {
+int elec_evaluate(mbpol_context* ctx, u64* energy, const std::function<int()>& after_setup, int fixed_steps) {
     Elec* E = ctx->elec;
     const mbpol_system& s = ctx->sys;
     hipStream_t st = ctx->stream;
@@ -930,7 +1045,7 @@ int elec_evaluate(mbpol_context* ctx, u64* energy) {
         case MBPOL_GUESS_LEARNED:
             if (int e = guess_predict(ctx, mu_in)) return e;
             break;
-        case MBPOL_GUESS_ZERO: HIP_TRY(hipMemsetAsync(mu_in, 0, mbytes, st)); break;
+        case MBPOL_GUESS_ZERO: launch_fill_zero(st, mu_in, mbytes); break;
         case MBPOL_GUESS_PREVIOUS:
             if (E->have_previous) {
                 HIP_TRY(hipMemcpyAsync(mu_in, E->mu_prev.p, mbytes, hipMemcpyDeviceToDevice, st));
@@ -947,31 +1062,40 @@ int elec_evaluate(mbpol_context* ctx, u64* energy) {
     }
     // accumulators of the energy phase, cleared before the solve: after the
     // solve's last host read the energy kernels follow without other work first
-    HIP_TRY(hipMemsetAsync(E->fsite.p, 0, sizeof(double) * 3 * n, st));
-    HIP_TRY(hipMemsetAsync(E->V.p, 0, sizeof(double) * n, st));
+    launch_fill_zero(st, E->fsite.p, sizeof(double) * 3 * n);
+    launch_fill_zero(st, E->V.p, sizeof(double) * n);
     timer_stop(ctx, T_ELEC_SETUP);
+    // the caller's work for the other stream is enqueued here: the set-up above
+    // keeps the GPU busy meanwhile, and the solve below waits on the host
+    if (after_setup)
+        if (int e = after_setup()) return e;
     const double self_term = P.pme ? (4.0 / 3.0) * (P.alpha * P.alpha * P.alpha) / mbpol_params::el_sqrt_pi : 0.0;
     SolveCtl c{};
+    if (fixed_steps > 0 && (E->solver != MBPOL_SOLVER_PCG || E->history))
+        return set_error(MBPOL_ERR_INVALID_ARGUMENT, "a fixed-length solve is a conjugate-gradient one without history");
     if (E->solver == MBPOL_SOLVER_PCG) {
         // conjugate gradients from the guess in mu_in; the solution stays in mu_in, and
         // the reciprocal induced potential of the energy is taken from it (mu_out = mu_in)
-        if (int e = solve_pcg(ctx, mu_in, self_term, c)) return e;
+        if (int e = solve_pcg(ctx, mu_in, self_term, c, fixed_steps)) return e;
         mu_out = mu_in;
     } else {
         if (E->ngroups)
             return set_error(MBPOL_ERR_INVALID_ARGUMENT, "independent groups are solved by conjugate gradients only");
         if (int e = solve_sor(ctx, E->mu_a.as<double>(), E->mu_b.as<double>(), self_term, mu_in, mu_out, c)) return e;
     }
-    const int it = c.iterations;
-    const double cur = c.eps;
-    const bool conv = c.converged != 0;
-    E->last_iters = it;
-    ctx->stats.iterations = it;
-    ctx->stats.epsilon = cur;
-    ctx->stats.converged = conv ? 1 : 0;
     E->mu_cur = mu_in;
-    if (!conv)
-        return set_error(MBPOL_ERR_NOT_CONVERGED, "Induced dipoles did not converge:  iterations=%d eps=%g", it, cur);
+    if (fixed_steps <= 0) {
+        const int it = c.iterations;
+        const double cur = c.eps;
+        const bool conv = c.converged != 0;
+        E->last_iters = it;
+        ctx->stats.iterations = it;
+        ctx->stats.epsilon = cur;
+        ctx->stats.converged = conv ? 1 : 0;
+        if (!conv)
+            return set_error(MBPOL_ERR_NOT_CONVERGED, "Induced dipoles did not converge:  iterations=%d eps=%g", it,
+                             cur);
+    }
     // energy and forces (calculateElectrostatic, :838-855 direct sum, :2718-2777 PME)
     timer_start(ctx, T_ELEC_ENERGY);
     launch_elec_energy(st, n, pos, S, P, R, mu_in, E->fsite.as<double>(), E->V.as<double>(), E->eblk.as<double>(),
@@ -992,9 +1116,24 @@ int elec_evaluate(mbpol_context* ctx, u64* energy) {
     if (P.redistribute) launch_charge_deriv_forces(st, E->nw, E->dq.as<double>(), E->V.as<double>(), E->fsite.as<double>());
     launch_add_forces_fix(st, n, E->fsite.as<double>(), ctx->force_fx.as<u64>());
     timer_stop(ctx, T_ELEC_ENERGY);
-    // converged dipoles kept for the next evaluation's "previous" start
-    HIP_TRY(hipMemcpyAsync(E->mu_prev.p, mu_in, mbytes, hipMemcpyDeviceToDevice, st));
-    E->have_previous = true;
+    // history of the learned previous-step guess: what was the last evaluation's becomes
+    // the one before it
+    if (E->history) {
+        if (E->have_previous) {
+            HIP_TRY(hipMemcpyAsync(E->mu_prev2.p, E->mu_prev.p, mbytes, hipMemcpyDeviceToDevice, st));
+            HIP_TRY(hipMemcpyAsync(E->efix_prev2.p, E->efix_prev.p, mbytes, hipMemcpyDeviceToDevice, st));
+            E->have_previous2 = true;
+        }
+        HIP_TRY(hipMemcpyAsync(E->efix_prev.p, E->efix_pol.p, mbytes, hipMemcpyDeviceToDevice, st));
+    }
+    // converged dipoles kept for the next evaluation's "previous" start (a
+    // fixed-length solve sets the flag once its convergence has been read)
+    if (fixed_steps <= 0) {
+        HIP_TRY(hipMemcpyAsync(E->mu_prev.p, mu_in, mbytes, hipMemcpyDeviceToDevice, st));
+        E->have_previous = true;
+    } else {
+        launch_copy_if_converged(st, 3LL * n, E->ctl.as<SolveCtl>(), mu_in, E->mu_prev.as<double>());
+    }
     return MBPOL_OK;
 }
 
@@ -1008,19 +1147,20 @@ int mbpol_context_get_dipoles(const mbpol_context* ctx, double* dipoles) {
     using namespace mbpol;
     if (!ctx->elec) return set_error(MBPOL_ERR_INVALID_ARGUMENT, "no electrostatics in this system");
     HIP_TRY(hipSetDevice(ctx->device));
-    HIP_TRY(hipMemcpy(dipoles, ctx->elec->mu_cur, sizeof(double) * 3 * ctx->elec->n, hipMemcpyDeviceToHost));
+    HIP_TRY(copy_on(ctx->stream, dipoles, ctx->elec->mu_cur, sizeof(double) * 3 * ctx->elec->n));
     return MBPOL_OK;
 }
 
 int mbpol_context_get_charges(const mbpol_context* ctx, double* charges) {
     using namespace mbpol;
     if (!ctx->elec) return set_error(MBPOL_ERR_INVALID_ARGUMENT, "no electrostatics in this system");
-    HIP_TRY(hipMemcpy(charges, ctx->elec->q.p, sizeof(double) * ctx->elec->n, hipMemcpyDeviceToHost));
+    HIP_TRY(copy_on(ctx->stream, charges, ctx->elec->q.p, sizeof(double) * ctx->elec->n));
     return MBPOL_OK;
 }
 
 int mbpol_context_set_dipole_guess(mbpol_context* ctx, int32_t mode, const double* dipoles) {
     using namespace mbpol;
+    if (ctx) ++ctx->graph_gen;
     if (!ctx->elec) return set_error(MBPOL_ERR_INVALID_ARGUMENT, "no electrostatics in this system");
     if (mode < MBPOL_GUESS_ALPHA_E || mode > MBPOL_GUESS_LEARNED)
         return set_error(MBPOL_ERR_INVALID_ARGUMENT, "bad guess mode");
@@ -1034,7 +1174,7 @@ int mbpol_context_set_dipole_guess(mbpol_context* ctx, int32_t mode, const doubl
         for (int i = 0; i < E->n; ++i)
             if (!E->h_pol[i]) g[3 * i] = g[3 * i + 1] = g[3 * i + 2] = 0.0;
         HIP_TRY(hipSetDevice(ctx->device));
-        HIP_TRY(hipMemcpy(E->mu_user.p, g.data(), sizeof(double) * 3 * (size_t)E->n, hipMemcpyHostToDevice));
+        HIP_TRY(copy_on(ctx->stream, E->mu_user.p, g.data(), sizeof(double) * 3 * (size_t)E->n));
     }
     ctx->elec->guess_mode = mode;
     return MBPOL_OK;
@@ -1042,6 +1182,10 @@ int mbpol_context_set_dipole_guess(mbpol_context* ctx, int32_t mode, const doubl
 
 int mbpol_context_set_dipole_convergence(mbpol_context* ctx, double target_eps, int32_t max_iter) {
     using namespace mbpol;
+    if (ctx) {
+        ++ctx->graph_gen;
+        ctx->graph_steps = 0;  // the captured solve's length is taken anew from the next solve
+    }
     if (!ctx) return set_error(MBPOL_ERR_INVALID_ARGUMENT, "null context");
     if (!ctx->elec) return set_error(MBPOL_ERR_INVALID_ARGUMENT, "no electrostatics in this system");
     if (!(target_eps > 0.0) || max_iter < 1)
@@ -1054,6 +1198,30 @@ int mbpol_context_set_dipole_convergence(mbpol_context* ctx, double target_eps, 
 int mbpol_context_get_tuning(const mbpol_context* ctx, const char* name, int32_t* value) {
     using namespace mbpol;
     if (!ctx || !name || !value) return set_error(MBPOL_ERR_INVALID_ARGUMENT, "null argument");
+    // launch shapes and capacities of the whole evaluation, and the graph's counters
+    const struct {
+        const char* name;
+        long long v;
+    } own[] = {{"pip_waves", ctx->pip_waves},
+               {"pip3_waves", ctx->pip3_waves},
+               {"disp_waves", ctx->disp_waves},
+               {"tune_evaluations", ctx->tune_evaluations},
+               {"pip_blocks", ctx->pip_blocks},
+               {"pip3_blocks", ctx->pip3_blocks},
+               {"pip3_kernel", ctx->pip3_kernel},
+               {"disp_blocks", ctx->disp_blocks},
+               {"dimer_capacity", ctx->dimer_cap},
+               {"lower_capacity", ctx->lower_cap},
+               {"trimer_capacity", ctx->trip_cap},
+               {"graph_steps", ctx->graph_steps},
+               {"graph_captures", ctx->graph_captures},
+               {"graph_replays", ctx->graph_replays},
+               {"graph_fallbacks", ctx->graph_fallbacks}};
+    for (const auto& o : own)
+        if (std::strcmp(name, o.name) == 0) {
+            *value = (int32_t)std::min<long long>(o.v, INT32_MAX);
+            return MBPOL_OK;
+        }
     int v = 0;
     if (int e = elec_tuning(ctx, name, &v)) return e;
     *value = v;
@@ -1062,6 +1230,7 @@ int mbpol_context_get_tuning(const mbpol_context* ctx, const char* name, int32_t
 
 int mbpol_context_set_solver(mbpol_context* ctx, int32_t solver) {
     using namespace mbpol;
+    if (ctx) ++ctx->graph_gen;
     if (!ctx) return set_error(MBPOL_ERR_INVALID_ARGUMENT, "null context");
     if (!ctx->elec) return set_error(MBPOL_ERR_INVALID_ARGUMENT, "no electrostatics in this system");
     if (solver != MBPOL_SOLVER_SOR && solver != MBPOL_SOLVER_PCG)
@@ -1079,12 +1248,12 @@ int mbpol_context_get_electrostatic_potential(mbpol_context* ctx, int32_t npoint
     DevBuf pts, out;
     HIP_TRY(pts.ensure(sizeof(double) * 3 * npoints));
     HIP_TRY(out.ensure(sizeof(double) * npoints));
-    HIP_TRY(hipMemcpy(pts.p, points, sizeof(double) * 3 * npoints, hipMemcpyHostToDevice));
+    HIP_TRY(copy_on(ctx->stream, pts.p, points, sizeof(double) * 3 * npoints));
     Elec* E = ctx->elec;
     launch_potential_at_points(ctx->stream, npoints, pts.as<double>(), E->n, ctx->pos.as<double>(), E->q.as<double>(),
                                E->mu_cur, E->P, out.as<double>());
     HIP_TRY(hipStreamSynchronize(ctx->stream));
-    HIP_TRY(hipMemcpy(potential, out.p, sizeof(double) * npoints, hipMemcpyDeviceToHost));
+    HIP_TRY(copy_on(ctx->stream, potential, out.p, sizeof(double) * npoints));
     return MBPOL_OK;
 }
 
@@ -1124,17 +1293,18 @@ int mbpol_context_get_system_electrostatics_moments(mbpol_context* ctx, const do
     DevBuf m, out;
     HIP_TRY(m.ensure(sizeof(double) * E->n));
     HIP_TRY(out.ensure(sizeof(double) * 13));
-    HIP_TRY(hipMemcpy(m.p, masses, sizeof(double) * E->n, hipMemcpyHostToDevice));
+    HIP_TRY(copy_on(ctx->stream, m.p, masses, sizeof(double) * E->n));
     launch_system_moments(ctx->stream, E->n, ctx->pos.as<double>(), E->q.as<double>(), E->mu_cur, m.as<double>(),
                           out.as<double>());
     HIP_TRY(hipStreamSynchronize(ctx->stream));
-    HIP_TRY(hipMemcpy(moments, out.p, sizeof(double) * 13, hipMemcpyDeviceToHost));
+    HIP_TRY(copy_on(ctx->stream, moments, out.p, sizeof(double) * 13));
     return MBPOL_OK;
 }
 
 int mbpol_context_update_electrostatics(mbpol_context* ctx, int32_t n, const double* charge, const int32_t* molecule,
                                         const int32_t* atom_type, const double* damping, const double* polarity) {
     using namespace mbpol;
+    ++ctx->graph_gen;
     if (!ctx->elec) return set_error(MBPOL_ERR_INVALID_ARGUMENT, "no electrostatics in this system");
     if (n != ctx->elec->n)
         return set_error(MBPOL_ERR_COUNT_CHANGED, "updateParametersInContext: The number of multipoles has changed");
@@ -1146,19 +1316,20 @@ int mbpol_context_update_electrostatics(mbpol_context* ctx, int32_t n, const dou
     s.el_pol.assign(polarity, polarity + n);
     HIP_TRY(hipSetDevice(ctx->device));
     Elec* E = ctx->elec;
-    if (int e = up(E->q0, s.el_q)) return e;
-    if (int e = up(E->damp6, sixth_roots(s.el_damp))) return e;
-    if (int e = up(E->pol, s.el_pol)) return e;
-    if (int e = up(E->mol, s.el_mol)) return e;
-    if (int e = up(E->type, s.el_type)) return e;
-    if (int e = set_polarisable(E, s.el_pol, s.el_mol)) return e;
+    if (int e = up(ctx->stream, E->q0, s.el_q)) return e;
+    if (int e = up(ctx->stream, E->damp6, sixth_roots(s.el_damp))) return e;
+    if (int e = up(ctx->stream, E->pol, s.el_pol)) return e;
+    if (int e = up(ctx->stream, E->mol, s.el_mol)) return e;
+    if (int e = up(ctx->stream, E->type, s.el_type)) return e;
+    if (int e = set_polarisable(ctx, E, s.el_pol, s.el_mol)) return e;
     HIP_TRY(E->partial.ensure(sizeof(double) * (iterate_blocks(E->npol) + 1)));
     // the polarisable set may have changed: clear the dipoles the iteration no longer visits
-    HIP_TRY(hipMemset(E->mu_a.p, 0, sizeof(double) * 3 * std::max(n, 1)));
-    HIP_TRY(hipMemset(E->mu_b.p, 0, sizeof(double) * 3 * std::max(n, 1)));
-    HIP_TRY(hipMemset(E->mu_prev.p, 0, sizeof(double) * 3 * std::max(n, 1)));
+    launch_fill_zero(ctx->stream, E->mu_a.p, sizeof(double) * 3 * std::max(n, 1));
+    launch_fill_zero(ctx->stream, E->mu_b.p, sizeof(double) * 3 * std::max(n, 1));
+    launch_fill_zero(ctx->stream, E->mu_prev.p, sizeof(double) * 3 * std::max(n, 1));
     E->mu_cur = E->mu_a.as<double>();
     E->have_previous = false;
+    E->have_previous2 = false;
     if (int e = set_groups(ctx)) return e;
     if (!E->P.pme) set_direct_caps(E);
     return MBPOL_OK;

@@ -53,6 +53,8 @@ struct BrickGeom {
 };
 
 void launch_site_charges_copy(hipStream_t s, int n, const double* q0, double* q);
+// dst = src (n doubles) when ctl records a converged stop, else nothing
+void launch_copy_if_converged(hipStream_t s, long long n, const SolveCtl* ctl, const double* src, double* dst);
 void launch_site_cells(hipStream_t s, int n, const double* pos, ElecParams P, int3 nc, int* cell_of, int* cell_count);
 void launch_bucket_fill(hipStream_t s, int n, const int* bucket_of, const int* bucket_start, int* bucket_fill,
                         int* members);

@@ -24,6 +24,12 @@ using namespace mbpol;
 struct mbpol_guess {
     mbpol_context* ctx = nullptr;
     int nw = 0, natoms = 0, max_snap = 0, nsnap = 0;
+    // MBPOL_GUESS_KIND_ALPHA_E: a correction to alpha E_fixed, anchored on it;
+    // MBPOL_GUESS_KIND_PREVIOUS: a correction to the previous evaluation's dipoles,
+    // anchored on the change of alpha E_fixed since that evaluation
+    int kind = MBPOL_GUESS_KIND_ALPHA_E;
+    DevBuf anchor;  // the anchor vectors of the previous-step kind, per site
+    DevBuf pred;    // a guess computed on request (mbpol_guess_predict_last)
     uint64_t seed = 0;
     // dataset: raw features and targets (R (mu - alpha E_fixed)) of every stored snapshot
     DevBuf feat_all, targ_all;
@@ -70,11 +76,13 @@ namespace {
 
 constexpr double kBoltzmann = 0.0083144626;  // kJ/mol/K
 
-// features and frames of the positions in the context's buffer, on its stream
-int compute_features(mbpol_guess* g) {
+// features of the positions in the context's buffer, and frames along `anchor`
+// (nullptr: alpha E_fixed), on the context's stream
+int compute_features(mbpol_guess* g, const double* anchor = nullptr) {
     mbpol_context* ctx = g->ctx;
     ElecView v;
     if (int e = elec_view(ctx, v)) return e;
+    if (!anchor) anchor = v.efix_pol;
     GuessGeom G{};
     G.periodic = v.periodic;
     G.box = v.box;
@@ -82,7 +90,7 @@ int compute_features(mbpol_guess* g) {
     G.use_cells = v.use_cells;
     G.reach = v.reach;
     G.nc = v.nc;
-    launch_guess_features(ctx->stream, g->nw, ctx->pos.as<double>(), v.type, v.efix_pol, G, v.cell_of, v.cell_start,
+    launch_guess_features(ctx->stream, g->nw, ctx->pos.as<double>(), v.type, anchor, G, v.cell_of, v.cell_start,
                           v.members, g->feat.as<float>(), g->frame.as<float>());
     return MBPOL_OK;
 }
@@ -93,9 +101,9 @@ int sync(mbpol_context* ctx) {
 }
 
 // input means and inverse deviations, and the output scale (rms target), from the stored snapshots
-int set_normalisation(mbpol_guess* g) {
+int set_normalisation(mbpol_guess* g, int nsnap) {
     hipStream_t st = g->ctx->stream;
-    const size_t rows = (size_t)g->nsnap * g->natoms;
+    const size_t rows = (size_t)nsnap * g->natoms;
     const size_t per = 4096;
     const int nb = colsum_blocks(rows, per);
     HIP_TRY(g->colsum.ensure(sizeof(double) * 2 * kGuessFeat * (size_t)std::max(nb, 1)));
@@ -121,8 +129,8 @@ int set_normalisation(mbpol_guess* g) {
     for (int b = 0; b < nb; ++b)
         for (int c = 0; c < 3; ++c) s2 += h[(size_t)(b * 3 + c) * 2 + 1];
     g->scale = (float)std::max(std::sqrt(s2 / (3.0 * rows)), 1e-12);
-    HIP_TRY(hipMemcpy(g->mean.p, mean.data(), sizeof(float) * kGuessFeat, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(g->inv_std.p, istd.data(), sizeof(float) * kGuessFeat, hipMemcpyHostToDevice));
+    HIP_TRY(copy_on(g->ctx->stream, g->mean.p, mean.data(), sizeof(float) * kGuessFeat));
+    HIP_TRY(copy_on(g->ctx->stream, g->inv_std.p, istd.data(), sizeof(float) * kGuessFeat));
     g->normalised = true;
     return MBPOL_OK;
 }
@@ -133,18 +141,39 @@ int set_normalisation(mbpol_guess* g) {
 int mbpol::guess_predict(mbpol_context* ctx, double* mu0) {
     mbpol_guess* g = ctx->guess;
     if (!g) return set_error(MBPOL_ERR_INVALID_ARGUMENT, "no learned guess attached");
-    if (int e = compute_features(g)) return e;
-    launch_mlp_infer(ctx->stream, g->natoms, g->feat.as<float>(), g->view(), g->y.as<float>());
     ElecView v;
     if (int e = elec_view(ctx, v)) return e;
-    launch_guess_apply(ctx->stream, g->nw, g->frame.as<float>(), g->y.as<float>(), g->scale, v.efix_pol, mu0);
+    hipStream_t st = ctx->stream;
+    if (g->kind == MBPOL_GUESS_KIND_PREVIOUS) {
+        if (!v.have_previous) {  // no previous evaluation yet: the reference's start
+            HIP_TRY(hipMemcpyAsync(mu0, v.efix_pol, sizeof(double) * 3 * v.n, hipMemcpyDeviceToDevice, st));
+            return MBPOL_OK;
+        }
+        launch_vec_sub(st, 3 * v.n, v.efix_pol, v.efix_prev, g->anchor.as<double>());
+        if (int e = compute_features(g, g->anchor.as<double>())) return e;
+        launch_mlp_infer(st, g->natoms, g->feat.as<float>(), g->view(), g->y.as<float>());
+        launch_guess_apply(st, g->nw, g->frame.as<float>(), g->y.as<float>(), g->scale, v.mu_prev,
+                           g->anchor.as<double>(), v.efix_pol, mu0);
+        return MBPOL_OK;
+    }
+    if (int e = compute_features(g)) return e;
+    launch_mlp_infer(st, g->natoms, g->feat.as<float>(), g->view(), g->y.as<float>());
+    launch_guess_apply(st, g->nw, g->frame.as<float>(), g->y.as<float>(), g->scale, v.efix_pol, v.efix_pol, nullptr,
+                       mu0);
     return MBPOL_OK;
 }
 
 extern "C" {
 
 int mbpol_guess_create(mbpol_context* ctx, int32_t max_snapshots, uint64_t seed, mbpol_guess** out) {
+    return mbpol_guess_create_kind(ctx, max_snapshots, seed, MBPOL_GUESS_KIND_ALPHA_E, out);
+}
+
+int mbpol_guess_create_kind(mbpol_context* ctx, int32_t max_snapshots, uint64_t seed, int32_t kind,
+                            mbpol_guess** out) {
     if (!ctx || !out || max_snapshots < 0) return set_error(MBPOL_ERR_INVALID_ARGUMENT, "bad argument");
+    if (kind != MBPOL_GUESS_KIND_ALPHA_E && kind != MBPOL_GUESS_KIND_PREVIOUS)
+        return set_error(MBPOL_ERR_INVALID_ARGUMENT, "unknown guess kind %d", kind);
     ElecView v;
     if (int e = elec_view(ctx, v)) return e;
     if (!v.water_layout)
@@ -157,6 +186,7 @@ int mbpol_guess_create(mbpol_context* ctx, int32_t max_snapshots, uint64_t seed,
     g->natoms = 3 * v.nw;
     g->max_snap = max_snapshots;
     g->seed = seed;
+    g->kind = kind;
     auto fail = [&](int code) {
         delete g;
         return code;
@@ -169,8 +199,11 @@ int mbpol_guess_create(mbpol_context* ctx, int32_t max_snapshots, uint64_t seed,
         g->m.ensure(sizeof(float) * kGuessParams) || g->v.ensure(sizeof(float) * kGuessParams) ||
         g->w1t.ensure(2 * kGuessFeat * kGuessHid) || g->w2.ensure(2 * kGuessHid * kGuessHid) ||
         g->w2t.ensure(2 * kGuessHid * kGuessHid) || g->mean.ensure(sizeof(float) * kGuessFeat) ||
-        g->inv_std.ensure(sizeof(float) * kGuessFeat) || g->loss.ensure(sizeof(float) * 4))
+        g->inv_std.ensure(sizeof(float) * kGuessFeat) || g->loss.ensure(sizeof(float) * 4) ||
+        g->anchor.ensure(sizeof(double) * 3 * std::max(v.n, 1)) || g->pred.ensure(sizeof(double) * 3 * std::max(v.n, 1)))
         return fail(set_error(MBPOL_ERR_HIP, "device allocation for the learned guess failed"));
+    // the previous-step kind reads the electrostatics' history of dipoles and fields
+    if (kind == MBPOL_GUESS_KIND_PREVIOUS) elec_keep_history(ctx, true);
     // He-style initial weights (deviation 1/sqrt(fan_in)), zero biases, a small last layer:
     // the untrained network proposes alpha E_fixed plus a small correction
     std::mt19937_64 rng(seed);
@@ -180,11 +213,11 @@ int mbpol_guess_create(mbpol_context* ctx, int32_t max_snapshots, uint64_t seed,
     for (int p = kOffW2; p < kOffB2; ++p) w[p] = (float)(nd(rng) / std::sqrt((double)kGuessHid));
     for (int p = kOffW3; p < kOffB3; ++p) w[p] = (float)(0.1 * nd(rng) / std::sqrt((double)kGuessHid));
     std::vector<float> zero(kGuessParams, 0.f), mean(kGuessFeat, 0.f), one(kGuessFeat, 1.f);
-    if (hipMemcpy(g->w.p, w.data(), sizeof(float) * kGuessParams, hipMemcpyHostToDevice) ||
-        hipMemcpy(g->m.p, zero.data(), sizeof(float) * kGuessParams, hipMemcpyHostToDevice) ||
-        hipMemcpy(g->v.p, zero.data(), sizeof(float) * kGuessParams, hipMemcpyHostToDevice) ||
-        hipMemcpy(g->mean.p, mean.data(), sizeof(float) * kGuessFeat, hipMemcpyHostToDevice) ||
-        hipMemcpy(g->inv_std.p, one.data(), sizeof(float) * kGuessFeat, hipMemcpyHostToDevice))
+    if (copy_on(ctx->stream, g->w.p, w.data(), sizeof(float) * kGuessParams) ||
+        copy_on(ctx->stream, g->m.p, zero.data(), sizeof(float) * kGuessParams) ||
+        copy_on(ctx->stream, g->v.p, zero.data(), sizeof(float) * kGuessParams) ||
+        copy_on(ctx->stream, g->mean.p, mean.data(), sizeof(float) * kGuessFeat) ||
+        copy_on(ctx->stream, g->inv_std.p, one.data(), sizeof(float) * kGuessFeat))
         return fail(set_error(MBPOL_ERR_HIP, "hipMemcpy failed"));
     launch_publish(ctx->stream, g->w.as<float>(), g->view());
     if (hipStreamSynchronize(ctx->stream) != hipSuccess) return fail(set_error(MBPOL_ERR_HIP, "publish failed"));
@@ -198,6 +231,7 @@ void mbpol_guess_destroy(mbpol_guess* g) {
         g->ctx->guess = nullptr;
         if (g->ctx->elec) (void)mbpol_context_set_dipole_guess(g->ctx, MBPOL_GUESS_ALPHA_E, nullptr);
     }
+    if (g->ctx && g->kind == MBPOL_GUESS_KIND_PREVIOUS) elec_keep_history(g->ctx, false);
     (void)hipSetDevice(g->ctx->device);
     (void)hipStreamSynchronize(g->ctx->stream);
     delete g;
@@ -209,15 +243,55 @@ int mbpol_guess_add_snapshot(mbpol_guess* g) {
     mbpol_context* ctx = g->ctx;
     if (!ctx->stats.converged) return set_error(MBPOL_ERR_INVALID_ARGUMENT, "the last evaluate has no converged dipoles");
     HIP_TRY(hipSetDevice(ctx->device));
-    if (int e = compute_features(g)) return e;
     ElecView v;
     if (int e = elec_view(ctx, v)) return e;
     const size_t off = (size_t)g->nsnap * g->natoms;
+    if (g->kind == MBPOL_GUESS_KIND_PREVIOUS) {
+        // the pair (previous evaluation, this one): base = the previous converged dipoles,
+        // anchor = the change of alpha E_fixed between the two
+        if (!v.have_previous2)
+            return set_error(MBPOL_ERR_INVALID_ARGUMENT, "a snapshot of this kind needs two earlier evaluations");
+        launch_vec_sub(ctx->stream, 3 * v.n, v.efix_pol, v.efix_prev2, g->anchor.as<double>());
+        if (int e = compute_features(g, g->anchor.as<double>())) return e;
+        launch_guess_targets(ctx->stream, g->nw, g->frame.as<float>(), v.mu, v.mu_prev2, g->anchor.as<double>(),
+                             v.efix_pol, g->targ_all.as<float>() + off * 3);
+    } else {
+        if (int e = compute_features(g)) return e;
+        launch_guess_targets(ctx->stream, g->nw, g->frame.as<float>(), v.mu, v.efix_pol, v.efix_pol, nullptr,
+                             g->targ_all.as<float>() + off * 3);
+    }
     HIP_TRY(hipMemcpyAsync(g->feat_all.as<float>() + off * kGuessFeat, g->feat.p, sizeof(float) * kGuessFeat * g->natoms,
                            hipMemcpyDeviceToDevice, ctx->stream));
-    launch_guess_targets(ctx->stream, g->nw, g->frame.as<float>(), v.mu, v.efix_pol, g->targ_all.as<float>() + off * 3);
     if (int e = sync(ctx)) return e;
     ++g->nsnap;
+    return MBPOL_OK;
+}
+
+int mbpol_guess_predict_last(mbpol_guess* g, double* dipoles) {
+    if (!g || !dipoles) return set_error(MBPOL_ERR_INVALID_ARGUMENT, "bad argument");
+    if (!g->normalised) return set_error(MBPOL_ERR_INVALID_ARGUMENT, "the network has not been trained");
+    mbpol_context* ctx = g->ctx;
+    HIP_TRY(hipSetDevice(ctx->device));
+    ElecView v;
+    if (int e = elec_view(ctx, v)) return e;
+    hipStream_t st = ctx->stream;
+    double* out = g->pred.as<double>();
+    if (g->kind == MBPOL_GUESS_KIND_PREVIOUS) {
+        if (!v.have_previous2)
+            return set_error(MBPOL_ERR_INVALID_ARGUMENT, "a guess of this kind needs two earlier evaluations");
+        launch_vec_sub(st, 3 * v.n, v.efix_pol, v.efix_prev2, g->anchor.as<double>());
+        if (int e = compute_features(g, g->anchor.as<double>())) return e;
+        launch_mlp_infer(st, g->natoms, g->feat.as<float>(), g->view(), g->y.as<float>());
+        launch_guess_apply(st, g->nw, g->frame.as<float>(), g->y.as<float>(), g->scale, v.mu_prev2,
+                           g->anchor.as<double>(), v.efix_pol, out);
+    } else {
+        if (int e = compute_features(g)) return e;
+        launch_mlp_infer(st, g->natoms, g->feat.as<float>(), g->view(), g->y.as<float>());
+        launch_guess_apply(st, g->nw, g->frame.as<float>(), g->y.as<float>(), g->scale, v.efix_pol, v.efix_pol,
+                           nullptr, out);
+    }
+    if (int e = sync(ctx)) return e;
+    HIP_TRY(copy_on(g->ctx->stream, dipoles, out, sizeof(double) * 3 * v.n));
     return MBPOL_OK;
 }
 
@@ -233,23 +307,27 @@ int mbpol_guess_get_snapshot(const mbpol_guess* g, int32_t k, float* features, f
     HIP_TRY(hipStreamSynchronize(g->ctx->stream));
     const size_t off = (size_t)k * g->natoms;
     if (features)
-        HIP_TRY(hipMemcpy(features, g->feat_all.as<float>() + off * kGuessFeat, sizeof(float) * kGuessFeat * g->natoms,
-                          hipMemcpyDeviceToHost));
+        HIP_TRY(copy_on(g->ctx->stream, features, g->feat_all.as<float>() + off * kGuessFeat, sizeof(float) * kGuessFeat * g->natoms));
     if (targets)
-        HIP_TRY(hipMemcpy(targets, g->targ_all.as<float>() + off * 3, sizeof(float) * 3 * g->natoms,
-                          hipMemcpyDeviceToHost));
+        HIP_TRY(copy_on(g->ctx->stream, targets, g->targ_all.as<float>() + off * 3, sizeof(float) * 3 * g->natoms));
     return MBPOL_OK;
 }
 
 int mbpol_guess_train(mbpol_guess* g, int32_t epochs, int32_t batch, double learning_rate, float* loss_per_epoch) {
+    if (!g) return set_error(MBPOL_ERR_INVALID_ARGUMENT, "null guess");
+    return mbpol_guess_train_range(g, g->nsnap, epochs, batch, learning_rate, loss_per_epoch);
+}
+
+int mbpol_guess_train_range(mbpol_guess* g, int32_t ntrain, int32_t epochs, int32_t batch, double learning_rate,
+                            float* loss_per_epoch) {
     if (!g || epochs < 0 || batch < 32) return set_error(MBPOL_ERR_INVALID_ARGUMENT, "bad argument");
-    if (g->nsnap == 0) return set_error(MBPOL_ERR_INVALID_ARGUMENT, "no snapshots to train on");
+    if (ntrain <= 0 || ntrain > g->nsnap) return set_error(MBPOL_ERR_INVALID_ARGUMENT, "no snapshots to train on");
     mbpol_context* ctx = g->ctx;
     hipStream_t st = ctx->stream;
     HIP_TRY(hipSetDevice(ctx->device));
     if (!g->normalised)
-        if (int e = set_normalisation(g)) return e;
-    const size_t rows = (size_t)g->nsnap * g->natoms;
+        if (int e = set_normalisation(g, ntrain)) return e;
+    const size_t rows = (size_t)ntrain * g->natoms;
     if (rows > (size_t)INT32_MAX) return set_error(MBPOL_ERR_INVALID_ARGUMENT, "too many samples");
     const int waves = mlp_waves(batch);
     HIP_TRY(g->grad.ensure(sizeof(float) * kGuessParams * (size_t)waves));
@@ -280,6 +358,38 @@ int mbpol_guess_train(mbpol_guess* g, int32_t epochs, int32_t batch, double lear
     return MBPOL_OK;
 }
 
+int mbpol_guess_validation_loss(mbpol_guess* g, int32_t first, int32_t count, int32_t batch, double* loss) {
+    if (!g || !loss || first < 0 || count <= 0 || first + count > g->nsnap || batch < 32)
+        return set_error(MBPOL_ERR_INVALID_ARGUMENT, "bad argument");
+    if (!g->normalised) return set_error(MBPOL_ERR_INVALID_ARGUMENT, "the network has not been trained");
+    mbpol_context* ctx = g->ctx;
+    hipStream_t st = ctx->stream;
+    HIP_TRY(hipSetDevice(ctx->device));
+    const size_t r0 = (size_t)first * g->natoms, rows = (size_t)count * g->natoms;
+    if (r0 + rows > (size_t)INT32_MAX) return set_error(MBPOL_ERR_INVALID_ARGUMENT, "too many samples");
+    const int waves = mlp_waves(batch);
+    HIP_TRY(g->grad.ensure(sizeof(float) * kGuessParams * (size_t)waves));
+    HIP_TRY(g->lossp.ensure(sizeof(float) * waves));
+    HIP_TRY(g->idx.ensure(sizeof(int) * rows));
+    std::vector<int> id(rows);
+    std::iota(id.begin(), id.end(), (int)r0);
+    HIP_TRY(hipMemcpyAsync(g->idx.p, id.data(), sizeof(int) * rows, hipMemcpyHostToDevice, st));
+    std::vector<float> hl(waves);
+    double tot = 0;
+    for (size_t b = 0; b < rows; b += (size_t)batch) {
+        const int nb = (int)std::min<size_t>(batch, rows - b);
+        launch_mlp_grad(st, nb, g->idx.as<int>() + b, g->feat_all.as<float>(), g->targ_all.as<float>(), g->view(),
+                        g->grad.as<float>(), g->lossp.as<float>());
+        HIP_TRY(hipMemcpyAsync(hl.data(), g->lossp.p, sizeof(float) * mlp_waves(nb), hipMemcpyDeviceToHost, st));
+        if (int e = sync(ctx)) return e;
+        double s = 0;
+        for (int k = 0; k < mlp_waves(nb); ++k) s += hl[k];
+        tot += s * nb;  // per-batch mean -> sum
+    }
+    *loss = tot / (double)rows;
+    return MBPOL_OK;
+}
+
 int mbpol_guess_attach(mbpol_guess* g, int32_t on) {
     if (!g) return set_error(MBPOL_ERR_INVALID_ARGUMENT, "null guess");
     mbpol_context* ctx = g->ctx;
@@ -295,10 +405,10 @@ int mbpol_guess_get_params(const mbpol_guess* g, float* params, float* norm) {
     if (!g) return set_error(MBPOL_ERR_INVALID_ARGUMENT, "null guess");
     HIP_TRY(hipSetDevice(g->ctx->device));
     HIP_TRY(hipStreamSynchronize(g->ctx->stream));
-    if (params) HIP_TRY(hipMemcpy(params, g->w.p, sizeof(float) * kGuessParams, hipMemcpyDeviceToHost));
+    if (params) HIP_TRY(copy_on(g->ctx->stream, params, g->w.p, sizeof(float) * kGuessParams));
     if (norm) {
-        HIP_TRY(hipMemcpy(norm, g->mean.p, sizeof(float) * kGuessFeat, hipMemcpyDeviceToHost));
-        HIP_TRY(hipMemcpy(norm + kGuessFeat, g->inv_std.p, sizeof(float) * kGuessFeat, hipMemcpyDeviceToHost));
+        HIP_TRY(copy_on(g->ctx->stream, norm, g->mean.p, sizeof(float) * kGuessFeat));
+        HIP_TRY(copy_on(g->ctx->stream, norm + kGuessFeat, g->inv_std.p, sizeof(float) * kGuessFeat));
         norm[2 * kGuessFeat] = g->scale;
     }
     return MBPOL_OK;
@@ -308,11 +418,11 @@ int mbpol_guess_set_params(mbpol_guess* g, const float* params, const float* nor
     if (!g) return set_error(MBPOL_ERR_INVALID_ARGUMENT, "null guess");
     HIP_TRY(hipSetDevice(g->ctx->device));
     HIP_TRY(hipStreamSynchronize(g->ctx->stream));
-    if (params) HIP_TRY(hipMemcpy(g->w.p, params, sizeof(float) * kGuessParams, hipMemcpyHostToDevice));
+    if (params) HIP_TRY(copy_on(g->ctx->stream, g->w.p, params, sizeof(float) * kGuessParams));
     if (norm) {
         if (!(norm[2 * kGuessFeat] > 0.f)) return set_error(MBPOL_ERR_INVALID_ARGUMENT, "output scale must be > 0");
-        HIP_TRY(hipMemcpy(g->mean.p, norm, sizeof(float) * kGuessFeat, hipMemcpyHostToDevice));
-        HIP_TRY(hipMemcpy(g->inv_std.p, norm + kGuessFeat, sizeof(float) * kGuessFeat, hipMemcpyHostToDevice));
+        HIP_TRY(copy_on(g->ctx->stream, g->mean.p, norm, sizeof(float) * kGuessFeat));
+        HIP_TRY(copy_on(g->ctx->stream, g->inv_std.p, norm + kGuessFeat, sizeof(float) * kGuessFeat));
         g->scale = norm[2 * kGuessFeat];
         g->normalised = true;
     }
@@ -333,9 +443,9 @@ int mbpol_guess_gradient(mbpol_guess* g, int32_t n, const float* x, const float*
     HIP_TRY(dl.ensure(sizeof(float) * waves));
     std::vector<int> id(n);
     std::iota(id.begin(), id.end(), 0);
-    HIP_TRY(hipMemcpy(dx.p, x, sizeof(float) * kGuessFeat * n, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(dt.p, t, sizeof(float) * 3 * n, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(di.p, id.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+    HIP_TRY(copy_on(g->ctx->stream, dx.p, x, sizeof(float) * kGuessFeat * n));
+    HIP_TRY(copy_on(g->ctx->stream, dt.p, t, sizeof(float) * 3 * n));
+    HIP_TRY(copy_on(g->ctx->stream, di.p, id.data(), sizeof(int) * n));
     launch_mlp_grad(st, n, di.as<int>(), dx.as<float>(), dt.as<float>(), g->view(), dg.as<float>(), dl.as<float>());
     std::vector<float> hg((size_t)kGuessParams * waves), hl(waves);
     HIP_TRY(hipMemcpyAsync(hg.data(), dg.p, sizeof(float) * hg.size(), hipMemcpyDeviceToHost, st));
@@ -360,10 +470,10 @@ int mbpol_guess_forward(mbpol_guess* g, int32_t n, const float* x, float* y) {
     DevBuf dx, dy;
     HIP_TRY(dx.ensure(sizeof(float) * kGuessFeat * n));
     HIP_TRY(dy.ensure(sizeof(float) * 3 * n));
-    HIP_TRY(hipMemcpy(dx.p, x, sizeof(float) * kGuessFeat * n, hipMemcpyHostToDevice));
+    HIP_TRY(copy_on(g->ctx->stream, dx.p, x, sizeof(float) * kGuessFeat * n));
     launch_mlp_infer(g->ctx->stream, n, dx.as<float>(), g->view(), dy.as<float>());
     if (int e = sync(g->ctx)) return e;
-    HIP_TRY(hipMemcpy(y, dy.p, sizeof(float) * 3 * n, hipMemcpyDeviceToHost));
+    HIP_TRY(copy_on(g->ctx->stream, y, dy.p, sizeof(float) * 3 * n));
     return MBPOL_OK;
 }
 
@@ -372,8 +482,8 @@ int mbpol_guess_features(mbpol_guess* g, float* features, float* frames) {
     HIP_TRY(hipSetDevice(g->ctx->device));
     if (int e = compute_features(g)) return e;
     if (int e = sync(g->ctx)) return e;
-    if (features) HIP_TRY(hipMemcpy(features, g->feat.p, sizeof(float) * kGuessFeat * g->natoms, hipMemcpyDeviceToHost));
-    if (frames) HIP_TRY(hipMemcpy(frames, g->frame.p, sizeof(float) * 9 * g->natoms, hipMemcpyDeviceToHost));
+    if (features) HIP_TRY(copy_on(g->ctx->stream, features, g->feat.p, sizeof(float) * kGuessFeat * g->natoms));
+    if (frames) HIP_TRY(copy_on(g->ctx->stream, frames, g->frame.p, sizeof(float) * 9 * g->natoms));
     return MBPOL_OK;
 }
 
@@ -411,8 +521,8 @@ int mbpol_md_create(mbpol_context* ctx, const double* masses, double temperature
     if (md->inv_mass.ensure(sizeof(double) * std::max(n, 1)) || md->vel.ensure(sizeof(double) * 3 * std::max(n, 1)) ||
         md->ke.ensure(sizeof(double)))
         return fail(set_error(MBPOL_ERR_HIP, "device allocation for the dynamics failed"));
-    if (hipMemcpy(md->inv_mass.p, im.data(), sizeof(double) * n, hipMemcpyHostToDevice) ||
-        hipMemcpy(md->vel.p, v.data(), sizeof(double) * 3 * n, hipMemcpyHostToDevice))
+    if (copy_on(md->ctx->stream, md->inv_mass.p, im.data(), sizeof(double) * n) ||
+        copy_on(md->ctx->stream, md->vel.p, v.data(), sizeof(double) * 3 * n))
         return fail(set_error(MBPOL_ERR_HIP, "hipMemcpy failed"));
     if (int e = mbpol_context_evaluate_resident(ctx, md->last_e, components)) return fail(e);
     *out = md;
@@ -455,7 +565,7 @@ int mbpol_md_get_positions(const mbpol_md* md, double* positions) {
     if (!md || !positions) return set_error(MBPOL_ERR_INVALID_ARGUMENT, "null argument");
     HIP_TRY(hipSetDevice(md->ctx->device));
     HIP_TRY(hipStreamSynchronize(md->ctx->stream));
-    HIP_TRY(hipMemcpy(positions, md->ctx->pos_resident.p, sizeof(double) * 3 * md->n, hipMemcpyDeviceToHost));
+    HIP_TRY(copy_on(md->ctx->stream, positions, md->ctx->pos_resident.p, sizeof(double) * 3 * md->n));
     return MBPOL_OK;
 }
 

@@ -49,13 +49,17 @@ struct GuessGeom {
     int3 nc;
 };
 
-void launch_guess_features(hipStream_t s, int nw, const double* pos, const int* type, const double* efix_pol,
+// features of every polarisable atom, and its frame (e1 along the anchor vector of its site)
+void launch_guess_features(hipStream_t s, int nw, const double* pos, const int* type, const double* anchor,
                            GuessGeom G, const int* cell_of, const int* cell_start, const int* members, float* feat,
                            float* frame);
-void launch_guess_targets(hipStream_t s, int nw, const float* frame, const double* mu, const double* efix_pol,
-                          float* targ);
-void launch_guess_apply(hipStream_t s, int nw, const float* frame, const float* y, float scale, const double* efix_pol,
-                        double* mu0);
+// targets R (mu - base) / |anchor| and guesses base + |anchor| scale R^T y; ref (optional): the
+// units |anchor| are at least 1% of |ref|
+void launch_guess_targets(hipStream_t s, int nw, const float* frame, const double* mu, const double* base,
+                          const double* anchor, const double* ref, float* targ);
+void launch_guess_apply(hipStream_t s, int nw, const float* frame, const float* y, float scale, const double* base,
+                        const double* anchor, const double* ref, double* mu0);
+void launch_vec_sub(hipStream_t s, int n3, const double* a, const double* b, double* out);
 int mlp_waves(int nrows);
 void launch_mlp_infer(hipStream_t s, int nrows, const float* feat, const GuessWeights& W, float* y);
 void launch_mlp_grad(hipStream_t s, int nrows, const int* idx, const float* feat, const float* targ,

@@ -152,6 +152,15 @@ __global__ void k_site_charges_copy(int n, const double* __restrict__ q0, double
     if (i < n) q[i] = q0[i];
 }
 
+// a copy that happens only when the solve stopped converged (the fixed-length
+// solve of a captured evaluation keeps the previous dipoles otherwise)
+__global__ void k_copy_if_converged(long long n, const SolveCtl* __restrict__ ctl, const double* __restrict__ src,
+                                    double* __restrict__ dst) {
+    if (!(ctl->stop && ctl->converged)) return;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
 __global__ void k_site_cells(int n, const double* __restrict__ pos, ElecParams P, int3 nc, int* __restrict__ cell_of,
                              int* __restrict__ cell_count) {
     critical_path_wave();
@@ -1636,6 +1645,11 @@ static inline int nblk(long long n, int b) { return (int)((n + b - 1) / b); }
 constexpr int kG = 16;   // lanes per site: prefactors and fixed field, energy
 constexpr int kGI = 32;  // lanes per site in the iteration (>= 25 for the PME gather)
 
+void launch_copy_if_converged(hipStream_t s, long long n, const SolveCtl* ctl, const double* src, double* dst) {
+    if (n > 0)
+        hipLaunchKernelGGL(k_copy_if_converged, (int)std::min<long long>((n + 255) / 256, 1024), 256, 0, s, n, ctl, src,
+                           dst);
+}
 void launch_site_charges_copy(hipStream_t s, int n, const double* q0, double* q) {
     hipLaunchKernelGGL(k_site_charges_copy, nblk(n, 256), 256, 0, s, n, q0, q);
 }

@@ -7,11 +7,14 @@
 // the electrostatics layout), 48 radial features -- 16 Gaussians on
 // 0.08..0.6 nm times 3 neighbour channels (O, H, M sites), with a cosine cutoff
 // at 0.6 nm -- pass through a 48 -> 64 -> 64 -> 3 network with SiLU
-// activations.  The 3 outputs are the correction mu - alpha E_fixed relative to
-// |alpha E_fixed| (at least kGuessFloor), in a frame of the atom: e1 along its
-// own alpha E_fixed, e2 along the part of a molecular direction orthogonal to
-// e1 (the HOH bisector for O, O -> H for an H), e3 = e1 x e2; in units of a
-// scale fixed at training time.  Mutual induction mostly amplifies the fixed-
+// activations.  The 3 outputs are a correction to a base guess relative to the
+// length of an anchor vector, in a frame of the atom: e1 along the anchor, e2
+// along the part of a molecular direction orthogonal to e1 (the HOH bisector
+// for O, O -> H for an H), e3 = e1 x e2; in units of a scale fixed at training
+// time.  From alpha E_fixed: base = anchor = alpha E_fixed, i.e. the outputs are
+// mu - alpha E_fixed relative to |alpha E_fixed| (at least kGuessFloor).  As a
+// correction to the previous step's dipoles: base = those dipoles, anchor =
+// the change of alpha E_fixed since that step.  Mutual induction mostly amplifies the fixed-
 // field dipole by a factor that depends on the surroundings, which is what the
 // radial features carry.  The matrix products run on the BF16 MFMA units
 // (v_mfma_f32_32x32x16_bf16) with FP32 accumulation; biases, activations and
@@ -50,7 +53,7 @@ __device__ __forceinline__ void mimage(double& dx, double& dy, double& dz, const
 constexpr int kGuessLanes = 16;
 
 __global__ void __launch_bounds__(256) k_guess_features(int nw, const double* __restrict__ pos, const int* __restrict__ type,
-                                                        const double* __restrict__ efix_pol, GuessGeom G,
+                                                        const double* __restrict__ anchor, GuessGeom G,
                                                         const int* __restrict__ cell_of,
                                                         const int* __restrict__ cell_start,
                                                         const int* __restrict__ members, float* __restrict__ feat,
@@ -115,7 +118,7 @@ __global__ void __launch_bounds__(256) k_guess_features(int nw, const double* __
 #pragma unroll
             for (int q = 0; q < kGuessRbf; ++q) f[c * kGuessRbf + q] = acc[c][q];
     } else if (lane == 1) {
-        // the atom's frame: e1 along alpha E_fixed, e2 from a molecular direction
+        // the atom's frame: e1 along the anchor vector, e2 from a molecular direction
         const int o = 4 * w;
         double ref[3];
         if (k == 0) {
@@ -132,7 +135,7 @@ __global__ void __launch_bounds__(256) k_guess_features(int nw, const double* __
             ref[2] = pos[3 * s + 2] - pos[3 * o + 2];
             mimage(ref[0], ref[1], ref[2], G);
         }
-        double e1[3] = {efix_pol[3 * s], efix_pol[3 * s + 1], efix_pol[3 * s + 2]}, e2[3], e3[3];
+        double e1[3] = {anchor[3 * s], anchor[3 * s + 1], anchor[3 * s + 2]}, e2[3], e3[3];
         double nrm = sqrt(e1[0] * e1[0] + e1[1] * e1[1] + e1[2] * e1[2]);
         if (!(nrm > 1e-12)) {  // no field: the molecular direction takes its place
             for (int d = 0; d < 3; ++d) e1[d] = ref[d];
@@ -162,28 +165,40 @@ __global__ void __launch_bounds__(256) k_guess_features(int nw, const double* __
     }
 }
 
-__device__ __forceinline__ double efix_norm(const double* efix_pol, int s) {
-    const double n = sqrt(efix_pol[3 * s] * efix_pol[3 * s] + efix_pol[3 * s + 1] * efix_pol[3 * s + 1] +
-                          efix_pol[3 * s + 2] * efix_pol[3 * s + 2]);
-    return fmax(n, kGuessFloor);
+// Length of the anchor vector of site s, the unit of the network's outputs:
+// at least kGuessFloor, and, when a reference is given (the learned correction
+// to the previous step's dipoles, anchored on the change of alpha E_fixed), at
+// least 1% of the reference's length (alpha E_fixed now).
+__device__ __forceinline__ double anchor_norm(const double* a, const double* ref, int s) {
+    const double n = sqrt(a[3 * s] * a[3 * s] + a[3 * s + 1] * a[3 * s + 1] + a[3 * s + 2] * a[3 * s + 2]);
+    double lo = kGuessFloor;
+    if (ref) {
+        const double r = sqrt(ref[3 * s] * ref[3 * s] + ref[3 * s + 1] * ref[3 * s + 1] + ref[3 * s + 2] * ref[3 * s + 2]);
+        lo = 0.01 * r;
+    }
+    return fmax(n, lo);
 }
 
-// training targets: R (mu - alpha E_fixed) / |alpha E_fixed| per atom
+// training targets: R (mu - base) / |anchor| per atom (base = anchor = alpha E_fixed for the
+// guess from alpha E_fixed; base = the previous step's dipoles, anchor = the change of alpha
+// E_fixed since then, for the correction to the previous step)
 __global__ void k_guess_targets(int nw, const float* __restrict__ frame, const double* __restrict__ mu,
-                                const double* __restrict__ efix_pol, float* __restrict__ targ) {
+                                const double* __restrict__ base, const double* __restrict__ anchor,
+                                const double* __restrict__ ref, float* __restrict__ targ) {
     const int a = blockIdx.x * blockDim.x + threadIdx.x;
     if (a >= 3 * nw) return;
     const int s = 4 * (a / 3) + a % 3;
-    const double inv = 1.0 / efix_norm(efix_pol, s);
-    const double d[3] = {(mu[3 * s] - efix_pol[3 * s]) * inv, (mu[3 * s + 1] - efix_pol[3 * s + 1]) * inv,
-                         (mu[3 * s + 2] - efix_pol[3 * s + 2]) * inv};
+    const double inv = 1.0 / anchor_norm(anchor, ref, s);
+    const double d[3] = {(mu[3 * s] - base[3 * s]) * inv, (mu[3 * s + 1] - base[3 * s + 1]) * inv,
+                         (mu[3 * s + 2] - base[3 * s + 2]) * inv};
     const float* R = frame + (size_t)a * 9;
     for (int r = 0; r < 3; ++r) targ[(size_t)a * 3 + r] = (float)(R[3 * r] * d[0] + R[3 * r + 1] * d[1] + R[3 * r + 2] * d[2]);
 }
 
-// mu0 = alpha E_fixed + |alpha E_fixed| scale R^T y on the polarisable atoms of every water; other sites 0
+// mu0 = base + |anchor| scale R^T y on the polarisable atoms of every water; other sites 0
 __global__ void k_guess_apply(int nw, const float* __restrict__ frame, const float* __restrict__ y, float scale,
-                              const double* __restrict__ efix_pol, double* __restrict__ mu0) {
+                              const double* __restrict__ base, const double* __restrict__ anchor,
+                              const double* __restrict__ ref, double* __restrict__ mu0) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= 4 * nw) return;
     const int w = s / 4, k = s % 4;
@@ -194,10 +209,16 @@ __global__ void k_guess_apply(int nw, const float* __restrict__ frame, const flo
     const int a = 3 * w + k;
     const float* R = frame + (size_t)a * 9;
     const float* ya = y + (size_t)a * 3;
-    const double f = (double)scale * efix_norm(efix_pol, s);
+    const double f = (double)scale * anchor_norm(anchor, ref, s);
     for (int d = 0; d < 3; ++d)
-        mu0[3 * s + d] = efix_pol[3 * s + d] + f * ((double)R[d] * ya[0] + (double)R[3 + d] * ya[1] +
-                                                    (double)R[6 + d] * ya[2]);
+        mu0[3 * s + d] = base[3 * s + d] + f * ((double)R[d] * ya[0] + (double)R[3 + d] * ya[1] +
+                                                (double)R[6 + d] * ya[2]);
+}
+
+// out = a - b, n3 values
+__global__ void k_vec_sub(int n3, const double* __restrict__ a, const double* __restrict__ b, double* __restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n3) out[i] = a[i] - b[i];
 }
 
 // ---------------------------------------------------------------------------
@@ -534,20 +555,25 @@ __global__ void k_md_kinetic(int n3, const double* __restrict__ v, const double*
 
 static inline int nblk(long long n, int b) { return (int)((n + b - 1) / b); }
 
-void launch_guess_features(hipStream_t s, int nw, const double* pos, const int* type, const double* efix_pol,
+void launch_guess_features(hipStream_t s, int nw, const double* pos, const int* type, const double* anchor,
                            GuessGeom G, const int* cell_of, const int* cell_start, const int* members, float* feat,
                            float* frame) {
     if (nw > 0)
         hipLaunchKernelGGL(k_guess_features, nblk((long long)3 * nw * kGuessLanes, 256), 256, 0, s, nw, pos, type,
-                           efix_pol, G, cell_of, cell_start, members, feat, frame);
+                           anchor, G, cell_of, cell_start, members, feat, frame);
 }
-void launch_guess_targets(hipStream_t s, int nw, const float* frame, const double* mu, const double* efix_pol,
-                          float* targ) {
-    if (nw > 0) hipLaunchKernelGGL(k_guess_targets, nblk(3 * nw, 256), 256, 0, s, nw, frame, mu, efix_pol, targ);
+void launch_guess_targets(hipStream_t s, int nw, const float* frame, const double* mu, const double* base,
+                          const double* anchor, const double* ref, float* targ) {
+    if (nw > 0)
+        hipLaunchKernelGGL(k_guess_targets, nblk(3 * nw, 256), 256, 0, s, nw, frame, mu, base, anchor, ref, targ);
 }
-void launch_guess_apply(hipStream_t s, int nw, const float* frame, const float* y, float scale, const double* efix_pol,
-                        double* mu0) {
-    if (nw > 0) hipLaunchKernelGGL(k_guess_apply, nblk(4 * nw, 256), 256, 0, s, nw, frame, y, scale, efix_pol, mu0);
+void launch_guess_apply(hipStream_t s, int nw, const float* frame, const float* y, float scale, const double* base,
+                        const double* anchor, const double* ref, double* mu0) {
+    if (nw > 0)
+        hipLaunchKernelGGL(k_guess_apply, nblk(4 * nw, 256), 256, 0, s, nw, frame, y, scale, base, anchor, ref, mu0);
+}
+void launch_vec_sub(hipStream_t s, int n3, const double* a, const double* b, double* out) {
+    if (n3 > 0) hipLaunchKernelGGL(k_vec_sub, nblk(n3, 256), 256, 0, s, n3, a, b, out);
 }
 int mlp_waves(int nrows) { return nblk(nrows, 32 * kWaves) * kWaves; }
 void launch_mlp_infer(hipStream_t s, int nrows, const float* feat, const GuessWeights& W, float* y) {

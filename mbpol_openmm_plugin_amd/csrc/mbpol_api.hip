@@ -2,6 +2,8 @@
 // context set-up and the evaluation driver that sequences the kernels of one
 // energy+force evaluation on the context's stream.
 #include <algorithm>
+#include <chrono>
+#include <vector>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -47,8 +49,11 @@ DevBuf::~DevBuf() {
     if (p) (void)hipFree(p);
 }
 
+thread_local bool mbpol::g_capturing = false;
+
 hipError_t DevBuf::ensure(size_t nbytes) {
     if (nbytes <= bytes && p) return hipSuccess;
+    if (mbpol::g_capturing) return hipErrorStreamCaptureUnsupported;
     if (p) {
         hipError_t e = hipFree(p);
         if (e != hipSuccess) return e;
@@ -306,7 +311,7 @@ static bool periodic_box(const mbpol_system& s) { return s.box[0] > 0 && s.box[1
 template <typename T>
 static int upload(mbpol_context* ctx, DevBuf& b, const std::vector<T>& v) {
     HIP_TRY(b.ensure(std::max<size_t>(1, v.size()) * sizeof(T)));
-    if (!v.empty()) HIP_TRY(hipMemcpy(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    if (!v.empty()) HIP_TRY(copy_on(ctx->stream, b.p, v.data(), v.size() * sizeof(T)));
     return MBPOL_OK;
 }
 
@@ -484,6 +489,8 @@ int mbpol_context_create(const mbpol_system* sys, int32_t precision, int32_t dev
             return fail(set_error(MBPOL_ERR_HIP, "hipGetDeviceProperties failed"));
         int waves = 1;
         if (const char* env = std::getenv("MBPOL_HIP_PIP_WAVES")) waves = std::max(0, std::atoi(env));
+        ctx->cus = prop.multiProcessorCount;
+        ctx->pip_waves = waves;
         ctx->pip_blocks = waves * 4 * prop.multiProcessorCount;
         // the 3-body kernel and its grid cap (MBPOL_HIP_3B_WAVES waves per SIMD): the
         // orbit kernel runs one block per 64 trimers (its short-lived blocks leave the
@@ -492,9 +499,11 @@ int mbpol_context_create(const mbpol_system* sys, int32_t precision, int32_t dev
         ctx->pip3_kernel = pip3_kernel_choice();
         int waves3 = ctx->pip3_kernel == 1 ? waves : 0;
         if (const char* env = std::getenv("MBPOL_HIP_3B_WAVES")) waves3 = std::max(0, std::atoi(env));
+        ctx->pip3_waves = waves3;
         ctx->pip3_blocks = waves3 * 4 * prop.multiProcessorCount;
         int dwaves = 2;  // the dispersion kernel: two waves per SIMD (MBPOL_HIP_DISP_WAVES)
         if (const char* env = std::getenv("MBPOL_HIP_DISP_WAVES")) dwaves = std::max(0, std::atoi(env));
+        ctx->disp_waves = dwaves;
         ctx->disp_blocks = dwaves * 4 * prop.multiProcessorCount;
     }
     const int n = sys->n;
@@ -550,6 +559,7 @@ void mbpol_context_destroy(mbpol_context* ctx) {
     if (ctx->h_counts) (void)hipHostFree(ctx->h_counts);
     if (ctx->h_pos) (void)hipHostFree(ctx->h_pos);
     if (ctx->h_force) (void)hipHostFree(ctx->h_force);
+    if (ctx->graph_exec) (void)hipGraphExecDestroy(ctx->graph_exec);
     if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);
     if (ctx->ev_join) (void)hipEventDestroy(ctx->ev_join);
     if (ctx->side) (void)hipStreamDestroy(ctx->side);
@@ -558,6 +568,7 @@ void mbpol_context_destroy(mbpol_context* ctx) {
 }
 
 int mbpol_context_set_box(mbpol_context* ctx, const double box_nm[3]) {
+    if (ctx) ++ctx->graph_gen;
     mbpol_system s = ctx->sys;
     for (int d = 0; d < 3; ++d) s.box[d] = box_nm[d];
     if (int e = check_box(s)) return e;
@@ -580,6 +591,7 @@ int mbpol_context_set_box(mbpol_context* ctx, const double box_nm[3]) {
 // (MBPolReferenceKernels.cpp:116-127, 401-414, 485-498): same molecule count, new index lists
 static int update_molecules(mbpol_context* ctx, int32_t n, const int32_t* idx, std::vector<int>& host, DevBuf& dev,
                             const char* what, const char* count_msg) {
+    if (ctx) ++ctx->graph_gen;
     if (!ctx) return set_error(MBPOL_ERR_INVALID_ARGUMENT, "null context");
     if (n != (int32_t)(host.size() / 3)) return set_error(MBPOL_ERR_COUNT_CHANGED, "updateParametersInContext: %s", count_msg);
     if (int e = check_molecules(&ctx->sys, n, idx, what)) return e;
@@ -593,7 +605,7 @@ static int update_molecules(mbpol_context* ctx, int32_t n, const int32_t* idx, s
     HIP_TRY(hipSetDevice(ctx->device));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     if (ctx->side) HIP_TRY(hipStreamSynchronize(ctx->side));
-    if (n > 0) HIP_TRY(hipMemcpy(dev.p, host.data(), sizeof(int) * host.size(), hipMemcpyHostToDevice));
+    if (n > 0) HIP_TRY(copy_on(ctx->stream, dev.p, host.data(), sizeof(int) * host.size()));
     return MBPOL_OK;
 }
 
@@ -623,7 +635,20 @@ int mbpol_context_update_threebody(mbpol_context* ctx, int32_t n, const int32_t*
 }
 
 int mbpol_context_set_list_mode(mbpol_context* ctx, int32_t full_chain_list) {
+    ++ctx->graph_gen;
     ctx->full_chain_list = full_chain_list ? 1 : 0;
+    return MBPOL_OK;
+}
+
+int mbpol_context_set_graph(mbpol_context* ctx, int32_t enabled) {
+    if (!ctx) return set_error(MBPOL_ERR_INVALID_ARGUMENT, "null context");
+    ctx->graph_on = enabled != 0;
+    if (!ctx->graph_on && ctx->graph_exec) {
+        HIP_TRY(hipSetDevice(ctx->device));
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        HIP_TRY(hipGraphExecDestroy(ctx->graph_exec));
+        ctx->graph_exec = nullptr;
+    }
     return MBPOL_OK;
 }
 
@@ -830,10 +855,12 @@ static int short_range(mbpol_context* ctx, hipStream_t sr, uint32_t comp) {
     return MBPOL_OK;
 }
 
-// forces_host_out: pinned host buffer that receives the forces with the same
-// final synchronisation as the energies (nullptr: forces stay on the device)
-static int evaluate(mbpol_context* ctx, double* forces_dev_out, double* energies_out, uint32_t comp,
-                    double* forces_host_out = nullptr) {
+// Everything of one evaluation, enqueued on the context's streams, up to the
+// copies to the host of the energies, the list lengths and (forces_host_out, a
+// pinned host buffer) the forces.  fixed_steps > 0 asks for the dipole solve of
+// that many field evaluations with no host read: the form a HIP graph captures.
+static int enqueue_evaluation(mbpol_context* ctx, double* forces_dev_out, uint32_t comp, double* forces_host_out,
+                              int fixed_steps) {
     const mbpol_system& s = ctx->sys;
     hipStream_t st = ctx->stream;
     u64* F = ctx->force_fx.as<u64>();
@@ -853,18 +880,152 @@ static int evaluate(mbpol_context* ctx, double* forces_dev_out, double* energies
     // after the evaluation).  With timing on, everything runs on one stream, so
     // that every timed span is attributable; the first evaluation also runs the
     // launch-shape tuning, which times kernels, and sizes the lists: unforked.
+    // The forked short-range work is enqueued after the electrostatics set-up
+    // (whose kernels keep the GPU busy while the host enqueues it) and before
+    // the dipole solve, whose host reads would otherwise delay it.
     const bool fork = !ctx->timer.enabled && ctx->evaluated;
+    const bool elec = (comp & MBPOL_ELECTROSTATICS) && ctx->elec;
     int rc = MBPOL_OK;
-    if (!fork) {
-        rc = short_range(ctx, st, comp);
-    } else {
+    bool forked = false;
+    auto fork_short_range = [&]() -> int {
         HIP_TRY(hipEventRecord(ctx->ev_fork, st));
         HIP_TRY(hipStreamWaitEvent(ctx->side, ctx->ev_fork, 0));
-        rc = short_range(ctx, ctx->side, comp);
+        forked = true;
+        const int e = short_range(ctx, ctx->side, comp);
+        // recorded also after a failure, so that the join below completes a capture
         HIP_TRY(hipEventRecord(ctx->ev_join, ctx->side));
+        return e;
+    };
+    if (!fork) {
+        rc = short_range(ctx, st, comp);
+        if (!rc && elec) rc = elec_evaluate(ctx, E + 4, nullptr, fixed_steps);
+    } else if (elec) {
+        rc = elec_evaluate(ctx, E + 4, fork_short_range, fixed_steps);
+        if (rc && !forked) {
+            // the solve failed before the short-range work was enqueued: enqueue it now, so that
+            // a re-run finds no half-built lists (and a capture joins its streams)
+            const int e = fork_short_range();
+            (void)e;
+        }
+    } else {
+        rc = fork_short_range();
     }
-    if (!rc && (comp & MBPOL_ELECTROSTATICS) && ctx->elec) rc = elec_evaluate(ctx, E + 4);
-    if (fork) HIP_TRY(hipStreamWaitEvent(st, ctx->ev_join, 0));
+    if (forked) HIP_TRY(hipStreamWaitEvent(st, ctx->ev_join, 0));
+    if (rc) return rc;
+    double* fout = forces_dev_out ? forces_dev_out : ctx->force.as<double>();
+    launch_forces_to_double(st, 3 * s.n, F, fout);
+    launch_fold_vsites(st, (int)s.vs_idx.size() / 4, ctx->vs_idx.as<int>(), ctx->vs_w.as<double>(), fout);
+    u64* he = static_cast<u64*>(ctx->h_counts) + 8;
+    HIP_TRY(hipMemcpyAsync(he, E, sizeof(u64) * 8, hipMemcpyDeviceToHost, st));
+    if (int e = list_lengths_to_host(ctx, st)) return e;
+    if (forces_host_out)
+        HIP_TRY(hipMemcpyAsync(forces_host_out, fout, sizeof(double) * 3 * s.n, hipMemcpyDeviceToHost, st));
+    return MBPOL_OK;
+}
+
+// the five component energies and their total, from the fixed-point sums read back
+static void energies_from_host(const mbpol_context* ctx, double* energies_out) {
+    if (!energies_out) return;
+    const u64* he = static_cast<const u64*>(ctx->h_counts) + 8;
+    double tot = 0;
+    for (int k = 0; k < 5; ++k) {
+        energies_out[k] = (double)(long long)he[k] * kFixInvHost;
+        tot += energies_out[k];
+    }
+    energies_out[5] = tot;
+}
+
+// Evaluation-level tuning of the short-range grid caps.  The 2-body, 3-body and
+// dispersion kernels run beside the electrostatics, so what counts is the
+// whole overlapped evaluation, not a kernel alone: on the first forked
+// evaluation, each cap (waves per SIMD; 0 = one block per 64 units) is tried in
+// turn on trial evaluations of the caller's configuration, timed on the host
+// from the first launch to the final synchronisation (median of three), and
+// kept when 1% faster than the one before it.  Every candidate leaves the
+// results bit for bit the same (fixed-point force and energy sums, per-unit
+// energies summed in list order), so the tuning is invisible but in time.  The
+// trial solves start from alpha E_fixed and the previous dipoles are put back
+// afterwards.  MBPOL_HIP_TUNE=0 turns it off; a cap fixed by its environment
+// variable is not tuned.
+static int evaluate_eager(mbpol_context* ctx, double* forces_dev_out, double* energies_out, uint32_t comp,
+                          double* forces_host_out);
+
+static int tune_overlap(mbpol_context* ctx, uint32_t comp) {
+    ctx->tuned = true;
+    if (const char* env = std::getenv("MBPOL_HIP_TUNE"))
+        if (std::atoi(env) == 0) return MBPOL_OK;
+    const mbpol_system& s = ctx->sys;
+    const bool elec = (comp & MBPOL_ELECTROSTATICS) && ctx->elec;
+    if (elec && !elec_tune_ok(ctx)) return MBPOL_OK;
+    struct Knob {
+        const char* env;
+        bool on;
+        int* waves;
+        int* blocks;
+        std::vector<int> cand;
+    } knobs[] = {
+        {"MBPOL_HIP_PIP_WAVES", (comp & MBPOL_TWOBODY) && !s.tb_idx.empty(), &ctx->pip_waves, &ctx->pip_blocks,
+         {1, 2, 0}},
+        {"MBPOL_HIP_3B_WAVES", (comp & MBPOL_THREEBODY) && !s.th_idx.empty(), &ctx->pip3_waves, &ctx->pip3_blocks,
+         {0, 1, 2}},
+        {"MBPOL_HIP_DISP_WAVES", (comp & MBPOL_DISPERSION) && ctx->disp, &ctx->disp_waves, &ctx->disp_blocks,
+         {2, 1, 0}},
+    };
+    if (elec)
+        if (int e = elec_tune_hold(ctx, true)) return e;
+    hipStream_t st = ctx->stream;
+    bool ok = true;
+    // median host time of three trial evaluations; false when one failed or did not fit
+    auto measure = [&](double& ms) -> bool {
+        double t[3];
+        for (double& ti : t) {
+            const auto t0 = std::chrono::steady_clock::now();
+            const int rc = enqueue_evaluation(ctx, nullptr, comp, nullptr, 0);
+            if (hipStreamSynchronize(st) != hipSuccess || rc != MBPOL_OK) return false;
+            ti = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+            ++ctx->tune_evaluations;
+            bool redo = false, redo_lists = false;
+            if (elec && elec_rows_check(ctx, redo) != MBPOL_OK) return false;
+            if (list_check(ctx, redo_lists) != MBPOL_OK || redo || redo_lists) return false;
+        }
+        std::sort(t, t + 3);
+        ms = t[1];
+        return true;
+    };
+    for (Knob& k : knobs) {
+        if (!ok || !k.on || std::getenv(k.env)) continue;
+        const int start = *k.waves;
+        double best_ms = 0;
+        if (!(ok = measure(best_ms))) break;
+        int best = start;
+        for (int w : k.cand) {
+            if (w == start) continue;
+            *k.waves = w;
+            *k.blocks = w * 4 * ctx->cus;
+            double ms = 0;
+            if (!(ok = measure(ms))) break;
+            if (ms < 0.99 * best_ms) {
+                best_ms = ms;
+                best = w;
+            }
+        }
+        *k.waves = best;
+        *k.blocks = best * 4 * ctx->cus;
+    }
+    (void)hipGetLastError();
+    HIP_TRY(hipStreamSynchronize(st));
+    if (elec)
+        if (int e = elec_tune_hold(ctx, false)) return e;
+    return MBPOL_OK;
+}
+
+static int evaluate_eager(mbpol_context* ctx, double* forces_dev_out, double* energies_out, uint32_t comp,
+                          double* forces_host_out) {
+    hipStream_t st = ctx->stream;
+    const bool fork = !ctx->timer.enabled && ctx->evaluated;
+    if (fork && !ctx->tuned)
+        if (int e = tune_overlap(ctx, comp)) return e;
+    const int rc = enqueue_evaluation(ctx, forces_dev_out, comp, forces_host_out, 0);
     if (rc) {
         // the short-range kernels may still be adding into the accumulators and
         // reading the positions, which a re-run (or the next call) clears and overwrites
@@ -876,17 +1037,9 @@ static int evaluate(mbpol_context* ctx, double* forces_dev_out, double* energies
         if (int e = list_lengths_to_host(ctx, st)) return e;
         HIP_TRY(hipStreamSynchronize(st));
         if (int e = list_check(ctx, redo_lists)) return e;
-        if (redo || redo_lists) return evaluate(ctx, forces_dev_out, energies_out, comp, forces_host_out);
+        if (redo || redo_lists) return evaluate_eager(ctx, forces_dev_out, energies_out, comp, forces_host_out);
         return rc;
     }
-    double* fout = forces_dev_out ? forces_dev_out : ctx->force.as<double>();
-    launch_forces_to_double(st, 3 * s.n, F, fout);
-    launch_fold_vsites(st, (int)s.vs_idx.size() / 4, ctx->vs_idx.as<int>(), ctx->vs_w.as<double>(), fout);
-    u64* he = static_cast<u64*>(ctx->h_counts) + 8;
-    HIP_TRY(hipMemcpyAsync(he, E, sizeof(u64) * 8, hipMemcpyDeviceToHost, st));
-    if (int e = list_lengths_to_host(ctx, st)) return e;
-    if (forces_host_out)
-        HIP_TRY(hipMemcpyAsync(forces_host_out, fout, sizeof(double) * 3 * s.n, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
     {
         // rows or lists that did not fit: grown, and the evaluation runs again (once:
@@ -895,18 +1048,130 @@ static int evaluate(mbpol_context* ctx, double* forces_dev_out, double* energies
         if (ctx->elec)
             if (int e = elec_rows_check(ctx, redo)) return e;
         if (int e = list_check(ctx, redo_lists)) return e;
-        if (redo || redo_lists) return evaluate(ctx, forces_dev_out, energies_out, comp, forces_host_out);
+        if (redo || redo_lists) return evaluate_eager(ctx, forces_dev_out, energies_out, comp, forces_host_out);
     }
     ctx->evaluated = true;
-    if (energies_out) {
-        double tot = 0;
-        for (int k = 0; k < 5; ++k) {
-            energies_out[k] = (double)(long long)he[k] * kFixInvHost;
-            tot += energies_out[k];
-        }
-        energies_out[5] = tot;
-    }
+    if (fork) ctx->fork_ready = true;
+    energies_from_host(ctx, energies_out);
     return MBPOL_OK;
+}
+
+// The host state a captured evaluation was recorded from: the arguments, the
+// parameter generation, the list and row capacities (their buffers' addresses
+// follow them), the solve's length and the electrostatics' guess start.
+static void graph_key(const mbpol_context* ctx, double* forces_dev_out, uint32_t comp, double* forces_host_out,
+                      long long key[16]) {
+    std::memset(key, 0, sizeof(long long) * 16);
+    key[0] = comp;
+    key[1] = (long long)(intptr_t)forces_dev_out;
+    key[2] = (long long)(intptr_t)forces_host_out;
+    key[3] = ctx->graph_gen;
+    key[4] = ctx->dimer_cap;
+    key[5] = ctx->lower_cap;
+    key[6] = ctx->trip_cap;
+    key[7] = ctx->graph_steps;
+    key[8] = (long long)(intptr_t)ctx->pos.p;
+    if ((comp & MBPOL_ELECTROSTATICS) && ctx->elec) elec_graph_key(ctx, key + 9);
+}
+
+static int capture_evaluation(mbpol_context* ctx, double* forces_dev_out, uint32_t comp, double* forces_host_out) {
+    hipStream_t st = ctx->stream;
+    HIP_TRY(hipStreamSynchronize(st));
+    if (ctx->graph_exec) {
+        HIP_TRY(hipGraphExecDestroy(ctx->graph_exec));
+        ctx->graph_exec = nullptr;
+    }
+    HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    mbpol::g_capturing = true;
+    const int rc = enqueue_evaluation(ctx, forces_dev_out, comp, forces_host_out, ctx->graph_steps);
+    mbpol::g_capturing = false;
+    hipGraph_t g = nullptr;
+    const hipError_t ec = hipStreamEndCapture(st, &g);
+    if (rc || ec != hipSuccess || !g) {
+        if (g) (void)hipGraphDestroy(g);
+        (void)hipGetLastError();
+        return rc ? rc : set_error(MBPOL_ERR_HIP, "capture of the evaluation failed: %s", hipGetErrorString(ec));
+    }
+    const hipError_t ei = hipGraphInstantiate(&ctx->graph_exec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ei != hipSuccess) {
+        ctx->graph_exec = nullptr;
+        (void)hipGetLastError();
+        return set_error(MBPOL_ERR_HIP, "instantiation of the evaluation graph failed: %s", hipGetErrorString(ei));
+    }
+    ctx->graph_lists_built = ctx->lists_built;
+    ++ctx->graph_captures;
+    return MBPOL_OK;
+}
+
+// Graph mode: the evaluation is captured once (after an eager, forked one has
+// sized every buffer) and replayed while the host state it was recorded from
+// holds.  After a replay the host runs the eager path's checks: a list or row
+// that overflowed, or a dipole solve that needed more field evaluations than
+// were captured, has the evaluation run again eagerly (and captured anew on the
+// next call).  `used` is false when the evaluation is not one a graph can hold
+// (then nothing has been enqueued).
+// field evaluations of the captured solve: the last solve's + 3, or MBPOL_HIP_GRAPH_STEPS
+static int graph_steps_for(const mbpol_context* ctx) {
+    if (const char* env = std::getenv("MBPOL_HIP_GRAPH_STEPS")) {
+        const int v = std::atoi(env);
+        if (v > 0) return std::min(v, ctx->sys.el_max_iter + 1);
+    }
+    return elec_graph_steps(ctx, 3);
+}
+
+static int evaluate_graph(mbpol_context* ctx, double* forces_dev_out, double* energies_out, uint32_t comp,
+                          double* forces_host_out, bool& used) {
+    used = false;
+    hipStream_t st = ctx->stream;
+    const bool elec = (comp & MBPOL_ELECTROSTATICS) && ctx->elec;
+    if (elec && !elec_graph_ok(ctx)) return MBPOL_OK;
+    if (elec && ctx->graph_steps == 0) ctx->graph_steps = graph_steps_for(ctx);
+    long long key[16];
+    graph_key(ctx, forces_dev_out, comp, forces_host_out, key);
+    if (!ctx->graph_exec || std::memcmp(key, ctx->graph_key, sizeof(key)) != 0) {
+        if (std::memcmp(key, ctx->graph_bad_key, sizeof(key)) == 0) return MBPOL_OK;
+        if (capture_evaluation(ctx, forces_dev_out, comp, forces_host_out) != MBPOL_OK) {
+            // eager from here on for this state; the stream must be idle before it is reused
+            std::memcpy(ctx->graph_bad_key, key, sizeof(key));
+            std::memset(ctx->graph_key, 0, sizeof(key));
+            HIP_TRY(hipStreamSynchronize(st));
+            return MBPOL_OK;
+        }
+        std::memcpy(ctx->graph_key, key, sizeof(key));
+    }
+    used = true;
+    HIP_TRY(hipGraphLaunch(ctx->graph_exec, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    ++ctx->graph_replays;
+    ctx->lists_built = ctx->graph_lists_built;
+    bool redo = false, redo_lists = false, more = false;
+    int rc = MBPOL_OK;
+    if (elec) {
+        rc = elec_graph_finish(ctx, more);
+        if (int e = elec_rows_check(ctx, redo)) return e;
+    }
+    if (int e = list_check(ctx, redo_lists)) return e;
+    if (redo || redo_lists || more) {
+        ++ctx->graph_fallbacks;
+        const int r = evaluate_eager(ctx, forces_dev_out, energies_out, comp, forces_host_out);
+        if (more && elec) ctx->graph_steps = graph_steps_for(ctx);
+        return r;
+    }
+    if (rc) return rc;
+    energies_from_host(ctx, energies_out);
+    return MBPOL_OK;
+}
+
+static int evaluate(mbpol_context* ctx, double* forces_dev_out, double* energies_out, uint32_t comp,
+                    double* forces_host_out = nullptr) {
+    if (ctx->graph_on && ctx->fork_ready && !ctx->timer.enabled) {
+        bool used = false;
+        const int rc = evaluate_graph(ctx, forces_dev_out, energies_out, comp, forces_host_out, used);
+        if (used) return rc;
+        if (rc) return rc;
+    }
+    return evaluate_eager(ctx, forces_dev_out, energies_out, comp, forces_host_out);
 }
 
 extern "C" {
@@ -944,10 +1209,10 @@ int mbpol_context_evaluate_device(mbpol_context* ctx, const double* d_positions,
 int mbpol_context_upload_positions(mbpol_context* ctx, const double* positions) {
     if (!ctx || !positions) return set_error(MBPOL_ERR_INVALID_ARGUMENT, "null argument");
     HIP_TRY(hipSetDevice(ctx->device));
-    HIP_TRY(hipMemcpy(ctx->pos.p, positions, sizeof(double) * 3 * ctx->sys.n, hipMemcpyHostToDevice));
+    HIP_TRY(copy_on(ctx->stream, ctx->pos.p, positions, sizeof(double) * 3 * ctx->sys.n));
     // keep an untouched copy: evaluate() rewrites the virtual-site rows in place
     HIP_TRY(ctx->pos_resident.ensure(sizeof(double) * 3 * ctx->sys.n));
-    HIP_TRY(hipMemcpy(ctx->pos_resident.p, ctx->pos.p, sizeof(double) * 3 * ctx->sys.n, hipMemcpyDeviceToDevice));
+    HIP_TRY(copy_on(ctx->stream, ctx->pos_resident.p, ctx->pos.p, sizeof(double) * 3 * ctx->sys.n));
     return MBPOL_OK;
 }
 
@@ -958,11 +1223,12 @@ int mbpol_context_evaluate_resident(mbpol_context* ctx, double* energies_out, ui
 
 int mbpol_context_download_forces(mbpol_context* ctx, double* forces_out) {
     HIP_TRY(hipSetDevice(ctx->device));
-    HIP_TRY(hipMemcpy(forces_out, ctx->force.p, sizeof(double) * 3 * ctx->sys.n, hipMemcpyDeviceToHost));
+    HIP_TRY(copy_on(ctx->stream, forces_out, ctx->force.p, sizeof(double) * 3 * ctx->sys.n));
     return MBPOL_OK;
 }
 
 int mbpol_context_set_timing(mbpol_context* ctx, int32_t enabled) {
+    ++ctx->graph_gen;
     timer_collect(ctx);
     ctx->timer.enabled = enabled != 0;
     for (int c = 0; c < T_NCAT; ++c) {
@@ -993,7 +1259,7 @@ int mbpol_context_get_group_energies(const mbpol_context* ctx, double* energies)
     if (!ctx->ngroups) return set_error(MBPOL_ERR_INVALID_ARGUMENT, "the system has no groups");
     HIP_TRY(hipSetDevice(ctx->device));
     std::vector<u64> h((size_t)8 * ctx->ngroups);
-    HIP_TRY(hipMemcpy(h.data(), ctx->g_energy.p, sizeof(u64) * h.size(), hipMemcpyDeviceToHost));
+    HIP_TRY(copy_on(ctx->stream, h.data(), ctx->g_energy.p, sizeof(u64) * h.size()));
     for (int g = 0; g < ctx->ngroups; ++g) {
         double tot = 0;
         for (int k = 0; k < 5; ++k) {
@@ -1026,7 +1292,7 @@ int mbpol_context_get_dimer_list(const mbpol_context* ctx, int32_t* pairs, int64
     *count = ctx->stats.dimers;
     if (pairs && capacity > 0 && ctx->stats.dimers > 0) {
         const long long m = std::min<long long>(capacity, ctx->stats.dimers);
-        HIP_TRY(hipMemcpy(pairs, ctx->dimers.p, sizeof(int2) * m, hipMemcpyDeviceToHost));
+        HIP_TRY(copy_on(ctx->stream, pairs, ctx->dimers.p, sizeof(int2) * m));
     }
     return MBPOL_OK;
 }
@@ -1035,7 +1301,7 @@ int mbpol_context_get_trimer_list(const mbpol_context* ctx, int32_t* trip, int64
     *count = ctx->stats.trimers;
     if (trip && capacity > 0 && ctx->stats.trimers > 0) {
         const long long m = std::min<long long>(capacity, ctx->stats.trimers);
-        HIP_TRY(hipMemcpy(trip, ctx->trips.p, sizeof(int) * 3 * m, hipMemcpyDeviceToHost));
+        HIP_TRY(copy_on(ctx->stream, trip, ctx->trips.p, sizeof(int) * 3 * m));
     }
     return MBPOL_OK;
 }

@@ -164,11 +164,15 @@ def test_learned_guess_on_a_trajectory(mb):
     later snapshots from the trained guess and from alpha * E_fixed (the
     reference's start), by SOR.  Checked: the training loss falls well below
     that of a zero output (3 in units of the output scale, one per component);
-    both starts reach the same converged energy; the trained start needs no
-    more iterations than alpha * E_fixed (one of slack).  The iteration counts
-    and the epsilon of each start are printed: the gain is small (about one
-    SOR iteration of ~58; none for the conjugate-gradient solver), because
-    radial features leave most of the direction of the correction open."""
+    both starts reach the same converged energy; the trained start (the
+    network attached to the context, so its guess is made inside the
+    evaluation) is closer to the converged dipoles than alpha * E_fixed (which
+    a network with every parameter zero returns through the same apply path).
+    The iteration counts and each start's epsilon (the reference's measure
+    after one SOR step, a scaled residual) are printed: the gain is small
+    (about one SOR iteration of ~58; none for the conjugate-gradient solver),
+    and the trained start's one-step residual is no smaller, because radial
+    features leave most of the direction of the correction open."""
     ctx, _ = config1_context(mb)
     ctx.setDipoleGuess('previous')
     md = mb.LangevinIntegrator(ctx, 300.0, 10.0, 0.0005, seed=7)
@@ -180,10 +184,14 @@ def test_learned_guess_on_a_trajectory(mb):
     losses = gm.train(40, batch=1024, learning_rate=1e-3)
     print('training loss', losses[0], '->', losses[-1], '(zero output: 3)', flush=True)
     assert losses[-1] < 2.0
+    from mbpol_openmm_plugin_amd.guess_study import rel_error, zero_network
+    gz = zero_network(ctx)
     its = {'alpha_e': [], 'learned': []}
     eps0 = {'alpha_e': [], 'learned': []}
+    err0 = {'alpha_e': [], 'learned': []}
     for _ in range(4):
         md.step(10)
+        starts = {'alpha_e': gz.predictLast(), 'learned': gm.predictLast()}
         e = {}
         ctx.setDipoleSolver('sor')
         for mode in ('alpha_e', 'learned'):
@@ -198,8 +206,16 @@ def test_learned_guess_on_a_trajectory(mb):
             eps0[mode].append(ctx.getStats()['epsilon'])
             ctx.setDipoleConvergence(1e-8, 200)
         ctx.setDipoleSolver('pcg')
+        ctx.setDipoleGuess('alpha_e')
+        ctx.setDipoleConvergence(1e-11, 200)
+        ctx.evaluateResident()
+        mu_star = ctx.getInducedDipoles()
+        ctx.setDipoleConvergence(1e-8, 200)
         ctx.setDipoleGuess('previous')
+        for mode in starts:
+            err0[mode].append(rel_error(starts[mode], mu_star))
         assert abs(e['learned'][5] - e['alpha_e'][5]) <= 1e-7 * abs(e['alpha_e'][5])
     print('SOR iterations alpha_e', its['alpha_e'], 'learned', its['learned'], flush=True)
     print('epsilon of the start alpha_e', eps0['alpha_e'], 'learned', eps0['learned'], flush=True)
-    assert np.mean(its['learned']) <= np.mean(its['alpha_e']) + 1
+    print('error of the start alpha_e', err0['alpha_e'], 'learned', err0['learned'], flush=True)
+    assert np.mean(err0['learned']) < np.mean(err0['alpha_e'])
