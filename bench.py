@@ -64,6 +64,7 @@ def parse():
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--no-fp64-path', action='store_true')
     ap.add_argument('--no-dipole-iterations', action='store_true')
+    ap.add_argument('--no-graph', action='store_true', help='skip the graph-mode leg')
     ap.add_argument('--cpu-config1-evals', type=int, default=1,
                     help='complete oracle evaluations of the 256-water box timed beside the sampled estimate')
     return ap.parse_args()
@@ -321,16 +322,18 @@ def main():
     }
     # the same evaluations replayed from a HIP graph (mbpol_context_set_graph): captured
     # once, checked after every replay; the energies must equal the eager ones bit for bit
-    ctx.setGraph(True)
-    for _ in range(max(args.warmup, 2)):
-        ctx.evaluateResident()
-    ms_graph, e_graph = run_steps(ctx, args.steps)
-    line['graph_path'] = {'value': 1000.0 / ms_graph, 'ms_per_step': ms_graph,
-                          'energies_equal_eager': bool(np.array_equal(e_graph, energies)),
-                          'captured_field_evaluations': ctx.getTuning('graph_steps'),
-                          'captures': ctx.getTuning('graph_captures'), 'replays': ctx.getTuning('graph_replays'),
-                          'fallbacks': ctx.getTuning('graph_fallbacks')}
-    ctx.setGraph(False)
+    if not args.no_graph:
+        ctx.setGraph(True)
+        for _ in range(max(args.warmup, 2)):
+            ctx.evaluateResident()
+        ms_graph, e_graph = run_steps(ctx, args.steps)
+        line['graph_path'] = {'value': 1000.0 / ms_graph, 'ms_per_step': ms_graph,
+                              'energies_equal_eager': bool(np.array_equal(e_graph, energies)),
+                              'captured_field_evaluations': ctx.getTuning('graph_steps'),
+                              'captures': ctx.getTuning('graph_captures'),
+                              'replays': ctx.getTuning('graph_replays'),
+                              'fallbacks': ctx.getTuning('graph_fallbacks')}
+        ctx.setGraph(False)
     # the same evaluation through the host-buffer entry point mbpol_context_evaluate
     # (positions copied in, virtual sites placed on the device, forces copied out)
     import ctypes

@@ -212,7 +212,7 @@ int mbpol_context_set_list_mode(mbpol_context* ctx, int32_t full_chain_list);
  * evaluations are captured in a HIP graph once and replayed while the host state they were captured from
  * holds -- the same arguments, parameters, list and row capacities and guess start; every setter above
  * marks a captured graph stale.  The dipole solve is captured as a fixed number of conjugate-gradient steps
- * (the last solve's + 3; steps after the stopping rule return at once); after each replay the host checks
+ * (the last solve's + 1; steps after the stopping rule return at once); after each replay the host checks
  * the stopping rule, the lists and the rows, and runs the evaluation again eagerly when one of them did not
  * fit (the next call captures anew).  Evaluations a graph cannot hold run eagerly: timing on, the SOR
  * solver, a learned guess or its history, hipFFT transforms, and the first evaluation of a context, which

@@ -5,7 +5,9 @@ one system, mixed precision, energies and forces of every cluster.
 
 Prints one JSON document: batch evaluations per second and clusters per
 second over `--steps` timed evaluations (positions resident on the GPU), the
-per-phase split from HIP events in a second pass, and the counts of the batch.
+same replayed from one captured HIP graph per evaluation (graph mode; its
+energies must equal the eager ones bit for bit), the per-phase split from HIP
+events in a further pass, and the counts of the batch.
 Run:  python -m mbpol_openmm_plugin_amd.cluster_bench
 """
 import argparse
@@ -37,6 +39,20 @@ def main(argv=None):
         ctx.evaluateResident()
     ms = (time.perf_counter() - t0) * 1000.0 / args.steps
     stats = ctx.getStats()
+    e_eager = ctx.evaluateResident()
+    ctx.setGraph(True)
+    for _ in range(2):
+        ctx.evaluateResident()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        e_graph = ctx.evaluateResident()
+    ms_graph = (time.perf_counter() - t0) * 1000.0 / args.steps
+    graph = {'batch_evals_per_s': 1000.0 / ms_graph, 'clusters_per_s': args.count * 1000.0 / ms_graph,
+             'ms_per_batch': ms_graph, 'energies_equal_eager': bool(np.array_equal(e_graph, e_eager)),
+             'captured_field_evaluations': ctx.getTuning('graph_steps'),
+             'captures': ctx.getTuning('graph_captures'), 'replays': ctx.getTuning('graph_replays'),
+             'fallbacks': ctx.getTuning('graph_fallbacks')}
+    ctx.setGraph(False)
     ctx.setTiming(True)
     for _ in range(args.steps):
         ctx.evaluateResident()
@@ -53,6 +69,8 @@ def main(argv=None):
         'clusters_per_s': len(clusters) * 1000.0 / ms,
         'ms_per_batch': ms,
         'steps': args.steps,
+        'graph_mode': graph,
+        'autotuned': {n: ctx.getTuning(n) for n in ('pip_waves', 'pip3_waves', 'disp_waves', 'tune_evaluations')},
         'counts': stats,
         'phase_ms_timed_serially': split,
     }

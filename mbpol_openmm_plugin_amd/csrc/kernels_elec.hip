@@ -1302,8 +1302,9 @@ __global__ void __launch_bounds__(256) k_cg_direction(int npol, const int* __res
     const int g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g < npol) {
         const int i = polidx[g];
+        // the first direction is z itself: p is not read (a fresh buffer may hold NaN, and 0 * NaN is NaN)
 #pragma unroll
-        for (int d = 0; d < 3; ++d) p[3 * i + d] = z[3 * i + d] + beta * p[3 * i + d];
+        for (int d = 0; d < 3; ++d) p[3 * i + d] = k == 0 ? z[3 * i + d] : z[3 * i + d] + beta * p[3 * i + d];
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         rz[k] = s_rz;

@@ -1111,13 +1111,13 @@ static int capture_evaluation(mbpol_context* ctx, double* forces_dev_out, uint32
 // were captured, has the evaluation run again eagerly (and captured anew on the
 // next call).  `used` is false when the evaluation is not one a graph can hold
 // (then nothing has been enqueued).
-// field evaluations of the captured solve: the last solve's + 3, or MBPOL_HIP_GRAPH_STEPS
+// field evaluations of the captured solve: the last solve's + 1, or MBPOL_HIP_GRAPH_STEPS
 static int graph_steps_for(const mbpol_context* ctx) {
     if (const char* env = std::getenv("MBPOL_HIP_GRAPH_STEPS")) {
         const int v = std::atoi(env);
         if (v > 0) return std::min(v, ctx->sys.el_max_iter + 1);
     }
-    return elec_graph_steps(ctx, 3);
+    return elec_graph_steps(ctx, 1);
 }
 
 static int evaluate_graph(mbpol_context* ctx, double* forces_dev_out, double* energies_out, uint32_t comp,

@@ -103,11 +103,11 @@ def test_graph_replays_equal_eager(mb, steps, monkeypatch):
           (caps, reps, falls, graph.getTuning('graph_steps')), flush=True)
     # the first two evaluations are eager (sizing, then the first forked one); the third captures
     assert reps == len(confs) - 2
-    assert caps <= 2
     if steps is None:
-        assert falls == 0
+        # a solve longer than the captured one falls back and is captured anew
+        assert caps <= 1 + falls and falls <= 2
     else:
-        assert falls == reps  # two field evaluations never reach the stopping rule
+        assert caps == 1 and falls == reps  # two field evaluations never reach the stopping rule
     assert eager.getTuning('graph_replays') == 0
 
 

@@ -127,6 +127,24 @@ def test_chain_trimer_list_is_brute_force_chain_set(oracle):
     assert got == want
 
 
+def test_periodic_chain_trimer_list_is_brute_force_chain_set(oracle):
+    """Periodic chain list (ReferenceThreeNeighborList.cpp:14-226) in a box of
+    4 voxels per axis (1.8 nm, rc 0.45 nm): every i > j > k with minimum-image
+    d(i,j) <= rc and d(j,k) <= rc.  Coordinates stay off the box faces, so the
+    voxel-edge rule (a wrapped coordinate equal to the box length) never
+    applies; tests/test_gpu_lists_edges.py covers that case."""
+    rng = np.random.default_rng(13)
+    L, rc, n = 1.8, 0.45, 90
+    o = rng.uniform(1e-3, L - 1e-3, (n, 3))
+    got = sorted(map(tuple, oracle.trimer_list(o, rc, True, [L] * 3).tolist()))
+    dv = o[:, None] - o[None]
+    dv -= L * np.round(dv / L)
+    d = np.linalg.norm(dv, axis=-1)
+    want = sorted((i, j, k) for i in range(n) for j in range(i) for k in range(j) if d[i, j] <= rc and d[j, k] <= rc)
+    assert len(want) > 100
+    assert got == want
+
+
 def test_dimer_list_is_brute_force_pair_set(oracle):
     rng = np.random.default_rng(4)
     o = rng.uniform(0, 2.0, (60, 3))
