@@ -123,9 +123,11 @@ def test_graph_recaptures_after_a_parameter_change(mb):
                 c.setDipoleConvergence(1e-10, 200)
         compare(k, evaluate(eager, p), evaluate(graph, p))
         other_eager_work(mb, graph, L)
-    caps, reps = graph.getTuning('graph_captures'), graph.getTuning('graph_replays')
-    print('captures %d, replays %d' % (caps, reps), flush=True)
-    assert reps == len(confs) - 2 and caps == 2
+    caps, reps, falls = (graph.getTuning(n) for n in ('graph_captures', 'graph_replays', 'graph_fallbacks'))
+    print('captures %d, replays %d, fallbacks %d' % (caps, reps, falls), flush=True)
+    # captured anew after the change; the tighter solve is longer than the captured one
+    # (the last solve's + 1), so its first replay falls back and the next call captures again
+    assert reps == len(confs) - 2 and 2 <= caps <= 2 + falls
 
 
 def test_graph_batched_clusters(mb):
