@@ -311,8 +311,8 @@ def main():
                                % (alpha, grid[0], grid[1], grid[2], args.precision),
                    'waters': 4096, 'box_nm': L, 'precision': args.precision},
         'iterations': stats['iterations'],
-        'autotuned': {n: ctx.getTuning(n) for n in ('site_lanes', 'pip_waves', 'pip3_waves', 'disp_waves',
-                                                     'tune_evaluations')},
+        'autotuned': {n: ctx.getTuning(n) for n in ('site_lanes', 'fork_early', 'pip_waves',
+                                                     'pip3_waves', 'disp_waves', 'tune_evaluations')},
         'dipole_solver': 'sor' if args.precision == 'double' else 'pcg',
         'counts': {k: stats[k] for k in ('num_dimers', 'num_trimers', 'num_elec_pairs', 'num_pol_pairs')},
         'energies_kj_mol': dict(zip(['onebody', 'twobody', 'threebody', 'dispersion', 'electrostatics', 'total'],

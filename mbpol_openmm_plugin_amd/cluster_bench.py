@@ -70,7 +70,8 @@ def main(argv=None):
         'ms_per_batch': ms,
         'steps': args.steps,
         'graph_mode': graph,
-        'autotuned': {n: ctx.getTuning(n) for n in ('pip_waves', 'pip3_waves', 'disp_waves', 'tune_evaluations')},
+        'autotuned': {n: ctx.getTuning(n) for n in ('fork_early', 'pip_waves', 'pip3_waves',
+                                                     'disp_waves', 'tune_evaluations')},
         'counts': stats,
         'phase_ms_timed_serially': split,
     }

@@ -1202,7 +1202,8 @@ int mbpol_context_get_tuning(const mbpol_context* ctx, const char* name, int32_t
     const struct {
         const char* name;
         long long v;
-    } own[] = {{"pip_waves", ctx->pip_waves},
+    } own[] = {{"fork_early", ctx->fork_early},
+               {"pip_waves", ctx->pip_waves},
                {"pip3_waves", ctx->pip3_waves},
                {"disp_waves", ctx->disp_waves},
                {"tune_evaluations", ctx->tune_evaluations},

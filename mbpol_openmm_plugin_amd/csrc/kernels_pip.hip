@@ -380,9 +380,12 @@ __global__ void __launch_bounds__(64) PIP3_OCCUPANCY k_threebody(long long nt, c
         vr R[9];
         d3 roo[3];
         double doo[3];
-        // forces on the first molecule, summed over the wave's runs of trimers that share it
+        // forces on the first and on the second molecule, each summed over the wave's runs of
+        // trimers that share it (the list is ordered by first, then second molecule)
         u64 fa[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        u64 fb[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
         const long long key = t < nt ? (long long)trip[3 * t] : -1;
+        const long long key2 = t < nt ? (long long)trip[3 * t + 1] : -1;
         if (t < nt && trimer_geometry(t, trip, idx, pos, periodic, box, at, R, roo, doo)) {
             real x[36], g[36];
 #pragma unroll
@@ -449,6 +452,10 @@ __global__ void __launch_bounds__(64) PIP3_OCCUPANCY k_threebody(long long nt, c
                     fa[3 * q] = to_fix(fx);
                     fa[3 * q + 1] = to_fix(fy);
                     fa[3 * q + 2] = to_fix(fz);
+                } else if (q < 6) {
+                    fb[3 * (q - 3)] = to_fix(fx);
+                    fb[3 * (q - 3) + 1] = to_fix(fy);
+                    fb[3 * (q - 3) + 2] = to_fix(fz);
                 } else {
                     add_force(force, at[q], fx, fy, fz);
                 }
@@ -461,6 +468,14 @@ __global__ void __launch_bounds__(64) PIP3_OCCUPANCY k_threebody(long long nt, c
                 const int a = idx[3 * key + q];
 #pragma unroll
                 for (int d = 0; d < 3; ++d) atomicAdd(&force[3 * a + d], fa[3 * q + d]);
+            }
+        }
+        if (wave_run_sum(key2, fb) && key2 >= 0) {
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                const int a = idx[3 * key2 + q];
+#pragma unroll
+                for (int d = 0; d < 3; ++d) atomicAdd(&force[3 * a + d], fb[3 * q + d]);
             }
         }
         if (eunit && t < nt) eunit[t] = e_out;

@@ -504,6 +504,7 @@ int mbpol_context_create(const mbpol_system* sys, int32_t precision, int32_t dev
         int dwaves = 2;  // the dispersion kernel: two waves per SIMD (MBPOL_HIP_DISP_WAVES)
         if (const char* env = std::getenv("MBPOL_HIP_DISP_WAVES")) dwaves = std::max(0, std::atoi(env));
         ctx->disp_waves = dwaves;
+        if (const char* env = std::getenv("MBPOL_HIP_FORK")) ctx->fork_early = std::strcmp(env, "late") != 0;
         ctx->disp_blocks = dwaves * 4 * prop.multiProcessorCount;
     }
     const int n = sys->n;
@@ -882,13 +883,17 @@ static int enqueue_evaluation(mbpol_context* ctx, double* forces_dev_out, uint32
     // launch-shape tuning, which times kernels, and sizes the lists: unforked.
     // The forked short-range work is enqueued after the electrostatics set-up
     // (whose kernels keep the GPU busy while the host enqueues it) and before
-    // the dipole solve, whose host reads would otherwise delay it.
+    // the dipole solve, whose host reads would otherwise delay it.  The fork
+    // point on the GPU is either right after the virtual sites are placed
+    // (fork_early: the short-range kernels also run beside the set-up's many
+    // short kernels) or after the set-up; the tuning picks one.
     const bool fork = !ctx->timer.enabled && ctx->evaluated;
     const bool elec = (comp & MBPOL_ELECTROSTATICS) && ctx->elec;
+    if (fork && ctx->fork_early) HIP_TRY(hipEventRecord(ctx->ev_fork, st));
     int rc = MBPOL_OK;
     bool forked = false;
     auto fork_short_range = [&]() -> int {
-        HIP_TRY(hipEventRecord(ctx->ev_fork, st));
+        if (!ctx->fork_early) HIP_TRY(hipEventRecord(ctx->ev_fork, st));
         HIP_TRY(hipStreamWaitEvent(ctx->side, ctx->ev_fork, 0));
         forked = true;
         const int e = short_range(ctx, ctx->side, comp);
@@ -935,18 +940,20 @@ static void energies_from_host(const mbpol_context* ctx, double* energies_out) {
     energies_out[5] = tot;
 }
 
-// Evaluation-level tuning of the short-range grid caps.  The 2-body, 3-body and
-// dispersion kernels run beside the electrostatics, so what counts is the
-// whole overlapped evaluation, not a kernel alone: on the first forked
-// evaluation, each cap (waves per SIMD; 0 = one block per 64 units) is tried in
-// turn on trial evaluations of the caller's configuration, timed on the host
-// from the first launch to the final synchronisation (median of three), and
+// Evaluation-level tuning of the overlap: the fork point of the short-range
+// stream and the short-range grid caps.  The 2-body, 3-body and dispersion
+// kernels run beside the electrostatics, so what counts is the whole
+// overlapped evaluation, not a kernel alone: on the first forked evaluation,
+// the fork point (early or after the electrostatics set-up) and then each cap
+// (waves per SIMD; 0 = one block per 64 units) is tried in turn on trial
+// evaluations of the caller's configuration, timed on the host
+// from the first launch to the final synchronisation (median of five), and
 // kept when 1% faster than the one before it.  Every candidate leaves the
 // results bit for bit the same (fixed-point force and energy sums, per-unit
 // energies summed in list order), so the tuning is invisible but in time.  The
 // trial solves start from alpha E_fixed and the previous dipoles are put back
-// afterwards.  MBPOL_HIP_TUNE=0 turns it off; a cap fixed by its environment
-// variable is not tuned.
+// afterwards.  MBPOL_HIP_TUNE=0 turns it off; a setting fixed by its environment
+// variable (MBPOL_HIP_FORK=early|late, MBPOL_HIP_*_WAVES) is not tuned.
 static int evaluate_eager(mbpol_context* ctx, double* forces_dev_out, double* energies_out, uint32_t comp,
                           double* forces_host_out);
 
@@ -961,9 +968,10 @@ static int tune_overlap(mbpol_context* ctx, uint32_t comp) {
         const char* env;
         bool on;
         int* waves;
-        int* blocks;
+        int* blocks;  // null: the value itself is the setting
         std::vector<int> cand;
     } knobs[] = {
+        {"MBPOL_HIP_FORK", true, &ctx->fork_early, nullptr, {1, 0}},
         {"MBPOL_HIP_PIP_WAVES", (comp & MBPOL_TWOBODY) && !s.tb_idx.empty(), &ctx->pip_waves, &ctx->pip_blocks,
          {1, 2, 0}},
         {"MBPOL_HIP_3B_WAVES", (comp & MBPOL_THREEBODY) && !s.th_idx.empty(), &ctx->pip3_waves, &ctx->pip3_blocks,
@@ -975,9 +983,9 @@ static int tune_overlap(mbpol_context* ctx, uint32_t comp) {
         if (int e = elec_tune_hold(ctx, true)) return e;
     hipStream_t st = ctx->stream;
     bool ok = true;
-    // median host time of three trial evaluations; false when one failed or did not fit
+    // median host time of five trial evaluations; false when one failed or did not fit
     auto measure = [&](double& ms) -> bool {
-        double t[3];
+        double t[5];
         for (double& ti : t) {
             const auto t0 = std::chrono::steady_clock::now();
             const int rc = enqueue_evaluation(ctx, nullptr, comp, nullptr, 0);
@@ -988,8 +996,8 @@ static int tune_overlap(mbpol_context* ctx, uint32_t comp) {
             if (elec && elec_rows_check(ctx, redo) != MBPOL_OK) return false;
             if (list_check(ctx, redo_lists) != MBPOL_OK || redo || redo_lists) return false;
         }
-        std::sort(t, t + 3);
-        ms = t[1];
+        std::sort(t, t + 5);
+        ms = t[2];
         return true;
     };
     for (Knob& k : knobs) {
@@ -1001,7 +1009,7 @@ static int tune_overlap(mbpol_context* ctx, uint32_t comp) {
         for (int w : k.cand) {
             if (w == start) continue;
             *k.waves = w;
-            *k.blocks = w * 4 * ctx->cus;
+            if (k.blocks) *k.blocks = w * 4 * ctx->cus;
             double ms = 0;
             if (!(ok = measure(ms))) break;
             if (ms < 0.99 * best_ms) {
@@ -1010,7 +1018,7 @@ static int tune_overlap(mbpol_context* ctx, uint32_t comp) {
             }
         }
         *k.waves = best;
-        *k.blocks = best * 4 * ctx->cus;
+        if (k.blocks) *k.blocks = best * 4 * ctx->cus;
     }
     (void)hipGetLastError();
     HIP_TRY(hipStreamSynchronize(st));
