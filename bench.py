@@ -40,18 +40,18 @@ PEAK_FP64_TFLOPS = 78.6
 FLOPS_PER_TRIMER = 112.5e3
 FLOPS_PER_DIMER = 31.0e3
 # HBM bytes per launch from rocprofv3 FETCH_SIZE + WRITE_SIZE counters on this
-# workload, calibrated against a plain copy kernel (FETCH_SIZE x 2.0, WRITE_SIZE
-# x 1.0; profiles/memory_counters_4096w_eval.md holds the runs); kernels not
-# listed report null
+# workload, calibrated against the runtime's plain copy kernel in the same runs
+# (FETCH_SIZE x 1.885, WRITE_SIZE x 1.000; the runs are summarised under
+# profiles/); kernels not listed report null
 MEASURED_TRAFFIC = {
-    'elec_iterate': 90978071,
-    'pme_spread': 61716016,
-    'fft_z_r2c': 25782308,
-    'fft_y': 26430314,
-    'fft_x_conv': 39895966,
-    'fft_z_c2r': 26099820,
-    'threebody': 227600009,
-    'twobody': 86314272,
+    'elec_iterate': 85794228,
+    'pme_spread': 58894442,
+    'fft_z_r2c': 25046395,
+    'fft_y': 25657023,
+    'fft_x_conv': 38351649,
+    'fft_z_c2r': 25327521,
+    'threebody': 198903898,
+    'twobody': 67952728,
 }
 
 

@@ -8,8 +8,9 @@ and energy sums, per-unit energies summed in list order).  Checked on the
 previous step's dipoles (so that the trial solves must leave that state as
 they found it):
 
-* contexts with caps fixed to other values by their environment variables
-  equal the untuned context bit for bit (energies, forces, dipoles, iterations);
+* contexts with caps and fork points fixed to other values by their
+  environment variables equal the untuned context bit for bit (energies,
+  forces, dipoles, iterations);
 * the tuned context equals the untuned one bit for bit, ran trial evaluations,
   and reports caps among its candidates.
 """
@@ -40,7 +41,7 @@ def run(mb, precision='mixed'):
         ctx.uploadPositions(pos + rng.normal(scale=0.002, size=pos.shape) * (k > 0))
         e = ctx.evaluateResident()
         out.append((e, ctx.downloadForces(), ctx.getInducedDipoles(), ctx.getStats()['iterations']))
-    caps = {n: ctx.getTuning(n) for n in ('pip_waves', 'pip3_waves', 'disp_waves', 'tune_evaluations')}
+    caps = {n: ctx.getTuning(n) for n in ('fork_early', 'pip_waves', 'pip3_waves', 'disp_waves', 'tune_evaluations')}
     return out, caps
 
 
@@ -57,15 +58,18 @@ def test_caps_change_no_result_and_tuning_is_invisible(mb, monkeypatch):
     ref, caps0 = run(mb)
     print('untuned', caps0, 'energies', ref[-1][0], flush=True)
     assert caps0['tune_evaluations'] == 0
-    for pip, pip3, disp in ((2, 1, 0), (0, 2, 1)):
+    for pip, pip3, disp, fork in ((2, 1, 0, 'late'), (0, 2, 1, 'early')):
         monkeypatch.setenv('MBPOL_HIP_PIP_WAVES', str(pip))
         monkeypatch.setenv('MBPOL_HIP_3B_WAVES', str(pip3))
         monkeypatch.setenv('MBPOL_HIP_DISP_WAVES', str(disp))
+        monkeypatch.setenv('MBPOL_HIP_FORK', fork)
         got, caps = run(mb)
         print('fixed caps', caps, flush=True)
         assert (caps['pip_waves'], caps['pip3_waves'], caps['disp_waves']) == (pip, pip3, disp)
+        assert caps['fork_early'] == (fork == 'early')
         same(ref, got)
-    for name in ('MBPOL_HIP_TUNE', 'MBPOL_HIP_PIP_WAVES', 'MBPOL_HIP_3B_WAVES', 'MBPOL_HIP_DISP_WAVES'):
+    for name in ('MBPOL_HIP_TUNE', 'MBPOL_HIP_PIP_WAVES', 'MBPOL_HIP_3B_WAVES', 'MBPOL_HIP_DISP_WAVES',
+                 'MBPOL_HIP_FORK'):
         monkeypatch.delenv(name)
     got, caps = run(mb)
     print('tuned', caps, flush=True)
