@@ -292,6 +292,34 @@ def main():
                 'algorithmic_' + ('flops' if 'flops_per_launch' in d else 'bytes') + '_per_launch':
                     d.get('flops_per_launch', d.get('bytes_per_launch')),
                 'all_kernels': kernels}
+    if dom == 'threebody':
+        # The evaluation runs the 3-body grid capped (waves per SIMD, tuned) so that it
+        # leaves the electrostatics room; the same kernel at full occupancy (one block per
+        # 64 trimers, two waves per SIMD), timed alone on a second context
+        roofline['production_waves_per_simd'] = ctx.getTuning('pip3_waves')
+        saved = {k: os.environ.get(k) for k in ('MBPOL_HIP_3B_WAVES', 'MBPOL_HIP_TUNE')}
+        os.environ['MBPOL_HIP_3B_WAVES'] = '0'
+        os.environ['MBPOL_HIP_TUNE'] = '0'
+        try:
+            full = m.Context(sysm, precision=args.precision)
+            full.uploadPositions(pos)
+            for _ in range(2):
+                full.evaluateResident()
+            full.setTiming(True)
+            run_steps(full, 5)
+            ms3, cnt3 = full.getTiming('threebody')
+            full.setTiming(False)
+            del full
+        finally:
+            for k, v in saved.items():
+                if v is None:
+                    os.environ.pop(k, None)
+                else:
+                    os.environ[k] = v
+        us3 = 1000.0 * ms3 / cnt3
+        rate3 = d['flops_per_launch'] / (us3 * 1e-6) / 1e12
+        roofline['full_occupancy'] = {'avg_kernel_us': us3, 'achieved': rate3, 'frac': rate3 / d['peak'],
+                                      'waves_per_simd': 'one block per 64 trimers (2 resident per SIMD)'}
     line = {
         'metric': 'MB-pol E+F evals/s @4096 waters PBC; dipole iters vs plain at 1e-6 rel E',
         'value': 1000.0 / ms_per_step,

@@ -124,7 +124,7 @@ struct mbpol_context {
     int pip3_kernel = 0;       // 3-body kernel: 0 the polynomial through its relabelling group, 1 term by term
     int pip3_blocks = 0;       // grid cap of the 3-body kernel (0: one block per 64 units)
     // the caps in waves per SIMD (blocks = waves x 4 x CUs), tuned on the first forked evaluation
-    int pip_waves = 1, pip3_waves = 0, disp_waves = 2, cus = 0;
+    int pip_waves = 1, pip3_waves = 1, disp_waves = 0, cus = 0;
     int fork_early = 1;  // the short-range stream forks right after the virtual sites (else after the set-up)
     bool tuned = false;
     int tune_evaluations = 0;  // trial evaluations the tuning ran

@@ -492,16 +492,16 @@ int mbpol_context_create(const mbpol_system* sys, int32_t precision, int32_t dev
         ctx->cus = prop.multiProcessorCount;
         ctx->pip_waves = waves;
         ctx->pip_blocks = waves * 4 * prop.multiProcessorCount;
-        // the 3-body kernel and its grid cap (MBPOL_HIP_3B_WAVES waves per SIMD): the
-        // orbit kernel runs one block per 64 trimers (its short-lived blocks leave the
-        // electrostatics room as they finish; capped at one or two waves per SIMD the
-        // evaluation was 1-2% slower), the term-by-term kernel one wave per SIMD
+        // the 3-body kernel and its grid cap (MBPOL_HIP_3B_WAVES waves per SIMD): one wave
+        // per SIMD, so that its 256-VGPR waves leave every SIMD room for the electrostatics
+        // kernels (uncapped, they hold every SIMD's registers and the next electrostatics
+        // launch waits for them to retire); tuned with the other caps
         ctx->pip3_kernel = pip3_kernel_choice();
-        int waves3 = ctx->pip3_kernel == 1 ? waves : 0;
+        int waves3 = 1;
         if (const char* env = std::getenv("MBPOL_HIP_3B_WAVES")) waves3 = std::max(0, std::atoi(env));
         ctx->pip3_waves = waves3;
         ctx->pip3_blocks = waves3 * 4 * prop.multiProcessorCount;
-        int dwaves = 2;  // the dispersion kernel: two waves per SIMD (MBPOL_HIP_DISP_WAVES)
+        int dwaves = 0;  // the dispersion kernel: one block per 64 atoms (MBPOL_HIP_DISP_WAVES)
         if (const char* env = std::getenv("MBPOL_HIP_DISP_WAVES")) dwaves = std::max(0, std::atoi(env));
         ctx->disp_waves = dwaves;
         if (const char* env = std::getenv("MBPOL_HIP_FORK")) ctx->fork_early = std::strcmp(env, "late") != 0;
@@ -972,12 +972,12 @@ static int tune_overlap(mbpol_context* ctx, uint32_t comp) {
         std::vector<int> cand;
     } knobs[] = {
         {"MBPOL_HIP_FORK", true, &ctx->fork_early, nullptr, {1, 0}},
-        {"MBPOL_HIP_PIP_WAVES", (comp & MBPOL_TWOBODY) && !s.tb_idx.empty(), &ctx->pip_waves, &ctx->pip_blocks,
-         {1, 2, 0}},
         {"MBPOL_HIP_3B_WAVES", (comp & MBPOL_THREEBODY) && !s.th_idx.empty(), &ctx->pip3_waves, &ctx->pip3_blocks,
-         {0, 1, 2}},
+         {1, 0, 2}},
         {"MBPOL_HIP_DISP_WAVES", (comp & MBPOL_DISPERSION) && ctx->disp, &ctx->disp_waves, &ctx->disp_blocks,
          {2, 1, 0}},
+        {"MBPOL_HIP_PIP_WAVES", (comp & MBPOL_TWOBODY) && !s.tb_idx.empty(), &ctx->pip_waves, &ctx->pip_blocks,
+         {1, 2, 0}},
     };
     if (elec)
         if (int e = elec_tune_hold(ctx, true)) return e;

@@ -58,7 +58,7 @@ def test_caps_change_no_result_and_tuning_is_invisible(mb, monkeypatch):
     ref, caps0 = run(mb)
     print('untuned', caps0, 'energies', ref[-1][0], flush=True)
     assert caps0['tune_evaluations'] == 0
-    for pip, pip3, disp, fork in ((2, 1, 0, 'late'), (0, 2, 1, 'early')):
+    for pip, pip3, disp, fork in ((2, 0, 2, 'late'), (0, 2, 1, 'early')):
         monkeypatch.setenv('MBPOL_HIP_PIP_WAVES', str(pip))
         monkeypatch.setenv('MBPOL_HIP_3B_WAVES', str(pip3))
         monkeypatch.setenv('MBPOL_HIP_DISP_WAVES', str(disp))
