@@ -50,8 +50,8 @@ MEASURED_TRAFFIC = {
     'fft_y': 25657023,
     'fft_x_conv': 38351649,
     'fft_z_c2r': 25327521,
-    'threebody': 198903898,
-    'twobody': 67952728,
+    'threebody': 113528877,
+    'twobody': 58695962,
 }
 
 

@@ -51,6 +51,38 @@ __device__ __forceinline__ bool wave_run_sum(long long key, unsigned long long (
     return lane == 63 || ((heads >> (lane + 1)) & 1ull);
 }
 
+// Sum of N fixed-point values over the lanes of the wave with equal key, in any
+// order of the keys: the (key, lane) pairs are sorted across the wave by a
+// bitonic network of shuffles (lane breaks ties, so the order is total), each
+// lane then takes the values of the lane whose pair it holds, and the runs of
+// equal keys are summed as in wave_run_sum.  Every lane must call it; on return
+// key holds the sorted key of this lane, and the function returns true on the
+// last lane of each run, which holds the run's sums in v.
+template <int N>
+__device__ __forceinline__ bool wave_sorted_run_sum(long long& key, unsigned long long (&v)[N]) {
+    const int lane = threadIdx.x & 63;
+    int src = lane;
+#pragma unroll
+    for (int k = 2; k <= 64; k <<= 1) {
+#pragma unroll
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            const long long pk = __shfl_xor(key, j, 64);
+            const int ps = __shfl_xor(src, j, 64);
+            const bool partner_less = pk < key || (pk == key && ps < src);
+            const bool ascending = (lane & k) == 0, lower = (lane & j) == 0;
+            // the lower lane of a pair keeps the smaller pair in an ascending block, the larger in a
+            // descending one; the upper lane the other
+            if (lower == ascending ? partner_less : !partner_less) {
+                key = pk;
+                src = ps;
+            }
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < N; ++i) v[i] = __shfl(v[i], src, 64);
+    return wave_run_sum(key, v);
+}
+
 // Wave-level sum (64 lanes).
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll

@@ -173,9 +173,12 @@ __global__ void __launch_bounds__(64) k_twobody(long long np, const long long* _
         vr R[10];
         d3 dOO;
         double rOO;
-        // forces on the first molecule, summed over the wave's runs of dimers that share it
+        // forces on the first molecule, summed over the wave's runs of dimers that share it, and
+        // on the second, summed over the wave's dimers that share it after a sort by molecule
         u64 fa[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        u64 fb[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
         const long long key = t < np ? (long long)pairs[t].x : -1;
+        long long key2 = t < np ? (long long)pairs[t].y : -1;
         if (t < np && dimer_geometry(t, pairs, idx, pos, periodic, boxA, at, R, dOO, rOO)) {
             real x[31], g[31];
 #pragma unroll
@@ -242,7 +245,9 @@ __global__ void __launch_bounds__(64) k_twobody(long long np, const long long* _
                     fa[3 * q + 1] = to_fix(fy);
                     fa[3 * q + 2] = to_fix(fz);
                 } else {
-                    add_force(force, at[q], fx, fy, fz);
+                    fb[3 * (q - 3)] = to_fix(fx);
+                    fb[3 * (q - 3) + 1] = to_fix(fy);
+                    fb[3 * (q - 3) + 2] = to_fix(fz);
                 }
             }
             e_out = sw * (double)Ep * cal2joule;
@@ -253,6 +258,14 @@ __global__ void __launch_bounds__(64) k_twobody(long long np, const long long* _
                 const int a = idx[3 * key + q];
 #pragma unroll
                 for (int d = 0; d < 3; ++d) atomicAdd(&force[3 * a + d], fa[3 * q + d]);
+            }
+        }
+        if (wave_sorted_run_sum(key2, fb) && key2 >= 0) {
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                const int a = idx[3 * key2 + q];
+#pragma unroll
+                for (int d = 0; d < 3; ++d) atomicAdd(&force[3 * a + d], fb[3 * q + d]);
             }
         }
         if (eunit && t < np) eunit[t] = e_out;
@@ -381,11 +394,14 @@ __global__ void __launch_bounds__(64) PIP3_OCCUPANCY k_threebody(long long nt, c
         d3 roo[3];
         double doo[3];
         // forces on the first and on the second molecule, each summed over the wave's runs of
-        // trimers that share it (the list is ordered by first, then second molecule)
+        // trimers that share it (the list is ordered by first, then second molecule), and on the
+        // third, summed over the wave's trimers that share it after a sort by molecule
         u64 fa[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
         u64 fb[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        u64 fc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
         const long long key = t < nt ? (long long)trip[3 * t] : -1;
         const long long key2 = t < nt ? (long long)trip[3 * t + 1] : -1;
+        long long key3 = t < nt ? (long long)trip[3 * t + 2] : -1;
         if (t < nt && trimer_geometry(t, trip, idx, pos, periodic, box, at, R, roo, doo)) {
             real x[36], g[36];
 #pragma unroll
@@ -457,7 +473,9 @@ __global__ void __launch_bounds__(64) PIP3_OCCUPANCY k_threebody(long long nt, c
                     fb[3 * (q - 3) + 1] = to_fix(fy);
                     fb[3 * (q - 3) + 2] = to_fix(fz);
                 } else {
-                    add_force(force, at[q], fx, fy, fz);
+                    fc[3 * (q - 6)] = to_fix(fx);
+                    fc[3 * (q - 6) + 1] = to_fix(fy);
+                    fc[3 * (q - 6) + 2] = to_fix(fz);
                 }
             }
             e_out = E * s * cal2joule;
@@ -476,6 +494,14 @@ __global__ void __launch_bounds__(64) PIP3_OCCUPANCY k_threebody(long long nt, c
                 const int a = idx[3 * key2 + q];
 #pragma unroll
                 for (int d = 0; d < 3; ++d) atomicAdd(&force[3 * a + d], fb[3 * q + d]);
+            }
+        }
+        if (wave_sorted_run_sum(key3, fc) && key3 >= 0) {
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                const int a = idx[3 * key3 + q];
+#pragma unroll
+                for (int d = 0; d < 3; ++d) atomicAdd(&force[3 * a + d], fc[3 * q + d]);
             }
         }
         if (eunit && t < nt) eunit[t] = e_out;
