@@ -50,8 +50,8 @@ MEASURED_TRAFFIC = {
     'fft_y': 25657023,
     'fft_x_conv': 38351649,
     'fft_z_c2r': 25327521,
-    'threebody': 113528877,
-    'twobody': 58695962,
+    'threebody': 112543201,
+    'twobody': 58403508,
 }
 
 
