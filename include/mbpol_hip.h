@@ -189,8 +189,12 @@ int mbpol_context_set_solver(mbpol_context* ctx, int32_t solver);
  * lists grow when a configuration needs more: an overflow is detected at the end of the evaluation, which
  * is then run again); "pip_blocks", "pip3_blocks", "disp_blocks" = grid caps of the 2-body, 3-body and
  * dispersion kernels (0 = one block per 64 units); "pip3_kernel" = 3-body kernel (0 relabelling-group loop,
- * 1 term by term); "graph_steps", "graph_captures", "graph_replays", "graph_fallbacks" = field evaluations
- * of the captured dipole solve and the graph mode's counters (mbpol_context_set_graph) */
+ * 1 term by term); "fork_early" = where the short-range stream forks (1 after the virtual sites, 0 after
+ * the electrostatics set-up, 2 after the virtual sites with its kernels enqueued by a host thread of the
+ * context's own); "pip_waves", "pip3_waves", "disp_waves" = the caps in waves per SIMD;
+ * "tune_evaluations" = trial evaluations the tuning of the second evaluation ran; "graph_steps",
+ * "graph_captures", "graph_replays", "graph_fallbacks" = field evaluations of the captured dipole solve and
+ * the graph mode's counters (mbpol_context_set_graph) */
 int mbpol_context_get_tuning(const mbpol_context* ctx, const char* name, int32_t* value);
 /* copyParametersToContext of MBPolOneBodyForce / MBPolTwoBodyForce / MBPolThreeBodyForce
  * (MBPolReferenceKernels.cpp:116-127, 401-414, 485-498): new {O,H1,H2} index lists for the same number of

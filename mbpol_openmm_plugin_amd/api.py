@@ -520,8 +520,11 @@ class Context:
         """Launch shape picked by the context's autotuning ('site_lanes'), a grid cap
         ('pip_blocks', 'pip3_blocks', 'disp_blocks'), the 3-body kernel ('pip3_kernel'),
         a capacity ('row_capacity_pol', 'row_capacity_other', 'dimer_capacity',
-        'lower_capacity', 'trimer_capacity') or a graph-mode counter ('graph_steps',
-        'graph_captures', 'graph_replays', 'graph_fallbacks')."""
+        'lower_capacity', 'trimer_capacity'), the tuned overlap ('fork_early': 1 fork
+        after the virtual sites, 0 after the electrostatics set-up, 2 after the virtual
+        sites with the short-range kernels enqueued by a host thread; 'pip_waves',
+        'pip3_waves', 'disp_waves'; 'tune_evaluations') or a graph-mode counter
+        ('graph_steps', 'graph_captures', 'graph_replays', 'graph_fallbacks')."""
         v = ctypes.c_int32(0)
         _check(library().mbpol_context_get_tuning(self._h, name.encode(), ctypes.byref(v)))
         return v.value

@@ -3,9 +3,12 @@
 // Reference kernels keep between calls, MBPolReferenceKernels.h:45-244).
 #pragma once
 #include <hip/hip_runtime.h>
+#include <condition_variable>
 #include <cstdint>
+#include <functional>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "kernels.h"
@@ -106,6 +109,22 @@ struct EvalStats {
 };
 
 
+// A host thread of the context's own that enqueues the forked short-range work
+// while the calling thread enqueues the electrostatics, so that both streams are
+// fed at once: a kernel launch costs the host several microseconds, and the
+// electrostatics set-up and solve are many short kernels that the GPU runs
+// faster than one thread enqueues them.  One job at a time: posted, then waited
+// for before the join is enqueued.
+struct SideEnqueuer {
+    std::thread th;
+    std::mutex m;
+    std::condition_variable cv;
+    std::function<int()> job;
+    bool pending = false, quit = false;
+    int rc = 0;
+    std::string err;  // the job's error text (error texts are per thread)
+};
+
 }  // namespace mbpol
 
 struct mbpol_guess;
@@ -125,7 +144,10 @@ struct mbpol_context {
     int pip3_blocks = 0;       // grid cap of the 3-body kernel (0: one block per 64 units)
     // the caps in waves per SIMD (blocks = waves x 4 x CUs), tuned on the first forked evaluation
     int pip_waves = 1, pip3_waves = 1, disp_waves = 0, cus = 0;
-    int fork_early = 1;  // the short-range stream forks right after the virtual sites (else after the set-up)
+    // the short-range stream forks right after the virtual sites (1), after the set-up (0), or
+    // right after the virtual sites with its work enqueued by the side enqueuer's thread (2)
+    int fork_early = 1;
+    mbpol::SideEnqueuer* side_enq = nullptr;  // started on the first evaluation that uses it
     bool tuned = false;
     int tune_evaluations = 0;  // trial evaluations the tuning ran
     // the 1-body waters are the electrostatics waters (sites 4w..4w+3): the 1-body term
@@ -158,7 +180,7 @@ struct mbpol_context {
     mbpol::DevBuf trip_count, trip_off, trips;
     long long trip_cap = 0;
     int lists_built = 0;        // lists built by this evaluation: 1 dimers, 2 trimers
-    mbpol::DevBuf list_len;     // the lower list's length widened to 64 bits, for the read-back
+    mbpol::DevBuf results_stage;  // energies and list lengths gathered for the one read-back (11 u64, every system)
 
     // HIP graph of the whole evaluation (mbpol_context_set_graph): captured on first
     // use, replayed while the host state it was captured from is unchanged

@@ -573,8 +573,6 @@ static int build_rows(mbpol_context* ctx) {
     const double* pos = ctx->pos.as<double>();
     const int ncell = E->nc.x * E->nc.y * E->nc.z;
     if (E->use_cells) {
-        launch_fill_zero(st, E->cell_count.p, sizeof(int) * (ncell + 1));
-        launch_fill_zero(st, E->cell_fill.p, sizeof(int) * (ncell + 1));
         launch_site_cells(st, n, pos, E->P, E->nc, E->cell_of.as<int>(), E->cell_count.as<int>());
         if (int e = scan_i32(ctx, E->cell_count.as<int>(), E->cell_start.as<int>(), ncell + 1)) return e;
         launch_bucket_fill(st, n, E->cell_of.as<int>(), E->cell_start.as<int>(), E->cell_fill.as<int>(),
@@ -594,7 +592,6 @@ static int build_rows(mbpol_context* ctx) {
         HIP_TRY(E->nbP.ensure(sizeof(int) * std::max<size_t>(rp, 1)));
         HIP_TRY(E->pf.ensure(sizeof(double2) * std::max<size_t>(rp, 1)));
         HIP_TRY(E->nbQ.ensure(sizeof(int) * std::max<size_t>(rq, 1)));
-        launch_fill_zero(st, E->maxcnt.p, sizeof(int) * 4);
         launch_site_lists(st, E->site_lanes, n, pos, E->pol.as<double>(), E->P, E->nc, E->reach, E->use_cells,
                           E->cell_of.as<int>(), E->cell_start.as<int>(), E->members.as<int>(), E->cpos.as<double4>(),
                           E->capP, E->capQ, E->nbP.as<int>(), E->cntP.as<int>(), E->nbQ.as<int>(), E->cntQ.as<int>(),
@@ -627,7 +624,6 @@ static int build_rows(mbpol_context* ctx) {
     }
     // prefactors of the P rows and the real-space part of the fixed field (the
     // reciprocal part is added to it after the charge grid)
-    launch_fill_zero(st, E->efix.p, sizeof(double) * 3 * n);
     launch_prefactors_field(st, E->npol, E->polidx.as<int>(), pos, site_params(E), E->P, rows(E), E->pf.as<double2>(),
                             E->efix.as<double>());
     if (E->P.pme) {
@@ -638,8 +634,6 @@ static int build_rows(mbpol_context* ctx) {
         const size_t nent = (size_t)E->npol * 125;
         HIP_TRY(E->esite.ensure(sizeof(int) * std::max<size_t>(nent, 1)));
         HIP_TRY(E->ecoef.ensure(sizeof(double) * 3 * std::max<size_t>(nent, 1)));
-        launch_fill_zero(st, E->rowcount.p, sizeof(int) * (ngrid + 1));
-        launch_fill_zero(st, E->rowfill.p, sizeof(int) * (ngrid + 1));
         launch_csr_count(st, E->npol, E->polidx.as<int>(), E->g0tab.as<int>(), E->P, E->rowcount.as<int>());
         if (int e = scan_i32(ctx, E->rowcount.as<int>(), E->rowstart.as<int>(), (int)ngrid + 1)) return e;
         launch_csr_fill(st, E->npol, E->polidx.as<int>(), E->g0tab.as<int>(), E->sptab.as<double>(), E->P,
@@ -647,8 +641,6 @@ static int build_rows(mbpol_context* ctx) {
         launch_csr_sort(st, ngrid, E->rowstart.as<int>(), E->esite.as<int>(), E->ecoef.as<double>());
         // charged sites by brick, for the charge spreading
         Elec::Bins& b = E->binq;
-        launch_fill_zero(st, b.count.p, sizeof(int) * (nbins + 1));
-        launch_fill_zero(st, b.fill.p, sizeof(int) * (nbins + 1));
         launch_site_bins(st, n, pos, E->q.as<double>(), E->P, E->B, b.of.as<int>(), b.count.as<int>());
         if (int e = scan_i32(ctx, b.count.as<int>(), b.start.as<int>(), nbins + 1)) return e;
         launch_bucket_fill(st, n, b.of.as<int>(), b.start.as<int>(), b.fill.as<int>(), E->unsorted.as<int>());
@@ -748,7 +740,6 @@ static int solve_sor(mbpol_context* ctx, double* buf0, double* buf1, double self
     SolveCtl* ctl = E->ctl.as<SolveCtl>();
     const int* stop = &ctl->stop;
     double* buf[2] = {buf0, buf1};
-    launch_fill_zero(st, ctl, sizeof(SolveCtl));
     int t = 0, batch = 0;
     c = SolveCtl{};
     while (t <= s.el_max_iter) {
@@ -824,7 +815,6 @@ static int solve_pcg(mbpol_context* ctx, double* x, double self_term, SolveCtl& 
         return MBPOL_OK;
     };
     launch_block_factor(st, B, pos, site_params(E), P, minv);
-    launch_fill_zero(st, ctl, sizeof(SolveCtl));
     // field evaluation f (0-based): f = 0 forms the initial residual and direction 0,
     // f = k + 1 takes step k and forms direction k + 1
     int f = 0, batch = 0;
@@ -1014,6 +1004,31 @@ int elec_evaluate(mbpol_context* ctx, u64* energy, const std::function<int()>& a
     const SiteParams S = site_params(E);
     // charges (setup, :857-897)
     timer_start(ctx, T_ELEC_SETUP);
+    // every counter and accumulator of this evaluation cleared by one launch: cell and
+    // row counters, the fixed field, the PME spreading-table and brick counters, the
+    // solver state, and the energy phase's site forces and potentials
+    {
+        FillList L(st);
+        const int ncell = E->nc.x * E->nc.y * E->nc.z;
+        if (E->use_cells) {
+            L.add(E->cell_count.p, sizeof(int) * (ncell + 1));
+            L.add(E->cell_fill.p, sizeof(int) * (ncell + 1));
+        }
+        L.add(E->maxcnt.p, sizeof(int) * 4);
+        L.add(E->efix.p, sizeof(double) * 3 * n);
+        if (P.pme) {
+            const size_t ngrid = (size_t)P.grid.x * P.grid.y * P.grid.z;
+            const int nbins = E->B.nb[0] * E->B.nb[1] * E->B.nb[2];
+            L.add(E->rowcount.p, sizeof(int) * (ngrid + 1));
+            L.add(E->rowfill.p, sizeof(int) * (ngrid + 1));
+            L.add(E->binq.count.p, sizeof(int) * (nbins + 1));
+            L.add(E->binq.fill.p, sizeof(int) * (nbins + 1));
+        }
+        L.add(E->ctl.p, sizeof(SolveCtl));
+        L.add(E->fsite.p, sizeof(double) * 3 * n);
+        L.add(E->V.p, sizeof(double) * n);
+        L.launch();
+    }
     launch_site_charges_copy(st, n, E->q0.as<double>(), E->q.as<double>());
     // sites of water w are 4w..4w+3 (the reference's ii += 4 loop, :881)
     if (P.redistribute) {
@@ -1060,10 +1075,6 @@ int elec_evaluate(mbpol_context* ctx, u64* energy, const std::function<int()>& a
             [[fallthrough]];
         default: HIP_TRY(hipMemcpyAsync(mu_in, E->efix_pol.p, mbytes, hipMemcpyDeviceToDevice, st));
     }
-    // accumulators of the energy phase, cleared before the solve: after the
-    // solve's last host read the energy kernels follow without other work first
-    launch_fill_zero(st, E->fsite.p, sizeof(double) * 3 * n);
-    launch_fill_zero(st, E->V.p, sizeof(double) * n);
     timer_stop(ctx, T_ELEC_SETUP);
     // the caller's work for the other stream is enqueued here: the set-up above
     // keeps the GPU busy meanwhile, and the solve below waits on the host

@@ -54,8 +54,34 @@ void launch_water_terms(hipStream_t s, int nw, bool onebody, bool charges, const
 void launch_forces_to_double(hipStream_t s, int n3, const u64* fx, double* f);
 // zero `bytes` (a multiple of 4) with a kernel (no memset node in a captured graph)
 void launch_fill_zero(hipStream_t s, void* p, size_t bytes);
-// *out = (long long)*in
-void launch_widen_i32(hipStream_t s, const int* in, long long* out);
+// several buffers zeroed by one launch (lengths in bytes, multiples of 4): the
+// buffers are gathered with add() and zeroed on stream s by launch(); a list
+// that fills up is launched by add() and starts again
+constexpr int kFillMax = 12;
+struct FillBatch {
+    int n = 0;
+    void* p[kFillMax] = {};
+    long long words[kFillMax] = {};
+};
+void launch_fill_multi(hipStream_t s, const FillBatch& B);
+struct FillList {
+    hipStream_t s;
+    FillBatch B;
+    explicit FillList(hipStream_t stream) : s(stream) {}
+    void add(void* ptr, size_t bytes) {
+        if (!ptr || bytes < 4) return;
+        if (B.n == kFillMax) launch();
+        B.p[B.n] = ptr;
+        B.words[B.n++] = (long long)(bytes / 4);
+    }
+    void launch() {
+        launch_fill_multi(s, B);
+        B.n = 0;
+    }
+};
+// energies (8) and the dimer, lower and trimer list lengths into out[0..10] (nullptr: not built)
+void launch_collect_results(hipStream_t s, const unsigned long long* energy, const long long* nd, const int* nl,
+                            const long long* nt, unsigned long long* out);
 void launch_fold_vsites(hipStream_t s, int n, const int* vs_idx, const double* vs_w, double* f);
 
 // ---- kernels_scan.hip: exclusive prefix sums (scratch of scan_scratch_bytes(n)) ----

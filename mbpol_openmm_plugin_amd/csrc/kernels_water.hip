@@ -86,8 +86,28 @@ __global__ void k_fill_zero(unsigned int* __restrict__ p, long long nwords) {
         p[i] = 0u;
 }
 
-// one int widened to 64 bits (for a read-back next to 64-bit lengths)
-__global__ void k_widen_i32(const int* __restrict__ in, long long* __restrict__ out) { *out = *in; }
+// zero up to kFillMax buffers in one launch: blockIdx.y picks the buffer
+__global__ void k_fill_multi(FillBatch L) {
+    const int b = blockIdx.y;
+    if (b >= L.n) return;
+    unsigned int* p = static_cast<unsigned int*>(L.p[b]);
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < L.words[b];
+         i += (long long)gridDim.x * blockDim.x)
+        p[i] = 0u;
+}
+
+// what the host reads after an evaluation, gathered for one copy: the eight
+// fixed-point energy sums, then the lengths of the dimer, lower and trimer lists
+// (0 for a list not built)
+__global__ void k_collect_results(const unsigned long long* __restrict__ energy, const long long* __restrict__ nd,
+                                  const int* __restrict__ nl, const long long* __restrict__ nt,
+                                  unsigned long long* __restrict__ out) {
+    const int t = threadIdx.x;
+    if (t < 8) out[t] = energy[t];
+    if (t == 8) out[8] = nd ? (unsigned long long)*nd : 0ull;
+    if (t == 9) out[9] = nl ? (unsigned long long)(long long)*nl : 0ull;
+    if (t == 10) out[10] = nt ? (unsigned long long)*nt : 0ull;
+}
 
 // fixed point -> double for every particle
 __global__ void k_forces_to_double(int n3, const unsigned long long* __restrict__ fx, double* __restrict__ f) {
@@ -161,7 +181,16 @@ void launch_fill_zero(hipStream_t s, void* p, size_t bytes) {
         hipLaunchKernelGGL(k_fill_zero, (int)std::min<long long>(nblk(nw, 256), 2048), 256, 0, s,
                            static_cast<unsigned int*>(p), nw);
 }
-void launch_widen_i32(hipStream_t s, const int* in, long long* out) { hipLaunchKernelGGL(k_widen_i32, 1, 1, 0, s, in, out); }
+void launch_fill_multi(hipStream_t s, const FillBatch& L) {
+    long long most = 0;
+    for (int b = 0; b < L.n; ++b) most = std::max(most, L.words[b]);
+    if (L.n > 0 && most > 0)
+        hipLaunchKernelGGL(k_fill_multi, dim3((unsigned)std::min<long long>(nblk(most, 256), 1024), L.n), 256, 0, s, L);
+}
+void launch_collect_results(hipStream_t s, const u64* energy, const long long* nd, const int* nl, const long long* nt,
+                            u64* out) {
+    hipLaunchKernelGGL(k_collect_results, 1, 64, 0, s, energy, nd, nl, nt, out);
+}
 void launch_forces_to_double(hipStream_t s, int n3, const u64* fx, double* f) {
     if (n3 > 0) hipLaunchKernelGGL(k_forces_to_double, nblk(n3, 256), 256, 0, s, n3, fx, f);
 }

@@ -9,6 +9,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <exception>
 #include <string>
 
 #include "../../include/mbpol_hip.h"
@@ -86,6 +87,68 @@ int mbpol::scan_i64(mbpol_context* ctx, const long long* in, long long* out, int
     HIP_TRY(tmp.ensure(scan_scratch_bytes(n)));
     launch_scan_i64(s, in, out, n, tmp.p);
     return MBPOL_OK;
+}
+
+// ---------------------------------------------------------------------------
+// the side enqueuer: a host thread that runs one posted job at a time with the
+// context's device current
+// ---------------------------------------------------------------------------
+static void side_enqueuer_loop(SideEnqueuer* w, int device) {
+    const hipError_t dev = hipSetDevice(device);
+    std::unique_lock<std::mutex> lk(w->m);
+    for (;;) {
+        w->cv.wait(lk, [w] { return w->pending || w->quit; });
+        if (!w->pending) return;
+        std::function<int()> f = std::move(w->job);
+        lk.unlock();
+        const int e = dev == hipSuccess ? f() : set_error(MBPOL_ERR_HIP, "hipSetDevice: %s", hipGetErrorString(dev));
+        std::string msg = e ? g_last_error : std::string();
+        lk.lock();
+        w->rc = e;
+        w->err = std::move(msg);
+        w->pending = false;
+        w->cv.notify_all();
+    }
+}
+
+static int side_post(mbpol_context* ctx, std::function<int()> f) {
+    if (!ctx->side_enq) {
+        auto* w = new SideEnqueuer;
+        try {
+            w->th = std::thread(side_enqueuer_loop, w, ctx->device);
+        } catch (const std::exception& ex) {
+            delete w;
+            return set_error(MBPOL_ERR_HIP, "side enqueuer thread: %s", ex.what());
+        }
+        ctx->side_enq = w;
+    }
+    SideEnqueuer* w = ctx->side_enq;
+    std::lock_guard<std::mutex> lk(w->m);
+    w->job = std::move(f);
+    w->pending = true;
+    w->cv.notify_all();
+    return MBPOL_OK;
+}
+
+// waits for the posted job; its error (code and text) becomes this thread's
+static int side_wait(mbpol_context* ctx) {
+    SideEnqueuer* w = ctx->side_enq;
+    std::unique_lock<std::mutex> lk(w->m);
+    w->cv.wait(lk, [w] { return !w->pending; });
+    return w->rc ? set_error(w->rc, "%s", w->err.c_str()) : MBPOL_OK;
+}
+
+static void side_stop(mbpol_context* ctx) {
+    SideEnqueuer* w = ctx->side_enq;
+    if (!w) return;
+    {
+        std::lock_guard<std::mutex> lk(w->m);
+        w->quit = true;
+        w->cv.notify_all();
+    }
+    w->th.join();
+    delete w;
+    ctx->side_enq = nullptr;
 }
 
 // ---------------------------------------------------------------------------
@@ -504,12 +567,14 @@ int mbpol_context_create(const mbpol_system* sys, int32_t precision, int32_t dev
         int dwaves = 0;  // the dispersion kernel: one block per 64 atoms (MBPOL_HIP_DISP_WAVES)
         if (const char* env = std::getenv("MBPOL_HIP_DISP_WAVES")) dwaves = std::max(0, std::atoi(env));
         ctx->disp_waves = dwaves;
-        if (const char* env = std::getenv("MBPOL_HIP_FORK")) ctx->fork_early = std::strcmp(env, "late") != 0;
+        if (const char* env = std::getenv("MBPOL_HIP_FORK"))
+            ctx->fork_early = !std::strcmp(env, "late") ? 0 : !std::strcmp(env, "thread") ? 2 : 1;
         ctx->disp_blocks = dwaves * 4 * prop.multiProcessorCount;
     }
     const int n = sys->n;
     if (ctx->pos.ensure(sizeof(double) * 3 * n) || ctx->force_fx.ensure(sizeof(u64) * 3 * n) ||
-        ctx->force.ensure(sizeof(double) * 3 * n) || ctx->energy_fx.ensure(sizeof(u64) * 8))
+        ctx->force.ensure(sizeof(double) * 3 * n) || ctx->energy_fx.ensure(sizeof(u64) * 8) ||
+        ctx->results_stage.ensure(sizeof(u64) * 16))
         return fail(set_error(MBPOL_ERR_HIP, "device allocation failed"));
     if (int e = upload(ctx, ctx->vs_idx, sys->vs_idx)) return fail(e);
     if (int e = upload(ctx, ctx->vs_w, sys->vs_w)) return fail(e);
@@ -526,8 +591,7 @@ int mbpol_context_create(const mbpol_system* sys, int32_t precision, int32_t dev
     if (n3 > 0) {
         if (int e = setup_grid(ctx, ctx->grid3, n3, sys->th_method == MBPOL_CUTOFF_PERIODIC, sys->th_cutoff)) return fail(e);
         if (ctx->lo_count.ensure(sizeof(int) * (n3 + 1)) || ctx->lo_start.ensure(sizeof(int) * (n3 + 1)) ||
-            ctx->trip_count.ensure(sizeof(long long) * (n3 + 1)) || ctx->trip_off.ensure(sizeof(long long) * (n3 + 1)) ||
-            ctx->list_len.ensure(sizeof(long long)))
+            ctx->trip_count.ensure(sizeof(long long) * (n3 + 1)) || ctx->trip_off.ensure(sizeof(long long) * (n3 + 1)))
             return fail(set_error(MBPOL_ERR_HIP, "device allocation failed"));
     }
     apply_list_groups(ctx);
@@ -551,6 +615,7 @@ int mbpol_context_create(const mbpol_system* sys, int32_t precision, int32_t dev
 void mbpol_context_destroy(mbpol_context* ctx) {
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
+    side_stop(ctx);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     if (ctx->side) (void)hipStreamSynchronize(ctx->side);
     timer_collect(ctx);
@@ -727,7 +792,7 @@ static int build_lists(mbpol_context* ctx, hipStream_t st, bool want2, bool want
 }
 
 // After an evaluation: the list lengths (read back with the energies into
-// h_counts[16..18]) against the capacities; when one did not fit, the lists
+// h_counts[16..18] by results_to_host) against the capacities; when one did not fit, the lists
 // grow and `redo` asks for the evaluation to be run again.
 static int list_check(mbpol_context* ctx, bool& redo) {
     const mbpol_system& s = ctx->sys;
@@ -759,20 +824,17 @@ static int list_check(mbpol_context* ctx, bool& redo) {
     return MBPOL_OK;
 }
 
-// device -> host copies of the three list lengths into h_counts[16..18]
-static int list_lengths_to_host(mbpol_context* ctx, hipStream_t st) {
+// device -> host, in one copy: the energy sums into h_counts[8..15] and the
+// three list lengths into h_counts[16..18] (gathered by one kernel)
+static int results_to_host(mbpol_context* ctx, hipStream_t st) {
     const mbpol_system& s = ctx->sys;
     const int n2 = (int)s.tb_idx.size() / 3, n3 = (int)s.th_idx.size() / 3;
-    long long* hc = static_cast<long long*>(ctx->h_counts);
-    if (ctx->lists_built & 1)
-        HIP_TRY(hipMemcpyAsync(&hc[16], ctx->dimer_off.as<long long>() + n2, sizeof(long long), hipMemcpyDeviceToHost,
-                               st));
-    if (ctx->lists_built & 2) {
-        launch_widen_i32(st, ctx->lo_start.as<int>() + n3, ctx->list_len.as<long long>());
-        HIP_TRY(hipMemcpyAsync(&hc[17], ctx->list_len.p, sizeof(long long), hipMemcpyDeviceToHost, st));
-        HIP_TRY(hipMemcpyAsync(&hc[18], ctx->trip_off.as<long long>() + n3, sizeof(long long), hipMemcpyDeviceToHost,
-                               st));
-    }
+    const bool l2 = ctx->lists_built & 1, l3 = ctx->lists_built & 2;
+    u64* stage = ctx->results_stage.as<u64>();
+    launch_collect_results(st, ctx->energy_fx.as<u64>(), l2 ? ctx->dimer_off.as<long long>() + n2 : nullptr,
+                           l3 ? ctx->lo_start.as<int>() + n3 : nullptr, l3 ? ctx->trip_off.as<long long>() + n3 : nullptr,
+                           stage);
+    HIP_TRY(hipMemcpyAsync(static_cast<u64*>(ctx->h_counts) + 8, stage, sizeof(u64) * 11, hipMemcpyDeviceToHost, st));
     return MBPOL_OK;
 }
 
@@ -867,9 +929,13 @@ static int enqueue_evaluation(mbpol_context* ctx, double* forces_dev_out, uint32
     u64* F = ctx->force_fx.as<u64>();
     u64* E = ctx->energy_fx.as<u64>();
     double* pos = ctx->pos.as<double>();
-    launch_fill_zero(st, F, sizeof(u64) * 3 * s.n);
-    launch_fill_zero(st, E, sizeof(u64) * 8);
-    if (ctx->ngroups) launch_fill_zero(st, ctx->g_energy.p, sizeof(u64) * 8 * ctx->ngroups);
+    {
+        FillList L(st);
+        L.add(F, sizeof(u64) * 3 * s.n);
+        L.add(E, sizeof(u64) * 8);
+        if (ctx->ngroups) L.add(ctx->g_energy.p, sizeof(u64) * 8 * ctx->ngroups);
+        L.launch();
+    }
     ctx->lists_built = 0;
     launch_place_vsites(st, (int)s.vs_idx.size() / 4, ctx->vs_idx.as<int>(), ctx->vs_w.as<double>(), pos);
     // the 1-body term runs with the charges when both are computed for the same waters
@@ -886,9 +952,13 @@ static int enqueue_evaluation(mbpol_context* ctx, double* forces_dev_out, uint32
     // the dipole solve, whose host reads would otherwise delay it.  The fork
     // point on the GPU is either right after the virtual sites are placed
     // (fork_early: the short-range kernels also run beside the set-up's many
-    // short kernels) or after the set-up; the tuning picks one.
+    // short kernels) or after the set-up; or, third, right after the virtual
+    // sites with the short-range work enqueued at once by the side enqueuer's
+    // thread while this one enqueues the electrostatics (not under capture);
+    // the tuning picks one.
     const bool fork = !ctx->timer.enabled && ctx->evaluated;
     const bool elec = (comp & MBPOL_ELECTROSTATICS) && ctx->elec;
+    const bool threaded = fork && elec && ctx->fork_early == 2 && fixed_steps == 0 && !mbpol::g_capturing;
     if (fork && ctx->fork_early) HIP_TRY(hipEventRecord(ctx->ev_fork, st));
     int rc = MBPOL_OK;
     bool forked = false;
@@ -904,6 +974,12 @@ static int enqueue_evaluation(mbpol_context* ctx, double* forces_dev_out, uint32
     if (!fork) {
         rc = short_range(ctx, st, comp);
         if (!rc && elec) rc = elec_evaluate(ctx, E + 4, nullptr, fixed_steps);
+    } else if (threaded) {
+        if (int e = side_post(ctx, fork_short_range)) return e;
+        rc = elec_evaluate(ctx, E + 4, nullptr, fixed_steps);
+        // always waited for: the job refers to this frame
+        const int e = side_wait(ctx);
+        if (!rc) rc = e;
     } else if (elec) {
         rc = elec_evaluate(ctx, E + 4, fork_short_range, fixed_steps);
         if (rc && !forked) {
@@ -920,9 +996,7 @@ static int enqueue_evaluation(mbpol_context* ctx, double* forces_dev_out, uint32
     double* fout = forces_dev_out ? forces_dev_out : ctx->force.as<double>();
     launch_forces_to_double(st, 3 * s.n, F, fout);
     launch_fold_vsites(st, (int)s.vs_idx.size() / 4, ctx->vs_idx.as<int>(), ctx->vs_w.as<double>(), fout);
-    u64* he = static_cast<u64*>(ctx->h_counts) + 8;
-    HIP_TRY(hipMemcpyAsync(he, E, sizeof(u64) * 8, hipMemcpyDeviceToHost, st));
-    if (int e = list_lengths_to_host(ctx, st)) return e;
+    if (int e = results_to_host(ctx, st)) return e;
     if (forces_host_out)
         HIP_TRY(hipMemcpyAsync(forces_host_out, fout, sizeof(double) * 3 * s.n, hipMemcpyDeviceToHost, st));
     return MBPOL_OK;
@@ -940,20 +1014,23 @@ static void energies_from_host(const mbpol_context* ctx, double* energies_out) {
     energies_out[5] = tot;
 }
 
-// Evaluation-level tuning of the overlap: the fork point of the short-range
-// stream and the short-range grid caps.  The 2-body, 3-body and dispersion
-// kernels run beside the electrostatics, so what counts is the whole
-// overlapped evaluation, not a kernel alone: on the first forked evaluation,
-// the fork point (early or after the electrostatics set-up) and then each cap
-// (waves per SIMD; 0 = one block per 64 units) is tried in turn on trial
-// evaluations of the caller's configuration, timed on the host
-// from the first launch to the final synchronisation (median of five), and
-// kept when 1% faster than the one before it.  Every candidate leaves the
-// results bit for bit the same (fixed-point force and energy sums, per-unit
-// energies summed in list order), so the tuning is invisible but in time.  The
-// trial solves start from alpha E_fixed and the previous dipoles are put back
-// afterwards.  MBPOL_HIP_TUNE=0 turns it off; a setting fixed by its environment
-// variable (MBPOL_HIP_FORK=early|late, MBPOL_HIP_*_WAVES) is not tuned.
+// Evaluation-level tuning of the overlap: the fork of the short-range stream
+// and the short-range grid caps.  The 2-body, 3-body and dispersion kernels run
+// beside the electrostatics, so what counts is the whole overlapped evaluation,
+// not a kernel alone: on the first forked evaluation each knob in turn -- the
+// fork (early, after the electrostatics set-up, or early with the short-range
+// work enqueued by a host thread of its own), then each cap (waves per SIMD;
+// 0 = one block per 64 units) -- has its candidates timed on trial evaluations
+// of the caller's configuration, from the first launch to the final
+// synchronisation on the host.  The candidates take turns (three rounds of
+// three evaluations each), so that a drift of the clocks falls on all of them
+// alike; a candidate replaces the current setting when its median is 1.5%
+// faster.  Every candidate leaves the results bit for bit the same
+// (fixed-point force and energy sums, per-unit energies summed in list order),
+// so the tuning is invisible but in time.  The trial solves start from
+// alpha E_fixed and the previous dipoles are put back afterwards.
+// MBPOL_HIP_TUNE=0 turns it off; a setting fixed by its environment variable
+// (MBPOL_HIP_FORK=early|late|thread, MBPOL_HIP_*_WAVES) is not tuned.
 static int evaluate_eager(mbpol_context* ctx, double* forces_dev_out, double* energies_out, uint32_t comp,
                           double* forces_host_out);
 
@@ -971,7 +1048,7 @@ static int tune_overlap(mbpol_context* ctx, uint32_t comp) {
         int* blocks;  // null: the value itself is the setting
         std::vector<int> cand;
     } knobs[] = {
-        {"MBPOL_HIP_FORK", true, &ctx->fork_early, nullptr, {1, 0}},
+        {"MBPOL_HIP_FORK", true, &ctx->fork_early, nullptr, {1, 0, 2}},
         {"MBPOL_HIP_3B_WAVES", (comp & MBPOL_THREEBODY) && !s.th_idx.empty(), &ctx->pip3_waves, &ctx->pip3_blocks,
          {1, 0, 2}},
         {"MBPOL_HIP_DISP_WAVES", (comp & MBPOL_DISPERSION) && ctx->disp, &ctx->disp_waves, &ctx->disp_blocks,
@@ -983,42 +1060,52 @@ static int tune_overlap(mbpol_context* ctx, uint32_t comp) {
         if (int e = elec_tune_hold(ctx, true)) return e;
     hipStream_t st = ctx->stream;
     bool ok = true;
-    // median host time of five trial evaluations; false when one failed or did not fit
-    auto measure = [&](double& ms) -> bool {
-        double t[5];
-        for (double& ti : t) {
+    auto set = [&](Knob& k, int w) {
+        *k.waves = w;
+        if (k.blocks) *k.blocks = w * 4 * ctx->cus;
+    };
+    // host time of `count` trial evaluations appended to t; false when one failed or did not fit
+    auto measure = [&](int count, std::vector<double>& t) -> bool {
+        for (int r = 0; r < count; ++r) {
             const auto t0 = std::chrono::steady_clock::now();
             const int rc = enqueue_evaluation(ctx, nullptr, comp, nullptr, 0);
             if (hipStreamSynchronize(st) != hipSuccess || rc != MBPOL_OK) return false;
-            ti = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+            t.push_back(std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
             ++ctx->tune_evaluations;
             bool redo = false, redo_lists = false;
             if (elec && elec_rows_check(ctx, redo) != MBPOL_OK) return false;
             if (list_check(ctx, redo_lists) != MBPOL_OK || redo || redo_lists) return false;
         }
-        std::sort(t, t + 5);
-        ms = t[2];
         return true;
+    };
+    auto median = [](std::vector<double> v) {
+        std::sort(v.begin(), v.end());
+        return v[v.size() / 2];
     };
     for (Knob& k : knobs) {
         if (!ok || !k.on || std::getenv(k.env)) continue;
         const int start = *k.waves;
-        double best_ms = 0;
-        if (!(ok = measure(best_ms))) break;
+        std::vector<int> cand{start};
+        for (int w : k.cand)
+            if (w != start) cand.push_back(w);
+        std::vector<std::vector<double>> t(cand.size());
+        for (int round = 0; ok && round < 3; ++round)
+            for (size_t c = 0; ok && c < cand.size(); ++c) {
+                set(k, cand[c]);
+                ok = measure(3, t[c]);
+            }
         int best = start;
-        for (int w : k.cand) {
-            if (w == start) continue;
-            *k.waves = w;
-            if (k.blocks) *k.blocks = w * 4 * ctx->cus;
-            double ms = 0;
-            if (!(ok = measure(ms))) break;
-            if (ms < 0.99 * best_ms) {
-                best_ms = ms;
-                best = w;
+        if (ok) {
+            double best_ms = median(t[0]);
+            for (size_t c = 1; c < cand.size(); ++c) {
+                const double ms = median(t[c]);
+                if (ms < 0.985 * best_ms) {
+                    best_ms = ms;
+                    best = cand[c];
+                }
             }
         }
-        *k.waves = best;
-        if (k.blocks) *k.blocks = best * 4 * ctx->cus;
+        set(k, best);
     }
     (void)hipGetLastError();
     HIP_TRY(hipStreamSynchronize(st));
@@ -1042,7 +1129,7 @@ static int evaluate_eager(mbpol_context* ctx, double* forces_dev_out, double* en
         bool redo = false, redo_lists = false;
         if (ctx->elec)
             if (int e = elec_rows_check(ctx, redo)) return e;
-        if (int e = list_lengths_to_host(ctx, st)) return e;
+        if (int e = results_to_host(ctx, st)) return e;
         HIP_TRY(hipStreamSynchronize(st));
         if (int e = list_check(ctx, redo_lists)) return e;
         if (redo || redo_lists) return evaluate_eager(ctx, forces_dev_out, energies_out, comp, forces_host_out);

@@ -10,7 +10,8 @@ they found it):
 
 * contexts with caps and fork points fixed to other values by their
   environment variables equal the untuned context bit for bit (energies,
-  forces, dipoles, iterations);
+  forces, dipoles, iterations), the short-range work enqueued by the side
+  enqueuer's host thread among them;
 * the tuned context equals the untuned one bit for bit, ran trial evaluations,
   and reports caps among its candidates.
 """
@@ -58,7 +59,7 @@ def test_caps_change_no_result_and_tuning_is_invisible(mb, monkeypatch):
     ref, caps0 = run(mb)
     print('untuned', caps0, 'energies', ref[-1][0], flush=True)
     assert caps0['tune_evaluations'] == 0
-    for pip, pip3, disp, fork in ((2, 0, 2, 'late'), (0, 2, 1, 'early')):
+    for pip, pip3, disp, fork in ((2, 0, 2, 'late'), (0, 2, 1, 'early'), (1, 1, 0, 'thread')):
         monkeypatch.setenv('MBPOL_HIP_PIP_WAVES', str(pip))
         monkeypatch.setenv('MBPOL_HIP_3B_WAVES', str(pip3))
         monkeypatch.setenv('MBPOL_HIP_DISP_WAVES', str(disp))
@@ -66,13 +67,13 @@ def test_caps_change_no_result_and_tuning_is_invisible(mb, monkeypatch):
         got, caps = run(mb)
         print('fixed caps', caps, flush=True)
         assert (caps['pip_waves'], caps['pip3_waves'], caps['disp_waves']) == (pip, pip3, disp)
-        assert caps['fork_early'] == (fork == 'early')
+        assert caps['fork_early'] == {'late': 0, 'early': 1, 'thread': 2}[fork]
         same(ref, got)
     for name in ('MBPOL_HIP_TUNE', 'MBPOL_HIP_PIP_WAVES', 'MBPOL_HIP_3B_WAVES', 'MBPOL_HIP_DISP_WAVES',
                  'MBPOL_HIP_FORK'):
         monkeypatch.delenv(name)
     got, caps = run(mb)
     print('tuned', caps, flush=True)
-    assert caps['tune_evaluations'] >= 3 * 5
+    assert caps['tune_evaluations'] >= 4 * 5
     assert caps['pip_waves'] in (0, 1, 2) and caps['pip3_waves'] in (0, 1, 2) and caps['disp_waves'] in (0, 1, 2)
     same(ref, got)
