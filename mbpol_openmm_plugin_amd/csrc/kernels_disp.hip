@@ -1,9 +1,10 @@
 // Damped-C6 dispersion pair kernel (python/mbpol.xml:37-108):
 //   E = -C6 tt6 / r^6,  tt6 = 1 - exp(-d6 r) sum_{k<=6} (d6 r)^k / k!
 // One wave per active particle; lanes stride over its candidates (27 cells
-// or every particle), the force on the wave's particle is reduced in
-// registers and written once, each pair is visited from both ends and its
-// energy halved.  Arithmetic type T: float (mixed) or double.
+// or every particle), queue those within the cutoff and evaluate them in full
+// waves; the force on the wave's particle is reduced in registers and written
+// once, each pair is visited from both ends and its energy halved.
+// Arithmetic type T: float (mixed) or double.
 #include "device_common.h"
 #include "disp_kernels.h"
 
@@ -54,27 +55,30 @@ __global__ void __launch_bounds__(64) k_disp(DispGeom g, const int* __restrict__
                                              const int* __restrict__ members, const double4* __restrict__ cpos,
                                              u64* __restrict__ force, double* __restrict__ eblk,
                                              unsigned long long* __restrict__ cnt_out) {
+    // The candidates within the cutoff (about one in nine of the 27-cell stencil)
+    // are queued in LDS and their damped-C6 terms evaluated 64 at a time, so that
+    // the exponential and the divisions run on full waves, not on the few lanes
+    // of each candidate batch that found a partner.
+    constexpr int kQ = 128;  // ring of queued pairs; at most 63 + 64 are pending
+    __shared__ double qx[kQ], qy[kQ], qz[kQ];
+    __shared__ int qt[kQ];
     const int lane = threadIdx.x;
+    const unsigned long long below = (1ull << lane) - 1ull;
+    const double rc2 = g.cutoff * g.cutoff;
+    // minimum image by products with 1/L: the image index differs from the one
+    // the quotient gives only where |dx| is within rounding of L/2, beyond the cutoff
+    const double ilx = 1.0 / g.L.x, ily = 1.0 / g.L.y, ilz = 1.0 / g.L.z;
     // one atom per wave; a capped grid strides over the atoms
     for (int a = blockIdx.x; a < g.na; a += gridDim.x) {
         const int i = act[a];
         const double xi = pos[3 * i], yi = pos[3 * i + 1], zi = pos[3 * i + 2];
         const int ti = type[i], mi = mol[i];
-        const double rc2 = g.cutoff * g.cutoff;
         double fx = 0, fy = 0, fz = 0, en = 0;
         unsigned long long cnt = 0;
-        // q: position of the candidate and its molecule * 16 + type (cell-ordered copy when cells are used)
-        auto visit = [&](const double4& q) {
-            const int key = (int)q.w, mj = key >> 4, tj = key & 15;
-            if (mj == mi) return;  // same molecule, the atom itself included
-            double dx = xi - q.x, dy = yi - q.y, dz = zi - q.z;
-            if (g.periodic) {
-                dx -= floor(dx / g.L.x + 0.5) * g.L.x;
-                dy -= floor(dy / g.L.y + 0.5) * g.L.y;
-                dz -= floor(dz / g.L.z + 0.5) * g.L.z;
-            }
+        int head = 0, tail = 0;  // wave-uniform queue positions
+        // the damped-C6 term of a queued pair (separation dx, dy, dz; partner type tj)
+        auto pair = [&](double dx, double dy, double dz, int tj) {
             const double r2d = dx * dx + dy * dy + dz * dz;
-            if (g.method != 0 && r2d >= rc2) return;
             const T C = (T)c6[ti + g.ntypes * tj], D = (T)d6[ti + g.ntypes * tj];
             const T r2 = (T)r2d, r = sqrt(r2), x = D * r;
             const T s = (T)1 + ((T)1 + ((T)1 + ((T)1 + ((T)1 + ((T)1 + x / (T)6) * x / (T)5) * x / (T)4) * x / (T)3) * x / (T)2) * x;
@@ -90,11 +94,48 @@ __global__ void __launch_bounds__(64) k_disp(DispGeom g, const int* __restrict__
             en += -(double)(C * tt6 / r6);
             ++cnt;
         };
+        // evaluate queued pairs 64 at a time (all of them when `all`)
+        auto drain = [&](bool all) {
+            while (tail - head >= 64 || (all && tail > head)) {
+                __syncthreads();  // the queue entries are written
+                const int k = head + lane;
+                if (k < tail) {
+                    const int q = k & (kQ - 1);
+                    pair(qx[q], qy[q], qz[q], qt[q]);
+                }
+                head += min(64, tail - head);
+                __syncthreads();  // read before the slots are written again
+            }
+        };
+        // candidate q (position, molecule * 16 + type); `valid` false for a lane past the range
+        auto offer = [&](bool valid, const double4& q) {
+            const int key = (int)q.w, mj = key >> 4;
+            double dx = xi - q.x, dy = yi - q.y, dz = zi - q.z;
+            if (g.periodic) {
+                dx -= floor(dx * ilx + 0.5) * g.L.x;
+                dy -= floor(dy * ily + 0.5) * g.L.y;
+                dz -= floor(dz * ilz + 0.5) * g.L.z;
+            }
+            const double r2d = dx * dx + dy * dy + dz * dz;
+            // another molecule (the atom itself excluded), within the cutoff unless there is none
+            const bool ok = valid && mj != mi && (g.method == 0 || r2d < rc2);
+            const unsigned long long m = __ballot(ok);
+            if (ok) {
+                const int q = (tail + __popcll(m & below)) & (kQ - 1);
+                qx[q] = dx;
+                qy[q] = dy;
+                qz[q] = dz;
+                qt[q] = key & 15;
+            }
+            tail += __popcll(m);
+            drain(false);
+        };
         if (!g.use_cells) {
             const int b0 = g.gs ? g.gs[a] : 0, b1 = g.gs ? g.ge[a] : g.na;
-            for (int b = b0 + lane; b < b1; b += 64) {
-                const int j = act[b];
-                visit(make_double4(pos[3 * j], pos[3 * j + 1], pos[3 * j + 2], (double)(mol[j] * 16 + type[j])));
+            for (int b = b0; b < b1; b += 64) {
+                const bool valid = b + lane < b1;
+                const int j = act[valid ? b + lane : b0];
+                offer(valid, make_double4(pos[3 * j], pos[3 * j + 1], pos[3 * j + 2], (double)(mol[j] * 16 + type[j])));
             }
         } else {
             const int cell = cell_of[a];
@@ -102,8 +143,11 @@ __global__ void __launch_bounds__(64) k_disp(DispGeom g, const int* __restrict__
             // the three cells along z of each (X, Y) column are consecutive in the cell order:
             // one or two contiguous member ranges
             auto scan = [&](int base, int za, int zb) {
-                const int e = cell_start[base + zb + 1];
-                for (int t = cell_start[base + za] + lane; t < e; t += 64) visit(cpos[t]);
+                const int b = cell_start[base + za], e = cell_start[base + zb + 1];
+                for (int t = b; t < e; t += 64) {
+                    const bool valid = t + lane < e;
+                    offer(valid, cpos[valid ? t + lane : b]);
+                }
             };
             for (int ax = -1; ax <= 1; ++ax)
                 for (int ay = -1; ay <= 1; ++ay) {
@@ -122,6 +166,7 @@ __global__ void __launch_bounds__(64) k_disp(DispGeom g, const int* __restrict__
                     }
                 }
         }
+        drain(true);
         fx = wave_sum(fx);
         fy = wave_sum(fy);
         fz = wave_sum(fz);

@@ -101,21 +101,24 @@ __device__ __forceinline__ double tdd_gamma(const ElecParams& P, int same, int t
     return P.thole[2];
 }
 
-// Ewald real-space screening functions bn0..bn3 (:2541-2557); alpha > 0
+// Ewald real-space screening functions bn0..bn3 (:2541-2557); alpha > 0.  The
+// reference's divisions by r and r^2 are multiplications by ri = 1/r and ri^2
+// (one division per pair, shared with the caller's own 1/r).
 struct Ewald {
     double bn0, bn1, bn2, bn3;
-    __device__ __forceinline__ Ewald(double alpha, double r, double r2) {
+    __device__ __forceinline__ Ewald(double alpha, double r) {
         const double ra = alpha * r;
-        bn0 = erfc(ra) / r;
+        const double ri = 1.0 / r, r2i = ri * ri;
+        bn0 = erfc(ra) * ri;
         const double alsq2 = 2.0 * alpha * alpha;
         double alsq2n = 1.0 / (el_sqrt_pi * alpha);
         const double ex = exp(-(ra * ra));
         alsq2n *= alsq2;
-        bn1 = (bn0 + alsq2n * ex) / r2;
+        bn1 = (bn0 + alsq2n * ex) * r2i;
         alsq2n *= alsq2;
-        bn2 = (3.0 * bn1 + alsq2n * ex) / r2;
+        bn2 = (3.0 * bn1 + alsq2n * ex) * r2i;
         alsq2n *= alsq2;
-        bn3 = (5.0 * bn2 + alsq2n * ex) / r2;
+        bn3 = (5.0 * bn2 + alsq2n * ex) * r2i;
     }
 };
 
@@ -370,7 +373,8 @@ __global__ void __launch_bounds__(256) k_prefactors_field(int npol, const int* _
                           double bn1) {
         if constexpr (!PME) {
             if (same) return;
-            const double rr3 = th.s3(th.ex(P.thole[0])) / (r2 * r);
+            const double ri = 1.0 / r;
+            const double rr3 = th.s3(th.ex(P.thole[0])) * (ri * ri * ri);
             const double f = -rr3 * S.q[j];
             ex += dx * f;
             ey += dy * f;
@@ -378,7 +382,8 @@ __global__ void __launch_bounds__(256) k_prefactors_field(int npol, const int* _
         } else {
             double s3 = th.s3(th.ex(P.thole[0]));
             if (same) s3 = 2;  // reproduced as in the reference (:1382-1385)
-            const double rr3 = (s3 - 1.) / (r2 * r);
+            const double ri = 1.0 / r;
+            const double rr3 = (s3 - 1.) * (ri * ri * ri);
             const double f = -(bn1 - rr3) * S.q[j];
             ex += dx * f;
             ey += dy * f;
@@ -401,10 +406,10 @@ __global__ void __launch_bounds__(256) k_prefactors_field(int npol, const int* _
             const double sc3 = th.s3(eg), sc5 = th.s5(gam, eg);
             double p1, p2, bn1 = 0.0;
             if constexpr (PME) {
-                const Ewald ew(P.alpha, r, r2);
-                const double r3 = r * r2, r5 = r3 * r2;
-                p1 = (1.0 - sc3) / r3 - ew.bn1;
-                p2 = ew.bn2 - 3.0 * (1.0 - sc5) / r5;
+                const Ewald ew(P.alpha, r);
+                const double ri = 1.0 / r, ri3 = ri * ri * ri;
+                p1 = (1.0 - sc3) * ri3 - ew.bn1;
+                p2 = ew.bn2 - 3.0 * (1.0 - sc5) * (ri3 * ri * ri);
                 bn1 = ew.bn1;
             } else {
                 const double ri = 1.0 / r, r2i = ri * ri;
@@ -425,7 +430,7 @@ __global__ void __launch_bounds__(256) k_prefactors_field(int npol, const int* _
             const double r2 = dx * dx + dy * dy + dz * dz, r = sqrt(r2);
             const Thole th(S.damp6[i], S.damp6[j], r);
             double bn1 = 0.0;
-            if constexpr (PME) bn1 = Ewald(P.alpha, r, r2).bn1;
+            if constexpr (PME) bn1 = Ewald(P.alpha, r).bn1;
             field_term(j, dx, dy, dz, r2, r, th, S.mol[i] == S.mol[j], bn1);
         }
     }
@@ -1392,7 +1397,8 @@ __global__ void __launch_bounds__(256, 3) k_elec_energy(int n, const double* __r
             const int same = S.mol[i] == S.mol[j];
             const Thole th(S.damp6[i], S.damp6[j], r);
             const double gdd = tdd_gamma(P, same, S.type[i], S.type[j]);
-            const double rr1 = 1.0 / r, rr3 = rr1 / r2, rr5 = 3.0 * rr3 / r2, rr7 = 5.0 * rr5 / r2;
+            const double rr1 = 1.0 / r, r2i = rr1 * rr1, rr3 = rr1 * r2i, rr5 = 3.0 * rr3 * r2i,
+                         rr7 = 5.0 * rr5 * r2i;
             const double sci3 = mix * dx + miy * dy + miz * dz;  // mu_i . d
             const double sci4 = mjx * dx + mjy * dy + mjz * dz;  // mu_j . d
             const double mm = mix * mjx + miy * mjy + miz * mjz;
@@ -1405,7 +1411,7 @@ __global__ void __launch_bounds__(256, 3) k_elec_energy(int n, const double* __r
             const double s5DD = th.s5(gdd, edd), s7DD = th.s7(gdd, edd);
             double gx, gy, gz;  // (ftm2 + ftm2i): force on i is -f * this
             if constexpr (PME) {
-                const Ewald ew(P.alpha, r, r2);
+                const Ewald ew(P.alpha, r);
                 double s1CC = 0., s3CD = 0., s3CC = 0., s5CD = 0.;
                 if (!same) {
                     s1CC = th.s1(P.thole[0], P.tcc_q, ecc);
