@@ -94,6 +94,9 @@ def kernel_models(stats, grid, nsites, npol):
         # row starts, entries (site + 3 FP64 coefficients), dipoles, grid write
         'pme_spread': ('bytes', (ngrid + 1) * 4 + npol * 125 * 28 + npol * 24 + ngrid * 8, 'hbm', PEAK_HBM_GBS, 'GB/s'),
         'fft_z_r2c': ('bytes', ngrid * 8 + nhalf * 16, 'hbm', PEAK_HBM_GBS, 'GB/s'),
+        # the dipole spreading fused with the z pass: row starts, entries, dipoles, half spectrum write
+        'spread_fft_z': ('bytes', (ngrid + 1) * 4 + npol * 125 * 28 + npol * 24 + nhalf * 16, 'hbm', PEAK_HBM_GBS,
+                         'GB/s'),
         'fft_y': ('bytes', nhalf * 32, 'hbm', PEAK_HBM_GBS, 'GB/s'),
         'fft_x_conv': ('bytes', nhalf * 40, 'hbm', PEAK_HBM_GBS, 'GB/s'),
         'fft_z_c2r': ('bytes', nhalf * 16 + ngrid * 8, 'hbm', PEAK_HBM_GBS, 'GB/s'),

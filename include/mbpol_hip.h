@@ -163,7 +163,9 @@ int mbpol_context_download_forces(mbpol_context* ctx, double* forces_out);
 /* per-kernel timing with HIP events on the context's stream (off by default);
  * name one of the phases "evaluate", "onebody", "lists", "twobody", "threebody", "dispersion", "elec_setup",
  * "elec_iteration", "elec_energy", or one of the single kernels of the dipole iteration "pme_spread",
- * "fft_z_r2c", "fft_y", "fft_x_conv", "fft_z_c2r", "fft_plane_fwd", "fft_plane_inv", "elec_iterate"; returns
+ * "fft_z_r2c", "fft_y", "fft_x_conv", "fft_z_c2r", "fft_plane_fwd", "fft_plane_inv", "elec_iterate",
+ * "spread_fft_z" (the dipole spreading fused with the z pass, used instead of "pme_spread" + "fft_z_r2c" for
+ * the dipole grids unless MBPOL_HIP_SPREAD_FUSED=0); returns
  * accumulated milliseconds and the number of timed spans since timing was last enabled */
 int mbpol_context_set_timing(mbpol_context* ctx, int32_t enabled);
 int mbpol_context_get_timing(mbpol_context* ctx, const char* name, double* total_ms, int64_t* count);

@@ -91,7 +91,7 @@ struct CellGrid {
 // timed spans: whole phases, and single kernels of the electrostatics iteration
 enum TimerCat { T_EVALUATE, T_ONEBODY, T_LISTS, T_TWOBODY, T_THREEBODY, T_DISPERSION, T_ELEC_SETUP, T_ELEC_ITER,
                 T_ELEC_ENERGY, T_PME_SPREAD, T_FFT_Z_R2C, T_FFT_Y, T_FFT_X_CONV, T_FFT_Z_C2R, T_ELEC_ITERATE,
-                T_FFT_PLANE_FWD, T_FFT_PLANE_INV, T_NCAT };
+                T_FFT_PLANE_FWD, T_FFT_PLANE_INV, T_SPREAD_FFT_Z, T_NCAT };
 struct Timer {
     bool enabled = false;
     std::vector<hipEvent_t> pool;

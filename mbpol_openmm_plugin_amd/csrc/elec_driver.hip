@@ -116,6 +116,7 @@ struct Elec {
     DevBuf tw[3];
     bool lds_fft = false;
     bool plane_fft = false;  // z and y passes fused per x plane
+    bool fuse_spread = true;  // the dipole spreading fused with the z pass (single-axis passes only)
     hipfftHandle plan_f = 0, plan_b = 0;
     bool plans = false;
     double* h_eps = nullptr;  // pinned, 128 bytes: [0] epsilon, ints from byte 32, a SolveCtl from byte 64
@@ -447,6 +448,8 @@ int elec_create(mbpol_context* ctx) {
         // so the five single-axis passes are the default (MBPOL_HIP_PME_PLANES=1: planes)
         const char* planes = std::getenv("MBPOL_HIP_PME_PLANES");
         E->plane_fft = lds && planes && std::string(planes) == "1" && fft_plane_ok(E->fft);
+        const char* fuse = std::getenv("MBPOL_HIP_SPREAD_FUSED");
+        E->fuse_spread = !(fuse && std::string(fuse) == "0");
         // half-spectrum rows: padded to 128 bytes for the LDS transforms, packed for hipFFT
         const int kh = grid[2] / 2 + 1;
         E->fft.khp = lds ? (kh + 7) / 8 * 8 : kh;
@@ -655,9 +658,16 @@ static int pme_grid(mbpol_context* ctx, int mode, const double* mu, const int* s
     hipStream_t st = ctx->stream;
     const ElecParams& P = E->P;
     const size_t nhalf = (size_t)P.grid.x * P.grid.y * (P.grid.z / 2 + 1);
+    // dipoles through the single-axis LDS passes: spreading and z pass in one kernel
+    const bool fused = mode == 1 && E->lds_fft && !E->plane_fft && E->fuse_spread;
     if (mode == 0) {
         launch_spread_bricks(st, mode, E->q.as<double>(), mu, P, E->B, E->binq.start.as<int>(),
                              E->binq.members.as<int>(), E->g0tab.as<int>(), E->sptab.as<double>(), E->grid.as<double>());
+    } else if (fused) {
+        const KEvents ev = timer_kernel(ctx, T_SPREAD_FFT_Z);
+        launch_spread_fft_z_r2c(st, E->fft, E->rowstart.as<int>(), E->esite.as<int>(), E->ecoef.as<double>(), mu,
+                                E->spec.as<double2>(), ev, stop);
+        timer_kernel_done(ctx, T_SPREAD_FFT_Z, ev);
     } else {
         const KEvents ev = timer_kernel(ctx, T_PME_SPREAD);
         launch_spread_csr(st, (size_t)P.grid.x * P.grid.y * P.grid.z, E->rowstart.as<int>(), E->esite.as<int>(),
@@ -673,9 +683,11 @@ static int pme_grid(mbpol_context* ctx, int mode, const double* mu, const int* s
             launch_fft_plane_fwd(st, E->fft, E->grid.as<double>(), H, ev, stop);
             timer_kernel_done(ctx, T_FFT_PLANE_FWD, ev);
         } else {
-            ev = timer_kernel(ctx, T_FFT_Z_R2C);
-            launch_fft_z_r2c(st, E->fft, E->grid.as<double>(), H, ev, stop);
-            timer_kernel_done(ctx, T_FFT_Z_R2C, ev);
+            if (!fused) {
+                ev = timer_kernel(ctx, T_FFT_Z_R2C);
+                launch_fft_z_r2c(st, E->fft, E->grid.as<double>(), H, ev, stop);
+                timer_kernel_done(ctx, T_FFT_Z_R2C, ev);
+            }
             ev = timer_kernel(ctx, T_FFT_Y);
             launch_fft_y(st, E->fft, H, 0, ev, stop);
             timer_kernel_done(ctx, T_FFT_Y, ev);

@@ -32,6 +32,11 @@ constexpr int kFftMaxLength = 1024;
 // stop: when non-null and *stop != 0 at launch, the pass does nothing (a dipole solve that has converged)
 void launch_fft_z_r2c(hipStream_t s, const PmeFft& F, const double* real, double2* H, const KEvents& ev,
                       const int* stop = nullptr);
+// the dipole spreading (rows of the spreading table in grid-point order: rowstart, esite,
+// ecoef; dipoles mu) fused with the z pass: the half spectrum of the dipole grid
+void launch_spread_fft_z_r2c(hipStream_t s, const PmeFft& F, const int* rowstart, const int* esite,
+                             const double* ecoef, const double* mu, double2* H, const KEvents& ev,
+                             const int* stop = nullptr);
 void launch_fft_y(hipStream_t s, const PmeFft& F, double2* H, int inverse, const KEvents& ev,
                   const int* stop = nullptr);
 void launch_fft_x_conv(hipStream_t s, const PmeFft& F, double2* H, const double* influence, const KEvents& ev,

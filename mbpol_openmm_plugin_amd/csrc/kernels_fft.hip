@@ -314,31 +314,13 @@ __device__ void fft_lines_inplace(double2* buf, int L, int ld, const FftAxis& ax
     }
 }
 
-// z pass, forward: lines of the real grid -> half spectrum
+// z pass, forward, after the block's lines are in LDS (lds[l * N + z], l < m):
+// transform them and store their half spectra
 template <int C>
-__global__ void __launch_bounds__(kFftThreads) k_fft_z_r2c(const double* __restrict__ real, double2* __restrict__ H,
-                                                           int nlines, FftAxis az, int Kh, int Khp,
-                                                           const int* __restrict__ stop) {
-    critical_path_wave();
-    extern __shared__ double2 lds[];
-    if (stop && *stop) return;
+__device__ __forceinline__ void z_r2c_lines(double2* lds, double2* __restrict__ H, int L0, int m, const FftAxis& az,
+                                            int Kh, int Khp) {
     const int N = az.n, M = kFftThreads >> az.lgP, P = 1 << az.lgP;
-    const int L0 = blockIdx.x * M, m = min(M, nlines - L0);
     const int l = threadIdx.x >> az.lgP, lane = threadIdx.x & (P - 1);
-    {  // all loads in flight before the LDS stores
-        double v[C];
-#pragma unroll
-        for (int u = 0; u < C; ++u) {
-            const int e = threadIdx.x + u * kFftThreads;
-            if (e < m * N) v[u] = real[(size_t)L0 * N + e];
-        }
-#pragma unroll
-        for (int u = 0; u < C; ++u) {
-            const int e = threadIdx.x + u * kFftThreads;
-            if (e < m * N) lds[e] = make_double2(v[u], 0.0);
-        }
-    }
-    __syncthreads();
     const double2* r = lds + l * N;
     if (az.inplace)
         fft_lines_inplace(lds, m, N, az, false);
@@ -354,6 +336,85 @@ __global__ void __launch_bounds__(kFftThreads) k_fft_z_r2c(const double* __restr
             }
         }
     }
+}
+
+// z pass, forward: lines of the real grid -> half spectrum
+template <int C>
+__global__ void __launch_bounds__(kFftThreads) k_fft_z_r2c(const double* __restrict__ real, double2* __restrict__ H,
+                                                           int nlines, FftAxis az, int Kh, int Khp,
+                                                           const int* __restrict__ stop) {
+    critical_path_wave();
+    extern __shared__ double2 lds[];
+    if (stop && *stop) return;
+    const int N = az.n, M = kFftThreads >> az.lgP;
+    const int L0 = blockIdx.x * M, m = min(M, nlines - L0);
+    {  // all loads in flight before the LDS stores
+        double v[C];
+#pragma unroll
+        for (int u = 0; u < C; ++u) {
+            const int e = threadIdx.x + u * kFftThreads;
+            if (e < m * N) v[u] = real[(size_t)L0 * N + e];
+        }
+#pragma unroll
+        for (int u = 0; u < C; ++u) {
+            const int e = threadIdx.x + u * kFftThreads;
+            if (e < m * N) lds[e] = make_double2(v[u], 0.0);
+        }
+    }
+    __syncthreads();
+    z_r2c_lines<C>(lds, H, L0, m, az, Kh, Khp);
+}
+
+// Dipole spreading and the forward z pass in one kernel: the block's grid
+// points (whole z lines) are summed from their rows of the spreading table
+// (rowstart / esite / ecoef, entries in site order, the same sums as a
+// separate spreading pass) straight into LDS, so the real grid is neither
+// written nor read back.  A thread's points advance through their rows in
+// lockstep, so that the loads of up to C entries are in flight together.
+template <int C>
+__global__ void __launch_bounds__(kFftThreads) k_spread_fft_z_r2c(const int* __restrict__ rowstart,
+                                                                  const int* __restrict__ esite,
+                                                                  const double* __restrict__ ecoef,
+                                                                  const double* __restrict__ mu, double2* __restrict__ H,
+                                                                  int nlines, FftAxis az, int Kh, int Khp,
+                                                                  const int* __restrict__ stop) {
+    critical_path_wave();
+    extern __shared__ double2 lds[];
+    if (stop && *stop) return;
+    const int N = az.n, M = kFftThreads >> az.lgP;
+    const int L0 = blockIdx.x * M, m = min(M, nlines - L0);
+    {
+        int rb[C], re[C];
+        double v[C];
+        int most = 0;
+#pragma unroll
+        for (int u = 0; u < C; ++u) {
+            const int e = threadIdx.x + u * kFftThreads;
+            const size_t g = (size_t)L0 * N + e;
+            rb[u] = e < m * N ? rowstart[g] : 0;
+            re[u] = e < m * N ? rowstart[g + 1] : 0;
+            v[u] = 0.0;
+            most = max(most, re[u] - rb[u]);
+        }
+        for (int k = 0; k < most; ++k) {
+#pragma unroll
+            for (int u = 0; u < C; ++u) {
+                const int e = rb[u] + k;
+                if (e < re[u]) {
+                    const int s = esite[e];
+                    v[u] += mu[3 * s] * ecoef[3 * (size_t)e] + mu[3 * s + 1] * ecoef[3 * (size_t)e + 1] +
+                            mu[3 * s + 2] * ecoef[3 * (size_t)e + 2];
+                }
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < C; ++u) {
+            const int e = threadIdx.x + u * kFftThreads;
+            if (e < m * N) lds[e] = make_double2(v[u], 0.0);
+        }
+    }
+    __syncthreads();
+    z_r2c_lines<C>(lds, H, L0, m, az, Kh, Khp);
 }
 
 // y pass (either direction): block = (x, tile of M consecutive kz)
@@ -748,6 +809,12 @@ void launch_fft_z_r2c(hipStream_t s, const PmeFft& F, const double* real, double
     const int nl = F.ax[0].n * F.ax[1].n, M = kFftThreads >> F.ax[2].lgP, Kh = F.ax[2].n / 2 + 1;
     MBPOL_FFT_LAUNCH(k_fft_z_r2c, F.ax[2], nblk(nl, M), kFftThreads, lds_bytes(F.ax[2]), s, real, H, nl, F.ax[2], Kh,
                      F.khp, stop)
+}
+void launch_spread_fft_z_r2c(hipStream_t s, const PmeFft& F, const int* rowstart, const int* esite,
+                             const double* ecoef, const double* mu, double2* H, const KEvents& ev, const int* stop) {
+    const int nl = F.ax[0].n * F.ax[1].n, M = kFftThreads >> F.ax[2].lgP, Kh = F.ax[2].n / 2 + 1;
+    MBPOL_FFT_LAUNCH(k_spread_fft_z_r2c, F.ax[2], nblk(nl, M), kFftThreads, lds_bytes(F.ax[2]), s, rowstart, esite,
+                     ecoef, mu, H, nl, F.ax[2], Kh, F.khp, stop)
 }
 void launch_fft_y(hipStream_t s, const PmeFft& F, double2* H, int inverse, const KEvents& ev, const int* stop) {
     const int K0 = F.ax[0].n, K1 = F.ax[1].n, Kh = F.ax[2].n / 2 + 1, M = kFftThreads >> F.ax[1].lgP;

@@ -1338,7 +1338,7 @@ int mbpol_context_get_timing(mbpol_context* ctx, const char* name, double* total
                                         "threebody",  "dispersion", "elec_setup",   "elec_iteration",
                                         "elec_energy", "pme_spread", "fft_z_r2c",   "fft_y",
                                         "fft_x_conv", "fft_z_c2r",  "elec_iterate", "fft_plane_fwd",
-                                        "fft_plane_inv"};
+                                        "fft_plane_inv", "spread_fft_z"};
     timer_collect(ctx);
     for (int c = 0; c < T_NCAT; ++c)
         if (std::strcmp(name, names[c]) == 0) {
