@@ -44,14 +44,14 @@ FLOPS_PER_DIMER = 31.0e3
 # (FETCH_SIZE x 1.885, WRITE_SIZE x 1.000; the runs are summarised under
 # profiles/); kernels not listed report null
 MEASURED_TRAFFIC = {
-    'elec_iterate': 85794228,
-    'pme_spread': 58894442,
-    'fft_z_r2c': 25046395,
-    'fft_y': 25657023,
-    'fft_x_conv': 38351649,
-    'fft_z_c2r': 25327521,
-    'threebody': 112543201,
-    'twobody': 58403508,
+    'elec_iterate': 85801862,
+    'spread_fft_z': 84297708,
+    'fft_z_r2c': 28174817,
+    'fft_y': 25648723,
+    'fft_x_conv': 38349385,
+    'fft_z_c2r': 25335341,
+    'threebody': 113899364,
+    'twobody': 58729957,
 }
 
 
