@@ -194,7 +194,8 @@ int mbpol_context_set_solver(mbpol_context* ctx, int32_t solver);
  * "dimer_capacity", "lower_capacity", "trimer_capacity" = entries of the 2- and 3-body work lists (rows and
  * lists grow when a configuration needs more: an overflow is detected at the end of the evaluation, which
  * is then run again); "pip_blocks", "pip3_blocks", "disp_blocks" = grid caps of the 2-body, 3-body and
- * dispersion kernels (0 = one block per 64 units); "pip3_kernel" = 3-body kernel (0 relabelling-group loop,
+ * dispersion kernels (0 = one block per 64 units); "pip2_kernel" / "pip3_kernel" = 2- / 3-body kernel
+ * (MBPOL_HIP_2B_KERNEL / MBPOL_HIP_3B_KERNEL=terms; 0 relabelling-group loop,
  * 1 term by term); "fork_early" = where the short-range stream forks (1 after the virtual sites, 0 after
  * the electrostatics set-up, 2 after the virtual sites with its kernels enqueued by a host thread of the
  * context's own); "pip_waves", "pip3_waves", "disp_waves" = the caps in waves per SIMD;

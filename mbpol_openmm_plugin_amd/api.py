@@ -518,7 +518,8 @@ class Context:
 
     def getTuning(self, name):
         """Launch shape picked by the context's autotuning ('site_lanes'), a grid cap
-        ('pip_blocks', 'pip3_blocks', 'disp_blocks'), the 3-body kernel ('pip3_kernel'),
+        ('pip_blocks', 'pip3_blocks', 'disp_blocks'), the 2- and 3-body kernels ('pip2_kernel', 'pip3_kernel':
+        0 the loop over the relabelling group, 1 term by term),
         a capacity ('row_capacity_pol', 'row_capacity_other', 'dimer_capacity',
         'lower_capacity', 'trimer_capacity'), the tuned overlap ('fork_early': 1 fork
         after the virtual sites, 0 after the electrostatics set-up, 2 after the virtual

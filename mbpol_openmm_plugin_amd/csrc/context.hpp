@@ -141,6 +141,7 @@ struct mbpol_context {
     int pip_blocks = 0;        // grid cap of the 2-body and 3-body kernels (0: one block per 64 units)
     int disp_blocks = 0;       // grid cap of the dispersion kernel (0: one wave per atom)
     int pip3_kernel = 0;       // 3-body kernel: 0 the polynomial through its relabelling group, 1 term by term
+    int pip2_kernel = 0;       // 2-body kernel: the same choice
     int pip3_blocks = 0;       // grid cap of the 3-body kernel (0: one block per 64 units)
     // the caps in waves per SIMD (blocks = waves x 4 x CUs), tuned on the first forked evaluation
     int pip_waves = 1, pip3_waves = 1, disp_waves = 0, cus = 0;

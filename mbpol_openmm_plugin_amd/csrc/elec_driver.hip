@@ -1233,6 +1233,7 @@ int mbpol_context_get_tuning(const mbpol_context* ctx, const char* name, int32_t
                {"pip_blocks", ctx->pip_blocks},
                {"pip3_blocks", ctx->pip3_blocks},
                {"pip3_kernel", ctx->pip3_kernel},
+               {"pip2_kernel", ctx->pip2_kernel},
                {"disp_blocks", ctx->disp_blocks},
                {"dimer_capacity", ctx->dimer_cap},
                {"lower_capacity", ctx->lower_cap},

@@ -36,6 +36,11 @@ instructions) stays in the instruction cache, and Q reads its coefficients
 from a table in order (scalar loads) instead of carrying them as
 instruction literals.
 
+The 2-body polynomial is written the same way over the 8 relabellings of a
+dimer (swap the waters; swap the hydrogens, and with them the two extra points,
+of either water): 1800 orbit representatives, all with stabilisers of order
+1, 2, 4 or 8, so R is empty and the kernel runs 8 passes of Q.
+
 Usage: gen_pip.py --out DIR  (writes DIR/pip_eval_gen.h and DIR/pip_orbit_gen.h)
 """
 import argparse
@@ -166,7 +171,7 @@ def emit(name, nvar, tree, table=None, accumulate=False):
         first_e = False
         w('g[%d] += s1;' % v0)
         stats['add'] += 1
-    w('return e;')
+    w('return %s;' % ('T(0)' if first_e else 'e'))  # an empty polynomial is 0
     w('}')
     if table is not None:
         out = prefetch_table_reads(out, len(table))
@@ -232,6 +237,32 @@ def trimer_group():
                 pi[3 * w] = 3 * t
                 pi[3 * w + 1], pi[3 * w + 2] = (3 * t + 2, 3 * t + 1) if sw[w] else (3 * t + 1, 3 * t + 2)
             out.add(tuple(idx[frozenset((pi[a], pi[b]))] for a, b in VARS3))
+    return sorted(out)
+
+
+# 2-body points: 0 Oa 1 Ha1 2 Ha2 3 Ob 4 Hb1 5 Hb2 6 Xa1 7 Xa2 8 Xb1 9 Xb2 (X: the extra
+# points of each water); variables in the reference order (mbpol_2body_constants.cpp)
+VARS2 = [(1, 2), (4, 5), (0, 1), (0, 2), (3, 4), (3, 5), (1, 4), (1, 5), (2, 4), (2, 5), (0, 4), (0, 5), (3, 1),
+         (3, 2), (0, 3), (6, 4), (6, 5), (7, 4), (7, 5), (8, 1), (8, 2), (9, 1), (9, 2), (0, 8), (0, 9), (3, 6),
+         (3, 7), (6, 8), (6, 9), (7, 8), (7, 9)]
+
+
+def dimer_group():
+    """Permutations of the 31 variables induced by the 8 relabellings of a
+    dimer: swap the waters, swap the hydrogens of either water (its two extra
+    points, which sit on either side of the molecular plane, swap with them)."""
+    idx = {frozenset(p): n for n, p in enumerate(VARS2)}
+    out = set()
+    for swap_w in (0, 1):
+        for sa in (0, 1):
+            for sb in (0, 1):
+                wa = [0, 2, 1, 7, 6] if sa else [0, 1, 2, 6, 7]  # O, H1, H2, X1, X2 of water a
+                wb = [3, 5, 4, 9, 8] if sb else [3, 4, 5, 8, 9]
+                src = wb + wa if swap_w else wa + wb
+                pi = [0] * 10
+                for t, q in zip([0, 1, 2, 6, 7, 3, 4, 5, 8, 9], src):
+                    pi[t] = q
+                out.add(tuple(idx[frozenset((pi[a], pi[b]))] for a, b in VARS2))
     return sorted(out)
 
 
@@ -361,34 +392,38 @@ def main():
     if not (os.path.exists(path) and open(path).read() == text):
         with open(path, 'w') as fh:
             fh.write(text)
-    # 3-body polynomial as a loop over the trimer's relabelling group (pip_orbit_gen.h)
-    group = trimer_group()
-    qrows, qcoef, rrows, rcoef = orbit_loop_split(data['poly3b'], t['threebody']['thefit'], group)
-    a, b, mask = group_cycle(group)
-    table = []
-    qcode, qstats = emit('pip3b_orbit_pass', 36, build_tree(qrows, qcoef), table=table, accumulate=True)
-    rcode, rstats = emit('pip3b_rest', 36, build_tree(rrows, rcoef))
-    summary['pip3b_orbit_pass'] = qstats
-    summary['pip3b_rest'] = rstats
-    parts = ['// Generated by gen_pip.py from pip_monomials.npz -- do not edit.', '#pragma once', '',
-             '// E(x) = pip3b_rest(x) + sum over the %d relabellings h of pip3b_orbit_pass(y_h), y_h[m] = x[h[m]].'
-             % len(group),
-             '// The relabellings are visited by applying pip3b_step_a or pip3b_step_b to the registers after',
-             '// each pass (pass k is followed by step a when bit k of pip3b_step_mask is set); the last step',
-             '// returns to the original labelling.  pip3b_orbit_pass reads its %d coefficients in order from'
-             % len(table),
-             '// pip3b_orbit_coef (double or float table) and adds its gradient into g.',
-             'constexpr int pip3b_npass = %d;' % len(group),
-             'constexpr unsigned pip3b_step_mask = 0x%xu;' % mask,
-             'constexpr int pip3b_step_a[36] = {%s};' % ', '.join(str(v) for v in a),
-             'constexpr int pip3b_step_b[36] = {%s};' % ', '.join(str(v) for v in b),
-             'constexpr int pip3b_orbit_ncoef = %d;' % len(table)]
-    for typ, suf in (('double', 'f64'), ('float', 'f32')):
-        parts.append('__constant__ %s pip3b_orbit_coef_%s[%d] = {' % (typ, suf, len(table)))
-        for i in range(0, len(table), 4):
-            parts.append('    ' + ', '.join(repr(float(v)) for v in table[i:i + 4]) + ',')
-        parts.append('};')
-    parts += ['', '// %s' % qstats, qcode, '', '// %s' % rstats, rcode.replace('__device__', '__host__ __device__'), '']
+    # the 2- and 3-body polynomials as loops over their relabelling groups (pip_orbit_gen.h)
+    parts = ['// Generated by gen_pip.py from pip_monomials.npz -- do not edit.', '#pragma once', '']
+    for prefix, key, nvar, coefs, group in (('pip2b', 'poly2b', 31, t['twobody']['thefit'], dimer_group()),
+                                            ('pip3b', 'poly3b', 36, t['threebody']['thefit'], trimer_group())):
+        qrows, qcoef, rrows, rcoef = orbit_loop_split(data[key], coefs, group)
+        a, b, mask = group_cycle(group)
+        table = []
+        qcode, qstats = emit(prefix + '_orbit_pass', nvar, build_tree(qrows, qcoef), table=table, accumulate=True)
+        rcode, rstats = emit(prefix + '_rest', nvar, build_tree(rrows, rcoef))
+        summary[prefix + '_orbit_pass'] = qstats
+        summary[prefix + '_rest'] = rstats
+        parts += ['// E(x) = %s_rest(x) + sum over the %d relabellings h of %s_orbit_pass(y_h), y_h[m] = x[h[m]].'
+                  % (prefix, len(group), prefix),
+                  '// The relabellings are visited by applying %s_step_a or %s_step_b to the registers after'
+                  % (prefix, prefix),
+                  '// each pass (pass k is followed by step a when bit k of %s_step_mask is set); the last step'
+                  % prefix,
+                  '// returns to the original labelling.  %s_orbit_pass reads its %d coefficients in order from'
+                  % (prefix, len(table)),
+                  '// %s_orbit_coef (double or float table) and adds its gradient into g.' % prefix,
+                  'constexpr int %s_npass = %d;' % (prefix, len(group)),
+                  'constexpr unsigned %s_step_mask = 0x%xu;' % (prefix, mask),
+                  'constexpr int %s_step_a[%d] = {%s};' % (prefix, nvar, ', '.join(str(v) for v in a)),
+                  'constexpr int %s_step_b[%d] = {%s};' % (prefix, nvar, ', '.join(str(v) for v in b)),
+                  'constexpr int %s_orbit_ncoef = %d;' % (prefix, len(table))]
+        for typ, suf in (('double', 'f64'), ('float', 'f32')):
+            parts.append('__constant__ %s %s_orbit_coef_%s[%d] = {' % (typ, prefix, suf, len(table)))
+            for i in range(0, len(table), 4):
+                parts.append('    ' + ', '.join(repr(float(v)) for v in table[i:i + 4]) + ',')
+            parts.append('};')
+        parts += ['', '// %s' % qstats, qcode, '', '// %s' % rstats,
+                  rcode.replace('__device__', '__host__ __device__'), '']
     text = '\n'.join(parts) + '\n'
     path = os.path.join(args.out, 'pip_orbit_gen.h')
     if not (os.path.exists(path) and open(path).read() == text):

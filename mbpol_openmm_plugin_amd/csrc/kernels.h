@@ -111,8 +111,12 @@ void launch_trimer_pass(hipStream_t s, int n, const int* idx, const double* pos,
     void name(hipStream_t s, long long n, const long long* ndev, const T* list, const int* idx, const double* pos, \
               int periodic, d3 box, u64* force, u64* energy, const KEvents& ev = KEvents(), double* eunit = nullptr, \
               int max_blocks = 0)
+// 2-body, term by term (the coefficients as instruction literals)
 MBPOL_PIP_LAUNCHER(launch_twobody_f32, int2);
 MBPOL_PIP_LAUNCHER(launch_twobody_f64, int2);
+// the 2-body polynomial through the dimer's relabelling group (pip_orbit_loop.h), FP32 and FP64
+MBPOL_PIP_LAUNCHER(launch_twobody_orbit_f32, int2);
+MBPOL_PIP_LAUNCHER(launch_twobody_orbit_f64, int2);
 // FP64, term by term (the coefficients as instruction literals)
 MBPOL_PIP_LAUNCHER(launch_threebody_f64, int);
 // the polynomial through its relabelling group (pip_orbit_loop.h), FP32 and FP64

@@ -11,9 +11,8 @@
 //
 // This file is compiled once per body order and arithmetic type:
 //   -DPIP_BODY=2|3  -DPIP_REAL=float|double
-// and once more per type for the 3-body term with -DPIP_ORBIT=1: the
-// polynomial evaluated through its relabelling group (pip_orbit_loop.h)
-// instead of term by term.
+// with -DPIP_ORBIT=1 the polynomial is evaluated through the relabelling group
+// of the dimer or trimer (pip_orbit_loop.h) instead of term by term.
 #include "device_common.h"
 // an empty asm that re-defines a variable in its register: no instruction
 #define PIP_FENCE(v) asm volatile("" : "+v"(v))
@@ -189,7 +188,11 @@ __global__ void __launch_bounds__(64) k_twobody(long long np, const long long* _
                 const real e = exp_t<real>((real)v.k * ((real)v.r0 - r));
                 x[n] = v.coul ? e / r : e;
             }
+#if PIP_ORBIT
+            real Ep = pip2b_orbit_eval<real>(x, g);
+#else
             real Ep = pip2b_eval<real>(x, g);
+#endif
             // Only t, Ep and g stay live across the polynomial: the geometry is
             // reloaded from memory (the opaque copy of t keeps the compiler from
             // holding the first loads in registers through the polynomial).

@@ -27,6 +27,10 @@ static int pip3_kernel_choice() {
     const char* env = std::getenv("MBPOL_HIP_3B_KERNEL");
     return env && !std::strcmp(env, "terms") ? 1 : 0;
 }
+static int pip2_kernel_choice() {
+    const char* env = std::getenv("MBPOL_HIP_2B_KERNEL");
+    return env && !std::strcmp(env, "terms") ? 1 : 0;
+}
 
 static thread_local std::string g_last_error;
 
@@ -560,6 +564,7 @@ int mbpol_context_create(const mbpol_system* sys, int32_t precision, int32_t dev
         // kernels (uncapped, they hold every SIMD's registers and the next electrostatics
         // launch waits for them to retire); tuned with the other caps
         ctx->pip3_kernel = pip3_kernel_choice();
+        ctx->pip2_kernel = pip2_kernel_choice();
         int waves3 = 1;
         if (const char* env = std::getenv("MBPOL_HIP_3B_WAVES")) waves3 = std::max(0, std::atoi(env));
         ctx->pip3_waves = waves3;
@@ -877,12 +882,11 @@ static int short_range(mbpol_context* ctx, hipStream_t sr, uint32_t comp) {
         const int per = s.tb_method == MBPOL_CUTOFF_PERIODIC;
         const KEvents ev = timer_kernel(ctx, T_TWOBODY);
         const long long* nd = ctx->dimer_off.as<long long>() + (int)s.tb_idx.size() / 3;  // list length (device)
-        if (dp)
-            launch_twobody_f64(sr, ctx->dimer_cap, nd, ctx->dimers.as<int2>(), ctx->tb_idx.as<int>(), pos, per, box, F,
-                               E + 1, ev, eu2, ctx->pip_blocks);
-        else
-            launch_twobody_f32(sr, ctx->dimer_cap, nd, ctx->dimers.as<int2>(), ctx->tb_idx.as<int>(), pos, per, box, F,
-                               E + 1, ev, eu2, ctx->pip_blocks);
+        // the polynomial through the dimer's relabelling group, or term by term (MBPOL_HIP_2B_KERNEL=terms)
+        auto launch2 = dp ? (ctx->pip2_kernel == 1 ? launch_twobody_f64 : launch_twobody_orbit_f64)
+                          : (ctx->pip2_kernel == 1 ? launch_twobody_f32 : launch_twobody_orbit_f32);
+        launch2(sr, ctx->dimer_cap, nd, ctx->dimers.as<int2>(), ctx->tb_idx.as<int>(), pos, per, box, F, E + 1, ev,
+                eu2, ctx->pip_blocks);
         timer_kernel_done(ctx, T_TWOBODY, ev);
         // dimers are listed by their first molecule, so a group's dimers are consecutive
         if (G)

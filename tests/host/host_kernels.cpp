@@ -1,5 +1,5 @@
 // Host build of the product's per-lane device math, for the CPU tests: the
-// 3-body orbit-loop polynomial (pip_orbit_loop.h) and the per-water terms
+// 2- and 3-body orbit-loop polynomials (pip_orbit_loop.h) and the per-water terms
 // (water_terms.h), compiled with g++ from the same headers the HIP kernels
 // include.  Test infrastructure only.
 #include <cmath>
@@ -20,6 +20,14 @@ using std::sqrt;
 #include "water_terms.h"
 
 extern "C" {
+
+double host_pip2b_orbit(const double* x31, double* g31) {
+    double x[31], g[31];
+    for (int i = 0; i < 31; ++i) x[i] = x31[i];
+    const double e = mbpol::pip2b_orbit_eval<double>(x, g);
+    for (int i = 0; i < 31; ++i) g31[i] = g[i];
+    return e;
+}
 
 double host_pip3b_orbit(const double* x36, double* g36) {
     double x[36], g[36];

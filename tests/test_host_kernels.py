@@ -10,6 +10,11 @@ the FP64 oracle and the reference's golden polynomial vectors.  CPU only.
   oracle's to 1e-12 on random variables.  The relabelling walk must visit
   every group element once and return to the identity, and the split must
   account for every monomial of the polynomial.
+* The same for the 2-body polynomial as its kernel evaluates it: 8 passes of
+  one orbit-representative polynomial over the dimer's relabelling group
+  (swap the waters; swap the hydrogens, and with them the extra points, of
+  either water), against tests/golden/pip_vectors.json (written from the
+  reference's poly-2b-v6x.cpp) and the oracle.
 * The per-water terms (csrc/water_terms.h: 1-body energy and gradient,
   dipole-moment-surface charges and their derivatives) against the oracle's
   restatements of MBPolReferenceOneBodyForce.cpp:69-201 and
@@ -48,6 +53,8 @@ def host(tmp_path_factory):
     dp = ctypes.POINTER(ctypes.c_double)
     L.host_pip3b_orbit.restype = ctypes.c_double
     L.host_pip3b_orbit.argtypes = [dp, dp]
+    L.host_pip2b_orbit.restype = ctypes.c_double
+    L.host_pip2b_orbit.argtypes = [dp, dp]
     L.host_water_onebody.restype = ctypes.c_double
     L.host_water_onebody.argtypes = [dp, dp]
     L.host_water_charges.argtypes = [dp, dp, dp]
@@ -91,6 +98,68 @@ def test_orbit_loop_matches_oracle(host, oracle):
         e_ref, g_ref = oracle.pip3b(y)
         assert abs(e - e_ref) <= 1e-12 * abs(e_ref)
         assert np.abs(g - g_ref).max() <= 1e-12 * np.abs(g_ref).max()
+
+
+def orbit2_eval(host, x):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    g = np.zeros(31)
+    return host.host_pip2b_orbit(_p(x), _p(g)), g
+
+
+def test_dimer_orbit_loop_matches_reference_vectors(host):
+    d = json.load(open(os.path.join(GOLDEN, 'pip_vectors.json')))
+    for v in d['poly2b']:
+        e, g = orbit2_eval(host, np.array(v['x']))
+        ref = np.array(v['g'])
+        assert abs(e - v['e']) <= 1e-11 * max(1.0, abs(v['e']))
+        np.testing.assert_allclose(g, ref, rtol=1e-10, atol=1e-10 * np.abs(ref).max())
+
+
+def test_dimer_orbit_loop_matches_oracle(host, oracle):
+    rng = np.random.default_rng(12)
+    for _ in range(8):
+        x = rng.uniform(0.02, 1.2, 31)
+        e, g = orbit2_eval(host, x)
+        e_ref, g_ref = oracle.pip2b(x)
+        print('E %.15g vs %.15g, grad rel err %.1e' % (e, e_ref, np.abs(g - g_ref).max() / np.abs(g_ref).max()))
+        assert abs(e - e_ref) <= 1e-12 * abs(e_ref)
+        assert np.abs(g - g_ref).max() <= 1e-12 * np.abs(g_ref).max()
+    x = rng.uniform(0.02, 1.2, 31)
+    for m in (0, 6, 14, 15, 27, 30):
+        y = x.copy()
+        y[m] *= 1.37
+        e, g = orbit2_eval(host, y)
+        e_ref, g_ref = oracle.pip2b(y)
+        assert abs(e - e_ref) <= 1e-12 * abs(e_ref)
+        assert np.abs(g - g_ref).max() <= 1e-12 * np.abs(g_ref).max()
+
+
+def test_dimer_orbit_split_and_walk():
+    sys.path.insert(0, CSRC)
+    import gen_pip
+    group = gen_pip.dimer_group()
+    assert len(group) == 8 and len(set(group)) == 8
+    gs = set(group)
+    for h1 in group:
+        for h2 in group:
+            assert gen_pip.compose(h1, h2) in gs
+    a, b, mask = gen_pip.group_cycle(group)
+    h, visited = tuple(range(31)), []
+    for k in range(8):
+        visited.append(h)
+        h = gen_pip.compose(h, a if (mask >> k) & 1 else b)
+    assert h == tuple(range(31)) and sorted(visited) == sorted(group)
+    data = np.load(os.path.join(PARAMS, 'pip_monomials.npz'))
+    with open(os.path.join(PARAMS, 'mbpol_tables.json')) as fh:
+        coef = json.load(fh)['twobody']['thefit']
+    qrows, qcoef, rrows, rcoef = gen_pip.orbit_loop_split(data['poly2b'], coef, group)
+    assert len(rrows) == 0  # every stabiliser order is a power of two
+    covered = {}
+    for r in qrows:
+        m = tuple(int(v) for v in r[1:] if v >= 0)
+        for o in {tuple(sorted(hh[v] for v in m)) for hh in group}:
+            covered[o] = covered.get(o, 0) + 1
+    assert len(covered) == len(data['poly2b']) and set(covered.values()) == {1}
 
 
 def test_orbit_split_and_walk():
