@@ -148,6 +148,9 @@ struct mbpol_context {
     // the short-range stream forks right after the virtual sites (1), after the set-up (0), or
     // right after the virtual sites with its work enqueued by the side enqueuer's thread (2)
     int fork_early = 1;
+    // order of the 3-body, 2-body and dispersion work on the second stream (after its lists): the 3-body
+    // kernel first overlaps more of the electrostatics set-up (MBPOL_HIP_SR_ORDER, a permutation of "32d")
+    char sr_order[4] = {'3', '2', 'd', 0};
     mbpol::SideEnqueuer* side_enq = nullptr;  // started on the first evaluation that uses it
     bool tuned = false;
     int tune_evaluations = 0;  // trial evaluations the tuning ran

@@ -572,6 +572,11 @@ int mbpol_context_create(const mbpol_system* sys, int32_t precision, int32_t dev
         int dwaves = 0;  // the dispersion kernel: one block per 64 atoms (MBPOL_HIP_DISP_WAVES)
         if (const char* env = std::getenv("MBPOL_HIP_DISP_WAVES")) dwaves = std::max(0, std::atoi(env));
         ctx->disp_waves = dwaves;
+        if (const char* env = std::getenv("MBPOL_HIP_SR_ORDER")) {  // a permutation of "23d"
+            std::string o(env), sorted_o(env);
+            std::sort(sorted_o.begin(), sorted_o.end());
+            if (sorted_o == "23d") std::memcpy(ctx->sr_order, o.c_str(), 4);
+        }
         if (const char* env = std::getenv("MBPOL_HIP_FORK"))
             ctx->fork_early = !std::strcmp(env, "late") ? 0 : !std::strcmp(env, "thread") ? 2 : 1;
         ctx->disp_blocks = dwaves * 4 * prop.multiProcessorCount;
@@ -878,46 +883,59 @@ static int short_range(mbpol_context* ctx, hipStream_t sr, uint32_t comp) {
     // terms cancel to ~1e-3 of their magnitudes, so FP32 evaluation loses three
     // digits of the 2- and 3-body energies.
     const bool dp = ctx->precision != MBPOL_PRECISION_SINGLE_PIP;
-    if (do2) {
-        const int per = s.tb_method == MBPOL_CUTOFF_PERIODIC;
-        const KEvents ev = timer_kernel(ctx, T_TWOBODY);
-        const long long* nd = ctx->dimer_off.as<long long>() + (int)s.tb_idx.size() / 3;  // list length (device)
-        // the polynomial through the dimer's relabelling group, or term by term (MBPOL_HIP_2B_KERNEL=terms)
-        auto launch2 = dp ? (ctx->pip2_kernel == 1 ? launch_twobody_f64 : launch_twobody_orbit_f64)
-                          : (ctx->pip2_kernel == 1 ? launch_twobody_f32 : launch_twobody_orbit_f32);
-        launch2(sr, ctx->dimer_cap, nd, ctx->dimers.as<int2>(), ctx->tb_idx.as<int>(), pos, per, box, F, E + 1, ev,
-                eu2, ctx->pip_blocks);
-        timer_kernel_done(ctx, T_TWOBODY, ev);
-        // dimers are listed by their first molecule, so a group's dimers are consecutive
-        if (G)
-            launch_group_sums(sr, G, ctx->g_wstart.as<int>(), 0, ctx->dimer_off.as<long long>(), eu2, EG, 1,
-                              ctx->dimer_cap);
-    }
-    if (do3) {
-        const int per = s.th_method == MBPOL_CUTOFF_PERIODIC;
-        const KEvents ev = timer_kernel(ctx, T_THREEBODY);
-        const long long nt = ctx->trip_cap;
-        const long long* ntd = ctx->trip_off.as<long long>() + (int)s.th_idx.size() / 3;  // list length (device)
-        const int* tr = ctx->trips.as<int>();
-        const int* ti = ctx->th_idx.as<int>();
-        const int cap = ctx->pip3_blocks;
-        if (!dp)
-            launch_threebody_orbit_f32(sr, nt, ntd, tr, ti, pos, per, box, F, E + 2, ev, eu3, cap);
-        else if (ctx->pip3_kernel == 1)
-            launch_threebody_f64(sr, nt, ntd, tr, ti, pos, per, box, F, E + 2, ev, eu3, cap);
-        else
-            launch_threebody_orbit_f64(sr, nt, ntd, tr, ti, pos, per, box, F, E + 2, ev, eu3, cap);
-        timer_kernel_done(ctx, T_THREEBODY, ev);
-        if (G)
-            launch_group_sums(sr, G, ctx->g_wstart.as<int>(), 0, ctx->trip_off.as<long long>(), eu3, EG, 2,
-                              ctx->trip_cap);
-    }
-    if ((comp & MBPOL_DISPERSION) && ctx->disp) {
-        timer_start(ctx, T_DISPERSION);
-        if (int e = disp_evaluate(ctx, sr, E + 3)) return e;
-        if (G)
-            if (int e = disp_group_sums(ctx, sr, EG)) return e;
-        timer_stop(ctx, T_DISPERSION);
+    auto run2 = [&]() -> int {
+        if (do2) {
+            const int per = s.tb_method == MBPOL_CUTOFF_PERIODIC;
+            const KEvents ev = timer_kernel(ctx, T_TWOBODY);
+            const long long* nd = ctx->dimer_off.as<long long>() + (int)s.tb_idx.size() / 3;  // list length (device)
+            // the polynomial through the dimer's relabelling group, or term by term (MBPOL_HIP_2B_KERNEL=terms)
+            auto launch2 = dp ? (ctx->pip2_kernel == 1 ? launch_twobody_f64 : launch_twobody_orbit_f64)
+                              : (ctx->pip2_kernel == 1 ? launch_twobody_f32 : launch_twobody_orbit_f32);
+            launch2(sr, ctx->dimer_cap, nd, ctx->dimers.as<int2>(), ctx->tb_idx.as<int>(), pos, per, box, F, E + 1, ev,
+                    eu2, ctx->pip_blocks);
+            timer_kernel_done(ctx, T_TWOBODY, ev);
+            // dimers are listed by their first molecule, so a group's dimers are consecutive
+            if (G)
+                launch_group_sums(sr, G, ctx->g_wstart.as<int>(), 0, ctx->dimer_off.as<long long>(), eu2, EG, 1,
+                                  ctx->dimer_cap);
+        }
+        return MBPOL_OK;
+    };
+    auto run3 = [&]() -> int {
+        if (do3) {
+            const int per = s.th_method == MBPOL_CUTOFF_PERIODIC;
+            const KEvents ev = timer_kernel(ctx, T_THREEBODY);
+            const long long nt = ctx->trip_cap;
+            const long long* ntd = ctx->trip_off.as<long long>() + (int)s.th_idx.size() / 3;  // list length (device)
+            const int* tr = ctx->trips.as<int>();
+            const int* ti = ctx->th_idx.as<int>();
+            const int cap = ctx->pip3_blocks;
+            if (!dp)
+                launch_threebody_orbit_f32(sr, nt, ntd, tr, ti, pos, per, box, F, E + 2, ev, eu3, cap);
+            else if (ctx->pip3_kernel == 1)
+                launch_threebody_f64(sr, nt, ntd, tr, ti, pos, per, box, F, E + 2, ev, eu3, cap);
+            else
+                launch_threebody_orbit_f64(sr, nt, ntd, tr, ti, pos, per, box, F, E + 2, ev, eu3, cap);
+            timer_kernel_done(ctx, T_THREEBODY, ev);
+            if (G)
+                launch_group_sums(sr, G, ctx->g_wstart.as<int>(), 0, ctx->trip_off.as<long long>(), eu3, EG, 2,
+                                  ctx->trip_cap);
+        }
+        return MBPOL_OK;
+    };
+    auto rund = [&]() -> int {
+        if ((comp & MBPOL_DISPERSION) && ctx->disp) {
+            timer_start(ctx, T_DISPERSION);
+            if (int e = disp_evaluate(ctx, sr, E + 3)) return e;
+            if (G)
+                if (int e = disp_group_sums(ctx, sr, EG)) return e;
+            timer_stop(ctx, T_DISPERSION);
+        }
+        return MBPOL_OK;
+    };
+    for (const char* o = ctx->sr_order; *o; ++o) {
+        const int e = *o == '2' ? run2() : *o == '3' ? run3() : *o == 'd' ? rund() : MBPOL_OK;
+        if (e) return e;
     }
     return MBPOL_OK;
 }

@@ -11,7 +11,8 @@ they found it):
 * contexts with caps and fork points fixed to other values by their
   environment variables equal the untuned context bit for bit (energies,
   forces, dipoles, iterations), the short-range work enqueued by the side
-  enqueuer's host thread among them;
+  enqueuer's host thread among them, and the short-range terms enqueued in
+  another order (MBPOL_HIP_SR_ORDER);
 * the tuned context equals the untuned one bit for bit, ran trial evaluations,
   and reports caps among its candidates.
 """
@@ -69,8 +70,13 @@ def test_caps_change_no_result_and_tuning_is_invisible(mb, monkeypatch):
         assert (caps['pip_waves'], caps['pip3_waves'], caps['disp_waves']) == (pip, pip3, disp)
         assert caps['fork_early'] == {'late': 0, 'early': 1, 'thread': 2}[fork]
         same(ref, got)
+    # the short-range terms in another order on their stream
+    monkeypatch.setenv('MBPOL_HIP_SR_ORDER', 'd23')
+    got, caps = run(mb)
+    print('dispersion first', caps, flush=True)
+    same(ref, got)
     for name in ('MBPOL_HIP_TUNE', 'MBPOL_HIP_PIP_WAVES', 'MBPOL_HIP_3B_WAVES', 'MBPOL_HIP_DISP_WAVES',
-                 'MBPOL_HIP_FORK'):
+                 'MBPOL_HIP_FORK', 'MBPOL_HIP_SR_ORDER'):
         monkeypatch.delenv(name)
     got, caps = run(mb)
     print('tuned', caps, flush=True)
