@@ -25,12 +25,12 @@
  * per-thread message (mbpol_last_error) -- no exceptions cross this ABI.
  * A context binds one GPU and two HIP streams: the electrostatics run on the
  * main stream and the 1-, 2- and 3-body terms and the dispersion on the second
- * one, forked from and joined to the main stream by events.  The calling thread
- * enqueues both, or (when the context's launch tuning picks it, "fork_early" 2)
- * a host thread owned by the context enqueues the second stream's work while
- * the calling thread enqueues the electrostatics; that thread's job is waited
- * for before the join, and every call returns after both streams' work has
- * finished.  Calls on one context must not overlap (the reference kernels are
+ * one, forked from and joined to the main stream by events.  A host thread owned
+ * by the context enqueues the second stream's work while the calling thread
+ * enqueues the electrostatics ("fork_early" 2, the default; the context's launch
+ * tuning may instead have the calling thread enqueue both); that thread's job is
+ * waited for before the join, and every call returns after both streams' work
+ * has finished.  Calls on one context must not overlap (the reference kernels are
  * not re-entrant either).
  */
 #ifndef MBPOL_HIP_H

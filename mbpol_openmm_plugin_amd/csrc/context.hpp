@@ -147,7 +147,7 @@ struct mbpol_context {
     int pip_waves = 1, pip3_waves = 1, disp_waves = 0, cus = 0;
     // the short-range stream forks right after the virtual sites (1), after the set-up (0), or
     // right after the virtual sites with its work enqueued by the side enqueuer's thread (2)
-    int fork_early = 1;
+    int fork_early = 2;
     // order of the 3-body, 2-body and dispersion work on the second stream (after its lists): the 3-body
     // kernel first overlaps more of the electrostatics set-up (MBPOL_HIP_SR_ORDER, a permutation of "32d")
     char sr_order[4] = {'3', '2', 'd', 0};

@@ -1070,7 +1070,7 @@ static int tune_overlap(mbpol_context* ctx, uint32_t comp) {
         int* blocks;  // null: the value itself is the setting
         std::vector<int> cand;
     } knobs[] = {
-        {"MBPOL_HIP_FORK", true, &ctx->fork_early, nullptr, {1, 0, 2}},
+        {"MBPOL_HIP_FORK", true, &ctx->fork_early, nullptr, {2, 1, 0}},
         {"MBPOL_HIP_3B_WAVES", (comp & MBPOL_THREEBODY) && !s.th_idx.empty(), &ctx->pip3_waves, &ctx->pip3_blocks,
          {1, 0, 2}},
         {"MBPOL_HIP_DISP_WAVES", (comp & MBPOL_DISPERSION) && ctx->disp, &ctx->disp_waves, &ctx->disp_blocks,
