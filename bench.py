@@ -44,14 +44,14 @@ FLOPS_PER_DIMER = 31.0e3
 # (FETCH_SIZE x 1.885, WRITE_SIZE x 1.000; the runs are summarised under
 # profiles/); kernels not listed report null
 MEASURED_TRAFFIC = {
-    'elec_iterate': 85801862,
-    'spread_fft_z': 84297708,
-    'fft_z_r2c': 28174817,
-    'fft_y': 25648723,
-    'fft_x_conv': 38349385,
-    'fft_z_c2r': 25335341,
-    'threebody': 113899364,
-    'twobody': 58729957,
+    'elec_iterate': 85809187,
+    'spread_fft_z': 84345091,
+    'fft_z_r2c': 28176671,
+    'fft_y': 25650677,
+    'fft_x_conv': 38353271,
+    'fft_z_c2r': 25337294,
+    'threebody': 108603875,
+    'twobody': 52412826,
 }
 
 
