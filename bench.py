@@ -51,7 +51,7 @@ MEASURED_TRAFFIC = {
     'fft_x_conv': 38353271,
     'fft_z_c2r': 25337294,
     'threebody': 108603875,
-    'twobody': 52412826,
+    'twobody': 58729957,
 }
 
 

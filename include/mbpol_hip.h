@@ -195,8 +195,8 @@ int mbpol_context_set_solver(mbpol_context* ctx, int32_t solver);
  * lists grow when a configuration needs more: an overflow is detected at the end of the evaluation, which
  * is then run again); "pip_blocks", "pip3_blocks", "disp_blocks" = grid caps of the 2-body, 3-body and
  * dispersion kernels (0 = one block per 64 units); "pip2_kernel" / "pip3_kernel" = 2- / 3-body kernel
- * (MBPOL_HIP_2B_KERNEL / MBPOL_HIP_3B_KERNEL=terms; 0 relabelling-group loop,
- * 1 term by term); "fork_early" = where the short-range stream forks (1 after the virtual sites, 0 after
+ * (0 relabelling-group loop, 1 term by term; the 2-body kernel is term by term unless
+ * MBPOL_HIP_2B_KERNEL=orbit, the 3-body one the loop unless MBPOL_HIP_3B_KERNEL=terms); "fork_early" = where the short-range stream forks (1 after the virtual sites, 0 after
  * the electrostatics set-up, 2 after the virtual sites with its kernels enqueued by a host thread of the
  * context's own); "pip_waves", "pip3_waves", "disp_waves" = the caps in waves per SIMD;
  * "tune_evaluations" = trial evaluations the tuning of the second evaluation ran; "graph_steps",

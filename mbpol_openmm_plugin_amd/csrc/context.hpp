@@ -141,16 +141,16 @@ struct mbpol_context {
     int pip_blocks = 0;        // grid cap of the 2-body and 3-body kernels (0: one block per 64 units)
     int disp_blocks = 0;       // grid cap of the dispersion kernel (0: one wave per atom)
     int pip3_kernel = 0;       // 3-body kernel: 0 the polynomial through its relabelling group, 1 term by term
-    int pip2_kernel = 0;       // 2-body kernel: the same choice
+    int pip2_kernel = 1;       // 2-body kernel: the same choice (term by term by default)
     int pip3_blocks = 0;       // grid cap of the 3-body kernel (0: one block per 64 units)
     // the caps in waves per SIMD (blocks = waves x 4 x CUs), tuned on the first forked evaluation
     int pip_waves = 1, pip3_waves = 1, disp_waves = 0, cus = 0;
     // the short-range stream forks right after the virtual sites (1), after the set-up (0), or
     // right after the virtual sites with its work enqueued by the side enqueuer's thread (2)
     int fork_early = 2;
-    // order of the 3-body, 2-body and dispersion work on the second stream (after its lists): the 3-body
-    // kernel first overlaps more of the electrostatics set-up (MBPOL_HIP_SR_ORDER, a permutation of "32d")
-    char sr_order[4] = {'3', '2', 'd', 0};
+    // order of the 2-body, 3-body and dispersion work on the second stream, after its lists
+    // (MBPOL_HIP_SR_ORDER, a permutation of "23d")
+    char sr_order[4] = {'2', '3', 'd', 0};
     mbpol::SideEnqueuer* side_enq = nullptr;  // started on the first evaluation that uses it
     bool tuned = false;
     int tune_evaluations = 0;  // trial evaluations the tuning ran

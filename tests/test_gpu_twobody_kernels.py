@@ -1,8 +1,8 @@
 """The 2-body kernels against each other and against the oracle on the
-256-water box: the default kernel (the polynomial as 8 passes over the dimer's
-relabelling group -- swap the waters, swap the hydrogens and extra points of
-either water -- csrc/pip_orbit_loop.h) and the term-by-term kernel with its
-coefficients as instruction literals (MBPOL_HIP_2B_KERNEL=terms), both FP64.
+256-water box: the polynomial as 8 passes over the dimer's relabelling group
+-- swap the waters, swap the hydrogens and extra points of either water --
+(csrc/pip_orbit_loop.h, MBPOL_HIP_2B_KERNEL=orbit) and the default
+term-by-term kernel with its coefficients as instruction literals, both FP64.
 They differ in summation order only (~1e-15 per dimer), but every dimer's
 energy and forces are rounded to 2^-32 fixed point before they are summed, so
 a rounding that flips moves the totals by 2^-32 kJ/mol per dimer: the bounds
