@@ -44,14 +44,14 @@ FLOPS_PER_DIMER = 31.0e3
 # (FETCH_SIZE x 1.885, WRITE_SIZE x 1.000; the runs are summarised under
 # profiles/); kernels not listed report null
 MEASURED_TRAFFIC = {
-    'elec_iterate': 85809187,
-    'spread_fft_z': 84345091,
-    'fft_z_r2c': 28176671,
-    'fft_y': 25650677,
-    'fft_x_conv': 38353271,
-    'fft_z_c2r': 25337294,
-    'threebody': 108603875,
-    'twobody': 58729957,
+    'elec_iterate': 85817539,
+    'spread_fft_z': 84252201,
+    'fft_z_r2c': 28177960,
+    'fft_y': 25651655,
+    'fft_x_conv': 38355242,
+    'fft_z_c2r': 25338291,
+    'threebody': 115149341,
+    'twobody': 58455177,
 }
 
 
