@@ -1385,6 +1385,9 @@ __global__ void __launch_bounds__(256, 3) k_elec_energy(int n, const double* __r
     const int i = (blockIdx.x * blockDim.x + threadIdx.x) / G, lane = threadIdx.x % G;
     double en = 0, fx = 0, fy = 0, fz = 0, vi = 0;
     if (i < n) {
+        // the potential at i is used only at the charged sites of a water (4w+1..4w+3)
+        // when the charges depend on the geometry (k_charge_deriv_forces)
+        const bool vneed = P.redistribute && (i & 3) != 0;
         const double xi = pos[3 * i], yi = pos[3 * i + 1], zi = pos[3 * i + 2];
         const double ci = S.q[i];
         const double mix = mu[3 * i], miy = mu[3 * i + 1], miz = mu[3 * i + 2];
@@ -1405,8 +1408,15 @@ __global__ void __launch_bounds__(256, 3) k_elec_energy(int n, const double* __r
             const double gl0 = ci * ck;
             const double gli1 = ck * sci3 - ci * sci4;
             // the charge-charge (TCC) and charge-dipole (TCD) scales of a pair
-            // within one molecule are never used
-            const double ecc = same ? 0. : th.ex(P.thole[0]), ecd = same ? 0. : th.ex(P.thole[1]);
+            // within one molecule are never used.  The charge-charge scales enter
+            // only multiplied by the charge product q_i q_j, or the potential at i
+            // (s1: times q_j; direct sum, s3: times the dipole of j); where those
+            // factors are zero (an uncharged site, a site whose potential is not
+            // used) their exponential, and the incomplete gamma function of s1,
+            // are skipped and the zero products stay zero.
+            const bool need_s1 = !same && ck != 0.0 && (ci != 0.0 || vneed);
+            const bool need_cc = PME ? need_s1 : (need_s1 || (!same && vneed));
+            const double ecc = need_cc ? th.ex(P.thole[0]) : 0., ecd = same ? 0. : th.ex(P.thole[1]);
             const double edd = th.ex(gdd);
             const double s5DD = th.s5(gdd, edd), s7DD = th.s7(gdd, edd);
             double gx, gy, gz;  // (ftm2 + ftm2i): force on i is -f * this
@@ -1414,9 +1424,9 @@ __global__ void __launch_bounds__(256, 3) k_elec_energy(int n, const double* __r
                 const Ewald ew(P.alpha, r);
                 double s1CC = 0., s3CD = 0., s3CC = 0., s5CD = 0.;
                 if (!same) {
-                    s1CC = th.s1(P.thole[0], P.tcc_q, ecc);
+                    s1CC = need_s1 ? th.s1(P.thole[0], P.tcc_q, ecc) : 0.;
                     s3CD = th.s3(ecd);
-                    s3CC = th.s3(ecc);
+                    s3CC = need_cc ? th.s3(ecc) : 0.;
                     s5CD = th.s5(P.thole[1], ecd);
                 }
                 const double e = ew.bn0 * gl0 - rr1 * gl0 * (1 - s1CC);
@@ -1436,8 +1446,8 @@ __global__ void __launch_bounds__(256, 3) k_elec_energy(int n, const double* __r
                 double bi = sci4 * rr5 * s5DD, bj = sci3 * rr5 * s5DD;
                 double a = rr5 * mm * s5DD - rr7 * sci3 * sci4 * s7DD;
                 if (!same) {
-                    const double s1CC = th.s1(P.thole[0], P.tcc_q, ecc), s3CD = th.s3(ecd);
-                    const double s3CC = th.s3(ecc), s5CD = th.s5(P.thole[1], ecd);
+                    const double s1CC = need_s1 ? th.s1(P.thole[0], P.tcc_q, ecc) : 0., s3CD = th.s3(ecd);
+                    const double s3CC = need_cc ? th.s3(ecc) : 0., s5CD = th.s5(P.thole[1], ecd);
                     en += rr1 * gl0 * s1CC + 0.5 * (rr3 * gli1 * s3CD);
                     a += rr3 * gl0 * s3CC + rr5 * gli1 * s5CD;
                     bi += -ck * rr3 * s3CD;
