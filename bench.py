@@ -44,14 +44,14 @@ FLOPS_PER_DIMER = 31.0e3
 # (FETCH_SIZE x 1.885, WRITE_SIZE x 1.000; the runs are summarised under
 # profiles/); kernels not listed report null
 MEASURED_TRAFFIC = {
-    'elec_iterate': 85817539,
-    'spread_fft_z': 84252201,
-    'fft_z_r2c': 28177960,
-    'fft_y': 25651655,
-    'fft_x_conv': 38355242,
-    'fft_z_c2r': 25338291,
-    'threebody': 115149341,
-    'twobody': 58455177,
+    'elec_iterate': 85816165,
+    'spread_fft_z': 84248140,
+    'fft_z_r2c': 28177550,
+    'fft_y': 25651329,
+    'fft_x_conv': 38354591,
+    'fft_z_c2r': 25337943,
+    'threebody': 114310400,
+    'twobody': 58527178,
 }
 
 
